@@ -1,0 +1,1111 @@
+// Host side of the C ABI: scene flattening, batch validation and staging,
+// launches and readback.
+//
+// Mirrors OptiXTransportHost::Impl (src/optical/optix/OptiXTransportHost.cpp):
+// one in-flight batch slot (:523-525), validation before any device work
+// (:572-666), pinned staging, stream-ordered upload/launch/readback, and a
+// completion call that waits on the batch and assembles the output
+// (:779-883).  Host-side checks all run before the first HIP call, so the
+// error paths are testable without a GPU.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <stdexcept>
+#include <string>
+#include <unordered_map>
+#include <unordered_set>
+#include <vector>
+
+#include "../../include/g4go_hip.h"
+#include "kernels.h"
+
+using namespace g4go;
+
+static_assert(sizeof(g4go_emission) == sizeof(Emission), "emission layout");
+static_assert(offsetof(g4go_emission, event_id) == offsetof(Emission, event_id), "emission layout");
+static_assert(offsetof(g4go_emission, type) == offsetof(Emission, type), "emission layout");
+static_assert(sizeof(g4go_hit) == sizeof(Hit), "hit layout");
+static_assert(offsetof(g4go_hit, time_ns) == offsetof(Hit, time), "hit layout");
+
+namespace {
+
+struct Failure {
+    g4go_status status;
+    std::string message;
+};
+
+[[noreturn]] void fail(g4go_status status, const std::string& message) { throw Failure{status, message}; }
+
+void hip_check(hipError_t err, const char* what) {
+    if (err != hipSuccess) {
+        fail(G4GO_ERROR_RUNTIME, std::string(what) + " failed: " + hipGetErrorString(err));
+    }
+}
+
+thread_local std::string g_create_error;
+
+// ---------------------------------------------------------------------------
+// Host scene (flattened Scene, see scene.h)
+// ---------------------------------------------------------------------------
+struct HostScene {
+    std::vector<float> pool;
+    std::vector<Material> materials;
+    std::vector<Surface> surfaces;
+    std::vector<Volume> volumes;
+    std::vector<uint32_t> daughters;
+    std::vector<Binding> bindings;
+    std::vector<BvhNode> bvh;
+    uint32_t world = kInvalid;
+    std::unordered_map<uint32_t, uint32_t> index_of;  // public volume ID -> dense index
+    std::vector<uint32_t> volume_ids;                 // dense index -> public volume ID
+};
+
+bool finite_all(const float* v, uint32_t n) {
+    for (uint32_t i = 0; i < n; ++i) {
+        if (!std::isfinite(v[i])) {
+            return false;
+        }
+    }
+    return true;
+}
+
+Prop add_property(HostScene& s, const g4go_property& p, const std::string& what) {
+    Prop out{0, 0, 0, 0};
+    if (p.count == 0) {
+        return out;
+    }
+    if (p.energy_ev == nullptr || p.values == nullptr) {
+        fail(G4GO_ERROR_INVALID_ARGUMENT, what + ": null property table");
+    }
+    if (!finite_all(p.energy_ev, p.count) || !finite_all(p.values, p.count)) {
+        fail(G4GO_ERROR_INVALID_ARGUMENT, what + ": non-finite property value");
+    }
+    for (uint32_t i = 1; i < p.count; ++i) {
+        if (p.energy_ev[i] < p.energy_ev[i - 1]) {
+            fail(G4GO_ERROR_INVALID_ARGUMENT, what + ": property energies must be ascending");
+        }
+    }
+    out.offset = static_cast<uint32_t>(s.pool.size());
+    out.count = p.count;
+    bool constant = true;
+    for (uint32_t i = 1; i < p.count; ++i) {
+        constant = constant && p.values[i] == p.values[0];
+    }
+    out.constant = constant ? 1u : 0u;  // PropertyTable::Constant (Scene.cpp:17-21)
+    s.pool.insert(s.pool.end(), p.energy_ev, p.energy_ev + p.count);
+    s.pool.insert(s.pool.end(), p.values, p.values + p.count);
+    return out;
+}
+
+// Normalised trapezoid CDF of a scintillation spectrum
+// (UploadSpectrumProperty, OptiXTransportHost.cpp:371-410).
+Prop add_spectrum(HostScene& s, const g4go_property& p, const std::string& what) {
+    Prop out{0, 0, 0, 0};
+    if (p.count == 0 || p.energy_ev == nullptr || p.values == nullptr) {
+        return out;
+    }
+    std::vector<float> cdf(p.count, 0.0f);
+    if (p.count == 1) {
+        cdf[0] = 1.0f;
+    } else {
+        for (uint32_t i = 1; i < p.count; ++i) {
+            const float de = p.energy_ev[i] - p.energy_ev[i - 1];
+            const float v0 = std::max(p.values[i - 1], 0.0f);
+            const float v1 = std::max(p.values[i], 0.0f);
+            if (!(de >= 0.0f) || !std::isfinite(de) || !std::isfinite(v0) || !std::isfinite(v1)) {
+                return out;
+            }
+            cdf[i] = cdf[i - 1] + 0.5f * de * (v0 + v1);
+        }
+        const float integral = cdf.back();
+        if (!(integral > 0.0f) || !std::isfinite(integral)) {
+            return out;
+        }
+        for (float& v : cdf) {
+            v /= integral;
+        }
+        cdf.back() = 1.0f;
+    }
+    (void)what;
+    out.offset = static_cast<uint32_t>(s.pool.size());
+    out.count = p.count;
+    out.constant = 0;
+    s.pool.insert(s.pool.end(), p.energy_ev, p.energy_ev + p.count);
+    s.pool.insert(s.pool.end(), cdf.begin(), cdf.end());
+    return out;
+}
+
+// Scene::FindBoundarySurface semantics (Scene.cpp:120-150) on dense indices.
+uint32_t host_find_surface(const HostScene& s, uint32_t from, uint32_t to) {
+    for (const Binding& b : s.bindings) {
+        if (b.from == from && b.to == to) {
+            return b.surface;
+        }
+    }
+    if (from >= s.volumes.size() || to >= s.volumes.size()) {
+        return kInvalid;
+    }
+    const Volume& f = s.volumes[from];
+    const Volume& t = s.volumes[to];
+    if (t.parent == from) {
+        return t.skin != kInvalid ? t.skin : f.skin;
+    }
+    return f.skin != kInvalid ? f.skin : t.skin;
+}
+
+void build_bvh(HostScene& s, Volume& vol, uint32_t begin, uint32_t count) {
+    // Median split on the longest axis of the centroid bounds; leaves of <= 4.
+    struct Task {
+        uint32_t node, first, count;
+    };
+    auto bounds = [&](uint32_t first, uint32_t n, float* lo, float* hi) {
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = std::numeric_limits<float>::infinity();
+            hi[k] = -std::numeric_limits<float>::infinity();
+        }
+        for (uint32_t i = first; i < first + n; ++i) {
+            const Volume& d = s.volumes[s.daughters[i]];
+            for (int k = 0; k < 3; ++k) {
+                lo[k] = std::min(lo[k], d.lo[k]);
+                hi[k] = std::max(hi[k], d.hi[k]);
+            }
+        }
+    };
+    vol.bvh_root = static_cast<uint32_t>(s.bvh.size());
+    s.bvh.push_back(BvhNode{});
+    std::vector<Task> stack{{vol.bvh_root, begin, count}};
+    while (!stack.empty()) {
+        const Task t = stack.back();
+        stack.pop_back();
+        BvhNode node{};
+        bounds(t.first, t.count, node.lo, node.hi);
+        if (t.count <= 4) {
+            node.left = t.first;
+            node.right_or_count = t.count | kBvhLeaf;
+            s.bvh[t.node] = node;
+            continue;
+        }
+        float clo[3], chi[3];
+        for (int k = 0; k < 3; ++k) {
+            clo[k] = std::numeric_limits<float>::infinity();
+            chi[k] = -std::numeric_limits<float>::infinity();
+        }
+        for (uint32_t i = t.first; i < t.first + t.count; ++i) {
+            const Volume& d = s.volumes[s.daughters[i]];
+            for (int k = 0; k < 3; ++k) {
+                const float c = 0.5f * (d.lo[k] + d.hi[k]);
+                clo[k] = std::min(clo[k], c);
+                chi[k] = std::max(chi[k], c);
+            }
+        }
+        int axis = 0;
+        for (int k = 1; k < 3; ++k) {
+            if (chi[k] - clo[k] > chi[axis] - clo[axis]) {
+                axis = k;
+            }
+        }
+        auto first = s.daughters.begin() + t.first;
+        auto mid = first + t.count / 2;
+        std::nth_element(first, mid, first + t.count, [&](uint32_t a, uint32_t b) {
+            const float ca = s.volumes[a].lo[axis] + s.volumes[a].hi[axis];
+            const float cb = s.volumes[b].lo[axis] + s.volumes[b].hi[axis];
+            return ca < cb || (ca == cb && a < b);
+        });
+        const uint32_t half = t.count / 2;
+        node.left = static_cast<uint32_t>(s.bvh.size());
+        node.right_or_count = node.left + 1;
+        s.bvh.push_back(BvhNode{});
+        s.bvh.push_back(BvhNode{});
+        s.bvh[t.node] = node;
+        stack.push_back({node.left, t.first, half});
+        stack.push_back({node.right_or_count, t.first + half, t.count - half});
+    }
+}
+
+HostScene build_scene(const g4go_scene& in, float tolerance) {
+    HostScene s;
+    if (in.n_volumes == 0 || in.volumes == nullptr) {
+        fail(G4GO_ERROR_INVALID_ARGUMENT, "scene has no volumes");
+    }
+    if ((in.n_materials != 0 && in.materials == nullptr) || (in.n_surfaces != 0 && in.surfaces == nullptr) ||
+        (in.n_bindings != 0 && in.bindings == nullptr)) {
+        fail(G4GO_ERROR_INVALID_ARGUMENT, "scene array pointer is null");
+    }
+    for (uint32_t i = 0; i < in.n_materials; ++i) {
+        const g4go_material& m = in.materials[i];
+        const std::string tag = "material " + std::to_string(i);
+        Material dm{};
+        dm.rindex = add_property(s, m.rindex, tag + " RINDEX");
+        dm.group_velocity = add_property(s, m.group_velocity_mm_per_ns, tag + " GROUPVEL");
+        dm.abs_length = add_property(s, m.abs_length_mm, tag + " ABSLENGTH");
+        dm.rayleigh_length = add_property(s, m.rayleigh_length_mm, tag + " RAYLEIGH");
+        for (int k = 0; k < 3; ++k) {
+            dm.spectrum[k] = add_spectrum(s, m.scintillation_spectrum[k], tag + " spectrum");
+        }
+        dm.rindex_max = 1.0f;
+        if (m.rindex.count != 0) {
+            dm.rindex_max = *std::max_element(m.rindex.values, m.rindex.values + m.rindex.count);
+        }
+        s.materials.push_back(dm);
+    }
+    for (uint32_t i = 0; i < in.n_surfaces; ++i) {
+        const g4go_surface& f = in.surfaces[i];
+        const std::string tag = "surface " + std::to_string(i);
+        if (f.type > 1 || f.model > 1 || f.finish > 1) {
+            fail(G4GO_ERROR_INVALID_ARGUMENT, tag + ": unknown type, model or finish");
+        }
+        Surface ds{};
+        ds.type = f.type;
+        ds.model = f.model;
+        ds.finish = f.finish;
+        ds.model_value = f.model_value;
+        ds.reflectivity = add_property(s, f.reflectivity, tag + " REFLECTIVITY");
+        ds.efficiency = add_property(s, f.efficiency, tag + " EFFICIENCY");
+        ds.transmittance = add_property(s, f.transmittance, tag + " TRANSMITTANCE");
+        ds.lobe = add_property(s, f.specular_lobe, tag + " SPECULARLOBECONSTANT");
+        ds.spike = add_property(s, f.specular_spike, tag + " SPECULARSPIKECONSTANT");
+        ds.backscatter = add_property(s, f.backscatter, tag + " BACKSCATTERCONSTANT");
+        ds.roughness = add_property(s, f.surface_roughness, tag + " SURFACEROUGHNESS");
+        s.surfaces.push_back(ds);
+    }
+    // volumes: dense indices in the given order
+    s.volume_ids.resize(in.n_volumes);
+    for (uint32_t i = 0; i < in.n_volumes; ++i) {
+        const uint32_t id = in.volumes[i].volume_id;
+        if (id == kInvalid || !s.index_of.emplace(id, i).second) {
+            fail(G4GO_ERROR_INVALID_ARGUMENT, "duplicate or invalid volume ID " + std::to_string(id));
+        }
+        s.volume_ids[i] = id;
+    }
+    const auto map_id = [&](uint32_t id, const char* what) -> uint32_t {
+        const auto it = s.index_of.find(id);
+        if (it == s.index_of.end()) {
+            fail(G4GO_ERROR_INVALID_ARGUMENT, std::string("unknown ") + what + " volume ID " + std::to_string(id));
+        }
+        return it->second;
+    };
+    s.world = map_id(in.world_volume_id, "world");
+    for (uint32_t i = 0; i < in.n_volumes; ++i) {
+        const g4go_volume& v = in.volumes[i];
+        Volume dv{};
+        for (int k = 0; k < 3; ++k) {
+            if (!std::isfinite(v.center_mm[k]) || !(v.half_mm[k] > 0.0f) || !std::isfinite(v.half_mm[k])) {
+                fail(G4GO_ERROR_INVALID_ARGUMENT, "volume " + std::to_string(v.volume_id) + ": invalid box");
+            }
+            dv.lo[k] = static_cast<float>(static_cast<double>(v.center_mm[k]) - v.half_mm[k]);
+            dv.hi[k] = static_cast<float>(static_cast<double>(v.center_mm[k]) + v.half_mm[k]);
+        }
+        if (v.material_id >= in.n_materials) {
+            fail(G4GO_ERROR_INVALID_ARGUMENT, "volume " + std::to_string(v.volume_id) + ": unknown material");
+        }
+        if (v.skin_surface_id != kInvalid && v.skin_surface_id >= in.n_surfaces) {
+            fail(G4GO_ERROR_INVALID_ARGUMENT, "volume " + std::to_string(v.volume_id) + ": unknown skin surface");
+        }
+        dv.material = v.material_id;
+        dv.parent = i == s.world ? kInvalid : map_id(v.parent_volume_id, "parent");
+        if (i == s.world && v.parent_volume_id != kInvalid) {
+            fail(G4GO_ERROR_INVALID_ARGUMENT, "the world volume must not have a parent");
+        }
+        if (i != s.world && dv.parent == i) {
+            fail(G4GO_ERROR_INVALID_ARGUMENT, "volume is its own parent");
+        }
+        dv.skin = v.skin_surface_id;
+        dv.sensor = v.sensor_id;
+        dv.entry_surface = kInvalid;
+        dv.bvh_root = kInvalid;
+        s.volumes.push_back(dv);
+    }
+    // every volume must reach the world
+    for (uint32_t i = 0; i < in.n_volumes; ++i) {
+        uint32_t v = i, hops = 0;
+        while (v != s.world) {
+            v = s.volumes[v].parent;
+            if (v == kInvalid || ++hops > in.n_volumes) {
+                fail(G4GO_ERROR_INVALID_ARGUMENT, "volume hierarchy does not reach the world");
+            }
+        }
+    }
+    // daughter lists
+    for (uint32_t p = 0; p < in.n_volumes; ++p) {
+        s.volumes[p].daughter_begin = static_cast<uint32_t>(s.daughters.size());
+        for (uint32_t c = 0; c < in.n_volumes; ++c) {
+            if (s.volumes[c].parent == p) {
+                s.daughters.push_back(c);
+            }
+        }
+        s.volumes[p].daughter_count = static_cast<uint32_t>(s.daughters.size()) - s.volumes[p].daughter_begin;
+    }
+    // border surfaces, sorted by (from, to) (OptiXTransportHost.cpp:1263-1277)
+    for (uint32_t i = 0; i < in.n_bindings; ++i) {
+        const g4go_surface_binding& b = in.bindings[i];
+        if (b.surface_id >= in.n_surfaces) {
+            fail(G4GO_ERROR_INVALID_ARGUMENT, "surface binding references an unknown surface");
+        }
+        s.bindings.push_back(Binding{map_id(b.from_volume_id, "boundary source"),
+                                     map_id(b.to_volume_id, "boundary target"), b.surface_id, 0});
+    }
+    // stable: among duplicate (from, to) pairs the first given wins, as in Scene::FindBoundarySurface
+    std::stable_sort(s.bindings.begin(), s.bindings.end(),
+                     [](const Binding& a, const Binding& b) { return a.from < b.from || (a.from == b.from && a.to < b.to); });
+    // entry surfaces and face neighbours
+    for (uint32_t i = 0; i < in.n_volumes; ++i) {
+        Volume& v = s.volumes[i];
+        if (v.parent != kInvalid) {
+            v.entry_surface = host_find_surface(s, v.parent, i);
+        }
+        for (int axis = 0; axis < 3; ++axis) {
+            for (int side = 0; side < 2; ++side) {
+                const int face = 2 * axis + side;
+                v.face_neighbor[face] = kInvalid;
+                v.face_surface[face] = kInvalid;
+                if (v.parent == kInvalid) {
+                    continue;  // leaving the world: escape
+                }
+                const float plane = side ? v.hi[axis] : v.lo[axis];
+                const int a1 = (axis + 1) % 3, a2 = (axis + 2) % 3;
+                std::vector<uint32_t> touching;
+                bool covered = false;
+                const Volume& par = s.volumes[v.parent];
+                for (uint32_t k = 0; k < par.daughter_count; ++k) {
+                    const uint32_t j = s.daughters[par.daughter_begin + k];
+                    if (j == i) {
+                        continue;
+                    }
+                    const Volume& o = s.volumes[j];
+                    const float oplane = side ? o.lo[axis] : o.hi[axis];
+                    if (std::fabs(oplane - plane) > tolerance) {
+                        continue;
+                    }
+                    const bool overlap = o.lo[a1] < v.hi[a1] - tolerance && o.hi[a1] > v.lo[a1] + tolerance &&
+                                         o.lo[a2] < v.hi[a2] - tolerance && o.hi[a2] > v.lo[a2] + tolerance;
+                    if (!overlap) {
+                        continue;
+                    }
+                    touching.push_back(j);
+                    covered = o.lo[a1] <= v.lo[a1] + tolerance && o.hi[a1] >= v.hi[a1] - tolerance &&
+                              o.lo[a2] <= v.lo[a2] + tolerance && o.hi[a2] >= v.hi[a2] - tolerance;
+                }
+                uint32_t next;
+                if (touching.empty()) {
+                    next = v.parent;
+                } else if (touching.size() == 1 && covered) {
+                    next = touching[0];
+                } else {
+                    v.face_neighbor[face] = kFaceDynamic;
+                    continue;
+                }
+                v.face_neighbor[face] = next;
+                v.face_surface[face] = host_find_surface(s, i, next);
+            }
+        }
+    }
+    // daughter BVHs for volumes with many daughters
+    for (uint32_t i = 0; i < in.n_volumes; ++i) {
+        Volume& v = s.volumes[i];
+        if (v.daughter_count > 8) {
+            build_bvh(s, v, v.daughter_begin, v.daughter_count);
+        }
+    }
+    if (s.pool.empty()) {
+        s.pool.push_back(0.0f);
+    }
+    return s;
+}
+
+// ---------------------------------------------------------------------------
+// Device buffers
+// ---------------------------------------------------------------------------
+struct DevBuf {
+    void* ptr = nullptr;
+    size_t cap = 0;
+    void ensure(size_t bytes, const char* what) {
+        if (bytes <= cap) {
+            return;
+        }
+        if (ptr != nullptr) {
+            (void)hipFree(ptr);
+            ptr = nullptr;
+            cap = 0;
+        }
+        hip_check(hipMalloc(&ptr, std::max<size_t>(bytes, 256)), what);
+        cap = std::max<size_t>(bytes, 256);
+    }
+    void release() {
+        if (ptr != nullptr) {
+            (void)hipFree(ptr);
+        }
+        ptr = nullptr;
+        cap = 0;
+    }
+    template <class T>
+    T* as() const {
+        return static_cast<T*>(ptr);
+    }
+};
+
+struct PinBuf {
+    void* ptr = nullptr;
+    size_t cap = 0;
+    void ensure(size_t bytes, const char* what) {
+        if (bytes <= cap) {
+            return;
+        }
+        if (ptr != nullptr) {
+            (void)hipHostFree(ptr);
+            ptr = nullptr;
+            cap = 0;
+        }
+        hip_check(hipHostMalloc(&ptr, std::max<size_t>(bytes, 256), hipHostMallocDefault), what);
+        cap = std::max<size_t>(bytes, 256);
+    }
+    void release() {
+        if (ptr != nullptr) {
+            (void)hipHostFree(ptr);
+        }
+        ptr = nullptr;
+        cap = 0;
+    }
+    template <class T>
+    T* as() const {
+        return static_cast<T*>(ptr);
+    }
+};
+
+size_t align256(size_t x) { return (x + 255) & ~static_cast<size_t>(255); }
+
+}  // namespace
+
+struct g4go_transport {
+    g4go_config cfg{};
+    g4go_launch launch{};
+    std::string error;
+
+    bool hip_ready = false;
+    int device = 0;
+    int cu_count = 256;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev_start = nullptr, ev_kernel0 = nullptr, ev_kernel1 = nullptr, ev_meta = nullptr;
+
+    HostScene scene;
+    bool scene_built = false;
+    bool scene_uploaded = false;
+    DevBuf d_scene;
+    SceneView view{};
+
+    // batch slot
+    bool busy = false;
+    int source = kSourceSynthetic;
+    uint64_t n_photons = 0;
+    uint32_t n_items = 0;
+    uint64_t device_hits = 0;  // detections of the last completed batch left in d_hits
+    uint32_t chunk = 64;
+    std::vector<uint32_t> event_ids;
+    std::vector<uint64_t> captured;
+    std::vector<uint32_t> captured_u32() const { return std::vector<uint32_t>(captured.begin(), captured.end()); }
+    std::chrono::steady_clock::time_point started;
+    uint64_t h2d_bytes = 0;
+
+    DevBuf d_inputs, d_hits, d_event_stats, d_meta;
+    PinBuf h_stage, h_meta, h_hits;
+
+    std::vector<g4go_hit> out_hits;
+    std::vector<g4go_event_stats> out_events;
+    uint64_t out_hist[kHistSlots] = {};
+
+    // d_meta layout: [counters (kCounterCount u64)] [hit_count u32, pad u32] [histogram u64 x kHistSlots]
+    static constexpr size_t kMetaCounters = 0;
+    static constexpr size_t kMetaHitCount = kCounterCount * 8;
+    static constexpr size_t kMetaHist = kMetaHitCount + 8;
+    static constexpr size_t kMetaBytes = kMetaHist + kHistSlots * 8;
+
+    void init_hip() {
+        if (hip_ready) {
+            return;
+        }
+        if (cfg.device >= 0) {
+            hip_check(hipSetDevice(cfg.device), "hipSetDevice");
+        }
+        hip_check(hipGetDevice(&device), "hipGetDevice");
+        hip_check(hipDeviceGetAttribute(&cu_count, hipDeviceAttributeMultiprocessorCount, device),
+                  "hipDeviceGetAttribute");
+        hip_check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
+        hip_check(hipEventCreate(&ev_start), "hipEventCreate");
+        hip_check(hipEventCreate(&ev_kernel0), "hipEventCreate");
+        hip_check(hipEventCreate(&ev_kernel1), "hipEventCreate");
+        hip_check(hipEventCreate(&ev_meta), "hipEventCreate");
+        hip_ready = true;
+    }
+
+    void upload_scene() {
+        if (scene_uploaded) {
+            return;
+        }
+        const HostScene& s = scene;
+        size_t off = 0;
+        const size_t o_pool = off;
+        off = align256(off + s.pool.size() * sizeof(float));
+        const size_t o_mat = off;
+        off = align256(off + std::max<size_t>(1, s.materials.size()) * sizeof(Material));
+        const size_t o_surf = off;
+        off = align256(off + std::max<size_t>(1, s.surfaces.size()) * sizeof(Surface));
+        const size_t o_vol = off;
+        off = align256(off + s.volumes.size() * sizeof(Volume));
+        const size_t o_dau = off;
+        off = align256(off + std::max<size_t>(1, s.daughters.size()) * sizeof(uint32_t));
+        const size_t o_bind = off;
+        off = align256(off + std::max<size_t>(1, s.bindings.size()) * sizeof(Binding));
+        const size_t o_bvh = off;
+        off = align256(off + std::max<size_t>(1, s.bvh.size()) * sizeof(BvhNode));
+        std::vector<char> blob(off, 0);
+        std::memcpy(blob.data() + o_pool, s.pool.data(), s.pool.size() * sizeof(float));
+        if (!s.materials.empty()) std::memcpy(blob.data() + o_mat, s.materials.data(), s.materials.size() * sizeof(Material));
+        if (!s.surfaces.empty()) std::memcpy(blob.data() + o_surf, s.surfaces.data(), s.surfaces.size() * sizeof(Surface));
+        std::memcpy(blob.data() + o_vol, s.volumes.data(), s.volumes.size() * sizeof(Volume));
+        if (!s.daughters.empty()) std::memcpy(blob.data() + o_dau, s.daughters.data(), s.daughters.size() * sizeof(uint32_t));
+        if (!s.bindings.empty()) std::memcpy(blob.data() + o_bind, s.bindings.data(), s.bindings.size() * sizeof(Binding));
+        if (!s.bvh.empty()) std::memcpy(blob.data() + o_bvh, s.bvh.data(), s.bvh.size() * sizeof(BvhNode));
+        d_scene.ensure(blob.size(), "hipMalloc scene");
+        hip_check(hipMemcpy(d_scene.ptr, blob.data(), blob.size(), hipMemcpyHostToDevice), "hipMemcpy scene");
+        char* base = d_scene.as<char>();
+        view = SceneView{};
+        view.pool = reinterpret_cast<const float*>(base + o_pool);
+        view.materials = reinterpret_cast<const Material*>(base + o_mat);
+        view.surfaces = reinterpret_cast<const Surface*>(base + o_surf);
+        view.volumes = reinterpret_cast<const Volume*>(base + o_vol);
+        view.daughters = reinterpret_cast<const uint32_t*>(base + o_dau);
+        view.bindings = reinterpret_cast<const Binding*>(base + o_bind);
+        view.bvh = reinterpret_cast<const BvhNode*>(base + o_bvh);
+        view.n_materials = static_cast<uint32_t>(s.materials.size());
+        view.n_surfaces = static_cast<uint32_t>(s.surfaces.size());
+        view.n_volumes = static_cast<uint32_t>(s.volumes.size());
+        view.n_bindings = static_cast<uint32_t>(s.bindings.size());
+        view.n_bvh_nodes = static_cast<uint32_t>(s.bvh.size());
+        view.world = s.world;
+        scene_uploaded = true;
+    }
+
+    void release() {
+        if (stream != nullptr) {
+            (void)hipStreamSynchronize(stream);
+        }
+        d_scene.release();
+        d_inputs.release();
+        d_hits.release();
+        d_event_stats.release();
+        d_meta.release();
+        h_stage.release();
+        h_meta.release();
+        h_hits.release();
+        for (hipEvent_t* e : {&ev_start, &ev_kernel0, &ev_kernel1, &ev_meta}) {
+            if (*e != nullptr) {
+                (void)hipEventDestroy(*e);
+                *e = nullptr;
+            }
+        }
+        if (stream != nullptr) {
+            (void)hipStreamDestroy(stream);
+            stream = nullptr;
+        }
+    }
+
+    KernelParams base_params() const {
+        KernelParams p{};
+        p.scene = view;
+        p.rng = Rng{static_cast<uint32_t>(cfg.seed), static_cast<uint32_t>(cfg.seed >> 32)};
+        p.cfg = StepConfig{cfg.boundary_tolerance_mm};
+        p.max_steps = cfg.max_bounces_per_photon;
+        p.refill_threshold = std::max<uint32_t>(1, std::min<uint32_t>(64, launch.refill_threshold));
+        p.hist_lo_ns = 0.0;
+        p.hist_width_ns = 200.0 / kHistBins;
+        return p;
+    }
+
+    // Stage the event list and the per-source chunk offsets, allocate outputs,
+    // launch and queue the metadata readback.  `emissions` is null for the
+    // synthetic source (one source per event).
+    void run_batch(const Emission* emissions, uint64_t n_emissions, const std::vector<uint32_t>& source_counts,
+                   const std::vector<uint32_t>& source_event, const SyntheticSource* synth) {
+        init_hip();
+        upload_scene();
+        const uint64_t n_events = event_ids.size();
+        const uint64_t n_sources = source_counts.size();
+        std::vector<uint32_t> offsets(n_sources + 1, 0);
+        for (uint64_t i = 0; i < n_sources; ++i) {
+            offsets[i + 1] = offsets[i] + (source_counts[i] + chunk - 1) / chunk;
+        }
+        n_items = offsets[n_sources];
+        const size_t em_bytes = n_emissions * sizeof(Emission);
+        const size_t ev_bytes = n_events * sizeof(uint32_t);
+        const size_t off_bytes = (n_sources + 1) * sizeof(uint32_t);
+        const size_t cnt_bytes = n_sources * sizeof(uint32_t);
+        const size_t sev_bytes = source_event.size() * sizeof(uint32_t);
+        const size_t o_ev = align256(em_bytes);
+        const size_t o_off = o_ev + align256(ev_bytes);
+        const size_t o_cnt = o_off + align256(off_bytes);
+        const size_t o_sev = o_cnt + align256(cnt_bytes);
+        const size_t stage_bytes = o_sev + align256(sev_bytes);
+        h_stage.ensure(stage_bytes, "hipHostMalloc staging");
+        char* st = h_stage.as<char>();
+        if (em_bytes) std::memcpy(st, emissions, em_bytes);
+        if (ev_bytes) std::memcpy(st + o_ev, event_ids.data(), ev_bytes);
+        std::memcpy(st + o_off, offsets.data(), off_bytes);
+        if (cnt_bytes) std::memcpy(st + o_cnt, source_counts.data(), cnt_bytes);
+        if (sev_bytes) std::memcpy(st + o_sev, source_event.data(), sev_bytes);
+        d_inputs.ensure(stage_bytes, "hipMalloc batch inputs");
+        d_hits.ensure(std::max<uint64_t>(n_photons, 1) * sizeof(Hit), "hipMalloc hits");
+        d_event_stats.ensure(std::max<uint64_t>(n_events, 1) * sizeof(EventStats), "hipMalloc event stats");
+        d_meta.ensure(kMetaBytes, "hipMalloc counters");
+        h_meta.ensure(kMetaBytes + n_events * sizeof(EventStats), "hipHostMalloc counters");
+
+        hip_check(hipEventRecord(ev_start, stream), "hipEventRecord");
+        hip_check(hipMemcpyAsync(d_inputs.ptr, st, stage_bytes, hipMemcpyHostToDevice, stream),
+                  "hipMemcpyAsync batch inputs");
+        h2d_bytes = em_bytes + ev_bytes + off_bytes + cnt_bytes + sev_bytes;
+        if (launch_zero(d_meta.ptr, kMetaBytes, stream) != 0 ||
+            launch_zero(d_event_stats.ptr, n_events * sizeof(EventStats), stream) != 0) {
+            fail(G4GO_ERROR_RUNTIME, "zero kernel launch failed");
+        }
+
+        KernelParams p = base_params();
+        if (synth != nullptr) {
+            p.synth = *synth;
+        }
+        char* in = d_inputs.as<char>();
+        p.emissions = reinterpret_cast<const Emission*>(in);
+        p.event_ids = reinterpret_cast<const uint32_t*>(in + o_ev);
+        p.item_offsets = reinterpret_cast<const uint32_t*>(in + o_off);
+        p.source_counts = reinterpret_cast<const uint32_t*>(in + o_cnt);
+        p.source_event = sev_bytes ? reinterpret_cast<const uint32_t*>(in + o_sev) : nullptr;
+        p.n_sources = static_cast<uint32_t>(n_sources);
+        p.chunk = chunk;
+        p.hits = d_hits.as<Hit>();
+        p.hit_capacity = static_cast<uint32_t>(std::min<uint64_t>(std::max<uint64_t>(n_photons, 1), 0xffffffffu));
+        p.hit_count = reinterpret_cast<uint32_t*>(d_meta.as<char>() + kMetaHitCount);
+        p.hit_source = nullptr;
+        p.histogram = reinterpret_cast<unsigned long long*>(d_meta.as<char>() + kMetaHist);
+        p.event_stats = d_event_stats.as<EventStats>();
+        p.counters = reinterpret_cast<unsigned long long*>(d_meta.as<char>() + kMetaCounters);
+
+        const int block = static_cast<int>(launch.block_size);
+        int grid;
+        if (launch.persistent) {
+            grid = static_cast<int>(std::max<uint32_t>(1, launch.blocks_per_cu) * static_cast<uint32_t>(cu_count));
+        } else {
+            const uint32_t waves_per_block = launch.block_size / 64;
+            grid = static_cast<int>((static_cast<uint64_t>(n_items) + waves_per_block - 1) / waves_per_block);
+        }
+        hip_check(hipEventRecord(ev_kernel0, stream), "hipEventRecord");
+        if (grid > 0 && launch_transport(p, source, launch.persistent != 0, block, grid, stream) != 0) {
+            fail(G4GO_ERROR_RUNTIME, "transport kernel launch failed");
+        }
+        hip_check(hipEventRecord(ev_kernel1, stream), "hipEventRecord");
+        hip_check(hipMemcpyAsync(h_meta.ptr, d_meta.ptr, kMetaBytes, hipMemcpyDeviceToHost, stream),
+                  "hipMemcpyAsync counters");
+        if (n_events) {
+            hip_check(hipMemcpyAsync(h_meta.as<char>() + kMetaBytes, d_event_stats.ptr, n_events * sizeof(EventStats),
+                                     hipMemcpyDeviceToHost, stream),
+                      "hipMemcpyAsync event stats");
+        }
+        hip_check(hipEventRecord(ev_meta, stream), "hipEventRecord");
+    }
+};
+
+namespace {
+
+template <class F>
+g4go_status guarded(g4go_transport* h, F&& f) {
+    try {
+        f();
+        return G4GO_OK;
+    } catch (const Failure& e) {
+        if (h != nullptr) {
+            h->error = e.message;
+        }
+        return e.status;
+    } catch (const std::bad_alloc&) {
+        if (h != nullptr) {
+            h->error = "host allocation failed";
+        }
+        return G4GO_ERROR_RUNTIME;
+    } catch (const std::exception& e) {
+        if (h != nullptr) {
+            h->error = e.what();
+        }
+        return G4GO_ERROR_RUNTIME;
+    }
+}
+
+void check_launch(const g4go_launch& l) {
+    if (l.block_size != 64 && l.block_size != 128 && l.block_size != 256) {
+        fail(G4GO_ERROR_INVALID_ARGUMENT, "block_size must be 64, 128 or 256");
+    }
+    if (l.persistent && (l.chunk == 0 || l.chunk > (1u << 20))) {
+        fail(G4GO_ERROR_INVALID_ARGUMENT, "chunk must be in [1, 2^20]");
+    }
+    if (l.refill_threshold == 0 || l.refill_threshold > 64) {
+        fail(G4GO_ERROR_INVALID_ARGUMENT, "refill_threshold must be in [1, 64]");
+    }
+    if (l.persistent && (l.blocks_per_cu == 0 || l.blocks_per_cu > 64)) {
+        fail(G4GO_ERROR_INVALID_ARGUMENT, "blocks_per_cu must be in [1, 64]");
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* g4go_version(void) { return "g4go-hip 0.1 (gfx950)"; }
+
+g4go_status g4go_create(const g4go_config* config, g4go_transport** out) {
+    if (out == nullptr || config == nullptr) {
+        g_create_error = "null argument";
+        return G4GO_ERROR_INVALID_ARGUMENT;
+    }
+    *out = nullptr;
+    if (!(config->boundary_tolerance_mm > 0.0f) || !std::isfinite(config->boundary_tolerance_mm)) {
+        g_create_error = "boundary tolerance must be positive";
+        return G4GO_ERROR_INVALID_ARGUMENT;
+    }
+    if (config->max_bounces_per_photon == 0) {
+        g_create_error = "max bounces must be positive";
+        return G4GO_ERROR_INVALID_ARGUMENT;
+    }
+    try {
+        auto* h = new g4go_transport();
+        h->cfg = *config;
+        h->launch = g4go_launch{0, 256, 64, 64, 8};
+        *out = h;
+        return G4GO_OK;
+    } catch (...) {
+        g_create_error = "host allocation failed";
+        return G4GO_ERROR_RUNTIME;
+    }
+}
+
+void g4go_destroy(g4go_transport* h) {
+    if (h == nullptr) {
+        return;
+    }
+    h->release();
+    delete h;
+}
+
+const char* g4go_last_error(const g4go_transport* h) { return h == nullptr ? g_create_error.c_str() : h->error.c_str(); }
+
+g4go_status g4go_set_launch(g4go_transport* h, const g4go_launch* launch) {
+    if (h == nullptr || launch == nullptr) {
+        return G4GO_ERROR_INVALID_ARGUMENT;
+    }
+    return guarded(h, [&] {
+        check_launch(*launch);
+        if (h->busy) {
+            fail(G4GO_ERROR_LOGIC, "launch form cannot change with a pending batch");
+        }
+        h->launch = *launch;
+    });
+}
+
+g4go_status g4go_get_launch(const g4go_transport* h, g4go_launch* launch) {
+    if (h == nullptr || launch == nullptr) {
+        return G4GO_ERROR_INVALID_ARGUMENT;
+    }
+    *launch = h->launch;
+    return G4GO_OK;
+}
+
+g4go_status g4go_prepare_scene(g4go_transport* h, const g4go_scene* scene) {
+    if (h == nullptr || scene == nullptr) {
+        return G4GO_ERROR_INVALID_ARGUMENT;
+    }
+    return guarded(h, [&] {
+        if (h->busy) {
+            fail(G4GO_ERROR_LOGIC, "scene cannot be rebuilt with pending batches");
+        }
+        HostScene s = build_scene(*scene, 0.5f * h->cfg.boundary_tolerance_mm);
+        h->scene = std::move(s);
+        h->scene_built = true;
+        h->scene_uploaded = false;
+    });
+}
+
+g4go_status g4go_enqueue_emissions(g4go_transport* h, const g4go_emission* emissions, uint64_t n_emissions,
+                                   const uint32_t* event_ids, uint64_t n_events,
+                                   const uint32_t* emission_event_indices) {
+    if (h == nullptr) {
+        return G4GO_ERROR_INVALID_ARGUMENT;
+    }
+    return guarded(h, [&] {
+        if (!h->scene_built) {
+            fail(G4GO_ERROR_LOGIC, "no scene has been prepared");
+        }
+        if (h->busy) {
+            fail(G4GO_ERROR_LOGIC, "transport queue is full");
+        }
+        if ((n_emissions != 0 && (emissions == nullptr || emission_event_indices == nullptr)) ||
+            (n_events != 0 && event_ids == nullptr)) {
+            fail(G4GO_ERROR_INVALID_ARGUMENT, "null batch array");
+        }
+        if (n_events > 0xffffffffull) {
+            fail(G4GO_ERROR_OVERFLOW, "event batch exceeds the supported event count");
+        }
+        std::unordered_map<uint32_t, uint32_t> index_of_event;
+        index_of_event.reserve(n_events);
+        for (uint64_t i = 0; i < n_events; ++i) {
+            if (!index_of_event.emplace(event_ids[i], static_cast<uint32_t>(i)).second) {
+                fail(G4GO_ERROR_INVALID_ARGUMENT, "event batch contains duplicate event IDs");
+            }
+        }
+        std::vector<uint64_t> captured(n_events, 0);
+        for (uint64_t i = 0; i < n_emissions; ++i) {
+            const uint32_t ei = emission_event_indices[i];
+            if (ei >= n_events || emissions[i].event_id != event_ids[ei]) {
+                fail(G4GO_ERROR_INVALID_ARGUMENT, "emission event index does not match event ID");
+            }
+            captured[ei] += emissions[i].photon_count;
+        }
+        if (n_emissions > 0xffffffffull) {
+            fail(G4GO_ERROR_OVERFLOW, "emission batch exceeds the supported launch size");
+        }
+        uint64_t photons = 0;
+        for (uint64_t i = 0; i < n_emissions; ++i) {
+            photons += emissions[i].photon_count;
+        }
+        if (photons > 0xffffffffull) {
+            fail(G4GO_ERROR_OVERFLOW, "emission batch exceeds the supported photon count");
+        }
+        std::vector<Emission> staged(n_emissions);
+        std::vector<uint32_t> counts(n_emissions), source_event(n_emissions);
+        for (uint64_t i = 0; i < n_emissions; ++i) {
+            std::memcpy(&staged[i], &emissions[i], sizeof(Emission));
+            const auto it = h->scene.index_of.find(emissions[i].volume_id);
+            if (it == h->scene.index_of.end()) {
+                fail(G4GO_ERROR_INVALID_ARGUMENT, "emission references an unknown scene volume ID");
+            }
+            if (emissions[i].type != 0 && emissions[i].material_id >= h->scene.materials.size()) {
+                fail(G4GO_ERROR_INVALID_ARGUMENT, "emission references an unknown scene material ID");
+            }
+            staged[i].volume = it->second;
+            counts[i] = emissions[i].photon_count;
+            source_event[i] = emission_event_indices[i];
+        }
+        h->source = kSourceEmission;
+        h->event_ids.assign(event_ids, event_ids + n_events);
+        h->captured = std::move(captured);
+        h->n_photons = photons;
+        h->chunk = h->launch.persistent ? h->launch.chunk : 64u;
+        h->started = std::chrono::steady_clock::now();
+        if (photons != 0) {
+            h->run_batch(staged.data(), n_emissions, counts, source_event, nullptr);
+        }
+        h->busy = true;
+    });
+}
+
+g4go_status g4go_enqueue_synthetic(g4go_transport* h, const g4go_synthetic_source* source, const uint32_t* event_ids,
+                                   const uint32_t* photon_counts, uint64_t n_events) {
+    if (h == nullptr) {
+        return G4GO_ERROR_INVALID_ARGUMENT;
+    }
+    return guarded(h, [&] {
+        if (!h->scene_built) {
+            fail(G4GO_ERROR_LOGIC, "no scene has been prepared");
+        }
+        if (h->busy) {
+            fail(G4GO_ERROR_LOGIC, "transport queue is full");
+        }
+        if (source == nullptr || (n_events != 0 && (event_ids == nullptr || photon_counts == nullptr))) {
+            fail(G4GO_ERROR_INVALID_ARGUMENT, "null synthetic batch argument");
+        }
+        if (n_events > 0xffffffffull) {
+            fail(G4GO_ERROR_OVERFLOW, "event batch exceeds the supported event count");
+        }
+        const auto it = h->scene.index_of.find(source->volume_id);
+        if (it == h->scene.index_of.end()) {
+            fail(G4GO_ERROR_INVALID_ARGUMENT, "synthetic source references an unknown volume ID");
+        }
+        for (int k = 0; k < 3; ++k) {
+            if (!std::isfinite(source->lo_mm[k]) || !std::isfinite(source->hi_mm[k]) || source->hi_mm[k] < source->lo_mm[k]) {
+                fail(G4GO_ERROR_INVALID_ARGUMENT, "synthetic source box is invalid");
+            }
+        }
+        if (!(source->wavelength_mean_nm > 0.0f) || !(source->wavelength_sigma_nm >= 0.0f) ||
+            !(source->decay_time_ns >= 0.0f)) {
+            fail(G4GO_ERROR_INVALID_ARGUMENT, "synthetic source distribution is invalid");
+        }
+        std::unordered_set<uint32_t> seen;
+        uint64_t photons = 0;
+        for (uint64_t i = 0; i < n_events; ++i) {
+            if (!seen.insert(event_ids[i]).second) {
+                fail(G4GO_ERROR_INVALID_ARGUMENT, "event batch contains duplicate event IDs");
+            }
+            photons += photon_counts[i];
+        }
+        if (photons > 0xffffffffull) {
+            fail(G4GO_ERROR_OVERFLOW, "synthetic batch exceeds the supported photon count");
+        }
+        SyntheticSource synth{};
+        for (int k = 0; k < 3; ++k) {
+            synth.lo[k] = source->lo_mm[k];
+            synth.hi[k] = source->hi_mm[k];
+        }
+        synth.wavelength_mean_nm = source->wavelength_mean_nm;
+        synth.wavelength_sigma_nm = source->wavelength_sigma_nm;
+        synth.decay_ns = source->decay_time_ns;
+        synth.volume = it->second;
+        h->source = kSourceSynthetic;
+        h->event_ids.assign(event_ids, event_ids + n_events);
+        h->captured.assign(photon_counts, photon_counts + n_events);
+        h->n_photons = photons;
+        h->chunk = h->launch.persistent ? h->launch.chunk : 64u;
+        h->started = std::chrono::steady_clock::now();
+        if (photons != 0) {
+            h->run_batch(nullptr, 0, h->captured_u32(), {}, &synth);
+        }
+        h->busy = true;
+    });
+}
+
+static g4go_status complete_batch(g4go_transport* h, g4go_batch_result* out, bool copy_hits,
+                                  const void** hits_device) {
+    if (h == nullptr || out == nullptr) {
+        return G4GO_ERROR_INVALID_ARGUMENT;
+    }
+    if (hits_device != nullptr) {
+        *hits_device = nullptr;
+    }
+    return guarded(h, [&] {
+        if (!h->busy) {
+            fail(G4GO_ERROR_LOGIC, "transport queue is empty");
+        }
+        h->busy = false;
+        h->device_hits = 0;
+        const uint64_t n_events = h->event_ids.size();
+        g4go_batch_stats bs{};
+        h->out_hits.clear();
+        h->out_events.assign(n_events, g4go_event_stats{});
+        std::fill(std::begin(h->out_hist), std::end(h->out_hist), 0ull);
+        for (uint64_t i = 0; i < n_events; ++i) {
+            h->out_events[i].event_id = h->event_ids[i];
+            h->out_events[i].captured = h->captured[i];
+            bs.captured += h->captured[i];
+        }
+        if (h->n_photons != 0) {
+            hip_check(hipEventSynchronize(h->ev_meta), "hipEventSynchronize");
+            const char* meta = h->h_meta.as<char>();
+            const auto* counters = reinterpret_cast<const uint64_t*>(meta + g4go_transport::kMetaCounters);
+            const uint32_t hit_count = *reinterpret_cast<const uint32_t*>(meta + g4go_transport::kMetaHitCount);
+            const auto* hist = reinterpret_cast<const uint64_t*>(meta + g4go_transport::kMetaHist);
+            const auto* es = reinterpret_cast<const EventStats*>(meta + g4go_transport::kMetaBytes);
+            const uint64_t n_hits = std::min<uint64_t>(hit_count, h->n_photons);
+            float kernel_ms = 0.0f, h2d_ms = 0.0f;
+            hip_check(hipEventElapsedTime(&kernel_ms, h->ev_kernel0, h->ev_kernel1), "hipEventElapsedTime");
+            hip_check(hipEventElapsedTime(&h2d_ms, h->ev_start, h->ev_kernel0), "hipEventElapsedTime");
+            const auto d2h0 = std::chrono::steady_clock::now();
+            h->device_hits = n_hits;
+            if (!copy_hits) {
+                if (hits_device != nullptr) {
+                    *hits_device = h->d_hits.ptr;
+                }
+            } else if (n_hits != 0) {
+                h->h_hits.ensure(n_hits * sizeof(Hit), "hipHostMalloc hits");
+                hip_check(hipMemcpyAsync(h->h_hits.ptr, h->d_hits.ptr, n_hits * sizeof(Hit), hipMemcpyDeviceToHost,
+                                         h->stream),
+                          "hipMemcpyAsync hits");
+                hip_check(hipStreamSynchronize(h->stream), "hipStreamSynchronize");
+                h->out_hits.resize(n_hits);
+                std::memcpy(h->out_hits.data(), h->h_hits.ptr, n_hits * sizeof(Hit));
+                if (h->cfg.sort_hits) {
+                    std::sort(h->out_hits.begin(), h->out_hits.end(), [](const g4go_hit& a, const g4go_hit& b) {
+                        return a.event_id < b.event_id || (a.event_id == b.event_id && a.photon_id < b.photon_id);
+                    });
+                }
+            }
+            const auto d2h1 = std::chrono::steady_clock::now();
+            std::memcpy(h->out_hist, hist, sizeof(h->out_hist));
+            for (uint64_t i = 0; i < n_events; ++i) {
+                g4go_event_stats& e = h->out_events[i];
+                e.detected = es[i].detected;
+                e.absorbed = es[i].absorbed;
+                e.escaped = es[i].escaped;
+                e.truncated = es[i].truncated;
+                e.max_bounce = es[i].max_bounce;
+                e.invalid = es[i].invalid;
+                e.zero_step = es[i].zero_step;
+                bs.detected += e.detected;
+                bs.absorbed += e.absorbed;
+                bs.escaped += e.escaped;
+                bs.truncated += e.truncated;
+                bs.invalid += e.invalid;
+                bs.zero_step += e.zero_step;
+                bs.max_bounce = std::max(bs.max_bounce, e.max_bounce);
+            }
+            bs.generated = counters[kCounterPhotons];
+            bs.total_steps = counters[kCounterSteps];
+            bs.dropped_hits = counters[kCounterDropped];
+            bs.kernel_ms = kernel_ms;
+            bs.host_to_device_ms = h2d_ms;
+            bs.device_to_host_ms = std::chrono::duration<double, std::milli>(d2h1 - d2h0).count();
+            bs.host_to_device_bytes = h->h2d_bytes;
+            bs.device_to_host_bytes = g4go_transport::kMetaBytes + n_events * sizeof(EventStats) + n_hits * sizeof(Hit);
+        }
+        bs.transport_time_ms =
+            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - h->started).count();
+        out->hits = copy_hits ? h->out_hits.data() : nullptr;
+        out->n_hits = copy_hits ? h->out_hits.size() : h->device_hits;
+        out->events = h->out_events.data();
+        out->n_events = h->out_events.size();
+        out->histogram = h->out_hist;
+        out->stats = bs;
+    });
+}
+
+g4go_status g4go_complete(g4go_transport* h, g4go_batch_result* out) { return complete_batch(h, out, true, nullptr); }
+
+g4go_status g4go_complete_device(g4go_transport* h, g4go_batch_result* out, const void** hits_device) {
+    return complete_batch(h, out, false, hits_device);
+}
+
+g4go_status g4go_eval_math(g4go_transport* h, int which, const float* in, float* out, uint64_t n) {
+    if (h == nullptr || in == nullptr || out == nullptr || which < 0 || which > 2 || n > (1u << 28)) {
+        return G4GO_ERROR_INVALID_ARGUMENT;
+    }
+    return guarded(h, [&] {
+        h->init_hip();
+        const size_t out_n = which == 2 ? 2 * n : n;
+        DevBuf din, dout;
+        din.ensure(n * sizeof(float), "hipMalloc");
+        dout.ensure(out_n * sizeof(float), "hipMalloc");
+        hip_check(hipMemcpy(din.ptr, in, n * sizeof(float), hipMemcpyHostToDevice), "hipMemcpy");
+        if (launch_eval_math(din.as<float>(), dout.as<float>(), static_cast<int>(n), which, h->stream) != 0) {
+            fail(G4GO_ERROR_RUNTIME, "eval kernel launch failed");
+        }
+        hip_check(hipStreamSynchronize(h->stream), "hipStreamSynchronize");
+        hip_check(hipMemcpy(out, dout.ptr, out_n * sizeof(float), hipMemcpyDeviceToHost), "hipMemcpy");
+        din.release();
+        dout.release();
+    });
+}
+
+g4go_status g4go_eval_fresnel(g4go_transport* h, const float* in, float* out, uint64_t n) {
+    if (h == nullptr || in == nullptr || out == nullptr || n > (1u << 24)) {
+        return G4GO_ERROR_INVALID_ARGUMENT;
+    }
+    return guarded(h, [&] {
+        h->init_hip();
+        DevBuf din, dout;
+        din.ensure(n * 11 * sizeof(float), "hipMalloc");
+        dout.ensure(n * 14 * sizeof(float), "hipMalloc");
+        hip_check(hipMemcpy(din.ptr, in, n * 11 * sizeof(float), hipMemcpyHostToDevice), "hipMemcpy");
+        if (launch_eval_fresnel(din.as<float>(), dout.as<float>(), static_cast<int>(n), h->stream) != 0) {
+            fail(G4GO_ERROR_RUNTIME, "eval kernel launch failed");
+        }
+        hip_check(hipStreamSynchronize(h->stream), "hipStreamSynchronize");
+        hip_check(hipMemcpy(out, dout.ptr, n * 14 * sizeof(float), hipMemcpyDeviceToHost), "hipMemcpy");
+        din.release();
+        dout.release();
+    });
+}
+
+}  // extern "C"

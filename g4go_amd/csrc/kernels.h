@@ -1,0 +1,96 @@
+// Launch-side declarations shared by the kernels and the C-ABI host code.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "transport.h"
+
+namespace g4go {
+
+constexpr int kHistBins = 256;
+constexpr int kHistSlots = kHistBins + 2;  // + underflow (256) + overflow (257)
+
+// Work is cut into items of `chunk` consecutive photons of one source record
+// (one event of a synthetic batch, or one emission record).  Items are not
+// stored: item c belongs to source s with item_offsets[s] <= c <
+// item_offsets[s + 1] (exclusive prefix of ceil(count / chunk)), found by a
+// wave-uniform binary search when the item is claimed.  This is the
+// reference's emissionOffsets layout (OptiXTransportHost.cpp:643-666) at item
+// rather than photon granularity.
+struct ItemRef {
+    uint32_t event_index;   // index into the batch's event list
+    uint32_t source_index;  // emission index (emission source) or event index (synthetic)
+    uint32_t first;         // first local photon index within the source record
+    uint32_t count;
+};
+
+// Per-event counters (src/optical/optix/OptiXDeviceData.cuh:159-167).
+struct EventStats {
+    unsigned long long detected, absorbed, escaped, truncated, max_bounce, invalid, zero_step;
+};
+
+enum Counter : int {
+    kCounterSteps = 0,      // transport steps executed (the reference's totalBounceCount)
+    kCounterPhotons = 1,    // photons generated
+    kCounterDropped = 2,    // detections beyond the hit capacity
+    kCounterQueueHead = 3,  // work-queue cursor (low 32 bits used)
+    kCounterCount = 4,
+};
+
+enum SourceKind : int { kSourceSynthetic = 0, kSourceEmission = 1 };
+
+struct KernelParams {
+    SceneView scene;
+    Rng rng;
+    StepConfig cfg;
+    uint32_t max_steps;
+    uint32_t refill_threshold;  // persistent form: refill when at least this many lanes idle
+    SyntheticSource synth;
+    const Emission* emissions;
+    const uint32_t* event_ids;       // per event index
+    const uint32_t* item_offsets;    // n_sources + 1, device resident (item count = item_offsets[n_sources])
+    const uint32_t* source_counts;   // photons per source record
+    const uint32_t* source_event;    // event index per source record (emission source)
+    uint32_t n_sources;
+    uint32_t chunk;
+    Hit* hits;
+    uint32_t hit_capacity;
+    uint32_t* hit_count;
+    uint32_t* hit_source;  // optional: work-item index of each hit (adjoint replay of emission sources)
+    unsigned long long* histogram;
+    EventStats* event_stats;
+    unsigned long long* counters;
+    double hist_lo_ns, hist_width_ns;
+};
+
+// Adjoint (path replay) parameters: the forward parameters plus the loss
+// weights per histogram slot and the parameter layout.
+struct AdjointLayout {
+    uint32_t n_abs_materials;        // materials whose absorption table is parameterised
+    uint32_t abs_material[64];       // device material index per parameter group
+    uint32_t abs_nodes;              // nodes per absorption table
+    uint32_t rayleigh_material;      // material whose Rayleigh length is a parameter (kInvalid = none)
+    uint32_t reflect_surface;        // surface whose reflectivity is a parameter
+    uint32_t pde_surface;            // surface whose efficiency table is parameterised
+    uint32_t pde_nodes;
+    uint32_t offset_rayleigh, offset_reflect, offset_pde, n_params;
+};
+
+struct AdjointParams {
+    KernelParams fwd;
+    const Hit* hits;
+    const uint32_t* hit_item;  // work-item index per hit (emission source), or null (synthetic)
+    uint32_t n_hits;
+    const double* slot_weight;  // dLoss/dN + dLoss/dh_b per histogram slot (kHistSlots)
+    double* grad;               // n_params
+    AdjointLayout layout;
+    unsigned long long* mismatch;  // replays whose path did not end in the recorded detection
+};
+
+int launch_transport(const KernelParams& p, int source, bool persistent, int block, int grid, hipStream_t stream);
+int launch_zero(void* ptr, size_t bytes, hipStream_t stream);
+int launch_adjoint(const AdjointParams& p, int source, int block, hipStream_t stream);
+int launch_eval_math(const float* in, float* out, int n, int which, hipStream_t stream);
+int launch_eval_fresnel(const float* in, float* out, int n, hipStream_t stream);
+
+}  // namespace g4go

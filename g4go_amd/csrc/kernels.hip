@@ -1,0 +1,397 @@
+// gfx950 transport kernels.
+//
+// One lane transports one photon at a time (wave64).  Two launch forms share
+// the same step code:
+//   * plain: each wave owns one work item of at most 64 photons, launched as
+//     one grid over the batch (the reference's one-thread-per-photon launch,
+//     OptiXTransportHost.cpp:746-750);
+//   * persistent: a fixed grid whose waves pop work items from a device
+//     queue and refill idle lanes in place once at least `refill_threshold`
+//     of the 64 lanes are idle, so long photon histories do not hold a whole
+//     wave.
+// Detections are compacted in place with one ballot, one popcount and one
+// global atomic per wave step (replacing the flag array and
+// cub::DeviceSelect::Flagged pass of OptiXHitCompaction.cu:8-30), and binned
+// into an LDS arrival-time histogram flushed once per block.
+#include "kernels.h"
+
+namespace g4go {
+
+namespace {
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ uint64_t lanes_below() {
+    const uint32_t l = lane_id();
+    return l == 0 ? 0ull : (~0ull >> (64u - l));
+}
+
+__device__ __forceinline__ uint32_t uniform(uint32_t v) {
+    return static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(v)));
+}
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        v += __shfl_xor(v, off);
+    }
+    return v;
+}
+
+__device__ __forceinline__ uint32_t wave_max(uint32_t v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        v = max(v, __shfl_xor(v, off));
+    }
+    return v;
+}
+
+struct LaneStats {
+    uint32_t detected, absorbed, escaped, truncated, invalid, zero_step, max_bounce;
+};
+
+__device__ __forceinline__ void flush_event(EventStats* es, uint32_t event, LaneStats& s) {
+    if (event == kInvalid) {
+        return;
+    }
+    EventStats& e = es[event];
+    if (s.detected) atomicAdd(&e.detected, static_cast<unsigned long long>(s.detected));
+    if (s.absorbed) atomicAdd(&e.absorbed, static_cast<unsigned long long>(s.absorbed));
+    if (s.escaped) atomicAdd(&e.escaped, static_cast<unsigned long long>(s.escaped));
+    if (s.truncated) atomicAdd(&e.truncated, static_cast<unsigned long long>(s.truncated));
+    if (s.invalid) atomicAdd(&e.invalid, static_cast<unsigned long long>(s.invalid));
+    if (s.zero_step) atomicAdd(&e.zero_step, static_cast<unsigned long long>(s.zero_step));
+    if (s.max_bounce) atomicMax(&e.max_bounce, static_cast<unsigned long long>(s.max_bounce));
+    s = LaneStats{0, 0, 0, 0, 0, 0, 0};
+}
+
+// End-of-kernel flush: one set of atomics per wave when every lane with
+// counts holds the same event, else per lane.
+__device__ __forceinline__ void flush_event_final(EventStats* es, uint32_t event, LaneStats& s) {
+    const uint64_t has = __ballot(event != kInvalid);
+    if (has == 0) {
+        return;
+    }
+    const uint32_t leader = static_cast<uint32_t>(__ffsll(static_cast<long long>(has)) - 1);
+    const uint32_t e0 = static_cast<uint32_t>(__shfl(static_cast<int>(event), static_cast<int>(leader)));
+    const uint64_t same = __ballot(event == e0 || event == kInvalid);
+    if (same == ~0ull) {
+        LaneStats r;
+        r.detected = wave_sum(s.detected);
+        r.absorbed = wave_sum(s.absorbed);
+        r.escaped = wave_sum(s.escaped);
+        r.truncated = wave_sum(s.truncated);
+        r.invalid = wave_sum(s.invalid);
+        r.zero_step = wave_sum(s.zero_step);
+        r.max_bounce = wave_max(s.max_bounce);
+        if (lane_id() == leader) {
+            flush_event(es, e0, r);
+        }
+        s = LaneStats{0, 0, 0, 0, 0, 0, 0};
+    } else {
+        flush_event(es, event, s);
+    }
+}
+
+__device__ __forceinline__ int hist_slot(double t, double lo, double width) {
+    const double x = (t - lo) / width;
+    if (!(x >= 0.0)) {
+        return kHistBins;  // underflow (and NaN)
+    }
+    if (x >= static_cast<double>(kHistBins)) {
+        return kHistBins + 1;
+    }
+    return static_cast<int>(x);
+}
+
+// Locates work item c (wave-uniform): binary search of the item offsets.
+template <int kSource>
+__device__ __forceinline__ ItemRef locate_item(const KernelParams& p, uint32_t c) {
+    uint32_t lo = 0, hi = p.n_sources;  // last s with item_offsets[s] <= c
+    while (hi - lo > 1u) {
+        const uint32_t mid = uniform((lo + hi) >> 1);
+        if (p.item_offsets[mid] <= c) {
+            lo = mid;
+        } else {
+            hi = mid;
+        }
+    }
+    ItemRef r;
+    r.source_index = uniform(lo);
+    r.first = uniform((c - p.item_offsets[lo]) * p.chunk);
+    r.count = uniform(min(p.chunk, p.source_counts[lo] - r.first));
+    r.event_index = kSource == kSourceSynthetic ? r.source_index : uniform(p.source_event[lo]);
+    return r;
+}
+
+template <int kSource>
+__device__ __forceinline__ void generate(Photon& ph, const KernelParams& p, uint32_t event_index, uint32_t source,
+                                         uint32_t local) {
+    if (kSource == kSourceSynthetic) {
+        generate_synthetic(ph, p.synth, p.rng, p.event_ids[event_index], local);
+    } else {
+        generate_emission(ph, p.emissions[source], p.scene, p.rng, local);
+    }
+}
+
+__device__ __forceinline__ void store_hit(Hit* dst, const Photon& ph, uint32_t sensor) {
+    float4* d = reinterpret_cast<float4*>(dst);
+    d[0] = make_float4(ph.pos.x, ph.pos.y, ph.pos.z, ph.dir.x);
+    d[1] = make_float4(ph.dir.y, ph.dir.z, ph.energy, __uint_as_float(sensor));
+    const uint64_t tb = __double_as_longlong(ph.time);
+    d[2] = make_float4(__uint_as_float(ph.event_id), __uint_as_float(ph.photon_id),
+                       __uint_as_float(static_cast<uint32_t>(tb)), __uint_as_float(static_cast<uint32_t>(tb >> 32)));
+}
+
+}  // namespace
+
+template <int kSource, bool kPersistent>
+__global__ void __launch_bounds__(256) transport_kernel(KernelParams p) {
+    __shared__ uint32_t s_hist[kHistSlots];
+    for (int i = threadIdx.x; i < kHistSlots; i += blockDim.x) {
+        s_hist[i] = 0u;
+    }
+    __syncthreads();
+
+    const uint32_t lane = lane_id();
+    const uint64_t below = lanes_below();
+    const uint32_t n_items = p.item_offsets[p.n_sources];
+
+    Photon ph;
+    ph.material = kInvalid;
+    bool alive = false;
+    uint32_t my_event = kInvalid;
+    uint32_t my_item = kInvalid;
+    LaneStats st{0, 0, 0, 0, 0, 0, 0};
+    uint32_t lane_steps = 0, lane_photons = 0;
+
+    // wave-uniform queue cursor
+    uint32_t it_index = kInvalid, it_event = 0, it_source = 0, it_first = 0, it_next = 0, it_end = 0;
+    bool done = false;
+    if (!kPersistent) {
+        const uint32_t wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+        if (wave < n_items) {
+            const ItemRef item = locate_item<kSource>(p, uniform(wave));
+            it_index = wave;
+            it_event = item.event_index;
+            it_source = item.source_index;
+            it_first = item.first;
+            it_end = item.count;
+        }
+        done = true;
+    }
+
+    for (;;) {
+        // ---- refill idle lanes ----
+        uint64_t idle = __ballot(!alive);
+        uint32_t n_idle = static_cast<uint32_t>(__popcll(idle));
+        if (!kPersistent || n_idle >= p.refill_threshold || n_idle == 64u) {
+            while (n_idle > 0u) {
+                if (it_next >= it_end) {
+                    if (done) {
+                        break;
+                    }
+                    uint32_t c = 0;
+                    if (lane == 0u) {
+                        c = atomicAdd(reinterpret_cast<uint32_t*>(p.counters + kCounterQueueHead), 1u);
+                    }
+                    c = uniform(static_cast<uint32_t>(__shfl(static_cast<int>(c), 0)));
+                    if (c >= n_items) {
+                        done = true;
+                        break;
+                    }
+                    const ItemRef item = locate_item<kSource>(p, c);
+                    it_index = c;
+                    it_event = item.event_index;
+                    it_source = item.source_index;
+                    it_first = item.first;
+                    it_next = 0;
+                    it_end = item.count;
+                    continue;
+                }
+                const uint32_t take = min(it_end - it_next, n_idle);
+                const uint32_t rank = static_cast<uint32_t>(__popcll(idle & below));
+                if (!alive && rank < take) {
+                    if (my_event != it_event) {
+                        flush_event(p.event_stats, my_event, st);
+                        my_event = it_event;
+                    }
+                    generate<kSource>(ph, p, it_event, it_source, it_first + it_next + rank);
+                    ph.material = kInvalid;
+                    my_item = it_index;
+                    ++lane_photons;
+                    if (start_photon(ph)) {
+                        alive = true;
+                    } else {
+                        ++st.invalid;
+                    }
+                }
+                it_next += take;
+                idle = __ballot(!alive);
+                n_idle = static_cast<uint32_t>(__popcll(idle));
+            }
+        }
+        if (__ballot(alive) == 0ull) {
+            if (done) {
+                break;
+            }
+            continue;
+        }
+
+        // ---- one transport step ----
+        int status = kAlive;
+        uint32_t sensor = kInvalid;
+        if (alive) {
+            bool zero = false;
+            status = transport_step<false>(ph, p.scene, p.rng, p.cfg, zero, sensor, nullptr);
+            ++ph.step;
+            ++lane_steps;
+            st.zero_step += zero ? 1u : 0u;
+            if (status == kAlive && ph.step >= p.max_steps) {
+                status = -1;  // truncated
+            }
+        }
+
+        // ---- fused hit compaction ----
+        const uint64_t det = __ballot(status == kDetected);
+        if (det != 0ull) {
+            uint32_t base = 0;
+            if (lane == 0u) {
+                base = atomicAdd(p.hit_count, static_cast<uint32_t>(__popcll(det)));
+            }
+            base = static_cast<uint32_t>(__shfl(static_cast<int>(base), 0));
+            if (status == kDetected) {
+                const uint32_t idx = base + static_cast<uint32_t>(__popcll(det & below));
+                if (idx < p.hit_capacity) {
+                    store_hit(p.hits + idx, ph, sensor);
+                    if (p.hit_source != nullptr) {
+                        p.hit_source[idx] = my_item;
+                    }
+                } else {
+                    atomicAdd(p.counters + kCounterDropped, 1ull);
+                }
+                atomicAdd(&s_hist[hist_slot(ph.time, p.hist_lo_ns, p.hist_width_ns)], 1u);
+            }
+        }
+        if (alive && status != kAlive) {
+            alive = false;
+            st.max_bounce = max(st.max_bounce, ph.step);
+            if (status == kDetected) {
+                ++st.detected;
+            } else if (status == kAbsorbedBulk || status == kAbsorbedSurface) {
+                ++st.absorbed;
+            } else if (status == kEscaped) {
+                ++st.escaped;
+            } else if (status == kInvalidState) {
+                ++st.invalid;
+            } else {
+                ++st.truncated;
+            }
+        }
+    }
+
+    flush_event_final(p.event_stats, my_event, st);
+    const uint32_t ws = wave_sum(lane_steps);
+    const uint32_t wp = wave_sum(lane_photons);
+    if (lane == 0u) {
+        if (ws) atomicAdd(p.counters + kCounterSteps, static_cast<unsigned long long>(ws));
+        if (wp) atomicAdd(p.counters + kCounterPhotons, static_cast<unsigned long long>(wp));
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kHistSlots; i += blockDim.x) {
+        if (s_hist[i] != 0u) {
+            atomicAdd(p.histogram + i, static_cast<unsigned long long>(s_hist[i]));
+        }
+    }
+}
+
+__global__ void zero_kernel(uint32_t* ptr, size_t words) {
+    for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; i < words;
+         i += static_cast<size_t>(gridDim.x) * blockDim.x) {
+        ptr[i] = 0u;
+    }
+}
+
+int launch_zero(void* ptr, size_t bytes, hipStream_t stream) {
+    const size_t words = bytes / 4;
+    if (words == 0) {
+        return 0;
+    }
+    const int grid = static_cast<int>(std::min<size_t>((words + 255) / 256, 1024));
+    hipLaunchKernelGGL(zero_kernel, dim3(grid), dim3(256), 0, stream, static_cast<uint32_t*>(ptr), words);
+    return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
+int launch_transport(const KernelParams& p, int source, bool persistent, int block, int grid, hipStream_t stream) {
+    if (source == kSourceSynthetic) {
+        if (persistent) {
+            hipLaunchKernelGGL((transport_kernel<kSourceSynthetic, true>), dim3(grid), dim3(block), 0, stream, p);
+        } else {
+            hipLaunchKernelGGL((transport_kernel<kSourceSynthetic, false>), dim3(grid), dim3(block), 0, stream, p);
+        }
+    } else {
+        if (persistent) {
+            hipLaunchKernelGGL((transport_kernel<kSourceEmission, true>), dim3(grid), dim3(block), 0, stream, p);
+        } else {
+            hipLaunchKernelGGL((transport_kernel<kSourceEmission, false>), dim3(grid), dim3(block), 0, stream, p);
+        }
+    }
+    return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
+// ---- element-wise evaluation kernels used by the parity tests ----
+__global__ void eval_math_kernel(const float* in, float* out, int n, int which) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) {
+        return;
+    }
+    const float x = in[i];
+    if (which == 0) {
+        out[i] = g4_logf(x);
+    } else if (which == 1) {
+        out[i] = g4_expf(x);
+    } else {
+        float s, c;
+        g4_sincos_turn(x, s, c);
+        out[2 * i] = s;
+        out[2 * i + 1] = c;
+    }
+}
+
+int launch_eval_math(const float* in, float* out, int n, int which, hipStream_t stream) {
+    if (n <= 0) {
+        return 0;
+    }
+    hipLaunchKernelGGL(eval_math_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, in, out, n, which);
+    return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
+// in: 11 floats per case (dir[3], pol[3], normal[3], n1, n2)
+// out: 14 floats per case (T, tir, reflected dir[3], transmitted dir[3], reflected pol[3], transmitted pol[3])
+__global__ void eval_fresnel_kernel(const float* in, float* out, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) {
+        return;
+    }
+    const float* a = in + 11 * i;
+    const Fresnel f = compute_fresnel(V3{a[0], a[1], a[2]}, V3{a[3], a[4], a[5]}, V3{a[6], a[7], a[8]}, a[9], a[10]);
+    float* o = out + 14 * i;
+    o[0] = f.transmittance;
+    o[1] = f.tir ? 1.0f : 0.0f;
+    const V3 vs[4] = {f.reflected_dir, f.transmitted_dir, f.reflected_pol, f.transmitted_pol};
+    for (int k = 0; k < 4; ++k) {
+        o[2 + 3 * k] = vs[k].x;
+        o[3 + 3 * k] = vs[k].y;
+        o[4 + 3 * k] = vs[k].z;
+    }
+}
+
+int launch_eval_fresnel(const float* in, float* out, int n, hipStream_t stream) {
+    if (n <= 0) {
+        return 0;
+    }
+    hipLaunchKernelGGL(eval_fresnel_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, in, out, n);
+    return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
+}  // namespace g4go

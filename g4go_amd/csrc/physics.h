@@ -1,0 +1,342 @@
+// Optical-photon physics shared by every transport kernel of the gfx950 backend.
+//
+// All arithmetic here is written so that a CPU restatement compiled with
+// IEEE single precision, no FP contraction and the same operation order
+// reproduces it bit for bit:
+//   * add, sub, mul, div and sqrt are correctly rounded on both sides
+//     (hipcc: -ffp-contract=off -fhip-fp32-correctly-rounded-divide-sqrt);
+//   * fused multiply-adds are written explicitly (fmaf), never contracted;
+//   * log, exp, sin and cos are project-written from Cephes (S. L. Moshier,
+//     "Methods and Programs for Mathematical Functions", 1989) minimax
+//     polynomials instead of the device library, whose last-ulp behaviour
+//     differs from any host libm.
+//
+// Reference algorithm: src/optical/optix/OptiXBoundaryPhysics.cuh and
+// src/optical/optix/OptiXTransportDevice.cu of G4GO.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define G4GO_HD __host__ __device__ __forceinline__
+
+namespace g4go {
+
+constexpr uint32_t kInvalid = 0xffffffffu;
+constexpr float kInf = __builtin_huge_valf();
+constexpr float kSpeedOfLightMmPerNs = 299.792458f;
+constexpr float kHcEvNm = 1239.84198f;  // h*c in eV*nm
+
+G4GO_HD uint32_t f2u(float x) { return __builtin_bit_cast(uint32_t, x); }
+G4GO_HD float u2f(uint32_t x) { return __builtin_bit_cast(float, x); }
+G4GO_HD bool finitef(float x) { return (f2u(x) & 0x7f800000u) != 0x7f800000u; }
+
+// ---------------------------------------------------------------------------
+// Philox4x32-10 (Salmon et al., SC'11).  Counter layout follows the reference
+// Uniform() (OptiXTransportDevice.cu:254-286):
+//   counter = {lo32(photonKey), hi32(photonKey), step, (process << 16) ^ block}
+//   key     = {lo32(seed), hi32(seed)}
+// with photonKey = (eventID << 32) | photonID.  One block yields four words;
+// word k of block b is draw 4b+k of that (step, process).  Draw 4b is the
+// reference's Uniform(seed, key, step, process, 4b) exactly.
+// ---------------------------------------------------------------------------
+struct U4 {
+    uint32_t x, y, z, w;
+};
+
+G4GO_HD U4 philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0, uint32_t k1) {
+#pragma unroll
+    for (int round = 0; round < 10; ++round) {
+        const uint64_t p0 = static_cast<uint64_t>(0xD2511F53u) * c0;
+        const uint64_t p1 = static_cast<uint64_t>(0xCD9E8D57u) * c2;
+        const uint32_t n0 = static_cast<uint32_t>(p1 >> 32) ^ c1 ^ k0;
+        const uint32_t n2 = static_cast<uint32_t>(p0 >> 32) ^ c3 ^ k1;
+        c1 = static_cast<uint32_t>(p1);
+        c3 = static_cast<uint32_t>(p0);
+        c0 = n0;
+        c2 = n2;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    return {c0, c1, c2, c3};
+}
+
+// (float(word) + 0.5) / 2^32, bit-identical to the reference mapping
+// (OptiXTransportDevice.cu:285); the result lies in (0, 1].
+G4GO_HD float u01(uint32_t word) { return fmaf(static_cast<float>(word), 0x1p-32f, 0x1p-33f); }
+
+// Process identifiers of this backend's draw map.
+enum Process : uint32_t {
+    kProcStep = 1,       // per step: w0 distance, w1 bulk process, w2 surface class, w3 Fresnel/detect
+    kProcRayleigh = 2,   // Rayleigh angular sampling, block = attempt
+    kProcFacet = 3,      // ground-surface facet normal, block = attempt
+    kProcLobe = 4,       // unified-model lobe choice and Lambertian draws
+    kProcRough = 5,      // roughness test and Lambertian draws
+    kProcSynthetic = 6,  // synthetic source generation (step 0)
+    kProcEmission = 7,   // emission-record expansion (step 0)
+};
+
+struct Rng {
+    uint32_t seed_lo, seed_hi;
+    G4GO_HD U4 block(uint32_t key_lo, uint32_t key_hi, uint32_t step, uint32_t process, uint32_t blk) const {
+        return philox(key_lo, key_hi, step, (process << 16) ^ blk, seed_lo, seed_hi);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Elementary functions (Cephes single precision, evaluated with explicit fma).
+// ---------------------------------------------------------------------------
+
+// Natural logarithm; x > 0.  Cephes logf: x = m * 2^e, m in [sqrt(1/2), sqrt(2)),
+// log(1+f) = f - f^2/2 + f^3 P(f), Cody-Waite split of ln 2.
+G4GO_HD float g4_logf(float x) {
+    if (!(x > 0.0f)) {
+        return x == 0.0f ? -kInf : u2f(0x7fc00000u);
+    }
+    if (!finitef(x)) {
+        return x;
+    }
+    uint32_t b = f2u(x);
+    int e = 0;
+    if (b < 0x00800000u) {  // subnormal input
+        x = x * 33554432.0f;
+        b = f2u(x);
+        e = -25;
+    }
+    e += static_cast<int>(b >> 23) - 126;
+    float m = u2f((b & 0x007fffffu) | 0x3f000000u);  // [0.5, 1)
+    if (m < 0.70710678118654752f) {
+        e -= 1;
+        m = (m + m) - 1.0f;
+    } else {
+        m = m - 1.0f;
+    }
+    const float z = m * m;
+    float y = 7.0376836292e-2f;
+    y = fmaf(y, m, -1.1514610310e-1f);
+    y = fmaf(y, m, 1.1676998740e-1f);
+    y = fmaf(y, m, -1.2420140846e-1f);
+    y = fmaf(y, m, 1.4249322787e-1f);
+    y = fmaf(y, m, -1.6668057665e-1f);
+    y = fmaf(y, m, 2.0000714765e-1f);
+    y = fmaf(y, m, -2.4999993993e-1f);
+    y = fmaf(y, m, 3.3333331174e-1f);
+    y = (y * m) * z;
+    const float fe = static_cast<float>(e);
+    y = fmaf(fe, -2.12194440e-4f, y);
+    y = fmaf(-0.5f, z, y);
+    float r = m + y;
+    r = fmaf(fe, 0.693359375f, r);
+    return r;
+}
+
+// Exponential.  Cephes expf: n = round(x / ln 2), Cody-Waite reduction,
+// degree-6 polynomial, exact scaling by 2^n.  Results below the normal
+// range are flushed to zero (the backend only uses exp(-a) probabilities).
+G4GO_HD float g4_expf(float x) {
+    if (x != x) {
+        return x;
+    }
+    if (x > 88.7228391f) {
+        return kInf;
+    }
+    if (x < -87.3365447f) {
+        return 0.0f;
+    }
+    const float n = floorf(fmaf(x, 1.44269504088896341f, 0.5f));
+    float r = fmaf(n, -0.693359375f, x);
+    r = fmaf(n, 2.12194440e-4f, r);
+    const float z = r * r;
+    float y = 1.9875691500e-4f;
+    y = fmaf(y, r, 1.3981999507e-3f);
+    y = fmaf(y, r, 8.3334519073e-3f);
+    y = fmaf(y, r, 4.1665795894e-2f);
+    y = fmaf(y, r, 1.6666665459e-1f);
+    y = fmaf(y, r, 5.0000001201e-1f);
+    y = fmaf(y, z, r);
+    y = y + 1.0f;
+    int ni = static_cast<int>(n);
+    if (ni > 127) {
+        y = y * 2.0f;
+        ni -= 1;
+    }
+    return y * u2f(static_cast<uint32_t>(ni + 127) << 23);
+}
+
+// sin and cos of 2*pi*u.  The reduction is done in turns (exact), then the
+// Cephes sinf/cosf minimax polynomials on [-pi/4, pi/4].
+G4GO_HD void g4_sincos_turn(float u, float& s, float& c) {
+    const float t = u * 4.0f;
+    const float q = floorf(t + 0.5f);
+    const float x = (t - q) * 1.57079632679489662f;
+    const float z = x * x;
+    const float sp = fmaf(fmaf(-1.9515295891e-4f, z, 8.3321608736e-3f), z, -1.6666654611e-1f);
+    const float sn = fmaf(sp * z, x, x);
+    const float cp = fmaf(fmaf(2.443315711809948e-5f, z, -1.388731625493765e-3f), z, 4.166664568298827e-2f);
+    const float cs = fmaf(cp * z, z, fmaf(-0.5f, z, 1.0f));
+    const int quadrant = static_cast<int>(q) & 3;
+    if (quadrant == 0) {
+        s = sn;
+        c = cs;
+    } else if (quadrant == 1) {
+        s = cs;
+        c = -sn;
+    } else if (quadrant == 2) {
+        s = -sn;
+        c = -cs;
+    } else {
+        s = -cs;
+        c = sn;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Vectors (same operation order as OptiXBoundaryPhysics.cuh:16-45).
+// ---------------------------------------------------------------------------
+struct V3 {
+    float x, y, z;
+};
+
+G4GO_HD V3 add(V3 a, V3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+G4GO_HD V3 sub(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+G4GO_HD V3 scale(V3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+G4GO_HD float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+G4GO_HD V3 cross(V3 a, V3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+G4GO_HD float axis_of(V3 a, int k) { return k == 0 ? a.x : (k == 1 ? a.y : a.z); }
+
+// OptiXBoundaryPhysics.cuh:47-54
+G4GO_HD V3 normalize(V3 a) {
+    const float l2 = dot(a, a);
+    if (!(l2 > 0.0f)) {
+        return {0.0f, 0.0f, 0.0f};
+    }
+    return scale(a, 1.0f / sqrtf(l2));
+}
+
+G4GO_HD float clamp01(float v) { return fminf(fmaxf(v, 0.0f), 1.0f); }
+
+// OptiXBoundaryPhysics.cuh:57-67
+G4GO_HD V3 project_polarization(V3 pol, V3 dir) {
+    V3 out = sub(pol, scale(dir, dot(pol, dir)));
+    if (dot(out, out) < 1.0e-12f) {
+        const V3 helper = fabsf(dir.z) < 0.9f ? V3{0.0f, 0.0f, 1.0f} : V3{1.0f, 0.0f, 0.0f};
+        out = cross(dir, helper);
+    }
+    return normalize(out);
+}
+
+// OptiXBoundaryPhysics.cuh:69-71
+G4GO_HD V3 reflect_direction(V3 dir, V3 n) { return normalize(sub(dir, scale(n, 2.0f * dot(dir, n)))); }
+
+// OptiXBoundaryPhysics.cuh:73-77
+G4GO_HD V3 reflect_polarization(V3 pol, V3 n, V3 dir) {
+    const V3 r = add(scale(pol, -1.0f), scale(n, 2.0f * dot(pol, n)));
+    return project_polarization(r, dir);
+}
+
+enum SurfaceOutcome : int { kAbsorb = 0, kDetect = 1, kDirectTransmit = 2, kInteraction = 3 };
+
+// OptiXBoundaryPhysics.cuh:79-94
+G4GO_HD int classify_surface(float reflectivity, float transmittance, bool has_r, bool has_t, float sample) {
+    const float u = clamp01(sample);
+    const float r = has_r ? clamp01(reflectivity) : 0.0f;
+    const float t = has_t ? clamp01(transmittance) : 0.0f;
+    if (!has_r && !has_t) {
+        return kInteraction;
+    }
+    if (u > r + t) {
+        return kAbsorb;
+    }
+    if (u > r) {
+        return kDirectTransmit;
+    }
+    return kInteraction;
+}
+
+// OptiXBoundaryPhysics.cuh:96-101
+G4GO_HD int evaluate_absorption(float efficiency, bool sensor, float sample) {
+    return (sensor && sample < clamp01(efficiency)) ? kDetect : kAbsorb;
+}
+
+// OptiXBoundaryPhysics.cuh:103-120 (cos/sin of the azimuth via g4_sincos_turn)
+G4GO_HD V3 sample_lambertian(V3 n, float radial_sample, float azimuth_sample) {
+    n = normalize(n);
+    radial_sample = clamp01(radial_sample);
+    azimuth_sample = clamp01(azimuth_sample);
+    const float radial = sqrtf(radial_sample);
+    float s, c;
+    g4_sincos_turn(azimuth_sample, s, c);
+    const float normal_component = sqrtf(fmaxf(0.0f, 1.0f - radial_sample));
+    const V3 helper = fabsf(n.z) < 0.9f ? V3{0.0f, 0.0f, 1.0f} : V3{1.0f, 0.0f, 0.0f};
+    const V3 tangent = normalize(cross(helper, n));
+    const V3 bitangent = cross(n, tangent);
+    return normalize(add(add(scale(tangent, radial * c), scale(bitangent, radial * s)), scale(n, normal_component)));
+}
+
+struct Fresnel {
+    V3 reflected_dir, transmitted_dir, reflected_pol, transmitted_pol;
+    float transmittance;
+    bool tir;
+};
+
+// Fresnel equations with polarisation (OptiXBoundaryPhysics.cuh:122-178).
+G4GO_HD Fresnel compute_fresnel(V3 dir, V3 pol, V3 n, float n1, float n2) {
+    dir = normalize(dir);
+    n = normalize(n);
+    pol = project_polarization(pol, dir);
+    const float cos_i = clamp01(-dot(dir, n));
+    const float eta = n1 / n2;
+    const float sin_t2 = eta * eta * (1.0f - cos_i * cos_i);
+    Fresnel out{};
+    out.reflected_dir = reflect_direction(dir, n);
+    out.tir = sin_t2 >= 1.0f;
+    out.transmittance = 0.0f;
+    if (out.tir) {
+        out.reflected_pol = reflect_polarization(pol, n, out.reflected_dir);
+        return out;
+    }
+    const float cos_t = sqrtf(fmaxf(0.0f, 1.0f - sin_t2));
+    out.transmitted_dir = normalize(add(scale(dir, eta), scale(n, eta * cos_i - cos_t)));
+    const V3 transverse = cross(dir, n);
+    if (dot(transverse, transverse) < 1.0e-12f) {
+        const float ra = (n1 - n2) / (n1 + n2);
+        out.transmittance = clamp01(1.0f - ra * ra);
+        out.reflected_pol = n2 > n1 ? scale(pol, -1.0f) : pol;
+        out.transmitted_pol = pol;
+        return out;
+    }
+    const V3 s_dir = normalize(transverse);
+    const V3 p_dir = normalize(cross(dir, s_dir));
+    const float in_s = dot(pol, s_dir);
+    const float in_p = dot(pol, p_dir);
+    const float s_den = n1 * cos_i + n2 * cos_t;
+    const float p_den = n2 * cos_i + n1 * cos_t;
+    const float rs = (n1 * cos_i - n2 * cos_t) / s_den;
+    const float rp = (n2 * cos_i - n1 * cos_t) / p_den;
+    const float ts = 2.0f * n1 * cos_i / s_den;
+    const float tp = 2.0f * n1 * cos_i / p_den;
+    const float refl = in_s * in_s * rs * rs + in_p * in_p * rp * rp;
+    out.transmittance = clamp01(1.0f - refl);
+    const V3 rp_dir = normalize(cross(out.reflected_dir, s_dir));
+    const V3 tp_dir = normalize(cross(out.transmitted_dir, s_dir));
+    out.reflected_pol = normalize(add(scale(s_dir, in_s * rs), scale(rp_dir, in_p * rp)));
+    out.transmitted_pol = normalize(add(scale(s_dir, in_s * ts), scale(tp_dir, in_p * tp)));
+    return out;
+}
+
+// Rotates v from the frame whose z axis is u into the global frame
+// (CLHEP Hep3Vector::rotateUz, used by Geant4 G4OpRayleigh).
+G4GO_HD V3 rotate_uz(V3 v, V3 u) {
+    const float up2 = u.x * u.x + u.y * u.y;
+    if (up2 > 0.0f) {
+        const float up = sqrtf(up2);
+        return {(u.x * u.z * v.x - u.y * v.y) / up + u.x * v.z, (u.y * u.z * v.x + u.x * v.y) / up + u.y * v.z,
+                -up * v.x + u.z * v.z};
+    }
+    if (u.z < 0.0f) {
+        return {-v.x, v.y, -v.z};
+    }
+    return v;
+}
+
+}  // namespace g4go

@@ -1,0 +1,87 @@
+// Flat, pointer-free scene layout uploaded once per PrepareScene.
+//
+// Reference model: include/g4go/optical/Scene.hpp:61-164 and the device copy
+// in src/optical/optix/OptiXDeviceData.cuh:68-144.  Differences:
+//   * geometry is an axis-aligned box per volume (analytic slab tests replace
+//     the OptiX triangle traversal), with a parent relation for nesting;
+//   * every property table lives in one float pool (energies then values),
+//     addressed by offset, so the whole scene is a handful of arrays;
+//   * the border/skin surface of every static box face, and of every
+//     parent->daughter entry, is resolved on the host (Scene.cpp:120-150).
+#pragma once
+
+#include <stdint.h>
+
+namespace g4go {
+
+constexpr uint32_t kFaceDynamic = 0xfffffffeu;  // neighbour found by a point query at run time
+
+struct Prop {
+    uint32_t offset;    // energies at pool[offset], values at pool[offset + count]
+    uint32_t count;     // 0 = property absent
+    uint32_t constant;  // all values equal (OptiXTransportHost.cpp:356-369)
+    uint32_t pad;
+};
+
+struct Material {
+    Prop rindex;
+    Prop group_velocity;
+    Prop abs_length;
+    Prop rayleigh_length;
+    Prop spectrum[3];  // normalised CDF over energy (OptiXTransportHost.cpp:371-410)
+    float rindex_max;
+    uint32_t pad[3];
+};
+
+enum SurfaceType : uint32_t { kDielectricDielectric = 0, kDielectricMetal = 1 };
+enum SurfaceModel : uint32_t { kGlisur = 0, kUnified = 1 };
+enum SurfaceFinish : uint32_t { kPolished = 0, kGround = 1 };
+
+struct Surface {
+    uint32_t type, model, finish;
+    float model_value;
+    Prop reflectivity, efficiency, transmittance, lobe, spike, backscatter, roughness;
+};
+
+struct Volume {
+    float lo[3];
+    uint32_t material;
+    float hi[3];
+    uint32_t parent;
+    uint32_t skin, sensor;
+    uint32_t daughter_begin, daughter_count;  // indices into the daughter list
+    uint32_t entry_surface;                   // surface crossed when entering from the parent
+    uint32_t bvh_root;                        // root node of the daughter BVH, kInvalid = linear list
+    uint32_t pad[2];
+    uint32_t face_neighbor[6];  // face = 2*axis + (outward sign > 0); volume, kInvalid (leave world) or kFaceDynamic
+    uint32_t face_surface[6];   // surface for a static face neighbour, kInvalid = none
+};
+
+struct Binding {
+    uint32_t from, to, surface, pad;  // sorted by (from, to)
+};
+
+// Bounding-volume-hierarchy node over the daughter boxes of one volume.
+// Leaves hold up to four daughters (indices into the daughter list).
+struct BvhNode {
+    float lo[3];
+    uint32_t left;  // inner: left child node; leaf: first daughter-list slot
+    float hi[3];
+    uint32_t right_or_count;  // inner: right child node; leaf: count | kBvhLeaf
+};
+constexpr uint32_t kBvhLeaf = 0x80000000u;
+
+struct SceneView {
+    const float* pool;
+    const Material* materials;
+    const Surface* surfaces;
+    const Volume* volumes;
+    const uint32_t* daughters;
+    const Binding* bindings;
+    const BvhNode* bvh;
+    uint32_t n_materials, n_surfaces, n_volumes, n_bindings;
+    uint32_t n_bvh_nodes, world;
+    uint32_t pad[2];
+};
+
+}  // namespace g4go

@@ -1,0 +1,918 @@
+// Per-photon transport: property lookup, box geometry, photon generation and
+// one transport step.  Used by the plain, persistent and adjoint kernels.
+//
+// Reference: src/optical/optix/OptiXTransportDevice.cu:198-595 (helpers) and
+// :660-941 (the bounce loop of __raygen__rg).
+#pragma once
+
+#include "physics.h"
+#include "scene.h"
+
+namespace g4go {
+
+// OpticalEmission, 128 bytes (include/g4go/optical/PhotonTransport.hpp:40-69).
+struct alignas(16) Emission {
+    float pos[3];
+    float time;
+    float dir[3];
+    float step_length;
+    float step_delta[3];
+    float pre_velocity;
+    float pol[3];
+    float delta_velocity;
+    float energy;
+    float pre_mean;
+    float post_mean;
+    float decay;
+    float rise;
+    float charge;
+    uint32_t event_id;
+    uint32_t first_photon_id;
+    uint32_t photon_count;
+    uint32_t volume;  // device volume index (remapped on the host)
+    uint32_t material;
+    uint32_t spectrum;
+    uint8_t type;  // 0 direct, 1 Cerenkov, 2 scintillation
+    uint8_t pad[15];
+};
+static_assert(sizeof(Emission) == 128, "OpticalEmission layout");
+
+// Detected-photon record written by the kernels (48 bytes, FP64 time).
+struct Hit {
+    float pos[3];
+    float dir[3];
+    float energy;
+    uint32_t sensor;
+    uint32_t event_id;
+    uint32_t photon_id;
+    double time;
+};
+static_assert(sizeof(Hit) == 48, "hit layout");
+
+// Synthetic source: photons uniform in a box, isotropic, random linear
+// polarisation, Gaussian wavelength and exponential emission time.
+struct SyntheticSource {
+    float lo[3], hi[3];
+    float wavelength_mean_nm, wavelength_sigma_nm;
+    float decay_ns;
+    uint32_t volume;
+};
+
+struct Photon {
+    V3 pos, dir, pol;
+    double time;
+    float energy;
+    uint32_t volume;
+    uint32_t step;
+    uint32_t photon_id, event_id;
+    uint32_t material;  // material whose values are cached below
+    float rindex, inv_velocity, interaction_length, absorb_fraction;
+};
+
+enum StepStatus : int {
+    kAlive = 0,
+    kDetected = 1,
+    kAbsorbedBulk = 2,
+    kAbsorbedSurface = 3,
+    kEscaped = 4,
+    kInvalidState = 5,
+};
+
+// What one step did, for the path-replay adjoint (likelihood-ratio scores).
+struct StepTrace {
+    float segment;      // path length travelled in the bulk of `material`
+    uint32_t material;  // material of that segment
+    uint32_t surface;   // surface met at the boundary (kInvalid = none or a bulk event)
+    int event;          // kTraceNone, kTraceRayleigh, kTraceInteraction, kTraceTransmit, kTraceDetect
+};
+enum TraceEvent : int {
+    kTraceNone = 0,
+    kTraceRayleigh = 1,     // bulk Rayleigh scatter (probability mu_s / mu)
+    kTraceInteraction = 2,  // surface classified as interaction (probability R)
+    kTraceTransmit = 3,     // surface classified as direct transmission (probability T)
+    kTraceDetect = 4,       // surface absorption (1 - R - T) followed by detection (efficiency)
+};
+
+// ---------------------------------------------------------------------------
+// Property tables (OptiXTransportDevice.cu:198-216): clamped piecewise-linear
+// interpolation over ascending energies.  The bracketing node is found by a
+// binary search, which selects the same node as the reference's linear scan.
+// ---------------------------------------------------------------------------
+G4GO_HD float prop_eval(const float* pool, const Prop& p, float e, float fallback) {
+    if (p.count == 0) {
+        return fallback;
+    }
+    const float* en = pool + p.offset;
+    const float* va = en + p.count;
+    if (p.constant != 0 || e <= en[0]) {
+        return va[0];
+    }
+    const uint32_t last = p.count - 1;
+    if (e >= en[last]) {
+        return va[last];
+    }
+    uint32_t lo = 1, hi = last;  // first j in [1, last] with e <= en[j]
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (e > en[mid]) {
+            lo = mid + 1;
+        } else {
+            hi = mid;
+        }
+    }
+    const uint32_t i = lo - 1;
+    const float f = (e - en[i]) / (en[i + 1] - en[i]);
+    return va[i] + f * (va[i + 1] - va[i]);
+}
+
+// Interpolation node and weight used by prop_eval (for parameter gradients):
+// value = (1 - w) * va[i] + w * va[i + 1] in exact arithmetic.
+G4GO_HD void prop_node(const float* pool, const Prop& p, float e, uint32_t& i, float& w) {
+    const float* en = pool + p.offset;
+    const uint32_t last = p.count - 1;
+    if (p.count <= 1 || p.constant != 0 || e <= en[0]) {
+        i = 0;
+        w = 0.0f;
+        return;
+    }
+    if (e >= en[last]) {
+        i = last;
+        w = 0.0f;
+        return;
+    }
+    uint32_t lo = 1, hi = last;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (e > en[mid]) {
+            lo = mid + 1;
+        } else {
+            hi = mid;
+        }
+    }
+    i = lo - 1;
+    w = (e - en[i]) / (en[i + 1] - en[i]);
+}
+
+// Inverse-CDF sampling of a normalised spectrum (OptiXTransportDevice.cu:322-344).
+G4GO_HD float sample_spectrum(const float* pool, const Prop& p, float random) {
+    if (p.count == 0) {
+        return u2f(0x7fc00000u);
+    }
+    const float* en = pool + p.offset;
+    const float* cdf = en + p.count;
+    const float u = clamp01(random);
+    if (p.count == 1 || u <= cdf[0]) {
+        return en[0];
+    }
+    const uint32_t last = p.count - 1;
+    if (u >= cdf[last]) {
+        return en[last];
+    }
+    uint32_t lo = 1, hi = last;  // first j with u <= cdf[j]
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (u > cdf[mid]) {
+            lo = mid + 1;
+        } else {
+            hi = mid;
+        }
+    }
+    const uint32_t i = lo - 1;
+    const float den = cdf[i + 1] - cdf[i];
+    if (!(den > 0.0f)) {
+        return en[i];
+    }
+    const float f = (u - cdf[i]) / den;
+    return en[i] + f * (en[i + 1] - en[i]);
+}
+
+// Caches the bulk properties of `material` at the photon energy.
+// Returns false when the material cannot transport (reference :720-727).
+G4GO_HD bool cache_material(Photon& ph, const SceneView& sc, uint32_t material) {
+    if (material >= sc.n_materials) {
+        return false;
+    }
+    const Material& m = sc.materials[material];
+    const float n = prop_eval(sc.pool, m.rindex, ph.energy, u2f(0x7fc00000u));
+    const bool n_ok = finitef(n) && n > 0.0f;
+    const float v = prop_eval(sc.pool, m.group_velocity, ph.energy,
+                              n_ok ? kSpeedOfLightMmPerNs / n : u2f(0x7fc00000u));
+    if (!(v > 0.0f) || !finitef(v)) {
+        return false;
+    }
+    const float la = prop_eval(sc.pool, m.abs_length, ph.energy, kInf);
+    const float lr = prop_eval(sc.pool, m.rayleigh_length, ph.energy, kInf);
+    const bool a_ok = finitef(la) && la > 0.0f;
+    const bool r_ok = finitef(lr) && lr > 0.0f;
+    ph.material = material;
+    ph.rindex = n;
+    ph.inv_velocity = 1.0f / v;
+    if (a_ok && r_ok) {
+        ph.interaction_length = (la * lr) / (la + lr);
+        ph.absorb_fraction = lr / (la + lr);
+    } else if (a_ok) {
+        ph.interaction_length = la;
+        ph.absorb_fraction = 1.0f;
+    } else if (r_ok) {
+        ph.interaction_length = lr;
+        ph.absorb_fraction = 0.0f;
+    } else {
+        ph.interaction_length = kInf;
+        ph.absorb_fraction = 1.0f;
+    }
+    return true;
+}
+
+// ---------------------------------------------------------------------------
+// Geometry: nearest boundary inside an axis-aligned box volume.
+// ---------------------------------------------------------------------------
+
+// Exit distance from the box along dir.  The face is chosen by comparing
+// (distance to plane) * |d_other| cross products, so only one division is
+// needed.  Axes with d == 0 never win.
+G4GO_HD void box_exit(const Volume& v, V3 p, V3 d, float& dist, int& axis) {
+    const float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+    const float nx = d.x > 0.0f ? fmaxf(v.hi[0] - p.x, 0.0f) : (d.x < 0.0f ? fmaxf(p.x - v.lo[0], 0.0f) : 1.0f);
+    const float ny = d.y > 0.0f ? fmaxf(v.hi[1] - p.y, 0.0f) : (d.y < 0.0f ? fmaxf(p.y - v.lo[1], 0.0f) : 1.0f);
+    const float nz = d.z > 0.0f ? fmaxf(v.hi[2] - p.z, 0.0f) : (d.z < 0.0f ? fmaxf(p.z - v.lo[2], 0.0f) : 1.0f);
+    int best = 0;
+    float bn = nx, ba = ax;
+    if (ny * ba < bn * ay) {
+        best = 1;
+        bn = ny;
+        ba = ay;
+    }
+    if (nz * ba < bn * az) {
+        best = 2;
+        bn = nz;
+        ba = az;
+    }
+    axis = best;
+    dist = bn / ba;
+}
+
+// Slab entry distance into box b from outside; returns kInf on a miss or when
+// the entry is closer than tmin.  inv = 1/d per axis (inf for d == 0).
+G4GO_HD float box_entry(const float* lo, const float* hi, V3 p, V3 inv, float tmin, int& axis) {
+    const float pc[3] = {p.x, p.y, p.z};
+    const float ic[3] = {inv.x, inv.y, inv.z};
+    float t_near = -kInf, t_far = kInf;
+    int a = 0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        if (finitef(ic[k])) {
+            const float t1 = (lo[k] - pc[k]) * ic[k];
+            const float t2 = (hi[k] - pc[k]) * ic[k];
+            const float tn = fminf(t1, t2);
+            const float tf = fmaxf(t1, t2);
+            if (tn > t_near) {
+                t_near = tn;
+                a = k;
+            }
+            t_far = fminf(t_far, tf);
+        } else if (pc[k] < lo[k] || pc[k] > hi[k]) {
+            return kInf;
+        }
+    }
+    if (t_near >= tmin && t_near <= t_far) {
+        axis = a;
+        return t_near;
+    }
+    return kInf;
+}
+
+// BVH node test: entry distance (may be negative when p is inside), kInf on a
+// miss or when the node lies entirely before tmin.
+G4GO_HD float node_entry(const float* lo, const float* hi, V3 p, V3 inv, float tmin) {
+    const float pc[3] = {p.x, p.y, p.z};
+    const float ic[3] = {inv.x, inv.y, inv.z};
+    float t_near = -kInf, t_far = kInf;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        if (finitef(ic[k])) {
+            const float t1 = (lo[k] - pc[k]) * ic[k];
+            const float t2 = (hi[k] - pc[k]) * ic[k];
+            t_near = fmaxf(t_near, fminf(t1, t2));
+            t_far = fminf(t_far, fmaxf(t1, t2));
+        } else if (pc[k] < lo[k] || pc[k] > hi[k]) {
+            return kInf;
+        }
+    }
+    return (t_near <= t_far && t_far >= tmin) ? t_near : kInf;
+}
+
+G4GO_HD bool box_contains(const Volume& v, V3 q) {
+    return q.x >= v.lo[0] && q.x <= v.hi[0] && q.y >= v.lo[1] && q.y <= v.hi[1] && q.z >= v.lo[2] && q.z <= v.hi[2];
+}
+
+// Border binding lookup with skin precedence (OptiXTransportDevice.cu:218-252).
+G4GO_HD uint32_t find_surface(const SceneView& sc, uint32_t from, uint32_t to) {
+    uint32_t first = 0, last = sc.n_bindings;
+    while (first < last) {
+        const uint32_t mid = first + (last - first) / 2;
+        const Binding& b = sc.bindings[mid];
+        if (b.from < from || (b.from == from && b.to < to)) {
+            first = mid + 1;
+        } else {
+            last = mid;
+        }
+    }
+    if (first < sc.n_bindings && sc.bindings[first].from == from && sc.bindings[first].to == to) {
+        return sc.bindings[first].surface;
+    }
+    if (from >= sc.n_volumes || to >= sc.n_volumes) {
+        return kInvalid;
+    }
+    const Volume& f = sc.volumes[from];
+    const Volume& t = sc.volumes[to];
+    if (t.parent == from) {
+        return t.skin != kInvalid ? t.skin : f.skin;
+    }
+    return f.skin != kInvalid ? f.skin : t.skin;
+}
+
+// Nearest daughter entry of volume `vol` (BVH walk when the volume has one).
+G4GO_HD float nearest_daughter(const SceneView& sc, const Volume& vol, V3 p, V3 d, float tmin, uint32_t& which,
+                               int& axis) {
+    const V3 inv = {1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
+    float best = kInf;
+    which = kInvalid;
+    if (vol.bvh_root == kInvalid) {
+        for (uint32_t k = 0; k < vol.daughter_count; ++k) {
+            const uint32_t j = sc.daughters[vol.daughter_begin + k];
+            int a = 0;
+            const float t = box_entry(sc.volumes[j].lo, sc.volumes[j].hi, p, inv, tmin, a);
+            if (t < best || (t == best && t < kInf && j < which)) {  // ties: lower volume index
+                best = t;
+                which = j;
+                axis = a;
+            }
+        }
+        return best;
+    }
+    uint32_t stack[32];
+    int top = 0;
+    stack[top++] = vol.bvh_root;
+    while (top > 0) {
+        const BvhNode& node = sc.bvh[stack[--top]];
+        int a = 0;
+        const float tn = node_entry(node.lo, node.hi, p, inv, tmin);
+        if (!(tn <= best)) {
+            continue;
+        }
+        if ((node.right_or_count & kBvhLeaf) != 0) {
+            const uint32_t count = node.right_or_count & ~kBvhLeaf;
+            for (uint32_t k = 0; k < count; ++k) {
+                const uint32_t j = sc.daughters[node.left + k];
+                const float t = box_entry(sc.volumes[j].lo, sc.volumes[j].hi, p, inv, tmin, a);
+                if (t < best || (t == best && t < kInf && j < which)) {
+                    best = t;
+                    which = j;
+                    axis = a;
+                }
+            }
+        } else if (top + 2 <= 32) {
+            stack[top++] = node.right_or_count;
+            stack[top++] = node.left;
+        }
+    }
+    return best;
+}
+
+// Neighbour across a face found at run time: the first sibling box that holds
+// the point just outside the face, else the parent volume.
+G4GO_HD uint32_t point_neighbor(const SceneView& sc, uint32_t current, V3 q) {
+    const Volume& cur = sc.volumes[current];
+    if (cur.parent == kInvalid) {
+        return kInvalid;
+    }
+    const Volume& par = sc.volumes[cur.parent];
+    for (uint32_t k = 0; k < par.daughter_count; ++k) {
+        const uint32_t j = sc.daughters[par.daughter_begin + k];
+        if (j != current && box_contains(sc.volumes[j], q)) {
+            return j;
+        }
+    }
+    return cur.parent;
+}
+
+// ---------------------------------------------------------------------------
+// Ground-surface sampling (OptiXTransportDevice.cu:492-595).  Each rejection
+// attempt consumes one Philox block of its own, so a rejected glisur smear is
+// redrawn (the reference re-evaluates the same draws inside its inner loop).
+// ---------------------------------------------------------------------------
+G4GO_HD V3 sample_facet_normal(const Surface& s, V3 n, V3 incoming, const Rng& rng, const Photon& ph) {
+    n = normalize(n);
+    if (s.model == kGlisur) {
+        const float polish = clamp01(s.model_value);
+        if (polish >= 1.0f) {
+            return n;
+        }
+        for (uint32_t attempt = 0; attempt < 128u; ++attempt) {
+            const U4 w = rng.block(ph.photon_id, ph.event_id, ph.step, kProcFacet, attempt);
+            const V3 smear = {2.0f * u01(w.x) - 1.0f, 2.0f * u01(w.y) - 1.0f, 2.0f * u01(w.z) - 1.0f};
+            if (dot(smear, smear) > 1.0f) {
+                continue;
+            }
+            const V3 facet = normalize(add(n, scale(smear, 1.0f - polish)));
+            if (dot(incoming, facet) < 0.0f) {
+                return facet;
+            }
+        }
+        return n;
+    }
+    if (s.model != kUnified) {
+        return n;
+    }
+    const float sigma = fmaxf(s.model_value, 0.0f);
+    if (!(sigma > 0.0f)) {
+        return n;
+    }
+    const V3 helper = fabsf(n.z) < 0.9f ? V3{0.0f, 0.0f, 1.0f} : V3{1.0f, 0.0f, 0.0f};
+    const V3 tangent = normalize(cross(n, helper));
+    const V3 bitangent = normalize(cross(n, tangent));
+    const float sigma2 = sigma * sigma;
+    for (uint32_t attempt = 0; attempt < 256u; ++attempt) {
+        const U4 w = rng.block(ph.photon_id, ph.event_id, ph.step, kProcFacet, attempt);
+        const float quarter = 0.25f * u01(w.x);  // alpha = (pi/2) u in turns
+        const float alpha = 1.57079632679489662f * u01(w.x);
+        float sa, ca;
+        g4_sincos_turn(quarter, sa, ca);
+        const float gaussian = g4_expf(-0.5f * alpha * alpha / sigma2);
+        if (u01(w.y) > gaussian * sa) {
+            continue;
+        }
+        float sp, cp;
+        g4_sincos_turn(u01(w.z), sp, cp);
+        const V3 facet = normalize(add(scale(n, ca), add(scale(tangent, sa * cp), scale(bitangent, sa * sp))));
+        if (dot(incoming, facet) < 0.0f) {
+            return facet;
+        }
+    }
+    return n;
+}
+
+G4GO_HD V3 sample_unified_reflection(const SceneView& sc, const Surface& s, V3 dir, V3 n, const Rng& rng,
+                                     const Photon& ph) {
+    const bool ground = s.finish == kGround;
+    const V3 facet = sample_facet_normal(s, n, dir, rng, ph);
+    const float spike = clamp01(prop_eval(sc.pool, s.spike, ph.energy, ground ? 0.0f : 1.0f));
+    const float lobe = clamp01(prop_eval(sc.pool, s.lobe, ph.energy, 0.0f));
+    const float back = clamp01(prop_eval(sc.pool, s.backscatter, ph.energy, 0.0f));
+    const U4 w = rng.block(ph.photon_id, ph.event_id, ph.step, kProcLobe, 0);
+    const float u = u01(w.x);
+    if (u < spike) {
+        return reflect_direction(dir, n);
+    }
+    if (u < spike + lobe) {
+        return reflect_direction(dir, facet);
+    }
+    if (u < spike + lobe + back) {
+        return scale(dir, -1.0f);
+    }
+    return sample_lambertian(n, u01(w.y), u01(w.z));
+}
+
+G4GO_HD V3 sample_surface_reflection(const SceneView& sc, const Surface& s, V3 dir, V3 n, const Rng& rng,
+                                     const Photon& ph) {
+    if (s.finish == kPolished) {
+        return reflect_direction(dir, n);
+    }
+    if (s.model == kGlisur) {
+        return reflect_direction(dir, sample_facet_normal(s, n, dir, rng, ph));
+    }
+    return sample_unified_reflection(sc, s, dir, n, rng, ph);
+}
+
+// Rayleigh scattering with the dipole distribution relative to the
+// polarisation (Geant4 11 G4OpRayleigh::PostStepDoIt: uniform direction,
+// polarisation = projection of the old one, accept with cos^2 of the angle
+// between old and new polarisations).  One Philox block per attempt:
+// w0 |cos theta|, w1 sign bits, w2 azimuth, w3 acceptance.
+G4GO_HD void rayleigh_scatter(Photon& ph, const Rng& rng) {
+    const V3 old_dir = ph.dir;
+    const V3 old_pol = ph.pol;
+    for (uint32_t attempt = 0; attempt < 64u; ++attempt) {
+        const U4 w = rng.block(ph.photon_id, ph.event_id, ph.step, kProcRayleigh, attempt);
+        float ct = u01(w.x);
+        const float st = sqrtf(fmaxf(0.0f, 1.0f - ct * ct));
+        if ((w.y & 0x80000000u) != 0u) {
+            ct = -ct;
+        }
+        float sp, cp;
+        g4_sincos_turn(u01(w.z), sp, cp);
+        const V3 new_dir = rotate_uz(V3{st * cp, st * sp, ct}, old_dir);
+        V3 new_pol = sub(old_pol, scale(new_dir, dot(new_dir, old_pol)));
+        const float l2 = dot(new_pol, new_pol);
+        if (!(l2 > 0.0f)) {
+            float sq, cq;
+            g4_sincos_turn(static_cast<float>(w.y & 0x00ffffffu) * 0x1p-24f, sq, cq);
+            new_pol = rotate_uz(V3{cq, sq, 0.0f}, new_dir);
+        } else {
+            new_pol = scale(new_pol, 1.0f / sqrtf(l2));
+            if ((w.y & 0x40000000u) != 0u) {
+                new_pol = scale(new_pol, -1.0f);
+            }
+        }
+        const float c = dot(new_pol, old_pol);
+        ph.dir = new_dir;
+        ph.pol = new_pol;
+        if (!(c * c < u01(w.w))) {
+            return;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Photon generation.
+// ---------------------------------------------------------------------------
+
+// Random linear polarisation perpendicular to (sin t cos p, sin t sin p, cos t):
+// cos(psi) e_theta + sin(psi) e_phi.
+G4GO_HD void isotropic(float cos_t, float sp, float cp, float spsi, float cpsi, V3& dir, V3& pol) {
+    const float sin_t = sqrtf(fmaxf(0.0f, 1.0f - cos_t * cos_t));
+    dir = V3{sin_t * cp, sin_t * sp, cos_t};
+    const V3 e_theta = {cos_t * cp, cos_t * sp, -sin_t};
+    const V3 e_phi = {-sp, cp, 0.0f};
+    pol = add(scale(e_theta, cpsi), scale(e_phi, spsi));
+}
+
+G4GO_HD void generate_synthetic(Photon& ph, const SyntheticSource& src, const Rng& rng, uint32_t event_id,
+                                uint32_t photon_id) {
+    ph.photon_id = photon_id;
+    ph.event_id = event_id;
+    ph.step = 0;
+    ph.volume = src.volume;
+    ph.material = kInvalid;
+    const U4 a = rng.block(photon_id, event_id, 0, kProcSynthetic, 0);
+    const U4 b = rng.block(photon_id, event_id, 0, kProcSynthetic, 1);
+    const float cos_t = 1.0f - 2.0f * u01(a.x);
+    float sp, cp, spsi, cpsi, sg, cg;
+    g4_sincos_turn(u01(a.y), sp, cp);
+    g4_sincos_turn(u01(a.z), spsi, cpsi);
+    isotropic(cos_t, sp, cp, spsi, cpsi, ph.dir, ph.pol);
+    // Box-Muller wavelength
+    const float radius = sqrtf(-2.0f * g4_logf(u01(a.w)));
+    g4_sincos_turn(u01(b.x), sg, cg);
+    const float wavelength = src.wavelength_mean_nm + src.wavelength_sigma_nm * (radius * cg);
+    ph.energy = kHcEvNm / wavelength;
+    ph.pos = V3{src.lo[0] + (src.hi[0] - src.lo[0]) * u01(b.y), src.lo[1] + (src.hi[1] - src.lo[1]) * u01(b.z),
+                src.lo[2] + (src.hi[2] - src.lo[2]) * u01(b.w)};
+    const U4 c = rng.block(photon_id, event_id, 0, kProcSynthetic, 2);
+    ph.time = static_cast<double>(-src.decay_ns * g4_logf(u01(c.x)));
+}
+
+// Expansion of one emission record into photon `local` of that record
+// (OptiXTransportDevice.cu:346-481).  Draws: block 0 {spectrum or
+// Cerenkov energy, cos theta, phi, polarisation angle}, block 1
+// {position fraction, decay}, further blocks for rejection loops.
+G4GO_HD void generate_emission(Photon& ph, const Emission& em, const SceneView& sc, const Rng& rng, uint32_t local) {
+    ph.photon_id = em.first_photon_id + local;
+    ph.event_id = em.event_id;
+    ph.step = 0;
+    ph.volume = em.volume;
+    ph.material = kInvalid;
+    ph.pos = V3{em.pos[0], em.pos[1], em.pos[2]};
+    ph.time = static_cast<double>(em.time);
+    ph.dir = normalize(V3{em.dir[0], em.dir[1], em.dir[2]});
+    ph.pol = V3{em.pol[0], em.pol[1], em.pol[2]};
+    ph.energy = em.energy;
+    if (em.type == 0) {
+        return;
+    }
+    if (em.material >= sc.n_materials) {
+        ph.energy = u2f(0x7fc00000u);
+        return;
+    }
+    const Material& mat = sc.materials[em.material];
+    const uint32_t key_lo = ph.photon_id, key_hi = ph.event_id;
+    float frac = 1.0f;
+    uint32_t blk = 2;
+    if (em.type == 1) {
+        // Cerenkov: energy by rejection on sin^2(theta), then the position along
+        // the step by rejection on the linearly varying mean photon count.
+        const float beta = (em.pre_velocity + 0.5f * em.delta_velocity) / kSpeedOfLightMmPerNs;
+        if (!(beta > 0.0f) || !(beta < 1.0f) || mat.rindex.count == 0) {
+            ph.energy = u2f(0x7fc00000u);
+            return;
+        }
+        const float* en = sc.pool + mat.rindex.offset;
+        const float e_min = en[0];
+        const float e_max = en[mat.rindex.count - 1];
+        const float n_max = mat.rindex_max;
+        const float beta_inv = 1.0f / beta;
+        const float max_cos = beta_inv / n_max;
+        const float max_sin2 = 1.0f - max_cos * max_cos;
+        const float de = e_max - e_min;
+        if (!(n_max > 0.0f) || !finitef(n_max) || !(max_cos > 0.0f) || !(max_cos < 1.0f) || !(de >= 0.0f) ||
+            !finitef(de) || !(max_sin2 > 0.0f)) {
+            ph.energy = u2f(0x7fc00000u);
+            return;
+        }
+        float energy = e_min, cos_t = 0.0f, sin2 = 0.0f;
+        bool accepted = false;
+        for (uint32_t attempt = 0; attempt < 1024u; ++attempt) {
+            const U4 w = rng.block(key_lo, key_hi, 0, kProcEmission, 16u + attempt);
+            energy = e_min + u01(w.x) * de;
+            const float n = prop_eval(sc.pool, mat.rindex, energy, u2f(0x7fc00000u));
+            cos_t = beta_inv / n;
+            sin2 = 1.0f - cos_t * cos_t;
+            if (finitef(n) && cos_t > 0.0f && cos_t < 1.0f && sin2 > 0.0f && u01(w.y) * max_sin2 <= sin2) {
+                accepted = true;
+                break;
+            }
+        }
+        if (!accepted) {
+            ph.energy = u2f(0x7fc00000u);
+            return;
+        }
+        const U4 w0 = rng.block(key_lo, key_hi, 0, kProcEmission, 0);
+        float sp, cp;
+        g4_sincos_turn(u01(w0.z), sp, cp);
+        const float sin_t = sqrtf(fmaxf(sin2, 0.0f));
+        const V3 axis = V3{em.dir[0], em.dir[1], em.dir[2]};
+        ph.dir = rotate_uz(V3{sin_t * cp, sin_t * sp, cos_t}, normalize(axis));
+        ph.pol = rotate_uz(V3{cos_t * cp, cos_t * sp, -sin_t}, normalize(axis));
+        ph.dir = normalize(ph.dir);
+        ph.pol = normalize(ph.pol);
+        const float max_mean = fmaxf(em.pre_mean, em.post_mean);
+        const float d_mean = em.pre_mean - em.post_mean;
+        for (uint32_t attempt = 0; attempt < 1024u; ++attempt) {
+            const U4 w = rng.block(key_lo, key_hi, 0, kProcEmission, 2048u + attempt);
+            frac = u01(w.x);
+            if (!(u01(w.y) * max_mean > em.pre_mean - frac * d_mean)) {
+                break;
+            }
+        }
+        ph.energy = energy;
+    } else if (em.type == 2) {
+        if (em.spectrum >= 3u) {
+            ph.energy = u2f(0x7fc00000u);
+            return;
+        }
+        const U4 w0 = rng.block(key_lo, key_hi, 0, kProcEmission, 0);
+        const U4 w1 = rng.block(key_lo, key_hi, 0, kProcEmission, 1);
+        ph.energy = sample_spectrum(sc.pool, mat.spectrum[em.spectrum], u01(w0.x));
+        const float cos_t = 1.0f - 2.0f * u01(w0.y);
+        float sp, cp, spsi, cpsi;
+        g4_sincos_turn(u01(w0.z), sp, cp);
+        g4_sincos_turn(u01(w0.w), spsi, cpsi);
+        isotropic(cos_t, sp, cp, spsi, cpsi, ph.dir, ph.pol);
+        frac = em.charge == 0.0f ? 1.0f : u01(w1.x);
+        const float decay = em.decay;
+        if (decay > 0.0f && finitef(decay)) {
+            if (!(em.rise > 0.0f) || !finitef(em.rise)) {
+                ph.time += static_cast<double>(-decay * g4_logf(fmaxf(u01(w1.y), 1.0e-7f)));
+            } else {
+                float sample = 0.0f;
+                for (uint32_t attempt = 0; attempt < 1024u; ++attempt) {
+                    const U4 w = rng.block(key_lo, key_hi, 0, kProcEmission, blk + attempt);
+                    sample = -decay * g4_logf(fmaxf(1.0e-7f, 1.0f - u01(w.x)));
+                    if (!(u01(w.y) > 1.0f - g4_expf(-sample / em.rise))) {
+                        break;
+                    }
+                }
+                ph.time += static_cast<double>(sample);
+            }
+        }
+    } else {
+        ph.energy = u2f(0x7fc00000u);
+        return;
+    }
+    ph.pos = add(V3{em.pos[0], em.pos[1], em.pos[2]}, scale(V3{em.step_delta[0], em.step_delta[1], em.step_delta[2]}, frac));
+    const float velocity = em.pre_velocity + 0.5f * frac * em.delta_velocity;
+    if (velocity > 0.0f && finitef(velocity)) {
+        ph.time += static_cast<double>(frac * em.step_length / velocity);
+    }
+}
+
+// Validity checks of a generated photon (OptiXTransportDevice.cu:693-702).
+G4GO_HD bool start_photon(Photon& ph) {
+    const float l2 = dot(ph.dir, ph.dir);
+    if (!(ph.energy > 0.0f) || !finitef(ph.energy) || !(l2 > 0.0f) || !finitef(l2)) {
+        return false;
+    }
+    ph.dir = normalize(ph.dir);
+    ph.pol = project_polarization(ph.pol, ph.dir);
+    return true;
+}
+
+// ---------------------------------------------------------------------------
+// One transport step (one pass of the bounce loop, OptiXTransportDevice.cu:708-931).
+// ---------------------------------------------------------------------------
+struct StepConfig {
+    float eps;  // boundary tolerance (mm)
+};
+
+template <bool kTrace>
+G4GO_HD int transport_step(Photon& ph, const SceneView& sc, const Rng& rng, const StepConfig& cfg, bool& zero_step,
+                           uint32_t& sensor_out, StepTrace* trace) {
+    zero_step = false;
+    if (ph.volume >= sc.n_volumes) {
+        return kInvalidState;
+    }
+    const Volume& vol = sc.volumes[ph.volume];
+    if (vol.material != ph.material && !cache_material(ph, sc, vol.material)) {
+        return kInvalidState;
+    }
+    if (kTrace) {
+        trace->segment = 0.0f;
+        trace->material = ph.material;
+        trace->surface = kInvalid;
+        trace->event = 0;
+    }
+
+    // nearest boundary: leave the box, or enter a daughter
+    float dist;
+    int axis;
+    box_exit(vol, ph.pos, ph.dir, dist, axis);
+    uint32_t next = kInvalid;
+    uint32_t surface = kInvalid;
+    bool entering = false;
+    if (vol.daughter_count != 0) {
+        uint32_t which;
+        int a2 = 0;
+        const float t = nearest_daughter(sc, vol, ph.pos, ph.dir, cfg.eps, which, a2);
+        if (which != kInvalid && t <= dist) {
+            dist = t;
+            axis = a2;
+            next = which;
+            surface = sc.volumes[which].entry_surface;
+            entering = true;
+        }
+    }
+    if (!finitef(dist)) {
+        return kEscaped;
+    }
+    if (dist <= cfg.eps) {
+        zero_step = true;
+    }
+
+    const U4 w = rng.block(ph.photon_id, ph.event_id, ph.step, kProcStep, 0);
+
+    // bulk interaction: absorption or Rayleigh (competing exponentials)
+    if (finitef(ph.interaction_length)) {
+        const float d_int = -g4_logf(u01(w.x)) * ph.interaction_length;
+        if (d_int < dist) {
+            ph.time += static_cast<double>(d_int * ph.inv_velocity);
+            if (kTrace) {
+                trace->segment = d_int;
+            }
+            if (u01(w.y) <= ph.absorb_fraction) {
+                return kAbsorbedBulk;
+            }
+            ph.pos = add(ph.pos, scale(ph.dir, d_int));
+            rayleigh_scatter(ph, rng);
+            if (kTrace) {
+                trace->event = kTraceRayleigh;
+            }
+            return kAlive;
+        }
+    }
+    ph.time += static_cast<double>(dist * ph.inv_velocity);
+    if (kTrace) {
+        trace->segment = dist;
+    }
+    V3 bpos = add(ph.pos, scale(ph.dir, dist));
+    const float dsign = axis_of(ph.dir, axis) > 0.0f ? 1.0f : -1.0f;
+    // snap onto the face plane
+    {
+        const Volume& face_box = entering ? sc.volumes[next] : vol;
+        const float plane = entering ? (dsign > 0.0f ? face_box.lo[axis] : face_box.hi[axis])
+                                     : (dsign > 0.0f ? face_box.hi[axis] : face_box.lo[axis]);
+        if (axis == 0) {
+            bpos.x = plane;
+        } else if (axis == 1) {
+            bpos.y = plane;
+        } else {
+            bpos.z = plane;
+        }
+    }
+    ph.pos = bpos;
+    // normal facing the incoming photon (reference :789-791)
+    const V3 normal = axis == 0 ? V3{-dsign, 0.0f, 0.0f} : (axis == 1 ? V3{0.0f, -dsign, 0.0f} : V3{0.0f, 0.0f, -dsign});
+
+    if (!entering) {
+        const int face = 2 * axis + (dsign > 0.0f ? 1 : 0);
+        next = vol.face_neighbor[face];
+        if (next == kFaceDynamic) {
+            const V3 q = add(bpos, scale(normal, -cfg.eps));
+            next = point_neighbor(sc, ph.volume, q);
+            surface = next == kInvalid ? kInvalid : find_surface(sc, ph.volume, next);
+        } else {
+            surface = vol.face_surface[face];
+        }
+        if (next == kInvalid) {
+            return ph.volume == sc.world ? kEscaped : kInvalidState;
+        }
+    }
+    const Volume& nvol = sc.volumes[next];
+    if (nvol.material >= sc.n_materials) {
+        return kInvalidState;
+    }
+    const Material& nmat = sc.materials[nvol.material];
+    const float n1 = ph.rindex;
+    const float e = ph.energy;
+
+    bool reflected = false, detected = false, absorbed = false, transmitted = false;
+    V3 dir = ph.dir, pol = ph.pol;
+    if (surface < sc.n_surfaces) {
+        const Surface& s = sc.surfaces[surface];
+        if (kTrace) {
+            trace->surface = surface;
+        }
+        const float refl = clamp01(prop_eval(sc.pool, s.reflectivity, e, 0.0f));
+        const float eff = clamp01(prop_eval(sc.pool, s.efficiency, e, 0.0f));
+        const float trans = clamp01(prop_eval(sc.pool, s.transmittance, e, 0.0f));
+        const int outcome = classify_surface(refl, trans, s.reflectivity.count != 0, s.transmittance.count != 0, u01(w.z));
+        if (kTrace) {
+            trace->event = outcome == kInteraction ? kTraceInteraction
+                                                   : (outcome == kDirectTransmit ? kTraceTransmit : kTraceNone);
+        }
+        if (outcome == kAbsorb) {
+            absorbed = true;
+            detected = evaluate_absorption(eff, nvol.sensor != kInvalid || vol.sensor != kInvalid, u01(w.w)) == kDetect;
+        } else if (outcome == kDirectTransmit) {
+            transmitted = true;
+        } else if (s.type == kDielectricDielectric) {
+            const float n2 = prop_eval(sc.pool, nmat.rindex, e, u2f(0x7fc00000u));
+            if (!(n1 > 0.0f) || !finitef(n1) || !(n2 > 0.0f) || !finitef(n2)) {
+                absorbed = true;
+            } else {
+                const V3 inormal = s.finish == kPolished ? normalize(normal) : sample_facet_normal(s, normal, dir, rng, ph);
+                const float rough = prop_eval(sc.pool, s.roughness, e, 0.0f);
+                bool rough_pass = true;
+                U4 wr = {0u, 0u, 0u, 0u};
+                if (rough > 0.0f && n1 > n2 && finitef(rough) && finitef(e) && e > 0.0f) {
+                    const float cost1 = fmaxf(0.0f, -dot(dir, inormal));
+                    const float wavelength_mm = 1.239841984e-3f / e;
+                    const float expo = 4.0f * 3.14159265358979323846f * rough * n1 * cost1 / wavelength_mm;
+                    wr = rng.block(ph.photon_id, ph.event_id, ph.step, kProcRough, 0);
+                    rough_pass = u01(wr.x) < g4_expf(-expo * expo);
+                }
+                const Fresnel f = compute_fresnel(dir, pol, inormal, n1, n2);
+                reflected = f.tir || u01(w.w) > f.transmittance;
+                if (reflected) {
+                    if (rough_pass) {
+                        if (s.model == kUnified && s.finish != kPolished) {
+                            dir = sample_unified_reflection(sc, s, dir, normal, rng, ph);
+                            pol = reflect_polarization(pol, normal, dir);
+                        } else {
+                            dir = f.reflected_dir;
+                            pol = f.reflected_pol;
+                        }
+                    } else {
+                        dir = sample_lambertian(normal, u01(wr.y), u01(wr.z));
+                        pol = project_polarization(pol, dir);
+                    }
+                } else {
+                    dir = f.transmitted_dir;
+                    pol = f.transmitted_pol;
+                    transmitted = true;
+                }
+            }
+        } else {
+            reflected = true;
+            dir = sample_surface_reflection(sc, s, dir, normal, rng, ph);
+            pol = reflect_polarization(pol, normal, dir);
+        }
+    } else {
+        const float n2 = prop_eval(sc.pool, nmat.rindex, e, u2f(0x7fc00000u));
+        if (!(n1 > 0.0f) || !finitef(n1) || !(n2 > 0.0f) || !finitef(n2)) {
+            absorbed = true;
+        } else {
+            const Fresnel f = compute_fresnel(dir, pol, normal, n1, n2);
+            reflected = u01(w.w) > f.transmittance;
+            if (reflected) {
+                dir = f.reflected_dir;
+                pol = f.reflected_pol;
+            } else {
+                dir = f.transmitted_dir;
+                pol = f.transmitted_pol;
+                transmitted = true;
+            }
+        }
+    }
+
+    if (detected) {
+        sensor_out = nvol.sensor != kInvalid ? nvol.sensor : vol.sensor;
+        if (sensor_out == kInvalid) {
+            return kInvalidState;
+        }
+        if (kTrace) {
+            trace->event = kTraceDetect;
+        }
+        return kDetected;
+    }
+    if (absorbed) {
+        return kAbsorbedSurface;
+    }
+    ph.dir = dir;
+    ph.pol = pol;
+    if (transmitted) {
+        ph.volume = next;
+    }
+    return kAlive;
+}
+
+}  // namespace g4go

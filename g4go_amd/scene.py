@@ -1,0 +1,206 @@
+"""Scene model mirroring G4GO's backend-neutral optical scene.
+
+Reference: include/g4go/optical/Scene.hpp:61-164 and src/optical/Scene.cpp.
+Volumes carry an axis-aligned box solid (centre, half lengths in mm) instead
+of a triangle mesh, and materials may carry a Rayleigh scattering length.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+from enum import IntEnum
+from typing import Optional, Sequence
+
+import numpy as np
+
+from . import abi
+
+INVALID_ID = abi.INVALID_ID
+
+
+class SurfaceType(IntEnum):
+    DielectricDielectric = 0
+    DielectricMetal = 1
+
+
+class SurfaceModel(IntEnum):
+    Glisur = 0
+    Unified = 1
+
+
+class SurfaceFinish(IntEnum):
+    Polished = 0
+    Ground = 1
+
+
+@dataclass
+class PropertyTable:
+    """(energy eV, value) pairs with ascending energies (Scene.hpp:61-67)."""
+
+    energy_ev: np.ndarray = field(default_factory=lambda: np.zeros(0, np.float32))
+    values: np.ndarray = field(default_factory=lambda: np.zeros(0, np.float32))
+
+    def __post_init__(self) -> None:
+        self.energy_ev = np.ascontiguousarray(self.energy_ev, dtype=np.float32)
+        self.values = np.ascontiguousarray(self.values, dtype=np.float32)
+        if self.energy_ev.shape != self.values.shape or self.energy_ev.ndim != 1:
+            raise ValueError("property energies and values must be 1-D arrays of equal length")
+
+    @staticmethod
+    def constant_value(value: float, energy_ev: float = 2.9) -> "PropertyTable":
+        return PropertyTable(np.array([energy_ev], np.float32), np.array([value], np.float32))
+
+    def empty(self) -> bool:
+        return self.energy_ev.size == 0
+
+    def constant(self) -> bool:
+        """PropertyTable::Constant (Scene.cpp:17-21)."""
+        return self.values.size == 1 or (self.values.size > 1 and bool(np.all(self.values[1:] == self.values[0])))
+
+
+@dataclass
+class Material:
+    name: str = ""
+    rindex: PropertyTable = field(default_factory=PropertyTable)
+    group_velocity_mm_per_ns: PropertyTable = field(default_factory=PropertyTable)
+    abs_length_mm: PropertyTable = field(default_factory=PropertyTable)
+    rayleigh_length_mm: PropertyTable = field(default_factory=PropertyTable)
+    scintillation_spectrum: Sequence[PropertyTable] = field(
+        default_factory=lambda: [PropertyTable(), PropertyTable(), PropertyTable()])
+
+
+@dataclass
+class Surface:
+    name: str = ""
+    type: SurfaceType = SurfaceType.DielectricDielectric
+    model: SurfaceModel = SurfaceModel.Unified
+    finish: SurfaceFinish = SurfaceFinish.Polished
+    model_value: float = 1.0
+    reflectivity: PropertyTable = field(default_factory=PropertyTable)
+    efficiency: PropertyTable = field(default_factory=PropertyTable)
+    transmittance: PropertyTable = field(default_factory=PropertyTable)
+    specular_lobe: PropertyTable = field(default_factory=PropertyTable)
+    specular_spike: PropertyTable = field(default_factory=PropertyTable)
+    backscatter: PropertyTable = field(default_factory=PropertyTable)
+    surface_roughness: PropertyTable = field(default_factory=PropertyTable)
+
+
+@dataclass
+class Volume:
+    name: str = ""
+    volume_id: int = INVALID_ID
+    material_id: int = INVALID_ID
+    parent_volume_id: int = INVALID_ID
+    skin_surface_id: int = INVALID_ID
+    sensor_id: int = INVALID_ID
+    center_mm: Sequence[float] = (0.0, 0.0, 0.0)
+    half_mm: Sequence[float] = (1.0, 1.0, 1.0)
+
+
+@dataclass
+class SurfaceBinding:
+    from_volume_id: int = INVALID_ID
+    to_volume_id: int = INVALID_ID
+    surface_id: int = INVALID_ID
+
+
+class Scene:
+    """Scene (Scene.hpp:128-164) with the same add/find API."""
+
+    def __init__(self) -> None:
+        self.materials: list[Material] = []
+        self.surfaces: list[Surface] = []
+        self.volumes: list[Volume] = []
+        self.surface_bindings: list[SurfaceBinding] = []
+        self.world_volume_id: int = INVALID_ID
+
+    def add_material(self, material: Material) -> int:
+        self.materials.append(material)
+        return len(self.materials) - 1
+
+    def add_surface(self, surface: Surface) -> int:
+        self.surfaces.append(surface)
+        return len(self.surfaces) - 1
+
+    def add_volume(self, volume: Volume) -> int:
+        """Scene::AddVolume: an unset ID becomes the volume's index (Scene.cpp:39-45)."""
+        if volume.volume_id == INVALID_ID:
+            volume.volume_id = len(self.volumes)
+        self.volumes.append(volume)
+        return len(self.volumes) - 1
+
+    def add_surface_binding(self, binding: SurfaceBinding) -> None:
+        self.surface_bindings.append(binding)
+
+    def find_surface(self, surface_id: int) -> Optional[Surface]:
+        return self.surfaces[surface_id] if 0 <= surface_id < len(self.surfaces) else None
+
+    def find_volume(self, volume_id: int) -> Optional[Volume]:
+        if 0 <= volume_id < len(self.volumes) and self.volumes[volume_id].volume_id == volume_id:
+            return self.volumes[volume_id]
+        return next((v for v in self.volumes if v.volume_id == volume_id), None)
+
+    def find_boundary_surface(self, from_volume_id: int, to_volume_id: int) -> Optional[Surface]:
+        """Scene::FindBoundarySurface (Scene.cpp:120-150): border binding, else skin precedence."""
+        for b in self.surface_bindings:
+            if b.from_volume_id == from_volume_id and b.to_volume_id == to_volume_id:
+                return self.find_surface(b.surface_id)
+        src = self.find_volume(from_volume_id)
+        dst = self.find_volume(to_volume_id)
+        if src is None or dst is None:
+            return None
+
+        def skin(v: Volume) -> Optional[Surface]:
+            return None if v.skin_surface_id == INVALID_ID else self.find_surface(v.skin_surface_id)
+
+        if dst.parent_volume_id == from_volume_id:
+            return skin(dst) or skin(src)
+        return skin(src) or skin(dst)
+
+    # ---- C ABI view ----
+    def to_abi(self) -> tuple[abi.Scene, list]:
+        """Returns the g4go_scene description and the objects that keep its arrays alive."""
+        keep: list = []
+
+        def prop(p: PropertyTable) -> abi.Property:
+            if p.empty():
+                return abi.Property(0, None, None)
+            keep.extend([p.energy_ev, p.values])
+            return abi.Property(p.energy_ev.size, p.energy_ev.ctypes.data_as(C.POINTER(C.c_float)),
+                                p.values.ctypes.data_as(C.POINTER(C.c_float)))
+
+        mats = (abi.Material * max(1, len(self.materials)))()
+        for i, m in enumerate(self.materials):
+            mats[i].rindex = prop(m.rindex)
+            mats[i].group_velocity_mm_per_ns = prop(m.group_velocity_mm_per_ns)
+            mats[i].abs_length_mm = prop(m.abs_length_mm)
+            mats[i].rayleigh_length_mm = prop(m.rayleigh_length_mm)
+            for k in range(3):
+                mats[i].scintillation_spectrum[k] = prop(m.scintillation_spectrum[k])
+        surfs = (abi.Surface * max(1, len(self.surfaces)))()
+        for i, s in enumerate(self.surfaces):
+            surfs[i].type = int(s.type)
+            surfs[i].model = int(s.model)
+            surfs[i].finish = int(s.finish)
+            surfs[i].model_value = float(s.model_value)
+            for name in ("reflectivity", "efficiency", "transmittance", "specular_lobe", "specular_spike",
+                         "backscatter", "surface_roughness"):
+                setattr(surfs[i], name, prop(getattr(s, name)))
+        vols = (abi.Volume * max(1, len(self.volumes)))()
+        for i, v in enumerate(self.volumes):
+            vols[i].volume_id = v.volume_id
+            vols[i].material_id = v.material_id
+            vols[i].parent_volume_id = v.parent_volume_id
+            vols[i].skin_surface_id = v.skin_surface_id
+            vols[i].sensor_id = v.sensor_id
+            vols[i].center_mm[:] = [float(x) for x in v.center_mm]
+            vols[i].half_mm[:] = [float(x) for x in v.half_mm]
+        binds = (abi.SurfaceBinding * max(1, len(self.surface_bindings)))()
+        for i, b in enumerate(self.surface_bindings):
+            binds[i].from_volume_id = b.from_volume_id
+            binds[i].to_volume_id = b.to_volume_id
+            binds[i].surface_id = b.surface_id
+        keep.extend([mats, surfs, vols, binds])
+        desc = abi.Scene(mats, len(self.materials), surfs, len(self.surfaces), vols, len(self.volumes), binds,
+                         len(self.surface_bindings), self.world_volume_id)
+        return desc, keep

@@ -1,0 +1,192 @@
+"""HipTransportHost: the reference backend interface over the C ABI.
+
+Mirrors G4GO::Optical::OptiXTransportHost
+(include/g4go/optical/optix/OptiXTransportHost.hpp:10-26): a constructor
+taking the transport configuration, PrepareScene, EnqueueEmissions and
+CompleteOldestBatch, with one in-flight batch.  C-ABI status codes are raised
+as the reference's exception kinds: runtime_error -> RuntimeError,
+invalid_argument -> ValueError, overflow_error -> OverflowError,
+logic_error -> LogicError.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+from typing import Optional
+
+import numpy as np
+
+from . import abi
+from .scene import Scene
+
+
+class LogicError(RuntimeError):
+    """std::logic_error of the reference (queue full or empty, no scene)."""
+
+
+_ERRORS = {
+    abi.G4GO_ERROR_RUNTIME: RuntimeError,
+    abi.G4GO_ERROR_INVALID_ARGUMENT: ValueError,
+    abi.G4GO_ERROR_OVERFLOW: OverflowError,
+    abi.G4GO_ERROR_LOGIC: LogicError,
+}
+
+
+@dataclass
+class PhotonTransportConfig:
+    """PhotonTransportConfig (include/g4go/optical/PhotonTransportConfig.hpp:26-40)."""
+
+    seed: int = 0
+    max_photons_per_event: int = 5_000_000
+    max_bounces_per_photon: int = 4096
+    boundary_tolerance_mm: float = 1.0e-4
+    target_photons_per_batch: int = 1_000_000
+    max_events_per_batch: int = 10_000
+    enable_performance_diagnostics: bool = False
+    device: int = -1
+    sort_hits: bool = True
+
+    def to_abi(self) -> abi.Config:
+        return abi.Config(self.seed, self.max_photons_per_event, self.max_bounces_per_photon,
+                          self.boundary_tolerance_mm, self.target_photons_per_batch, self.max_events_per_batch,
+                          int(self.enable_performance_diagnostics), self.device, int(self.sort_hits))
+
+
+@dataclass
+class LaunchForm:
+    persistent: bool = False
+    block_size: int = 256
+    chunk: int = 64
+    refill_threshold: int = 64
+    blocks_per_cu: int = 8
+
+    def to_abi(self) -> abi.Launch:
+        return abi.Launch(int(self.persistent), self.block_size, self.chunk, self.refill_threshold,
+                          self.blocks_per_cu)
+
+
+@dataclass
+class PhotonTransportOutput:
+    """PhotonTransportOutput (PhotonTransport.hpp:152-157) plus the arrival-time histogram."""
+
+    detections: np.ndarray                 # abi.HIT_DTYPE records
+    statistics: dict                       # batch totals
+    event_statistics: np.ndarray           # abi.EVENT_DTYPE records
+    histogram: np.ndarray                  # 258 u64: 256 bins over [0, 200) ns, underflow, overflow
+    performance: dict = field(default_factory=dict)
+
+
+class HipTransportHost:
+    def __init__(self, configuration: Optional[PhotonTransportConfig] = None) -> None:
+        self._lib = abi.load()
+        self.configuration = configuration or PhotonTransportConfig()
+        self._cfg = self.configuration.to_abi()
+        handle = C.c_void_p()
+        status = self._lib.g4go_create(C.byref(self._cfg), C.byref(handle))
+        if status != abi.G4GO_OK:
+            raise _ERRORS.get(status, RuntimeError)(self._lib.g4go_last_error(None).decode())
+        self._h = handle
+        self._scene: Optional[Scene] = None
+        self._scene_keep: list = []
+
+    # ---- lifecycle ----
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.g4go_destroy(self._h)
+            self._h = None
+
+    def __del__(self) -> None:
+        self.close()
+
+    def __enter__(self) -> "HipTransportHost":
+        return self
+
+    def __exit__(self, *exc) -> None:
+        self.close()
+
+    def _check(self, status: int) -> None:
+        if status != abi.G4GO_OK:
+            message = self._lib.g4go_last_error(self._h).decode()
+            raise _ERRORS.get(status, RuntimeError)(message)
+
+    def set_launch(self, launch: LaunchForm) -> None:
+        form = launch.to_abi()
+        self._check(self._lib.g4go_set_launch(self._h, C.byref(form)))
+
+    def launch(self) -> LaunchForm:
+        form = abi.Launch()
+        self._check(self._lib.g4go_get_launch(self._h, C.byref(form)))
+        return LaunchForm(bool(form.persistent), form.block_size, form.chunk, form.refill_threshold,
+                          form.blocks_per_cu)
+
+    # ---- the reference interface ----
+    def prepare_scene(self, scene: Scene) -> None:
+        """PrepareScene: rebuilds when a different Scene object is given (OptiXTransportHost.cpp:511-519)."""
+        if scene is self._scene:
+            return
+        desc, keep = scene.to_abi()
+        self._check(self._lib.g4go_prepare_scene(self._h, C.byref(desc)))
+        self._scene = scene
+        self._scene_keep = keep
+
+    def enqueue_emissions(self, scene: Scene, emissions: np.ndarray, event_ids: np.ndarray,
+                          emission_event_indices: np.ndarray) -> None:
+        """EnqueueEmissions(scene, {emissions, eventIDs, emissionEventIndices})."""
+        self.prepare_scene(scene)
+        em = np.ascontiguousarray(emissions, dtype=abi.EMISSION_DTYPE)
+        ev = np.ascontiguousarray(event_ids, dtype=np.uint32)
+        idx = np.ascontiguousarray(emission_event_indices, dtype=np.uint32)
+        self._check(self._lib.g4go_enqueue_emissions(self._h, em.ctypes.data if em.size else None, em.size,
+                                                     ev.ctypes.data if ev.size else None, ev.size,
+                                                     idx.ctypes.data if idx.size else None))
+
+    def enqueue_synthetic(self, scene: Scene, source: abi.SyntheticSource, event_ids: np.ndarray,
+                          photon_counts: np.ndarray) -> None:
+        self.prepare_scene(scene)
+        ev = np.ascontiguousarray(event_ids, dtype=np.uint32)
+        counts = np.ascontiguousarray(photon_counts, dtype=np.uint32)
+        if ev.shape != counts.shape:
+            raise ValueError("event_ids and photon_counts must have the same length")
+        self._check(self._lib.g4go_enqueue_synthetic(self._h, C.byref(source), ev.ctypes.data if ev.size else None,
+                                                     counts.ctypes.data if counts.size else None, ev.size))
+
+    def complete_oldest_batch(self) -> PhotonTransportOutput:
+        res = abi.BatchResult()
+        self._check(self._lib.g4go_complete(self._h, C.byref(res)))
+        return self._output(res, copy_hits=True)
+
+    def complete_device(self) -> tuple[PhotonTransportOutput, int]:
+        """Completes the batch leaving detections in HBM; returns (output, device pointer of the hits)."""
+        res = abi.BatchResult()
+        ptr = C.c_void_p()
+        self._check(self._lib.g4go_complete_device(self._h, C.byref(res), C.byref(ptr)))
+        return self._output(res, copy_hits=False), int(ptr.value or 0)
+
+    def _output(self, res: abi.BatchResult, copy_hits: bool) -> PhotonTransportOutput:
+        n_hits = int(res.n_hits)
+        hits = np.empty(n_hits if copy_hits else 0, dtype=abi.HIT_DTYPE)
+        if copy_hits and n_hits:
+            C.memmove(hits.ctypes.data, res.hits, n_hits * abi.HIT_DTYPE.itemsize)
+        events = np.empty(int(res.n_events), dtype=abi.EVENT_DTYPE)
+        if res.n_events:
+            C.memmove(events.ctypes.data, res.events, int(res.n_events) * abi.EVENT_DTYPE.itemsize)
+        hist = np.ctypeslib.as_array(res.histogram, shape=(abi.HIST_SLOTS,)).copy()
+        st = res.stats
+        stats = {name: getattr(st, name) for name, _ in abi.BatchStats._fields_}
+        stats["n_hits"] = n_hits
+        perf = {k: stats[k] for k in ("kernel_ms", "host_to_device_ms", "device_to_host_ms", "host_to_device_bytes",
+                                      "device_to_host_bytes", "total_steps")}
+        return PhotonTransportOutput(hits, stats, events, hist, perf)
+
+    # ---- device evaluation of the shared physics (tests) ----
+    def eval_math(self, which: int, x: np.ndarray) -> np.ndarray:
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        out = np.empty(2 * x.size if which == 2 else x.size, dtype=np.float32)
+        self._check(self._lib.g4go_eval_math(self._h, which, x.ctypes.data, out.ctypes.data, x.size))
+        return out.reshape(-1, 2) if which == 2 else out
+
+    def eval_fresnel(self, cases: np.ndarray) -> np.ndarray:
+        cases = np.ascontiguousarray(cases, dtype=np.float32).reshape(-1, 11)
+        out = np.empty((cases.shape[0], 14), dtype=np.float32)
+        self._check(self._lib.g4go_eval_fresnel(self._h, cases.ctypes.data, out.ctypes.data, cases.shape[0]))
+        return out

@@ -1,0 +1,245 @@
+"""Scenes and seeded inputs of the five benchmark workloads.
+
+* single crystal 3x3x20 mm (n = 1.82, absorption 400 mm in 32 wavelength
+  nodes, Rayleigh 500 mm) in an air world, wrapped on five faces by a
+  polished dielectric_metal reflector (R = 0.95, the reference's
+  "Reflector" border surface, DetectorConstruction.cpp:149-154) and read out
+  by a 3x3 mm SiPM on the +z face (polished dielectric_metal "Cathode" skin
+  with R = 0 and a 32-node PDE curve resampled from the reference's sensor
+  efficiency table, DetectorConstruction.cpp:62-68, 163-169);
+* an 8x8 array of the same crystals at 3.2 mm pitch with reflective septa
+  (one wrap border surface per crystal) and one SiPM per crystal;
+* synthetic photons uniform in the crystal (isotropic, random linear
+  polarisation, wavelength N(420, 20) nm, emission time Exp(40 ns));
+* emission records of 1 MeV electrons on the crystal axis, synthesised by a
+  seeded generator (Geant4 is not available to produce them);
+* log-normal event sizes for the batched workloads.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import abi
+from .scene import (INVALID_ID, Material, PropertyTable, Scene, Surface, SurfaceBinding, SurfaceFinish,
+                    SurfaceModel, SurfaceType, Volume)
+
+HC_EV_NM = 1239.84198
+N_NODES = 32
+WAVELENGTH_LO_NM = 340.0
+WAVELENGTH_HI_NM = 500.0
+
+# Reference SiPM efficiency curve (DetectorConstruction.cpp:62-68), energy in eV.
+REFERENCE_SENSOR_ENERGY_EV = np.array([1.771068, 2.101763, 2.478823, 2.66882, 2.756137, 2.849362, 2.949113, 3.02425,
+                                       3.099632, 3.178868, 3.3981, 3.649811, 3.877416, 4.128759, 4.348776])
+REFERENCE_SENSOR_EFFICIENCY = np.array([0.26319, 0.395706, 0.503681, 0.525767, 0.555215, 0.570552, 0.597546,
+                                        0.59816, 0.599387, 0.593252, 0.521472, 0.466258, 0.458896, 0.408589,
+                                        0.196933])
+
+
+def node_energies() -> np.ndarray:
+    """Energies (ascending) of the 32 wavelength-bin centres over 340-500 nm (5 nm bins)."""
+    width = (WAVELENGTH_HI_NM - WAVELENGTH_LO_NM) / N_NODES
+    wl = WAVELENGTH_LO_NM + width * (np.arange(N_NODES) + 0.5)
+    return np.sort((HC_EV_NM / wl).astype(np.float32))
+
+
+def reference_pde_nodes() -> np.ndarray:
+    """Reference SiPM efficiency, linearly interpolated at the node energies."""
+    return np.interp(node_energies().astype(np.float64), REFERENCE_SENSOR_ENERGY_EV,
+                     REFERENCE_SENSOR_EFFICIENCY).astype(np.float32)
+
+
+def gaussian_wavelength_spectrum(mean_nm: float = 420.0, sigma_nm: float = 20.0, points: int = 64) -> PropertyTable:
+    """Emission spectrum N(mean, sigma) in wavelength, as a density over energy."""
+    wl = np.linspace(mean_nm - 4 * sigma_nm, mean_nm + 4 * sigma_nm, points)
+    e = HC_EV_NM / wl
+    density = np.exp(-0.5 * ((wl - mean_nm) / sigma_nm) ** 2) * wl * wl / HC_EV_NM
+    order = np.argsort(e)
+    return PropertyTable(e[order].astype(np.float32), density[order].astype(np.float32))
+
+
+@dataclass
+class CrystalParams:
+    rindex: float = 1.82
+    abs_length_mm: float = 400.0
+    rayleigh_length_mm: float = 500.0
+    wrap_reflectivity: float = 0.95
+    half_mm: tuple = (1.5, 1.5, 10.0)
+    sipm_thickness_mm: float = 0.1
+
+
+def _crystal_material(p: CrystalParams, abs_nodes=None, name="crystal") -> Material:
+    nodes = node_energies()
+    absl = np.full(N_NODES, p.abs_length_mm, np.float32) if abs_nodes is None else np.asarray(abs_nodes, np.float32)
+    return Material(
+        name=name,
+        rindex=PropertyTable.constant_value(p.rindex),
+        abs_length_mm=PropertyTable(nodes, absl),
+        rayleigh_length_mm=PropertyTable.constant_value(p.rayleigh_length_mm),
+        scintillation_spectrum=[gaussian_wavelength_spectrum(), PropertyTable(), PropertyTable()],
+    )
+
+
+def _common_materials_and_surfaces(scene: Scene, p: CrystalParams, pde_nodes=None):
+    air = scene.add_material(Material(name="air", rindex=PropertyTable.constant_value(1.0)))
+    silicon = scene.add_material(Material(name="silicon", rindex=PropertyTable.constant_value(3.5)))
+    wrap = scene.add_surface(Surface(name="Reflector", type=SurfaceType.DielectricMetal, model=SurfaceModel.Unified,
+                                     finish=SurfaceFinish.Polished,
+                                     reflectivity=PropertyTable.constant_value(p.wrap_reflectivity)))
+    pde = reference_pde_nodes() if pde_nodes is None else np.asarray(pde_nodes, np.float32)
+    cathode = scene.add_surface(Surface(name="Cathode", type=SurfaceType.DielectricMetal, model=SurfaceModel.Unified,
+                                        finish=SurfaceFinish.Polished,
+                                        reflectivity=PropertyTable.constant_value(0.0),
+                                        efficiency=PropertyTable(node_energies(), pde)))
+    return air, silicon, wrap, cathode
+
+
+@dataclass
+class CrystalScene:
+    scene: Scene
+    world_id: int
+    crystal_ids: list
+    sipm_ids: list
+    crystal_materials: list
+    wrap_surface: int
+    cathode_surface: int
+    params: CrystalParams
+
+
+def crystal_scene(p: CrystalParams | None = None, abs_nodes=None, pde_nodes=None) -> CrystalScene:
+    """Single wrapped crystal with a SiPM on its +z face."""
+    p = p or CrystalParams()
+    scene = Scene()
+    air, silicon, wrap, cathode = _common_materials_and_surfaces(scene, p, pde_nodes)
+    crystal_mat = scene.add_material(_crystal_material(p, abs_nodes))
+    world = scene.add_volume(Volume(name="World", material_id=air, center_mm=(0, 0, 0), half_mm=(100, 100, 100)))
+    scene.world_volume_id = world
+    hz = p.half_mm[2]
+    crystal = scene.add_volume(Volume(name="Crystal", material_id=crystal_mat, parent_volume_id=world,
+                                      center_mm=(0, 0, 0), half_mm=p.half_mm))
+    sipm = scene.add_volume(Volume(name="SiPM", material_id=silicon, parent_volume_id=world,
+                                   skin_surface_id=cathode, sensor_id=0,
+                                   center_mm=(0, 0, hz + 0.5 * p.sipm_thickness_mm),
+                                   half_mm=(p.half_mm[0], p.half_mm[1], 0.5 * p.sipm_thickness_mm)))
+    scene.add_surface_binding(SurfaceBinding(crystal, world, wrap))
+    return CrystalScene(scene, world, [crystal], [sipm], [crystal_mat], wrap, cathode, p)
+
+
+def array_scene(p: CrystalParams | None = None, n: int = 8, pitch_mm: float = 3.2, abs_nodes=None,
+                pde_nodes=None, septa: bool = True) -> CrystalScene:
+    """n x n crystals with reflective septa, one SiPM per crystal (2 n^2 boxes).
+
+    septa=False leaves the crystal faces bare (Fresnel crystal/air), so photons
+    cross the world between crystals.
+    """
+    p = p or CrystalParams()
+    scene = Scene()
+    air, silicon, wrap, cathode = _common_materials_and_surfaces(scene, p, pde_nodes)
+    world = scene.add_volume(Volume(name="World", material_id=air, center_mm=(0, 0, 0), half_mm=(100, 100, 100)))
+    scene.world_volume_id = world
+    hz = p.half_mm[2]
+    crystals, sipms, mats = [], [], []
+    for i in range(n):
+        for j in range(n):
+            k = i * n + j
+            x = (i - 0.5 * (n - 1)) * pitch_mm
+            y = (j - 0.5 * (n - 1)) * pitch_mm
+            nodes = None if abs_nodes is None else np.asarray(abs_nodes)[k]
+            m = scene.add_material(_crystal_material(p, nodes, name=f"crystal{k}"))
+            c = scene.add_volume(Volume(name=f"Crystal{k}", material_id=m, parent_volume_id=world,
+                                        center_mm=(x, y, 0), half_mm=p.half_mm))
+            s = scene.add_volume(Volume(name=f"SiPM{k}", material_id=silicon, parent_volume_id=world,
+                                        skin_surface_id=cathode, sensor_id=k,
+                                        center_mm=(x, y, hz + 0.5 * p.sipm_thickness_mm),
+                                        half_mm=(p.half_mm[0], p.half_mm[1], 0.5 * p.sipm_thickness_mm)))
+            if septa:
+                scene.add_surface_binding(SurfaceBinding(c, world, wrap))
+            crystals.append(c)
+            sipms.append(s)
+            mats.append(m)
+    return CrystalScene(scene, world, crystals, sipms, mats, wrap, cathode, p)
+
+
+def synthetic_source(cs: CrystalScene, crystal_index: int = 0, mean_nm: float = 420.0, sigma_nm: float = 20.0,
+                     decay_ns: float = 40.0) -> abi.SyntheticSource:
+    vol = cs.scene.volumes[cs.crystal_ids[crystal_index]]
+    lo = [c - h for c, h in zip(vol.center_mm, vol.half_mm)]
+    hi = [c + h for c, h in zip(vol.center_mm, vol.half_mm)]
+    src = abi.SyntheticSource()
+    src.lo_mm[:] = lo
+    src.hi_mm[:] = hi
+    src.wavelength_mean_nm = mean_nm
+    src.wavelength_sigma_nm = sigma_nm
+    src.decay_time_ns = decay_ns
+    src.volume_id = vol.volume_id
+    return src
+
+
+def lognormal_counts(n_events: int, median: float, sigma: float = 1.5, lo: int = 10, hi: int = 1_000_000,
+                     seed: int = 0) -> np.ndarray:
+    """clip(round(exp(ln(median) + sigma z)), lo, hi), z ~ N(0, 1), seeded."""
+    rng = np.random.default_rng(seed)
+    z = rng.standard_normal(n_events)
+    return np.clip(np.rint(np.exp(np.log(median) + sigma * z)), lo, hi).astype(np.uint32)
+
+
+def electron_emissions(cs: CrystalScene, n_events: int, seed: int = 0, energy_mev: float = 1.0,
+                       yield_per_kev: float = 30.0, steps: int = 20, range_mm: float = 1.3,
+                       first_event_id: int = 0, crystal_index: int = 0):
+    """Scintillation emission records of `n_events` electrons entering the crystal on its axis.
+
+    Each electron is a chain of `steps` straight segments from the -z face
+    centre with a small random deflection per segment, depositing equal
+    energy per segment; photons per segment ~ Poisson(yield * dE).  Returns
+    (emissions, event_ids, emission_event_indices).
+    """
+    rng = np.random.default_rng(seed)
+    vol = cs.scene.volumes[cs.crystal_ids[crystal_index]]
+    mat = vol.material_id
+    start = np.array([vol.center_mm[0], vol.center_mm[1], vol.center_mm[2] - vol.half_mm[2] + 1e-3])
+    seg = range_mm / steps
+    mean_per_step = yield_per_kev * energy_mev * 1000.0 / steps
+    em = np.zeros(n_events * steps, dtype=abi.EMISSION_DTYPE)
+    idx = np.repeat(np.arange(n_events, dtype=np.uint32), steps)
+    event_ids = np.arange(first_event_id, first_event_id + n_events, dtype=np.uint32)
+    beta = 0.94
+    v = beta * 299.792458
+    for e in range(n_events):
+        pos = start.copy()
+        d = np.array([0.0, 0.0, 1.0])
+        first = 0
+        counts = rng.poisson(mean_per_step, steps)
+        for s in range(steps):
+            d = d + rng.normal(0.0, 0.25, 3)
+            d /= np.linalg.norm(d)
+            if d[2] < 0.2:
+                d[2] = 0.2
+                d /= np.linalg.norm(d)
+            delta = d * seg
+            r = em[e * steps + s]
+            r["position_mm"] = pos
+            r["time_ns"] = s * seg / v
+            r["direction"] = d
+            r["step_length_mm"] = seg
+            r["step_delta_mm"] = delta
+            r["pre_velocity_mm_per_ns"] = v
+            r["delta_velocity_mm_per_ns"] = -0.01 * v
+            r["energy_ev"] = 0.0
+            r["pre_mean_photon_count"] = mean_per_step
+            r["post_mean_photon_count"] = mean_per_step
+            r["decay_time_ns"] = 40.0
+            r["rise_time_ns"] = 0.0
+            r["charge"] = -1.0
+            r["event_id"] = event_ids[e]
+            r["first_photon_id"] = first
+            r["photon_count"] = counts[s]
+            r["volume_id"] = vol.volume_id
+            r["material_id"] = mat
+            r["spectrum_id"] = 0
+            r["type"] = 2
+            em[e * steps + s] = r
+            first += int(counts[s])
+            pos = pos + delta
+    return em, event_ids, idx

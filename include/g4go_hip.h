@@ -1,0 +1,257 @@
+/*
+ * g4go_hip.h -- C ABI of the MI355X (gfx950) optical-photon transport backend.
+ *
+ * This is the drop-in boundary for G4GO's GPU optical backend.  The reference
+ * binds its backend through the C++ class
+ *     G4GO::Optical::OptiXTransportHost
+ * (include/g4go/optical/optix/OptiXTransportHost.hpp:10-26) with the methods
+ *     OptiXTransportHost(PhotonTransportConfig)      -> g4go_create
+ *     PrepareScene(const Scene&)                      -> g4go_prepare_scene
+ *     EnqueueEmissions(const Scene&, const Batch&)    -> g4go_enqueue_emissions
+ *     CompleteOldestBatch() -> PhotonTransportOutput  -> g4go_complete
+ * and the CUB compaction helpers QueryOptiXHitCompactionBytes/CompactOptiXHits
+ * (include/g4go/optical/optix/OptiXHitCompaction.hpp:10-14), which this
+ * backend fuses into the transport kernel (no separate entry point).
+ *
+ * Every function returns a g4go_status; on failure g4go_last_error() holds a
+ * message.  No C++ exception crosses this boundary.  Plain pointers and sizes
+ * only; all host buffers passed in are borrowed for the duration of the call.
+ */
+#ifndef G4GO_HIP_H
+#define G4GO_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes, mapped by the C++ shim onto the reference's exception types
+ * (OptiXTransportHost.cpp:47-52, 574-630, 514-525, 780-781). */
+typedef enum g4go_status {
+    G4GO_OK = 0,
+    G4GO_ERROR_RUNTIME = 1,          /* std::runtime_error: HIP failure, device fault */
+    G4GO_ERROR_INVALID_ARGUMENT = 2, /* std::invalid_argument: bad scene or batch */
+    G4GO_ERROR_OVERFLOW = 3,         /* std::overflow_error: > 2^32-1 photons/emissions/events */
+    G4GO_ERROR_LOGIC = 4             /* std::logic_error: queue full/empty, no scene */
+} g4go_status;
+
+#define G4GO_INVALID_ID 0xffffffffu
+
+typedef struct g4go_transport g4go_transport;
+
+/* PhotonTransportConfig (include/g4go/optical/PhotonTransportConfig.hpp:26-40),
+ * plus the launch form of this backend. */
+typedef struct g4go_config {
+    uint64_t seed;
+    uint32_t max_photons_per_event;
+    uint32_t max_bounces_per_photon;
+    float boundary_tolerance_mm;
+    uint32_t target_photons_per_batch;
+    uint32_t max_events_per_batch;
+    uint32_t enable_performance_diagnostics;
+    int32_t device;        /* HIP device ordinal, -1 = current */
+    uint32_t sort_hits;    /* 1: detections returned in (eventID, photonID) order */
+} g4go_config;
+
+/* Launch form.  persistent=0: one wave per work item of <= 64 photons over a
+ * grid covering the batch.  persistent=1: `grid_blocks` blocks (0 = 2 per CU
+ * per wave slot, see blocks_per_cu) pop work items of `chunk` photons from a
+ * device queue and refill idle lanes when at least refill_threshold of 64 are
+ * idle. */
+typedef struct g4go_launch {
+    uint32_t persistent;
+    uint32_t block_size;       /* 64, 128, 256 */
+    uint32_t chunk;            /* photons per work item (persistent form) */
+    uint32_t refill_threshold; /* 1..64 */
+    uint32_t blocks_per_cu;    /* persistent grid = blocks_per_cu * CU count */
+} g4go_launch;
+
+/* A property table: `count` (energy eV, value) pairs with ascending energies;
+ * count = 0 means the property is absent.  Mirrors PropertyTable
+ * (include/g4go/optical/Scene.hpp:61-67). */
+typedef struct g4go_property {
+    uint32_t count;
+    const float* energy_ev;
+    const float* values;
+} g4go_property;
+
+/* Material (Scene.hpp:69-76) plus a Rayleigh scattering length, which the
+ * reference exporter rejects (Geant4SceneExporter.cpp:501-503). */
+typedef struct g4go_material {
+    g4go_property rindex;
+    g4go_property group_velocity_mm_per_ns;
+    g4go_property abs_length_mm;
+    g4go_property rayleigh_length_mm;
+    g4go_property scintillation_spectrum[3]; /* raw spectra; the CDF is built here */
+} g4go_material;
+
+/* Surface (Scene.hpp:78-92). type: 0 dielectric_dielectric, 1 dielectric_metal;
+ * model: 0 glisur, 1 unified; finish: 0 polished, 1 ground. */
+typedef struct g4go_surface {
+    uint32_t type, model, finish;
+    float model_value;
+    g4go_property reflectivity, efficiency, transmittance;
+    g4go_property specular_lobe, specular_spike, backscatter, surface_roughness;
+} g4go_surface;
+
+/* Volume (Scene.hpp:107-120) with an axis-aligned box solid in world
+ * coordinates (centre and half lengths, mm) in place of a triangle mesh. */
+typedef struct g4go_volume {
+    uint32_t volume_id;
+    uint32_t material_id;
+    uint32_t parent_volume_id; /* G4GO_INVALID_ID for the world */
+    uint32_t skin_surface_id;  /* G4GO_INVALID_ID = none */
+    uint32_t sensor_id;        /* G4GO_INVALID_ID = not a sensor */
+    float center_mm[3];
+    float half_mm[3];
+} g4go_volume;
+
+typedef struct g4go_surface_binding { /* SurfaceBinding (Scene.hpp:122-126) */
+    uint32_t from_volume_id, to_volume_id, surface_id;
+} g4go_surface_binding;
+
+typedef struct g4go_scene {
+    const g4go_material* materials;
+    uint32_t n_materials;
+    const g4go_surface* surfaces;
+    uint32_t n_surfaces;
+    const g4go_volume* volumes;
+    uint32_t n_volumes;
+    const g4go_surface_binding* bindings;
+    uint32_t n_bindings;
+    uint32_t world_volume_id;
+} g4go_scene;
+
+/* OpticalEmission, 128 bytes, layout-identical to
+ * include/g4go/optical/PhotonTransport.hpp:40-69 (type: 0 Direct,
+ * 1 Cerenkov, 2 Scintillation). */
+typedef struct g4go_emission {
+    float position_mm[3];
+    float time_ns;
+    float direction[3];
+    float step_length_mm;
+    float step_delta_mm[3];
+    float pre_velocity_mm_per_ns;
+    float polarization[3];
+    float delta_velocity_mm_per_ns;
+    float energy_ev;
+    float pre_mean_photon_count;
+    float post_mean_photon_count;
+    float decay_time_ns;
+    float rise_time_ns;
+    float charge;
+    uint32_t event_id;
+    uint32_t first_photon_id;
+    uint32_t photon_count;
+    uint32_t volume_id;
+    uint32_t material_id;
+    uint32_t spectrum_id;
+    uint8_t type;
+    uint8_t reserved[15];
+} g4go_emission;
+
+/* Synthetic source: positions uniform in [lo, hi], isotropic directions,
+ * random linear polarisation, wavelength N(mean, sigma) nm, emission time
+ * Exp(decay) ns; `volume_id` is the volume the photons start in. */
+typedef struct g4go_synthetic_source {
+    float lo_mm[3];
+    float hi_mm[3];
+    float wavelength_mean_nm;
+    float wavelength_sigma_nm;
+    float decay_time_ns;
+    uint32_t volume_id;
+} g4go_synthetic_source;
+
+/* Detected photon (PhotonDetection, PhotonTransport.hpp:71-81, with the
+ * arrival time in FP64). 48 bytes. */
+typedef struct g4go_hit {
+    float position_mm[3];
+    float direction[3];
+    float energy_ev;
+    uint32_t sensor_id;
+    uint32_t event_id;
+    uint32_t photon_id;
+    double time_ns;
+} g4go_hit;
+
+/* Per-event statistics (PhotonTransportEventStatistics, PhotonTransport.hpp:37-39). */
+typedef struct g4go_event_stats {
+    uint32_t event_id;
+    uint32_t reserved;
+    uint64_t captured, detected, absorbed, escaped, truncated, max_bounce, invalid, zero_step;
+} g4go_event_stats;
+
+#define G4GO_HIST_BINS 256
+#define G4GO_HIST_SLOTS 258 /* bins 0..255 over [0, 200) ns, 256 underflow, 257 overflow */
+
+typedef struct g4go_batch_stats {
+    uint64_t captured, generated, detected, absorbed, escaped, truncated, max_bounce, invalid, zero_step;
+    uint64_t total_steps;      /* transport steps (totalBounceCount) */
+    uint64_t dropped_hits;     /* detections beyond the hit buffer (always 0 when sized by photons) */
+    double transport_time_ms;  /* enqueue -> completion wall time */
+    double kernel_ms;          /* transport kernel, HIP events */
+    double host_to_device_ms;
+    double device_to_host_ms;
+    uint64_t host_to_device_bytes;
+    uint64_t device_to_host_bytes;
+} g4go_batch_stats;
+
+/* Results of one completed batch.  Arrays are owned by the handle and stay
+ * valid until the next g4go_complete or g4go_destroy. */
+typedef struct g4go_batch_result {
+    const g4go_hit* hits;
+    uint64_t n_hits;
+    const g4go_event_stats* events;
+    uint64_t n_events;
+    const uint64_t* histogram; /* G4GO_HIST_SLOTS counts */
+    g4go_batch_stats stats;
+} g4go_batch_result;
+
+/* ---- lifecycle (OptiXTransportHost ctor/dtor) ---- */
+g4go_status g4go_create(const g4go_config* config, g4go_transport** out);
+void g4go_destroy(g4go_transport* handle);
+/* Message of the last failure on this handle (or of the last g4go_create
+ * failure when handle is NULL). */
+const char* g4go_last_error(const g4go_transport* handle);
+g4go_status g4go_set_launch(g4go_transport* handle, const g4go_launch* launch);
+g4go_status g4go_get_launch(const g4go_transport* handle, g4go_launch* launch);
+
+/* ---- PrepareScene (OptiXTransportHost.cpp:511-519) ---- */
+g4go_status g4go_prepare_scene(g4go_transport* handle, const g4go_scene* scene);
+
+/* ---- EnqueueEmissions (OptiXTransportHost.cpp:521-777) ----
+ * emissions[n_emissions], event_ids[n_events], emission_event_indices[n_emissions]. */
+g4go_status g4go_enqueue_emissions(g4go_transport* handle, const g4go_emission* emissions, uint64_t n_emissions,
+                                   const uint32_t* event_ids, uint64_t n_events,
+                                   const uint32_t* emission_event_indices);
+
+/* Synthetic batch: event_ids[n_events] with photon_counts[n_events] photons
+ * each, generated on the device from (seed, eventID, photonID). */
+g4go_status g4go_enqueue_synthetic(g4go_transport* handle, const g4go_synthetic_source* source,
+                                   const uint32_t* event_ids, const uint32_t* photon_counts, uint64_t n_events);
+
+/* ---- CompleteOldestBatch (OptiXTransportHost.cpp:779-883) ---- */
+g4go_status g4go_complete(g4go_transport* handle, g4go_batch_result* out);
+
+/* Same as g4go_complete, but the detections stay in device memory: out->hits
+ * is NULL, out->n_hits is their count and *hits_device points at n_hits
+ * g4go_hit records in HBM (unordered), valid until the next enqueue. */
+g4go_status g4go_complete_device(g4go_transport* handle, g4go_batch_result* out, const void** hits_device);
+
+/* ---- test entry points (device evaluation of the shared physics) ----
+ * which: 0 log(x), 1 exp(x), 2 sin/cos(2*pi*x) (two outputs per input). */
+g4go_status g4go_eval_math(g4go_transport* handle, int which, const float* in, float* out, uint64_t n);
+/* 11 inputs per case (dir[3], pol[3], normal[3], n1, n2) -> 14 outputs
+ * (transmittance, tir, reflected dir[3], transmitted dir[3], reflected pol[3], transmitted pol[3]). */
+g4go_status g4go_eval_fresnel(g4go_transport* handle, const float* in, float* out, uint64_t n);
+
+/* Library version string. */
+const char* g4go_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* G4GO_HIP_H */

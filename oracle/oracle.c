@@ -1,0 +1,1461 @@
+/*
+ * oracle.c -- CPU restatement of G4GO's optical-photon transport, used ONLY
+ * as the checker of the HIP backend (tests, smoke check, CPU baseline leg of
+ * bench.py).  It is never linked into or called by the product.
+ *
+ * The same source builds twice:
+ *   liboracle32.so  FP32 state, FP64 time: bit-exact replay of the HIP
+ *                   kernels on the same Philox streams (IEEE float, no FP
+ *                   contraction, explicit fmaf, Cephes elementary functions);
+ *   liboracle64.so  FP64 everywhere with libm elementary functions: the
+ *                   reference semantics without FP32 rounding, used for the
+ *                   statistical gates and the FP64 replay gradient.
+ *
+ * Reference algorithm, function by function (paths relative to the G4GO tree):
+ *   or_philox / or_uniform       src/optical/optix/OptiXTransportDevice.cu:254-286
+ *   or_prop                      OptiXTransportDevice.cu:198-216 (linear scan)
+ *   or_spectrum                  OptiXTransportDevice.cu:322-344,
+ *                                CDF: src/optical/optix/OptiXTransportHost.cpp:371-410
+ *   or_normalize .. or_fresnel   src/optical/optix/OptiXBoundaryPhysics.cuh:47-178
+ *   or_classify / or_absorption  OptiXBoundaryPhysics.cuh:79-101
+ *   or_lambertian                OptiXBoundaryPhysics.cuh:103-120
+ *   or_find_surface              OptiXTransportDevice.cu:218-252, src/optical/Scene.cpp:120-150
+ *   or_facet / or_unified ...    OptiXTransportDevice.cu:492-595
+ *   or_generate_emission         OptiXTransportDevice.cu:346-481
+ *   or_step                      OptiXTransportDevice.cu:708-931 (bounce loop body)
+ *   or_run                       OptiXTransportDevice.cu:660-941 + the batch
+ *                                statistics of OptiXTransportHost.cpp:809-844
+ *   Rayleigh (absent from the reference GPU path): Geant4 11 G4OpRayleigh,
+ *   rotateUz: CLHEP Hep3Vector::rotateUz.
+ *
+ * Deliberate differences from the reference, shared with the HIP backend:
+ * the draw map (four draws per Philox block, see physics.h), box solids with
+ * analytic faces instead of OptiX triangles, no epsilon push-off (the photon
+ * is snapped onto the face plane), the competing-exponential bulk step
+ * (absorption + Rayleigh), FP64 time, sin/cos of 2*pi*u by turns.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "../include/g4go_hip.h"
+
+#ifdef ORACLE_DOUBLE
+typedef double real;
+#define RF(x) x
+#define R_SQRT sqrt
+#define R_FABS fabs
+#define R_FMIN fmin
+#define R_FMAX fmax
+#define R_FLOOR floor
+#define R_FMA fma
+#define R_INF ((double)INFINITY)
+#define ORACLE_NAME(n) oracle64_##n
+#else
+typedef float real;
+#define RF(x) x##f
+#define R_SQRT sqrtf
+#define R_FABS fabsf
+#define R_FMIN fminf
+#define R_FMAX fmaxf
+#define R_FLOOR floorf
+#define R_FMA fmaf
+#define R_INF INFINITY
+#define ORACLE_NAME(n) oracle32_##n
+#endif
+
+#define INVALID 0xffffffffu
+
+/* ------------------------------------------------------------------ */
+/* Philox4x32-10                                                       */
+/* ------------------------------------------------------------------ */
+void ORACLE_NAME(philox)(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+    uint32_t x0 = ctr[0], x1 = ctr[1], x2 = ctr[2], x3 = ctr[3];
+    uint32_t k0 = key[0], k1 = key[1];
+    for (int r = 0; r < 10; ++r) {
+        const uint64_t a = (uint64_t)0xD2511F53u * x0;
+        const uint64_t b = (uint64_t)0xCD9E8D57u * x2;
+        const uint32_t y0 = (uint32_t)(b >> 32) ^ x1 ^ k0;
+        const uint32_t y1 = (uint32_t)b;
+        const uint32_t y2 = (uint32_t)(a >> 32) ^ x3 ^ k1;
+        const uint32_t y3 = (uint32_t)a;
+        x0 = y0;
+        x1 = y1;
+        x2 = y2;
+        x3 = y3;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    out[0] = x0;
+    out[1] = x1;
+    out[2] = x2;
+    out[3] = x3;
+}
+
+/* The reference Uniform(): one block per draw, word draw % 4. */
+float ORACLE_NAME(uniform)(uint64_t seed, uint64_t photon_key, uint32_t bounce, uint32_t process, uint32_t draw) {
+    const uint32_t ctr[4] = {(uint32_t)photon_key, (uint32_t)(photon_key >> 32), bounce, (process << 16) ^ draw};
+    const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    uint32_t w[4];
+    ORACLE_NAME(philox)(ctr, key, w);
+    return ((float)w[draw % 4u] + 0.5f) / 4294967296.0f;
+}
+
+typedef struct {
+    uint32_t k0, k1;
+} rng_t;
+
+static void block4(const rng_t* r, uint32_t photon, uint32_t event, uint32_t step, uint32_t process, uint32_t blk,
+                   uint32_t w[4]) {
+    const uint32_t ctr[4] = {photon, event, step, (process << 16) ^ blk};
+    const uint32_t key[2] = {r->k0, r->k1};
+    ORACLE_NAME(philox)(ctr, key, w);
+}
+
+enum { P_STEP = 1, P_RAYLEIGH = 2, P_FACET = 3, P_LOBE = 4, P_ROUGH = 5, P_SYNTH = 6, P_EMISSION = 7 };
+
+#ifdef ORACLE_DOUBLE
+static real u01(uint32_t w) { return ((double)w + 0.5) / 4294967296.0; }
+static real r_log(real x) { return log(x); }
+static real r_exp(real x) { return exp(x); }
+static void r_sincos_turn(real u, real* s, real* c) {
+    const double a = 6.283185307179586476925 * u;
+    *s = sin(a);
+    *c = cos(a);
+}
+#else
+/* (float(w) + 0.5) / 2^32 written as one fma: identical rounding. */
+static real u01(uint32_t w) { return fmaf((float)w, 0x1p-32f, 0x1p-33f); }
+
+static uint32_t fbits(float x) {
+    uint32_t b;
+    memcpy(&b, &x, 4);
+    return b;
+}
+static float bitsf(uint32_t b) {
+    float x;
+    memcpy(&x, &b, 4);
+    return x;
+}
+
+/* Cephes logf (Moshier 1989). */
+static float r_log(float x) {
+    if (!(x > 0.0f)) {
+        return x == 0.0f ? -INFINITY : NAN;
+    }
+    if (isinf(x)) {
+        return x;
+    }
+    uint32_t b = fbits(x);
+    int e = 0;
+    if (b < 0x00800000u) {
+        x = x * 33554432.0f;
+        b = fbits(x);
+        e = -25;
+    }
+    e += (int)(b >> 23) - 126;
+    float m = bitsf((b & 0x007fffffu) | 0x3f000000u);
+    if (m < 0.70710678118654752f) {
+        e -= 1;
+        m = (m + m) - 1.0f;
+    } else {
+        m = m - 1.0f;
+    }
+    const float z = m * m;
+    float y = 7.0376836292e-2f;
+    y = fmaf(y, m, -1.1514610310e-1f);
+    y = fmaf(y, m, 1.1676998740e-1f);
+    y = fmaf(y, m, -1.2420140846e-1f);
+    y = fmaf(y, m, 1.4249322787e-1f);
+    y = fmaf(y, m, -1.6668057665e-1f);
+    y = fmaf(y, m, 2.0000714765e-1f);
+    y = fmaf(y, m, -2.4999993993e-1f);
+    y = fmaf(y, m, 3.3333331174e-1f);
+    y = (y * m) * z;
+    const float fe = (float)e;
+    y = fmaf(fe, -2.12194440e-4f, y);
+    y = fmaf(-0.5f, z, y);
+    float r = m + y;
+    r = fmaf(fe, 0.693359375f, r);
+    return r;
+}
+
+/* Cephes expf with exact 2^n scaling; flush below the normal range. */
+static float r_exp(float x) {
+    if (x != x) {
+        return x;
+    }
+    if (x > 88.7228391f) {
+        return INFINITY;
+    }
+    if (x < -87.3365447f) {
+        return 0.0f;
+    }
+    const float n = floorf(fmaf(x, 1.44269504088896341f, 0.5f));
+    float r = fmaf(n, -0.693359375f, x);
+    r = fmaf(n, 2.12194440e-4f, r);
+    const float z = r * r;
+    float y = 1.9875691500e-4f;
+    y = fmaf(y, r, 1.3981999507e-3f);
+    y = fmaf(y, r, 8.3334519073e-3f);
+    y = fmaf(y, r, 4.1665795894e-2f);
+    y = fmaf(y, r, 1.6666665459e-1f);
+    y = fmaf(y, r, 5.0000001201e-1f);
+    y = fmaf(y, z, r);
+    y = y + 1.0f;
+    int ni = (int)n;
+    if (ni > 127) {
+        y = y * 2.0f;
+        ni -= 1;
+    }
+    return y * bitsf((uint32_t)(ni + 127) << 23);
+}
+
+/* sin, cos of 2*pi*u: reduction in turns, Cephes sinf/cosf polynomials. */
+static void r_sincos_turn(float u, float* s, float* c) {
+    const float t = u * 4.0f;
+    const float q = floorf(t + 0.5f);
+    const float x = (t - q) * 1.57079632679489662f;
+    const float z = x * x;
+    const float sp = fmaf(fmaf(-1.9515295891e-4f, z, 8.3321608736e-3f), z, -1.6666654611e-1f);
+    const float sn = fmaf(sp * z, x, x);
+    const float cp = fmaf(fmaf(2.443315711809948e-5f, z, -1.388731625493765e-3f), z, 4.166664568298827e-2f);
+    const float cs = fmaf(cp * z, z, fmaf(-0.5f, z, 1.0f));
+    const int quadrant = ((int)q) & 3;
+    if (quadrant == 0) {
+        *s = sn;
+        *c = cs;
+    } else if (quadrant == 1) {
+        *s = cs;
+        *c = -sn;
+    } else if (quadrant == 2) {
+        *s = -sn;
+        *c = -cs;
+    } else {
+        *s = -cs;
+        *c = sn;
+    }
+}
+#endif
+
+real ORACLE_NAME(log)(real x) { return r_log(x); }
+real ORACLE_NAME(exp)(real x) { return r_exp(x); }
+void ORACLE_NAME(sincos_turn)(real u, real* s, real* c) { r_sincos_turn(u, s, c); }
+
+/* ------------------------------------------------------------------ */
+/* Vectors and boundary physics (OptiXBoundaryPhysics.cuh)             */
+/* ------------------------------------------------------------------ */
+typedef struct {
+    real x, y, z;
+} v3;
+
+static v3 mk(real x, real y, real z) {
+    v3 v = {x, y, z};
+    return v;
+}
+static v3 vadd(v3 a, v3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+static v3 vsub(v3 a, v3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+static v3 vscale(v3 a, real s) { return mk(a.x * s, a.y * s, a.z * s); }
+static real vdot(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static v3 vcross(v3 a, v3 b) { return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+static real vaxis(v3 a, int k) { return k == 0 ? a.x : (k == 1 ? a.y : a.z); }
+static int finite_r(real x) { return isfinite(x); }
+
+static v3 or_normalize(v3 a) {
+    const real l2 = vdot(a, a);
+    if (!(l2 > RF(0.0))) {
+        return mk(RF(0.0), RF(0.0), RF(0.0));
+    }
+    return vscale(a, RF(1.0) / R_SQRT(l2));
+}
+
+static real clamp01(real v) { return R_FMIN(R_FMAX(v, RF(0.0)), RF(1.0)); }
+
+static v3 or_project(v3 pol, v3 dir) {
+    v3 out = vsub(pol, vscale(dir, vdot(pol, dir)));
+    if (vdot(out, out) < RF(1.0e-12)) {
+        const v3 helper = R_FABS(dir.z) < RF(0.9) ? mk(RF(0.0), RF(0.0), RF(1.0)) : mk(RF(1.0), RF(0.0), RF(0.0));
+        out = vcross(dir, helper);
+    }
+    return or_normalize(out);
+}
+
+static v3 or_reflect_dir(v3 d, v3 n) { return or_normalize(vsub(d, vscale(n, RF(2.0) * vdot(d, n)))); }
+
+static v3 or_reflect_pol(v3 p, v3 n, v3 d) {
+    const v3 r = vadd(vscale(p, RF(-1.0)), vscale(n, RF(2.0) * vdot(p, n)));
+    return or_project(r, d);
+}
+
+enum { O_ABSORB = 0, O_DETECT = 1, O_TRANSMIT = 2, O_INTERACT = 3 };
+
+int ORACLE_NAME(classify)(real r, real t, int has_r, int has_t, real sample) {
+    const real u = clamp01(sample);
+    const real rr = has_r ? clamp01(r) : RF(0.0);
+    const real tt = has_t ? clamp01(t) : RF(0.0);
+    if (!has_r && !has_t) {
+        return O_INTERACT;
+    }
+    if (u > rr + tt) {
+        return O_ABSORB;
+    }
+    if (u > rr) {
+        return O_TRANSMIT;
+    }
+    return O_INTERACT;
+}
+
+int ORACLE_NAME(absorption)(real eff, int sensor, real sample) {
+    return (sensor && sample < clamp01(eff)) ? O_DETECT : O_ABSORB;
+}
+
+static v3 or_lambertian(v3 n, real radial_s, real azim_s) {
+    n = or_normalize(n);
+    radial_s = clamp01(radial_s);
+    azim_s = clamp01(azim_s);
+    const real radial = R_SQRT(radial_s);
+    real s, c;
+    r_sincos_turn(azim_s, &s, &c);
+    const real nc = R_SQRT(R_FMAX(RF(0.0), RF(1.0) - radial_s));
+    const v3 helper = R_FABS(n.z) < RF(0.9) ? mk(RF(0.0), RF(0.0), RF(1.0)) : mk(RF(1.0), RF(0.0), RF(0.0));
+    const v3 t = or_normalize(vcross(helper, n));
+    const v3 b = vcross(n, t);
+    return or_normalize(vadd(vadd(vscale(t, radial * c), vscale(b, radial * s)), vscale(n, nc)));
+}
+
+void ORACLE_NAME(lambertian)(const real n[3], real radial, real azimuth, real out[3]) {
+    const v3 d = or_lambertian(mk(n[0], n[1], n[2]), radial, azimuth);
+    out[0] = d.x;
+    out[1] = d.y;
+    out[2] = d.z;
+}
+
+typedef struct {
+    v3 rdir, tdir, rpol, tpol;
+    real t;
+    int tir;
+} fresnel_t;
+
+static fresnel_t or_fresnel(v3 d, v3 p, v3 n, real n1, real n2) {
+    fresnel_t o;
+    memset(&o, 0, sizeof(o));
+    d = or_normalize(d);
+    n = or_normalize(n);
+    p = or_project(p, d);
+    const real ci = clamp01(-vdot(d, n));
+    const real eta = n1 / n2;
+    const real st2 = eta * eta * (RF(1.0) - ci * ci);
+    o.rdir = or_reflect_dir(d, n);
+    o.tir = st2 >= RF(1.0);
+    o.t = RF(0.0);
+    if (o.tir) {
+        o.rpol = or_reflect_pol(p, n, o.rdir);
+        return o;
+    }
+    const real ct = R_SQRT(R_FMAX(RF(0.0), RF(1.0) - st2));
+    o.tdir = or_normalize(vadd(vscale(d, eta), vscale(n, eta * ci - ct)));
+    const v3 tr = vcross(d, n);
+    if (vdot(tr, tr) < RF(1.0e-12)) {
+        const real ra = (n1 - n2) / (n1 + n2);
+        o.t = clamp01(RF(1.0) - ra * ra);
+        o.rpol = n2 > n1 ? vscale(p, RF(-1.0)) : p;
+        o.tpol = p;
+        return o;
+    }
+    const v3 sd = or_normalize(tr);
+    const v3 pd = or_normalize(vcross(d, sd));
+    const real is = vdot(p, sd);
+    const real ip = vdot(p, pd);
+    const real sden = n1 * ci + n2 * ct;
+    const real pden = n2 * ci + n1 * ct;
+    const real rs = (n1 * ci - n2 * ct) / sden;
+    const real rp = (n2 * ci - n1 * ct) / pden;
+    const real ts = RF(2.0) * n1 * ci / sden;
+    const real tp = RF(2.0) * n1 * ci / pden;
+    const real refl = is * is * rs * rs + ip * ip * rp * rp;
+    o.t = clamp01(RF(1.0) - refl);
+    const v3 rpd = or_normalize(vcross(o.rdir, sd));
+    const v3 tpd = or_normalize(vcross(o.tdir, sd));
+    o.rpol = or_normalize(vadd(vscale(sd, is * rs), vscale(rpd, ip * rp)));
+    o.tpol = or_normalize(vadd(vscale(sd, is * ts), vscale(tpd, ip * tp)));
+    return o;
+}
+
+/* 11 inputs -> 14 outputs, same packing as g4go_eval_fresnel. */
+void ORACLE_NAME(fresnel)(const real* in, real* out) {
+    const fresnel_t f = or_fresnel(mk(in[0], in[1], in[2]), mk(in[3], in[4], in[5]), mk(in[6], in[7], in[8]), in[9], in[10]);
+    out[0] = f.t;
+    out[1] = f.tir ? RF(1.0) : RF(0.0);
+    const v3 vs[4] = {f.rdir, f.tdir, f.rpol, f.tpol};
+    for (int k = 0; k < 4; ++k) {
+        out[2 + 3 * k] = vs[k].x;
+        out[3 + 3 * k] = vs[k].y;
+        out[4 + 3 * k] = vs[k].z;
+    }
+}
+
+/* CLHEP Hep3Vector::rotateUz. */
+static v3 or_rotate_uz(v3 v, v3 u) {
+    const real up2 = u.x * u.x + u.y * u.y;
+    if (up2 > RF(0.0)) {
+        const real up = R_SQRT(up2);
+        return mk((u.x * u.z * v.x - u.y * v.y) / up + u.x * v.z, (u.y * u.z * v.x + u.x * v.y) / up + u.y * v.z,
+                  -up * v.x + u.z * v.z);
+    }
+    if (u.z < RF(0.0)) {
+        return mk(-v.x, v.y, -v.z);
+    }
+    return v;
+}
+
+/* ------------------------------------------------------------------ */
+/* Scene (built from the public g4go_scene description)               */
+/* ------------------------------------------------------------------ */
+typedef struct {
+    uint32_t n;
+    real* e;
+    real* v;
+    int constant;
+} prop_t;
+
+typedef struct {
+    prop_t rindex, groupvel, abslen, rayleigh, spectrum[3];
+    real rindex_max;
+} material_t;
+
+typedef struct {
+    uint32_t type, model, finish;
+    real model_value;
+    prop_t refl, eff, trans, lobe, spike, back, rough;
+} surface_t;
+
+typedef struct {
+    float lo[3], hi[3]; /* box bounds are FP32 in both builds (as uploaded) */
+    uint32_t id, material, parent, skin, sensor;
+} volume_t;
+
+typedef struct {
+    uint32_t n_mat, n_surf, n_vol, n_bind, world;
+    material_t* mat;
+    surface_t* surf;
+    volume_t* vol;
+    uint32_t* bind; /* triples (from, to, surface) in input order */
+} oscene_t;
+
+static prop_t make_prop(const g4go_property* p) {
+    prop_t o;
+    memset(&o, 0, sizeof(o));
+    if (p->count == 0) {
+        return o;
+    }
+    o.n = p->count;
+    o.e = (real*)malloc(sizeof(real) * p->count);
+    o.v = (real*)malloc(sizeof(real) * p->count);
+    o.constant = 1;
+    for (uint32_t i = 0; i < p->count; ++i) {
+        o.e[i] = p->energy_ev[i];
+        o.v[i] = p->values[i];
+        if (p->values[i] != p->values[0]) {
+            o.constant = 0;
+        }
+    }
+    return o;
+}
+
+/* Trapezoid CDF in FP32 exactly as the host code computes it. */
+static prop_t make_spectrum(const g4go_property* p) {
+    prop_t o;
+    memset(&o, 0, sizeof(o));
+    if (p->count == 0) {
+        return o;
+    }
+    float* cdf = (float*)calloc(p->count, sizeof(float));
+    if (p->count == 1) {
+        cdf[0] = 1.0f;
+    } else {
+        for (uint32_t i = 1; i < p->count; ++i) {
+            const float de = p->energy_ev[i] - p->energy_ev[i - 1];
+            const float v0 = p->values[i - 1] > 0.0f ? p->values[i - 1] : 0.0f;
+            const float v1 = p->values[i] > 0.0f ? p->values[i] : 0.0f;
+            if (!(de >= 0.0f) || !isfinite(de) || !isfinite(v0) || !isfinite(v1)) {
+                free(cdf);
+                return o;
+            }
+            cdf[i] = cdf[i - 1] + 0.5f * de * (v0 + v1);
+        }
+        const float integral = cdf[p->count - 1];
+        if (!(integral > 0.0f) || !isfinite(integral)) {
+            free(cdf);
+            return o;
+        }
+        for (uint32_t i = 0; i < p->count; ++i) {
+            cdf[i] /= integral;
+        }
+        cdf[p->count - 1] = 1.0f;
+    }
+    o.n = p->count;
+    o.e = (real*)malloc(sizeof(real) * p->count);
+    o.v = (real*)malloc(sizeof(real) * p->count);
+    for (uint32_t i = 0; i < p->count; ++i) {
+        o.e[i] = p->energy_ev[i];
+        o.v[i] = cdf[i];
+    }
+    free(cdf);
+    return o;
+}
+
+static void free_prop(prop_t* p) {
+    free(p->e);
+    free(p->v);
+}
+
+static uint32_t vol_index(const oscene_t* s, uint32_t id) {
+    for (uint32_t i = 0; i < s->n_vol; ++i) {
+        if (s->vol[i].id == id) {
+            return i;
+        }
+    }
+    return INVALID;
+}
+
+static oscene_t* build(const g4go_scene* in) {
+    oscene_t* s = (oscene_t*)calloc(1, sizeof(oscene_t));
+    s->n_mat = in->n_materials;
+    s->n_surf = in->n_surfaces;
+    s->n_vol = in->n_volumes;
+    s->n_bind = in->n_bindings;
+    s->mat = (material_t*)calloc(s->n_mat + 1, sizeof(material_t));
+    s->surf = (surface_t*)calloc(s->n_surf + 1, sizeof(surface_t));
+    s->vol = (volume_t*)calloc(s->n_vol + 1, sizeof(volume_t));
+    s->bind = (uint32_t*)calloc(3 * s->n_bind + 3, sizeof(uint32_t));
+    for (uint32_t i = 0; i < s->n_mat; ++i) {
+        const g4go_material* m = &in->materials[i];
+        material_t* d = &s->mat[i];
+        d->rindex = make_prop(&m->rindex);
+        d->groupvel = make_prop(&m->group_velocity_mm_per_ns);
+        d->abslen = make_prop(&m->abs_length_mm);
+        d->rayleigh = make_prop(&m->rayleigh_length_mm);
+        for (int k = 0; k < 3; ++k) {
+            d->spectrum[k] = make_spectrum(&m->scintillation_spectrum[k]);
+        }
+        d->rindex_max = RF(1.0);
+        if (m->rindex.count) {
+            float mx = m->rindex.values[0];
+            for (uint32_t j = 1; j < m->rindex.count; ++j) {
+                mx = m->rindex.values[j] > mx ? m->rindex.values[j] : mx;
+            }
+            d->rindex_max = mx;
+        }
+    }
+    for (uint32_t i = 0; i < s->n_surf; ++i) {
+        const g4go_surface* f = &in->surfaces[i];
+        surface_t* d = &s->surf[i];
+        d->type = f->type;
+        d->model = f->model;
+        d->finish = f->finish;
+        d->model_value = f->model_value;
+        d->refl = make_prop(&f->reflectivity);
+        d->eff = make_prop(&f->efficiency);
+        d->trans = make_prop(&f->transmittance);
+        d->lobe = make_prop(&f->specular_lobe);
+        d->spike = make_prop(&f->specular_spike);
+        d->back = make_prop(&f->backscatter);
+        d->rough = make_prop(&f->surface_roughness);
+    }
+    for (uint32_t i = 0; i < s->n_vol; ++i) {
+        const g4go_volume* v = &in->volumes[i];
+        volume_t* d = &s->vol[i];
+        for (int k = 0; k < 3; ++k) {
+            d->lo[k] = (float)((double)v->center_mm[k] - v->half_mm[k]);
+            d->hi[k] = (float)((double)v->center_mm[k] + v->half_mm[k]);
+        }
+        d->id = v->volume_id;
+        d->material = v->material_id;
+        d->skin = v->skin_surface_id;
+        d->sensor = v->sensor_id;
+    }
+    for (uint32_t i = 0; i < s->n_vol; ++i) {
+        s->vol[i].parent = in->volumes[i].parent_volume_id == INVALID ? INVALID : vol_index(s, in->volumes[i].parent_volume_id);
+    }
+    s->world = vol_index(s, in->world_volume_id);
+    for (uint32_t i = 0; i < s->n_bind; ++i) {
+        s->bind[3 * i] = vol_index(s, in->bindings[i].from_volume_id);
+        s->bind[3 * i + 1] = vol_index(s, in->bindings[i].to_volume_id);
+        s->bind[3 * i + 2] = in->bindings[i].surface_id;
+    }
+    return s;
+}
+
+static void destroy(oscene_t* s) {
+    for (uint32_t i = 0; i < s->n_mat; ++i) {
+        free_prop(&s->mat[i].rindex);
+        free_prop(&s->mat[i].groupvel);
+        free_prop(&s->mat[i].abslen);
+        free_prop(&s->mat[i].rayleigh);
+        for (int k = 0; k < 3; ++k) {
+            free_prop(&s->mat[i].spectrum[k]);
+        }
+    }
+    for (uint32_t i = 0; i < s->n_surf; ++i) {
+        surface_t* d = &s->surf[i];
+        free_prop(&d->refl);
+        free_prop(&d->eff);
+        free_prop(&d->trans);
+        free_prop(&d->lobe);
+        free_prop(&d->spike);
+        free_prop(&d->back);
+        free_prop(&d->rough);
+    }
+    free(s->mat);
+    free(s->surf);
+    free(s->vol);
+    free(s->bind);
+    free(s);
+}
+
+/* SampleProperty: clamped linear interpolation, node found by linear scan. */
+static real or_prop(const prop_t* p, real e, real fallback) {
+    if (p->n == 0) {
+        return fallback;
+    }
+    if (p->constant || e <= p->e[0]) {
+        return p->v[0];
+    }
+    const uint32_t last = p->n - 1;
+    if (e >= p->e[last]) {
+        return p->v[last];
+    }
+    uint32_t i = 0;
+    while (i + 1 < p->n && e > p->e[i + 1]) {
+        ++i;
+    }
+    const real f = (e - p->e[i]) / (p->e[i + 1] - p->e[i]);
+    return p->v[i] + f * (p->v[i + 1] - p->v[i]);
+}
+
+/* SampleSpectrum: inverse CDF by linear scan. */
+static real or_spectrum(const prop_t* p, real random) {
+    if (p->n == 0) {
+        return NAN;
+    }
+    const real u = clamp01(random);
+    if (p->n == 1 || u <= p->v[0]) {
+        return p->e[0];
+    }
+    const uint32_t last = p->n - 1;
+    if (u >= p->v[last]) {
+        return p->e[last];
+    }
+    uint32_t i = 0;
+    while (i + 1 < p->n && u > p->v[i + 1]) {
+        ++i;
+    }
+    const real den = p->v[i + 1] - p->v[i];
+    if (!(den > RF(0.0))) {
+        return p->e[i];
+    }
+    const real f = (u - p->v[i]) / den;
+    return p->e[i] + f * (p->e[i + 1] - p->e[i]);
+}
+
+/* FindSurfaceID / Scene::FindBoundarySurface. */
+static uint32_t or_find_surface(const oscene_t* s, uint32_t from, uint32_t to) {
+    for (uint32_t i = 0; i < s->n_bind; ++i) {
+        if (s->bind[3 * i] == from && s->bind[3 * i + 1] == to) {
+            return s->bind[3 * i + 2];
+        }
+    }
+    if (from >= s->n_vol || to >= s->n_vol) {
+        return INVALID;
+    }
+    const volume_t* f = &s->vol[from];
+    const volume_t* t = &s->vol[to];
+    if (t->parent == from) {
+        return t->skin != INVALID ? t->skin : f->skin;
+    }
+    return f->skin != INVALID ? f->skin : t->skin;
+}
+
+/* Public wrapper for the surface-precedence fixtures (index space = input order). */
+uint32_t ORACLE_NAME(find_surface)(const g4go_scene* scene, uint32_t from_id, uint32_t to_id) {
+    oscene_t* s = build(scene);
+    const uint32_t r = or_find_surface(s, vol_index(s, from_id), vol_index(s, to_id));
+    destroy(s);
+    return r;
+}
+
+/* ------------------------------------------------------------------ */
+/* Photon state and geometry                                           */
+/* ------------------------------------------------------------------ */
+typedef struct {
+    v3 pos, dir, pol;
+    double time;
+    real energy;
+    uint32_t volume, step, photon, event, material;
+    real rindex, inv_v, ilen, afrac;
+} photon_t;
+
+static int cache_material(photon_t* ph, const oscene_t* s, uint32_t m) {
+    if (m >= s->n_mat) {
+        return 0;
+    }
+    const material_t* mt = &s->mat[m];
+    const real n = or_prop(&mt->rindex, ph->energy, NAN);
+    const int n_ok = finite_r(n) && n > RF(0.0);
+    const real v = or_prop(&mt->groupvel, ph->energy, n_ok ? RF(299.792458) / n : NAN);
+    if (!(v > RF(0.0)) || !finite_r(v)) {
+        return 0;
+    }
+    const real la = or_prop(&mt->abslen, ph->energy, R_INF);
+    const real lr = or_prop(&mt->rayleigh, ph->energy, R_INF);
+    const int a_ok = finite_r(la) && la > RF(0.0);
+    const int r_ok = finite_r(lr) && lr > RF(0.0);
+    ph->material = m;
+    ph->rindex = n;
+    ph->inv_v = RF(1.0) / v;
+    if (a_ok && r_ok) {
+        ph->ilen = (la * lr) / (la + lr);
+        ph->afrac = lr / (la + lr);
+    } else if (a_ok) {
+        ph->ilen = la;
+        ph->afrac = RF(1.0);
+    } else if (r_ok) {
+        ph->ilen = lr;
+        ph->afrac = RF(0.0);
+    } else {
+        ph->ilen = R_INF;
+        ph->afrac = RF(1.0);
+    }
+    return 1;
+}
+
+/* Exit face of a box: the smallest (distance to plane)/|d| chosen with cross
+ * products, x then y then z, strict comparisons. */
+static void box_exit(const volume_t* v, v3 p, v3 d, real* dist, int* axis) {
+    const real ax = R_FABS(d.x), ay = R_FABS(d.y), az = R_FABS(d.z);
+    const real nx = d.x > RF(0.0) ? R_FMAX((real)v->hi[0] - p.x, RF(0.0))
+                                  : (d.x < RF(0.0) ? R_FMAX(p.x - (real)v->lo[0], RF(0.0)) : RF(1.0));
+    const real ny = d.y > RF(0.0) ? R_FMAX((real)v->hi[1] - p.y, RF(0.0))
+                                  : (d.y < RF(0.0) ? R_FMAX(p.y - (real)v->lo[1], RF(0.0)) : RF(1.0));
+    const real nz = d.z > RF(0.0) ? R_FMAX((real)v->hi[2] - p.z, RF(0.0))
+                                  : (d.z < RF(0.0) ? R_FMAX(p.z - (real)v->lo[2], RF(0.0)) : RF(1.0));
+    int best = 0;
+    real bn = nx, ba = ax;
+    if (ny * ba < bn * ay) {
+        best = 1;
+        bn = ny;
+        ba = ay;
+    }
+    if (nz * ba < bn * az) {
+        best = 2;
+        bn = nz;
+        ba = az;
+    }
+    *axis = best;
+    *dist = bn / ba;
+}
+
+static real box_entry(const volume_t* b, v3 p, v3 inv, real tmin, int* axis) {
+    const real pc[3] = {p.x, p.y, p.z};
+    const real ic[3] = {inv.x, inv.y, inv.z};
+    real tn_max = -R_INF, tf_min = R_INF;
+    int a = 0;
+    for (int k = 0; k < 3; ++k) {
+        if (finite_r(ic[k])) {
+            const real t1 = ((real)b->lo[k] - pc[k]) * ic[k];
+            const real t2 = ((real)b->hi[k] - pc[k]) * ic[k];
+            const real tn = R_FMIN(t1, t2);
+            const real tf = R_FMAX(t1, t2);
+            if (tn > tn_max) {
+                tn_max = tn;
+                a = k;
+            }
+            tf_min = R_FMIN(tf_min, tf);
+        } else if (pc[k] < (real)b->lo[k] || pc[k] > (real)b->hi[k]) {
+            return R_INF;
+        }
+    }
+    if (tn_max >= tmin && tn_max <= tf_min) {
+        *axis = a;
+        return tn_max;
+    }
+    return R_INF;
+}
+
+static int box_contains(const volume_t* v, v3 q) {
+    return q.x >= (real)v->lo[0] && q.x <= (real)v->hi[0] && q.y >= (real)v->lo[1] && q.y <= (real)v->hi[1] &&
+           q.z >= (real)v->lo[2] && q.z <= (real)v->hi[2];
+}
+
+/* Nearest daughter entry by brute force; ties go to the lower index. */
+static real nearest_daughter(const oscene_t* s, uint32_t cur, v3 p, v3 d, real tmin, uint32_t* which, int* axis) {
+    const v3 inv = mk(RF(1.0) / d.x, RF(1.0) / d.y, RF(1.0) / d.z);
+    real best = R_INF;
+    *which = INVALID;
+    for (uint32_t j = 0; j < s->n_vol; ++j) {
+        if (s->vol[j].parent != cur) {
+            continue;
+        }
+        int a = 0;
+        const real t = box_entry(&s->vol[j], p, inv, tmin, &a);
+        if (t < best || (t == best && t < R_INF && j < *which)) {
+            best = t;
+            *which = j;
+            *axis = a;
+        }
+    }
+    return best;
+}
+
+static uint32_t point_neighbor(const oscene_t* s, uint32_t cur, v3 q) {
+    const uint32_t parent = s->vol[cur].parent;
+    if (parent == INVALID) {
+        return INVALID;
+    }
+    for (uint32_t j = 0; j < s->n_vol; ++j) {
+        if (j != cur && s->vol[j].parent == parent && box_contains(&s->vol[j], q)) {
+            return j;
+        }
+    }
+    return parent;
+}
+
+/* ------------------------------------------------------------------ */
+/* Surface sampling (OptiXTransportDevice.cu:492-595)                  */
+/* ------------------------------------------------------------------ */
+static v3 or_facet(const surface_t* sf, v3 n, v3 incoming, const rng_t* r, const photon_t* ph) {
+    n = or_normalize(n);
+    uint32_t w[4];
+    if (sf->model == 0) {
+        const real polish = clamp01(sf->model_value);
+        if (polish >= RF(1.0)) {
+            return n;
+        }
+        for (uint32_t a = 0; a < 128u; ++a) {
+            block4(r, ph->photon, ph->event, ph->step, P_FACET, a, w);
+            const v3 smear = mk(RF(2.0) * u01(w[0]) - RF(1.0), RF(2.0) * u01(w[1]) - RF(1.0), RF(2.0) * u01(w[2]) - RF(1.0));
+            if (vdot(smear, smear) > RF(1.0)) {
+                continue;
+            }
+            const v3 facet = or_normalize(vadd(n, vscale(smear, RF(1.0) - polish)));
+            if (vdot(incoming, facet) < RF(0.0)) {
+                return facet;
+            }
+        }
+        return n;
+    }
+    if (sf->model != 1) {
+        return n;
+    }
+    const real sigma = R_FMAX(sf->model_value, RF(0.0));
+    if (!(sigma > RF(0.0))) {
+        return n;
+    }
+    const v3 helper = R_FABS(n.z) < RF(0.9) ? mk(RF(0.0), RF(0.0), RF(1.0)) : mk(RF(1.0), RF(0.0), RF(0.0));
+    const v3 t = or_normalize(vcross(n, helper));
+    const v3 b = or_normalize(vcross(n, t));
+    const real s2 = sigma * sigma;
+    for (uint32_t a = 0; a < 256u; ++a) {
+        block4(r, ph->photon, ph->event, ph->step, P_FACET, a, w);
+        const real quarter = RF(0.25) * u01(w[0]);
+        const real alpha = RF(1.57079632679489662) * u01(w[0]);
+        real sa, ca;
+        r_sincos_turn(quarter, &sa, &ca);
+        const real g = r_exp(RF(-0.5) * alpha * alpha / s2);
+        if (u01(w[1]) > g * sa) {
+            continue;
+        }
+        real sp, cp;
+        r_sincos_turn(u01(w[2]), &sp, &cp);
+        const v3 facet = or_normalize(vadd(vscale(n, ca), vadd(vscale(t, sa * cp), vscale(b, sa * sp))));
+        if (vdot(incoming, facet) < RF(0.0)) {
+            return facet;
+        }
+    }
+    return n;
+}
+
+static v3 or_unified(const surface_t* sf, v3 d, v3 n, const rng_t* r, const photon_t* ph) {
+    const int ground = sf->finish == 1;
+    const v3 facet = or_facet(sf, n, d, r, ph);
+    const real spike = clamp01(or_prop(&sf->spike, ph->energy, ground ? RF(0.0) : RF(1.0)));
+    const real lobe = clamp01(or_prop(&sf->lobe, ph->energy, RF(0.0)));
+    const real back = clamp01(or_prop(&sf->back, ph->energy, RF(0.0)));
+    uint32_t w[4];
+    block4(r, ph->photon, ph->event, ph->step, P_LOBE, 0, w);
+    const real u = u01(w[0]);
+    if (u < spike) {
+        return or_reflect_dir(d, n);
+    }
+    if (u < spike + lobe) {
+        return or_reflect_dir(d, facet);
+    }
+    if (u < spike + lobe + back) {
+        return vscale(d, RF(-1.0));
+    }
+    return or_lambertian(n, u01(w[1]), u01(w[2]));
+}
+
+static v3 or_surface_reflection(const surface_t* sf, v3 d, v3 n, const rng_t* r, const photon_t* ph) {
+    if (sf->finish == 0) {
+        return or_reflect_dir(d, n);
+    }
+    if (sf->model == 0) {
+        return or_reflect_dir(d, or_facet(sf, n, d, r, ph));
+    }
+    return or_unified(sf, d, n, r, ph);
+}
+
+/* Geant4 G4OpRayleigh dipole sampling, one block per attempt. */
+static void or_rayleigh(photon_t* ph, const rng_t* r) {
+    const v3 od = ph->dir, op = ph->pol;
+    uint32_t w[4];
+    for (uint32_t a = 0; a < 64u; ++a) {
+        block4(r, ph->photon, ph->event, ph->step, P_RAYLEIGH, a, w);
+        real ct = u01(w[0]);
+        const real st = R_SQRT(R_FMAX(RF(0.0), RF(1.0) - ct * ct));
+        if (w[1] & 0x80000000u) {
+            ct = -ct;
+        }
+        real sp, cp;
+        r_sincos_turn(u01(w[2]), &sp, &cp);
+        const v3 nd = or_rotate_uz(mk(st * cp, st * sp, ct), od);
+        v3 np = vsub(op, vscale(nd, vdot(nd, op)));
+        const real l2 = vdot(np, np);
+        if (!(l2 > RF(0.0))) {
+            real sq, cq;
+            r_sincos_turn((real)(w[1] & 0x00ffffffu) * RF(0x1p-24), &sq, &cq);
+            np = or_rotate_uz(mk(cq, sq, RF(0.0)), nd);
+        } else {
+            np = vscale(np, RF(1.0) / R_SQRT(l2));
+            if (w[1] & 0x40000000u) {
+                np = vscale(np, RF(-1.0));
+            }
+        }
+        const real c = vdot(np, op);
+        ph->dir = nd;
+        ph->pol = np;
+        if (!(c * c < u01(w[3]))) {
+            return;
+        }
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* Photon generation                                                   */
+/* ------------------------------------------------------------------ */
+static void isotropic(real ct, real sp, real cp, real spsi, real cpsi, v3* dir, v3* pol) {
+    const real st = R_SQRT(R_FMAX(RF(0.0), RF(1.0) - ct * ct));
+    *dir = mk(st * cp, st * sp, ct);
+    const v3 et = mk(ct * cp, ct * sp, -st);
+    const v3 ep = mk(-sp, cp, RF(0.0));
+    *pol = vadd(vscale(et, cpsi), vscale(ep, spsi));
+}
+
+static void gen_synthetic(photon_t* ph, const g4go_synthetic_source* src, uint32_t volume, const rng_t* r,
+                          uint32_t event, uint32_t photon) {
+    memset(ph, 0, sizeof(*ph));
+    ph->photon = photon;
+    ph->event = event;
+    ph->volume = volume;
+    ph->material = INVALID;
+    uint32_t a[4], b[4], c[4];
+    block4(r, photon, event, 0, P_SYNTH, 0, a);
+    block4(r, photon, event, 0, P_SYNTH, 1, b);
+    const real ct = RF(1.0) - RF(2.0) * u01(a[0]);
+    real sp, cp, spsi, cpsi, sg, cg;
+    r_sincos_turn(u01(a[1]), &sp, &cp);
+    r_sincos_turn(u01(a[2]), &spsi, &cpsi);
+    isotropic(ct, sp, cp, spsi, cpsi, &ph->dir, &ph->pol);
+    const real radius = R_SQRT(RF(-2.0) * r_log(u01(a[3])));
+    r_sincos_turn(u01(b[0]), &sg, &cg);
+    const real wl = (real)src->wavelength_mean_nm + (real)src->wavelength_sigma_nm * (radius * cg);
+    ph->energy = RF(1239.84198) / wl;
+    ph->pos = mk((real)src->lo_mm[0] + ((real)src->hi_mm[0] - (real)src->lo_mm[0]) * u01(b[1]),
+                 (real)src->lo_mm[1] + ((real)src->hi_mm[1] - (real)src->lo_mm[1]) * u01(b[2]),
+                 (real)src->lo_mm[2] + ((real)src->hi_mm[2] - (real)src->lo_mm[2]) * u01(b[3]));
+    block4(r, photon, event, 0, P_SYNTH, 2, c);
+    ph->time = (double)(-(real)src->decay_time_ns * r_log(u01(c[0])));
+}
+
+static void gen_emission(photon_t* ph, const g4go_emission* em, uint32_t volume, const oscene_t* s, const rng_t* r,
+                         uint32_t local) {
+    memset(ph, 0, sizeof(*ph));
+    ph->photon = em->first_photon_id + local;
+    ph->event = em->event_id;
+    ph->volume = volume;
+    ph->material = INVALID;
+    ph->pos = mk(em->position_mm[0], em->position_mm[1], em->position_mm[2]);
+    ph->time = (double)em->time_ns;
+    ph->dir = or_normalize(mk(em->direction[0], em->direction[1], em->direction[2]));
+    ph->pol = mk(em->polarization[0], em->polarization[1], em->polarization[2]);
+    ph->energy = em->energy_ev;
+    if (em->type == 0) {
+        return;
+    }
+    if (em->material_id >= s->n_mat) {
+        ph->energy = NAN;
+        return;
+    }
+    const material_t* m = &s->mat[em->material_id];
+    uint32_t w0[4], w1[4], w[4];
+    real frac = RF(1.0);
+    if (em->type == 1) {
+        const real beta = ((real)em->pre_velocity_mm_per_ns + RF(0.5) * (real)em->delta_velocity_mm_per_ns) / RF(299.792458);
+        if (!(beta > RF(0.0)) || !(beta < RF(1.0)) || m->rindex.n == 0) {
+            ph->energy = NAN;
+            return;
+        }
+        const real emin = m->rindex.e[0], emax = m->rindex.e[m->rindex.n - 1];
+        const real nmax = m->rindex_max;
+        const real binv = RF(1.0) / beta;
+        const real maxcos = binv / nmax;
+        const real maxsin2 = RF(1.0) - maxcos * maxcos;
+        const real de = emax - emin;
+        if (!(nmax > RF(0.0)) || !finite_r(nmax) || !(maxcos > RF(0.0)) || !(maxcos < RF(1.0)) || !(de >= RF(0.0)) ||
+            !finite_r(de) || !(maxsin2 > RF(0.0))) {
+            ph->energy = NAN;
+            return;
+        }
+        real energy = emin, ct = RF(0.0), s2 = RF(0.0);
+        int ok = 0;
+        for (uint32_t a = 0; a < 1024u; ++a) {
+            block4(r, ph->photon, ph->event, 0, P_EMISSION, 16u + a, w);
+            energy = emin + u01(w[0]) * de;
+            const real n = or_prop(&m->rindex, energy, NAN);
+            ct = binv / n;
+            s2 = RF(1.0) - ct * ct;
+            if (finite_r(n) && ct > RF(0.0) && ct < RF(1.0) && s2 > RF(0.0) && u01(w[1]) * maxsin2 <= s2) {
+                ok = 1;
+                break;
+            }
+        }
+        if (!ok) {
+            ph->energy = NAN;
+            return;
+        }
+        block4(r, ph->photon, ph->event, 0, P_EMISSION, 0, w0);
+        real sp, cp;
+        r_sincos_turn(u01(w0[2]), &sp, &cp);
+        const real st = R_SQRT(R_FMAX(s2, RF(0.0)));
+        const v3 axis = or_normalize(mk(em->direction[0], em->direction[1], em->direction[2]));
+        ph->dir = or_normalize(or_rotate_uz(mk(st * cp, st * sp, ct), axis));
+        ph->pol = or_normalize(or_rotate_uz(mk(ct * cp, ct * sp, -st), axis));
+        const real maxmean = R_FMAX((real)em->pre_mean_photon_count, (real)em->post_mean_photon_count);
+        const real dmean = (real)em->pre_mean_photon_count - (real)em->post_mean_photon_count;
+        for (uint32_t a = 0; a < 1024u; ++a) {
+            block4(r, ph->photon, ph->event, 0, P_EMISSION, 2048u + a, w);
+            frac = u01(w[0]);
+            if (!(u01(w[1]) * maxmean > (real)em->pre_mean_photon_count - frac * dmean)) {
+                break;
+            }
+        }
+        ph->energy = energy;
+    } else if (em->type == 2) {
+        if (em->spectrum_id >= 3u) {
+            ph->energy = NAN;
+            return;
+        }
+        block4(r, ph->photon, ph->event, 0, P_EMISSION, 0, w0);
+        block4(r, ph->photon, ph->event, 0, P_EMISSION, 1, w1);
+        ph->energy = or_spectrum(&m->spectrum[em->spectrum_id], u01(w0[0]));
+        const real ct = RF(1.0) - RF(2.0) * u01(w0[1]);
+        real sp, cp, spsi, cpsi;
+        r_sincos_turn(u01(w0[2]), &sp, &cp);
+        r_sincos_turn(u01(w0[3]), &spsi, &cpsi);
+        isotropic(ct, sp, cp, spsi, cpsi, &ph->dir, &ph->pol);
+        frac = em->charge == 0.0f ? RF(1.0) : u01(w1[0]);
+        const real decay = em->decay_time_ns;
+        if (decay > RF(0.0) && finite_r(decay)) {
+            if (!(em->rise_time_ns > 0.0f) || !isfinite(em->rise_time_ns)) {
+                ph->time += (double)(-decay * r_log(R_FMAX(u01(w1[1]), RF(1.0e-7))));
+            } else {
+                real sample = RF(0.0);
+                for (uint32_t a = 0; a < 1024u; ++a) {
+                    block4(r, ph->photon, ph->event, 0, P_EMISSION, 2u + a, w);
+                    sample = -decay * r_log(R_FMAX(RF(1.0e-7), RF(1.0) - u01(w[0])));
+                    if (!(u01(w[1]) > RF(1.0) - r_exp(-sample / (real)em->rise_time_ns))) {
+                        break;
+                    }
+                }
+                ph->time += (double)sample;
+            }
+        }
+    } else {
+        ph->energy = NAN;
+        return;
+    }
+    ph->pos = vadd(mk(em->position_mm[0], em->position_mm[1], em->position_mm[2]),
+                   vscale(mk(em->step_delta_mm[0], em->step_delta_mm[1], em->step_delta_mm[2]), frac));
+    const real vel = (real)em->pre_velocity_mm_per_ns + RF(0.5) * frac * (real)em->delta_velocity_mm_per_ns;
+    if (vel > RF(0.0) && finite_r(vel)) {
+        ph->time += (double)(frac * (real)em->step_length_mm / vel);
+    }
+}
+
+static int start_photon(photon_t* ph) {
+    const real l2 = vdot(ph->dir, ph->dir);
+    if (!(ph->energy > RF(0.0)) || !finite_r(ph->energy) || !(l2 > RF(0.0)) || !finite_r(l2)) {
+        return 0;
+    }
+    ph->dir = or_normalize(ph->dir);
+    ph->pol = or_project(ph->pol, ph->dir);
+    return 1;
+}
+
+/* ------------------------------------------------------------------ */
+/* One step of the bounce loop                                         */
+/* ------------------------------------------------------------------ */
+enum { S_ALIVE = 0, S_DETECTED, S_ABS_BULK, S_ABS_SURFACE, S_ESCAPED, S_INVALID };
+
+/* Per-step record for the likelihood-ratio replay. */
+typedef struct {
+    real segment;
+    uint32_t material, surface;
+    int event; /* 0 none, 1 Rayleigh, 2 interaction, 3 direct transmit, 4 detect */
+} trace_t;
+
+static int or_step(photon_t* ph, const oscene_t* s, const rng_t* r, real eps, int* zero, uint32_t* sensor, trace_t* tr) {
+    *zero = 0;
+    if (ph->volume >= s->n_vol) {
+        return S_INVALID;
+    }
+    const volume_t* vol = &s->vol[ph->volume];
+    if (vol->material != ph->material && !cache_material(ph, s, vol->material)) {
+        return S_INVALID;
+    }
+    tr->segment = RF(0.0);
+    tr->material = ph->material;
+    tr->surface = INVALID;
+    tr->event = 0;
+    real dist;
+    int axis;
+    box_exit(vol, ph->pos, ph->dir, &dist, &axis);
+    uint32_t next = INVALID, surface = INVALID;
+    int entering = 0;
+    {
+        uint32_t which;
+        int a2 = 0;
+        const real t = nearest_daughter(s, ph->volume, ph->pos, ph->dir, eps, &which, &a2);
+        if (which != INVALID && t <= dist) {
+            dist = t;
+            axis = a2;
+            next = which;
+            surface = or_find_surface(s, ph->volume, which);
+            entering = 1;
+        }
+    }
+    if (!finite_r(dist)) {
+        return S_ESCAPED;
+    }
+    if (dist <= eps) {
+        *zero = 1;
+    }
+    uint32_t w[4];
+    block4(r, ph->photon, ph->event, ph->step, P_STEP, 0, w);
+    if (finite_r(ph->ilen)) {
+        const real dint = -r_log(u01(w[0])) * ph->ilen;
+        if (dint < dist) {
+            ph->time += (double)(dint * ph->inv_v);
+            tr->segment = dint;
+            if (u01(w[1]) <= ph->afrac) {
+                return S_ABS_BULK;
+            }
+            ph->pos = vadd(ph->pos, vscale(ph->dir, dint));
+            or_rayleigh(ph, r);
+            tr->event = 1;
+            return S_ALIVE;
+        }
+    }
+    ph->time += (double)(dist * ph->inv_v);
+    tr->segment = dist;
+    v3 bpos = vadd(ph->pos, vscale(ph->dir, dist));
+    const real dsign = vaxis(ph->dir, axis) > RF(0.0) ? RF(1.0) : RF(-1.0);
+    {
+        const volume_t* fb = entering ? &s->vol[next] : vol;
+        const real plane = entering ? (dsign > RF(0.0) ? (real)fb->lo[axis] : (real)fb->hi[axis])
+                                    : (dsign > RF(0.0) ? (real)fb->hi[axis] : (real)fb->lo[axis]);
+        if (axis == 0) {
+            bpos.x = plane;
+        } else if (axis == 1) {
+            bpos.y = plane;
+        } else {
+            bpos.z = plane;
+        }
+    }
+    ph->pos = bpos;
+    const v3 normal = axis == 0 ? mk(-dsign, RF(0.0), RF(0.0)) : (axis == 1 ? mk(RF(0.0), -dsign, RF(0.0)) : mk(RF(0.0), RF(0.0), -dsign));
+    if (!entering) {
+        const v3 q = vadd(bpos, vscale(normal, -eps));
+        next = point_neighbor(s, ph->volume, q);
+        if (next == INVALID) {
+            return ph->volume == s->world ? S_ESCAPED : S_INVALID;
+        }
+        surface = or_find_surface(s, ph->volume, next);
+    }
+    const volume_t* nvol = &s->vol[next];
+    if (nvol->material >= s->n_mat) {
+        return S_INVALID;
+    }
+    const material_t* nmat = &s->mat[nvol->material];
+    const real n1 = ph->rindex;
+    const real e = ph->energy;
+    int reflected = 0, detected = 0, absorbed = 0, transmitted = 0;
+    v3 dir = ph->dir, pol = ph->pol;
+    if (surface < s->n_surf) {
+        const surface_t* sf = &s->surf[surface];
+        tr->surface = surface;
+        const real refl = clamp01(or_prop(&sf->refl, e, RF(0.0)));
+        const real eff = clamp01(or_prop(&sf->eff, e, RF(0.0)));
+        const real trans = clamp01(or_prop(&sf->trans, e, RF(0.0)));
+        const int outcome = ORACLE_NAME(classify)(refl, trans, sf->refl.n != 0, sf->trans.n != 0, u01(w[2]));
+        tr->event = outcome == O_INTERACT ? 2 : (outcome == O_TRANSMIT ? 3 : 0);
+        if (outcome == O_ABSORB) {
+            absorbed = 1;
+            detected = ORACLE_NAME(absorption)(eff, nvol->sensor != INVALID || vol->sensor != INVALID, u01(w[3])) == O_DETECT;
+        } else if (outcome == O_TRANSMIT) {
+            transmitted = 1;
+        } else if (sf->type == 0) {
+            const real n2 = or_prop(&nmat->rindex, e, NAN);
+            if (!(n1 > RF(0.0)) || !finite_r(n1) || !(n2 > RF(0.0)) || !finite_r(n2)) {
+                absorbed = 1;
+            } else {
+                const v3 inormal = sf->finish == 0 ? or_normalize(normal) : or_facet(sf, normal, dir, r, ph);
+                const real rough = or_prop(&sf->rough, e, RF(0.0));
+                int rough_pass = 1;
+                uint32_t wr[4] = {0, 0, 0, 0};
+                if (rough > RF(0.0) && n1 > n2 && finite_r(rough) && finite_r(e) && e > RF(0.0)) {
+                    const real cost1 = R_FMAX(RF(0.0), -vdot(dir, inormal));
+                    const real wl = RF(1.239841984e-3) / e;
+                    const real expo = RF(4.0) * RF(3.14159265358979323846) * rough * n1 * cost1 / wl;
+                    block4(r, ph->photon, ph->event, ph->step, P_ROUGH, 0, wr);
+                    rough_pass = u01(wr[0]) < r_exp(-expo * expo);
+                }
+                const fresnel_t f = or_fresnel(dir, pol, inormal, n1, n2);
+                reflected = f.tir || u01(w[3]) > f.t;
+                if (reflected) {
+                    if (rough_pass) {
+                        if (sf->model == 1 && sf->finish != 0) {
+                            dir = or_unified(sf, dir, normal, r, ph);
+                            pol = or_reflect_pol(pol, normal, dir);
+                        } else {
+                            dir = f.rdir;
+                            pol = f.rpol;
+                        }
+                    } else {
+                        dir = or_lambertian(normal, u01(wr[1]), u01(wr[2]));
+                        pol = or_project(pol, dir);
+                    }
+                } else {
+                    dir = f.tdir;
+                    pol = f.tpol;
+                    transmitted = 1;
+                }
+            }
+        } else {
+            reflected = 1;
+            dir = or_surface_reflection(sf, dir, normal, r, ph);
+            pol = or_reflect_pol(pol, normal, dir);
+        }
+    } else {
+        const real n2 = or_prop(&nmat->rindex, e, NAN);
+        if (!(n1 > RF(0.0)) || !finite_r(n1) || !(n2 > RF(0.0)) || !finite_r(n2)) {
+            absorbed = 1;
+        } else {
+            const fresnel_t f = or_fresnel(dir, pol, normal, n1, n2);
+            reflected = u01(w[3]) > f.t;
+            if (reflected) {
+                dir = f.rdir;
+                pol = f.rpol;
+            } else {
+                dir = f.tdir;
+                pol = f.tpol;
+                transmitted = 1;
+            }
+        }
+    }
+    (void)reflected;
+    if (detected) {
+        *sensor = nvol->sensor != INVALID ? nvol->sensor : vol->sensor;
+        if (*sensor == INVALID) {
+            return S_INVALID;
+        }
+        tr->event = 4;
+        return S_DETECTED;
+    }
+    if (absorbed) {
+        return S_ABS_SURFACE;
+    }
+    ph->dir = dir;
+    ph->pol = pol;
+    if (transmitted) {
+        ph->volume = next;
+    }
+    return S_ALIVE;
+}
+
+/* ------------------------------------------------------------------ */
+/* Batch driver                                                        */
+/* ------------------------------------------------------------------ */
+typedef struct {
+    uint64_t detected, absorbed, escaped, truncated, max_bounce, invalid, zero_step, steps, generated;
+} ostats_t;
+
+static int hist_slot(double t) {
+    const double x = (t - 0.0) / (200.0 / 256.0);
+    if (!(x >= 0.0)) {
+        return 256;
+    }
+    if (x >= 256.0) {
+        return 257;
+    }
+    return (int)x;
+}
+
+/* Transports one photon to termination; returns the final status (or -1 for
+ * truncation) and fills the hit on detection. */
+static int transport_one(photon_t* ph, const oscene_t* s, const rng_t* r, real eps, uint32_t max_steps,
+                         ostats_t* st, g4go_hit* hit) {
+    st->generated++;
+    if (!start_photon(ph)) {
+        st->invalid++;
+        return S_INVALID;
+    }
+    for (;;) {
+        int zero = 0;
+        uint32_t sensor = INVALID;
+        trace_t tr;
+        int status = or_step(ph, s, r, eps, &zero, &sensor, &tr);
+        ph->step++;
+        st->steps++;
+        st->zero_step += (uint64_t)zero;
+        if (status == S_ALIVE && ph->step >= max_steps) {
+            status = -1;
+        }
+        if (status == S_ALIVE) {
+            continue;
+        }
+        if (ph->step > st->max_bounce) {
+            st->max_bounce = ph->step;
+        }
+        if (status == S_DETECTED) {
+            st->detected++;
+            hit->position_mm[0] = (float)ph->pos.x;
+            hit->position_mm[1] = (float)ph->pos.y;
+            hit->position_mm[2] = (float)ph->pos.z;
+            hit->direction[0] = (float)ph->dir.x;
+            hit->direction[1] = (float)ph->dir.y;
+            hit->direction[2] = (float)ph->dir.z;
+            hit->energy_ev = (float)ph->energy;
+            hit->sensor_id = sensor;
+            hit->event_id = ph->event;
+            hit->photon_id = ph->photon;
+            hit->time_ns = ph->time;
+        } else if (status == S_ABS_BULK || status == S_ABS_SURFACE) {
+            st->absorbed++;
+        } else if (status == S_ESCAPED) {
+            st->escaped++;
+        } else if (status == S_INVALID) {
+            st->invalid++;
+        } else {
+            st->truncated++;
+        }
+        return status;
+    }
+}
+
+/* Synthetic batch.  events: n_events (event_id, photon_count) pairs.
+ * Hits are written in (event, photon) order; stats per event (9 u64 each:
+ * detected, absorbed, escaped, truncated, max_bounce, invalid, zero_step,
+ * steps, generated); hist: 258 u64 slots.  Returns the hit count, or -1 when
+ * hit_capacity is too small. */
+int64_t ORACLE_NAME(run_synthetic)(const g4go_scene* scene, const g4go_synthetic_source* src, uint64_t seed,
+                                   float eps, uint32_t max_steps, const uint32_t* event_ids,
+                                   const uint32_t* photon_counts, uint32_t n_events, uint32_t photon_begin,
+                                   uint32_t photon_end, g4go_hit* hits, uint64_t hit_capacity, uint64_t* stats,
+                                   uint64_t* hist) {
+    oscene_t* s = build(scene);
+    const uint32_t volume = vol_index(s, src->volume_id);
+    const rng_t r = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    uint64_t n_hits = 0;
+    for (uint32_t e = 0; e < n_events; ++e) {
+        ostats_t st;
+        memset(&st, 0, sizeof(st));
+        const uint32_t end = photon_counts[e] < photon_end ? photon_counts[e] : photon_end;
+        for (uint32_t k = photon_begin; k < end; ++k) {
+            photon_t ph;
+            gen_synthetic(&ph, src, volume, &r, event_ids[e], k);
+            g4go_hit hit;
+            const int status = transport_one(&ph, s, &r, (real)eps, max_steps, &st, &hit);
+            if (status == S_DETECTED) {
+                if (n_hits >= hit_capacity) {
+                    destroy(s);
+                    return -1;
+                }
+                hits[n_hits++] = hit;
+                hist[hist_slot(hit.time_ns)]++;
+            }
+        }
+        uint64_t* o = stats + 9 * (uint64_t)e;
+        o[0] += st.detected;
+        o[1] += st.absorbed;
+        o[2] += st.escaped;
+        o[3] += st.truncated;
+        o[4] = st.max_bounce > o[4] ? st.max_bounce : o[4];
+        o[5] += st.invalid;
+        o[6] += st.zero_step;
+        o[7] += st.steps;
+        o[8] += st.generated;
+    }
+    destroy(s);
+    return (int64_t)n_hits;
+}
+
+/* Emission batch (reference semantics of EnqueueEmissions + __raygen__rg):
+ * emission_event_indices map each record to an event slot of `stats`. */
+int64_t ORACLE_NAME(run_emissions)(const g4go_scene* scene, const g4go_emission* emissions, uint64_t n_emissions,
+                                   const uint32_t* emission_event_indices, uint32_t n_events, uint64_t seed,
+                                   float eps, uint32_t max_steps, g4go_hit* hits, uint64_t hit_capacity,
+                                   uint64_t* stats, uint64_t* hist) {
+    oscene_t* s = build(scene);
+    const rng_t r = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    uint64_t n_hits = 0;
+    for (uint64_t i = 0; i < n_emissions; ++i) {
+        const g4go_emission* em = &emissions[i];
+        const uint32_t ev = emission_event_indices[i];
+        if (ev >= n_events) {
+            destroy(s);
+            return -2;
+        }
+        const uint32_t volume = vol_index(s, em->volume_id);
+        ostats_t st;
+        memset(&st, 0, sizeof(st));
+        for (uint32_t k = 0; k < em->photon_count; ++k) {
+            photon_t ph;
+            gen_emission(&ph, em, volume, s, &r, k);
+            g4go_hit hit;
+            const int status = transport_one(&ph, s, &r, (real)eps, max_steps, &st, &hit);
+            if (status == S_DETECTED) {
+                if (n_hits >= hit_capacity) {
+                    destroy(s);
+                    return -1;
+                }
+                hits[n_hits++] = hit;
+                hist[hist_slot(hit.time_ns)]++;
+            }
+        }
+        uint64_t* o = stats + 9 * (uint64_t)ev;
+        o[0] += st.detected;
+        o[1] += st.absorbed;
+        o[2] += st.escaped;
+        o[3] += st.truncated;
+        o[4] = st.max_bounce > o[4] ? st.max_bounce : o[4];
+        o[5] += st.invalid;
+        o[6] += st.zero_step;
+        o[7] += st.steps;
+        o[8] += st.generated;
+    }
+    destroy(s);
+    return (int64_t)n_hits;
+}

@@ -1,0 +1,214 @@
+"""ctypes wrapper of the oracle libraries (see oracle.c for what each follows).
+
+Test infrastructure only: used as the checker by tests/, the smoke check and
+the CPU-baseline leg of bench.py.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from concurrent.futures import ThreadPoolExecutor
+from pathlib import Path
+
+import numpy as np
+
+from g4go_amd import abi
+
+ORACLE_DIR = Path(__file__).resolve().parent
+LIB32 = ORACLE_DIR / "_build" / "liboracle32.so"
+LIB64 = ORACLE_DIR / "_build" / "liboracle64.so"
+LIBREF = ORACLE_DIR / "_ref" / "libref_boundary.so"
+
+_libs: dict = {}
+
+
+def _load(precision: int) -> C.CDLL:
+    if precision in _libs:
+        return _libs[precision]
+    path = LIB32 if precision == 32 else LIB64
+    if not path.is_file():
+        raise RuntimeError(f"{path} is missing: build it with `python -m oracle.build`")
+    lib = C.CDLL(str(path))
+    p = f"oracle{precision}_"
+    real = C.c_float if precision == 32 else C.c_double
+    getattr(lib, p + "philox").argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+    getattr(lib, p + "uniform").argtypes = [C.c_uint64, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32]
+    getattr(lib, p + "uniform").restype = C.c_float
+    getattr(lib, p + "log").argtypes = [real]
+    getattr(lib, p + "log").restype = real
+    getattr(lib, p + "exp").argtypes = [real]
+    getattr(lib, p + "exp").restype = real
+    getattr(lib, p + "sincos_turn").argtypes = [real, C.POINTER(real), C.POINTER(real)]
+    getattr(lib, p + "classify").argtypes = [real, real, C.c_int, C.c_int, real]
+    getattr(lib, p + "absorption").argtypes = [real, C.c_int, real]
+    getattr(lib, p + "fresnel").argtypes = [C.c_void_p, C.c_void_p]
+    getattr(lib, p + "lambertian").argtypes = [C.c_void_p, real, real, C.c_void_p]
+    getattr(lib, p + "find_surface").argtypes = [C.POINTER(abi.Scene), C.c_uint32, C.c_uint32]
+    getattr(lib, p + "find_surface").restype = C.c_uint32
+    getattr(lib, p + "run_synthetic").argtypes = [
+        C.POINTER(abi.Scene), C.POINTER(abi.SyntheticSource), C.c_uint64, C.c_float, C.c_uint32, C.c_void_p,
+        C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p]
+    getattr(lib, p + "run_synthetic").restype = C.c_int64
+    getattr(lib, p + "run_emissions").argtypes = [
+        C.POINTER(abi.Scene), C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint32, C.c_uint64, C.c_float, C.c_uint32,
+        C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p]
+    getattr(lib, p + "run_emissions").restype = C.c_int64
+    _libs[precision] = lib
+    return lib
+
+
+def _fn(precision: int, name: str):
+    return getattr(_load(precision), f"oracle{precision}_{name}")
+
+
+def _real(precision: int):
+    return np.float32 if precision == 32 else np.float64
+
+
+# ---- elementary pieces ----
+def philox(ctr, key) -> np.ndarray:
+    c = np.ascontiguousarray(ctr, dtype=np.uint32)
+    k = np.ascontiguousarray(key, dtype=np.uint32)
+    out = np.zeros(4, np.uint32)
+    _fn(32, "philox")(c.ctypes.data, k.ctypes.data, out.ctypes.data)
+    return out
+
+
+def uniform(seed: int, photon_key: int, bounce: int, process: int, draw: int) -> float:
+    return _fn(32, "uniform")(seed, photon_key, bounce, process, draw)
+
+
+def log(x: float, precision: int = 32) -> float:
+    return _fn(precision, "log")(x)
+
+
+def exp(x: float, precision: int = 32) -> float:
+    return _fn(precision, "exp")(x)
+
+
+def sincos_turn(u: float, precision: int = 32) -> tuple[float, float]:
+    real = C.c_float if precision == 32 else C.c_double
+    s, c = real(), real()
+    _fn(precision, "sincos_turn")(u, C.byref(s), C.byref(c))
+    return s.value, c.value
+
+
+def classify(r, t, has_r, has_t, sample, precision: int = 32) -> int:
+    return _fn(precision, "classify")(r, t, int(has_r), int(has_t), sample)
+
+
+def absorption(eff, sensor, sample, precision: int = 32) -> int:
+    return _fn(precision, "absorption")(eff, int(sensor), sample)
+
+
+def fresnel(cases: np.ndarray, precision: int = 32) -> np.ndarray:
+    real = _real(precision)
+    cases = np.ascontiguousarray(cases, dtype=real).reshape(-1, 11)
+    out = np.zeros((cases.shape[0], 14), dtype=real)
+    f = _fn(precision, "fresnel")
+    for i in range(cases.shape[0]):
+        f(cases[i].ctypes.data, out[i].ctypes.data)
+    return out
+
+
+def lambertian(normal, radial, azimuth, precision: int = 32) -> np.ndarray:
+    real = _real(precision)
+    n = np.ascontiguousarray(normal, dtype=real)
+    out = np.zeros(3, real)
+    _fn(precision, "lambertian")(n.ctypes.data, radial, azimuth, out.ctypes.data)
+    return out
+
+
+def find_surface(scene, from_id: int, to_id: int) -> int:
+    desc, keep = scene.to_abi()
+    r = _fn(32, "find_surface")(C.byref(desc), from_id, to_id)
+    del keep
+    return r
+
+
+# ---- whole-path runs ----
+class OracleResult:
+    def __init__(self, hits: np.ndarray, stats: np.ndarray, hist: np.ndarray) -> None:
+        self.hits = hits        # abi.HIT_DTYPE, (event, photon) order
+        self.stats = stats      # (n_events, 9): detected absorbed escaped truncated max_bounce invalid zero steps generated
+        self.hist = hist        # 258 slots
+
+    STAT_NAMES = ("detected", "absorbed", "escaped", "truncated", "max_bounce", "invalid", "zero_step", "steps",
+                  "generated")
+
+
+def run_synthetic(scene, source: abi.SyntheticSource, seed: int, event_ids, photon_counts, precision: int = 32,
+                  eps: float = 1e-4, max_steps: int = 4096, photon_begin: int = 0, photon_end: int = 0xFFFFFFFF,
+                  threads: int = 1) -> OracleResult:
+    """Transports the synthetic batch on the CPU; `threads` splits the photon range of each event."""
+    desc, keep = scene.to_abi()
+    ev = np.ascontiguousarray(event_ids, dtype=np.uint32)
+    counts = np.ascontiguousarray(photon_counts, dtype=np.uint32)
+    fn = _fn(precision, "run_synthetic")
+    total = int(np.minimum(counts, photon_end).astype(np.int64).sum())
+
+    def work(lo: int, hi: int):
+        cap = max(1, total)
+        hits = np.zeros(cap, dtype=abi.HIT_DTYPE)
+        stats = np.zeros((ev.size, 9), np.uint64)
+        hist = np.zeros(abi.HIST_SLOTS, np.uint64)
+        n = fn(C.byref(desc), C.byref(source), seed, eps, max_steps, ev.ctypes.data, counts.ctypes.data, ev.size, lo,
+               hi, hits.ctypes.data, cap, stats.ctypes.data, hist.ctypes.data)
+        if n < 0:
+            raise RuntimeError("oracle hit buffer overflow")
+        return hits[:n], stats, hist
+
+    end = min(int(counts.max()) if counts.size else 0, photon_end)
+    if threads <= 1 or end - photon_begin < 2 * threads:
+        parts = [work(photon_begin, end)]
+    else:
+        edges = np.linspace(photon_begin, end, threads + 1).astype(np.int64)
+        with ThreadPoolExecutor(threads) as pool:
+            parts = list(pool.map(lambda i: work(int(edges[i]), int(edges[i + 1])), range(threads)))
+    hits = np.concatenate([p[0] for p in parts])
+    hits = hits[np.lexsort((hits["photon_id"], hits["event_id"]))]
+    stats = np.zeros((ev.size, 9), np.uint64)
+    for p in parts:
+        stats[:, [0, 1, 2, 3, 5, 6, 7, 8]] += p[1][:, [0, 1, 2, 3, 5, 6, 7, 8]]
+        stats[:, 4] = np.maximum(stats[:, 4], p[1][:, 4])
+    hist = sum(p[2] for p in parts)
+    del keep
+    return OracleResult(hits, stats, hist)
+
+
+def run_emissions(scene, emissions: np.ndarray, event_ids, emission_event_indices, seed: int,
+                  precision: int = 32, eps: float = 1e-4, max_steps: int = 4096) -> OracleResult:
+    desc, keep = scene.to_abi()
+    em = np.ascontiguousarray(emissions, dtype=abi.EMISSION_DTYPE)
+    idx = np.ascontiguousarray(emission_event_indices, dtype=np.uint32)
+    n_events = len(event_ids)
+    cap = max(1, int(em["photon_count"].astype(np.int64).sum()))
+    hits = np.zeros(cap, dtype=abi.HIT_DTYPE)
+    stats = np.zeros((n_events, 9), np.uint64)
+    hist = np.zeros(abi.HIST_SLOTS, np.uint64)
+    n = _fn(precision, "run_emissions")(C.byref(desc), em.ctypes.data, em.size, idx.ctypes.data, n_events, seed, eps,
+                                        max_steps, hits.ctypes.data, cap, stats.ctypes.data, hist.ctypes.data)
+    if n < 0:
+        raise RuntimeError(f"oracle run failed ({n})")
+    hits = hits[:n]
+    hits = hits[np.lexsort((hits["photon_id"], hits["event_id"]))]
+    del keep
+    return OracleResult(hits, stats, hist)
+
+
+# ---- the reference's own boundary physics, compiled from its header ----
+def reference_library():
+    """libref_boundary.so or None when it has not been built."""
+    if "ref" in _libs:
+        return _libs["ref"]
+    if not LIBREF.is_file():
+        return None
+    lib = C.CDLL(str(LIBREF))
+    lib.ref_compute_fresnel.argtypes = [C.c_void_p, C.c_void_p]
+    lib.ref_project_polarization.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+    lib.ref_reflect_direction.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+    lib.ref_reflect_polarization.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    lib.ref_classify_surface.argtypes = [C.c_float, C.c_float, C.c_int, C.c_int, C.c_float]
+    lib.ref_evaluate_absorption.argtypes = [C.c_float, C.c_int, C.c_float]
+    lib.ref_sample_lambertian.argtypes = [C.c_void_p, C.c_float, C.c_float, C.c_void_p]
+    _libs["ref"] = lib
+    return lib

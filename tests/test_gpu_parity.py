@@ -1,0 +1,158 @@
+"""GPU parity: the HIP kernels against the FP32 oracle on the same Philox streams.
+
+Integer, index and record outputs are compared bit for bit: the detected-hit
+list (sorted by event and photon), per-event counters and the arrival-time
+histogram.  At full size (1e8 photons) the checks are size-independent
+invariants (every photon ends in exactly one category, histogram total =
+detections = hit count).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from g4go_amd import abi, workloads
+from g4go_amd.transport import HipTransportHost, LaunchForm, PhotonTransportConfig
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+STAT_FIELDS = ("detected", "absorbed", "escaped", "truncated", "max_bounce", "invalid", "zero_step")
+
+
+def log(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def host(seed=42, **kw):
+    return HipTransportHost(PhotonTransportConfig(seed=seed, sort_hits=True, **kw))
+
+
+def assert_same_hits(gpu_hits, ref_hits):
+    assert gpu_hits.shape == ref_hits.shape, (gpu_hits.shape, ref_hits.shape)
+    if gpu_hits.tobytes() != ref_hits.tobytes():
+        a = gpu_hits.view(np.uint8).reshape(-1, 48)
+        b = ref_hits.view(np.uint8).reshape(-1, 48)
+        bad = np.nonzero(np.any(a != b, axis=1))[0]
+        raise AssertionError(f"{bad.size} hit records differ; first {bad[:5]}: gpu {gpu_hits[bad[0]]} "
+                             f"oracle {ref_hits[bad[0]]}")
+
+
+def assert_same_stats(out, ref):
+    ev = out.event_statistics
+    for k, name in enumerate(("detected", "absorbed", "escaped", "truncated", "max_bounce", "invalid", "zero_step")):
+        np.testing.assert_array_equal(ev[name], ref.stats[:, k].astype(np.uint64), err_msg=name)
+    np.testing.assert_array_equal(out.histogram, ref.hist)
+    assert out.statistics["total_steps"] == int(ref.stats[:, 7].sum())
+    assert out.statistics["generated"] == int(ref.stats[:, 8].sum())
+
+
+def test_device_math_bit_exact():
+    log("case: device log/exp/sincos vs oracle")
+    rng = np.random.default_rng(1)
+    words = rng.integers(0, 2**32, 4000, dtype=np.uint64)
+    u = ((words.astype(np.float32) + np.float32(0.5)) / np.float32(4294967296.0)).astype(np.float32)
+    xs_log = np.concatenate([u, np.float32([1.0, 0.5, 2.0, 1e-30, 3e-39, 1e30, 123.456])]).astype(np.float32)
+    xs_exp = rng.uniform(-90, 90, 4000).astype(np.float32)
+    xs_sc = np.concatenate([u, np.float32([0.0, 0.125, 0.25, 0.375, 0.5, 0.75, 0.999999])]).astype(np.float32)
+    with host() as h:
+        got_log = h.eval_math(0, xs_log)
+        got_exp = h.eval_math(1, xs_exp)
+        got_sc = h.eval_math(2, xs_sc)
+    ref_log = np.array([O.log(float(x)) for x in xs_log], np.float32)
+    ref_exp = np.array([O.exp(float(x)) for x in xs_exp], np.float32)
+    ref_sc = np.array([O.sincos_turn(float(x)) for x in xs_sc], np.float32)
+    np.testing.assert_array_equal(got_log.view(np.uint32), ref_log.view(np.uint32))
+    np.testing.assert_array_equal(got_exp.view(np.uint32), ref_exp.view(np.uint32))
+    np.testing.assert_array_equal(got_sc.view(np.uint32), ref_sc.view(np.uint32))
+
+
+def random_fresnel_cases(n, seed=3):
+    rng = np.random.default_rng(seed)
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    nrm = rng.normal(size=(n, 3))
+    nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+    flip = np.sum(d * nrm, axis=1) > 0
+    nrm[flip] *= -1
+    p = rng.normal(size=(n, 3))
+    n1 = rng.uniform(1.0, 2.0, n)
+    n2 = rng.uniform(1.0, 2.0, n)
+    cases = np.concatenate([d, p, nrm, n1[:, None], n2[:, None]], axis=1).astype(np.float32)
+    # axis-aligned normals and normal incidence
+    cases[:8, 6:9] = [0, 0, -1]
+    cases[:4, 0:3] = [0, 0, 1]
+    return cases
+
+
+def test_device_fresnel_bit_exact():
+    log("case: device Fresnel vs oracle")
+    cases = random_fresnel_cases(3000)
+    with host() as h:
+        got = h.eval_fresnel(cases)
+    ref = O.fresnel(cases)
+    np.testing.assert_array_equal(got.view(np.uint32), ref.view(np.uint32))
+
+
+@pytest.mark.parametrize("form", ["plain", "persistent"])
+def test_synthetic_crystal_bit_exact(crystal, form):
+    log(f"case: synthetic crystal {form}")
+    src = workloads.synthetic_source(crystal)
+    event_ids = np.array([7, 3, 11], np.uint32)
+    counts = np.array([40000, 1000, 7], np.uint32)
+    ref = O.run_synthetic(crystal.scene, src, 42, event_ids, counts, threads=4)
+    with host(42) as h:
+        if form == "persistent":
+            h.set_launch(LaunchForm(persistent=True, block_size=256, chunk=256, refill_threshold=16, blocks_per_cu=2))
+        h.enqueue_synthetic(crystal.scene, src, event_ids, counts)
+        out = h.complete_oldest_batch()
+    assert out.statistics["dropped_hits"] == 0
+    assert_same_hits(out.detections, ref.hits)
+    assert_same_stats(out, ref)
+
+
+def test_emission_crystal_bit_exact(crystal):
+    log("case: scintillation emission records")
+    em, event_ids, idx = workloads.electron_emissions(crystal, 4, seed=5)
+    ref = O.run_emissions(crystal.scene, em, event_ids, idx, 77)
+    with host(77) as h:
+        h.enqueue_emissions(crystal.scene, em, event_ids, idx)
+        out = h.complete_oldest_batch()
+    assert_same_hits(out.detections, ref.hits)
+    assert_same_stats(out, ref)
+    np.testing.assert_array_equal(out.event_statistics["captured"],
+                                  np.bincount(idx, weights=em["photon_count"], minlength=4).astype(np.uint64))
+
+
+def test_array_open_bit_exact():
+    """8x8 array without septa: photons leave crystals and cross the world (BVH walk)."""
+    log("case: open 8x8 array (BVH)")
+    cs = workloads.array_scene(septa=False)
+    src = workloads.synthetic_source(cs, crystal_index=27)
+    ref = O.run_synthetic(cs.scene, src, 9, [0], [20000], threads=4)
+    with host(9) as h:
+        h.enqueue_synthetic(cs.scene, src, np.array([0], np.uint32), np.array([20000], np.uint32))
+        out = h.complete_oldest_batch()
+    assert out.statistics["escaped"] > 0
+    assert_same_hits(out.detections, ref.hits)
+    assert_same_stats(out, ref)
+
+
+def test_full_size_invariants(crystal):
+    """1e8 photons in one batch: conservation and histogram/hit consistency."""
+    log("case: 1e8 synthetic photons, invariants")
+    src = workloads.synthetic_source(crystal)
+    with host(2024, sort_hits=False) as h:
+        h.set_launch(LaunchForm(persistent=True, block_size=256, chunk=1024, refill_threshold=16, blocks_per_cu=2))
+        h.enqueue_synthetic(crystal.scene, src, np.array([0], np.uint32), np.array([100_000_000], np.uint32))
+        out, ptr = h.complete_device()
+    st = out.statistics
+    ev = out.event_statistics[0]
+    assert st["generated"] == 100_000_000
+    total = sum(int(ev[k]) for k in ("detected", "absorbed", "escaped", "truncated", "invalid"))
+    assert total == 100_000_000
+    assert st["n_hits"] == int(ev["detected"]) == int(out.histogram.sum())
+    assert st["dropped_hits"] == 0 and ptr != 0
+    frac = int(ev["detected"]) / 1e8
+    assert 0.2 < frac < 0.35, frac
