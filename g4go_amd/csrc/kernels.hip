@@ -376,7 +376,7 @@ __global__ void eval_fresnel_kernel(const float* in, float* out, int n) {
     const float* a = in + 11 * i;
     const Fresnel f = compute_fresnel(V3{a[0], a[1], a[2]}, V3{a[3], a[4], a[5]}, V3{a[6], a[7], a[8]}, a[9], a[10]);
     float* o = out + 14 * i;
-    o[0] = f.transmittance;
+    o[0] = static_cast<float>(f.transmittance);
     o[1] = f.tir ? 1.0f : 0.0f;
     const V3 vs[4] = {f.reflected_dir, f.transmitted_dir, f.reflected_pol, f.transmitted_pol};
     for (int k = 0; k < 4; ++k) {

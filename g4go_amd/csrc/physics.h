@@ -275,36 +275,68 @@ G4GO_HD V3 sample_lambertian(V3 n, float radial_sample, float azimuth_sample) {
 
 struct Fresnel {
     V3 reflected_dir, transmitted_dir, reflected_pol, transmitted_pol;
-    float transmittance;
+    double transmittance;
     bool tir;
 };
 
+struct D3 {
+    double x, y, z;
+};
+G4GO_HD D3 dvec(V3 a) { return {static_cast<double>(a.x), static_cast<double>(a.y), static_cast<double>(a.z)}; }
+G4GO_HD double ddot(D3 a, D3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+G4GO_HD D3 dcross(D3 a, D3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+G4GO_HD D3 dnormalize(D3 a) {
+    const double inv = 1.0 / sqrt(ddot(a, a));
+    return {a.x * inv, a.y * inv, a.z * inv};
+}
+
 // Fresnel equations with polarisation (OptiXBoundaryPhysics.cuh:122-178).
+// Directions and polarisations are FP32 with the reference's operation
+// order; the reflect/transmit probability and the total-internal-reflection
+// test are evaluated in FP64 from the same FP32 state (as Geant4's
+// G4OpBoundaryProcess does), which keeps them accurate near the critical
+// angle where 1 - sin^2(theta_t) cancels.
 G4GO_HD Fresnel compute_fresnel(V3 dir, V3 pol, V3 n, float n1, float n2) {
     dir = normalize(dir);
     n = normalize(n);
     pol = project_polarization(pol, dir);
+    const D3 dd = dvec(dir), dn = dvec(n), dp = dvec(pol);
+    const double n1d = n1, n2d = n2;
+    const double ci_d = fmin(fmax(-ddot(dd, dn), 0.0), 1.0);
+    const double eta_d = n1d / n2d;
+    const double st2_d = eta_d * eta_d * (1.0 - ci_d * ci_d);
     const float cos_i = clamp01(-dot(dir, n));
     const float eta = n1 / n2;
     const float sin_t2 = eta * eta * (1.0f - cos_i * cos_i);
     Fresnel out{};
     out.reflected_dir = reflect_direction(dir, n);
-    out.tir = sin_t2 >= 1.0f;
-    out.transmittance = 0.0f;
+    out.tir = st2_d >= 1.0;
+    out.transmittance = 0.0;
     if (out.tir) {
         out.reflected_pol = reflect_polarization(pol, n, out.reflected_dir);
         return out;
     }
+    const double ct_d = sqrt(fmax(0.0, 1.0 - st2_d));
     const float cos_t = sqrtf(fmaxf(0.0f, 1.0f - sin_t2));
     out.transmitted_dir = normalize(add(scale(dir, eta), scale(n, eta * cos_i - cos_t)));
     const V3 transverse = cross(dir, n);
     if (dot(transverse, transverse) < 1.0e-12f) {
-        const float ra = (n1 - n2) / (n1 + n2);
-        out.transmittance = clamp01(1.0f - ra * ra);
+        const double ra = (n1d - n2d) / (n1d + n2d);
+        out.transmittance = fmin(fmax(1.0 - ra * ra, 0.0), 1.0);
         out.reflected_pol = n2 > n1 ? scale(pol, -1.0f) : pol;
         out.transmitted_pol = pol;
         return out;
     }
+    // FP64 probability
+    {
+        const D3 sd = dnormalize(dcross(dd, dn));
+        const D3 pd = dnormalize(dcross(dd, sd));
+        const double es = ddot(dp, sd), ep = ddot(dp, pd);
+        const double rs = (n1d * ci_d - n2d * ct_d) / (n1d * ci_d + n2d * ct_d);
+        const double rp = (n2d * ci_d - n1d * ct_d) / (n2d * ci_d + n1d * ct_d);
+        out.transmittance = fmin(fmax(1.0 - (es * es * rs * rs + ep * ep * rp * rp), 0.0), 1.0);
+    }
+    // FP32 outgoing polarisations
     const V3 s_dir = normalize(transverse);
     const V3 p_dir = normalize(cross(dir, s_dir));
     const float in_s = dot(pol, s_dir);
@@ -315,8 +347,6 @@ G4GO_HD Fresnel compute_fresnel(V3 dir, V3 pol, V3 n, float n1, float n2) {
     const float rp = (n2 * cos_i - n1 * cos_t) / p_den;
     const float ts = 2.0f * n1 * cos_i / s_den;
     const float tp = 2.0f * n1 * cos_i / p_den;
-    const float refl = in_s * in_s * rs * rs + in_p * in_p * rp * rp;
-    out.transmittance = clamp01(1.0f - refl);
     const V3 rp_dir = normalize(cross(out.reflected_dir, s_dir));
     const V3 tp_dir = normalize(cross(out.transmitted_dir, s_dir));
     out.reflected_pol = normalize(add(scale(s_dir, in_s * rs), scale(rp_dir, in_p * rp)));

@@ -851,7 +851,7 @@ G4GO_HD int transport_step(Photon& ph, const SceneView& sc, const Rng& rng, cons
                     rough_pass = u01(wr.x) < g4_expf(-expo * expo);
                 }
                 const Fresnel f = compute_fresnel(dir, pol, inormal, n1, n2);
-                reflected = f.tir || u01(w.w) > f.transmittance;
+                reflected = f.tir || static_cast<double>(u01(w.w)) > f.transmittance;
                 if (reflected) {
                     if (rough_pass) {
                         if (s.model == kUnified && s.finish != kPolished) {
@@ -882,7 +882,7 @@ G4GO_HD int transport_step(Photon& ph, const SceneView& sc, const Rng& rng, cons
             absorbed = true;
         } else {
             const Fresnel f = compute_fresnel(dir, pol, normal, n1, n2);
-            reflected = u01(w.w) > f.transmittance;
+            reflected = static_cast<double>(u01(w.w)) > f.transmittance;
             if (reflected) {
                 dir = f.reflected_dir;
                 pol = f.reflected_pol;

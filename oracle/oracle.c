@@ -333,35 +333,70 @@ void ORACLE_NAME(lambertian)(const real n[3], real radial, real azimuth, real ou
 
 typedef struct {
     v3 rdir, tdir, rpol, tpol;
-    real t;
+    double t; /* transmission probability, evaluated in FP64 as Geant4 does */
     int tir;
 } fresnel_t;
 
+typedef struct {
+    double x, y, z;
+} d3;
+static d3 dv(v3 a) {
+    d3 r = {(double)a.x, (double)a.y, (double)a.z};
+    return r;
+}
+static double ddot(d3 a, d3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static d3 dcross(d3 a, d3 b) {
+    d3 r = {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+    return r;
+}
+static d3 dnorm(d3 a) {
+    const double inv = 1.0 / sqrt(ddot(a, a));
+    d3 r = {a.x * inv, a.y * inv, a.z * inv};
+    return r;
+}
+
+/* ComputeFresnel (OptiXBoundaryPhysics.cuh:122-178): vectors in the state
+ * precision with the reference's operation order; the reflect/transmit
+ * probability and the TIR test in FP64 from that state. */
 static fresnel_t or_fresnel(v3 d, v3 p, v3 n, real n1, real n2) {
     fresnel_t o;
     memset(&o, 0, sizeof(o));
     d = or_normalize(d);
     n = or_normalize(n);
     p = or_project(p, d);
+    const d3 dd = dv(d), dn = dv(n), dp = dv(p);
+    const double n1d = n1, n2d = n2;
+    const double ci_d = fmin(fmax(-ddot(dd, dn), 0.0), 1.0);
+    const double eta_d = n1d / n2d;
+    const double st2_d = eta_d * eta_d * (1.0 - ci_d * ci_d);
     const real ci = clamp01(-vdot(d, n));
     const real eta = n1 / n2;
     const real st2 = eta * eta * (RF(1.0) - ci * ci);
     o.rdir = or_reflect_dir(d, n);
-    o.tir = st2 >= RF(1.0);
-    o.t = RF(0.0);
+    o.tir = st2_d >= 1.0;
+    o.t = 0.0;
     if (o.tir) {
         o.rpol = or_reflect_pol(p, n, o.rdir);
         return o;
     }
+    const double ct_d = sqrt(fmax(0.0, 1.0 - st2_d));
     const real ct = R_SQRT(R_FMAX(RF(0.0), RF(1.0) - st2));
     o.tdir = or_normalize(vadd(vscale(d, eta), vscale(n, eta * ci - ct)));
     const v3 tr = vcross(d, n);
     if (vdot(tr, tr) < RF(1.0e-12)) {
-        const real ra = (n1 - n2) / (n1 + n2);
-        o.t = clamp01(RF(1.0) - ra * ra);
+        const double ra = (n1d - n2d) / (n1d + n2d);
+        o.t = fmin(fmax(1.0 - ra * ra, 0.0), 1.0);
         o.rpol = n2 > n1 ? vscale(p, RF(-1.0)) : p;
         o.tpol = p;
         return o;
+    }
+    {
+        const d3 sd = dnorm(dcross(dd, dn));
+        const d3 pd = dnorm(dcross(dd, sd));
+        const double es = ddot(dp, sd), ep = ddot(dp, pd);
+        const double rs = (n1d * ci_d - n2d * ct_d) / (n1d * ci_d + n2d * ct_d);
+        const double rp = (n2d * ci_d - n1d * ct_d) / (n2d * ci_d + n1d * ct_d);
+        o.t = fmin(fmax(1.0 - (es * es * rs * rs + ep * ep * rp * rp), 0.0), 1.0);
     }
     const v3 sd = or_normalize(tr);
     const v3 pd = or_normalize(vcross(d, sd));
@@ -373,8 +408,6 @@ static fresnel_t or_fresnel(v3 d, v3 p, v3 n, real n1, real n2) {
     const real rp = (n2 * ci - n1 * ct) / pden;
     const real ts = RF(2.0) * n1 * ci / sden;
     const real tp = RF(2.0) * n1 * ci / pden;
-    const real refl = is * is * rs * rs + ip * ip * rp * rp;
-    o.t = clamp01(RF(1.0) - refl);
     const v3 rpd = or_normalize(vcross(o.rdir, sd));
     const v3 tpd = or_normalize(vcross(o.tdir, sd));
     o.rpol = or_normalize(vadd(vscale(sd, is * rs), vscale(rpd, ip * rp)));
@@ -385,7 +418,7 @@ static fresnel_t or_fresnel(v3 d, v3 p, v3 n, real n1, real n2) {
 /* 11 inputs -> 14 outputs, same packing as g4go_eval_fresnel. */
 void ORACLE_NAME(fresnel)(const real* in, real* out) {
     const fresnel_t f = or_fresnel(mk(in[0], in[1], in[2]), mk(in[3], in[4], in[5]), mk(in[6], in[7], in[8]), in[9], in[10]);
-    out[0] = f.t;
+    out[0] = (real)f.t;
     out[1] = f.tir ? RF(1.0) : RF(0.0);
     const v3 vs[4] = {f.rdir, f.tdir, f.rpol, f.tpol};
     for (int k = 0; k < 4; ++k) {
@@ -1233,7 +1266,7 @@ static int or_step(photon_t* ph, const oscene_t* s, const rng_t* r, real eps, in
                     rough_pass = u01(wr[0]) < r_exp(-expo * expo);
                 }
                 const fresnel_t f = or_fresnel(dir, pol, inormal, n1, n2);
-                reflected = f.tir || u01(w[3]) > f.t;
+                reflected = f.tir || (double)u01(w[3]) > f.t;
                 if (reflected) {
                     if (rough_pass) {
                         if (sf->model == 1 && sf->finish != 0) {
@@ -1264,7 +1297,7 @@ static int or_step(photon_t* ph, const oscene_t* s, const rng_t* r, real eps, in
             absorbed = 1;
         } else {
             const fresnel_t f = or_fresnel(dir, pol, normal, n1, n2);
-            reflected = u01(w[3]) > f.t;
+            reflected = (double)u01(w[3]) > f.t;
             if (reflected) {
                 dir = f.rdir;
                 pol = f.rpol;
@@ -1458,4 +1491,65 @@ int64_t ORACLE_NAME(run_emissions)(const g4go_scene* scene, const g4go_emission*
     }
     destroy(s);
     return (int64_t)n_hits;
+}
+
+/* ------------------------------------------------------------------ */
+/* Exports for the closed-form tests                                   */
+/* ------------------------------------------------------------------ */
+void ORACLE_NAME(rayleigh)(const real dir[3], const real pol[3], uint64_t seed, uint32_t photon, uint32_t event,
+                           uint32_t step, real out_dir[3], real out_pol[3]) {
+    photon_t ph;
+    memset(&ph, 0, sizeof(ph));
+    ph.dir = mk(dir[0], dir[1], dir[2]);
+    ph.pol = mk(pol[0], pol[1], pol[2]);
+    ph.photon = photon;
+    ph.event = event;
+    ph.step = step;
+    const rng_t r = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    or_rayleigh(&ph, &r);
+    out_dir[0] = ph.dir.x;
+    out_dir[1] = ph.dir.y;
+    out_dir[2] = ph.dir.z;
+    out_pol[0] = ph.pol.x;
+    out_pol[1] = ph.pol.y;
+    out_pol[2] = ph.pol.z;
+}
+
+void ORACLE_NAME(box_exit)(const float lo[3], const float hi[3], const real p[3], const real d[3], real* dist,
+                           int* axis) {
+    volume_t v;
+    memset(&v, 0, sizeof(v));
+    for (int k = 0; k < 3; ++k) {
+        v.lo[k] = lo[k];
+        v.hi[k] = hi[k];
+    }
+    box_exit(&v, mk(p[0], p[1], p[2]), mk(d[0], d[1], d[2]), dist, axis);
+}
+
+void ORACLE_NAME(project_polarization)(const real pol[3], const real dir[3], real out[3]) {
+    const v3 r = or_project(mk(pol[0], pol[1], pol[2]), mk(dir[0], dir[1], dir[2]));
+    out[0] = r.x;
+    out[1] = r.y;
+    out[2] = r.z;
+}
+
+void ORACLE_NAME(reflect_direction)(const real dir[3], const real n[3], real out[3]) {
+    const v3 r = or_reflect_dir(mk(dir[0], dir[1], dir[2]), mk(n[0], n[1], n[2]));
+    out[0] = r.x;
+    out[1] = r.y;
+    out[2] = r.z;
+}
+
+void ORACLE_NAME(reflect_polarization)(const real pol[3], const real n[3], const real dir[3], real out[3]) {
+    const v3 r = or_reflect_pol(mk(pol[0], pol[1], pol[2]), mk(n[0], n[1], n[2]), mk(dir[0], dir[1], dir[2]));
+    out[0] = r.x;
+    out[1] = r.y;
+    out[2] = r.z;
+}
+
+void ORACLE_NAME(normalize)(const real a[3], real out[3]) {
+    const v3 r = or_normalize(mk(a[0], a[1], a[2]));
+    out[0] = r.x;
+    out[1] = r.y;
+    out[2] = r.z;
 }

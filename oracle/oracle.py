@@ -52,6 +52,14 @@ def _load(precision: int) -> C.CDLL:
         C.POINTER(abi.Scene), C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint32, C.c_uint64, C.c_float, C.c_uint32,
         C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p]
     getattr(lib, p + "run_emissions").restype = C.c_int64
+    getattr(lib, p + "rayleigh").argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32,
+                                             C.c_void_p, C.c_void_p]
+    getattr(lib, p + "box_exit").argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(real),
+                                             C.POINTER(C.c_int)]
+    for name in ("project_polarization", "reflect_direction"):
+        getattr(lib, p + name).argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+    getattr(lib, p + "normalize").argtypes = [C.c_void_p, C.c_void_p]
+    getattr(lib, p + "reflect_polarization").argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
     _libs[precision] = lib
     return lib
 
@@ -116,6 +124,38 @@ def lambertian(normal, radial, azimuth, precision: int = 32) -> np.ndarray:
     out = np.zeros(3, real)
     _fn(precision, "lambertian")(n.ctypes.data, radial, azimuth, out.ctypes.data)
     return out
+
+
+def vec_op(name: str, *vectors, precision: int = 32) -> np.ndarray:
+    """normalize(a), project_polarization(pol, dir), reflect_direction(dir, n), reflect_polarization(pol, n, dir)."""
+    real = _real(precision)
+    args = [np.ascontiguousarray(v, dtype=real) for v in vectors]
+    out = np.zeros(3, real)
+    _fn(precision, name)(*[a.ctypes.data for a in args], out.ctypes.data)
+    return out
+
+
+def rayleigh(direction, polarization, seed, photon, event=0, step=0, precision: int = 64):
+    real = _real(precision)
+    d = np.ascontiguousarray(direction, dtype=real)
+    p = np.ascontiguousarray(polarization, dtype=real)
+    od, op = np.zeros(3, real), np.zeros(3, real)
+    _fn(precision, "rayleigh")(d.ctypes.data, p.ctypes.data, seed, photon, event, step, od.ctypes.data,
+                               op.ctypes.data)
+    return od, op
+
+
+def box_exit(lo, hi, p, d, precision: int = 32):
+    real = _real(precision)
+    lo = np.ascontiguousarray(lo, dtype=np.float32)
+    hi = np.ascontiguousarray(hi, dtype=np.float32)
+    pp = np.ascontiguousarray(p, dtype=real)
+    dd = np.ascontiguousarray(d, dtype=real)
+    dist = (C.c_float if precision == 32 else C.c_double)()
+    axis = C.c_int()
+    _fn(precision, "box_exit")(lo.ctypes.data, hi.ctypes.data, pp.ctypes.data, dd.ctypes.data, C.byref(dist),
+                               C.byref(axis))
+    return dist.value, axis.value
 
 
 def find_surface(scene, from_id: int, to_id: int) -> int:

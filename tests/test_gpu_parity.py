@@ -26,7 +26,8 @@ def log(msg):
 
 
 def host(seed=42, **kw):
-    return HipTransportHost(PhotonTransportConfig(seed=seed, sort_hits=True, **kw))
+    kw.setdefault("sort_hits", True)
+    return HipTransportHost(PhotonTransportConfig(seed=seed, **kw))
 
 
 def assert_same_hits(gpu_hits, ref_hits):
