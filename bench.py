@@ -22,6 +22,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import shutil
 import sys
 import time
 
@@ -35,8 +36,8 @@ W_GEN, W_STEP, W_DET = 290.0, 193.0, 15.0
 VALU_PEAK_TLANEOPS = 39.3   # 256 CU x 64 lanes x 2.4 GHz
 HBM_PEAK_GBS = 8000.0
 HIT_BYTES = 48
-PERSISTENT = dict(persistent=True, block_size=256, chunk=1024, refill_threshold=16, blocks_per_cu=2)
 PLAIN = dict(persistent=False, block_size=256, chunk=64, refill_threshold=64, blocks_per_cu=8)
+TUNE_PHOTONS = 20_000_000
 
 
 def parse():
@@ -49,13 +50,16 @@ def parse():
     ap.add_argument("--cpu-sample", type=int, default=4_000_000, help="photons in the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--retune", action="store_true", help="ignore the autotuner cache")
+    ap.add_argument("--launch", type=str, default=None,
+                    help="explicit persistent form block,blocks_per_cu,chunk,refill (skips the autotuner)")
     return ap.parse_args()
 
 
 def run_form(host, cs, src, photons, launch, steps, warmup):
     from g4go_amd.transport import LaunchForm
 
-    host.set_launch(LaunchForm(**launch))
+    host.set_launch(launch if isinstance(launch, LaunchForm) else LaunchForm(**launch))
     ev = np.array([0], np.uint32)
     cnt = np.array([photons], np.uint32)
     for _ in range(warmup):
@@ -95,13 +99,33 @@ def main():
     if not torch.cuda.is_available():
         print(json.dumps({"error": "no GPU visible"}))
         sys.exit(1)
-    from g4go_amd import workloads
-    from g4go_amd.transport import HipTransportHost, PhotonTransportConfig
+    from g4go_amd import autotune, workloads
+    from g4go_amd.transport import HipTransportHost, LaunchForm, PhotonTransportConfig
 
     cs = workloads.crystal_scene()
     src = workloads.synthetic_source(cs)
     host = HipTransportHost(PhotonTransportConfig(seed=args.seed, sort_hits=False))
-    wall, kms, out = run_form(host, cs, src, args.photons, PERSISTENT, args.steps, args.warmup)
+    device, _ = host.device_info()
+    ev1 = np.array([0], np.uint32)
+
+    def tune_batch():
+        host.enqueue_synthetic(cs.scene, src, ev1, np.array([TUNE_PHOTONS], np.uint32))
+        out, _ = host.complete_device()
+        return out.statistics["kernel_ms"]
+
+    if args.launch:
+        bs, bpc, chunk, refill = (int(x) for x in args.launch.split(","))
+        persistent = LaunchForm(persistent=True, block_size=bs, chunk=chunk, refill_threshold=refill,
+                                blocks_per_cu=bpc)
+    else:
+        persistent, table = autotune.tuned_launch(host, device, "crystal-synthetic", tune_batch,
+                                                  retune=args.retune)
+        out_dir = os.path.join(ROOT, "gpurun_out")
+        if table is not None and os.path.isdir(out_dir):
+            shutil.copy(autotune.CACHE, os.path.join(out_dir, "autotune_cache.json"))
+            with open(os.path.join(out_dir, "autotune_table.json"), "w") as f:
+                json.dump(table, f, indent=0)
+    wall, kms, out = run_form(host, cs, src, args.photons, persistent, args.steps, args.warmup)
     plain = None
     if not args.no_plain:
         p_wall, p_kms, p_out = run_form(host, cs, src, args.photons, PLAIN, args.steps, args.warmup)
@@ -131,7 +155,8 @@ def main():
         "config": {
             "workload": "1e8 synthetic photons, 3x3x20 mm wrapped crystal + SiPM, one batch, hits + 256-bin "
                         "0-200 ns histogram in HBM",
-            "photons": args.photons, "form": "persistent", "launch": PERSISTENT, "seed": args.seed,
+            "photons": args.photons, "form": "persistent", "launch": persistent.__dict__, "seed": args.seed,
+            "device": device,
             "time": "fp64",
         },
         "plain": plain,

@@ -1067,6 +1067,21 @@ g4go_status g4go_complete_device(g4go_transport* h, g4go_batch_result* out, cons
     return complete_batch(h, out, false, hits_device);
 }
 
+g4go_status g4go_device_info(g4go_transport* h, char* name, uint64_t name_len, int32_t* cu_count) {
+    if (h == nullptr || name == nullptr || name_len == 0 || cu_count == nullptr) {
+        return G4GO_ERROR_INVALID_ARGUMENT;
+    }
+    return guarded(h, [&] {
+        h->init_hip();
+        hipDeviceProp_t prop{};
+        hip_check(hipGetDeviceProperties(&prop, h->device), "hipGetDeviceProperties");
+        const std::string text = std::string(prop.gcnArchName) + " " + prop.name;
+        std::strncpy(name, text.c_str(), name_len - 1);
+        name[name_len - 1] = '\0';
+        *cu_count = h->cu_count;
+    });
+}
+
 g4go_status g4go_eval_math(g4go_transport* h, int which, const float* in, float* out, uint64_t n) {
     if (h == nullptr || in == nullptr || out == nullptr || which < 0 || which > 2 || n > (1u << 28)) {
         return G4GO_ERROR_INVALID_ARGUMENT;

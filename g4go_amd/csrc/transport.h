@@ -873,8 +873,29 @@ G4GO_HD int transport_step(Photon& ph, const SceneView& sc, const Rng& rng, cons
             }
         } else {
             reflected = true;
-            dir = sample_surface_reflection(sc, s, dir, normal, rng, ph);
-            pol = reflect_polarization(pol, normal, dir);
+            if (s.finish == kPolished) {
+                // Specular reflection at an axis-aligned face is exact: the
+                // normal component of the direction changes sign, and the
+                // reflected polarisation -pol + 2 (pol.n) n keeps only its
+                // normal component.  Norms and pol.dir are preserved, so no
+                // renormalisation is needed.
+                if (axis == 0) {
+                    dir.x = -dir.x;
+                    pol.y = -pol.y;
+                    pol.z = -pol.z;
+                } else if (axis == 1) {
+                    dir.y = -dir.y;
+                    pol.x = -pol.x;
+                    pol.z = -pol.z;
+                } else {
+                    dir.z = -dir.z;
+                    pol.x = -pol.x;
+                    pol.y = -pol.y;
+                }
+            } else {
+                dir = sample_surface_reflection(sc, s, dir, normal, rng, ph);
+                pol = reflect_polarization(pol, normal, dir);
+            }
         }
     } else {
         const float n2 = prop_eval(sc.pool, nmat.rindex, e, u2f(0x7fc00000u));

@@ -178,6 +178,12 @@ class HipTransportHost:
                                       "device_to_host_bytes", "total_steps")}
         return PhotonTransportOutput(hits, stats, events, hist, perf)
 
+    def device_info(self) -> tuple[str, int]:
+        buf = C.create_string_buffer(256)
+        cu = C.c_int32()
+        self._check(self._lib.g4go_device_info(self._h, buf, 256, C.byref(cu)))
+        return buf.value.decode(), cu.value
+
     # ---- device evaluation of the shared physics (tests) ----
     def eval_math(self, which: int, x: np.ndarray) -> np.ndarray:
         x = np.ascontiguousarray(x, dtype=np.float32)

@@ -247,6 +247,10 @@ g4go_status g4go_eval_math(g4go_transport* handle, int which, const float* in, f
  * (transmittance, tir, reflected dir[3], transmitted dir[3], reflected pol[3], transmitted pol[3]). */
 g4go_status g4go_eval_fresnel(g4go_transport* handle, const float* in, float* out, uint64_t n);
 
+/* Name (gcnArchName and marketing name) and compute-unit count of the
+ * handle's device; initialises HIP on the device. */
+g4go_status g4go_device_info(g4go_transport* handle, char* name, uint64_t name_len, int32_t* cu_count);
+
 /* Library version string. */
 const char* g4go_version(void);
 

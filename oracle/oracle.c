@@ -1288,8 +1288,25 @@ static int or_step(photon_t* ph, const oscene_t* s, const rng_t* r, real eps, in
             }
         } else {
             reflected = 1;
-            dir = or_surface_reflection(sf, dir, normal, r, ph);
-            pol = or_reflect_pol(pol, normal, dir);
+            if (sf->finish == 0) {
+                /* exact specular reflection at an axis-aligned face */
+                if (axis == 0) {
+                    dir.x = -dir.x;
+                    pol.y = -pol.y;
+                    pol.z = -pol.z;
+                } else if (axis == 1) {
+                    dir.y = -dir.y;
+                    pol.x = -pol.x;
+                    pol.z = -pol.z;
+                } else {
+                    dir.z = -dir.z;
+                    pol.x = -pol.x;
+                    pol.y = -pol.y;
+                }
+            } else {
+                dir = or_surface_reflection(sf, dir, normal, r, ph);
+                pol = or_reflect_pol(pol, normal, dir);
+            }
         }
     } else {
         const real n2 = or_prop(&nmat->rindex, e, NAN);
