@@ -237,7 +237,7 @@ assert C.sizeof(Config) == 40 and C.sizeof(Launch) == 20
 EXPORTED_SYMBOLS = (
     "g4go_create", "g4go_destroy", "g4go_last_error", "g4go_set_launch", "g4go_get_launch",
     "g4go_prepare_scene", "g4go_enqueue_emissions", "g4go_enqueue_synthetic", "g4go_complete",
-    "g4go_complete_device", "g4go_eval_math", "g4go_eval_fresnel", "g4go_device_info", "g4go_version",
+    "g4go_complete_device", "g4go_eval_math", "g4go_eval_fresnel", "g4go_device_info", "g4go_trace_synthetic", "g4go_version",
 )
 
 _lib = None
@@ -267,9 +267,11 @@ def load() -> C.CDLL:
     lib.g4go_eval_math.argtypes = [H, C.c_int, C.c_void_p, C.c_void_p, C.c_uint64]
     lib.g4go_eval_fresnel.argtypes = [H, C.c_void_p, C.c_void_p, C.c_uint64]
     lib.g4go_device_info.argtypes = [H, C.c_char_p, C.c_uint64, C.POINTER(C.c_int32)]
+    lib.g4go_trace_synthetic.argtypes = [H, C.POINTER(SyntheticSource), C.c_uint32, C.c_uint32, C.c_uint32,
+                                         C.c_void_p, C.POINTER(C.c_uint32)]
     lib.g4go_version.restype = C.c_char_p
     for name in ("g4go_create", "g4go_set_launch", "g4go_get_launch", "g4go_prepare_scene", "g4go_enqueue_emissions",
-                 "g4go_enqueue_synthetic", "g4go_complete", "g4go_complete_device", "g4go_eval_math", "g4go_eval_fresnel", "g4go_device_info"):
+                 "g4go_enqueue_synthetic", "g4go_complete", "g4go_complete_device", "g4go_eval_math", "g4go_eval_fresnel", "g4go_device_info", "g4go_trace_synthetic"):
         getattr(lib, name).restype = C.c_int
     _lib = lib
     return lib

@@ -1082,6 +1082,46 @@ g4go_status g4go_device_info(g4go_transport* h, char* name, uint64_t name_len, i
     });
 }
 
+g4go_status g4go_trace_synthetic(g4go_transport* h, const g4go_synthetic_source* source, uint32_t event_id,
+                                 uint32_t photon_id, uint32_t max_steps, float* out, uint32_t* n_steps) {
+    if (h == nullptr || source == nullptr || out == nullptr || n_steps == nullptr || max_steps == 0 ||
+        max_steps > 65536) {
+        return G4GO_ERROR_INVALID_ARGUMENT;
+    }
+    return guarded(h, [&] {
+        if (!h->scene_built) {
+            fail(G4GO_ERROR_LOGIC, "no scene has been prepared");
+        }
+        const auto it = h->scene.index_of.find(source->volume_id);
+        if (it == h->scene.index_of.end()) {
+            fail(G4GO_ERROR_INVALID_ARGUMENT, "synthetic source references an unknown volume ID");
+        }
+        h->init_hip();
+        h->upload_scene();
+        KernelParams p = h->base_params();
+        for (int k = 0; k < 3; ++k) {
+            p.synth.lo[k] = source->lo_mm[k];
+            p.synth.hi[k] = source->hi_mm[k];
+        }
+        p.synth.wavelength_mean_nm = source->wavelength_mean_nm;
+        p.synth.wavelength_sigma_nm = source->wavelength_sigma_nm;
+        p.synth.decay_ns = source->decay_time_ns;
+        p.synth.volume = it->second;
+        DevBuf dout, dn;
+        dout.ensure(static_cast<size_t>(max_steps) * 16 * sizeof(float), "hipMalloc");
+        dn.ensure(sizeof(uint32_t), "hipMalloc");
+        if (launch_trace(p, event_id, photon_id, max_steps, dout.as<float>(), dn.as<uint32_t>(), h->stream) != 0) {
+            fail(G4GO_ERROR_RUNTIME, "trace kernel launch failed");
+        }
+        hip_check(hipStreamSynchronize(h->stream), "hipStreamSynchronize");
+        hip_check(hipMemcpy(n_steps, dn.ptr, sizeof(uint32_t), hipMemcpyDeviceToHost), "hipMemcpy");
+        hip_check(hipMemcpy(out, dout.ptr, static_cast<size_t>(*n_steps) * 16 * sizeof(float), hipMemcpyDeviceToHost),
+                  "hipMemcpy");
+        dout.release();
+        dn.release();
+    });
+}
+
 g4go_status g4go_eval_math(g4go_transport* h, int which, const float* in, float* out, uint64_t n) {
     if (h == nullptr || in == nullptr || out == nullptr || which < 0 || which > 2 || n > (1u << 28)) {
         return G4GO_ERROR_INVALID_ARGUMENT;

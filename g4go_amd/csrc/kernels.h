@@ -90,6 +90,8 @@ struct AdjointParams {
 int launch_transport(const KernelParams& p, int source, bool persistent, int block, int grid, hipStream_t stream);
 int launch_zero(void* ptr, size_t bytes, hipStream_t stream);
 int launch_adjoint(const AdjointParams& p, int source, int block, hipStream_t stream);
+int launch_trace(const KernelParams& p, uint32_t event_id, uint32_t photon_id, uint32_t max_steps, float* out,
+                 uint32_t* n_out, hipStream_t stream);
 int launch_eval_math(const float* in, float* out, int n, int which, hipStream_t stream);
 int launch_eval_fresnel(const float* in, float* out, int n, hipStream_t stream);
 

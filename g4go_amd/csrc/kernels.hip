@@ -339,6 +339,56 @@ int launch_transport(const KernelParams& p, int source, bool persistent, int blo
     return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 
+// ---- step-by-step trace of one synthetic photon (tests and diagnostics) ----
+// Per step, 16 floats: pos[3], dir[3], pol[3], time (float), status, zero,
+// step, volume, energy, 0.
+__global__ void trace_kernel(KernelParams p, uint32_t event_id, uint32_t photon_id, uint32_t max_steps, float* out,
+                             uint32_t* n_out) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) {
+        return;
+    }
+    Photon ph;
+    generate_synthetic(ph, p.synth, p.rng, event_id, photon_id);
+    ph.material = kInvalid;
+    uint32_t n = 0;
+    if (start_photon(ph)) {
+        for (; n < max_steps;) {
+            bool zero = false;
+            uint32_t sensor = kInvalid;
+            const int status = transport_step<false>(ph, p.scene, p.rng, p.cfg, zero, sensor, nullptr);
+            ++ph.step;
+            float* o = out + 16 * n;
+            o[0] = ph.pos.x;
+            o[1] = ph.pos.y;
+            o[2] = ph.pos.z;
+            o[3] = ph.dir.x;
+            o[4] = ph.dir.y;
+            o[5] = ph.dir.z;
+            o[6] = ph.pol.x;
+            o[7] = ph.pol.y;
+            o[8] = ph.pol.z;
+            o[9] = static_cast<float>(ph.time);
+            o[10] = static_cast<float>(status);
+            o[11] = zero ? 1.0f : 0.0f;
+            o[12] = static_cast<float>(ph.step);
+            o[13] = static_cast<float>(ph.volume);
+            o[14] = ph.energy;
+            o[15] = 0.0f;
+            ++n;
+            if (status != kAlive) {
+                break;
+            }
+        }
+    }
+    *n_out = n;
+}
+
+int launch_trace(const KernelParams& p, uint32_t event_id, uint32_t photon_id, uint32_t max_steps, float* out,
+                 uint32_t* n_out, hipStream_t stream) {
+    hipLaunchKernelGGL(trace_kernel, dim3(1), dim3(64), 0, stream, p, event_id, photon_id, max_steps, out, n_out);
+    return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
 // ---- element-wise evaluation kernels used by the parity tests ----
 __global__ void eval_math_kernel(const float* in, float* out, int n, int which) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;

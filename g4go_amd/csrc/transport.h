@@ -874,24 +874,14 @@ G4GO_HD int transport_step(Photon& ph, const SceneView& sc, const Rng& rng, cons
         } else {
             reflected = true;
             if (s.finish == kPolished) {
-                // Specular reflection at an axis-aligned face is exact: the
-                // normal component of the direction changes sign, and the
-                // reflected polarisation -pol + 2 (pol.n) n keeps only its
-                // normal component.  Norms and pol.dir are preserved, so no
-                // renormalisation is needed.
-                if (axis == 0) {
-                    dir.x = -dir.x;
-                    pol.y = -pol.y;
-                    pol.z = -pol.z;
-                } else if (axis == 1) {
-                    dir.y = -dir.y;
-                    pol.x = -pol.x;
-                    pol.z = -pol.z;
-                } else {
-                    dir.z = -dir.z;
-                    pol.x = -pol.x;
-                    pol.y = -pol.y;
-                }
+                // Specular reflection at an axis-aligned face without
+                // renormalisation: with a unit normal along one axis, every
+                // operation of d - 2 (d.n) n and -p + 2 (p.n) n is exact (the
+                // normal component changes sign), so norms and p.d are kept.
+                const float dn = dot(dir, normal);
+                const float pn = dot(pol, normal);
+                dir = sub(dir, scale(normal, 2.0f * dn));
+                pol = add(scale(pol, -1.0f), scale(normal, 2.0f * pn));
             } else {
                 dir = sample_surface_reflection(sc, s, dir, normal, rng, ph);
                 pol = reflect_polarization(pol, normal, dir);

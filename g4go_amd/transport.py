@@ -184,6 +184,16 @@ class HipTransportHost:
         self._check(self._lib.g4go_device_info(self._h, buf, 256, C.byref(cu)))
         return buf.value.decode(), cu.value
 
+    def trace_synthetic(self, scene: Scene, source: abi.SyntheticSource, event_id: int, photon_id: int,
+                        max_steps: int = 4096) -> np.ndarray:
+        """Per-step records (n, 16) of one synthetic photon transported on the device."""
+        self.prepare_scene(scene)
+        out = np.zeros((max_steps, 16), np.float32)
+        n = C.c_uint32()
+        self._check(self._lib.g4go_trace_synthetic(self._h, C.byref(source), event_id, photon_id, max_steps,
+                                                   out.ctypes.data, C.byref(n)))
+        return out[:n.value]
+
     # ---- device evaluation of the shared physics (tests) ----
     def eval_math(self, which: int, x: np.ndarray) -> np.ndarray:
         x = np.ascontiguousarray(x, dtype=np.float32)

@@ -247,6 +247,12 @@ g4go_status g4go_eval_math(g4go_transport* handle, int which, const float* in, f
  * (transmittance, tir, reflected dir[3], transmitted dir[3], reflected pol[3], transmitted pol[3]). */
 g4go_status g4go_eval_fresnel(g4go_transport* handle, const float* in, float* out, uint64_t n);
 
+/* Step-by-step trace of one synthetic photon (tests and diagnostics): up to
+ * max_steps records of 16 floats (pos[3], dir[3], pol[3], time, status,
+ * zero-step flag, step count, volume index, energy, 0). */
+g4go_status g4go_trace_synthetic(g4go_transport* handle, const g4go_synthetic_source* source, uint32_t event_id,
+                                 uint32_t photon_id, uint32_t max_steps, float* out, uint32_t* n_steps);
+
 /* Name (gcnArchName and marketing name) and compute-unit count of the
  * handle's device; initialises HIP on the device. */
 g4go_status g4go_device_info(g4go_transport* handle, char* name, uint64_t name_len, int32_t* cu_count);

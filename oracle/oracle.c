@@ -1289,20 +1289,11 @@ static int or_step(photon_t* ph, const oscene_t* s, const rng_t* r, real eps, in
         } else {
             reflected = 1;
             if (sf->finish == 0) {
-                /* exact specular reflection at an axis-aligned face */
-                if (axis == 0) {
-                    dir.x = -dir.x;
-                    pol.y = -pol.y;
-                    pol.z = -pol.z;
-                } else if (axis == 1) {
-                    dir.y = -dir.y;
-                    pol.x = -pol.x;
-                    pol.z = -pol.z;
-                } else {
-                    dir.z = -dir.z;
-                    pol.x = -pol.x;
-                    pol.y = -pol.y;
-                }
+                /* specular reflection at an axis-aligned face, exact without renormalisation */
+                const real dn = vdot(dir, normal);
+                const real pn = vdot(pol, normal);
+                dir = vsub(dir, vscale(normal, RF(2.0) * dn));
+                pol = vadd(vscale(pol, RF(-1.0)), vscale(normal, RF(2.0) * pn));
             } else {
                 dir = or_surface_reflection(sf, dir, normal, r, ph);
                 pol = or_reflect_pol(pol, normal, dir);

@@ -157,3 +157,28 @@ def test_full_size_invariants(crystal):
     assert st["dropped_hits"] == 0 and ptr != 0
     frac = int(ev["detected"]) / 1e8
     assert 0.2 < frac < 0.35, frac
+
+
+def test_photon_trace_invariants(crystal):
+    """Per-step device trace: unit direction, polarisation perpendicular to it,
+    non-decreasing time, no repeated zero-length steps, and the end point of a
+    detected photon equals its record in the batch output."""
+    log("case: device photon traces")
+    src = workloads.synthetic_source(crystal)
+    ref = O.run_synthetic(crystal.scene, src, 42, [3], [64])
+    with host(42) as h:
+        for pid in range(64):
+            t = h.trace_synthetic(crystal.scene, src, 3, pid, 4096)
+            assert len(t) > 0
+            d, p = t[:, 3:6].astype(np.float64), t[:, 6:9].astype(np.float64)
+            assert np.max(np.abs(np.sum(d * d, axis=1) - 1)) < 1e-5
+            assert np.max(np.abs(np.sum(d * p, axis=1))) < 1e-5
+            assert np.all(np.diff(t[:, 9]) >= 0)
+            zero = t[:, 11] > 0
+            assert not np.any(zero[1:] & zero[:-1]), f"photon {pid}: consecutive zero-length steps"
+            if t[-1, 10] == 1:  # detected
+                rec = ref.hits[ref.hits["photon_id"] == pid]
+                assert len(rec) == 1
+                np.testing.assert_array_equal(t[-1, 0:3], rec["position_mm"][0])
+            else:
+                assert pid not in set(ref.hits["photon_id"].tolist())
