@@ -134,8 +134,44 @@ __device__ __forceinline__ void generate(Photon& ph, const KernelParams& p, uint
     }
 }
 
-__device__ __forceinline__ void store_hit(Hit* dst, const Photon& ph, uint32_t sensor) {
-    float4* d = reinterpret_cast<float4*>(dst);
+// Detected photons are staged per wave in LDS and moved to the global hit
+// list in groups: one atomic reservation per group instead of one per wave
+// step (a single global counter serialises at the L2 otherwise).
+constexpr uint32_t kStageCap = 96;    // staged records per wave
+constexpr uint32_t kStageFlush = 32;  // flush once this many are staged (<= 31 + 64 < kStageCap)
+constexpr int kMaxWavesPerBlock = 4;
+
+__device__ __forceinline__ void flush_stage(const KernelParams& p, const float4* stage, const uint32_t* src,
+                                            uint32_t staged, uint32_t lane) {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    uint32_t base = 0;
+    if (lane == 0u) {
+        base = atomicAdd(p.hit_count, staged);
+    }
+    base = uniform(static_cast<uint32_t>(__shfl(static_cast<int>(base), 0)));
+    float4* dst = reinterpret_cast<float4*>(p.hits);
+    for (uint32_t i = lane; i < 3u * staged; i += 64u) {
+        if (base + i / 3u < p.hit_capacity) {
+            dst[3ull * base + i] = stage[i];
+        }
+    }
+    if (p.hit_source != nullptr) {
+        for (uint32_t i = lane; i < staged; i += 64u) {
+            if (base + i < p.hit_capacity) {
+                p.hit_source[base + i] = src[i];
+            }
+        }
+    }
+    if (lane == 0u && base + staged > p.hit_capacity) {
+        const uint32_t kept = base >= p.hit_capacity ? 0u : p.hit_capacity - base;
+        atomicAdd(p.counters + kCounterDropped, static_cast<unsigned long long>(staged - kept));
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+__device__ __forceinline__ void store_hit(float4* d, const Photon& ph, uint32_t sensor) {
     d[0] = make_float4(ph.pos.x, ph.pos.y, ph.pos.z, ph.dir.x);
     d[1] = make_float4(ph.dir.y, ph.dir.z, ph.energy, __uint_as_float(sensor));
     const uint64_t tb = __double_as_longlong(ph.time);
@@ -148,10 +184,15 @@ __device__ __forceinline__ void store_hit(Hit* dst, const Photon& ph, uint32_t s
 template <int kSource, bool kPersistent>
 __global__ void __launch_bounds__(256) transport_kernel(KernelParams p) {
     __shared__ uint32_t s_hist[kHistSlots];
+    __shared__ float4 s_stage[kMaxWavesPerBlock][3 * kStageCap];
+    __shared__ uint32_t s_stage_src[kMaxWavesPerBlock][kStageCap];
     for (int i = threadIdx.x; i < kHistSlots; i += blockDim.x) {
         s_hist[i] = 0u;
     }
     __syncthreads();
+    float4* const stage = s_stage[threadIdx.x >> 6];
+    uint32_t* const stage_src = s_stage_src[threadIdx.x >> 6];
+    uint32_t staged = 0;  // wave-uniform
 
     const uint32_t lane = lane_id();
     const uint64_t below = lanes_below();
@@ -252,25 +293,19 @@ __global__ void __launch_bounds__(256) transport_kernel(KernelParams p) {
             }
         }
 
-        // ---- fused hit compaction ----
+        // ---- fused hit compaction: ballot + prefix into the wave's LDS stage ----
         const uint64_t det = __ballot(status == kDetected);
         if (det != 0ull) {
-            uint32_t base = 0;
-            if (lane == 0u) {
-                base = atomicAdd(p.hit_count, static_cast<uint32_t>(__popcll(det)));
-            }
-            base = static_cast<uint32_t>(__shfl(static_cast<int>(base), 0));
             if (status == kDetected) {
-                const uint32_t idx = base + static_cast<uint32_t>(__popcll(det & below));
-                if (idx < p.hit_capacity) {
-                    store_hit(p.hits + idx, ph, sensor);
-                    if (p.hit_source != nullptr) {
-                        p.hit_source[idx] = my_item;
-                    }
-                } else {
-                    atomicAdd(p.counters + kCounterDropped, 1ull);
-                }
+                const uint32_t slot = staged + static_cast<uint32_t>(__popcll(det & below));
+                store_hit(stage + 3u * slot, ph, sensor);
+                stage_src[slot] = my_item;
                 atomicAdd(&s_hist[hist_slot(ph.time, p.hist_lo_ns, p.hist_width_ns)], 1u);
+            }
+            staged += static_cast<uint32_t>(__popcll(det));
+            if (staged >= kStageFlush) {
+                flush_stage(p, stage, stage_src, staged, lane);
+                staged = 0;
             }
         }
         if (alive && status != kAlive) {
@@ -290,6 +325,9 @@ __global__ void __launch_bounds__(256) transport_kernel(KernelParams p) {
         }
     }
 
+    if (staged != 0u) {
+        flush_stage(p, stage, stage_src, staged, lane);
+    }
     flush_event_final(p.event_stats, my_event, st);
     const uint32_t ws = wave_sum(lane_steps);
     const uint32_t wp = wave_sum(lane_photons);
