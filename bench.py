@@ -76,6 +76,23 @@ def run_form(host, cs, src, photons, launch, steps, warmup):
     return wall, kernel_ms, outs[-1]
 
 
+def run_uneven(host, cs, src, counts, launch, steps, warmup):
+    """Batches of uneven events: wall time per batch, kernel ms."""
+    host.set_launch(launch)
+    ev = np.arange(counts.size, dtype=np.uint32)
+    for _ in range(warmup):
+        host.enqueue_synthetic(cs.scene, src, ev, counts)
+        host.complete_device()
+    kms = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        host.enqueue_synthetic(cs.scene, src, ev, counts)
+        out, _ = host.complete_device()
+        kms.append(out.statistics["kernel_ms"])
+    wall = time.perf_counter() - t0
+    return wall, float(np.mean(kms)), out
+
+
 def cpu_baseline(cs, src, seed, sample):
     from oracle import oracle as O
 
@@ -131,6 +148,21 @@ def main():
         p_wall, p_kms, p_out = run_form(host, cs, src, args.photons, PLAIN, args.steps, args.warmup)
         plain = {"photons_per_s": args.photons * args.steps / p_wall, "ms_per_step": 1e3 * p_wall / args.steps,
                  "kernel_ms": float(np.mean(p_kms))}
+    # uneven events: 4096 log-normal events per batch, batched (persistent + hipGraph) vs plain per event
+    counts = workloads.lognormal_counts(4096, 5000, sigma=1.5, lo=10, hi=1_000_000, seed=args.seed)
+    n_uneven = int(counts.astype(np.int64).sum())
+    graphed = LaunchForm(**{**persistent.__dict__, "graph": True})
+    b_wall, b_kms, b_out = run_uneven(host, cs, src, counts, graphed, args.steps, args.warmup)
+    uneven = {"events": int(counts.size), "photons": n_uneven, "median": 5000, "sigma": 1.5,
+              "batched_photons_per_s": n_uneven * args.steps / b_wall, "batched_kernel_ms": b_kms,
+              "batched_launch": graphed.__dict__}
+    if not args.no_plain:
+        pe = LaunchForm(persistent=False, per_event=True)
+        e_wall, e_kms, e_out = run_uneven(host, cs, src, counts, pe, args.steps, args.warmup)
+        assert e_out.statistics["detected"] == b_out.statistics["detected"]
+        uneven.update({"plain_per_event_photons_per_s": n_uneven * args.steps / e_wall,
+                       "plain_per_event_kernel_ms": e_kms,
+                       "batched_over_plain": e_wall / b_wall})
     host.close()
 
     st = out.statistics
@@ -160,6 +192,7 @@ def main():
             "time": "fp64",
         },
         "plain": plain,
+        "uneven_events": uneven,
         "batched_over_plain": (None if plain is None else (args.photons * args.steps / wall) / plain["photons_per_s"]),
         "counts": {"photons": P, "steps": S, "detected": D, "steps_per_photon": S / max(P, 1),
                    "detected_fraction": D / max(P, 1)},

@@ -44,6 +44,8 @@ class Launch(C.Structure):
         ("chunk", C.c_uint32),
         ("refill_threshold", C.c_uint32),
         ("blocks_per_cu", C.c_uint32),
+        ("per_event", C.c_uint32),
+        ("graph", C.c_uint32),
     ]
 
 
@@ -232,7 +234,7 @@ EMISSION_DTYPE = np.dtype([
 ])
 assert EMISSION_DTYPE.itemsize == C.sizeof(Emission) == 128
 
-assert C.sizeof(Config) == 40 and C.sizeof(Launch) == 20
+assert C.sizeof(Config) == 40 and C.sizeof(Launch) == 28
 
 EXPORTED_SYMBOLS = (
     "g4go_create", "g4go_destroy", "g4go_last_error", "g4go_set_launch", "g4go_get_launch",

@@ -508,6 +508,10 @@ struct g4go_transport {
     uint64_t h2d_bytes = 0;
 
     DevBuf d_inputs, d_hits, d_event_stats, d_meta;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    KernelParams graph_params{};
+    int graph_grid = 0, graph_block = 0, graph_source = -1;
     PinBuf h_stage, h_meta, h_hits;
 
     std::vector<g4go_hit> out_hits;
@@ -590,6 +594,7 @@ struct g4go_transport {
         if (stream != nullptr) {
             (void)hipStreamSynchronize(stream);
         }
+        drop_graph();
         d_scene.release();
         d_inputs.release();
         d_hits.release();
@@ -611,7 +616,8 @@ struct g4go_transport {
     }
 
     KernelParams base_params() const {
-        KernelParams p{};
+        KernelParams p;
+        std::memset(&p, 0, sizeof(p));  // padding included: captured parameters are compared bytewise
         p.scene = view;
         p.rng = Rng{static_cast<uint32_t>(cfg.seed), static_cast<uint32_t>(cfg.seed >> 32)};
         p.cfg = StepConfig{cfg.boundary_tolerance_mm};
@@ -622,8 +628,53 @@ struct g4go_transport {
         return p;
     }
 
-    // Stage the event list and the per-source chunk offsets, allocate outputs,
-    // launch and queue the metadata readback.  `emissions` is null for the
+    // Device input layout, sized by capacities so that every pointer stays
+    // fixed while a batch fits (a captured graph can then be replayed):
+    // [header 256 B: n_sources, chunk, n_events][emissions][event ids]
+    // [item offsets][source photon counts][source event indices].
+    struct Layout {
+        uint64_t cap_em = 0, cap_ev = 0, cap_src = 0;
+        size_t o_em = 0, o_ev = 0, o_off = 0, o_cnt = 0, o_sev = 0, bytes = 0;
+    };
+    Layout layout;
+
+    void ensure_layout(uint64_t n_em, uint64_t n_ev, uint64_t n_src) {
+        if (n_em <= layout.cap_em && n_ev <= layout.cap_ev && n_src <= layout.cap_src && d_inputs.ptr != nullptr) {
+            return;
+        }
+        Layout l;
+        l.cap_em = std::max<uint64_t>({n_em, layout.cap_em, 1});
+        l.cap_ev = std::max<uint64_t>({n_ev, layout.cap_ev, 1});
+        l.cap_src = std::max<uint64_t>({n_src, layout.cap_src, 1});
+        l.cap_em += l.cap_em / 2;
+        l.cap_ev += l.cap_ev / 2;
+        l.cap_src += l.cap_src / 2;
+        l.o_em = 256;
+        l.o_ev = l.o_em + align256(l.cap_em * sizeof(Emission));
+        l.o_off = l.o_ev + align256(l.cap_ev * sizeof(uint32_t));
+        l.o_cnt = l.o_off + align256((l.cap_src + 1) * sizeof(uint32_t));
+        l.o_sev = l.o_cnt + align256(l.cap_src * sizeof(uint32_t));
+        l.bytes = l.o_sev + align256(l.cap_src * sizeof(uint32_t));
+        d_inputs.release();
+        d_inputs.ensure(l.bytes, "hipMalloc batch inputs");
+        h_stage.ensure(l.bytes, "hipHostMalloc staging");
+        layout = l;
+    }
+
+    void drop_graph() {
+        if (graph_exec != nullptr) {
+            (void)hipGraphExecDestroy(graph_exec);
+            graph_exec = nullptr;
+        }
+        if (graph != nullptr) {
+            (void)hipGraphDestroy(graph);
+            graph = nullptr;
+        }
+    }
+
+    // Stage the batch, allocate outputs, launch (plain, plain per event, or
+    // persistent, the latter replayed from a captured graph when enabled)
+    // and queue the metadata readback.  `emissions` is null for the
     // synthetic source (one source per event).
     void run_batch(const Emission* emissions, uint64_t n_emissions, const std::vector<uint32_t>& source_counts,
                    const std::vector<uint32_t>& source_event, const SyntheticSource* synth) {
@@ -636,52 +687,54 @@ struct g4go_transport {
             offsets[i + 1] = offsets[i] + (source_counts[i] + chunk - 1) / chunk;
         }
         n_items = offsets[n_sources];
+        ensure_layout(n_emissions, n_events, n_sources);
+        const Layout& l = layout;
+        d_hits.ensure(std::max<uint64_t>(n_photons, 1) * sizeof(Hit), "hipMalloc hits");
+        d_event_stats.ensure(l.cap_ev * sizeof(EventStats), "hipMalloc event stats");
+        d_meta.ensure(kMetaBytes, "hipMalloc counters");
+        h_meta.ensure(kMetaBytes + n_events * sizeof(EventStats), "hipHostMalloc counters");
+
+        char* st = h_stage.as<char>();
+        auto* header = reinterpret_cast<uint32_t*>(st);
+        header[0] = static_cast<uint32_t>(n_sources);
+        header[1] = chunk;
+        header[2] = static_cast<uint32_t>(n_events);
+        header[3] = 0;
         const size_t em_bytes = n_emissions * sizeof(Emission);
         const size_t ev_bytes = n_events * sizeof(uint32_t);
         const size_t off_bytes = (n_sources + 1) * sizeof(uint32_t);
         const size_t cnt_bytes = n_sources * sizeof(uint32_t);
         const size_t sev_bytes = source_event.size() * sizeof(uint32_t);
-        const size_t o_ev = align256(em_bytes);
-        const size_t o_off = o_ev + align256(ev_bytes);
-        const size_t o_cnt = o_off + align256(off_bytes);
-        const size_t o_sev = o_cnt + align256(cnt_bytes);
-        const size_t stage_bytes = o_sev + align256(sev_bytes);
-        h_stage.ensure(stage_bytes, "hipHostMalloc staging");
-        char* st = h_stage.as<char>();
-        if (em_bytes) std::memcpy(st, emissions, em_bytes);
-        if (ev_bytes) std::memcpy(st + o_ev, event_ids.data(), ev_bytes);
-        std::memcpy(st + o_off, offsets.data(), off_bytes);
-        if (cnt_bytes) std::memcpy(st + o_cnt, source_counts.data(), cnt_bytes);
-        if (sev_bytes) std::memcpy(st + o_sev, source_event.data(), sev_bytes);
-        d_inputs.ensure(stage_bytes, "hipMalloc batch inputs");
-        d_hits.ensure(std::max<uint64_t>(n_photons, 1) * sizeof(Hit), "hipMalloc hits");
-        d_event_stats.ensure(std::max<uint64_t>(n_events, 1) * sizeof(EventStats), "hipMalloc event stats");
-        d_meta.ensure(kMetaBytes, "hipMalloc counters");
-        h_meta.ensure(kMetaBytes + n_events * sizeof(EventStats), "hipHostMalloc counters");
-
+        if (em_bytes) std::memcpy(st + l.o_em, emissions, em_bytes);
+        if (ev_bytes) std::memcpy(st + l.o_ev, event_ids.data(), ev_bytes);
+        std::memcpy(st + l.o_off, offsets.data(), off_bytes);
+        if (cnt_bytes) std::memcpy(st + l.o_cnt, source_counts.data(), cnt_bytes);
+        if (sev_bytes) std::memcpy(st + l.o_sev, source_event.data(), sev_bytes);
+        char* in = d_inputs.as<char>();
         hip_check(hipEventRecord(ev_start, stream), "hipEventRecord");
-        hip_check(hipMemcpyAsync(d_inputs.ptr, st, stage_bytes, hipMemcpyHostToDevice, stream),
-                  "hipMemcpyAsync batch inputs");
-        h2d_bytes = em_bytes + ev_bytes + off_bytes + cnt_bytes + sev_bytes;
-        if (launch_zero(d_meta.ptr, kMetaBytes, stream) != 0 ||
-            launch_zero(d_event_stats.ptr, n_events * sizeof(EventStats), stream) != 0) {
-            fail(G4GO_ERROR_RUNTIME, "zero kernel launch failed");
+        const std::pair<size_t, size_t> regions[] = {
+            {0, 16}, {l.o_em, em_bytes}, {l.o_ev, ev_bytes}, {l.o_off, off_bytes}, {l.o_cnt, cnt_bytes},
+            {l.o_sev, sev_bytes}};
+        for (const auto& r : regions) {
+            if (r.second != 0) {
+                hip_check(hipMemcpyAsync(in + r.first, st + r.first, r.second, hipMemcpyHostToDevice, stream),
+                          "hipMemcpyAsync batch inputs");
+            }
         }
+        h2d_bytes = 16 + em_bytes + ev_bytes + off_bytes + cnt_bytes + sev_bytes;
 
         KernelParams p = base_params();
         if (synth != nullptr) {
             p.synth = *synth;
         }
-        char* in = d_inputs.as<char>();
-        p.emissions = reinterpret_cast<const Emission*>(in);
-        p.event_ids = reinterpret_cast<const uint32_t*>(in + o_ev);
-        p.item_offsets = reinterpret_cast<const uint32_t*>(in + o_off);
-        p.source_counts = reinterpret_cast<const uint32_t*>(in + o_cnt);
-        p.source_event = sev_bytes ? reinterpret_cast<const uint32_t*>(in + o_sev) : nullptr;
-        p.n_sources = static_cast<uint32_t>(n_sources);
-        p.chunk = chunk;
+        p.header = reinterpret_cast<const uint32_t*>(in);
+        p.emissions = reinterpret_cast<const Emission*>(in + l.o_em);
+        p.event_ids = reinterpret_cast<const uint32_t*>(in + l.o_ev);
+        p.item_offsets = reinterpret_cast<const uint32_t*>(in + l.o_off);
+        p.source_counts = reinterpret_cast<const uint32_t*>(in + l.o_cnt);
+        p.source_event = reinterpret_cast<const uint32_t*>(in + l.o_sev);
         p.hits = d_hits.as<Hit>();
-        p.hit_capacity = static_cast<uint32_t>(std::min<uint64_t>(std::max<uint64_t>(n_photons, 1), 0xffffffffu));
+        p.hit_capacity = static_cast<uint32_t>(std::min<size_t>(d_hits.cap / sizeof(Hit), 0xffffffffu));
         p.hit_count = reinterpret_cast<uint32_t*>(d_meta.as<char>() + kMetaHitCount);
         p.hit_source = nullptr;
         p.histogram = reinterpret_cast<unsigned long long*>(d_meta.as<char>() + kMetaHist);
@@ -689,16 +742,74 @@ struct g4go_transport {
         p.counters = reinterpret_cast<unsigned long long*>(d_meta.as<char>() + kMetaCounters);
 
         const int block = static_cast<int>(launch.block_size);
-        int grid;
+        const uint32_t waves_per_block = launch.block_size / 64;
+        const auto zero_outputs = [&] {
+            if (launch_zero(d_meta.ptr, kMetaBytes, stream) != 0 ||
+                launch_zero(d_event_stats.ptr, d_event_stats.cap, stream) != 0) {
+                fail(G4GO_ERROR_RUNTIME, "zero kernel launch failed");
+            }
+        };
         if (launch.persistent) {
-            grid = static_cast<int>(std::max<uint32_t>(1, launch.blocks_per_cu) * static_cast<uint32_t>(cu_count));
+            const int grid = static_cast<int>(std::max<uint32_t>(1, launch.blocks_per_cu) * static_cast<uint32_t>(cu_count));
+            if (launch.graph) {
+                // zero kernels + transport kernel, captured once per parameter set
+                if (graph_exec == nullptr || std::memcmp(&graph_params, &p, sizeof(p)) != 0 || graph_grid != grid ||
+                    graph_block != block || graph_source != source) {
+                    drop_graph();
+                    hip_check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+                    bool ok = launch_zero(d_meta.ptr, kMetaBytes, stream) == 0 &&
+                              launch_zero(d_event_stats.ptr, d_event_stats.cap, stream) == 0 &&
+                              launch_transport(p, source, true, block, grid, stream) == 0;
+                    const hipError_t end = hipStreamEndCapture(stream, &graph);
+                    if (!ok || end != hipSuccess) {
+                        drop_graph();
+                        fail(G4GO_ERROR_RUNTIME, "graph capture of the transport launch failed");
+                    }
+                    hip_check(hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+                    graph_params = p;
+                    graph_grid = grid;
+                    graph_block = block;
+                    graph_source = source;
+                }
+                hip_check(hipEventRecord(ev_kernel0, stream), "hipEventRecord");
+                hip_check(hipGraphLaunch(graph_exec, stream), "hipGraphLaunch");
+            } else {
+                zero_outputs();
+                hip_check(hipEventRecord(ev_kernel0, stream), "hipEventRecord");
+                if (launch_transport(p, source, true, block, grid, stream) != 0) {
+                    fail(G4GO_ERROR_RUNTIME, "transport kernel launch failed");
+                }
+            }
         } else {
-            const uint32_t waves_per_block = launch.block_size / 64;
-            grid = static_cast<int>((static_cast<uint64_t>(n_items) + waves_per_block - 1) / waves_per_block);
-        }
-        hip_check(hipEventRecord(ev_kernel0, stream), "hipEventRecord");
-        if (grid > 0 && launch_transport(p, source, launch.persistent != 0, block, grid, stream) != 0) {
-            fail(G4GO_ERROR_RUNTIME, "transport kernel launch failed");
+            zero_outputs();
+            hip_check(hipEventRecord(ev_kernel0, stream), "hipEventRecord");
+            if (!launch.per_event) {
+                const int grid = static_cast<int>((static_cast<uint64_t>(n_items) + waves_per_block - 1) / waves_per_block);
+                if (grid > 0 && launch_transport(p, source, false, block, grid, stream) != 0) {
+                    fail(G4GO_ERROR_RUNTIME, "transport kernel launch failed");
+                }
+            } else {
+                // one launch per event: runs of consecutive sources of the same event
+                uint64_t s0 = 0;
+                while (s0 < n_sources) {
+                    uint64_t s1 = s0 + 1;
+                    const uint32_t ev = source_event.empty() ? static_cast<uint32_t>(s0) : source_event[s0];
+                    while (!source_event.empty() && s1 < n_sources && source_event[s1] == ev) {
+                        ++s1;
+                    }
+                    KernelParams q = p;
+                    q.item_begin = offsets[s0];
+                    q.item_end = offsets[s1];
+                    const uint32_t items = q.item_end - q.item_begin;
+                    if (items != 0) {
+                        const int grid = static_cast<int>((items + waves_per_block - 1) / waves_per_block);
+                        if (launch_transport(q, source, false, block, grid, stream) != 0) {
+                            fail(G4GO_ERROR_RUNTIME, "transport kernel launch failed");
+                        }
+                    }
+                    s0 = s1;
+                }
+            }
         }
         hip_check(hipEventRecord(ev_kernel1, stream), "hipEventRecord");
         hip_check(hipMemcpyAsync(h_meta.ptr, d_meta.ptr, kMetaBytes, hipMemcpyDeviceToHost, stream),
@@ -747,6 +858,9 @@ void check_launch(const g4go_launch& l) {
     if (l.refill_threshold == 0 || l.refill_threshold > 64) {
         fail(G4GO_ERROR_INVALID_ARGUMENT, "refill_threshold must be in [1, 64]");
     }
+    if (l.per_event > 1 || l.graph > 1 || (l.per_event && l.persistent) || (l.graph && !l.persistent)) {
+        fail(G4GO_ERROR_INVALID_ARGUMENT, "per_event applies to the plain form and graph to the persistent form");
+    }
     if (l.persistent && (l.blocks_per_cu == 0 || l.blocks_per_cu > 64)) {
         fail(G4GO_ERROR_INVALID_ARGUMENT, "blocks_per_cu must be in [1, 64]");
     }
@@ -775,7 +889,7 @@ g4go_status g4go_create(const g4go_config* config, g4go_transport** out) {
     try {
         auto* h = new g4go_transport();
         h->cfg = *config;
-        h->launch = g4go_launch{0, 256, 64, 64, 8};
+        h->launch = g4go_launch{0, 256, 64, 64, 8, 0, 0};
         *out = h;
         return G4GO_OK;
     } catch (...) {

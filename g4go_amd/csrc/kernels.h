@@ -47,12 +47,12 @@ struct KernelParams {
     uint32_t refill_threshold;  // persistent form: refill when at least this many lanes idle
     SyntheticSource synth;
     const Emission* emissions;
+    const uint32_t* header;          // device-resident batch header: {n_sources, chunk, n_events, 0}
     const uint32_t* event_ids;       // per event index
-    const uint32_t* item_offsets;    // n_sources + 1, device resident (item count = item_offsets[n_sources])
+    const uint32_t* item_offsets;    // n_sources + 1 (item count = item_offsets[n_sources])
     const uint32_t* source_counts;   // photons per source record
     const uint32_t* source_event;    // event index per source record (emission source)
-    uint32_t n_sources;
-    uint32_t chunk;
+    uint32_t item_begin, item_end;   // item range of this launch; item_end == 0: the whole batch
     Hit* hits;
     uint32_t hit_capacity;
     uint32_t* hit_count;

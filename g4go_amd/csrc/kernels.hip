@@ -106,8 +106,9 @@ __device__ __forceinline__ int hist_slot(double t, double lo, double width) {
 
 // Locates work item c (wave-uniform): binary search of the item offsets.
 template <int kSource>
-__device__ __forceinline__ ItemRef locate_item(const KernelParams& p, uint32_t c) {
-    uint32_t lo = 0, hi = p.n_sources;  // last s with item_offsets[s] <= c
+__device__ __forceinline__ ItemRef locate_item(const KernelParams& p, uint32_t c, uint32_t n_sources,
+                                               uint32_t chunk) {
+    uint32_t lo = 0, hi = n_sources;  // last s with item_offsets[s] <= c
     while (hi - lo > 1u) {
         const uint32_t mid = uniform((lo + hi) >> 1);
         if (p.item_offsets[mid] <= c) {
@@ -118,8 +119,8 @@ __device__ __forceinline__ ItemRef locate_item(const KernelParams& p, uint32_t c
     }
     ItemRef r;
     r.source_index = uniform(lo);
-    r.first = uniform((c - p.item_offsets[lo]) * p.chunk);
-    r.count = uniform(min(p.chunk, p.source_counts[lo] - r.first));
+    r.first = uniform((c - p.item_offsets[lo]) * chunk);
+    r.count = uniform(min(chunk, p.source_counts[lo] - r.first));
     r.event_index = kSource == kSourceSynthetic ? r.source_index : uniform(p.source_event[lo]);
     return r;
 }
@@ -196,7 +197,10 @@ __global__ void __launch_bounds__(256) transport_kernel(KernelParams p) {
 
     const uint32_t lane = lane_id();
     const uint64_t below = lanes_below();
-    const uint32_t n_items = p.item_offsets[p.n_sources];
+    const uint32_t n_sources = uniform(p.header[0]);
+    const uint32_t chunk = uniform(p.header[1]);
+    const uint32_t first_item = p.item_end != 0u ? p.item_begin : 0u;
+    const uint32_t end_item = p.item_end != 0u ? p.item_end : uniform(p.item_offsets[n_sources]);
 
     Photon ph;
     ph.material = kInvalid;
@@ -210,9 +214,9 @@ __global__ void __launch_bounds__(256) transport_kernel(KernelParams p) {
     uint32_t it_index = kInvalid, it_event = 0, it_source = 0, it_first = 0, it_next = 0, it_end = 0;
     bool done = false;
     if (!kPersistent) {
-        const uint32_t wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-        if (wave < n_items) {
-            const ItemRef item = locate_item<kSource>(p, uniform(wave));
+        const uint32_t wave = first_item + blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+        if (wave < end_item) {
+            const ItemRef item = locate_item<kSource>(p, uniform(wave), n_sources, chunk);
             it_index = wave;
             it_event = item.event_index;
             it_source = item.source_index;
@@ -236,12 +240,12 @@ __global__ void __launch_bounds__(256) transport_kernel(KernelParams p) {
                     if (lane == 0u) {
                         c = atomicAdd(reinterpret_cast<uint32_t*>(p.counters + kCounterQueueHead), 1u);
                     }
-                    c = uniform(static_cast<uint32_t>(__shfl(static_cast<int>(c), 0)));
-                    if (c >= n_items) {
+                    c = first_item + uniform(static_cast<uint32_t>(__shfl(static_cast<int>(c), 0)));
+                    if (c >= end_item) {
                         done = true;
                         break;
                     }
-                    const ItemRef item = locate_item<kSource>(p, c);
+                    const ItemRef item = locate_item<kSource>(p, c, n_sources, chunk);
                     it_index = c;
                     it_event = item.event_index;
                     it_source = item.source_index;

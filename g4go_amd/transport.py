@@ -59,10 +59,12 @@ class LaunchForm:
     chunk: int = 64
     refill_threshold: int = 64
     blocks_per_cu: int = 8
+    per_event: bool = False
+    graph: bool = False
 
     def to_abi(self) -> abi.Launch:
         return abi.Launch(int(self.persistent), self.block_size, self.chunk, self.refill_threshold,
-                          self.blocks_per_cu)
+                          self.blocks_per_cu, int(self.per_event), int(self.graph))
 
 
 @dataclass
@@ -117,7 +119,7 @@ class HipTransportHost:
         form = abi.Launch()
         self._check(self._lib.g4go_get_launch(self._h, C.byref(form)))
         return LaunchForm(bool(form.persistent), form.block_size, form.chunk, form.refill_threshold,
-                          form.blocks_per_cu)
+                          form.blocks_per_cu, bool(form.per_event), bool(form.graph))
 
     # ---- the reference interface ----
     def prepare_scene(self, scene: Scene) -> None:

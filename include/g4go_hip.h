@@ -55,17 +55,23 @@ typedef struct g4go_config {
     uint32_t sort_hits;    /* 1: detections returned in (eventID, photonID) order */
 } g4go_config;
 
-/* Launch form.  persistent=0: one wave per work item of <= 64 photons over a
- * grid covering the batch.  persistent=1: `grid_blocks` blocks (0 = 2 per CU
- * per wave slot, see blocks_per_cu) pop work items of `chunk` photons from a
- * device queue and refill idle lanes when at least refill_threshold of 64 are
- * idle. */
+/* Launch form.
+ *   persistent=0: plain form, one wave per work item of <= 64 photons over a
+ *     grid covering the batch; with per_event=1, one such launch per event
+ *     (the per-event launches a caller without batching would make).
+ *   persistent=1: blocks_per_cu x CU-count blocks pop work items of `chunk`
+ *     photons from a device queue and refill idle lanes once at least
+ *     refill_threshold of the 64 lanes are idle; with graph=1 the launch is
+ *     captured once in a hipGraph and replayed for every batch whose buffers
+ *     still fit. */
 typedef struct g4go_launch {
     uint32_t persistent;
     uint32_t block_size;       /* 64, 128, 256 */
     uint32_t chunk;            /* photons per work item (persistent form) */
     uint32_t refill_threshold; /* 1..64 */
     uint32_t blocks_per_cu;    /* persistent grid = blocks_per_cu * CU count */
+    uint32_t per_event;        /* plain form: one launch per event */
+    uint32_t graph;            /* persistent form: replay a captured hipGraph */
 } g4go_launch;
 
 /* A property table: `count` (energy eV, value) pairs with ascending energies;

@@ -182,3 +182,29 @@ def test_photon_trace_invariants(crystal):
                 np.testing.assert_array_equal(t[-1, 0:3], rec["position_mm"][0])
             else:
                 assert pid not in set(ref.hits["photon_id"].tolist())
+
+
+def test_batched_graph_replays_and_per_event_launches(crystal):
+    """Uneven events: the persistent form replayed from one captured hipGraph
+    over several different batches (with eager work in between) and the plain
+    per-event launches give the same results as an eager plain launch."""
+    log("case: uneven events, graph replays and per-event launches")
+    src = workloads.synthetic_source(crystal)
+    batches = [workloads.lognormal_counts(n, 300, sigma=1.5, lo=1, hi=20000, seed=s) for n, s in
+               ((40, 1), (25, 2), (40, 3))]
+    with host(5) as eager, host(5) as graphed, host(5) as per_event:
+        graphed.set_launch(LaunchForm(persistent=True, block_size=256, chunk=256, refill_threshold=16,
+                                      blocks_per_cu=4, graph=True))
+        per_event.set_launch(LaunchForm(persistent=False, per_event=True))
+        for k, counts in enumerate(batches):
+            ev = np.arange(100 * k, 100 * k + counts.size, dtype=np.uint32)
+            eager.enqueue_synthetic(crystal.scene, src, ev, counts)
+            want = eager.complete_oldest_batch()
+            for h in (graphed, per_event):
+                h.enqueue_synthetic(crystal.scene, src, ev, counts)
+                got = h.complete_oldest_batch()
+                assert_same_hits(got.detections, want.detections)
+                assert got.event_statistics.tobytes() == want.event_statistics.tobytes()
+                np.testing.assert_array_equal(got.histogram, want.histogram)
+        ref = O.run_synthetic(crystal.scene, src, 5, ev, batches[-1], threads=4)
+    assert_same_hits(want.detections, ref.hits)
