@@ -65,31 +65,29 @@ __device__ __forceinline__ void flush_event(EventStats* es, uint32_t event, Lane
     s = LaneStats{0, 0, 0, 0, 0, 0, 0};
 }
 
-// End-of-kernel flush: one set of atomics per wave when every lane with
-// counts holds the same event, else per lane.
-__device__ __forceinline__ void flush_event_final(EventStats* es, uint32_t event, LaneStats& s) {
-    const uint64_t has = __ballot(event != kInvalid);
-    if (has == 0) {
-        return;
-    }
-    const uint32_t leader = static_cast<uint32_t>(__ffsll(static_cast<long long>(has)) - 1);
-    const uint32_t e0 = static_cast<uint32_t>(__shfl(static_cast<int>(event), static_cast<int>(leader)));
-    const uint64_t same = __ballot(event == e0 || event == kInvalid);
-    if (same == ~0ull) {
+// Flushes the counters of the lanes in `needs` to their events with one set
+// of atomics per distinct event among them (wave-uniform control flow).
+__device__ __forceinline__ void flush_event_group(EventStats* es, uint32_t event, LaneStats& s, bool needs) {
+    uint64_t pending = __ballot(needs);
+    while (pending != 0ull) {
+        const uint32_t leader = static_cast<uint32_t>(__ffsll(static_cast<long long>(pending)) - 1);
+        const uint32_t e = uniform(static_cast<uint32_t>(__shfl(static_cast<int>(event), static_cast<int>(leader))));
+        const bool member = needs && event == e;
         LaneStats r;
-        r.detected = wave_sum(s.detected);
-        r.absorbed = wave_sum(s.absorbed);
-        r.escaped = wave_sum(s.escaped);
-        r.truncated = wave_sum(s.truncated);
-        r.invalid = wave_sum(s.invalid);
-        r.zero_step = wave_sum(s.zero_step);
-        r.max_bounce = wave_max(s.max_bounce);
+        r.detected = wave_sum(member ? s.detected : 0u);
+        r.absorbed = wave_sum(member ? s.absorbed : 0u);
+        r.escaped = wave_sum(member ? s.escaped : 0u);
+        r.truncated = wave_sum(member ? s.truncated : 0u);
+        r.invalid = wave_sum(member ? s.invalid : 0u);
+        r.zero_step = wave_sum(member ? s.zero_step : 0u);
+        r.max_bounce = wave_max(member ? s.max_bounce : 0u);
         if (lane_id() == leader) {
-            flush_event(es, e0, r);
+            flush_event(es, e, r);
         }
-        s = LaneStats{0, 0, 0, 0, 0, 0, 0};
-    } else {
-        flush_event(es, event, s);
+        if (member) {
+            s = LaneStats{0, 0, 0, 0, 0, 0, 0};
+        }
+        pending &= ~__ballot(member);
     }
 }
 
@@ -256,11 +254,10 @@ __global__ void __launch_bounds__(256) transport_kernel(KernelParams p) {
                 }
                 const uint32_t take = min(it_end - it_next, n_idle);
                 const uint32_t rank = static_cast<uint32_t>(__popcll(idle & below));
-                if (!alive && rank < take) {
-                    if (my_event != it_event) {
-                        flush_event(p.event_stats, my_event, st);
-                        my_event = it_event;
-                    }
+                const bool fill = !alive && rank < take;
+                flush_event_group(p.event_stats, my_event, st, fill && my_event != it_event && my_event != kInvalid);
+                if (fill) {
+                    my_event = it_event;
                     generate<kSource>(ph, p, it_event, it_source, it_first + it_next + rank);
                     ph.material = kInvalid;
                     my_item = it_index;
@@ -332,7 +329,7 @@ __global__ void __launch_bounds__(256) transport_kernel(KernelParams p) {
     if (staged != 0u) {
         flush_stage(p, stage, stage_src, staged, lane);
     }
-    flush_event_final(p.event_stats, my_event, st);
+    flush_event_group(p.event_stats, my_event, st, my_event != kInvalid);
     const uint32_t ws = wave_sum(lane_steps);
     const uint32_t wp = wave_sum(lane_photons);
     if (lane == 0u) {
