@@ -1,0 +1,74 @@
+"""Runs one launch of a named workload, for profilers (rocprofv3 passes).
+
+    python -m g4go_amd.profile_run --workload crystal --photons 100000000
+    python -m g4go_amd.profile_run --workload uneven
+    python -m g4go_amd.profile_run --copy-bytes 1073741824   # calibration copy
+
+The launch form is the cached autotuner result for the device when present,
+else the default persistent form.  --copy-bytes additionally runs one device
+to device copy of that many bytes (hipMemcpy through torch), whose FETCH_SIZE
+and WRITE_SIZE calibrate the traffic counters against a known byte count.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import sys
+
+import numpy as np
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--workload", choices=("crystal", "uneven", "array", "none"), default="crystal")
+    ap.add_argument("--photons", type=int, default=100_000_000)
+    ap.add_argument("--events", type=int, default=4096)
+    ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--form", choices=("tuned", "plain"), default="tuned")
+    ap.add_argument("--copy-bytes", type=int, default=0)
+    ap.add_argument("--repeat", type=int, default=1)
+    args = ap.parse_args(argv)
+
+    from . import autotune, workloads
+    from .transport import HipTransportHost, LaunchForm, PhotonTransportConfig
+
+    if args.copy_bytes:
+        import torch
+
+        a = torch.empty(args.copy_bytes // 4, dtype=torch.int32, device="cuda")
+        b = torch.empty_like(a)
+        a.fill_(1)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": args.copy_bytes}), flush=True)
+    if args.workload == "none":
+        return 0
+    cs = workloads.array_scene() if args.workload == "array" else workloads.crystal_scene()
+    src = workloads.synthetic_source(cs, crystal_index=27 if args.workload == "array" else 0)
+    host = HipTransportHost(PhotonTransportConfig(seed=args.seed, sort_hits=False))
+    device, _ = host.device_info()
+    form = autotune.cached(device, "crystal-synthetic") or LaunchForm(persistent=True, block_size=256, chunk=256,
+                                                                      refill_threshold=16, blocks_per_cu=4)
+    if args.form == "plain":
+        form = LaunchForm(persistent=False)
+    if args.workload == "uneven":
+        counts = workloads.lognormal_counts(args.events, 5000, sigma=1.5, lo=10, hi=1_000_000, seed=args.seed)
+        form = LaunchForm(**{**form.__dict__, "graph": form.persistent})
+    else:
+        counts = np.array([args.photons], np.uint32)
+    host.set_launch(form)
+    ev = np.arange(counts.size, dtype=np.uint32)
+    for _ in range(args.repeat):
+        host.enqueue_synthetic(cs.scene, src, ev, counts)
+        out, _ = host.complete_device()
+        st = out.statistics
+        print(json.dumps({"workload": args.workload, "photons": int(counts.astype(np.int64).sum()),
+                          "kernel_ms": st["kernel_ms"], "steps": st["total_steps"], "detected": st["detected"],
+                          "launch": form.__dict__}), flush=True)
+    host.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
