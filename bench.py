@@ -105,6 +105,16 @@ def cpu_baseline(cs, src, seed, sample):
                       f"{threads} threads, {dt:.2f} s wall"}
 
 
+def profiled_counters(photons):
+    """Calibrated rocprofv3 counters of this workload (profiles/), or None."""
+    path = os.path.join(ROOT, "profiles", "transport_kernel_crystal_1e8_counters.json")
+    if not os.path.isfile(path):
+        return None
+    with open(path) as f:
+        data = json.load(f)
+    return data if data.get("photons") == photons else None
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -170,7 +180,9 @@ def main():
     k_ms = float(np.mean(kms))
     work = W_GEN * P + W_STEP * S + W_DET * D
     achieved = work / (k_ms * 1e-3) / 1e12
-    alg_bytes = HIT_BYTES * D + 4 * 258 * 2
+    alg_bytes = HIT_BYTES * D + 8 * 258 * 2
+    prof = profiled_counters(args.photons)
+    traffic = None if prof is None else prof["hbm_bytes_per_launch"]
     result = {
         "metric": "optical photons/sec on 1 MI355X (batched vs plain) + adjoint/forward cost ratio",
         "value": args.photons * args.steps / wall,
@@ -197,9 +209,14 @@ def main():
         "counts": {"photons": P, "steps": S, "detected": D, "steps_per_photon": S / max(P, 1),
                    "detected_fraction": D / max(P, 1)},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": VALU_PEAK_TLANEOPS, "unit": "Tlane-op/s",
-                     "frac": achieved / VALU_PEAK_TLANEOPS, "traffic": None,
+                     "frac": achieved / VALU_PEAK_TLANEOPS, "traffic": traffic,
                      "kernel": "transport_kernel<synthetic, persistent>", "kernel_ms": k_ms,
                      "work_model": {"per_photon": W_GEN, "per_step": W_STEP, "per_detection": W_DET}},
+        "valu_counters": None if prof is None else {
+            "source": "profiles/transport_kernel_crystal_1e8_counters.md",
+            "VALUBusy_pct": prof["valu"]["VALUBusy"], "VALUUtilization_pct": prof["valu"]["VALUUtilization"],
+            "active_lane_ops_per_step": prof["sq"]["SQ_INSTS_VALU"] * 64 * prof["valu"]["VALUUtilization"] / 100
+            / prof["steps"]},
         "roofline_hbm": {"achieved": alg_bytes / (k_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": alg_bytes / (k_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "bytes_per_launch": alg_bytes},
     }
