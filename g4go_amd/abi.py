@@ -147,6 +147,18 @@ class SyntheticSource(C.Structure):
     ]
 
 
+class ParamTable(C.Structure):
+    _fields_ = [("kind", C.c_uint32), ("index", C.c_uint32)]
+
+
+class AdjointStats(C.Structure):
+    _fields_ = [("replayed", C.c_uint64), ("mismatched", C.c_uint64), ("unsupported", C.c_uint64),
+                ("steps", C.c_uint64), ("kernel_ms", C.c_double)]
+
+
+PARAM_ABS_LENGTH, PARAM_RAYLEIGH_LENGTH, PARAM_REFLECTIVITY, PARAM_EFFICIENCY = 0, 1, 2, 3
+
+
 class Hit(C.Structure):
     _fields_ = [
         ("position_mm", C.c_float * 3),
@@ -239,7 +251,8 @@ assert C.sizeof(Config) == 40 and C.sizeof(Launch) == 28
 EXPORTED_SYMBOLS = (
     "g4go_create", "g4go_destroy", "g4go_last_error", "g4go_set_launch", "g4go_get_launch",
     "g4go_prepare_scene", "g4go_enqueue_emissions", "g4go_enqueue_synthetic", "g4go_complete",
-    "g4go_complete_device", "g4go_eval_math", "g4go_eval_fresnel", "g4go_device_info", "g4go_trace_synthetic", "g4go_version",
+    "g4go_complete_device", "g4go_eval_math", "g4go_eval_fresnel", "g4go_device_info", "g4go_trace_synthetic", "g4go_set_parameter_layout", "g4go_get_parameters",
+    "g4go_set_parameters", "g4go_adjoint", "g4go_version",
 )
 
 _lib = None
@@ -271,9 +284,14 @@ def load() -> C.CDLL:
     lib.g4go_device_info.argtypes = [H, C.c_char_p, C.c_uint64, C.POINTER(C.c_int32)]
     lib.g4go_trace_synthetic.argtypes = [H, C.POINTER(SyntheticSource), C.c_uint32, C.c_uint32, C.c_uint32,
                                          C.c_void_p, C.POINTER(C.c_uint32)]
+    lib.g4go_set_parameter_layout.argtypes = [H, C.POINTER(ParamTable), C.c_uint32, C.POINTER(C.c_uint32)]
+    lib.g4go_get_parameters.argtypes = [H, C.c_void_p, C.c_uint64]
+    lib.g4go_set_parameters.argtypes = [H, C.c_void_p, C.c_uint64]
+    lib.g4go_adjoint.argtypes = [H, C.c_void_p, C.c_void_p, C.c_uint64, C.POINTER(AdjointStats)]
     lib.g4go_version.restype = C.c_char_p
     for name in ("g4go_create", "g4go_set_launch", "g4go_get_launch", "g4go_prepare_scene", "g4go_enqueue_emissions",
-                 "g4go_enqueue_synthetic", "g4go_complete", "g4go_complete_device", "g4go_eval_math", "g4go_eval_fresnel", "g4go_device_info", "g4go_trace_synthetic"):
+                 "g4go_enqueue_synthetic", "g4go_complete", "g4go_complete_device", "g4go_eval_math", "g4go_eval_fresnel", "g4go_device_info", "g4go_trace_synthetic",
+                 "g4go_set_parameter_layout", "g4go_get_parameters", "g4go_set_parameters", "g4go_adjoint"):
         getattr(lib, name).restype = C.c_int
     _lib = lib
     return lib

@@ -508,6 +508,17 @@ struct g4go_transport {
     uint64_t h2d_bytes = 0;
 
     DevBuf d_inputs, d_hits, d_event_stats, d_meta;
+    // differentiation
+    struct ParamTable {
+        uint32_t kind, index, offset, count;
+    };
+    std::vector<ParamTable> param_tables;
+    uint32_t n_params = 0;
+    std::vector<uint32_t> param_map;  // [abs(n_mat)][rayleigh(n_mat)][reflectivity(n_surf)][efficiency(n_surf)]
+    DevBuf d_param_map, d_hit_source, d_adjoint;
+    KernelParams last_params{};
+    int last_source = -1;
+    bool last_valid = false;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     KernelParams graph_params{};
@@ -595,6 +606,9 @@ struct g4go_transport {
             (void)hipStreamSynchronize(stream);
         }
         drop_graph();
+        d_param_map.release();
+        d_hit_source.release();
+        d_adjoint.release();
         d_scene.release();
         d_inputs.release();
         d_hits.release();
@@ -737,9 +751,16 @@ struct g4go_transport {
         p.hit_capacity = static_cast<uint32_t>(std::min<size_t>(d_hits.cap / sizeof(Hit), 0xffffffffu));
         p.hit_count = reinterpret_cast<uint32_t*>(d_meta.as<char>() + kMetaHitCount);
         p.hit_source = nullptr;
+        if (source == kSourceEmission && !param_tables.empty()) {
+            d_hit_source.ensure(std::max<uint64_t>(n_photons, 1) * sizeof(uint32_t), "hipMalloc hit sources");
+            p.hit_source = d_hit_source.as<uint32_t>();
+        }
         p.histogram = reinterpret_cast<unsigned long long*>(d_meta.as<char>() + kMetaHist);
         p.event_stats = d_event_stats.as<EventStats>();
         p.counters = reinterpret_cast<unsigned long long*>(d_meta.as<char>() + kMetaCounters);
+        last_params = p;
+        last_source = source;
+        last_valid = true;
 
         const int block = static_cast<int>(launch.block_size);
         const uint32_t waves_per_block = launch.block_size / 64;
@@ -941,6 +962,10 @@ g4go_status g4go_prepare_scene(g4go_transport* h, const g4go_scene* scene) {
         h->scene = std::move(s);
         h->scene_built = true;
         h->scene_uploaded = false;
+        h->param_tables.clear();
+        h->n_params = 0;
+        h->param_map.clear();
+        h->last_valid = false;
     });
 }
 
@@ -1010,6 +1035,7 @@ g4go_status g4go_enqueue_emissions(g4go_transport* h, const g4go_emission* emiss
         h->n_photons = photons;
         h->chunk = h->launch.persistent ? h->launch.chunk : 64u;
         h->started = std::chrono::steady_clock::now();
+        h->last_valid = false;
         if (photons != 0) {
             h->run_batch(staged.data(), n_emissions, counts, source_event, nullptr);
         }
@@ -1074,6 +1100,7 @@ g4go_status g4go_enqueue_synthetic(g4go_transport* h, const g4go_synthetic_sourc
         h->n_photons = photons;
         h->chunk = h->launch.persistent ? h->launch.chunk : 64u;
         h->started = std::chrono::steady_clock::now();
+        h->last_valid = false;
         if (photons != 0) {
             h->run_batch(nullptr, 0, h->captured_u32(), {}, &synth);
         }
@@ -1193,6 +1220,178 @@ g4go_status g4go_device_info(g4go_transport* h, char* name, uint64_t name_len, i
         std::strncpy(name, text.c_str(), name_len - 1);
         name[name_len - 1] = '\0';
         *cu_count = h->cu_count;
+    });
+}
+
+namespace {
+
+Prop* bound_prop(HostScene& s, uint32_t kind, uint32_t index) {
+    if (kind <= 1) {
+        if (index >= s.materials.size()) {
+            return nullptr;
+        }
+        return kind == 0 ? &s.materials[index].abs_length : &s.materials[index].rayleigh_length;
+    }
+    if (kind <= 3 && index < s.surfaces.size()) {
+        return kind == 2 ? &s.surfaces[index].reflectivity : &s.surfaces[index].efficiency;
+    }
+    return nullptr;
+}
+
+}  // namespace
+
+g4go_status g4go_set_parameter_layout(g4go_transport* h, const g4go_param_table* tables, uint32_t n_tables,
+                                      uint32_t* n_params) {
+    if (h == nullptr || (n_tables != 0 && tables == nullptr) || n_params == nullptr) {
+        return G4GO_ERROR_INVALID_ARGUMENT;
+    }
+    return guarded(h, [&] {
+        if (!h->scene_built) {
+            fail(G4GO_ERROR_LOGIC, "no scene has been prepared");
+        }
+        if (h->busy) {
+            fail(G4GO_ERROR_LOGIC, "the parameter layout cannot change with a pending batch");
+        }
+        const uint32_t n_mat = static_cast<uint32_t>(h->scene.materials.size());
+        const uint32_t n_surf = static_cast<uint32_t>(h->scene.surfaces.size());
+        std::vector<uint32_t> map(2 * n_mat + 2 * n_surf, kInvalid);
+        std::vector<g4go_transport::ParamTable> out;
+        uint32_t offset = 0;
+        for (uint32_t t = 0; t < n_tables; ++t) {
+            const Prop* prop = bound_prop(h->scene, tables[t].kind, tables[t].index);
+            if (prop == nullptr) {
+                fail(G4GO_ERROR_INVALID_ARGUMENT, "parameter table " + std::to_string(t) + " names no scene table");
+            }
+            if (prop->count == 0) {
+                fail(G4GO_ERROR_INVALID_ARGUMENT, "parameter table " + std::to_string(t) + " is an absent property");
+            }
+            const uint32_t slot = tables[t].kind <= 1 ? tables[t].kind * n_mat + tables[t].index
+                                                      : 2 * n_mat + (tables[t].kind - 2) * n_surf + tables[t].index;
+            if (map[slot] != kInvalid) {
+                fail(G4GO_ERROR_INVALID_ARGUMENT, "parameter table bound twice");
+            }
+            map[slot] = offset;
+            out.push_back({tables[t].kind, tables[t].index, offset, prop->count});
+            offset += prop->count;
+        }
+        h->param_tables = std::move(out);
+        h->param_map = std::move(map);
+        h->n_params = offset;
+        *n_params = offset;
+    });
+}
+
+g4go_status g4go_get_parameters(g4go_transport* h, double* theta, uint64_t n) {
+    if (h == nullptr || (n != 0 && theta == nullptr)) {
+        return G4GO_ERROR_INVALID_ARGUMENT;
+    }
+    return guarded(h, [&] {
+        if (n != h->n_params) {
+            fail(G4GO_ERROR_INVALID_ARGUMENT, "theta length does not match the parameter layout");
+        }
+        for (const auto& t : h->param_tables) {
+            const Prop* prop = bound_prop(h->scene, t.kind, t.index);
+            const float* values = h->scene.pool.data() + prop->offset + prop->count;
+            for (uint32_t k = 0; k < t.count; ++k) {
+                theta[t.offset + k] = values[k];
+            }
+        }
+    });
+}
+
+g4go_status g4go_set_parameters(g4go_transport* h, const double* theta, uint64_t n) {
+    if (h == nullptr || (n != 0 && theta == nullptr)) {
+        return G4GO_ERROR_INVALID_ARGUMENT;
+    }
+    return guarded(h, [&] {
+        if (n != h->n_params) {
+            fail(G4GO_ERROR_INVALID_ARGUMENT, "theta length does not match the parameter layout");
+        }
+        if (h->busy) {
+            fail(G4GO_ERROR_LOGIC, "parameters cannot change with a pending batch");
+        }
+        for (uint64_t k = 0; k < n; ++k) {
+            if (!std::isfinite(theta[k])) {
+                fail(G4GO_ERROR_INVALID_ARGUMENT, "theta holds a non-finite value");
+            }
+        }
+        for (const auto& t : h->param_tables) {
+            Prop* prop = bound_prop(h->scene, t.kind, t.index);
+            float* values = h->scene.pool.data() + prop->offset + prop->count;
+            bool constant = true;
+            for (uint32_t k = 0; k < t.count; ++k) {
+                values[k] = static_cast<float>(theta[t.offset + k]);
+                constant = constant && values[k] == values[0];
+            }
+            prop->constant = constant ? 1u : 0u;
+        }
+        h->scene_uploaded = false;
+        h->last_valid = false;
+    });
+}
+
+g4go_status g4go_adjoint(g4go_transport* h, const double* slot_weight, double* grad, uint64_t n,
+                         g4go_adjoint_stats* stats) {
+    if (h == nullptr || slot_weight == nullptr || (n != 0 && grad == nullptr)) {
+        return G4GO_ERROR_INVALID_ARGUMENT;
+    }
+    return guarded(h, [&] {
+        if (n != h->n_params || h->param_tables.empty()) {
+            fail(G4GO_ERROR_INVALID_ARGUMENT, "grad length does not match the parameter layout");
+        }
+        if (h->busy || !h->last_valid) {
+            fail(G4GO_ERROR_LOGIC, "the adjoint replays the last completed batch: complete it first");
+        }
+        for (int k = 0; k < kHistSlots; ++k) {
+            if (!std::isfinite(slot_weight[k])) {
+                fail(G4GO_ERROR_INVALID_ARGUMENT, "slot weights must be finite");
+            }
+        }
+        h->init_hip();
+        const size_t map_bytes = h->param_map.size() * sizeof(uint32_t);
+        // [map][slot weights 258 f64][grad n f64][counters 4 u64]
+        const size_t o_w = align256(map_bytes), o_g = o_w + align256(kHistSlots * sizeof(double));
+        const size_t o_c = o_g + align256(n * sizeof(double)), bytes = o_c + 64;
+        h->d_adjoint.ensure(bytes, "hipMalloc adjoint");
+        char* d = h->d_adjoint.as<char>();
+        hip_check(hipMemcpyAsync(d, h->param_map.data(), map_bytes, hipMemcpyHostToDevice, h->stream), "hipMemcpyAsync");
+        hip_check(hipMemcpyAsync(d + o_w, slot_weight, kHistSlots * sizeof(double), hipMemcpyHostToDevice, h->stream),
+                  "hipMemcpyAsync");
+        if (launch_zero(d + o_g, o_c + 64 - o_g, h->stream) != 0) {
+            fail(G4GO_ERROR_RUNTIME, "zero kernel launch failed");
+        }
+        AdjointParams a{};
+        a.fwd = h->last_params;
+        a.hits = h->d_hits.as<Hit>();
+        a.hit_item = h->last_source == kSourceEmission ? h->d_hit_source.as<uint32_t>() : nullptr;
+        a.n_hits = static_cast<uint32_t>(h->device_hits);
+        a.slot_weight = reinterpret_cast<const double*>(d + o_w);
+        a.grad = reinterpret_cast<double*>(d + o_g);
+        const uint32_t n_mat = static_cast<uint32_t>(h->scene.materials.size());
+        const uint32_t n_surf = static_cast<uint32_t>(h->scene.surfaces.size());
+        const auto* map = reinterpret_cast<const uint32_t*>(d);
+        a.map = ParamMap{map, map + n_mat, map + 2 * n_mat, map + 2 * n_mat + n_surf, h->n_params};
+        a.counters = reinterpret_cast<unsigned long long*>(d + o_c);
+        if (h->last_source == kSourceEmission && a.hit_item == nullptr) {
+            fail(G4GO_ERROR_LOGIC, "the batch ran before a parameter layout was set");
+        }
+        const int block = 256;
+        const int grid = static_cast<int>(std::max<uint64_t>(
+            1, std::min<uint64_t>((a.n_hits + block - 1) / block, static_cast<uint64_t>(h->cu_count) * 8)));
+        hip_check(hipEventRecord(h->ev_kernel0, h->stream), "hipEventRecord");
+        if (a.n_hits != 0 && launch_adjoint(a, h->last_source, block, grid, h->stream) != 0) {
+            fail(G4GO_ERROR_RUNTIME, "adjoint kernel launch failed");
+        }
+        hip_check(hipEventRecord(h->ev_kernel1, h->stream), "hipEventRecord");
+        uint64_t counters[4] = {0, 0, 0, 0};
+        hip_check(hipMemcpyAsync(grad, d + o_g, n * sizeof(double), hipMemcpyDeviceToHost, h->stream), "hipMemcpyAsync");
+        hip_check(hipMemcpyAsync(counters, d + o_c, sizeof(counters), hipMemcpyDeviceToHost, h->stream), "hipMemcpyAsync");
+        hip_check(hipStreamSynchronize(h->stream), "hipStreamSynchronize");
+        float ms = 0.0f;
+        hip_check(hipEventElapsedTime(&ms, h->ev_kernel0, h->ev_kernel1), "hipEventElapsedTime");
+        if (stats != nullptr) {
+            *stats = g4go_adjoint_stats{counters[0], counters[1], counters[2], counters[3], ms};
+        }
     });
 }
 

@@ -63,33 +63,32 @@ struct KernelParams {
     double hist_lo_ns, hist_width_ns;
 };
 
-// Adjoint (path replay) parameters: the forward parameters plus the loss
-// weights per histogram slot and the parameter layout.
-struct AdjointLayout {
-    uint32_t n_abs_materials;        // materials whose absorption table is parameterised
-    uint32_t abs_material[64];       // device material index per parameter group
-    uint32_t abs_nodes;              // nodes per absorption table
-    uint32_t rayleigh_material;      // material whose Rayleigh length is a parameter (kInvalid = none)
-    uint32_t reflect_surface;        // surface whose reflectivity is a parameter
-    uint32_t pde_surface;            // surface whose efficiency table is parameterised
-    uint32_t pde_nodes;
-    uint32_t offset_rayleigh, offset_reflect, offset_pde, n_params;
+// Parameterised property tables for the adjoint: theta offset of each
+// material's absorption / Rayleigh table and each surface's reflectivity /
+// efficiency table (kInvalid = not a parameter).  A table of n values
+// occupies theta[offset .. offset + n).
+struct ParamMap {
+    const uint32_t* material_abs;
+    const uint32_t* material_rayleigh;
+    const uint32_t* surface_reflectivity;
+    const uint32_t* surface_efficiency;
+    uint32_t n_params;
 };
 
 struct AdjointParams {
-    KernelParams fwd;
-    const Hit* hits;
-    const uint32_t* hit_item;  // work-item index per hit (emission source), or null (synthetic)
+    KernelParams fwd;          // the forward batch being replayed
+    const Hit* hits;           // its detections (device order)
+    const uint32_t* hit_item;  // work item of each detection (emission source)
     uint32_t n_hits;
     const double* slot_weight;  // dLoss/dN + dLoss/dh_b per histogram slot (kHistSlots)
-    double* grad;               // n_params
-    AdjointLayout layout;
-    unsigned long long* mismatch;  // replays whose path did not end in the recorded detection
+    double* grad;               // n_params, accumulated
+    ParamMap map;
+    unsigned long long* counters;  // [0] replayed, [1] path mismatches, [2] unsupported paths, [3] replay steps
 };
 
 int launch_transport(const KernelParams& p, int source, bool persistent, int block, int grid, hipStream_t stream);
 int launch_zero(void* ptr, size_t bytes, hipStream_t stream);
-int launch_adjoint(const AdjointParams& p, int source, int block, hipStream_t stream);
+int launch_adjoint(const AdjointParams& p, int source, int block, int grid, hipStream_t stream);
 int launch_trace(const KernelParams& p, uint32_t event_id, uint32_t photon_id, uint32_t max_steps, float* out,
                  uint32_t* n_out, hipStream_t stream);
 int launch_eval_math(const float* in, float* out, int n, int which, hipStream_t stream);

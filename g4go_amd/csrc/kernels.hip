@@ -378,6 +378,246 @@ int launch_transport(const KernelParams& p, int source, bool persistent, int blo
     return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 
+// ---- path-replay adjoint ----------------------------------------------------
+// Each detected photon of the forward batch is regenerated from its Philox
+// counters and transported again with the same arithmetic, so it follows the
+// recorded path bit for bit (checked against the recorded end point and
+// time).  Along the way the likelihood-ratio (score-function) derivatives of
+// the path density are collected:
+//   bulk segment of length s in material m, ended at a boundary or by a
+//     Rayleigh scatter: d log p / dL_abs = s / L_abs^2,
+//     d log p / dL_ray = s / L_ray^2 - [scatter] / L_ray;
+//   surface classified as interaction (reflection): d log p / dR = 1 / R;
+//   detection: absorption 1 - R - T then efficiency: d/dR = -1 / (1 - R - T),
+//     d/d eff = 1 / eff;
+// each chained to the two interpolation nodes of the table at the photon
+// energy and weighted by dLoss/dN + dLoss/dh_b of the photon's arrival-time
+// slot b.  The loss weight is known before the replay, so nothing is stored
+// per step.  A path may visit up to two materials and two parameterised
+// reflecting surfaces; longer ones are counted as unsupported.
+constexpr uint32_t kAdjointLdsParams = 4096;
+
+namespace {
+
+struct MatSlot {
+    uint32_t material;
+    uint32_t rayleigh;
+    double length;
+};
+
+struct SurfSlot {
+    uint32_t surface;
+    uint32_t count;
+};
+
+__device__ __forceinline__ void add_table_grad(double* grad, const float* pool, const Prop& prop, uint32_t offset,
+                                               float e, double g) {
+    uint32_t i;
+    float w;
+    prop_node(pool, prop, e, i, w);
+    atomicAdd(grad + offset + i, g * (1.0 - static_cast<double>(w)));
+    if (w != 0.0f) {
+        atomicAdd(grad + offset + i + 1, g * static_cast<double>(w));
+    }
+}
+
+__device__ __forceinline__ bool add_segment(MatSlot& a, MatSlot& b, uint32_t material, float length, bool rayleigh) {
+    MatSlot* slot = a.material == material ? &a : (b.material == material ? &b : nullptr);
+    if (slot == nullptr) {
+        slot = a.material == kInvalid ? &a : (b.material == kInvalid ? &b : nullptr);
+        if (slot == nullptr) {
+            return false;
+        }
+        slot->material = material;
+    }
+    slot->length += static_cast<double>(length);
+    slot->rayleigh += rayleigh ? 1u : 0u;
+    return true;
+}
+
+__device__ __forceinline__ bool add_reflection(SurfSlot& a, SurfSlot& b, uint32_t surface) {
+    SurfSlot* slot = a.surface == surface ? &a : (b.surface == surface ? &b : nullptr);
+    if (slot == nullptr) {
+        slot = a.surface == kInvalid ? &a : (b.surface == kInvalid ? &b : nullptr);
+        if (slot == nullptr) {
+            return false;
+        }
+        slot->surface = surface;
+    }
+    ++slot->count;
+    return true;
+}
+
+// Per-lane item lookup (the forward's wave-uniform search, for arbitrary c).
+__device__ __forceinline__ uint32_t item_source(const KernelParams& p, uint32_t c, uint32_t n_sources) {
+    uint32_t lo = 0, hi = n_sources;
+    while (hi - lo > 1u) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (p.item_offsets[mid] <= c) {
+            lo = mid;
+        } else {
+            hi = mid;
+        }
+    }
+    return lo;
+}
+
+__device__ __forceinline__ void add_material_grads(double* grad, const SceneView& sc, const ParamMap& map,
+                                                   const MatSlot& m, float e, double weight) {
+    if (m.material == kInvalid || m.material >= sc.n_materials) {
+        return;
+    }
+    const Material& mat = sc.materials[m.material];
+    const uint32_t oa = map.material_abs[m.material];
+    if (oa != kInvalid && mat.abs_length.count != 0) {
+        const float la = prop_eval(sc.pool, mat.abs_length, e, kInf);
+        if (finitef(la) && la > 0.0f) {
+            const double l = la;
+            add_table_grad(grad, sc.pool, mat.abs_length, oa, e, weight * m.length / (l * l));
+        }
+    }
+    const uint32_t orr = map.material_rayleigh[m.material];
+    if (orr != kInvalid && mat.rayleigh_length.count != 0) {
+        const float lr = prop_eval(sc.pool, mat.rayleigh_length, e, kInf);
+        if (finitef(lr) && lr > 0.0f) {
+            const double l = lr;
+            add_table_grad(grad, sc.pool, mat.rayleigh_length, orr, e,
+                           weight * (m.length / (l * l) - static_cast<double>(m.rayleigh) / l));
+        }
+    }
+}
+
+}  // namespace
+
+template <int kSource>
+__global__ void __launch_bounds__(256) adjoint_kernel(AdjointParams a) {
+    extern __shared__ double s_grad[];
+    const uint32_t n_params = a.map.n_params;
+    const bool in_lds = n_params <= kAdjointLdsParams;
+    double* grad = in_lds ? s_grad : a.grad;
+    if (in_lds) {
+        for (uint32_t i = threadIdx.x; i < n_params; i += blockDim.x) {
+            s_grad[i] = 0.0;
+        }
+        __syncthreads();
+    }
+    const KernelParams& p = a.fwd;
+    const SceneView& sc = p.scene;
+    const uint32_t n_sources = p.header != nullptr ? p.header[0] : 0u;
+    uint32_t replayed = 0, mismatch = 0, unsupported = 0, steps = 0;
+    for (uint32_t h = blockIdx.x * blockDim.x + threadIdx.x; h < a.n_hits; h += gridDim.x * blockDim.x) {
+        const Hit hit = a.hits[h];
+        Photon ph;
+        if (kSource == kSourceSynthetic) {
+            generate_synthetic(ph, p.synth, p.rng, hit.event_id, hit.photon_id);
+        } else {
+            const uint32_t src = item_source(p, a.hit_item[h], n_sources);
+            const Emission& em = p.emissions[src];
+            generate_emission(ph, em, sc, p.rng, hit.photon_id - em.first_photon_id);
+        }
+        ph.material = kInvalid;
+        if (!start_photon(ph)) {
+            ++mismatch;
+            continue;
+        }
+        MatSlot m0{kInvalid, 0u, 0.0}, m1{kInvalid, 0u, 0.0};
+        SurfSlot r0{kInvalid, 0u}, r1{kInvalid, 0u};
+        bool ok = true;
+        int status = kAlive;
+        uint32_t sensor = kInvalid, det_surface = kInvalid;
+        for (;;) {
+            bool zero = false;
+            StepTrace tr;
+            status = transport_step<true>(ph, sc, p.rng, p.cfg, zero, sensor, &tr);
+            ++ph.step;
+            ++steps;
+            if (tr.segment > 0.0f || tr.event == kTraceRayleigh) {
+                ok = add_segment(m0, m1, tr.material, tr.segment, tr.event == kTraceRayleigh) && ok;
+            }
+            if (tr.event == kTraceInteraction && tr.surface < sc.n_surfaces &&
+                a.map.surface_reflectivity[tr.surface] != kInvalid) {
+                ok = add_reflection(r0, r1, tr.surface) && ok;
+            }
+            if (status == kDetected) {
+                det_surface = tr.surface;
+            }
+            if (status != kAlive || ph.step >= p.max_steps) {
+                break;
+            }
+        }
+        if (status != kDetected || ph.time != hit.time || ph.pos.x != hit.pos[0] || ph.pos.y != hit.pos[1] ||
+            ph.pos.z != hit.pos[2]) {
+            ++mismatch;
+            continue;
+        }
+        if (!ok) {
+            ++unsupported;
+            continue;
+        }
+        ++replayed;
+        const float e = ph.energy;
+        const double weight = a.slot_weight[hist_slot(hit.time, p.hist_lo_ns, p.hist_width_ns)];
+        add_material_grads(grad, sc, a.map, m0, e, weight);
+        add_material_grads(grad, sc, a.map, m1, e, weight);
+        const SurfSlot rs[2] = {r0, r1};
+        for (int k = 0; k < 2; ++k) {
+            if (rs[k].surface == kInvalid) {
+                continue;
+            }
+            const Surface& sf = sc.surfaces[rs[k].surface];
+            const float r = prop_eval(sc.pool, sf.reflectivity, e, 0.0f);
+            if (r > 0.0f && r < 1.0f) {
+                add_table_grad(grad, sc.pool, sf.reflectivity, a.map.surface_reflectivity[rs[k].surface], e,
+                               weight * static_cast<double>(rs[k].count) / static_cast<double>(r));
+            }
+        }
+        if (det_surface < sc.n_surfaces) {
+            const Surface& sf = sc.surfaces[det_surface];
+            const uint32_t oe = a.map.surface_efficiency[det_surface];
+            if (oe != kInvalid && sf.efficiency.count != 0) {
+                const float eff = prop_eval(sc.pool, sf.efficiency, e, 0.0f);
+                if (eff > 0.0f && eff < 1.0f) {
+                    add_table_grad(grad, sc.pool, sf.efficiency, oe, e, weight / static_cast<double>(eff));
+                }
+            }
+            const uint32_t orf = a.map.surface_reflectivity[det_surface];
+            if (orf != kInvalid && sf.reflectivity.count != 0) {
+                const float r = prop_eval(sc.pool, sf.reflectivity, e, 0.0f);
+                const float t = sf.transmittance.count != 0 ? clamp01(prop_eval(sc.pool, sf.transmittance, e, 0.0f)) : 0.0f;
+                const double absorb = 1.0 - static_cast<double>(r) - static_cast<double>(t);
+                if (r > 0.0f && r < 1.0f && absorb > 0.0) {
+                    add_table_grad(grad, sc.pool, sf.reflectivity, orf, e, -weight / absorb);
+                }
+            }
+        }
+    }
+    const uint32_t wr = wave_sum(replayed), wm = wave_sum(mismatch), wu = wave_sum(unsupported), ws = wave_sum(steps);
+    if (lane_id() == 0u) {
+        if (wr) atomicAdd(a.counters + 0, static_cast<unsigned long long>(wr));
+        if (wm) atomicAdd(a.counters + 1, static_cast<unsigned long long>(wm));
+        if (wu) atomicAdd(a.counters + 2, static_cast<unsigned long long>(wu));
+        if (ws) atomicAdd(a.counters + 3, static_cast<unsigned long long>(ws));
+    }
+    if (in_lds) {
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < n_params; i += blockDim.x) {
+            if (s_grad[i] != 0.0) {
+                atomicAdd(a.grad + i, s_grad[i]);
+            }
+        }
+    }
+}
+
+int launch_adjoint(const AdjointParams& p, int source, int block, int grid, hipStream_t stream) {
+    const size_t lds = p.map.n_params <= kAdjointLdsParams ? p.map.n_params * sizeof(double) : 0;
+    if (source == kSourceSynthetic) {
+        hipLaunchKernelGGL((adjoint_kernel<kSourceSynthetic>), dim3(grid), dim3(block), lds, stream, p);
+    } else {
+        hipLaunchKernelGGL((adjoint_kernel<kSourceEmission>), dim3(grid), dim3(block), lds, stream, p);
+    }
+    return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
 // ---- step-by-step trace of one synthetic photon (tests and diagnostics) ----
 // Per step, 16 floats: pos[3], dir[3], pol[3], time (float), status, zero,
 // step, volume, energy, 0.

@@ -125,12 +125,14 @@ G4GO_HD float prop_eval(const float* pool, const Prop& p, float e, float fallbac
     return va[i] + f * (va[i + 1] - va[i]);
 }
 
-// Interpolation node and weight used by prop_eval (for parameter gradients):
-// value = (1 - w) * va[i] + w * va[i + 1] in exact arithmetic.
+// Interpolation node and weight of prop_eval (for parameter gradients):
+// value = (1 - w) * va[i] + w * va[i + 1] in exact arithmetic.  The constant
+// shortcut is ignored: a parameterised table may hold equal values, and its
+// value is the same either way.
 G4GO_HD void prop_node(const float* pool, const Prop& p, float e, uint32_t& i, float& w) {
     const float* en = pool + p.offset;
     const uint32_t last = p.count - 1;
-    if (p.count <= 1 || p.constant != 0 || e <= en[0]) {
+    if (p.count <= 1 || e <= en[0]) {
         i = 0;
         w = 0.0f;
         return;

@@ -90,6 +90,7 @@ class HipTransportHost:
         self._h = handle
         self._scene: Optional[Scene] = None
         self._scene_keep: list = []
+        self._n_params = 0
 
     # ---- lifecycle ----
     def close(self) -> None:
@@ -195,6 +196,34 @@ class HipTransportHost:
         self._check(self._lib.g4go_trace_synthetic(self._h, C.byref(source), event_id, photon_id, max_steps,
                                                    out.ctypes.data, C.byref(n)))
         return out[:n.value]
+
+    # ---- differentiation ----
+    def set_parameter_layout(self, tables) -> int:
+        """Binds (kind, index) property tables to theta (kinds: abi.PARAM_*); returns len(theta)."""
+        arr = (abi.ParamTable * max(1, len(tables)))(*[abi.ParamTable(int(k), int(i)) for k, i in tables])
+        n = C.c_uint32()
+        self._check(self._lib.g4go_set_parameter_layout(self._h, arr, len(tables), C.byref(n)))
+        self._n_params = n.value
+        return n.value
+
+    def get_parameters(self) -> np.ndarray:
+        theta = np.zeros(self._n_params, np.float64)
+        self._check(self._lib.g4go_get_parameters(self._h, theta.ctypes.data, theta.size))
+        return theta
+
+    def set_parameters(self, theta) -> None:
+        theta = np.ascontiguousarray(theta, dtype=np.float64)
+        self._check(self._lib.g4go_set_parameters(self._h, theta.ctypes.data, theta.size))
+
+    def adjoint(self, slot_weight) -> tuple[np.ndarray, dict]:
+        """dLoss/dtheta of the last completed batch; slot_weight[258] = dLoss/dN + dLoss/dh_b."""
+        w = np.ascontiguousarray(slot_weight, dtype=np.float64)
+        if w.shape != (abi.HIST_SLOTS,):
+            raise ValueError("slot_weight must hold 258 values")
+        grad = np.zeros(self._n_params, np.float64)
+        st = abi.AdjointStats()
+        self._check(self._lib.g4go_adjoint(self._h, w.ctypes.data, grad.ctypes.data, grad.size, C.byref(st)))
+        return grad, {name: getattr(st, name) for name, _ in abi.AdjointStats._fields_}
 
     # ---- device evaluation of the shared physics (tests) ----
     def eval_math(self, which: int, x: np.ndarray) -> np.ndarray:

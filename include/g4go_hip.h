@@ -246,6 +246,42 @@ g4go_status g4go_complete(g4go_transport* handle, g4go_batch_result* out);
  * g4go_hit records in HBM (unordered), valid until the next enqueue. */
 g4go_status g4go_complete_device(g4go_transport* handle, g4go_batch_result* out, const void** hits_device);
 
+/* ---- differentiation (path-replay adjoint) ----
+ * A parameter layout binds property tables of the prepared scene to a flat
+ * parameter vector theta, table after table in the order given:
+ *   kind 0: absorption length of material `index` (all its values),
+ *   kind 1: Rayleigh length of material `index`,
+ *   kind 2: reflectivity of surface `index`,
+ *   kind 3: efficiency of surface `index`. */
+typedef struct g4go_param_table {
+    uint32_t kind;
+    uint32_t index;
+} g4go_param_table;
+
+typedef struct g4go_adjoint_stats {
+    uint64_t replayed;     /* detected photons replayed and scored */
+    uint64_t mismatched;   /* replays that did not end in their recorded detection (0 expected) */
+    uint64_t unsupported;  /* paths visiting more than two materials or parameterised surfaces */
+    uint64_t steps;        /* transport steps replayed */
+    double kernel_ms;
+} g4go_adjoint_stats;
+
+/* Binds tables to theta; *n_params receives the length of theta. */
+g4go_status g4go_set_parameter_layout(g4go_transport* handle, const g4go_param_table* tables, uint32_t n_tables,
+                                      uint32_t* n_params);
+/* Current values of theta from the scene. */
+g4go_status g4go_get_parameters(g4go_transport* handle, double* theta, uint64_t n_params);
+/* Writes theta into the bound tables (rounded to FP32) for the next batches. */
+g4go_status g4go_set_parameters(g4go_transport* handle, const double* theta, uint64_t n_params);
+/* Gradient of a loss of the last completed batch's detected count N and
+ * histogram h: slot_weight[G4GO_HIST_SLOTS] = dLoss/dN + dLoss/dh_b per
+ * arrival-time slot (bins 0..255, underflow, overflow).  Writes
+ * grad[n_params] = dLoss/dtheta (likelihood-ratio estimator over the batch's
+ * detected photons, replayed on the device from their Philox counters).
+ * Must follow g4go_complete or g4go_complete_device of that batch. */
+g4go_status g4go_adjoint(g4go_transport* handle, const double* slot_weight, double* grad, uint64_t n_params,
+                         g4go_adjoint_stats* stats);
+
 /* ---- test entry points (device evaluation of the shared physics) ----
  * which: 0 log(x), 1 exp(x), 2 sin/cos(2*pi*x) (two outputs per input). */
 g4go_status g4go_eval_math(g4go_transport* handle, int which, const float* in, float* out, uint64_t n);

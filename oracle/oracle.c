@@ -1561,3 +1561,187 @@ void ORACLE_NAME(normalize)(const real a[3], real out[3]) {
     out[1] = r.y;
     out[2] = r.z;
 }
+
+/* ------------------------------------------------------------------ */
+/* Path-replay likelihood-ratio gradient (the adjoint of the backend)  */
+/* ------------------------------------------------------------------ */
+static void or_prop_node(const prop_t* p, real e, uint32_t* i, real* w) {
+    if (p->n <= 1 || e <= p->e[0]) {
+        *i = 0;
+        *w = RF(0.0);
+        return;
+    }
+    const uint32_t last = p->n - 1;
+    if (e >= p->e[last]) {
+        *i = last;
+        *w = RF(0.0);
+        return;
+    }
+    uint32_t k = 0;
+    while (k + 1 < p->n && e > p->e[k + 1]) {
+        ++k;
+    }
+    *i = k;
+    *w = (e - p->e[k]) / (p->e[k + 1] - p->e[k]);
+}
+
+static void or_add_table(double* grad, const prop_t* p, uint32_t offset, real e, double g) {
+    uint32_t i;
+    real w;
+    or_prop_node(p, e, &i, &w);
+    grad[offset + i] += g * (1.0 - (double)w);
+    if (w != RF(0.0)) {
+        grad[offset + i + 1] += g * (double)w;
+    }
+}
+
+typedef struct {
+    uint32_t material, rayleigh;
+    double length;
+} mslot_t;
+
+typedef struct {
+    uint32_t surface, count;
+} sslot_t;
+
+static int or_add_segment(mslot_t* a, mslot_t* b, uint32_t m, real len, int rayleigh) {
+    mslot_t* s = a->material == m ? a : (b->material == m ? b : NULL);
+    if (!s) {
+        s = a->material == INVALID ? a : (b->material == INVALID ? b : NULL);
+        if (!s) {
+            return 0;
+        }
+        s->material = m;
+    }
+    s->length += (double)len;
+    s->rayleigh += rayleigh ? 1u : 0u;
+    return 1;
+}
+
+static int or_add_reflection(sslot_t* a, sslot_t* b, uint32_t sf) {
+    sslot_t* s = a->surface == sf ? a : (b->surface == sf ? b : NULL);
+    if (!s) {
+        s = a->surface == INVALID ? a : (b->surface == INVALID ? b : NULL);
+        if (!s) {
+            return 0;
+        }
+        s->surface = sf;
+    }
+    ++s->count;
+    return 1;
+}
+
+/* tables: n_tables (kind, index) pairs bound to theta in order (kind 0 absorption,
+ * 1 Rayleigh of a material; 2 reflectivity, 3 efficiency of a surface).
+ * counters: replayed, mismatched, unsupported, steps.  Returns 0, or -1 when the
+ * layout does not fit n_params. */
+int ORACLE_NAME(adjoint_synthetic)(const g4go_scene* scene, const g4go_synthetic_source* src, uint64_t seed, float eps,
+                                   uint32_t max_steps, const g4go_hit* hits, uint64_t n_hits,
+                                   const double* slot_weight, const uint32_t* tables, uint32_t n_tables, double* grad,
+                                   uint64_t n_params, uint64_t* counters) {
+    oscene_t* s = build(scene);
+    uint32_t* mabs = (uint32_t*)malloc(sizeof(uint32_t) * (s->n_mat + 1));
+    uint32_t* mray = (uint32_t*)malloc(sizeof(uint32_t) * (s->n_mat + 1));
+    uint32_t* srefl = (uint32_t*)malloc(sizeof(uint32_t) * (s->n_surf + 1));
+    uint32_t* seff = (uint32_t*)malloc(sizeof(uint32_t) * (s->n_surf + 1));
+    for (uint32_t i = 0; i <= s->n_mat; ++i) mabs[i] = mray[i] = INVALID;
+    for (uint32_t i = 0; i <= s->n_surf; ++i) srefl[i] = seff[i] = INVALID;
+    uint32_t offset = 0;
+    for (uint32_t t = 0; t < n_tables; ++t) {
+        const uint32_t kind = tables[2 * t], idx = tables[2 * t + 1];
+        const prop_t* p = NULL;
+        if (kind == 0 && idx < s->n_mat) { mabs[idx] = offset; p = &s->mat[idx].abslen; }
+        if (kind == 1 && idx < s->n_mat) { mray[idx] = offset; p = &s->mat[idx].rayleigh; }
+        if (kind == 2 && idx < s->n_surf) { srefl[idx] = offset; p = &s->surf[idx].refl; }
+        if (kind == 3 && idx < s->n_surf) { seff[idx] = offset; p = &s->surf[idx].eff; }
+        if (!p) { offset = 0xffffffffu; break; }
+        offset += p->n;
+    }
+    if (offset != n_params) {
+        free(mabs); free(mray); free(srefl); free(seff);
+        destroy(s);
+        return -1;
+    }
+    const uint32_t volume = vol_index(s, src->volume_id);
+    const rng_t r = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    for (uint64_t h = 0; h < n_hits; ++h) {
+        const g4go_hit* hit = &hits[h];
+        photon_t ph;
+        gen_synthetic(&ph, src, volume, &r, hit->event_id, hit->photon_id);
+        if (!start_photon(&ph)) { counters[1]++; continue; }
+        mslot_t m[2] = {{INVALID, 0, 0.0}, {INVALID, 0, 0.0}};
+        sslot_t rs[2] = {{INVALID, 0}, {INVALID, 0}};
+        int ok = 1, status = S_ALIVE, zero = 0;
+        uint32_t sensor = INVALID, det_surface = INVALID;
+        for (;;) {
+            trace_t tr;
+            status = or_step(&ph, s, &r, (real)eps, &zero, &sensor, &tr);
+            ph.step++;
+            counters[3]++;
+            if (tr.segment > RF(0.0) || tr.event == 1) {
+                ok = or_add_segment(&m[0], &m[1], tr.material, tr.segment, tr.event == 1) && ok;
+            }
+            if (tr.event == 2 && tr.surface < s->n_surf && srefl[tr.surface] != INVALID) {
+                ok = or_add_reflection(&rs[0], &rs[1], tr.surface) && ok;
+            }
+            if (status == S_DETECTED) det_surface = tr.surface;
+            if (status != S_ALIVE || ph.step >= max_steps) break;
+        }
+        if (status != S_DETECTED || ph.time != hit->time_ns || (float)ph.pos.x != hit->position_mm[0] ||
+            (float)ph.pos.y != hit->position_mm[1] || (float)ph.pos.z != hit->position_mm[2]) {
+            counters[1]++;
+            continue;
+        }
+        if (!ok) { counters[2]++; continue; }
+        counters[0]++;
+        const real e = ph.energy;
+        const double wgt = slot_weight[hist_slot(hit->time_ns)];
+        for (int k = 0; k < 2; ++k) {
+            if (m[k].material == INVALID || m[k].material >= s->n_mat) continue;
+            const material_t* mt = &s->mat[m[k].material];
+            if (mabs[m[k].material] != INVALID && mt->abslen.n) {
+                const real la = or_prop(&mt->abslen, e, R_INF);
+                if (finite_r(la) && la > RF(0.0)) {
+                    const double l = (double)la;
+                    or_add_table(grad, &mt->abslen, mabs[m[k].material], e, wgt * m[k].length / (l * l));
+                }
+            }
+            if (mray[m[k].material] != INVALID && mt->rayleigh.n) {
+                const real lr = or_prop(&mt->rayleigh, e, R_INF);
+                if (finite_r(lr) && lr > RF(0.0)) {
+                    const double l = (double)lr;
+                    or_add_table(grad, &mt->rayleigh, mray[m[k].material], e,
+                                 wgt * (m[k].length / (l * l) - (double)m[k].rayleigh / l));
+                }
+            }
+        }
+        for (int k = 0; k < 2; ++k) {
+            if (rs[k].surface == INVALID) continue;
+            const surface_t* sf = &s->surf[rs[k].surface];
+            const real rr = or_prop(&sf->refl, e, RF(0.0));
+            if (rr > RF(0.0) && rr < RF(1.0)) {
+                or_add_table(grad, &sf->refl, srefl[rs[k].surface], e, wgt * (double)rs[k].count / (double)rr);
+            }
+        }
+        if (det_surface < s->n_surf) {
+            const surface_t* sf = &s->surf[det_surface];
+            if (seff[det_surface] != INVALID && sf->eff.n) {
+                const real ef = or_prop(&sf->eff, e, RF(0.0));
+                if (ef > RF(0.0) && ef < RF(1.0)) {
+                    or_add_table(grad, &sf->eff, seff[det_surface], e, wgt / (double)ef);
+                }
+            }
+            if (srefl[det_surface] != INVALID && sf->refl.n) {
+                const real rr = or_prop(&sf->refl, e, RF(0.0));
+                const real tt = sf->trans.n ? clamp01(or_prop(&sf->trans, e, RF(0.0))) : RF(0.0);
+                const double ab = 1.0 - (double)rr - (double)tt;
+                if (rr > RF(0.0) && rr < RF(1.0) && ab > 0.0) {
+                    or_add_table(grad, &sf->refl, srefl[det_surface], e, -wgt / ab);
+                }
+            }
+        }
+    }
+    free(mabs); free(mray); free(srefl); free(seff);
+    destroy(s);
+    return 0;
+}

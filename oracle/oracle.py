@@ -56,6 +56,9 @@ def _load(precision: int) -> C.CDLL:
                                              C.c_void_p, C.c_void_p]
     getattr(lib, p + "box_exit").argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(real),
                                              C.POINTER(C.c_int)]
+    getattr(lib, p + "adjoint_synthetic").argtypes = [
+        C.POINTER(abi.Scene), C.POINTER(abi.SyntheticSource), C.c_uint64, C.c_float, C.c_uint32, C.c_void_p,
+        C.c_uint64, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint64, C.c_void_p]
     for name in ("project_polarization", "reflect_direction"):
         getattr(lib, p + name).argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
     getattr(lib, p + "normalize").argtypes = [C.c_void_p, C.c_void_p]
@@ -233,6 +236,24 @@ def run_emissions(scene, emissions: np.ndarray, event_ids, emission_event_indice
     hits = hits[np.lexsort((hits["photon_id"], hits["event_id"]))]
     del keep
     return OracleResult(hits, stats, hist)
+
+
+def adjoint_synthetic(scene, source, seed: int, hits: np.ndarray, slot_weight, tables, n_params: int,
+                      precision: int = 32, eps: float = 1e-4, max_steps: int = 4096):
+    """Replay gradient over `hits` (FP64 accumulation); returns (grad, counters)."""
+    desc, keep = scene.to_abi()
+    h = np.ascontiguousarray(hits, dtype=abi.HIT_DTYPE)
+    w = np.ascontiguousarray(slot_weight, dtype=np.float64)
+    t = np.ascontiguousarray(np.asarray(tables, dtype=np.uint32).reshape(-1, 2))
+    grad = np.zeros(n_params, np.float64)
+    counters = np.zeros(4, np.uint64)
+    rc = _fn(precision, "adjoint_synthetic")(C.byref(desc), C.byref(source), seed, eps, max_steps,
+                                             h.ctypes.data if h.size else None, h.size, w.ctypes.data, t.ctypes.data,
+                                             t.shape[0], grad.ctypes.data, n_params, counters.ctypes.data)
+    if rc != 0:
+        raise ValueError("parameter layout does not match n_params")
+    del keep
+    return grad, counters
 
 
 # ---- the reference's own boundary physics, compiled from its header ----

@@ -1,0 +1,93 @@
+"""Closed-form pins of the path-replay likelihood-ratio gradient (CPU).
+
+Slab of thickness D = 10 mm and 2000 mm lateral extent, photons starting on
+its bottom face with isotropic directions, no Rayleigh scattering and a
+refractive index that makes every face a mirror or a sensor (dielectric_metal
+surfaces only):
+  * bottom face: polished mirror of reflectivity R,
+  * top face: SiPM of efficiency p (R = 0),
+  * side faces: absorbing (R = 0).
+A photon with direction cosine mu toward the top survives with probability
+exp(-D / (mu L)); a downward photon is first reflected with probability R.
+With E_n the exponential integrals,
+  E[N] = N0/2 p (1 + R) E_2(D/L),
+  dE[N]/dL = N0/2 p (1 + R) E_1(D/L) D / L^2,
+  dE[N]/dR = N0/2 p E_2(D/L),   dE[N]/dp = N0/2 (1 + R) E_2(D/L).
+The replay gradient with loss weight 1 in every slot (L = N) estimates these.
+"""
+import numpy as np
+import pytest
+from scipy import special
+
+from g4go_amd import abi
+from g4go_amd.scene import (Material, PropertyTable, Scene, Surface, SurfaceBinding, SurfaceFinish, SurfaceModel,
+                            SurfaceType, Volume)
+from oracle import oracle as O
+
+D, L_ABS, R_BOTTOM, PDE = 10.0, 50.0, 0.6, 0.5
+
+
+def slab_scene():
+    sc = Scene()
+    air = sc.add_material(Material(name="air", rindex=PropertyTable.constant_value(1.0)))
+    slab_mat = sc.add_material(Material(name="slab", rindex=PropertyTable.constant_value(1.5),
+                                        abs_length_mm=PropertyTable(np.float32([2.0, 4.0]),
+                                                                    np.float32([L_ABS, L_ABS]))))
+    si = sc.add_material(Material(name="si", rindex=PropertyTable.constant_value(3.5)))
+    black = sc.add_surface(Surface(name="black", type=SurfaceType.DielectricMetal,
+                                   reflectivity=PropertyTable.constant_value(0.0)))
+    mirror = sc.add_surface(Surface(name="mirror", type=SurfaceType.DielectricMetal, finish=SurfaceFinish.Polished,
+                                    model=SurfaceModel.Unified,
+                                    reflectivity=PropertyTable.constant_value(R_BOTTOM)))
+    cathode = sc.add_surface(Surface(name="cathode", type=SurfaceType.DielectricMetal,
+                                     reflectivity=PropertyTable.constant_value(0.0),
+                                     efficiency=PropertyTable(np.float32([2.0, 4.0]), np.float32([PDE, PDE]))))
+    world = sc.add_volume(Volume(name="world", material_id=air, half_mm=(3000, 3000, 3000)))
+    sc.world_volume_id = world
+    slab = sc.add_volume(Volume(name="slab", material_id=slab_mat, parent_volume_id=world, half_mm=(1000, 1000, D / 2)))
+    top = sc.add_volume(Volume(name="sipm", material_id=si, parent_volume_id=world, skin_surface_id=cathode,
+                               sensor_id=0, center_mm=(0, 0, D / 2 + 0.5), half_mm=(1000, 1000, 0.5)))
+    bottom = sc.add_volume(Volume(name="mirror", material_id=si, parent_volume_id=world,
+                                  center_mm=(0, 0, -D / 2 - 0.5), half_mm=(1000, 1000, 0.5)))
+    sc.add_surface_binding(SurfaceBinding(slab, bottom, mirror))
+    sc.add_surface_binding(SurfaceBinding(slab, world, black))
+    src = abi.SyntheticSource()
+    src.lo_mm[:] = [-1.0, -1.0, -D / 2]
+    src.hi_mm[:] = [1.0, 1.0, -D / 2]
+    src.wavelength_mean_nm, src.wavelength_sigma_nm, src.decay_time_ns = 420.0, 5.0, 0.0
+    src.volume_id = slab
+    tables = [(abi.PARAM_ABS_LENGTH, slab_mat), (abi.PARAM_REFLECTIVITY, mirror), (abi.PARAM_EFFICIENCY, cathode)]
+    return sc, src, tables
+
+
+@pytest.mark.parametrize("precision", [32, 64])
+def test_replay_gradient_closed_form(precision):
+    sc, src, tables = slab_scene()
+    n0 = 400_000
+    run = O.run_synthetic(sc, src, 17, [0], [n0], precision=precision, threads=8)
+    a = D / L_ABS
+    e2 = special.expn(2, a)
+    e1 = special.expn(1, a)
+    expect_n = n0 / 2 * PDE * (1 + R_BOTTOM) * e2
+    assert abs(len(run.hits) - expect_n) < 5 * np.sqrt(expect_n)
+    grad, counters = O.adjoint_synthetic(sc, src, 17, run.hits, np.ones(abi.HIST_SLOTS), tables, 5,
+                                         precision=precision)
+    assert counters[0] == len(run.hits) and counters[1] == 0 and counters[2] == 0
+    d_abs = grad[0] + grad[1]        # both nodes hold the same length
+    d_refl = grad[2]
+    d_pde = grad[3] + grad[4]
+    np.testing.assert_allclose(d_abs, n0 / 2 * PDE * (1 + R_BOTTOM) * e1 * D / L_ABS**2, rtol=0.03)
+    np.testing.assert_allclose(d_refl, n0 / 2 * PDE * e2, rtol=0.03)
+    np.testing.assert_allclose(d_pde, n0 / 2 * (1 + R_BOTTOM) * e2, rtol=0.03)
+    # the efficiency score is 1/p for every detection: exactly N_det / p
+    assert d_pde == pytest.approx(len(run.hits) / PDE, rel=1e-12)
+
+
+def test_replay_detects_path_mismatch():
+    """A recorded hit whose path does not replay (wrong time) is counted, not scored."""
+    sc, src, tables = slab_scene()
+    run = O.run_synthetic(sc, src, 3, [0], [2000], precision=32)
+    hits = run.hits.copy()
+    hits["time_ns"][0] += 1.0
+    grad, counters = O.adjoint_synthetic(sc, src, 3, hits, np.ones(abi.HIST_SLOTS), tables, 5)
+    assert counters[1] == 1 and counters[0] == len(hits) - 1
