@@ -91,3 +91,43 @@ def test_replay_detects_path_mismatch():
     hits["time_ns"][0] += 1.0
     grad, counters = O.adjoint_synthetic(sc, src, 3, hits, np.ones(abi.HIST_SLOTS), tables, 5)
     assert counters[1] == 1 and counters[0] == len(hits) - 1
+
+
+def test_rayleigh_score_has_zero_mean_when_every_photon_is_detected():
+    """Cube of side 20 mm, no absorption, Rayleigh length 10 mm, every face a
+    sensor of efficiency 1: N = N0 for every L_R, so dN/dL_R = 0, and the replay
+    sum of the scores S/L_R^2 - n_R/L_R over all photons has zero mean.  Its
+    standard error is about sqrt(N0 E[n_R]) / L_R with E[n_R] = E[S] / L_R; a
+    wrong sign or a dropped segment moves it by ~N0 E[n_R] / L_R, hundreds of
+    standard errors."""
+    lr = 10.0
+    sc = Scene()
+    air = sc.add_material(Material(name="air", rindex=PropertyTable.constant_value(1.0)))
+    mat = sc.add_material(Material(name="scatterer", rindex=PropertyTable.constant_value(1.5),
+                                   rayleigh_length_mm=PropertyTable.constant_value(lr)))
+    sensor = sc.add_surface(Surface(name="sensor", type=SurfaceType.DielectricMetal,
+                                    reflectivity=PropertyTable.constant_value(0.0),
+                                    efficiency=PropertyTable.constant_value(1.0)))
+    world = sc.add_volume(Volume(name="world", material_id=air, half_mm=(100, 100, 100)))
+    sc.world_volume_id = world
+    cube = sc.add_volume(Volume(name="cube", material_id=mat, parent_volume_id=world, sensor_id=0,
+                                half_mm=(10, 10, 10)))
+    sc.add_surface_binding(SurfaceBinding(cube, world, sensor))
+    src = abi.SyntheticSource()
+    src.lo_mm[:] = [-10.0, -10.0, -10.0]
+    src.hi_mm[:] = [10.0, 10.0, 10.0]
+    src.wavelength_mean_nm, src.wavelength_sigma_nm, src.decay_time_ns = 420.0, 5.0, 0.0
+    src.volume_id = cube
+    n0 = 200_000
+    run = O.run_synthetic(sc, src, 23, [0], [n0], threads=8)
+    assert len(run.hits) == n0
+    grad, counters = O.adjoint_synthetic(sc, src, 23, run.hits, np.ones(abi.HIST_SLOTS),
+                                         [(abi.PARAM_RAYLEIGH_LENGTH, mat)], 1)
+    assert counters[0] == n0 and counters[1] == 0 and counters[2] == 0
+    mean_path = float(np.mean(run.hits["time_ns"])) * 299.792458 / 1.5
+    mean_scatters = mean_path / lr
+    assert mean_scatters > 0.5          # the score terms are not negligible
+    scale = n0 * mean_scatters / lr     # size of either term of the score sum
+    se = np.sqrt(n0 * mean_scatters) / lr
+    assert abs(grad[0]) < 5 * se, (grad[0], se, scale)
+    assert scale > 100 * se
