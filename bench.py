@@ -15,7 +15,10 @@ plain one-wave-per-64-photons launch is timed beside it.  The roofline object
 uses the transport kernel's own HIP-event duration, averaged over the timed
 steps, and the VALU lane-op model of the path (per generated photon, per
 transport step, per detection).  The CPU baseline transports a bounded sample
-of the same workload with the FP32 oracle on the host's cores.
+of the same workload with the FP32 oracle on the host's cores.  The
+`adjoint` object times a gradient pass of the differentiation workload (1e7
+photons, 66 optical parameters, MSE loss on count + histogram; forward, loss
+and the path-replay adjoint through torch.autograd) against a forward pass.
 """
 from __future__ import annotations
 
@@ -47,9 +50,10 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--photons", type=int, default=100_000_000)
     ap.add_argument("--seed", type=int, default=2024)
-    ap.add_argument("--cpu-sample", type=int, default=4_000_000, help="photons in the CPU-baseline sample")
+    ap.add_argument("--cpu-sample", type=int, default=40_000_000, help="photons in the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--retune", action="store_true", help="ignore the autotuner cache")
     ap.add_argument("--launch", type=str, default=None,
                     help="explicit persistent form block,blocks_per_cu,chunk,refill (skips the autotuner)")
@@ -91,6 +95,66 @@ def run_uneven(host, cs, src, counts, launch, steps, warmup):
         kms.append(out.statistics["kernel_ms"])
     wall = time.perf_counter() - t0
     return wall, float(np.mean(kms)), out
+
+
+def run_adjoint(cs, src, seed, launch, steps, warmup, photons=10_000_000):
+    """Differentiation workload: 1e7 photons, 66 parameters (32 absorption nodes,
+    Rayleigh length, wrap reflectivity, 32 efficiency nodes), loss = (N - N*)^2 +
+    sum_b (h_b - h*_b)^2 against a target run at a seed-perturbed (+-20%) theta.
+    A gradient pass is forward + loss + torch.autograd backward (the path-replay
+    adjoint kernel); it is timed against a forward pass on the same batch."""
+    import torch
+
+    from g4go_amd.autograd import histogram_loss, transport_histogram
+    from g4go_amd.calibrate import clamp_probabilities, crystal_layout, perturbed
+    from g4go_amd.transport import HipTransportHost, PhotonTransportConfig
+
+    ev = np.array([0], np.uint32)
+    cnt = np.array([photons], np.uint32)
+    hosts = []
+    for s in (seed, seed + 1000):
+        h = HipTransportHost(PhotonTransportConfig(seed=s, sort_hits=False))
+        h.set_launch(launch)
+        h.prepare_scene(cs.scene)
+        h.set_parameter_layout(crystal_layout(cs))
+        hosts.append(h)
+    host, target_host = hosts
+    theta0 = host.get_parameters()
+    with torch.no_grad():
+        n_t, h_t = transport_histogram(torch.from_numpy(clamp_probabilities(perturbed(theta0, seed + 1), 32, 32)),
+                                       target_host, cs.scene, src, ev, cnt)
+
+    def forward():
+        host.set_parameters(theta0)
+        host.enqueue_synthetic(cs.scene, src, ev, cnt)
+        out, _ = host.complete_device()
+        return out.statistics["kernel_ms"]
+
+    def gradient():
+        theta = torch.tensor(theta0, dtype=torch.float64, requires_grad=True)
+        n, hist = transport_histogram(theta, host, cs.scene, src, ev, cnt)
+        histogram_loss(n, hist, n_t, h_t).backward()
+        return theta.grad
+
+    for _ in range(warmup):
+        forward()
+        gradient()
+    t0 = time.perf_counter()
+    f_kms = [forward() for _ in range(steps)]
+    f_wall = (time.perf_counter() - t0) / steps
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        g = gradient()
+    g_wall = (time.perf_counter() - t0) / steps
+    f_k = forward()
+    _, stats = host.adjoint(np.ones(258))
+    for h in hosts:
+        h.close()
+    return {"photons": photons, "parameters": int(theta0.size), "forward_ms": 1e3 * f_wall,
+            "gradient_pass_ms": 1e3 * g_wall, "gradient_over_forward": g_wall / f_wall,
+            "forward_kernel_ms": float(np.mean(f_kms)), "adjoint_kernel_ms": stats["kernel_ms"],
+            "adjoint_over_forward_kernel": stats["kernel_ms"] / f_k, "replayed_photons": stats["replayed"],
+            "replay_mismatches": stats["mismatched"], "grad_norm": float(torch.linalg.vector_norm(g))}
 
 
 def cpu_baseline(cs, src, seed, sample):
@@ -174,6 +238,7 @@ def main():
                        "plain_per_event_kernel_ms": e_kms,
                        "batched_over_plain": e_wall / b_wall})
     host.close()
+    adjoint = None if args.no_adjoint else run_adjoint(cs, src, args.seed, persistent, args.steps, args.warmup)
 
     st = out.statistics
     P, S, D = st["generated"], st["total_steps"], st["n_hits"]
@@ -205,6 +270,7 @@ def main():
         },
         "plain": plain,
         "uneven_events": uneven,
+        "adjoint": adjoint,
         "batched_over_plain": (None if plain is None else (args.photons * args.steps / wall) / plain["photons_per_s"]),
         "counts": {"photons": P, "steps": S, "detected": D, "steps_per_photon": S / max(P, 1),
                    "detected_fraction": D / max(P, 1)},
