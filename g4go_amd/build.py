@@ -16,6 +16,8 @@ PACKAGE_DIR = Path(__file__).resolve().parent
 CSRC = PACKAGE_DIR / "csrc"
 INCLUDE = PACKAGE_DIR.parent / "include"
 LIBRARY = PACKAGE_DIR / "libg4go_hip.so"
+SHIM_TEST_SRC = PACKAGE_DIR.parent / "tests" / "cpp" / "shim_test.cpp"
+SHIM_TEST = PACKAGE_DIR.parent / "tests" / "cpp" / "shim_test"
 
 SOURCES = ["kernels.hip", "capi.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -59,7 +61,19 @@ def build(verbose: bool = True) -> Path:
     os.replace(tmp, LIBRARY)
     if verbose:
         print(f"built {LIBRARY}")
+    build_shim_test(verbose)
     return LIBRARY
+
+
+def build_shim_test(verbose: bool = True) -> Path:
+    """Host-only C++20 test program of include/g4go_hip_host.hpp, linked to the library."""
+    tmp = SHIM_TEST.with_name(SHIM_TEST.name + ".tmp")
+    _run(["g++", "-std=c++20", "-O2", "-Wall", "-Wextra", f"-I{INCLUDE}", str(SHIM_TEST_SRC), f"-L{PACKAGE_DIR}",
+          "-lg4go_hip", "-Wl,-rpath,$ORIGIN/../../g4go_amd", "-o", str(tmp)])
+    os.replace(tmp, SHIM_TEST)
+    if verbose:
+        print(f"built {SHIM_TEST}")
+    return SHIM_TEST
 
 
 if __name__ == "__main__":

@@ -219,21 +219,41 @@ def run_synthetic(scene, source: abi.SyntheticSource, seed: int, event_ids, phot
 
 
 def run_emissions(scene, emissions: np.ndarray, event_ids, emission_event_indices, seed: int,
-                  precision: int = 32, eps: float = 1e-4, max_steps: int = 4096) -> OracleResult:
+                  precision: int = 32, eps: float = 1e-4, max_steps: int = 4096, threads: int = 1) -> OracleResult:
+    """Transports the emission batch on the CPU; `threads` splits the records (by photon count)."""
     desc, keep = scene.to_abi()
     em = np.ascontiguousarray(emissions, dtype=abi.EMISSION_DTYPE)
     idx = np.ascontiguousarray(emission_event_indices, dtype=np.uint32)
     n_events = len(event_ids)
-    cap = max(1, int(em["photon_count"].astype(np.int64).sum()))
-    hits = np.zeros(cap, dtype=abi.HIT_DTYPE)
-    stats = np.zeros((n_events, 9), np.uint64)
-    hist = np.zeros(abi.HIST_SLOTS, np.uint64)
-    n = _fn(precision, "run_emissions")(C.byref(desc), em.ctypes.data, em.size, idx.ctypes.data, n_events, seed, eps,
-                                        max_steps, hits.ctypes.data, cap, stats.ctypes.data, hist.ctypes.data)
-    if n < 0:
-        raise RuntimeError(f"oracle run failed ({n})")
-    hits = hits[:n]
+    fn = _fn(precision, "run_emissions")
+
+    def work(lo: int, hi: int):
+        part, part_idx = em[lo:hi], idx[lo:hi]
+        cap = max(1, int(part["photon_count"].astype(np.int64).sum()))
+        hits = np.zeros(cap, dtype=abi.HIT_DTYPE)
+        stats = np.zeros((n_events, 9), np.uint64)
+        hist = np.zeros(abi.HIST_SLOTS, np.uint64)
+        n = fn(C.byref(desc), part.ctypes.data, part.size, part_idx.ctypes.data, n_events, seed, eps, max_steps,
+               hits.ctypes.data, cap, stats.ctypes.data, hist.ctypes.data)
+        if n < 0:
+            raise RuntimeError(f"oracle run failed ({n})")
+        return hits[:n], stats, hist
+
+    if threads <= 1 or em.size < 2 * threads:
+        parts = [work(0, em.size)]
+    else:
+        cum = np.cumsum(em["photon_count"].astype(np.int64))
+        cuts = np.searchsorted(cum, np.linspace(0, cum[-1], threads + 1)[1:-1])
+        edges = [0, *[int(c) for c in cuts], em.size]
+        with ThreadPoolExecutor(threads) as pool:
+            parts = list(pool.map(lambda i: work(edges[i], edges[i + 1]), range(threads)))
+    hits = np.concatenate([p[0] for p in parts])
     hits = hits[np.lexsort((hits["photon_id"], hits["event_id"]))]
+    stats = np.zeros((n_events, 9), np.uint64)
+    for p in parts:
+        stats[:, [0, 1, 2, 3, 5, 6, 7, 8]] += p[1][:, [0, 1, 2, 3, 5, 6, 7, 8]]
+        stats[:, 4] = np.maximum(stats[:, 4], p[1][:, 4])
+    hist = sum(p[2] for p in parts)
     del keep
     return OracleResult(hits, stats, hist)
 

@@ -1,0 +1,333 @@
+// Tests of the C++ host shim (include/g4go_hip_host.hpp) through the C ABI.
+//
+//   shim_test cpu            host-side checks: mesh -> box flattening, the
+//                            exception type of every failure class, an
+//                            empty batch through all three methods
+//   shim_test gpu OUT.bin    transports a crystal batch of scintillation,
+//                            Cerenkov and direct emission records through
+//                            the three-method interface and writes the
+//                            records and the output to OUT.bin (layout below)
+//
+// OUT.bin: u64 n_emissions, n_events, n_detections; emissions (128 B each);
+// event IDs (u32); emission event indices (u32); detections (48 B each);
+// per event: u32 eventID, u32 validFields, u64 captured, detected, absorbed,
+// escaped, truncated, maxBounce, invalidState, zeroStep; batch: u32
+// validFields, u32 0, u64 captured, detected, absorbed, escaped, truncated,
+// maxBounce, invalidState, zeroStep, totalBounceCount.
+#include "g4go_hip_host.hpp"
+
+#include <cstdio>
+#include <cstdlib>
+#include <functional>
+#include <string>
+#include <typeinfo>
+
+namespace {
+
+using namespace g4go_hip::api;
+
+int failures = 0;
+
+void expect(bool ok, const std::string& what) {
+    if (!ok) {
+        std::printf("FAIL: %s\n", what.c_str());
+        ++failures;
+    }
+}
+
+template <class E>
+void expect_throws(const std::function<void()>& f, const std::string& what) {
+    try {
+        f();
+    } catch (const E&) {
+        return;
+    } catch (const std::exception& e) {
+        std::printf("FAIL: %s threw %s (%s), expected %s\n", what.c_str(), typeid(e).name(), e.what(),
+                    typeid(E).name());
+        ++failures;
+        return;
+    }
+    std::printf("FAIL: %s did not throw\n", what.c_str());
+    ++failures;
+}
+
+// Triangle mesh of the box [-h, h] (12 triangles).
+Geometry box_mesh(double hx, double hy, double hz) {
+    Geometry g;
+    for (int k = 0; k < 8; ++k) {
+        g.mesh.verticesMm.push_back(ThreeVector{(k & 1) ? hx : -hx, (k & 2) ? hy : -hy, (k & 4) ? hz : -hz});
+    }
+    const std::uint32_t faces[6][4] = {{0, 2, 6, 4}, {1, 5, 7, 3}, {0, 4, 5, 1}, {2, 3, 7, 6}, {0, 1, 3, 2}, {4, 6, 7, 5}};
+    for (const auto& f : faces) {
+        for (std::uint32_t i : {f[0], f[1], f[2], f[0], f[2], f[3]}) {
+            g.mesh.indices.push_back(i);
+        }
+    }
+    return g;
+}
+
+PropertyTable table(std::initializer_list<float> e, std::initializer_list<float> v) {
+    return PropertyTable{std::vector<float>(e), std::vector<float>(v)};
+}
+
+// Crystal 3x3x20 mm (a 20x3x3 mesh turned by 90 degrees about y) in an air
+// world, wrapped (R = 0.95) on its world border, with a 3x3x0.1 mm SiPM on +z.
+struct CrystalScene {
+    Scene scene;
+    std::uint32_t world, crystal, sipm;
+};
+
+CrystalScene crystal_scene() {
+    CrystalScene c;
+    Scene& s = c.scene;
+    Material air;
+    air.name = "air";
+    air.rindex = table({1.5F, 4.5F}, {1.0F, 1.0F});
+    Material crystal;
+    crystal.name = "crystal";
+    crystal.rindex = table({1.5F, 4.5F}, {1.82F, 1.82F});
+    crystal.rindexMax = 1.82F;
+    crystal.absLengthMm = table({1.5F, 4.5F}, {400.0F, 400.0F});
+    crystal.rayleighLengthMm = table({1.5F, 4.5F}, {500.0F, 500.0F});
+    crystal.scintillationSpectrum[0] = table({2.6F, 2.95F, 3.3F}, {0.0F, 1.0F, 0.0F});
+    Material silicon;
+    silicon.name = "silicon";
+    silicon.rindex = table({1.5F, 4.5F}, {3.5F, 3.5F});
+    const std::uint32_t m_air = s.AddMaterial(air), m_crystal = s.AddMaterial(crystal);
+    const std::uint32_t m_si = s.AddMaterial(silicon);
+    Surface wrap;
+    wrap.name = "Reflector";
+    wrap.type = SurfaceType::DielectricMetal;
+    wrap.reflectivity = table({2.9F}, {0.95F});
+    Surface cathode;
+    cathode.name = "Cathode";
+    cathode.type = SurfaceType::DielectricMetal;
+    cathode.reflectivity = table({2.9F}, {0.0F});
+    cathode.efficiency = table({2.5F, 3.5F}, {0.3F, 0.5F});
+    const std::uint32_t s_wrap = s.AddSurface(wrap), s_cathode = s.AddSurface(cathode);
+    const std::uint32_t g_world = s.AddGeometry(box_mesh(100, 100, 100));
+    const std::uint32_t g_crystal = s.AddGeometry(box_mesh(10, 1.5, 1.5));
+    const std::uint32_t g_sipm = s.AddGeometry(box_mesh(1.5, 1.5, 0.05));
+    Volume world;
+    world.name = "World";
+    world.geometryID = g_world;
+    world.materialID = m_air;
+    c.world = s.AddVolume(world);
+    s.WorldVolumeID(c.world);
+    Volume xtal;
+    xtal.name = "Crystal";
+    xtal.geometryID = g_crystal;
+    xtal.materialID = m_crystal;
+    xtal.parentVolumeID = c.world;
+    xtal.transform.rotation = Rotation{0, 0, -1, 0, 1, 0, 1, 0, 0};  // local x -> world z
+    c.crystal = s.AddVolume(xtal);
+    Volume sipm;
+    sipm.name = "SiPM";
+    sipm.geometryID = g_sipm;
+    sipm.materialID = m_si;
+    sipm.parentVolumeID = c.world;
+    sipm.skinSurfaceID = s_cathode;
+    sipm.sensorID = 0;
+    sipm.transform.translationMm = ThreeVector{0, 0, 10.05};
+    c.sipm = s.AddVolume(sipm);
+    s.AddSurfaceBinding(SurfaceBinding{c.crystal, c.world, s_wrap});
+    return c;
+}
+
+std::uint32_t lcg_state = 12345u;
+float lcg() {  // uniform [0, 1)
+    lcg_state = lcg_state * 1664525u + 1013904223u;
+    return static_cast<float>(lcg_state >> 8) * (1.0F / 16777216.0F);
+}
+
+void make_batch(const CrystalScene& c, std::vector<OpticalEmission>& em, std::vector<std::uint32_t>& events,
+                std::vector<std::uint32_t>& index) {
+    for (std::uint32_t e = 0; e < 6; ++e) {
+        const std::uint32_t event_id = 100 + 7 * e;
+        events.push_back(event_id);
+        std::uint32_t next_photon = 0;
+        for (int k = 0; k < 9; ++k) {
+            OpticalEmission r{};
+            r.positionMm = {2.0F * lcg() - 1.0F, 2.0F * lcg() - 1.0F, -9.0F + 1.5F * k};
+            r.timeNs = 0.1F * k;
+            r.direction = {0.1F, -0.05F, 1.0F};
+            r.stepLengthMm = 1.2F;
+            r.stepDeltaMm = {0.05F, 0.02F, 1.2F};
+            r.preVelocityMmPerNs = 280.0F;
+            r.deltaVelocityMmPerNs = -20.0F;
+            r.polarization = {1.0F, 0.0F, -0.1F};
+            r.energyEv = 2.9F;
+            r.eventID = event_id;
+            r.firstPhotonID = next_photon;
+            r.volumeID = c.crystal;
+            r.materialID = 1;
+            if (k % 3 == 2) {
+                r.type = OpticalEmissionType::Cerenkov;
+                r.preMeanPhotonCount = 12.0F;
+                r.postMeanPhotonCount = 6.0F;
+                r.charge = -1.0F;
+                r.photonCount = 300;
+            } else if (k % 3 == 1) {
+                r.type = OpticalEmissionType::Direct;
+                r.photonCount = 40;
+            } else {
+                r.type = OpticalEmissionType::Scintillation;
+                r.decayTimeNs = 40.0F;
+                r.riseTimeNs = k == 3 ? 0.5F : 0.0F;
+                r.charge = -1.0F;
+                r.photonCount = 1500 + 100 * e;
+            }
+            next_photon += r.photonCount;
+            em.push_back(r);
+            index.push_back(e);
+        }
+    }
+}
+
+int run_cpu() {
+    // mesh -> world box
+    {
+        CrystalScene c = crystal_scene();
+        g4go_hip::FlatScene<Scene> flat(c.scene, 1e-4F);
+        const auto& v = flat.volumes();
+        expect(v.size() == 3, "three volumes");
+        expect(v[1].half_mm[0] == 1.5F && v[1].half_mm[1] == 1.5F && v[1].half_mm[2] == 10.0F,
+               "rotated crystal mesh gives half lengths 1.5 x 1.5 x 10");
+        expect(v[1].center_mm[0] == 0.0F && v[1].center_mm[2] == 0.0F, "crystal centred");
+        expect(std::fabs(v[2].center_mm[2] - 10.05F) < 1e-6F && v[2].half_mm[2] == 0.05F, "SiPM translated to z=10.05");
+        expect(flat.view().world_volume_id == c.world && flat.view().n_bindings == 1, "world and binding");
+        expect(flat.view().materials[1].rayleigh_length_mm.count == 2, "Rayleigh table passed through");
+    }
+    // non-box meshes are rejected
+    {
+        CrystalScene c = crystal_scene();
+        Geometry tetra;
+        tetra.mesh.verticesMm = {ThreeVector{0, 0, 0}, ThreeVector{1, 0, 0}, ThreeVector{0, 1, 0}, ThreeVector{0, 0, 1}};
+        const std::uint32_t g = c.scene.AddGeometry(tetra);
+        Volume v;
+        v.geometryID = g;
+        v.materialID = 0;
+        v.parentVolumeID = c.world;
+        c.scene.AddVolume(v);
+        expect_throws<std::invalid_argument>([&] { g4go_hip::FlatScene<Scene> f(c.scene, 1e-4F); }, "tetrahedron mesh");
+        CrystalScene d = crystal_scene();
+        Volume turned;
+        turned.geometryID = 2;
+        turned.materialID = 0;
+        turned.parentVolumeID = d.world;
+        const float s = 0.70710678F;
+        turned.transform.rotation = Rotation{s, -s, 0, s, s, 0, 0, 0, 1};  // 45 degrees about z
+        d.scene.AddVolume(turned);
+        expect_throws<std::invalid_argument>([&] { g4go_hip::FlatScene<Scene> f(d.scene, 1e-4F); }, "box turned 45 deg");
+    }
+    // exception types
+    {
+        PhotonTransportConfig bad;
+        bad.boundaryToleranceMm = 0.0F;
+        expect_throws<std::invalid_argument>([&] { g4go_hip::HipTransportHost h(bad); }, "zero boundary tolerance");
+        g4go_hip::HipTransportHost host{PhotonTransportConfig{}};
+        expect_throws<std::logic_error>([&] { host.CompleteOldestBatch(); }, "complete with an empty queue");
+        CrystalScene c = crystal_scene();
+        Scene broken = c.scene;
+        Material m;
+        m.rindex = PropertyTable{{1.0F, 2.0F}, {1.5F}};
+        broken.AddMaterial(m);
+        expect_throws<std::invalid_argument>([&] { host.PrepareScene(broken); }, "table size mismatch");
+        Scene unknown = c.scene;
+        Volume v;
+        v.geometryID = 1;
+        v.materialID = 77;
+        v.parentVolumeID = c.world;
+        unknown.AddVolume(v);
+        expect_throws<std::invalid_argument>([&] { host.PrepareScene(unknown); }, "unknown material");
+        host.PrepareScene(c.scene);
+        std::vector<OpticalEmission> em(1);
+        em[0].eventID = 5;
+        em[0].volumeID = c.crystal;
+        std::vector<std::uint32_t> ev = {5, 5}, idx = {0};
+        expect_throws<std::invalid_argument>(
+            [&] { host.EnqueueEmissions(c.scene, PhotonTransportBatch{em, ev, idx}); }, "duplicate event IDs");
+        ev = {6};
+        expect_throws<std::invalid_argument>(
+            [&] { host.EnqueueEmissions(c.scene, PhotonTransportBatch{em, ev, idx}); }, "event index mismatch");
+        std::vector<std::uint32_t> no_idx;
+        expect_throws<std::invalid_argument>(
+            [&] { host.EnqueueEmissions(c.scene, PhotonTransportBatch{em, ev, no_idx}); }, "index count mismatch");
+        // empty batch (zero photons) completes without touching the device
+        em[0].eventID = 6;
+        host.EnqueueEmissions(c.scene, PhotonTransportBatch{em, ev, idx});
+        expect_throws<std::logic_error>(
+            [&] { host.EnqueueEmissions(c.scene, PhotonTransportBatch{em, ev, idx}); }, "second batch in flight");
+        PhotonTransportOutput out = host.CompleteOldestBatch();
+        expect(out.detections.empty() && out.eventStatistics.size() == 1 && out.eventStatistics[0].eventID == 6,
+               "empty batch output");
+        expect(out.statistics.capturedCount == 0, "nothing captured");
+    }
+    std::printf("%s (%d failures)\n", failures == 0 ? "PASS" : "FAIL", failures);
+    return failures == 0 ? 0 : 1;
+}
+
+template <class T>
+void put(std::FILE* f, const T& x) {
+    std::fwrite(&x, sizeof(T), 1, f);
+}
+
+int run_gpu(const char* path) {
+    CrystalScene c = crystal_scene();
+    std::vector<OpticalEmission> em;
+    std::vector<std::uint32_t> events, index;
+    make_batch(c, em, events, index);
+    PhotonTransportConfig cfg;
+    cfg.seed = 4242;
+    g4go_hip::HipTransportHost host{cfg};
+    host.PrepareScene(c.scene);
+    host.EnqueueEmissions(c.scene, PhotonTransportBatch{em, events, index});
+    PhotonTransportOutput out = host.CompleteOldestBatch();
+    std::FILE* f = std::fopen(path, "wb");
+    if (f == nullptr) {
+        std::printf("cannot write %s\n", path);
+        return 1;
+    }
+    put<std::uint64_t>(f, em.size());
+    put<std::uint64_t>(f, events.size());
+    put<std::uint64_t>(f, out.detections.size());
+    std::fwrite(em.data(), sizeof(OpticalEmission), em.size(), f);
+    std::fwrite(events.data(), sizeof(std::uint32_t), events.size(), f);
+    std::fwrite(index.data(), sizeof(std::uint32_t), index.size(), f);
+    std::fwrite(out.detections.data(), sizeof(PhotonDetection), out.detections.size(), f);
+    auto put_stats = [&](std::uint32_t id, const PhotonTransportStatistics& s) {
+        put(f, id);
+        put(f, s.validFields);
+        for (std::uint64_t v : {s.capturedCount, s.detectedCount, s.absorbedCount, s.escapedCount, s.truncatedCount,
+                                s.maxBounceCount, s.invalidStateCount, s.zeroStepCount}) {
+            put(f, v);
+        }
+    };
+    for (const auto& e : out.eventStatistics) {
+        put_stats(e.eventID, e.statistics);
+    }
+    put_stats(0u, out.statistics);
+    put<std::uint64_t>(f, out.performance.totalBounceCount);
+    std::fclose(f);
+    std::printf("transported %zu records, %zu detections, kernel %.3f ms\n", em.size(), out.detections.size(),
+                out.performance.optiXKernelMs);
+    return 0;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    try {
+        if (argc >= 2 && std::string(argv[1]) == "cpu") {
+            return run_cpu();
+        }
+        if (argc >= 3 && std::string(argv[1]) == "gpu") {
+            return run_gpu(argv[2]);
+        }
+    } catch (const std::exception& e) {
+        std::printf("FAIL: uncaught %s: %s\n", typeid(e).name(), e.what());
+        return 1;
+    }
+    std::printf("usage: shim_test cpu | shim_test gpu OUT.bin\n");
+    return 2;
+}

@@ -1,0 +1,138 @@
+"""The C ABI as a library (exports) and the C++ three-method host shim.
+
+CPU: libg4go_hip.so loads and exports every function include/g4go_hip.h
+declares; tests/cpp/shim_test checks the shim's mesh-to-box flattening, the
+exception type of each failure class and an empty batch.
+GPU: the shim transports scintillation, Cerenkov and direct emission records
+in a crystal given as rotated/translated box meshes; its detections and
+per-event counters equal the FP32 oracle's on the same records and an
+equivalent box scene.
+"""
+import ctypes
+import re
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from g4go_amd import abi
+from g4go_amd.scene import (Material, PropertyTable, Scene, Surface, SurfaceBinding, SurfaceFinish, SurfaceModel,
+                            SurfaceType, Volume)
+
+ROOT = Path(__file__).resolve().parent.parent
+SHIM_TEST = ROOT / "tests" / "cpp" / "shim_test"
+
+DETECTION_DTYPE = np.dtype([("position_mm", "<f4", 3), ("time_ns", "<f4"), ("direction", "<f4", 3),
+                            ("energy_ev", "<f4"), ("event_id", "<u4"), ("photon_id", "<u4"), ("sensor_id", "<u4"),
+                            ("pad", "<u4")])
+STATS_DTYPE = np.dtype([("event_id", "<u4"), ("valid", "<u4"), ("captured", "<u8"), ("detected", "<u8"),
+                        ("absorbed", "<u8"), ("escaped", "<u8"), ("truncated", "<u8"), ("max_bounce", "<u8"),
+                        ("invalid", "<u8"), ("zero_step", "<u8")])
+
+
+def declared_functions():
+    text = (ROOT / "include" / "g4go_hip.h").read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?[a-z_0-9]+\s*\*?\s*(g4go_[a-z_0-9]+)\s*\(", text, flags=re.M)))
+
+
+def test_library_exports_every_declared_function():
+    lib = abi.load()
+    names = declared_functions()
+    assert len(names) >= 19, names
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    assert set(abi.EXPORTED_SYMBOLS) <= set(names)
+    lib.g4go_version.restype = ctypes.c_char_p
+    assert lib.g4go_version().decode()
+
+
+def shim_binary():
+    if not SHIM_TEST.is_file():
+        from g4go_amd import build
+
+        build.build_shim_test(verbose=False)
+    return SHIM_TEST
+
+
+def test_shim_host_checks():
+    proc = subprocess.run([str(shim_binary()), "cpu"], capture_output=True, text=True, timeout=60)
+    assert proc.returncode == 0 and "PASS" in proc.stdout, proc.stdout + proc.stderr
+
+
+def shim_crystal_scene():
+    """The box-scene equivalent of the shim test's mesh scene (same IDs and tables)."""
+    def t(e, v):
+        return PropertyTable(np.float32(e), np.float32(v))
+
+    sc = Scene()
+    air = sc.add_material(Material(name="air", rindex=t([1.5, 4.5], [1.0, 1.0])))
+    crystal = sc.add_material(Material(name="crystal", rindex=t([1.5, 4.5], [1.82, 1.82]),
+                                       abs_length_mm=t([1.5, 4.5], [400.0, 400.0]),
+                                       rayleigh_length_mm=t([1.5, 4.5], [500.0, 500.0]),
+                                       scintillation_spectrum=[t([2.6, 2.95, 3.3], [0.0, 1.0, 0.0]), PropertyTable(),
+                                                               PropertyTable()]))
+    si = sc.add_material(Material(name="silicon", rindex=t([1.5, 4.5], [3.5, 3.5])))
+    wrap = sc.add_surface(Surface(name="Reflector", type=SurfaceType.DielectricMetal, model=SurfaceModel.Unified,
+                                  finish=SurfaceFinish.Polished, reflectivity=t([2.9], [0.95])))
+    cathode = sc.add_surface(Surface(name="Cathode", type=SurfaceType.DielectricMetal, model=SurfaceModel.Unified,
+                                     finish=SurfaceFinish.Polished, reflectivity=t([2.9], [0.0]),
+                                     efficiency=t([2.5, 3.5], [0.3, 0.5])))
+    world = sc.add_volume(Volume(name="World", material_id=air, half_mm=(100, 100, 100)))
+    sc.world_volume_id = world
+    xtal = sc.add_volume(Volume(name="Crystal", material_id=crystal, parent_volume_id=world, half_mm=(1.5, 1.5, 10)))
+    sc.add_volume(Volume(name="SiPM", material_id=si, parent_volume_id=world, skin_surface_id=cathode, sensor_id=0,
+                         center_mm=(0, 0, 10.05), half_mm=(1.5, 1.5, 0.05)))
+    sc.add_surface_binding(SurfaceBinding(xtal, world, wrap))
+    return sc
+
+
+def read_shim_output(path):
+    raw = path.read_bytes()
+    n_em, n_ev, n_det = np.frombuffer(raw, "<u8", 3)
+    off = 24
+    em = np.frombuffer(raw, abi.EMISSION_DTYPE, int(n_em), off)
+    off += em.nbytes
+    events = np.frombuffer(raw, "<u4", int(n_ev), off)
+    off += events.nbytes
+    idx = np.frombuffer(raw, "<u4", int(n_em), off)
+    off += idx.nbytes
+    det = np.frombuffer(raw, DETECTION_DTYPE, int(n_det), off)
+    off += det.nbytes
+    ev_stats = np.frombuffer(raw, STATS_DTYPE, int(n_ev), off)
+    off += ev_stats.nbytes
+    batch = np.frombuffer(raw, STATS_DTYPE, 1, off)[0]
+    off += STATS_DTYPE.itemsize
+    bounces = int(np.frombuffer(raw, "<u8", 1, off)[0])
+    return em, events, idx, det, ev_stats, batch, bounces
+
+
+@pytest.mark.gpu
+def test_shim_transport_matches_oracle(tmp_path):
+    from oracle import oracle as O
+
+    out = tmp_path / "shim.bin"
+    print("case: C++ shim, crystal mesh scene, emission records", flush=True)
+    proc = subprocess.run([str(shim_binary()), "gpu", str(out)], capture_output=True, text=True, timeout=100)
+    assert proc.returncode == 0, proc.stdout + proc.stderr
+    em, events, idx, det, ev_stats, batch, bounces = read_shim_output(out)
+    assert set(em["type"]) == {0, 1, 2}
+    ref = O.run_emissions(shim_crystal_scene(), em, events, idx, 4242)
+    assert len(det) == len(ref.hits) > 1000
+    for name in ("position_mm", "direction", "energy_ev", "event_id", "photon_id", "sensor_id"):
+        np.testing.assert_array_equal(det[name], ref.hits[name], err_msg=name)
+    np.testing.assert_array_equal(det["time_ns"].view(np.uint32),
+                                  ref.hits["time_ns"].astype(np.float32).view(np.uint32))
+    for k, name in enumerate(("detected", "absorbed", "escaped", "truncated", "max_bounce", "invalid", "zero_step")):
+        np.testing.assert_array_equal(ev_stats[name], ref.stats[:, k], err_msg=name)
+        # maxBounceCount is a maximum (OptiXTransportDevice.cu:81-83), the others are sums
+        expect = ref.stats[:, k].max() if name == "max_bounce" else ref.stats[:, k].sum()
+        assert int(batch[name]) == int(expect), name
+    np.testing.assert_array_equal(ev_stats["event_id"], events)
+    np.testing.assert_array_equal(ev_stats["captured"],
+                                  np.bincount(idx, weights=em["photon_count"], minlength=len(events)).astype(np.uint64))
+    assert bounces == int(ref.stats[:, 7].sum())
+    full = (1 << 2) | (1 << 3) | (1 << 4) | (1 << 5) | (1 << 6) | (1 << 7) | (1 << 8)
+    assert int(batch["valid"]) == full | (1 << 1) | (1 << 9)
+    assert all(int(v) == full | (1 << 1) for v in ev_stats["valid"])
