@@ -507,6 +507,16 @@ public:
                                      batch.emissions.size(), batch.eventIDs.data(), batch.eventIDs.size(),
                                      batch.emissionEventIndices.data()),
               handle_);
+        // which events own an emission record, and whether the batch has photons:
+        // they decide the valid-field masks of the output (OptiXTransportHost.cpp:587-632)
+        has_emission_.assign(batch.eventIDs.size(), false);
+        for (std::uint32_t i : batch.emissionEventIndices) {
+            has_emission_[i] = true;
+        }
+        batch_photons_ = 0;
+        for (const auto& e : batch.emissions) {
+            batch_photons_ += e.photonCount;
+        }
     }
 
     // CompleteOldestBatch (OptiXTransportHost.cpp:779-883).
@@ -540,10 +550,9 @@ public:
         st.invalidStateCount = r.stats.invalid;
         st.zeroStepCount = r.stats.zero_step;
         st.transportTimeMs = r.stats.transport_time_ms;
-        st.validFields = bit(Field::Captured);
-        if (r.stats.captured != 0) {
-            st.validFields |= counts | bit(Field::TransportTime);
-        }
+        // an empty batch reports no batch fields; a batch with photons reports
+        // captured, the terminal counts and the transport time
+        st.validFields = batch_photons_ != 0 ? bit(Field::Captured) | counts | bit(Field::TransportTime) : 0u;
         out.eventStatistics.reserve(r.n_events);
         for (std::uint64_t i = 0; i < r.n_events; ++i) {
             const g4go_event_stats& e = r.events[i];
@@ -557,7 +566,9 @@ public:
             es.statistics.maxBounceCount = e.max_bounce;
             es.statistics.invalidStateCount = e.invalid;
             es.statistics.zeroStepCount = e.zero_step;
-            es.statistics.validFields = counts | (e.captured != 0 ? bit(Field::Captured) : 0u);
+            const bool owns = i < has_emission_.size() && has_emission_[i];
+            // captured when the event owns a record; terminal counts whenever the batch has records
+            es.statistics.validFields = (owns ? bit(Field::Captured) : 0u) | (any_emission() ? counts : 0u);
             out.eventStatistics.push_back(es);
         }
         auto& perf = out.performance;
@@ -575,9 +586,20 @@ public:
 private:
     static constexpr std::uint32_t bit(Field f) { return std::uint32_t{1} << static_cast<std::uint32_t>(f); }
 
+    bool any_emission() const {
+        for (bool b : has_emission_) {
+            if (b) {
+                return true;
+            }
+        }
+        return false;
+    }
+
     g4go_transport* handle_ = nullptr;
     const Scene* prepared_ = nullptr;
     float tolerance_mm_;
+    std::vector<bool> has_emission_;
+    std::uint64_t batch_photons_ = 0;
 };
 
 using HipTransportHost = BasicHipTransportHost<>;

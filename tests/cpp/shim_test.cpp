@@ -262,6 +262,24 @@ int run_cpu() {
         expect(out.detections.empty() && out.eventStatistics.size() == 1 && out.eventStatistics[0].eventID == 6,
                "empty batch output");
         expect(out.statistics.capturedCount == 0, "nothing captured");
+        // a batch whose records hold no photons: no batch fields, per event captured
+        // and the terminal counts (OptiXTransportHost.cpp:587-620)
+        using F = PhotonTransportStatisticField;
+        const std::uint32_t terminal = StatisticFieldBit(F::Detected) | StatisticFieldBit(F::Absorbed) |
+                                       StatisticFieldBit(F::Escaped) | StatisticFieldBit(F::Truncated) |
+                                       StatisticFieldBit(F::MaxBounce) | StatisticFieldBit(F::InvalidState) |
+                                       StatisticFieldBit(F::ZeroStep);
+        expect(out.statistics.validFields == 0, "empty batch has no batch fields");
+        expect(out.eventStatistics[0].statistics.validFields == (terminal | StatisticFieldBit(F::Captured)),
+               "empty batch event fields");
+        // no records at all: no fields anywhere
+        std::vector<OpticalEmission> none;
+        std::vector<std::uint32_t> none_idx;
+        host.EnqueueEmissions(c.scene, PhotonTransportBatch{none, ev, none_idx});
+        out = host.CompleteOldestBatch();
+        expect(out.statistics.validFields == 0 && out.eventStatistics.size() == 1 &&
+                   out.eventStatistics[0].statistics.validFields == 0,
+               "batch without records has no fields");
     }
     std::printf("%s (%d failures)\n", failures == 0 ? "PASS" : "FAIL", failures);
     return failures == 0 ? 0 : 1;
