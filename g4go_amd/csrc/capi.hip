@@ -573,7 +573,15 @@ struct g4go_transport {
         off = align256(off + std::max<size_t>(1, s.bindings.size()) * sizeof(Binding));
         const size_t o_bvh = off;
         off = align256(off + std::max<size_t>(1, s.bvh.size()) * sizeof(BvhNode));
+        const size_t o_dbox = off;
+        off = align256(off + std::max<size_t>(1, s.daughters.size()) * sizeof(DaughterBox));
         std::vector<char> blob(off, 0);
+        // daughter boxes in daughter-list order (after the BVH build reordered the list)
+        for (size_t k = 0; k < s.daughters.size(); ++k) {
+            const Volume& d = s.volumes[s.daughters[k]];
+            DaughterBox b{{d.lo[0], d.lo[1], d.lo[2]}, s.daughters[k], {d.hi[0], d.hi[1], d.hi[2]}, 0u};
+            std::memcpy(blob.data() + o_dbox + k * sizeof(DaughterBox), &b, sizeof(b));
+        }
         std::memcpy(blob.data() + o_pool, s.pool.data(), s.pool.size() * sizeof(float));
         if (!s.materials.empty()) std::memcpy(blob.data() + o_mat, s.materials.data(), s.materials.size() * sizeof(Material));
         if (!s.surfaces.empty()) std::memcpy(blob.data() + o_surf, s.surfaces.data(), s.surfaces.size() * sizeof(Surface));
@@ -592,6 +600,8 @@ struct g4go_transport {
         view.daughters = reinterpret_cast<const uint32_t*>(base + o_dau);
         view.bindings = reinterpret_cast<const Binding*>(base + o_bind);
         view.bvh = reinterpret_cast<const BvhNode*>(base + o_bvh);
+        view.dbox = reinterpret_cast<const DaughterBox*>(base + o_dbox);
+        view.n_daughters = static_cast<uint32_t>(s.daughters.size());
         view.n_materials = static_cast<uint32_t>(s.materials.size());
         view.n_surfaces = static_cast<uint32_t>(s.surfaces.size());
         view.n_volumes = static_cast<uint32_t>(s.volumes.size());
@@ -639,6 +649,10 @@ struct g4go_transport {
         p.refill_threshold = std::max<uint32_t>(1, std::min<uint32_t>(64, launch.refill_threshold));
         p.hist_lo_ns = 0.0;
         p.hist_width_ns = 200.0 / kHistBins;
+        // scenes with a daughter BVH: every block stages the nodes and the
+        // daughter boxes in LDS when they fit
+        const uint64_t entries = static_cast<uint64_t>(view.n_bvh_nodes) + view.n_daughters;
+        p.lds_entries = view.n_bvh_nodes != 0 && entries * 32u <= kLdsSceneBytes ? static_cast<uint32_t>(entries) : 0u;
         return p;
     }
 

@@ -7,6 +7,7 @@
 
 namespace g4go {
 
+constexpr uint32_t kLdsSceneBytes = 32768;  // LDS budget of the staged BVH + daughter boxes per block
 constexpr int kHistBins = 256;
 constexpr int kHistSlots = kHistBins + 2;  // + underflow (256) + overflow (257)
 
@@ -61,6 +62,11 @@ struct KernelParams {
     EventStats* event_stats;
     unsigned long long* counters;
     double hist_lo_ns, hist_width_ns;
+    // BVH nodes + daughter boxes staged into LDS by every block (32-byte
+    // entries; 0 = walk them in global memory).  The launch allocates
+    // lds_entries * 32 bytes of dynamic LDS.
+    uint32_t lds_entries;
+    uint32_t pad_lds;
 };
 
 // Parameterised property tables for the adjoint: theta offset of each

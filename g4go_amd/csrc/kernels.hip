@@ -185,8 +185,21 @@ __global__ void __launch_bounds__(256) transport_kernel(KernelParams p) {
     __shared__ uint32_t s_hist[kHistSlots];
     __shared__ float4 s_stage[kMaxWavesPerBlock][3 * kStageCap];
     __shared__ uint32_t s_stage_src[kMaxWavesPerBlock][kStageCap];
+    extern __shared__ float4 s_scene[];  // p.lds_entries x 32 B: BVH nodes, then daughter boxes
     for (int i = threadIdx.x; i < kHistSlots; i += blockDim.x) {
         s_hist[i] = 0u;
+    }
+    SceneView sv = p.scene;
+    if (p.lds_entries != 0u) {
+        const uint32_t node_words = 2u * p.scene.n_bvh_nodes;
+        const uint32_t words = 2u * p.lds_entries;
+        const float4* nodes = reinterpret_cast<const float4*>(p.scene.bvh);
+        const float4* boxes = reinterpret_cast<const float4*>(p.scene.dbox);
+        for (uint32_t i = threadIdx.x; i < words; i += blockDim.x) {
+            s_scene[i] = i < node_words ? nodes[i] : boxes[i - node_words];
+        }
+        sv.bvh = reinterpret_cast<const BvhNode*>(s_scene);
+        sv.dbox = reinterpret_cast<const DaughterBox*>(s_scene + node_words);
     }
     __syncthreads();
     float4* const stage = s_stage[threadIdx.x >> 6];
@@ -285,7 +298,7 @@ __global__ void __launch_bounds__(256) transport_kernel(KernelParams p) {
         uint32_t sensor = kInvalid;
         if (alive) {
             bool zero = false;
-            status = transport_step<false>(ph, p.scene, p.rng, p.cfg, zero, sensor, nullptr);
+            status = transport_step<false>(ph, sv, p.rng, p.cfg, zero, sensor, nullptr);
             ++ph.step;
             ++lane_steps;
             st.zero_step += zero ? 1u : 0u;
@@ -362,17 +375,21 @@ int launch_zero(void* ptr, size_t bytes, hipStream_t stream) {
 }
 
 int launch_transport(const KernelParams& p, int source, bool persistent, int block, int grid, hipStream_t stream) {
+    const size_t lds = static_cast<size_t>(p.lds_entries) * 32u;
+    if (lds > kLdsSceneBytes) {
+        return 1;
+    }
     if (source == kSourceSynthetic) {
         if (persistent) {
-            hipLaunchKernelGGL((transport_kernel<kSourceSynthetic, true>), dim3(grid), dim3(block), 0, stream, p);
+            hipLaunchKernelGGL((transport_kernel<kSourceSynthetic, true>), dim3(grid), dim3(block), lds, stream, p);
         } else {
-            hipLaunchKernelGGL((transport_kernel<kSourceSynthetic, false>), dim3(grid), dim3(block), 0, stream, p);
+            hipLaunchKernelGGL((transport_kernel<kSourceSynthetic, false>), dim3(grid), dim3(block), lds, stream, p);
         }
     } else {
         if (persistent) {
-            hipLaunchKernelGGL((transport_kernel<kSourceEmission, true>), dim3(grid), dim3(block), 0, stream, p);
+            hipLaunchKernelGGL((transport_kernel<kSourceEmission, true>), dim3(grid), dim3(block), lds, stream, p);
         } else {
-            hipLaunchKernelGGL((transport_kernel<kSourceEmission, false>), dim3(grid), dim3(block), 0, stream, p);
+            hipLaunchKernelGGL((transport_kernel<kSourceEmission, false>), dim3(grid), dim3(block), lds, stream, p);
         }
     }
     return hipGetLastError() == hipSuccess ? 0 : 1;

@@ -71,6 +71,16 @@ struct BvhNode {
 };
 constexpr uint32_t kBvhLeaf = 0x80000000u;
 
+// Box of the daughter in one slot of the daughter list (slot order = the
+// list's order, so BVH leaves address it directly).  32 bytes: the BVH walk
+// reads two aligned 16-byte words per candidate instead of a Volume record.
+struct DaughterBox {
+    float lo[3];
+    uint32_t volume;
+    float hi[3];
+    uint32_t pad;
+};
+
 struct SceneView {
     const float* pool;
     const Material* materials;
@@ -79,9 +89,10 @@ struct SceneView {
     const uint32_t* daughters;
     const Binding* bindings;
     const BvhNode* bvh;
+    const DaughterBox* dbox;  // one per daughter-list slot
     uint32_t n_materials, n_surfaces, n_volumes, n_bindings;
-    uint32_t n_bvh_nodes, world;
-    uint32_t pad[2];
+    uint32_t n_bvh_nodes, world, n_daughters;
+    uint32_t pad;
 };
 
 }  // namespace g4go

@@ -341,9 +341,10 @@ G4GO_HD float nearest_daughter(const SceneView& sc, const Volume& vol, V3 p, V3 
     which = kInvalid;
     if (vol.bvh_root == kInvalid) {
         for (uint32_t k = 0; k < vol.daughter_count; ++k) {
-            const uint32_t j = sc.daughters[vol.daughter_begin + k];
+            const DaughterBox& b = sc.dbox[vol.daughter_begin + k];
+            const uint32_t j = b.volume;
             int a = 0;
-            const float t = box_entry(sc.volumes[j].lo, sc.volumes[j].hi, p, inv, tmin, a);
+            const float t = box_entry(b.lo, b.hi, p, inv, tmin, a);
             if (t < best || (t == best && t < kInf && j < which)) {  // ties: lower volume index
                 best = t;
                 which = j;
@@ -365,8 +366,9 @@ G4GO_HD float nearest_daughter(const SceneView& sc, const Volume& vol, V3 p, V3 
         if ((node.right_or_count & kBvhLeaf) != 0) {
             const uint32_t count = node.right_or_count & ~kBvhLeaf;
             for (uint32_t k = 0; k < count; ++k) {
-                const uint32_t j = sc.daughters[node.left + k];
-                const float t = box_entry(sc.volumes[j].lo, sc.volumes[j].hi, p, inv, tmin, a);
+                const DaughterBox& b = sc.dbox[node.left + k];
+                const uint32_t j = b.volume;
+                const float t = box_entry(b.lo, b.hi, p, inv, tmin, a);
                 if (t < best || (t == best && t < kInf && j < which)) {
                     best = t;
                     which = j;
@@ -390,9 +392,10 @@ G4GO_HD uint32_t point_neighbor(const SceneView& sc, uint32_t current, V3 q) {
     }
     const Volume& par = sc.volumes[cur.parent];
     for (uint32_t k = 0; k < par.daughter_count; ++k) {
-        const uint32_t j = sc.daughters[par.daughter_begin + k];
-        if (j != current && box_contains(sc.volumes[j], q)) {
-            return j;
+        const DaughterBox& b = sc.dbox[par.daughter_begin + k];
+        if (b.volume != current && q.x >= b.lo[0] && q.x <= b.hi[0] && q.y >= b.lo[1] && q.y <= b.hi[1] &&
+            q.z >= b.lo[2] && q.z <= b.hi[2]) {
+            return b.volume;
         }
     }
     return cur.parent;

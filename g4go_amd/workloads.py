@@ -185,28 +185,31 @@ def lognormal_counts(n_events: int, median: float, sigma: float = 1.5, lo: int =
     return np.clip(np.rint(np.exp(np.log(median) + sigma * z)), lo, hi).astype(np.uint32)
 
 
-def electron_emissions(cs: CrystalScene, n_events: int, seed: int = 0, energy_mev: float = 1.0,
+def electron_emissions(cs: CrystalScene, n_events: int, seed: int = 0, energy_mev=1.0,
                        yield_per_kev: float = 30.0, steps: int = 20, range_mm: float = 1.3,
-                       first_event_id: int = 0, crystal_index: int = 0):
-    """Scintillation emission records of `n_events` electrons entering the crystal on its axis.
+                       first_event_id: int = 0, crystal_index=0):
+    """Scintillation emission records of `n_events` electrons entering a crystal on its axis.
 
     Each electron is a chain of `steps` straight segments from the -z face
     centre with a small random deflection per segment, depositing equal
-    energy per segment; photons per segment ~ Poisson(yield * dE).  Returns
-    (emissions, event_ids, emission_event_indices).
+    energy per segment; photons per segment ~ Poisson(yield * dE).
+    `energy_mev` and `crystal_index` are scalars or one value per event.
+    Returns (emissions, event_ids, emission_event_indices).
     """
     rng = np.random.default_rng(seed)
-    vol = cs.scene.volumes[cs.crystal_ids[crystal_index]]
-    mat = vol.material_id
-    start = np.array([vol.center_mm[0], vol.center_mm[1], vol.center_mm[2] - vol.half_mm[2] + 1e-3])
+    energies = np.broadcast_to(np.asarray(energy_mev, np.float64), (n_events,))
+    crystals = np.broadcast_to(np.asarray(crystal_index, np.int64), (n_events,))
     seg = range_mm / steps
-    mean_per_step = yield_per_kev * energy_mev * 1000.0 / steps
     em = np.zeros(n_events * steps, dtype=abi.EMISSION_DTYPE)
     idx = np.repeat(np.arange(n_events, dtype=np.uint32), steps)
     event_ids = np.arange(first_event_id, first_event_id + n_events, dtype=np.uint32)
     beta = 0.94
     v = beta * 299.792458
     for e in range(n_events):
+        vol = cs.scene.volumes[cs.crystal_ids[int(crystals[e])]]
+        mat = vol.material_id
+        start = np.array([vol.center_mm[0], vol.center_mm[1], vol.center_mm[2] - vol.half_mm[2] + 1e-3])
+        mean_per_step = yield_per_kev * float(energies[e]) * 1000.0 / steps
         pos = start.copy()
         d = np.array([0.0, 0.0, 1.0])
         first = 0
@@ -243,3 +246,16 @@ def electron_emissions(cs: CrystalScene, n_events: int, seed: int = 0, energy_me
             first += int(counts[s])
             pos = pos + delta
     return em, event_ids, idx
+
+
+def array_events(cs: CrystalScene, n_events: int, median: float = 2e4, sigma: float = 1.5, lo: int = 10,
+                 hi: int = 1_000_000, seed: int = 0, yield_per_kev: float = 30.0):
+    """Uneven electron events in a crystal array: event k deposits the energy of
+    lognormal_counts(...)[k] photons (at `yield_per_kev`) in a uniformly chosen
+    crystal.  Returns (emissions, event_ids, emission_event_indices, crystal per event)."""
+    counts = lognormal_counts(n_events, median, sigma=sigma, lo=lo, hi=hi, seed=seed)
+    crystals = np.random.default_rng(seed + 1).integers(0, len(cs.crystal_ids), n_events)
+    energies = counts.astype(np.float64) / (yield_per_kev * 1000.0)
+    em, ev, idx = electron_emissions(cs, n_events, seed=seed + 2, energy_mev=energies, yield_per_kev=yield_per_kev,
+                                     crystal_index=crystals)
+    return em, ev, idx, crystals

@@ -1631,15 +1631,70 @@ static int or_add_reflection(sslot_t* a, sslot_t* b, uint32_t sf) {
     return 1;
 }
 
+/* Regenerates the photon of a recorded hit; returns 0 when the hit names no
+ * photon of the batch. */
+typedef int (*regen_fn)(photon_t* ph, const void* ctx, const oscene_t* s, const rng_t* r, const g4go_hit* hit);
+
+typedef struct {
+    const g4go_synthetic_source* src;
+    uint32_t volume;
+} synth_ctx_t;
+
+static int regen_synthetic(photon_t* ph, const void* c, const oscene_t* s, const rng_t* r, const g4go_hit* hit) {
+    const synth_ctx_t* x = (const synth_ctx_t*)c;
+    (void)s;
+    gen_synthetic(ph, x->src, x->volume, r, hit->event_id, hit->photon_id);
+    return 1;
+}
+
+typedef struct {
+    uint32_t event_id, first, index, pad;
+} em_key_t;
+
+typedef struct {
+    const g4go_emission* em;
+    const em_key_t* keys;  /* records sorted by (event_id, first_photon_id) */
+    uint64_t n;
+} em_ctx_t;
+
+static int em_key_cmp(const void* a, const void* b) {
+    const em_key_t* x = (const em_key_t*)a;
+    const em_key_t* y = (const em_key_t*)b;
+    if (x->event_id != y->event_id) return x->event_id < y->event_id ? -1 : 1;
+    if (x->first != y->first) return x->first < y->first ? -1 : 1;
+    return x->index < y->index ? -1 : (x->index > y->index);
+}
+
+/* The record of event hit->event_id whose photon range holds hit->photon_id
+ * (photon IDs are assigned per record from first_photon_id,
+ * OptiXTransportDevice.cu:359). */
+static int regen_emission(photon_t* ph, const void* c, const oscene_t* s, const rng_t* r, const g4go_hit* hit) {
+    const em_ctx_t* x = (const em_ctx_t*)c;
+    uint64_t lo = 0, hi = x->n;
+    while (lo < hi) { /* first key > (event, photon) */
+        const uint64_t mid = (lo + hi) / 2;
+        const em_key_t* k = &x->keys[mid];
+        if (k->event_id < hit->event_id || (k->event_id == hit->event_id && k->first <= hit->photon_id)) {
+            lo = mid + 1;
+        } else {
+            hi = mid;
+        }
+    }
+    if (lo == 0) return 0;
+    const em_key_t* k = &x->keys[lo - 1];
+    const g4go_emission* em = &x->em[k->index];
+    if (k->event_id != hit->event_id || hit->photon_id - k->first >= em->photon_count) return 0;
+    gen_emission(ph, em, vol_index(s, em->volume_id), s, r, hit->photon_id - k->first);
+    return 1;
+}
+
 /* tables: n_tables (kind, index) pairs bound to theta in order (kind 0 absorption,
  * 1 Rayleigh of a material; 2 reflectivity, 3 efficiency of a surface).
  * counters: replayed, mismatched, unsupported, steps.  Returns 0, or -1 when the
  * layout does not fit n_params. */
-int ORACLE_NAME(adjoint_synthetic)(const g4go_scene* scene, const g4go_synthetic_source* src, uint64_t seed, float eps,
-                                   uint32_t max_steps, const g4go_hit* hits, uint64_t n_hits,
-                                   const double* slot_weight, const uint32_t* tables, uint32_t n_tables, double* grad,
-                                   uint64_t n_params, uint64_t* counters) {
-    oscene_t* s = build(scene);
+static int adjoint_core(oscene_t* s, regen_fn regen, const void* ctx, uint64_t seed, float eps, uint32_t max_steps,
+                        const g4go_hit* hits, uint64_t n_hits, const double* slot_weight, const uint32_t* tables,
+                        uint32_t n_tables, double* grad, uint64_t n_params, uint64_t* counters) {
     uint32_t* mabs = (uint32_t*)malloc(sizeof(uint32_t) * (s->n_mat + 1));
     uint32_t* mray = (uint32_t*)malloc(sizeof(uint32_t) * (s->n_mat + 1));
     uint32_t* srefl = (uint32_t*)malloc(sizeof(uint32_t) * (s->n_surf + 1));
@@ -1659,16 +1714,13 @@ int ORACLE_NAME(adjoint_synthetic)(const g4go_scene* scene, const g4go_synthetic
     }
     if (offset != n_params) {
         free(mabs); free(mray); free(srefl); free(seff);
-        destroy(s);
         return -1;
     }
-    const uint32_t volume = vol_index(s, src->volume_id);
     const rng_t r = {(uint32_t)seed, (uint32_t)(seed >> 32)};
     for (uint64_t h = 0; h < n_hits; ++h) {
         const g4go_hit* hit = &hits[h];
         photon_t ph;
-        gen_synthetic(&ph, src, volume, &r, hit->event_id, hit->photon_id);
-        if (!start_photon(&ph)) { counters[1]++; continue; }
+        if (!regen(&ph, ctx, s, &r, hit) || !start_photon(&ph)) { counters[1]++; continue; }
         mslot_t m[2] = {{INVALID, 0, 0.0}, {INVALID, 0, 0.0}};
         sslot_t rs[2] = {{INVALID, 0}, {INVALID, 0}};
         int ok = 1, status = S_ALIVE, zero = 0;
@@ -1742,6 +1794,40 @@ int ORACLE_NAME(adjoint_synthetic)(const g4go_scene* scene, const g4go_synthetic
         }
     }
     free(mabs); free(mray); free(srefl); free(seff);
-    destroy(s);
     return 0;
+}
+
+int ORACLE_NAME(adjoint_synthetic)(const g4go_scene* scene, const g4go_synthetic_source* src, uint64_t seed, float eps,
+                                   uint32_t max_steps, const g4go_hit* hits, uint64_t n_hits,
+                                   const double* slot_weight, const uint32_t* tables, uint32_t n_tables, double* grad,
+                                   uint64_t n_params, uint64_t* counters) {
+    oscene_t* s = build(scene);
+    const synth_ctx_t ctx = {src, vol_index(s, src->volume_id)};
+    const int rc = adjoint_core(s, regen_synthetic, &ctx, seed, eps, max_steps, hits, n_hits, slot_weight, tables,
+                                n_tables, grad, n_params, counters);
+    destroy(s);
+    return rc;
+}
+
+/* The replay gradient of an emission-record batch: each hit's photon is
+ * regenerated from the record that owns its photon ID. */
+int ORACLE_NAME(adjoint_emissions)(const g4go_scene* scene, const g4go_emission* emissions, uint64_t n_emissions,
+                                   uint64_t seed, float eps, uint32_t max_steps, const g4go_hit* hits, uint64_t n_hits,
+                                   const double* slot_weight, const uint32_t* tables, uint32_t n_tables, double* grad,
+                                   uint64_t n_params, uint64_t* counters) {
+    oscene_t* s = build(scene);
+    em_key_t* keys = (em_key_t*)malloc(sizeof(em_key_t) * (n_emissions + 1));
+    for (uint64_t i = 0; i < n_emissions; ++i) {
+        keys[i].event_id = emissions[i].event_id;
+        keys[i].first = emissions[i].first_photon_id;
+        keys[i].index = (uint32_t)i;
+        keys[i].pad = 0;
+    }
+    qsort(keys, n_emissions, sizeof(em_key_t), em_key_cmp);
+    const em_ctx_t ctx = {emissions, keys, n_emissions};
+    const int rc = adjoint_core(s, regen_emission, &ctx, seed, eps, max_steps, hits, n_hits, slot_weight, tables,
+                                n_tables, grad, n_params, counters);
+    free(keys);
+    destroy(s);
+    return rc;
 }

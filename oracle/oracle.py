@@ -59,6 +59,9 @@ def _load(precision: int) -> C.CDLL:
     getattr(lib, p + "adjoint_synthetic").argtypes = [
         C.POINTER(abi.Scene), C.POINTER(abi.SyntheticSource), C.c_uint64, C.c_float, C.c_uint32, C.c_void_p,
         C.c_uint64, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint64, C.c_void_p]
+    getattr(lib, p + "adjoint_emissions").argtypes = [
+        C.POINTER(abi.Scene), C.c_void_p, C.c_uint64, C.c_uint64, C.c_float, C.c_uint32, C.c_void_p, C.c_uint64,
+        C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint64, C.c_void_p]
     for name in ("project_polarization", "reflect_direction"):
         getattr(lib, p + name).argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
     getattr(lib, p + "normalize").argtypes = [C.c_void_p, C.c_void_p]
@@ -258,22 +261,51 @@ def run_emissions(scene, emissions: np.ndarray, event_ids, emission_event_indice
     return OracleResult(hits, stats, hist)
 
 
-def adjoint_synthetic(scene, source, seed: int, hits: np.ndarray, slot_weight, tables, n_params: int,
-                      precision: int = 32, eps: float = 1e-4, max_steps: int = 4096):
-    """Replay gradient over `hits` (FP64 accumulation); returns (grad, counters)."""
-    desc, keep = scene.to_abi()
+def _adjoint(call, hits, slot_weight, tables, n_params: int, threads: int):
     h = np.ascontiguousarray(hits, dtype=abi.HIT_DTYPE)
     w = np.ascontiguousarray(slot_weight, dtype=np.float64)
     t = np.ascontiguousarray(np.asarray(tables, dtype=np.uint32).reshape(-1, 2))
-    grad = np.zeros(n_params, np.float64)
-    counters = np.zeros(4, np.uint64)
-    rc = _fn(precision, "adjoint_synthetic")(C.byref(desc), C.byref(source), seed, eps, max_steps,
-                                             h.ctypes.data if h.size else None, h.size, w.ctypes.data, t.ctypes.data,
-                                             t.shape[0], grad.ctypes.data, n_params, counters.ctypes.data)
-    if rc != 0:
-        raise ValueError("parameter layout does not match n_params")
+
+    def work(lo: int, hi: int):
+        part = h[lo:hi]
+        grad = np.zeros(n_params, np.float64)
+        counters = np.zeros(4, np.uint64)
+        rc = call(part.ctypes.data if part.size else None, part.size, w.ctypes.data, t.ctypes.data, t.shape[0],
+                  grad.ctypes.data, n_params, counters.ctypes.data)
+        if rc != 0:
+            raise ValueError("parameter layout does not match n_params")
+        return grad, counters
+
+    if threads <= 1 or h.size < 2 * threads:
+        parts = [work(0, h.size)]
+    else:
+        edges = np.linspace(0, h.size, threads + 1).astype(np.int64)
+        with ThreadPoolExecutor(threads) as pool:
+            parts = list(pool.map(lambda i: work(int(edges[i]), int(edges[i + 1])), range(threads)))
+    return sum(p[0] for p in parts), sum(p[1] for p in parts)
+
+
+def adjoint_synthetic(scene, source, seed: int, hits: np.ndarray, slot_weight, tables, n_params: int,
+                      precision: int = 32, eps: float = 1e-4, max_steps: int = 4096, threads: int = 1):
+    """Replay gradient over `hits` of a synthetic batch (FP64 accumulation); returns (grad, counters)."""
+    desc, keep = scene.to_abi()
+    fn = _fn(precision, "adjoint_synthetic")
+    out = _adjoint(lambda *a: fn(C.byref(desc), C.byref(source), seed, eps, max_steps, *a), hits, slot_weight,
+                   tables, n_params, threads)
     del keep
-    return grad, counters
+    return out
+
+
+def adjoint_emissions(scene, emissions: np.ndarray, seed: int, hits: np.ndarray, slot_weight, tables, n_params: int,
+                      precision: int = 32, eps: float = 1e-4, max_steps: int = 4096, threads: int = 1):
+    """Replay gradient over `hits` of an emission-record batch; returns (grad, counters)."""
+    desc, keep = scene.to_abi()
+    em = np.ascontiguousarray(emissions, dtype=abi.EMISSION_DTYPE)
+    fn = _fn(precision, "adjoint_emissions")
+    out = _adjoint(lambda *a: fn(C.byref(desc), em.ctypes.data, em.size, seed, eps, max_steps, *a), hits,
+                   slot_weight, tables, n_params, threads)
+    del keep
+    return out
 
 
 # ---- the reference's own boundary physics, compiled from its header ----

@@ -131,3 +131,49 @@ def test_rayleigh_score_has_zero_mean_when_every_photon_is_detected():
     se = np.sqrt(n0 * mean_scatters) / lr
     assert abs(grad[0]) < 5 * se, (grad[0], se, scale)
     assert scale > 100 * se
+
+
+def test_emission_replay_closed_form():
+    """Direct emission records on the slab's bottom face, each with a fixed
+    direction cosine mu_i toward the top: a photon is detected with
+    probability p exp(-D / (mu_i L)), so E[N] = sum_i n_i p exp(-D/(mu_i L)),
+    dE[N]/dL = sum_i n_i p exp(-D/(mu_i L)) D / (mu_i L^2) and dE[N]/dp = E[N] / p.
+    Several records per event exercise the photon-ID -> record lookup."""
+    sc, _, tables = slab_scene()
+    slab = 1
+    mus = [0.9, 0.6, 0.35, 0.8]
+    per_record = 25_000
+    recs = []
+    for ev in (3, 8, 5):
+        first = 0
+        for k, mu in enumerate(mus):
+            r = np.zeros(1, abi.EMISSION_DTYPE)[0]
+            r["position_mm"] = (0.3 * k, -0.2 * k, -D / 2)
+            r["direction"] = (np.sqrt(1 - mu * mu), 0.0, mu)
+            r["polarization"] = (0.0, 1.0, 0.0)
+            r["energy_ev"] = 2.9
+            r["event_id"] = ev
+            r["first_photon_id"] = first
+            r["photon_count"] = per_record
+            r["volume_id"] = slab
+            r["type"] = 0
+            first += per_record
+            recs.append(r)
+    em = np.array(recs, abi.EMISSION_DTYPE)
+    event_ids = np.array([3, 8, 5], np.uint32)
+    idx = np.repeat(np.arange(3, dtype=np.uint32), len(mus))
+    run = O.run_emissions(sc, em, event_ids, idx, 41, threads=4)
+    surv = np.array([np.exp(-D / (mu * L_ABS)) for mu in mus])
+    n_exp = 3 * per_record * PDE * surv.sum()
+    assert abs(len(run.hits) - n_exp) < 5 * np.sqrt(n_exp)
+    grad, counters = O.adjoint_emissions(sc, em, 41, run.hits, np.ones(abi.HIST_SLOTS), tables, 5, threads=4)
+    assert counters[0] == len(run.hits) and counters[1] == 0 and counters[2] == 0
+    d_abs = 3 * per_record * PDE * float(np.sum(surv * D / (np.array(mus) * L_ABS**2)))
+    np.testing.assert_allclose(grad[0] + grad[1], d_abs, rtol=0.02)
+    assert grad[3] + grad[4] == pytest.approx(len(run.hits) / PDE, rel=1e-12)
+    assert grad[2] == 0.0  # no photon is reflected on the way up
+    # a hit whose photon ID lies beyond its event's records is not replayed
+    bad = run.hits[:3].copy()
+    bad["photon_id"][0] = len(mus) * per_record + 7
+    _, c2 = O.adjoint_emissions(sc, em, 41, bad, np.ones(abi.HIST_SLOTS), tables, 5)
+    assert c2[1] == 1 and c2[0] == 2
