@@ -18,7 +18,11 @@ transport step, per detection).  The CPU baseline transports a bounded sample
 of the same workload with the FP32 oracle on the host's cores.  The
 `adjoint` object times a gradient pass of the differentiation workload (1e7
 photons, 66 optical parameters, MSE loss on count + histogram; forward, loss
-and the path-replay adjoint through torch.autograd) against a forward pass.
+and the path-replay adjoint through torch.autograd) against a forward pass;
+`array_events` times 1024 uneven electron events in the 8x8 crystal array
+(batched vs per-event launches, 2048-parameter adjoint); `electron_events`
+times 5000 1 MeV electron events in the crystal, with the FP32 oracle at 6
+threads and at all host threads on a sample of them.
 """
 from __future__ import annotations
 
@@ -54,6 +58,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-array", action="store_true")
+    ap.add_argument("--no-c1", action="store_true")
     ap.add_argument("--retune", action="store_true", help="ignore the autotuner cache")
     ap.add_argument("--launch", type=str, default=None,
                     help="explicit persistent form block,blocks_per_cu,chunk,refill (skips the autotuner)")
@@ -157,6 +163,83 @@ def run_adjoint(cs, src, seed, launch, steps, warmup, photons=10_000_000):
             "replay_mismatches": stats["mismatched"], "grad_norm": float(torch.linalg.vector_norm(g))}
 
 
+def run_emission_batches(host, scene, em, ev, idx, launch, steps, warmup):
+    """Emission-record batches: wall seconds per batch, mean kernel ms, last output."""
+    host.set_launch(launch)
+    for _ in range(warmup):
+        host.enqueue_emissions(scene, em, ev, idx)
+        host.complete_device()
+    kms = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        host.enqueue_emissions(scene, em, ev, idx)
+        out, _ = host.complete_device()
+        kms.append(out.statistics["kernel_ms"])
+    return (time.perf_counter() - t0) / steps, float(np.mean(kms)), out
+
+
+def run_array(seed, launch, steps, warmup, plain):
+    """8x8 array with septa (128 boxes), 1024 uneven electron events (log-normal
+    median 2e4 photons): batched (persistent + hipGraph) vs per-event plain
+    launches, and the 2048-parameter per-crystal absorption gradient."""
+    from g4go_amd import abi, workloads
+    from g4go_amd.transport import HipTransportHost, LaunchForm, PhotonTransportConfig
+
+    cs = workloads.array_scene()
+    em, ev, idx, _ = workloads.array_events(cs, 1024, median=2e4, seed=seed)
+    photons = int(em["photon_count"].astype(np.int64).sum())
+    host = HipTransportHost(PhotonTransportConfig(seed=seed, sort_hits=False))
+    host.prepare_scene(cs.scene)
+    host.set_parameter_layout([(abi.PARAM_ABS_LENGTH, m) for m in cs.crystal_materials])
+    graphed = LaunchForm(**{**launch.__dict__, "graph": True})
+    b_wall, b_kms, b_out = run_emission_batches(host, cs.scene, em, ev, idx, graphed, steps, warmup)
+    _, adj = host.adjoint(np.ones(abi.HIST_SLOTS))
+    res = {"events": 1024, "photons": photons, "boxes": 128, "parameters": 64 * 32,
+           "batched_photons_per_s": photons / b_wall, "batched_kernel_ms": b_kms,
+           "steps_per_photon": b_out.statistics["total_steps"] / max(1, b_out.statistics["generated"]),
+           "detected_fraction": b_out.statistics["detected"] / max(1, b_out.statistics["generated"]),
+           "adjoint_kernel_ms": adj["kernel_ms"], "adjoint_over_forward_kernel": adj["kernel_ms"] / b_kms,
+           "replay_mismatches": adj["mismatched"]}
+    if plain:
+        e_wall, e_kms, e_out = run_emission_batches(host, cs.scene, em, ev, idx,
+                                                    LaunchForm(persistent=False, per_event=True), steps, warmup)
+        assert e_out.statistics["detected"] == b_out.statistics["detected"]
+        res.update({"plain_per_event_photons_per_s": photons / e_wall, "plain_per_event_kernel_ms": e_kms,
+                    "batched_over_plain": e_wall / b_wall})
+    host.close()
+    return res
+
+
+def run_c1(seed, launch, steps, warmup, cpu, cpu_events):
+    """5000 events of 1 MeV electrons in the crystal (scintillation records, 30
+    photons/keV): device photons/s, and the FP32 oracle on `cpu_events` events at
+    6 threads and at all (up to 16) host threads."""
+    from g4go_amd import workloads
+    from g4go_amd.transport import HipTransportHost, PhotonTransportConfig
+
+    cs = workloads.crystal_scene()
+    em, ev, idx = workloads.electron_emissions(cs, 5000, seed=seed)
+    photons = int(em["photon_count"].astype(np.int64).sum())
+    host = HipTransportHost(PhotonTransportConfig(seed=seed, sort_hits=False))
+    host.prepare_scene(cs.scene)
+    wall, kms, out = run_emission_batches(host, cs.scene, em, ev, idx, launch, steps, warmup)
+    host.close()
+    res = {"events": 5000, "photons": photons, "photons_per_s": photons / wall, "kernel_ms": kms,
+           "h2d_ms": out.statistics["host_to_device_ms"], "record_bytes": int(em.nbytes)}
+    if cpu:
+        from oracle import oracle as O
+
+        sel = idx < cpu_events
+        n = int(em["photon_count"][sel].astype(np.int64).sum())
+        for threads in (6, max(1, min(16, os.cpu_count() or 1))):
+            t0 = time.perf_counter()
+            O.run_emissions(cs.scene, em[sel], ev[:cpu_events], idx[sel], seed, threads=threads)
+            dt = time.perf_counter() - t0
+            res[f"cpu_{threads}_threads_photons_per_s"] = n / dt
+        res["cpu_sample"] = f"first {cpu_events} events ({n} photons), FP32 oracle (oracle/oracle.c)"
+    return res
+
+
 def cpu_baseline(cs, src, seed, sample):
     from oracle import oracle as O
 
@@ -197,6 +280,7 @@ def main():
     src = workloads.synthetic_source(cs)
     host = HipTransportHost(PhotonTransportConfig(seed=args.seed, sort_hits=False))
     device, _ = host.device_info()
+    tune_key = autotune.device_key(host)
     ev1 = np.array([0], np.uint32)
 
     def tune_batch():
@@ -209,7 +293,7 @@ def main():
         persistent = LaunchForm(persistent=True, block_size=bs, chunk=chunk, refill_threshold=refill,
                                 blocks_per_cu=bpc)
     else:
-        persistent, table = autotune.tuned_launch(host, device, "crystal-synthetic", tune_batch,
+        persistent, table = autotune.tuned_launch(host, tune_key, "crystal-synthetic", tune_batch,
                                                   retune=args.retune)
         out_dir = os.path.join(ROOT, "gpurun_out")
         if table is not None and os.path.isdir(out_dir):
@@ -239,6 +323,8 @@ def main():
                        "batched_over_plain": e_wall / b_wall})
     host.close()
     adjoint = None if args.no_adjoint else run_adjoint(cs, src, args.seed, persistent, args.steps, args.warmup)
+    array = None if args.no_array else run_array(args.seed, persistent, args.steps, args.warmup, not args.no_plain)
+    c1 = None if args.no_c1 else run_c1(args.seed, persistent, args.steps, args.warmup, not args.no_cpu, 200)
 
     st = out.statistics
     P, S, D = st["generated"], st["total_steps"], st["n_hits"]
@@ -271,6 +357,8 @@ def main():
         "plain": plain,
         "uneven_events": uneven,
         "adjoint": adjoint,
+        "array_events": array,
+        "electron_events": c1,
         "batched_over_plain": (None if plain is None else (args.photons * args.steps / wall) / plain["photons_per_s"]),
         "counts": {"photons": P, "steps": S, "detected": D, "steps_per_photon": S / max(P, 1),
                    "detected_fraction": D / max(P, 1)},

@@ -5,7 +5,8 @@ blocks per CU (grid depth), photons per queue item and the lane-refill
 threshold -- on a sample of the caller's workload, timing each candidate by
 the transport kernel's own HIP-event duration, and keeps the fastest.  Results
 are cached per device and workload tag in autotune_cache.json next to this
-file.
+file.  The device key is the architecture name and compute-unit count
+(device_key), so that boxes reporting different marketing names share it.
 """
 from __future__ import annotations
 
@@ -51,6 +52,12 @@ def tune(host: HipTransportHost, run_batch: Callable[[], float], space: Optional
             best, best_ms = form, ms
     host.set_launch(best)
     return best, table
+
+
+def device_key(host: HipTransportHost) -> str:
+    """'<gcnArchName>/<CUs>CU' of the host's device, e.g. 'gfx950:sramecc+:xnack-/256CU'."""
+    name, cu = host.device_info()
+    return f"{name.split()[0] if name else 'unknown'}/{cu}CU"
 
 
 def _load_cache() -> dict:

@@ -158,7 +158,7 @@ uint32_t host_find_surface(const HostScene& s, uint32_t from, uint32_t to) {
 void build_bvh(HostScene& s, Volume& vol, uint32_t begin, uint32_t count) {
     // Median split on the longest axis of the centroid bounds; leaves of <= 4.
     struct Task {
-        uint32_t node, first, count;
+        uint32_t node, first, count, depth;
     };
     auto bounds = [&](uint32_t first, uint32_t n, float* lo, float* hi) {
         for (int k = 0; k < 3; ++k) {
@@ -175,10 +175,13 @@ void build_bvh(HostScene& s, Volume& vol, uint32_t begin, uint32_t count) {
     };
     vol.bvh_root = static_cast<uint32_t>(s.bvh.size());
     s.bvh.push_back(BvhNode{});
-    std::vector<Task> stack{{vol.bvh_root, begin, count}};
+    std::vector<Task> stack{{vol.bvh_root, begin, count, 0u}};
     while (!stack.empty()) {
         const Task t = stack.back();
         stack.pop_back();
+        if (t.depth > static_cast<uint32_t>(kBvhMaxDepth)) {
+            fail(G4GO_ERROR_INVALID_ARGUMENT, "daughter BVH deeper than the device traversal stack");
+        }
         BvhNode node{};
         bounds(t.first, t.count, node.lo, node.hi);
         if (t.count <= 4) {
@@ -219,8 +222,8 @@ void build_bvh(HostScene& s, Volume& vol, uint32_t begin, uint32_t count) {
         s.bvh.push_back(BvhNode{});
         s.bvh.push_back(BvhNode{});
         s.bvh[t.node] = node;
-        stack.push_back({node.left, t.first, half});
-        stack.push_back({node.right_or_count, t.first + half, t.count - half});
+        stack.push_back({node.left, t.first, half, t.depth + 1});
+        stack.push_back({node.right_or_count, t.first + half, t.count - half, t.depth + 1});
     }
 }
 

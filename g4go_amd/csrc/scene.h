@@ -70,6 +70,7 @@ struct BvhNode {
     uint32_t right_or_count;  // inner: right child node; leaf: count | kBvhLeaf
 };
 constexpr uint32_t kBvhLeaf = 0x80000000u;
+constexpr int kBvhMaxDepth = 15;  // root at depth 0; the host rejects deeper trees
 
 // Box of the daughter in one slot of the daughter list (slot order = the
 // list's order, so BVH leaves address it directly).  32 bytes: the BVH walk

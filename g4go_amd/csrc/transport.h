@@ -353,31 +353,61 @@ G4GO_HD float nearest_daughter(const SceneView& sc, const Volume& vol, V3 p, V3 
         }
         return best;
     }
-    uint32_t stack[32];
+    // Near-first traversal: the nearer child is visited first and every node
+    // whose entry distance exceeds the best hit so far is skipped, so the walk
+    // usually ends after the leaves around the answer.  The stack holds at most
+    // one entry per level (the farther child) plus one; the host limits the
+    // tree depth to kBvhMaxDepth.
+    uint32_t stack[kBvhMaxDepth + 1];
+    float stack_t[kBvhMaxDepth + 1];
     int top = 0;
-    stack[top++] = vol.bvh_root;
+    {
+        const BvhNode& root = sc.bvh[vol.bvh_root];
+        const float t = node_entry(root.lo, root.hi, p, inv, tmin);
+        if (t < kInf) {
+            stack[0] = vol.bvh_root;
+            stack_t[0] = t;
+            top = 1;
+        }
+    }
     while (top > 0) {
-        const BvhNode& node = sc.bvh[stack[--top]];
-        int a = 0;
-        const float tn = node_entry(node.lo, node.hi, p, inv, tmin);
-        if (!(tn <= best)) {
+        --top;
+        if (!(stack_t[top] <= best)) {
             continue;
         }
+        const BvhNode& node = sc.bvh[stack[top]];
         if ((node.right_or_count & kBvhLeaf) != 0) {
             const uint32_t count = node.right_or_count & ~kBvhLeaf;
             for (uint32_t k = 0; k < count; ++k) {
                 const DaughterBox& b = sc.dbox[node.left + k];
                 const uint32_t j = b.volume;
+                int a = 0;
                 const float t = box_entry(b.lo, b.hi, p, inv, tmin, a);
-                if (t < best || (t == best && t < kInf && j < which)) {
+                if (t < best || (t == best && t < kInf && j < which)) {  // ties: lower volume index
                     best = t;
                     which = j;
                     axis = a;
                 }
             }
-        } else if (top + 2 <= 32) {
-            stack[top++] = node.right_or_count;
-            stack[top++] = node.left;
+        } else {
+            const uint32_t l = node.left, r = node.right_or_count;
+            const BvhNode& nl = sc.bvh[l];
+            const BvhNode& nr = sc.bvh[r];
+            const float tl = node_entry(nl.lo, nl.hi, p, inv, tmin);
+            const float tr = node_entry(nr.lo, nr.hi, p, inv, tmin);
+            const bool left_near = tl <= tr;
+            const float t_far = left_near ? tr : tl;
+            const float t_near = left_near ? tl : tr;
+            if (t_far < kInf && t_far <= best && top <= kBvhMaxDepth) {
+                stack[top] = left_near ? r : l;
+                stack_t[top] = t_far;
+                ++top;
+            }
+            if (t_near < kInf && t_near <= best && top <= kBvhMaxDepth) {
+                stack[top] = left_near ? l : r;
+                stack_t[top] = t_near;
+                ++top;
+            }
         }
     }
     return best;

@@ -47,8 +47,7 @@ def main(argv=None) -> int:
     cs = workloads.array_scene() if args.workload == "array" else workloads.crystal_scene()
     src = workloads.synthetic_source(cs, crystal_index=27 if args.workload == "array" else 0)
     host = HipTransportHost(PhotonTransportConfig(seed=args.seed, sort_hits=False))
-    device, _ = host.device_info()
-    form = autotune.cached(device, "crystal-synthetic") or LaunchForm(persistent=True, block_size=256, chunk=256,
+    form = autotune.cached(autotune.device_key(host), "crystal-synthetic") or LaunchForm(persistent=True, block_size=256, chunk=256,
                                                                       refill_threshold=16, blocks_per_cu=4)
     if args.form == "plain":
         form = LaunchForm(persistent=False)

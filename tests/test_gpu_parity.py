@@ -208,3 +208,57 @@ def test_batched_graph_replays_and_per_event_launches(crystal):
                 np.testing.assert_array_equal(got.histogram, want.histogram)
         ref = O.run_synthetic(crystal.scene, src, 5, ev, batches[-1], threads=4)
     assert_same_hits(want.detections, ref.hits)
+
+
+def forest_scene(n_side=7, seed=4):
+    """Air world holding n_side^3 absorbing, scattering glass boxes of random
+    sizes on a jittered grid (every fifth box a sensor with a partly reflecting
+    skin): photons start below the forest and walk the daughter BVH at every
+    world step."""
+    from g4go_amd.scene import Material, PropertyTable, Scene, Surface, SurfaceType, Volume
+
+    rng = np.random.default_rng(seed)
+    sc = Scene()
+    air = sc.add_material(Material(name="air", rindex=PropertyTable.constant_value(1.0)))
+    glass = sc.add_material(Material(name="glass", rindex=PropertyTable.constant_value(1.5),
+                                     abs_length_mm=PropertyTable.constant_value(30.0),
+                                     rayleigh_length_mm=PropertyTable.constant_value(80.0)))
+    det = sc.add_surface(Surface(name="det", type=SurfaceType.DielectricMetal,
+                                 reflectivity=PropertyTable.constant_value(0.2),
+                                 efficiency=PropertyTable.constant_value(0.6)))
+    world = sc.add_volume(Volume(name="world", material_id=air, half_mm=(60, 60, 60)))
+    sc.world_volume_id = world
+    cell = 80.0 / n_side
+    k = 0
+    for i in range(n_side):
+        for j in range(n_side):
+            for m in range(n_side):
+                half = rng.uniform(0.15, 0.45, 3) * cell
+                c = np.array([-40 + (i + 0.5) * cell, -40 + (j + 0.5) * cell, -40 + (m + 0.5) * cell])
+                c += rng.uniform(-1, 1, 3) * (0.5 * cell - half)
+                sensor = k % 5 == 0
+                sc.add_volume(Volume(name=f"b{k}", material_id=glass, parent_volume_id=world,
+                                     skin_surface_id=det if sensor else abi.INVALID_ID,
+                                     sensor_id=k if sensor else abi.INVALID_ID, center_mm=tuple(c),
+                                     half_mm=tuple(half)))
+                k += 1
+    src = abi.SyntheticSource()
+    src.lo_mm[:] = [-45.0, -45.0, -50.0]
+    src.hi_mm[:] = [45.0, 45.0, -45.0]
+    src.wavelength_mean_nm, src.wavelength_sigma_nm, src.decay_time_ns = 420.0, 20.0, 5.0
+    src.volume_id = world
+    return sc, src
+
+
+def test_box_forest_bvh_bit_exact():
+    """343 daughters (LDS-staged BVH): hits and counters bit-exact against the oracle."""
+    log("case: 343-box forest, BVH walk in LDS")
+    sc, src = forest_scene()
+    ref = O.run_synthetic(sc, src, 9, [0, 1], [20000, 7000], threads=4)
+    with host(9) as h:
+        h.set_launch(LaunchForm(persistent=True, block_size=256, chunk=256, refill_threshold=16, blocks_per_cu=4))
+        h.enqueue_synthetic(sc, src, np.array([0, 1], np.uint32), np.array([20000, 7000], np.uint32))
+        out = h.complete_oldest_batch()
+    assert out.statistics["detected"] > 500 and out.statistics["escaped"] > 1000
+    assert_same_hits(out.detections, ref.hits)
+    assert_same_stats(out, ref)
