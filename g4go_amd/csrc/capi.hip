@@ -58,6 +58,7 @@ struct HostScene {
     std::vector<Binding> bindings;
     std::vector<BvhNode> bvh;
     uint32_t world = kInvalid;
+    uint32_t hot_offset = 0, hot_count = 0;  // hot energy grid (mark_hot_grid)
     std::unordered_map<uint32_t, uint32_t> index_of;  // public volume ID -> dense index
     std::vector<uint32_t> volume_ids;                 // dense index -> public volume ID
 };
@@ -93,7 +94,8 @@ Prop add_property(HostScene& s, const g4go_property& p, const std::string& what)
     for (uint32_t i = 1; i < p.count; ++i) {
         constant = constant && p.values[i] == p.values[0];
     }
-    out.constant = constant ? 1u : 0u;  // PropertyTable::Constant (Scene.cpp:17-21)
+    out.flags = constant ? kPropConstant : 0u;  // PropertyTable::Constant (Scene.cpp:17-21)
+    out.value0 = p.values[0];
     s.pool.insert(s.pool.end(), p.energy_ev, p.energy_ev + p.count);
     s.pool.insert(s.pool.end(), p.values, p.values + p.count);
     return out;
@@ -131,7 +133,8 @@ Prop add_spectrum(HostScene& s, const g4go_property& p, const std::string& what)
     (void)what;
     out.offset = static_cast<uint32_t>(s.pool.size());
     out.count = p.count;
-    out.constant = 0;
+    out.flags = 0;
+    out.value0 = cdf[0];
     s.pool.insert(s.pool.end(), p.energy_ev, p.energy_ev + p.count);
     s.pool.insert(s.pool.end(), cdf.begin(), cdf.end());
     return out;
@@ -224,6 +227,57 @@ void build_bvh(HostScene& s, Volume& vol, uint32_t begin, uint32_t count) {
         s.bvh[t.node] = node;
         stack.push_back({node.left, t.first, half, t.depth + 1});
         stack.push_back({node.right_or_count, t.first + half, t.count - half, t.depth + 1});
+    }
+}
+
+// The energy grid shared by the most non-constant tables of materials and
+// surfaces (ties: the first seen) becomes the hot grid; every table on it is
+// flagged kPropHotGrid.  Spectrum CDFs are sampled, not interpolated, and stay
+// out.  Tables on one grid have bit-identical energy arrays.
+void mark_hot_grid(HostScene& s) {
+    std::vector<Prop*> props;
+    for (Material& m : s.materials) {
+        for (Prop* p : {&m.rindex, &m.group_velocity, &m.abs_length, &m.rayleigh_length}) {
+            props.push_back(p);
+        }
+    }
+    for (Surface& f : s.surfaces) {
+        for (Prop* p : {&f.reflectivity, &f.efficiency, &f.transmittance, &f.lobe, &f.spike, &f.backscatter,
+                        &f.roughness}) {
+            props.push_back(p);
+        }
+    }
+    auto same_grid = [&](const Prop& a, const Prop& b) {
+        return a.count == b.count &&
+               std::memcmp(s.pool.data() + a.offset, s.pool.data() + b.offset, a.count * sizeof(float)) == 0;
+    };
+    const Prop* best = nullptr;
+    size_t best_uses = 0;
+    for (const Prop* a : props) {
+        if (a->count < 2) {
+            continue;
+        }
+        size_t uses = 0;
+        for (const Prop* b : props) {
+            uses += b->count >= 2 && (b->flags & kPropConstant) == 0 && same_grid(*a, *b) ? 1u : 0u;
+        }
+        if (uses > best_uses) {
+            best = a;
+            best_uses = uses;
+        }
+    }
+    s.hot_offset = 0;
+    s.hot_count = 0;
+    if (best == nullptr || best_uses == 0) {
+        return;
+    }
+    const Prop grid = *best;
+    s.hot_offset = grid.offset;
+    s.hot_count = grid.count;
+    for (Prop* p : props) {
+        if (p->count >= 2 && same_grid(grid, *p)) {
+            p->flags |= kPropHotGrid;
+        }
     }
 }
 
@@ -414,6 +468,7 @@ HostScene build_scene(const g4go_scene& in, float tolerance) {
     if (s.pool.empty()) {
         s.pool.push_back(0.0f);
     }
+    mark_hot_grid(s);
     return s;
 }
 
@@ -605,6 +660,8 @@ struct g4go_transport {
         view.bvh = reinterpret_cast<const BvhNode*>(base + o_bvh);
         view.dbox = reinterpret_cast<const DaughterBox*>(base + o_dbox);
         view.n_daughters = static_cast<uint32_t>(s.daughters.size());
+        view.hot_offset = s.hot_offset;
+        view.hot_count = s.hot_count;
         view.n_materials = static_cast<uint32_t>(s.materials.size());
         view.n_surfaces = static_cast<uint32_t>(s.surfaces.size());
         view.n_volumes = static_cast<uint32_t>(s.volumes.size());
@@ -1340,7 +1397,8 @@ g4go_status g4go_set_parameters(g4go_transport* h, const double* theta, uint64_t
                 values[k] = static_cast<float>(theta[t.offset + k]);
                 constant = constant && values[k] == values[0];
             }
-            prop->constant = constant ? 1u : 0u;
+            prop->flags = (prop->flags & ~kPropConstant) | (constant ? kPropConstant : 0u);
+            prop->value0 = values[0];
         }
         h->scene_uploaded = false;
         h->last_valid = false;

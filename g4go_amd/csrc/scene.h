@@ -17,11 +17,16 @@ namespace g4go {
 constexpr uint32_t kFaceDynamic = 0xfffffffeu;  // neighbour found by a point query at run time
 
 struct Prop {
-    uint32_t offset;    // energies at pool[offset], values at pool[offset + count]
-    uint32_t count;     // 0 = property absent
-    uint32_t constant;  // all values equal (OptiXTransportHost.cpp:356-369)
-    uint32_t pad;
+    uint32_t offset;  // energies at pool[offset], values at pool[offset + count]
+    uint32_t count;   // 0 = property absent
+    uint32_t flags;   // kPropConstant | kPropHotGrid
+    float value0;     // values[0]: the value of a constant table, read without the pool
 };
+// all values equal (OptiXTransportHost.cpp:356-369)
+constexpr uint32_t kPropConstant = 1u;
+// energies identical to the scene's hot grid (SceneView::hot_offset): evaluated
+// from the node index and weight each photon computes once for its energy
+constexpr uint32_t kPropHotGrid = 2u;
 
 struct Material {
     Prop rindex;
@@ -93,7 +98,11 @@ struct SceneView {
     const DaughterBox* dbox;  // one per daughter-list slot
     uint32_t n_materials, n_surfaces, n_volumes, n_bindings;
     uint32_t n_bvh_nodes, world, n_daughters;
-    uint32_t pad;
+    // The energy grid shared by the most non-constant tables (energies at
+    // pool[hot_offset], hot_count nodes; hot_count = 0: none).  A photon's
+    // energy never changes, so its node on this grid is located once.
+    uint32_t hot_offset, hot_count;
+    uint32_t pad[3];
 };
 
 }  // namespace g4go

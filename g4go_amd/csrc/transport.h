@@ -67,6 +67,10 @@ struct Photon {
     uint32_t photon_id, event_id;
     uint32_t material;  // material whose values are cached below
     float rindex, inv_velocity, interaction_length, absorb_fraction;
+    // node of the photon's energy on the scene's hot grid: grid_i = kInvalid
+    // until located; grid_f < 0 marks a clamped value (values[grid_i] itself)
+    uint32_t grid_i;
+    float grid_f;
 };
 
 enum StepStatus : int {
@@ -102,9 +106,12 @@ G4GO_HD float prop_eval(const float* pool, const Prop& p, float e, float fallbac
     if (p.count == 0) {
         return fallback;
     }
+    if ((p.flags & kPropConstant) != 0) {
+        return p.value0;
+    }
     const float* en = pool + p.offset;
     const float* va = en + p.count;
-    if (p.constant != 0 || e <= en[0]) {
+    if (e <= en[0]) {
         return va[0];
     }
     const uint32_t last = p.count - 1;
@@ -123,6 +130,55 @@ G4GO_HD float prop_eval(const float* pool, const Prop& p, float e, float fallbac
     const uint32_t i = lo - 1;
     const float f = (e - en[i]) / (en[i + 1] - en[i]);
     return va[i] + f * (va[i + 1] - va[i]);
+}
+
+// Node of energy e on a grid of `count` ascending energies, as prop_eval
+// selects it: f < 0 marks a clamped value (e at or below the first node, or at
+// or above the last), otherwise the value is va[i] + f (va[i+1] - va[i]) with
+// the f prop_eval computes.
+G4GO_HD void grid_locate(const float* en, uint32_t count, float e, uint32_t& i, float& f) {
+    const uint32_t last = count - 1;
+    if (count <= 1 || e <= en[0]) {
+        i = 0;
+        f = -1.0f;
+        return;
+    }
+    if (e >= en[last]) {
+        i = last;
+        f = -1.0f;
+        return;
+    }
+    uint32_t lo = 1, hi = last;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (e > en[mid]) {
+            lo = mid + 1;
+        } else {
+            hi = mid;
+        }
+    }
+    i = lo - 1;
+    f = (e - en[i]) / (en[i + 1] - en[i]);
+}
+
+// prop_eval at the photon's energy, through its located hot-grid node when the
+// table lies on the hot grid (bit-identical to prop_eval).
+G4GO_HD float prop_eval_ph(const float* pool, const Prop& p, const Photon& ph, float fallback) {
+    if (p.count == 0) {
+        return fallback;
+    }
+    if ((p.flags & kPropConstant) != 0) {
+        return p.value0;
+    }
+    if ((p.flags & kPropHotGrid) != 0 && ph.grid_i != kInvalid) {
+        const float* va = pool + p.offset + p.count;
+        const uint32_t i = ph.grid_i;
+        if (ph.grid_f < 0.0f) {
+            return va[i];
+        }
+        return va[i] + ph.grid_f * (va[i + 1] - va[i]);
+    }
+    return prop_eval(pool, p, ph.energy, fallback);
 }
 
 // Interpolation node and weight of prop_eval (for parameter gradients):
@@ -195,15 +251,15 @@ G4GO_HD bool cache_material(Photon& ph, const SceneView& sc, uint32_t material) 
         return false;
     }
     const Material& m = sc.materials[material];
-    const float n = prop_eval(sc.pool, m.rindex, ph.energy, u2f(0x7fc00000u));
+    const float n = prop_eval_ph(sc.pool, m.rindex, ph, u2f(0x7fc00000u));
     const bool n_ok = finitef(n) && n > 0.0f;
-    const float v = prop_eval(sc.pool, m.group_velocity, ph.energy,
+    const float v = prop_eval_ph(sc.pool, m.group_velocity, ph,
                               n_ok ? kSpeedOfLightMmPerNs / n : u2f(0x7fc00000u));
     if (!(v > 0.0f) || !finitef(v)) {
         return false;
     }
-    const float la = prop_eval(sc.pool, m.abs_length, ph.energy, kInf);
-    const float lr = prop_eval(sc.pool, m.rayleigh_length, ph.energy, kInf);
+    const float la = prop_eval_ph(sc.pool, m.abs_length, ph, kInf);
+    const float lr = prop_eval_ph(sc.pool, m.rayleigh_length, ph, kInf);
     const bool a_ok = finitef(la) && la > 0.0f;
     const bool r_ok = finitef(lr) && lr > 0.0f;
     ph.material = material;
@@ -491,9 +547,9 @@ G4GO_HD V3 sample_unified_reflection(const SceneView& sc, const Surface& s, V3 d
                                      const Photon& ph) {
     const bool ground = s.finish == kGround;
     const V3 facet = sample_facet_normal(s, n, dir, rng, ph);
-    const float spike = clamp01(prop_eval(sc.pool, s.spike, ph.energy, ground ? 0.0f : 1.0f));
-    const float lobe = clamp01(prop_eval(sc.pool, s.lobe, ph.energy, 0.0f));
-    const float back = clamp01(prop_eval(sc.pool, s.backscatter, ph.energy, 0.0f));
+    const float spike = clamp01(prop_eval_ph(sc.pool, s.spike, ph, ground ? 0.0f : 1.0f));
+    const float lobe = clamp01(prop_eval_ph(sc.pool, s.lobe, ph, 0.0f));
+    const float back = clamp01(prop_eval_ph(sc.pool, s.backscatter, ph, 0.0f));
     const U4 w = rng.block(ph.photon_id, ph.event_id, ph.step, kProcLobe, 0);
     const float u = u01(w.x);
     if (u < spike) {
@@ -579,6 +635,7 @@ G4GO_HD void generate_synthetic(Photon& ph, const SyntheticSource& src, const Rn
     ph.step = 0;
     ph.volume = src.volume;
     ph.material = kInvalid;
+    ph.grid_i = kInvalid;
     const U4 a = rng.block(photon_id, event_id, 0, kProcSynthetic, 0);
     const U4 b = rng.block(photon_id, event_id, 0, kProcSynthetic, 1);
     const float cos_t = 1.0f - 2.0f * u01(a.x);
@@ -607,6 +664,7 @@ G4GO_HD void generate_emission(Photon& ph, const Emission& em, const SceneView& 
     ph.step = 0;
     ph.volume = em.volume;
     ph.material = kInvalid;
+    ph.grid_i = kInvalid;
     ph.pos = V3{em.pos[0], em.pos[1], em.pos[2]};
     ph.time = static_cast<double>(em.time);
     ph.dir = normalize(V3{em.dir[0], em.dir[1], em.dir[2]});
@@ -746,6 +804,9 @@ G4GO_HD int transport_step(Photon& ph, const SceneView& sc, const Rng& rng, cons
     if (ph.volume >= sc.n_volumes) {
         return kInvalidState;
     }
+    if (ph.grid_i == kInvalid && sc.hot_count != 0) {
+        grid_locate(sc.pool + sc.hot_offset, sc.hot_count, ph.energy, ph.grid_i, ph.grid_f);
+    }
     const Volume& vol = sc.volumes[ph.volume];
     if (vol.material != ph.material && !cache_material(ph, sc, vol.material)) {
         return kInvalidState;
@@ -856,9 +917,8 @@ G4GO_HD int transport_step(Photon& ph, const SceneView& sc, const Rng& rng, cons
         if (kTrace) {
             trace->surface = surface;
         }
-        const float refl = clamp01(prop_eval(sc.pool, s.reflectivity, e, 0.0f));
-        const float eff = clamp01(prop_eval(sc.pool, s.efficiency, e, 0.0f));
-        const float trans = clamp01(prop_eval(sc.pool, s.transmittance, e, 0.0f));
+        const float refl = clamp01(prop_eval_ph(sc.pool, s.reflectivity, ph, 0.0f));
+        const float trans = clamp01(prop_eval_ph(sc.pool, s.transmittance, ph, 0.0f));
         const int outcome = classify_surface(refl, trans, s.reflectivity.count != 0, s.transmittance.count != 0, u01(w.z));
         if (kTrace) {
             trace->event = outcome == kInteraction ? kTraceInteraction
@@ -866,16 +926,17 @@ G4GO_HD int transport_step(Photon& ph, const SceneView& sc, const Rng& rng, cons
         }
         if (outcome == kAbsorb) {
             absorbed = true;
+            const float eff = clamp01(prop_eval_ph(sc.pool, s.efficiency, ph, 0.0f));
             detected = evaluate_absorption(eff, nvol.sensor != kInvalid || vol.sensor != kInvalid, u01(w.w)) == kDetect;
         } else if (outcome == kDirectTransmit) {
             transmitted = true;
         } else if (s.type == kDielectricDielectric) {
-            const float n2 = prop_eval(sc.pool, nmat.rindex, e, u2f(0x7fc00000u));
+            const float n2 = prop_eval_ph(sc.pool, nmat.rindex, ph, u2f(0x7fc00000u));
             if (!(n1 > 0.0f) || !finitef(n1) || !(n2 > 0.0f) || !finitef(n2)) {
                 absorbed = true;
             } else {
                 const V3 inormal = s.finish == kPolished ? normalize(normal) : sample_facet_normal(s, normal, dir, rng, ph);
-                const float rough = prop_eval(sc.pool, s.roughness, e, 0.0f);
+                const float rough = prop_eval_ph(sc.pool, s.roughness, ph, 0.0f);
                 bool rough_pass = true;
                 U4 wr = {0u, 0u, 0u, 0u};
                 if (rough > 0.0f && n1 > n2 && finitef(rough) && finitef(e) && e > 0.0f) {
@@ -923,7 +984,7 @@ G4GO_HD int transport_step(Photon& ph, const SceneView& sc, const Rng& rng, cons
             }
         }
     } else {
-        const float n2 = prop_eval(sc.pool, nmat.rindex, e, u2f(0x7fc00000u));
+        const float n2 = prop_eval_ph(sc.pool, nmat.rindex, ph, u2f(0x7fc00000u));
         if (!(n1 > 0.0f) || !finitef(n1) || !(n2 > 0.0f) || !finitef(n2)) {
             absorbed = true;
         } else {
