@@ -566,6 +566,11 @@ struct g4go_transport {
     uint64_t h2d_bytes = 0;
 
     DevBuf d_inputs, d_hits, d_event_stats, d_meta;
+    // launch index of each hit, and the launch-order copies (cfg.sort_hits)
+    DevBuf d_hit_launch, d_hits_ordered, d_launch_ordered, d_order;
+    bool ordered = false;  // the last batch's final hits are the launch-order copies
+    const Hit* final_hits() const { return ordered ? d_hits_ordered.as<Hit>() : d_hits.as<Hit>(); }
+    const uint32_t* final_launch() const { return ordered ? d_launch_ordered.as<uint32_t>() : d_hit_launch.as<uint32_t>(); }
     // differentiation
     struct ParamTable {
         uint32_t kind, index, offset, count;
@@ -573,7 +578,7 @@ struct g4go_transport {
     std::vector<ParamTable> param_tables;
     uint32_t n_params = 0;
     std::vector<uint32_t> param_map;  // [abs(n_mat)][rayleigh(n_mat)][reflectivity(n_surf)][efficiency(n_surf)]
-    DevBuf d_param_map, d_hit_source, d_adjoint;
+    DevBuf d_param_map, d_adjoint;
     KernelParams last_params{};
     int last_source = -1;
     bool last_valid = false;
@@ -583,7 +588,7 @@ struct g4go_transport {
     int graph_grid = 0, graph_block = 0, graph_source = -1;
     PinBuf h_stage, h_meta, h_hits;
 
-    std::vector<g4go_hit> out_hits;
+    uint64_t out_n_hits = 0;  // hits of the last completed batch in h_hits
     std::vector<g4go_event_stats> out_events;
     uint64_t out_hist[kHistSlots] = {};
 
@@ -677,7 +682,10 @@ struct g4go_transport {
         }
         drop_graph();
         d_param_map.release();
-        d_hit_source.release();
+        d_hit_launch.release();
+        d_hits_ordered.release();
+        d_launch_ordered.release();
+        d_order.release();
         d_adjoint.release();
         d_scene.release();
         d_inputs.release();
@@ -722,7 +730,7 @@ struct g4go_transport {
     // [item offsets][source photon counts][source event indices].
     struct Layout {
         uint64_t cap_em = 0, cap_ev = 0, cap_src = 0;
-        size_t o_em = 0, o_ev = 0, o_off = 0, o_cnt = 0, o_sev = 0, bytes = 0;
+        size_t o_em = 0, o_ev = 0, o_off = 0, o_cnt = 0, o_sev = 0, o_poff = 0, bytes = 0;
     };
     Layout layout;
 
@@ -742,7 +750,8 @@ struct g4go_transport {
         l.o_off = l.o_ev + align256(l.cap_ev * sizeof(uint32_t));
         l.o_cnt = l.o_off + align256((l.cap_src + 1) * sizeof(uint32_t));
         l.o_sev = l.o_cnt + align256(l.cap_src * sizeof(uint32_t));
-        l.bytes = l.o_sev + align256(l.cap_src * sizeof(uint32_t));
+        l.o_poff = l.o_sev + align256(l.cap_src * sizeof(uint32_t));
+        l.bytes = l.o_poff + align256((l.cap_src + 1) * sizeof(uint32_t));
         d_inputs.release();
         d_inputs.ensure(l.bytes, "hipMalloc batch inputs");
         h_stage.ensure(l.bytes, "hipHostMalloc staging");
@@ -770,14 +779,16 @@ struct g4go_transport {
         upload_scene();
         const uint64_t n_events = event_ids.size();
         const uint64_t n_sources = source_counts.size();
-        std::vector<uint32_t> offsets(n_sources + 1, 0);
+        std::vector<uint32_t> offsets(n_sources + 1, 0), photon_offsets(n_sources + 1, 0);
         for (uint64_t i = 0; i < n_sources; ++i) {
             offsets[i + 1] = offsets[i] + (source_counts[i] + chunk - 1) / chunk;
+            photon_offsets[i + 1] = photon_offsets[i] + source_counts[i];  // < 2^32: checked at enqueue
         }
         n_items = offsets[n_sources];
         ensure_layout(n_emissions, n_events, n_sources);
         const Layout& l = layout;
         d_hits.ensure(std::max<uint64_t>(n_photons, 1) * sizeof(Hit), "hipMalloc hits");
+        d_hit_launch.ensure(std::max<uint64_t>(n_photons, 1) * sizeof(uint32_t), "hipMalloc hit launch indices");
         d_event_stats.ensure(l.cap_ev * sizeof(EventStats), "hipMalloc event stats");
         d_meta.ensure(kMetaBytes, "hipMalloc counters");
         h_meta.ensure(kMetaBytes + n_events * sizeof(EventStats), "hipHostMalloc counters");
@@ -798,18 +809,20 @@ struct g4go_transport {
         std::memcpy(st + l.o_off, offsets.data(), off_bytes);
         if (cnt_bytes) std::memcpy(st + l.o_cnt, source_counts.data(), cnt_bytes);
         if (sev_bytes) std::memcpy(st + l.o_sev, source_event.data(), sev_bytes);
+        const size_t poff_bytes = (n_sources + 1) * sizeof(uint32_t);
+        std::memcpy(st + l.o_poff, photon_offsets.data(), poff_bytes);
         char* in = d_inputs.as<char>();
         hip_check(hipEventRecord(ev_start, stream), "hipEventRecord");
         const std::pair<size_t, size_t> regions[] = {
             {0, 16}, {l.o_em, em_bytes}, {l.o_ev, ev_bytes}, {l.o_off, off_bytes}, {l.o_cnt, cnt_bytes},
-            {l.o_sev, sev_bytes}};
+            {l.o_sev, sev_bytes}, {l.o_poff, poff_bytes}};
         for (const auto& r : regions) {
             if (r.second != 0) {
                 hip_check(hipMemcpyAsync(in + r.first, st + r.first, r.second, hipMemcpyHostToDevice, stream),
                           "hipMemcpyAsync batch inputs");
             }
         }
-        h2d_bytes = 16 + em_bytes + ev_bytes + off_bytes + cnt_bytes + sev_bytes;
+        h2d_bytes = 16 + em_bytes + ev_bytes + off_bytes + cnt_bytes + sev_bytes + poff_bytes;
 
         KernelParams p = base_params();
         if (synth != nullptr) {
@@ -821,14 +834,11 @@ struct g4go_transport {
         p.item_offsets = reinterpret_cast<const uint32_t*>(in + l.o_off);
         p.source_counts = reinterpret_cast<const uint32_t*>(in + l.o_cnt);
         p.source_event = reinterpret_cast<const uint32_t*>(in + l.o_sev);
+        p.source_photon_offset = reinterpret_cast<const uint32_t*>(in + l.o_poff);
         p.hits = d_hits.as<Hit>();
         p.hit_capacity = static_cast<uint32_t>(std::min<size_t>(d_hits.cap / sizeof(Hit), 0xffffffffu));
         p.hit_count = reinterpret_cast<uint32_t*>(d_meta.as<char>() + kMetaHitCount);
-        p.hit_source = nullptr;
-        if (source == kSourceEmission && !param_tables.empty()) {
-            d_hit_source.ensure(std::max<uint64_t>(n_photons, 1) * sizeof(uint32_t), "hipMalloc hit sources");
-            p.hit_source = d_hit_source.as<uint32_t>();
-        }
+        p.hit_launch = d_hit_launch.as<uint32_t>();
         p.histogram = reinterpret_cast<unsigned long long*>(d_meta.as<char>() + kMetaHist);
         p.event_stats = d_event_stats.as<EventStats>();
         p.counters = reinterpret_cast<unsigned long long*>(d_meta.as<char>() + kMetaCounters);
@@ -907,6 +917,29 @@ struct g4go_transport {
             }
         }
         hip_check(hipEventRecord(ev_kernel1, stream), "hipEventRecord");
+        ordered = cfg.sort_hits != 0 && n_photons != 0;
+        if (ordered) {
+            // hits in the launch order of their photons (a stable compaction)
+            const uint64_t words = (n_photons + 31) / 32;
+            const uint64_t blocks = (words + kOrderWordsPerBlock - 1) / kOrderWordsPerBlock;
+            d_hits_ordered.ensure(std::max<uint64_t>(n_photons, 1) * sizeof(Hit), "hipMalloc ordered hits");
+            d_launch_ordered.ensure(std::max<uint64_t>(n_photons, 1) * sizeof(uint32_t), "hipMalloc ordered launch");
+            d_order.ensure((2 * words + blocks) * sizeof(uint32_t), "hipMalloc order scratch");
+            OrderParams o{};
+            o.hits = d_hits.as<Hit>();
+            o.launch = d_hit_launch.as<uint32_t>();
+            o.hit_count = p.hit_count;
+            o.capacity = static_cast<uint32_t>(std::min<uint64_t>(n_photons, p.hit_capacity));
+            o.n_photons = static_cast<uint32_t>(n_photons);
+            o.mask = d_order.as<uint32_t>();
+            o.word_prefix = o.mask + words;
+            o.block_prefix = o.word_prefix + words;
+            o.out_hits = d_hits_ordered.as<Hit>();
+            o.out_launch = d_launch_ordered.as<uint32_t>();
+            if (launch_order_hits(o, stream) != 0) {
+                fail(G4GO_ERROR_RUNTIME, "hit ordering launch failed");
+            }
+        }
         hip_check(hipMemcpyAsync(h_meta.ptr, d_meta.ptr, kMetaBytes, hipMemcpyDeviceToHost, stream),
                   "hipMemcpyAsync counters");
         if (n_events) {
@@ -1196,9 +1229,9 @@ static g4go_status complete_batch(g4go_transport* h, g4go_batch_result* out, boo
         }
         h->busy = false;
         h->device_hits = 0;
+        h->out_n_hits = 0;
         const uint64_t n_events = h->event_ids.size();
         g4go_batch_stats bs{};
-        h->out_hits.clear();
         h->out_events.assign(n_events, g4go_event_stats{});
         std::fill(std::begin(h->out_hist), std::end(h->out_hist), 0ull);
         for (uint64_t i = 0; i < n_events; ++i) {
@@ -1221,21 +1254,15 @@ static g4go_status complete_batch(g4go_transport* h, g4go_batch_result* out, boo
             h->device_hits = n_hits;
             if (!copy_hits) {
                 if (hits_device != nullptr) {
-                    *hits_device = h->d_hits.ptr;
+                    *hits_device = h->final_hits();
                 }
             } else if (n_hits != 0) {
                 h->h_hits.ensure(n_hits * sizeof(Hit), "hipHostMalloc hits");
-                hip_check(hipMemcpyAsync(h->h_hits.ptr, h->d_hits.ptr, n_hits * sizeof(Hit), hipMemcpyDeviceToHost,
+                hip_check(hipMemcpyAsync(h->h_hits.ptr, h->final_hits(), n_hits * sizeof(Hit), hipMemcpyDeviceToHost,
                                          h->stream),
                           "hipMemcpyAsync hits");
                 hip_check(hipStreamSynchronize(h->stream), "hipStreamSynchronize");
-                h->out_hits.resize(n_hits);
-                std::memcpy(h->out_hits.data(), h->h_hits.ptr, n_hits * sizeof(Hit));
-                if (h->cfg.sort_hits) {
-                    std::sort(h->out_hits.begin(), h->out_hits.end(), [](const g4go_hit& a, const g4go_hit& b) {
-                        return a.event_id < b.event_id || (a.event_id == b.event_id && a.photon_id < b.photon_id);
-                    });
-                }
+                h->out_n_hits = n_hits;
             }
             const auto d2h1 = std::chrono::steady_clock::now();
             std::memcpy(h->out_hist, hist, sizeof(h->out_hist));
@@ -1267,8 +1294,8 @@ static g4go_status complete_batch(g4go_transport* h, g4go_batch_result* out, boo
         }
         bs.transport_time_ms =
             std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - h->started).count();
-        out->hits = copy_hits ? h->out_hits.data() : nullptr;
-        out->n_hits = copy_hits ? h->out_hits.size() : h->device_hits;
+        out->hits = copy_hits && h->out_n_hits != 0 ? h->h_hits.as<g4go_hit>() : nullptr;
+        out->n_hits = copy_hits ? h->out_n_hits : h->device_hits;
         out->events = h->out_events.data();
         out->n_events = h->out_events.size();
         out->histogram = h->out_hist;
@@ -1437,8 +1464,8 @@ g4go_status g4go_adjoint(g4go_transport* h, const double* slot_weight, double* g
         }
         AdjointParams a{};
         a.fwd = h->last_params;
-        a.hits = h->d_hits.as<Hit>();
-        a.hit_item = h->last_source == kSourceEmission ? h->d_hit_source.as<uint32_t>() : nullptr;
+        a.hits = h->final_hits();
+        a.hit_launch = h->final_launch();
         a.n_hits = static_cast<uint32_t>(h->device_hits);
         a.slot_weight = reinterpret_cast<const double*>(d + o_w);
         a.grad = reinterpret_cast<double*>(d + o_g);
@@ -1447,9 +1474,6 @@ g4go_status g4go_adjoint(g4go_transport* h, const double* slot_weight, double* g
         const auto* map = reinterpret_cast<const uint32_t*>(d);
         a.map = ParamMap{map, map + n_mat, map + 2 * n_mat, map + 2 * n_mat + n_surf, h->n_params};
         a.counters = reinterpret_cast<unsigned long long*>(d + o_c);
-        if (h->last_source == kSourceEmission && a.hit_item == nullptr) {
-            fail(G4GO_ERROR_LOGIC, "the batch ran before a parameter layout was set");
-        }
         const int block = 256;
         const int grid = static_cast<int>(std::max<uint64_t>(
             1, std::min<uint64_t>((a.n_hits + block - 1) / block, static_cast<uint64_t>(h->cu_count) * 8)));

@@ -53,11 +53,12 @@ struct KernelParams {
     const uint32_t* item_offsets;    // n_sources + 1 (item count = item_offsets[n_sources])
     const uint32_t* source_counts;   // photons per source record
     const uint32_t* source_event;    // event index per source record (emission source)
+    const uint32_t* source_photon_offset;  // n_sources + 1: first launch index of each source's photons
     uint32_t item_begin, item_end;   // item range of this launch; item_end == 0: the whole batch
     Hit* hits;
     uint32_t hit_capacity;
     uint32_t* hit_count;
-    uint32_t* hit_source;  // optional: work-item index of each hit (adjoint replay of emission sources)
+    uint32_t* hit_launch;  // launch index of each hit's photon (source_photon_offset[source] + local index)
     unsigned long long* histogram;
     EventStats* event_stats;
     unsigned long long* counters;
@@ -84,7 +85,7 @@ struct ParamMap {
 struct AdjointParams {
     KernelParams fwd;          // the forward batch being replayed
     const Hit* hits;           // its detections (device order)
-    const uint32_t* hit_item;  // work item of each detection (emission source)
+    const uint32_t* hit_launch;  // launch index of each detection's photon
     uint32_t n_hits;
     const double* slot_weight;  // dLoss/dN + dLoss/dh_b per histogram slot (kHistSlots)
     double* grad;               // n_params, accumulated
@@ -95,6 +96,25 @@ struct AdjointParams {
 int launch_transport(const KernelParams& p, int source, bool persistent, int block, int grid, hipStream_t stream);
 int launch_zero(void* ptr, size_t bytes, hipStream_t stream);
 int launch_adjoint(const AdjointParams& p, int source, int block, int grid, hipStream_t stream);
+
+// Launch-order output: reorders the hits of a batch into the order of their
+// photons' launch indices (the order of a stable compaction over the launch,
+// OptiXHitCompaction.cu:8-30).  Scratch: mask and word_prefix hold
+// ceil(n_photons / 32) words, block_prefix ceil(words / kOrderWordsPerBlock).
+constexpr uint32_t kOrderWordsPerBlock = 1024;
+struct OrderParams {
+    const Hit* hits;              // unordered, hit_count[0] of them (clamped to capacity)
+    const uint32_t* launch;       // their launch indices
+    const uint32_t* hit_count;
+    uint32_t capacity;
+    uint32_t n_photons;
+    uint32_t* mask;               // one bit per launch index
+    uint32_t* word_prefix;        // exclusive popcount prefix inside each block of words
+    uint32_t* block_prefix;       // exclusive prefix over blocks of words
+    Hit* out_hits;                // launch order
+    uint32_t* out_launch;
+};
+int launch_order_hits(const OrderParams& p, hipStream_t stream);
 int launch_trace(const KernelParams& p, uint32_t event_id, uint32_t photon_id, uint32_t max_steps, float* out,
                  uint32_t* n_out, hipStream_t stream);
 int launch_eval_math(const float* in, float* out, int n, int which, hipStream_t stream);

@@ -155,10 +155,10 @@ __device__ __forceinline__ void flush_stage(const KernelParams& p, const float4*
             dst[3ull * base + i] = stage[i];
         }
     }
-    if (p.hit_source != nullptr) {
+    if (p.hit_launch != nullptr) {
         for (uint32_t i = lane; i < staged; i += 64u) {
             if (base + i < p.hit_capacity) {
-                p.hit_source[base + i] = src[i];
+                p.hit_launch[base + i] = src[i];
             }
         }
     }
@@ -217,18 +217,17 @@ __global__ void __launch_bounds__(256) transport_kernel(KernelParams p) {
     ph.material = kInvalid;
     bool alive = false;
     uint32_t my_event = kInvalid;
-    uint32_t my_item = kInvalid;
+    uint32_t my_launch = kInvalid;
     LaneStats st{0, 0, 0, 0, 0, 0, 0};
     uint32_t lane_steps = 0, lane_photons = 0;
 
     // wave-uniform queue cursor
-    uint32_t it_index = kInvalid, it_event = 0, it_source = 0, it_first = 0, it_next = 0, it_end = 0;
+    uint32_t it_event = 0, it_source = 0, it_first = 0, it_next = 0, it_end = 0;
     bool done = false;
     if (!kPersistent) {
         const uint32_t wave = first_item + blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
         if (wave < end_item) {
             const ItemRef item = locate_item<kSource>(p, uniform(wave), n_sources, chunk);
-            it_index = wave;
             it_event = item.event_index;
             it_source = item.source_index;
             it_first = item.first;
@@ -257,7 +256,6 @@ __global__ void __launch_bounds__(256) transport_kernel(KernelParams p) {
                         break;
                     }
                     const ItemRef item = locate_item<kSource>(p, c, n_sources, chunk);
-                    it_index = c;
                     it_event = item.event_index;
                     it_source = item.source_index;
                     it_first = item.first;
@@ -273,7 +271,7 @@ __global__ void __launch_bounds__(256) transport_kernel(KernelParams p) {
                     my_event = it_event;
                     generate<kSource>(ph, p, it_event, it_source, it_first + it_next + rank);
                     ph.material = kInvalid;
-                    my_item = it_index;
+                    my_launch = p.source_photon_offset[it_source] + it_first + it_next + rank;
                     ++lane_photons;
                     if (start_photon(ph)) {
                         alive = true;
@@ -313,7 +311,7 @@ __global__ void __launch_bounds__(256) transport_kernel(KernelParams p) {
             if (status == kDetected) {
                 const uint32_t slot = staged + static_cast<uint32_t>(__popcll(det & below));
                 store_hit(stage + 3u * slot, ph, sensor);
-                stage_src[slot] = my_item;
+                stage_src[slot] = my_launch;
                 atomicAdd(&s_hist[hist_slot(ph.time, p.hist_lo_ns, p.hist_width_ns)], 1u);
             }
             staged += static_cast<uint32_t>(__popcll(det));
@@ -465,12 +463,12 @@ __device__ __forceinline__ bool add_reflection(SurfSlot& a, SurfSlot& b, uint32_
     return true;
 }
 
-// Per-lane item lookup (the forward's wave-uniform search, for arbitrary c).
-__device__ __forceinline__ uint32_t item_source(const KernelParams& p, uint32_t c, uint32_t n_sources) {
+// Source of launch index g: last s with source_photon_offset[s] <= g.
+__device__ __forceinline__ uint32_t launch_source(const KernelParams& p, uint32_t g, uint32_t n_sources) {
     uint32_t lo = 0, hi = n_sources;
     while (hi - lo > 1u) {
         const uint32_t mid = (lo + hi) >> 1;
-        if (p.item_offsets[mid] <= c) {
+        if (p.source_photon_offset[mid] <= g) {
             lo = mid;
         } else {
             hi = mid;
@@ -528,7 +526,7 @@ __global__ void __launch_bounds__(256) adjoint_kernel(AdjointParams a) {
         if (kSource == kSourceSynthetic) {
             generate_synthetic(ph, p.synth, p.rng, hit.event_id, hit.photon_id);
         } else {
-            const uint32_t src = item_source(p, a.hit_item[h], n_sources);
+            const uint32_t src = launch_source(p, a.hit_launch[h], n_sources);
             const Emission& em = p.emissions[src];
             generate_emission(ph, em, sc, p.rng, hit.photon_id - em.first_photon_id);
         }
@@ -632,6 +630,113 @@ int launch_adjoint(const AdjointParams& p, int source, int block, int grid, hipS
     } else {
         hipLaunchKernelGGL((adjoint_kernel<kSourceEmission>), dim3(grid), dim3(block), lds, stream, p);
     }
+    return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
+// ---- launch-order output ----------------------------------------------------
+// (1) mark: one bit per detected launch index; (2) per block of
+// kOrderWordsPerBlock mask words, the exclusive popcount prefix of each word
+// and the block total; (3) exclusive prefix over the block totals (one block);
+// (4) scatter: the rank of a hit among the detected launch indices is
+// block_prefix + word_prefix + popcount of the lower bits of its word.
+__global__ void order_mark_kernel(OrderParams p) {
+    const uint32_t n = min(*p.hit_count, p.capacity);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t g = p.launch[i];
+        if (g < p.n_photons) {
+            atomicOr(p.mask + (g >> 5), 1u << (g & 31u));
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256) order_word_prefix_kernel(OrderParams p, uint32_t n_words) {
+    __shared__ uint32_t s_sum[256];
+    const uint32_t base = blockIdx.x * kOrderWordsPerBlock + threadIdx.x * 4u;  // 4 words per thread
+    uint32_t c[4];
+    uint32_t total = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint32_t w = base + k;
+        c[k] = w < n_words ? static_cast<uint32_t>(__popc(p.mask[w])) : 0u;
+        total += c[k];
+    }
+    s_sum[threadIdx.x] = total;
+    __syncthreads();
+    // inclusive Hillis-Steele scan over the 256 thread totals
+    for (uint32_t off = 1; off < 256u; off <<= 1) {
+        const uint32_t v = threadIdx.x >= off ? s_sum[threadIdx.x - off] : 0u;
+        __syncthreads();
+        s_sum[threadIdx.x] += v;
+        __syncthreads();
+    }
+    uint32_t run = s_sum[threadIdx.x] - total;  // exclusive prefix of this thread's words
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint32_t w = base + k;
+        if (w < n_words) {
+            p.word_prefix[w] = run;
+        }
+        run += c[k];
+    }
+    if (threadIdx.x == 255u) {
+        p.block_prefix[blockIdx.x] = s_sum[255];  // block total, turned into a prefix next
+    }
+}
+
+__global__ void __launch_bounds__(1024) order_block_prefix_kernel(OrderParams p, uint32_t n_blocks) {
+    __shared__ uint32_t s_sum[1024];
+    uint32_t carry = 0;
+    for (uint32_t start = 0; start < n_blocks; start += 1024u) {
+        const uint32_t i = start + threadIdx.x;
+        const uint32_t v = i < n_blocks ? p.block_prefix[i] : 0u;
+        s_sum[threadIdx.x] = v;
+        __syncthreads();
+        for (uint32_t off = 1; off < 1024u; off <<= 1) {
+            const uint32_t u = threadIdx.x >= off ? s_sum[threadIdx.x - off] : 0u;
+            __syncthreads();
+            s_sum[threadIdx.x] += u;
+            __syncthreads();
+        }
+        if (i < n_blocks) {
+            p.block_prefix[i] = carry + s_sum[threadIdx.x] - v;
+        }
+        const uint32_t chunk_total = s_sum[1023];
+        __syncthreads();
+        carry += chunk_total;
+    }
+}
+
+__global__ void order_scatter_kernel(OrderParams p) {
+    const uint32_t n = min(*p.hit_count, p.capacity);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t g = p.launch[i];
+        if (g >= p.n_photons) {
+            continue;
+        }
+        const uint32_t w = g >> 5;
+        const uint32_t below = p.mask[w] & ((1u << (g & 31u)) - 1u);
+        const uint32_t rank = p.block_prefix[w / kOrderWordsPerBlock] + p.word_prefix[w] +
+                              static_cast<uint32_t>(__popc(below));
+        if (rank < n) {
+            p.out_hits[rank] = p.hits[i];
+            p.out_launch[rank] = g;
+        }
+    }
+}
+
+int launch_order_hits(const OrderParams& p, hipStream_t stream) {
+    const uint32_t n_words = (p.n_photons + 31u) / 32u;
+    const uint32_t n_blocks = (n_words + kOrderWordsPerBlock - 1) / kOrderWordsPerBlock;
+    if (n_words == 0) {
+        return 0;
+    }
+    const int grid = static_cast<int>(std::min<uint64_t>((static_cast<uint64_t>(p.capacity) + 255) / 256, 4096));
+    hipLaunchKernelGGL(zero_kernel, dim3(std::max<uint32_t>(1, std::min<uint32_t>((n_words + 255) / 256, 4096))),
+                       dim3(256), 0, stream, p.mask, static_cast<size_t>(n_words));
+    hipLaunchKernelGGL(order_mark_kernel, dim3(std::max(grid, 1)), dim3(256), 0, stream, p);
+    hipLaunchKernelGGL(order_word_prefix_kernel, dim3(n_blocks), dim3(256), 0, stream, p, n_words);
+    hipLaunchKernelGGL(order_block_prefix_kernel, dim3(1), dim3(1024), 0, stream, p, n_blocks);
+    hipLaunchKernelGGL(order_scatter_kernel, dim3(std::max(grid, 1)), dim3(256), 0, stream, p);
     return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 

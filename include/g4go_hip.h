@@ -52,7 +52,9 @@ typedef struct g4go_config {
     uint32_t max_events_per_batch;
     uint32_t enable_performance_diagnostics;
     int32_t device;        /* HIP device ordinal, -1 = current */
-    uint32_t sort_hits;    /* 1: detections returned in (eventID, photonID) order */
+    uint32_t sort_hits;    /* 1: detections in launch order -- the order of their photons in the batch
+                              (records in order, photons in order), as the reference's stable
+                              compaction returns them (OptiXHitCompaction.cu:8-30); 0: unordered */
 } g4go_config;
 
 /* Launch form.

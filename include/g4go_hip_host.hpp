@@ -469,7 +469,7 @@ public:
         c.max_events_per_batch = configuration.maxEventsPerBatch;
         c.enable_performance_diagnostics = configuration.enablePerformanceDiagnostics ? 1u : 0u;
         c.device = -1;
-        c.sort_hits = 1;  // detections in (eventID, photonID) order
+        c.sort_hits = 1;  // detections in launch order, as the reference compacts them
         check(g4go_create(&c, &handle_), nullptr);
         if (launch != nullptr) {
             const g4go_status s = g4go_set_launch(handle_, launch);

@@ -174,7 +174,7 @@ def find_surface(scene, from_id: int, to_id: int) -> int:
 # ---- whole-path runs ----
 class OracleResult:
     def __init__(self, hits: np.ndarray, stats: np.ndarray, hist: np.ndarray) -> None:
-        self.hits = hits        # abi.HIT_DTYPE, (event, photon) order
+        self.hits = hits        # abi.HIT_DTYPE, in launch order (the order of the batch's photons)
         self.stats = stats      # (n_events, 9): detected absorbed escaped truncated max_bounce invalid zero steps generated
         self.hist = hist        # 258 slots
 
@@ -211,7 +211,10 @@ def run_synthetic(scene, source: abi.SyntheticSource, seed: int, event_ids, phot
         with ThreadPoolExecutor(threads) as pool:
             parts = list(pool.map(lambda i: work(int(edges[i]), int(edges[i + 1])), range(threads)))
     hits = np.concatenate([p[0] for p in parts])
-    hits = hits[np.lexsort((hits["photon_id"], hits["event_id"]))]
+    # launch order: events in batch order, photons ascending
+    position = {int(e): i for i, e in enumerate(ev)}
+    pos = np.array([position[int(e)] for e in hits["event_id"]], np.int64) if hits.size else np.zeros(0, np.int64)
+    hits = hits[np.lexsort((hits["photon_id"], pos))]
     stats = np.zeros((ev.size, 9), np.uint64)
     for p in parts:
         stats[:, [0, 1, 2, 3, 5, 6, 7, 8]] += p[1][:, [0, 1, 2, 3, 5, 6, 7, 8]]
@@ -250,8 +253,7 @@ def run_emissions(scene, emissions: np.ndarray, event_ids, emission_event_indice
         edges = [0, *[int(c) for c in cuts], em.size]
         with ThreadPoolExecutor(threads) as pool:
             parts = list(pool.map(lambda i: work(edges[i], edges[i + 1]), range(threads)))
-    hits = np.concatenate([p[0] for p in parts])
-    hits = hits[np.lexsort((hits["photon_id"], hits["event_id"]))]
+    hits = np.concatenate([p[0] for p in parts])  # launch order: records in order, photons ascending
     stats = np.zeros((n_events, 9), np.uint64)
     for p in parts:
         stats[:, [0, 1, 2, 3, 5, 6, 7, 8]] += p[1][:, [0, 1, 2, 3, 5, 6, 7, 8]]
