@@ -34,6 +34,7 @@ class Config(C.Structure):
         ("enable_performance_diagnostics", C.c_uint32),
         ("device", C.c_int32),
         ("sort_hits", C.c_uint32),
+        ("max_in_flight", C.c_uint32),
     ]
 
 
@@ -246,7 +247,7 @@ EMISSION_DTYPE = np.dtype([
 ])
 assert EMISSION_DTYPE.itemsize == C.sizeof(Emission) == 128
 
-assert C.sizeof(Config) == 40 and C.sizeof(Launch) == 28
+assert C.sizeof(Config) == 48 and C.sizeof(Launch) == 28
 
 EXPORTED_SYMBOLS = (
     "g4go_create", "g4go_destroy", "g4go_last_error", "g4go_set_launch", "g4go_get_launch",

@@ -16,6 +16,7 @@
 #include <string>
 #include <unordered_map>
 #include <unordered_set>
+#include <memory>
 #include <vector>
 
 #include "../../include/g4go_hip.h"
@@ -535,7 +536,8 @@ size_t align256(size_t x) { return (x + 255) & ~static_cast<size_t>(255); }
 
 }  // namespace
 
-struct g4go_transport {
+// One batch slot: its own stream, events, buffers and a copy of the scene.
+struct Core {
     g4go_config cfg{};
     g4go_launch launch{};
     std::string error;
@@ -954,7 +956,7 @@ struct g4go_transport {
 namespace {
 
 template <class F>
-g4go_status guarded(g4go_transport* h, F&& f) {
+g4go_status guarded(Core* h, F&& f) {
     try {
         f();
         return G4GO_OK;
@@ -1000,43 +1002,7 @@ extern "C" {
 
 const char* g4go_version(void) { return "g4go-hip 0.1 (gfx950)"; }
 
-g4go_status g4go_create(const g4go_config* config, g4go_transport** out) {
-    if (out == nullptr || config == nullptr) {
-        g_create_error = "null argument";
-        return G4GO_ERROR_INVALID_ARGUMENT;
-    }
-    *out = nullptr;
-    if (!(config->boundary_tolerance_mm > 0.0f) || !std::isfinite(config->boundary_tolerance_mm)) {
-        g_create_error = "boundary tolerance must be positive";
-        return G4GO_ERROR_INVALID_ARGUMENT;
-    }
-    if (config->max_bounces_per_photon == 0) {
-        g_create_error = "max bounces must be positive";
-        return G4GO_ERROR_INVALID_ARGUMENT;
-    }
-    try {
-        auto* h = new g4go_transport();
-        h->cfg = *config;
-        h->launch = g4go_launch{0, 256, 64, 64, 8, 0, 0};
-        *out = h;
-        return G4GO_OK;
-    } catch (...) {
-        g_create_error = "host allocation failed";
-        return G4GO_ERROR_RUNTIME;
-    }
-}
-
-void g4go_destroy(g4go_transport* h) {
-    if (h == nullptr) {
-        return;
-    }
-    h->release();
-    delete h;
-}
-
-const char* g4go_last_error(const g4go_transport* h) { return h == nullptr ? g_create_error.c_str() : h->error.c_str(); }
-
-g4go_status g4go_set_launch(g4go_transport* h, const g4go_launch* launch) {
+static g4go_status core_set_launch(Core* h, const g4go_launch* launch) {
     if (h == nullptr || launch == nullptr) {
         return G4GO_ERROR_INVALID_ARGUMENT;
     }
@@ -1049,7 +1015,7 @@ g4go_status g4go_set_launch(g4go_transport* h, const g4go_launch* launch) {
     });
 }
 
-g4go_status g4go_get_launch(const g4go_transport* h, g4go_launch* launch) {
+static g4go_status core_get_launch(const Core* h, g4go_launch* launch) {
     if (h == nullptr || launch == nullptr) {
         return G4GO_ERROR_INVALID_ARGUMENT;
     }
@@ -1057,7 +1023,7 @@ g4go_status g4go_get_launch(const g4go_transport* h, g4go_launch* launch) {
     return G4GO_OK;
 }
 
-g4go_status g4go_prepare_scene(g4go_transport* h, const g4go_scene* scene) {
+static g4go_status core_prepare_scene(Core* h, const g4go_scene* scene) {
     if (h == nullptr || scene == nullptr) {
         return G4GO_ERROR_INVALID_ARGUMENT;
     }
@@ -1076,7 +1042,7 @@ g4go_status g4go_prepare_scene(g4go_transport* h, const g4go_scene* scene) {
     });
 }
 
-g4go_status g4go_enqueue_emissions(g4go_transport* h, const g4go_emission* emissions, uint64_t n_emissions,
+static g4go_status core_enqueue_emissions(Core* h, const g4go_emission* emissions, uint64_t n_emissions,
                                    const uint32_t* event_ids, uint64_t n_events,
                                    const uint32_t* emission_event_indices) {
     if (h == nullptr) {
@@ -1150,7 +1116,7 @@ g4go_status g4go_enqueue_emissions(g4go_transport* h, const g4go_emission* emiss
     });
 }
 
-g4go_status g4go_enqueue_synthetic(g4go_transport* h, const g4go_synthetic_source* source, const uint32_t* event_ids,
+static g4go_status core_enqueue_synthetic(Core* h, const g4go_synthetic_source* source, const uint32_t* event_ids,
                                    const uint32_t* photon_counts, uint64_t n_events) {
     if (h == nullptr) {
         return G4GO_ERROR_INVALID_ARGUMENT;
@@ -1215,7 +1181,7 @@ g4go_status g4go_enqueue_synthetic(g4go_transport* h, const g4go_synthetic_sourc
     });
 }
 
-static g4go_status complete_batch(g4go_transport* h, g4go_batch_result* out, bool copy_hits,
+static g4go_status complete_batch(Core* h, g4go_batch_result* out, bool copy_hits,
                                   const void** hits_device) {
     if (h == nullptr || out == nullptr) {
         return G4GO_ERROR_INVALID_ARGUMENT;
@@ -1242,10 +1208,10 @@ static g4go_status complete_batch(g4go_transport* h, g4go_batch_result* out, boo
         if (h->n_photons != 0) {
             hip_check(hipEventSynchronize(h->ev_meta), "hipEventSynchronize");
             const char* meta = h->h_meta.as<char>();
-            const auto* counters = reinterpret_cast<const uint64_t*>(meta + g4go_transport::kMetaCounters);
-            const uint32_t hit_count = *reinterpret_cast<const uint32_t*>(meta + g4go_transport::kMetaHitCount);
-            const auto* hist = reinterpret_cast<const uint64_t*>(meta + g4go_transport::kMetaHist);
-            const auto* es = reinterpret_cast<const EventStats*>(meta + g4go_transport::kMetaBytes);
+            const auto* counters = reinterpret_cast<const uint64_t*>(meta + Core::kMetaCounters);
+            const uint32_t hit_count = *reinterpret_cast<const uint32_t*>(meta + Core::kMetaHitCount);
+            const auto* hist = reinterpret_cast<const uint64_t*>(meta + Core::kMetaHist);
+            const auto* es = reinterpret_cast<const EventStats*>(meta + Core::kMetaBytes);
             const uint64_t n_hits = std::min<uint64_t>(hit_count, h->n_photons);
             float kernel_ms = 0.0f, h2d_ms = 0.0f;
             hip_check(hipEventElapsedTime(&kernel_ms, h->ev_kernel0, h->ev_kernel1), "hipEventElapsedTime");
@@ -1290,7 +1256,7 @@ static g4go_status complete_batch(g4go_transport* h, g4go_batch_result* out, boo
             bs.host_to_device_ms = h2d_ms;
             bs.device_to_host_ms = std::chrono::duration<double, std::milli>(d2h1 - d2h0).count();
             bs.host_to_device_bytes = h->h2d_bytes;
-            bs.device_to_host_bytes = g4go_transport::kMetaBytes + n_events * sizeof(EventStats) + n_hits * sizeof(Hit);
+            bs.device_to_host_bytes = Core::kMetaBytes + n_events * sizeof(EventStats) + n_hits * sizeof(Hit);
         }
         bs.transport_time_ms =
             std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - h->started).count();
@@ -1303,13 +1269,13 @@ static g4go_status complete_batch(g4go_transport* h, g4go_batch_result* out, boo
     });
 }
 
-g4go_status g4go_complete(g4go_transport* h, g4go_batch_result* out) { return complete_batch(h, out, true, nullptr); }
+static g4go_status core_complete(Core* h, g4go_batch_result* out) { return complete_batch(h, out, true, nullptr); }
 
-g4go_status g4go_complete_device(g4go_transport* h, g4go_batch_result* out, const void** hits_device) {
+static g4go_status core_complete_device(Core* h, g4go_batch_result* out, const void** hits_device) {
     return complete_batch(h, out, false, hits_device);
 }
 
-g4go_status g4go_device_info(g4go_transport* h, char* name, uint64_t name_len, int32_t* cu_count) {
+static g4go_status core_device_info(Core* h, char* name, uint64_t name_len, int32_t* cu_count) {
     if (h == nullptr || name == nullptr || name_len == 0 || cu_count == nullptr) {
         return G4GO_ERROR_INVALID_ARGUMENT;
     }
@@ -1341,7 +1307,7 @@ Prop* bound_prop(HostScene& s, uint32_t kind, uint32_t index) {
 
 }  // namespace
 
-g4go_status g4go_set_parameter_layout(g4go_transport* h, const g4go_param_table* tables, uint32_t n_tables,
+static g4go_status core_set_parameter_layout(Core* h, const g4go_param_table* tables, uint32_t n_tables,
                                       uint32_t* n_params) {
     if (h == nullptr || (n_tables != 0 && tables == nullptr) || n_params == nullptr) {
         return G4GO_ERROR_INVALID_ARGUMENT;
@@ -1356,7 +1322,7 @@ g4go_status g4go_set_parameter_layout(g4go_transport* h, const g4go_param_table*
         const uint32_t n_mat = static_cast<uint32_t>(h->scene.materials.size());
         const uint32_t n_surf = static_cast<uint32_t>(h->scene.surfaces.size());
         std::vector<uint32_t> map(2 * n_mat + 2 * n_surf, kInvalid);
-        std::vector<g4go_transport::ParamTable> out;
+        std::vector<Core::ParamTable> out;
         uint32_t offset = 0;
         for (uint32_t t = 0; t < n_tables; ++t) {
             const Prop* prop = bound_prop(h->scene, tables[t].kind, tables[t].index);
@@ -1382,7 +1348,7 @@ g4go_status g4go_set_parameter_layout(g4go_transport* h, const g4go_param_table*
     });
 }
 
-g4go_status g4go_get_parameters(g4go_transport* h, double* theta, uint64_t n) {
+static g4go_status core_get_parameters(Core* h, double* theta, uint64_t n) {
     if (h == nullptr || (n != 0 && theta == nullptr)) {
         return G4GO_ERROR_INVALID_ARGUMENT;
     }
@@ -1400,7 +1366,7 @@ g4go_status g4go_get_parameters(g4go_transport* h, double* theta, uint64_t n) {
     });
 }
 
-g4go_status g4go_set_parameters(g4go_transport* h, const double* theta, uint64_t n) {
+static g4go_status core_set_parameters(Core* h, const double* theta, uint64_t n) {
     if (h == nullptr || (n != 0 && theta == nullptr)) {
         return G4GO_ERROR_INVALID_ARGUMENT;
     }
@@ -1432,7 +1398,7 @@ g4go_status g4go_set_parameters(g4go_transport* h, const double* theta, uint64_t
     });
 }
 
-g4go_status g4go_adjoint(g4go_transport* h, const double* slot_weight, double* grad, uint64_t n,
+static g4go_status core_adjoint(Core* h, const double* slot_weight, double* grad, uint64_t n,
                          g4go_adjoint_stats* stats) {
     if (h == nullptr || slot_weight == nullptr || (n != 0 && grad == nullptr)) {
         return G4GO_ERROR_INVALID_ARGUMENT;
@@ -1494,7 +1460,7 @@ g4go_status g4go_adjoint(g4go_transport* h, const double* slot_weight, double* g
     });
 }
 
-g4go_status g4go_trace_synthetic(g4go_transport* h, const g4go_synthetic_source* source, uint32_t event_id,
+static g4go_status core_trace_synthetic(Core* h, const g4go_synthetic_source* source, uint32_t event_id,
                                  uint32_t photon_id, uint32_t max_steps, float* out, uint32_t* n_steps) {
     if (h == nullptr || source == nullptr || out == nullptr || n_steps == nullptr || max_steps == 0 ||
         max_steps > 65536) {
@@ -1534,7 +1500,7 @@ g4go_status g4go_trace_synthetic(g4go_transport* h, const g4go_synthetic_source*
     });
 }
 
-g4go_status g4go_eval_math(g4go_transport* h, int which, const float* in, float* out, uint64_t n) {
+static g4go_status core_eval_math(Core* h, int which, const float* in, float* out, uint64_t n) {
     if (h == nullptr || in == nullptr || out == nullptr || which < 0 || which > 2 || n > (1u << 28)) {
         return G4GO_ERROR_INVALID_ARGUMENT;
     }
@@ -1555,7 +1521,7 @@ g4go_status g4go_eval_math(g4go_transport* h, int which, const float* in, float*
     });
 }
 
-g4go_status g4go_eval_fresnel(g4go_transport* h, const float* in, float* out, uint64_t n) {
+static g4go_status core_eval_fresnel(Core* h, const float* in, float* out, uint64_t n) {
     if (h == nullptr || in == nullptr || out == nullptr || n > (1u << 24)) {
         return G4GO_ERROR_INVALID_ARGUMENT;
     }
@@ -1573,6 +1539,266 @@ g4go_status g4go_eval_fresnel(g4go_transport* h, const float* in, float* out, ui
         din.release();
         dout.release();
     });
+}
+
+// ---------------------------------------------------------------------------
+// The handle: a ring of batch slots.  With max_in_flight = 1 (the reference's
+// single slot, OptiXTransportHost.cpp:523-525) a second enqueue before the
+// completion fails with "transport queue is full"; with up to kMaxSlots the
+// next batches are enqueued while earlier ones run, so host staging and
+// readback overlap the device work (each slot has its own stream and buffers).
+// Batches complete in enqueue order.
+// ---------------------------------------------------------------------------
+}  // extern "C"
+
+constexpr uint32_t kMaxSlots = 4;
+
+struct g4go_transport {
+    g4go_config cfg{};
+    std::string error;
+    std::vector<std::unique_ptr<Core>> cores;
+    uint32_t head = 0;   // slot of the oldest batch in flight
+    uint32_t count = 0;  // batches in flight
+    int last = -1;       // slot of the last completed batch
+    Core* slot(uint32_t k) { return cores[k % cores.size()].get(); }
+    g4go_status pass(Core* c, g4go_status s) {
+        if (s != G4GO_OK) {
+            error = c->error;
+        }
+        return s;
+    }
+    g4go_status logic(const char* message) {
+        error = message;
+        return G4GO_ERROR_LOGIC;
+    }
+    template <class F>
+    g4go_status each(F&& f) {
+        for (auto& c : cores) {
+            const g4go_status s = f(c.get());
+            if (s != G4GO_OK) {
+                error = c->error;
+                return s;
+            }
+        }
+        return G4GO_OK;
+    }
+};
+
+extern "C" {
+
+g4go_status g4go_create(const g4go_config* config, g4go_transport** out) {
+    if (out == nullptr || config == nullptr) {
+        g_create_error = "null argument";
+        return G4GO_ERROR_INVALID_ARGUMENT;
+    }
+    *out = nullptr;
+    if (!(config->boundary_tolerance_mm > 0.0f) || !std::isfinite(config->boundary_tolerance_mm)) {
+        g_create_error = "boundary tolerance must be positive";
+        return G4GO_ERROR_INVALID_ARGUMENT;
+    }
+    if (config->max_bounces_per_photon == 0) {
+        g_create_error = "max bounces must be positive";
+        return G4GO_ERROR_INVALID_ARGUMENT;
+    }
+    if (config->max_in_flight > kMaxSlots) {
+        g_create_error = "at most 4 batches can be in flight";
+        return G4GO_ERROR_INVALID_ARGUMENT;
+    }
+    try {
+        auto t = std::make_unique<g4go_transport>();
+        t->cfg = *config;
+        const uint32_t n = std::max<uint32_t>(1, config->max_in_flight);
+        for (uint32_t k = 0; k < n; ++k) {
+            auto c = std::make_unique<Core>();
+            c->cfg = *config;
+            c->launch = g4go_launch{0, 256, 64, 64, 8, 0, 0};
+            t->cores.push_back(std::move(c));
+        }
+        *out = t.release();
+        return G4GO_OK;
+    } catch (...) {
+        g_create_error = "host allocation failed";
+        return G4GO_ERROR_RUNTIME;
+    }
+}
+
+void g4go_destroy(g4go_transport* t) {
+    if (t == nullptr) {
+        return;
+    }
+    for (auto& c : t->cores) {
+        c->release();
+    }
+    delete t;
+}
+
+const char* g4go_last_error(const g4go_transport* t) { return t == nullptr ? g_create_error.c_str() : t->error.c_str(); }
+
+g4go_status g4go_set_launch(g4go_transport* t, const g4go_launch* launch) {
+    if (t == nullptr || launch == nullptr) {
+        return G4GO_ERROR_INVALID_ARGUMENT;
+    }
+    if (t->count != 0) {
+        return t->logic("launch form cannot change with a pending batch");
+    }
+    return t->each([&](Core* c) { return core_set_launch(c, launch); });
+}
+
+g4go_status g4go_get_launch(const g4go_transport* t, g4go_launch* launch) {
+    if (t == nullptr) {
+        return G4GO_ERROR_INVALID_ARGUMENT;
+    }
+    return core_get_launch(t->cores[0].get(), launch);
+}
+
+g4go_status g4go_prepare_scene(g4go_transport* t, const g4go_scene* scene) {
+    if (t == nullptr || scene == nullptr) {
+        return G4GO_ERROR_INVALID_ARGUMENT;
+    }
+    if (t->count != 0) {
+        return t->logic("scene cannot be rebuilt with pending batches");
+    }
+    t->last = -1;
+    return t->each([&](Core* c) { return core_prepare_scene(c, scene); });
+}
+
+g4go_status g4go_enqueue_emissions(g4go_transport* t, const g4go_emission* emissions, uint64_t n_emissions,
+                                   const uint32_t* event_ids, uint64_t n_events,
+                                   const uint32_t* emission_event_indices) {
+    if (t == nullptr) {
+        return G4GO_ERROR_INVALID_ARGUMENT;
+    }
+    if (t->count == t->cores.size()) {
+        return t->logic("transport queue is full");
+    }
+    Core* c = t->slot(t->head + t->count);
+    const g4go_status s = t->pass(c, core_enqueue_emissions(c, emissions, n_emissions, event_ids, n_events,
+                                                            emission_event_indices));
+    if (s == G4GO_OK) {
+        ++t->count;
+    }
+    return s;
+}
+
+g4go_status g4go_enqueue_synthetic(g4go_transport* t, const g4go_synthetic_source* source, const uint32_t* event_ids,
+                                   const uint32_t* photon_counts, uint64_t n_events) {
+    if (t == nullptr) {
+        return G4GO_ERROR_INVALID_ARGUMENT;
+    }
+    if (t->count == t->cores.size()) {
+        return t->logic("transport queue is full");
+    }
+    Core* c = t->slot(t->head + t->count);
+    const g4go_status s = t->pass(c, core_enqueue_synthetic(c, source, event_ids, photon_counts, n_events));
+    if (s == G4GO_OK) {
+        ++t->count;
+    }
+    return s;
+}
+
+static g4go_status complete_oldest(g4go_transport* t, g4go_batch_result* out, const void** hits_device) {
+    if (t == nullptr || out == nullptr) {
+        return G4GO_ERROR_INVALID_ARGUMENT;
+    }
+    if (t->count == 0) {
+        return t->logic("transport queue is empty");
+    }
+    const uint32_t k = t->head % static_cast<uint32_t>(t->cores.size());
+    Core* c = t->slot(k);
+    const g4go_status s = t->pass(c, hits_device != nullptr ? core_complete_device(c, out, hits_device)
+                                                            : core_complete(c, out));
+    if (!c->busy) {  // the slot is consumed once its completion started
+        t->head = (k + 1) % static_cast<uint32_t>(t->cores.size());
+        --t->count;
+        t->last = static_cast<int>(k);
+    }
+    return s;
+}
+
+g4go_status g4go_complete(g4go_transport* t, g4go_batch_result* out) { return complete_oldest(t, out, nullptr); }
+
+g4go_status g4go_complete_device(g4go_transport* t, g4go_batch_result* out, const void** hits_device) {
+    if (hits_device == nullptr) {
+        const void* ignored = nullptr;
+        return complete_oldest(t, out, &ignored);
+    }
+    return complete_oldest(t, out, hits_device);
+}
+
+g4go_status g4go_device_info(g4go_transport* t, char* name, uint64_t name_len, int32_t* cu_count) {
+    if (t == nullptr) {
+        return G4GO_ERROR_INVALID_ARGUMENT;
+    }
+    Core* c = t->cores[0].get();
+    return t->pass(c, core_device_info(c, name, name_len, cu_count));
+}
+
+g4go_status g4go_set_parameter_layout(g4go_transport* t, const g4go_param_table* tables, uint32_t n_tables,
+                                      uint32_t* n_params) {
+    if (t == nullptr) {
+        return G4GO_ERROR_INVALID_ARGUMENT;
+    }
+    if (t->count != 0) {
+        return t->logic("the parameter layout cannot change with a pending batch");
+    }
+    t->last = -1;
+    return t->each([&](Core* c) { return core_set_parameter_layout(c, tables, n_tables, n_params); });
+}
+
+g4go_status g4go_get_parameters(g4go_transport* t, double* theta, uint64_t n) {
+    if (t == nullptr) {
+        return G4GO_ERROR_INVALID_ARGUMENT;
+    }
+    Core* c = t->cores[0].get();
+    return t->pass(c, core_get_parameters(c, theta, n));
+}
+
+g4go_status g4go_set_parameters(g4go_transport* t, const double* theta, uint64_t n) {
+    if (t == nullptr) {
+        return G4GO_ERROR_INVALID_ARGUMENT;
+    }
+    if (t->count != 0) {
+        return t->logic("parameters cannot change with a pending batch");
+    }
+    t->last = -1;
+    return t->each([&](Core* c) { return core_set_parameters(c, theta, n); });
+}
+
+g4go_status g4go_adjoint(g4go_transport* t, const double* slot_weight, double* grad, uint64_t n,
+                         g4go_adjoint_stats* stats) {
+    if (t == nullptr) {
+        return G4GO_ERROR_INVALID_ARGUMENT;
+    }
+    if (t->last < 0) {
+        return t->logic("the adjoint replays the last completed batch: complete it first");
+    }
+    Core* c = t->slot(static_cast<uint32_t>(t->last));
+    return t->pass(c, core_adjoint(c, slot_weight, grad, n, stats));
+}
+
+g4go_status g4go_trace_synthetic(g4go_transport* t, const g4go_synthetic_source* source, uint32_t event_id,
+                                 uint32_t photon_id, uint32_t max_steps, float* out, uint32_t* n_steps) {
+    if (t == nullptr) {
+        return G4GO_ERROR_INVALID_ARGUMENT;
+    }
+    Core* c = t->cores[0].get();
+    return t->pass(c, core_trace_synthetic(c, source, event_id, photon_id, max_steps, out, n_steps));
+}
+
+g4go_status g4go_eval_math(g4go_transport* t, int which, const float* in, float* out, uint64_t n) {
+    if (t == nullptr) {
+        return G4GO_ERROR_INVALID_ARGUMENT;
+    }
+    Core* c = t->cores[0].get();
+    return t->pass(c, core_eval_math(c, which, in, out, n));
+}
+
+g4go_status g4go_eval_fresnel(g4go_transport* t, const float* in, float* out, uint64_t n) {
+    if (t == nullptr) {
+        return G4GO_ERROR_INVALID_ARGUMENT;
+    }
+    Core* c = t->cores[0].get();
+    return t->pass(c, core_eval_fresnel(c, in, out, n));
 }
 
 }  // extern "C"

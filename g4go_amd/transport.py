@@ -44,12 +44,14 @@ class PhotonTransportConfig:
     max_events_per_batch: int = 10_000
     enable_performance_diagnostics: bool = False
     device: int = -1
-    sort_hits: bool = True
+    sort_hits: bool = True          # detections in launch order
+    max_in_flight: int = 1          # batches enqueued before the oldest completes (1..4)
 
     def to_abi(self) -> abi.Config:
         return abi.Config(self.seed, self.max_photons_per_event, self.max_bounces_per_photon,
                           self.boundary_tolerance_mm, self.target_photons_per_batch, self.max_events_per_batch,
-                          int(self.enable_performance_diagnostics), self.device, int(self.sort_hits))
+                          int(self.enable_performance_diagnostics), self.device, int(self.sort_hits),
+                          int(self.max_in_flight))
 
 
 @dataclass

@@ -55,6 +55,9 @@ typedef struct g4go_config {
     uint32_t sort_hits;    /* 1: detections in launch order -- the order of their photons in the batch
                               (records in order, photons in order), as the reference's stable
                               compaction returns them (OptiXHitCompaction.cu:8-30); 0: unordered */
+    uint32_t max_in_flight; /* batches enqueued before the oldest completes: 0 or 1 = the reference's
+                              single slot (OptiXTransportHost.cpp:523-525), up to 4: a ring of slots,
+                              each with its own stream and buffers, completed in enqueue order */
 } g4go_config;
 
 /* Launch form.

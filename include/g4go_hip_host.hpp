@@ -34,6 +34,7 @@
 #include <cstddef>
 #include <cstdint>
 #include <cstring>
+#include <deque>
 #include <limits>
 #include <span>
 #include <stdexcept>
@@ -458,7 +459,10 @@ public:
                   "OpticalEmission field offsets differ from g4go_emission");
 
     // OptiXTransportHost(PhotonTransportConfig) (OptiXTransportHost.hpp:12).
-    explicit BasicHipTransportHost(Config configuration, const g4go_launch* launch = nullptr)
+    // in_flight > 1 lets that many batches be enqueued before the oldest is
+    // completed (the reference keeps one, OptiXTransportHost.cpp:523-525).
+    explicit BasicHipTransportHost(Config configuration, const g4go_launch* launch = nullptr,
+                                   std::uint32_t in_flight = 1)
         : tolerance_mm_(configuration.boundaryToleranceMm) {
         g4go_config c{};
         c.seed = configuration.seed;
@@ -470,6 +474,7 @@ public:
         c.enable_performance_diagnostics = configuration.enablePerformanceDiagnostics ? 1u : 0u;
         c.device = -1;
         c.sort_hits = 1;  // detections in launch order, as the reference compacts them
+        c.max_in_flight = in_flight;
         check(g4go_create(&c, &handle_), nullptr);
         if (launch != nullptr) {
             const g4go_status s = g4go_set_launch(handle_, launch);
@@ -509,20 +514,26 @@ public:
               handle_);
         // which events own an emission record, and whether the batch has photons:
         // they decide the valid-field masks of the output (OptiXTransportHost.cpp:587-632)
-        has_emission_.assign(batch.eventIDs.size(), false);
+        Pending pending;
+        pending.has_emission.assign(batch.eventIDs.size(), false);
         for (std::uint32_t i : batch.emissionEventIndices) {
-            has_emission_[i] = true;
+            pending.has_emission[i] = true;
         }
-        batch_photons_ = 0;
         for (const auto& e : batch.emissions) {
-            batch_photons_ += e.photonCount;
+            pending.photons += e.photonCount;
         }
+        pending_.push_back(std::move(pending));
     }
 
     // CompleteOldestBatch (OptiXTransportHost.cpp:779-883).
     Output CompleteOldestBatch() {
         g4go_batch_result r{};
         check(g4go_complete(handle_, &r), handle_);
+        Pending batch;
+        if (!pending_.empty()) {
+            batch = std::move(pending_.front());
+            pending_.pop_front();
+        }
         Output out{};
         const std::uint32_t counts = bit(Field::Detected) | bit(Field::Absorbed) | bit(Field::Escaped) |
                                      bit(Field::Truncated) | bit(Field::MaxBounce) | bit(Field::InvalidState) |
@@ -552,7 +563,7 @@ public:
         st.transportTimeMs = r.stats.transport_time_ms;
         // an empty batch reports no batch fields; a batch with photons reports
         // captured, the terminal counts and the transport time
-        st.validFields = batch_photons_ != 0 ? bit(Field::Captured) | counts | bit(Field::TransportTime) : 0u;
+        st.validFields = batch.photons != 0 ? bit(Field::Captured) | counts | bit(Field::TransportTime) : 0u;
         out.eventStatistics.reserve(r.n_events);
         for (std::uint64_t i = 0; i < r.n_events; ++i) {
             const g4go_event_stats& e = r.events[i];
@@ -566,9 +577,9 @@ public:
             es.statistics.maxBounceCount = e.max_bounce;
             es.statistics.invalidStateCount = e.invalid;
             es.statistics.zeroStepCount = e.zero_step;
-            const bool owns = i < has_emission_.size() && has_emission_[i];
+            const bool owns = i < batch.has_emission.size() && batch.has_emission[i];
             // captured when the event owns a record; terminal counts whenever the batch has records
-            es.statistics.validFields = (owns ? bit(Field::Captured) : 0u) | (any_emission() ? counts : 0u);
+            es.statistics.validFields = (owns ? bit(Field::Captured) : 0u) | (batch.any() ? counts : 0u);
             out.eventStatistics.push_back(es);
         }
         auto& perf = out.performance;
@@ -586,20 +597,24 @@ public:
 private:
     static constexpr std::uint32_t bit(Field f) { return std::uint32_t{1} << static_cast<std::uint32_t>(f); }
 
-    bool any_emission() const {
-        for (bool b : has_emission_) {
-            if (b) {
-                return true;
+    // what the output masks need of each batch in flight, in enqueue order
+    struct Pending {
+        std::vector<bool> has_emission;  // per event: owns an emission record
+        std::uint64_t photons = 0;
+        bool any() const {
+            for (bool b : has_emission) {
+                if (b) {
+                    return true;
+                }
             }
+            return false;
         }
-        return false;
-    }
+    };
 
     g4go_transport* handle_ = nullptr;
     const Scene* prepared_ = nullptr;
     float tolerance_mm_;
-    std::vector<bool> has_emission_;
-    std::uint64_t batch_photons_ = 0;
+    std::deque<Pending> pending_;
 };
 
 using HipTransportHost = BasicHipTransportHost<>;
