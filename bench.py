@@ -22,7 +22,9 @@ and the path-replay adjoint through torch.autograd) against a forward pass;
 `array_events` times 1024 uneven electron events in the 8x8 crystal array
 (batched vs per-event launches, 2048-parameter adjoint); `electron_events`
 times 5000 1 MeV electron events in the crystal, with the FP32 oracle at 6
-threads and at all host threads on a sample of them.
+threads and at all host threads on a sample of them; `host_batches` times the
+drop-in path at the reference scheduler's batch size (~1e6 photons, hits read
+back to the host) with one and with two batches in flight.
 """
 from __future__ import annotations
 
@@ -240,6 +242,45 @@ def run_c1(seed, launch, steps, warmup, cpu, cpu_events):
     return res
 
 
+def run_host_batches(seed, launch, n_batches=40, events_per_batch=33):
+    """The drop-in path at the reference scheduler's batch size (~1e6 photons,
+    PhotonTransportConfig::targetPhotonsPerBatch): 1 MeV electron events in the
+    crystal, enqueue + completion with the hits read back to the host in launch
+    order, with one batch in flight (the reference's single slot) and with two."""
+    from g4go_amd import workloads
+    from g4go_amd.transport import HipTransportHost, LaunchForm, PhotonTransportConfig
+
+    cs = workloads.crystal_scene()
+    em, ev, idx = workloads.electron_emissions(cs, n_batches * events_per_batch, seed=seed)
+    batches = []
+    for b in range(n_batches):
+        sel = (idx >= events_per_batch * b) & (idx < events_per_batch * (b + 1))
+        batches.append((em[sel], ev[events_per_batch * b:events_per_batch * (b + 1)],
+                        (idx[sel] - events_per_batch * b).astype(np.uint32)))
+    photons = int(em["photon_count"].astype(np.int64).sum())
+    res = {"batches": n_batches, "photons_per_batch": photons / n_batches, "readback": "host, launch order"}
+    for slots in (1, 2):
+        host = HipTransportHost(PhotonTransportConfig(seed=seed, max_in_flight=slots))
+        host.set_launch(LaunchForm(**{**launch.__dict__, "graph": True}))
+        host.prepare_scene(cs.scene)
+        for rep in range(2):  # the first pass warms the graphs and buffers
+            t0 = time.perf_counter()
+            inflight = 0
+            for e, i, x in batches:
+                if inflight == slots:
+                    host.complete_oldest_batch()
+                    inflight -= 1
+                host.enqueue_emissions(cs.scene, e, i, x)
+                inflight += 1
+            while inflight:
+                host.complete_oldest_batch()
+                inflight -= 1
+            dt = time.perf_counter() - t0
+        res[f"slots_{slots}_photons_per_s"] = photons / dt
+        host.close()
+    return res
+
+
 def cpu_baseline(cs, src, seed, sample):
     from oracle import oracle as O
 
@@ -325,6 +366,7 @@ def main():
     adjoint = None if args.no_adjoint else run_adjoint(cs, src, args.seed, persistent, args.steps, args.warmup)
     array = None if args.no_array else run_array(args.seed, persistent, args.steps, args.warmup, not args.no_plain)
     c1 = None if args.no_c1 else run_c1(args.seed, persistent, args.steps, args.warmup, not args.no_cpu, 200)
+    host_batches = None if args.no_c1 else run_host_batches(args.seed, persistent)
 
     st = out.statistics
     P, S, D = st["generated"], st["total_steps"], st["n_hits"]
@@ -359,6 +401,7 @@ def main():
         "adjoint": adjoint,
         "array_events": array,
         "electron_events": c1,
+        "host_batches": host_batches,
         "batched_over_plain": (None if plain is None else (args.photons * args.steps / wall) / plain["photons_per_s"]),
         "counts": {"photons": P, "steps": S, "detected": D, "steps_per_photon": S / max(P, 1),
                    "detected_fraction": D / max(P, 1)},
