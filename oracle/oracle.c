@@ -1631,6 +1631,22 @@ static int or_add_reflection(sslot_t* a, sslot_t* b, uint32_t sf) {
     return 1;
 }
 
+/* Whether a replayed path ends at its recorded detection.  The FP32 replay
+ * repeats the forward arithmetic and must match bit for bit; the FP64 replay of
+ * an FP32 run follows the same decisions with other roundings, so it matches
+ * within 1e-6 relative in time and 1e-3 mm in position. */
+static int same_end(const photon_t* ph, int status, const g4go_hit* hit) {
+    if (status != S_DETECTED) return 0;
+#ifdef ORACLE_DOUBLE
+    return fabs(ph->time - hit->time_ns) <= 1e-6 * (1.0 + fabs(hit->time_ns)) &&
+           fabs(ph->pos.x - (double)hit->position_mm[0]) <= 1e-3 && fabs(ph->pos.y - (double)hit->position_mm[1]) <= 1e-3 &&
+           fabs(ph->pos.z - (double)hit->position_mm[2]) <= 1e-3;
+#else
+    return ph->time == hit->time_ns && (float)ph->pos.x == hit->position_mm[0] && (float)ph->pos.y == hit->position_mm[1] &&
+           (float)ph->pos.z == hit->position_mm[2];
+#endif
+}
+
 /* Regenerates the photon of a recorded hit; returns 0 when the hit names no
  * photon of the batch. */
 typedef int (*regen_fn)(photon_t* ph, const void* ctx, const oscene_t* s, const rng_t* r, const g4go_hit* hit);
@@ -1739,8 +1755,7 @@ static int adjoint_core(oscene_t* s, regen_fn regen, const void* ctx, uint64_t s
             if (status == S_DETECTED) det_surface = tr.surface;
             if (status != S_ALIVE || ph.step >= max_steps) break;
         }
-        if (status != S_DETECTED || ph.time != hit->time_ns || (float)ph.pos.x != hit->position_mm[0] ||
-            (float)ph.pos.y != hit->position_mm[1] || (float)ph.pos.z != hit->position_mm[2]) {
+        if (!same_end(&ph, status, hit)) {
             counters[1]++;
             continue;
         }

@@ -236,3 +236,38 @@ def test_parameters_round_trip_and_restore(crystal):
         again = h.complete_oldest_batch().detections.copy()
     assert a.statistics["detected"] == len(first)
     assert first.tobytes() == again.tobytes()
+
+
+def test_adjoint_matches_fp64_replay(crystal):
+    """Config 4 on 1e5 photons: the device gradient against the FP64 CPU replay of
+    the same hits.  FP64 paths may leave a few FP32 paths (they are counted as
+    mismatches and left out); the rest agree to 1e-3 of the gradient norm."""
+    log("case: adjoint vs FP64 oracle replay, 1e5 photons")
+    src = workloads.synthetic_source(crystal)
+    ev = np.array([0], np.uint32)
+    counts = np.array([100_000], np.uint32)
+    w = np.random.default_rng(12).normal(size=abi.HIST_SLOTS)
+    with open_host(crystal, 41) as h:
+        h.enqueue_synthetic(crystal.scene, src, ev, counts)
+        out = h.complete_oldest_batch()
+        grad, stats = h.adjoint(w)
+    ref, counters = O.adjoint_synthetic(crystal.scene, src, 41, out.detections, w, crystal_layout(crystal), N_PARAMS,
+                                        precision=64, threads=8)
+    log(f"  FP64 replay: {int(counters[0])} replayed, {int(counters[1])} left the FP32 path")
+    assert int(counters[1]) <= 0.001 * len(out.detections)
+    assert int(counters[0]) + int(counters[1]) + int(counters[2]) == len(out.detections)
+    np.testing.assert_allclose(grad, ref, rtol=0, atol=1e-3 * np.linalg.norm(ref))
+
+
+def test_calibration_reduces_loss():
+    """Five Adam steps of the calibration harness (1e6 photons) lower the loss."""
+    import json
+    import subprocess
+
+    log("case: calibration harness, 5 iterations")
+    proc = subprocess.run([sys.executable, "-m", "g4go_amd.calibrate", "--photons", "1000000", "--iterations", "5"],
+                          capture_output=True, text=True, timeout=110)
+    assert proc.returncode == 0, proc.stderr[-2000:]
+    hist = [json.loads(l) for l in proc.stdout.splitlines() if l.startswith("{")]
+    log(f"  losses {[round(x['loss'], 1) for x in hist]}")
+    assert len(hist) == 5 and hist[-1]["loss"] < hist[0]["loss"]
