@@ -49,3 +49,10 @@ def transport_histogram(theta, host, scene, source, event_ids, counts):
 def histogram_loss(n, h, n_target, h_target):
     """(N - N*)^2 + sum_b (h_b - h*_b)^2."""
     return (n - n_target) ** 2 + torch.sum((h - h_target) ** 2)
+
+
+def chi2_loss(n, h, n_target, h_target):
+    """Pearson chi-square of (N, h) against the target counts (Poisson variances):
+    (N - N*)^2 / N* + sum_b (h_b - h*_b)^2 / max(h*_b, 1)."""
+    return (n - n_target) ** 2 / torch.clamp(n_target, min=1.0) + torch.sum(
+        (h - h_target) ** 2 / torch.clamp(h_target, min=1.0))

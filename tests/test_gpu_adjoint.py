@@ -260,14 +260,17 @@ def test_adjoint_matches_fp64_replay(crystal):
 
 
 def test_calibration_reduces_loss():
-    """Five Adam steps of the calibration harness (1e6 photons) lower the loss."""
+    """Twelve steps of the calibration harness (66 parameters, 2e6 photons, chi-square
+    loss against a +-20% perturbed target) bring the loss down by more than 50x,
+    to within a few times the noise level of two independent samples (~2 x 257)."""
     import json
     import subprocess
 
-    log("case: calibration harness, 5 iterations")
-    proc = subprocess.run([sys.executable, "-m", "g4go_amd.calibrate", "--photons", "1000000", "--iterations", "5"],
+    log("case: calibration harness, 12 iterations")
+    proc = subprocess.run([sys.executable, "-m", "g4go_amd.calibrate", "--photons", "2000000", "--iterations", "12"],
                           capture_output=True, text=True, timeout=110)
     assert proc.returncode == 0, proc.stderr[-2000:]
     hist = [json.loads(l) for l in proc.stdout.splitlines() if l.startswith("{")]
     log(f"  losses {[round(x['loss'], 1) for x in hist]}")
-    assert len(hist) == 5 and hist[-1]["loss"] < hist[0]["loss"]
+    assert len(hist) == 12 and hist[-1]["loss"] < hist[0]["loss"] / 50
+    assert hist[-1]["loss"] < 5 * 2 * 257
