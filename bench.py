@@ -407,7 +407,7 @@ def main():
                    "detected_fraction": D / max(P, 1)},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": VALU_PEAK_TLANEOPS, "unit": "Tlane-op/s",
                      "frac": achieved / VALU_PEAK_TLANEOPS, "traffic": traffic,
-                     "kernel": "transport_kernel<synthetic, persistent>", "kernel_ms": k_ms,
+                     "kernel": "transport_kernel_persistent<synthetic>", "kernel_ms": k_ms,
                      "work_model": {"per_photon": W_GEN, "per_step": W_STEP, "per_detection": W_DET}},
         "valu_counters": None if prof is None else {
             "source": "profiles/transport_kernel_crystal_1e8_counters.md",

@@ -21,9 +21,9 @@ namespace {
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
-__device__ __forceinline__ uint64_t lanes_below() {
-    const uint32_t l = lane_id();
-    return l == 0 ? 0ull : (~0ull >> (64u - l));
+__device__ __forceinline__ uint32_t rank_below(uint64_t mask) {
+    return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),
+                                     __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
 }
 
 __device__ __forceinline__ uint32_t uniform(uint32_t v) {
@@ -150,12 +150,14 @@ __device__ __forceinline__ void flush_stage(const KernelParams& p, const float4*
     }
     base = uniform(static_cast<uint32_t>(__shfl(static_cast<int>(base), 0)));
     float4* dst = reinterpret_cast<float4*>(p.hits);
+#pragma unroll 1
     for (uint32_t i = lane; i < 3u * staged; i += 64u) {
         if (base + i / 3u < p.hit_capacity) {
             dst[3ull * base + i] = stage[i];
         }
     }
     if (p.hit_launch != nullptr) {
+#pragma unroll 1
         for (uint32_t i = lane; i < staged; i += 64u) {
             if (base + i < p.hit_capacity) {
                 p.hit_launch[base + i] = src[i];
@@ -181,7 +183,7 @@ __device__ __forceinline__ void store_hit(float4* d, const Photon& ph, uint32_t 
 }  // namespace
 
 template <int kSource, bool kPersistent>
-__global__ void __launch_bounds__(256) transport_kernel(KernelParams p) {
+__device__ __forceinline__ void transport_body(const KernelParams& p) {
     __shared__ uint32_t s_hist[kHistSlots];
     __shared__ float4 s_stage[kMaxWavesPerBlock][3 * kStageCap];
     __shared__ uint32_t s_stage_src[kMaxWavesPerBlock][kStageCap];
@@ -207,7 +209,6 @@ __global__ void __launch_bounds__(256) transport_kernel(KernelParams p) {
     uint32_t staged = 0;  // wave-uniform
 
     const uint32_t lane = lane_id();
-    const uint64_t below = lanes_below();
     const uint32_t n_sources = uniform(p.header[0]);
     const uint32_t chunk = uniform(p.header[1]);
     const uint32_t first_item = p.item_end != 0u ? p.item_begin : 0u;
@@ -219,7 +220,7 @@ __global__ void __launch_bounds__(256) transport_kernel(KernelParams p) {
     uint32_t my_event = kInvalid;
     uint32_t my_launch = kInvalid;
     LaneStats st{0, 0, 0, 0, 0, 0, 0};
-    uint32_t lane_steps = 0, lane_photons = 0;
+    unsigned long long wave_steps = 0, wave_photons = 0;
 
     // wave-uniform queue cursor
     uint32_t it_event = 0, it_source = 0, it_first = 0, it_next = 0, it_end = 0;
@@ -264,7 +265,7 @@ __global__ void __launch_bounds__(256) transport_kernel(KernelParams p) {
                     continue;
                 }
                 const uint32_t take = min(it_end - it_next, n_idle);
-                const uint32_t rank = static_cast<uint32_t>(__popcll(idle & below));
+                const uint32_t rank = rank_below(idle);
                 const bool fill = !alive && rank < take;
                 flush_event_group(p.event_stats, my_event, st, fill && my_event != it_event && my_event != kInvalid);
                 if (fill) {
@@ -272,7 +273,6 @@ __global__ void __launch_bounds__(256) transport_kernel(KernelParams p) {
                     generate<kSource>(ph, p, it_event, it_source, it_first + it_next + rank);
                     ph.material = kInvalid;
                     my_launch = p.source_photon_offset[it_source] + it_first + it_next + rank;
-                    ++lane_photons;
                     if (start_photon(ph)) {
                         alive = true;
                     } else {
@@ -280,16 +280,19 @@ __global__ void __launch_bounds__(256) transport_kernel(KernelParams p) {
                     }
                 }
                 it_next += take;
+                wave_photons += take;
                 idle = __ballot(!alive);
                 n_idle = static_cast<uint32_t>(__popcll(idle));
             }
         }
-        if (__ballot(alive) == 0ull) {
+        const uint64_t alive_lanes = __ballot(alive);
+        if (alive_lanes == 0ull) {
             if (done) {
                 break;
             }
             continue;
         }
+        wave_steps += static_cast<unsigned long long>(__popcll(alive_lanes));
 
         // ---- one transport step ----
         int status = kAlive;
@@ -298,7 +301,6 @@ __global__ void __launch_bounds__(256) transport_kernel(KernelParams p) {
             bool zero = false;
             status = transport_step<false>(ph, sv, p.rng, p.cfg, zero, sensor, nullptr);
             ++ph.step;
-            ++lane_steps;
             st.zero_step += zero ? 1u : 0u;
             if (status == kAlive && ph.step >= p.max_steps) {
                 status = -1;  // truncated
@@ -309,7 +311,7 @@ __global__ void __launch_bounds__(256) transport_kernel(KernelParams p) {
         const uint64_t det = __ballot(status == kDetected);
         if (det != 0ull) {
             if (status == kDetected) {
-                const uint32_t slot = staged + static_cast<uint32_t>(__popcll(det & below));
+                const uint32_t slot = staged + rank_below(det);
                 store_hit(stage + 3u * slot, ph, sensor);
                 stage_src[slot] = my_launch;
                 atomicAdd(&s_hist[hist_slot(ph.time, p.hist_lo_ns, p.hist_width_ns)], 1u);
@@ -341,11 +343,9 @@ __global__ void __launch_bounds__(256) transport_kernel(KernelParams p) {
         flush_stage(p, stage, stage_src, staged, lane);
     }
     flush_event_group(p.event_stats, my_event, st, my_event != kInvalid);
-    const uint32_t ws = wave_sum(lane_steps);
-    const uint32_t wp = wave_sum(lane_photons);
     if (lane == 0u) {
-        if (ws) atomicAdd(p.counters + kCounterSteps, static_cast<unsigned long long>(ws));
-        if (wp) atomicAdd(p.counters + kCounterPhotons, static_cast<unsigned long long>(wp));
+        if (wave_steps) atomicAdd(p.counters + kCounterSteps, wave_steps);
+        if (wave_photons) atomicAdd(p.counters + kCounterPhotons, wave_photons);
     }
     __syncthreads();
     for (int i = threadIdx.x; i < kHistSlots; i += blockDim.x) {
@@ -353,6 +353,21 @@ __global__ void __launch_bounds__(256) transport_kernel(KernelParams p) {
             atomicAdd(p.histogram + i, static_cast<unsigned long long>(s_hist[i]));
         }
     }
+}
+
+// Persistent form: four waves per SIMD, i.e. at most 128 VGPRs.  Left alone
+// the compiler schedules the body into ~140 registers (three waves); the
+// fourth wave hides more of the dependent scene-load latency of a step than
+// the few spills it forces cost.
+template <int kSource>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
+transport_kernel_persistent(KernelParams p) {
+    transport_body<kSource, true>(p);
+}
+
+template <int kSource>
+__global__ void __launch_bounds__(256) transport_kernel_plain(KernelParams p) {
+    transport_body<kSource, false>(p);
 }
 
 __global__ void zero_kernel(uint32_t* ptr, size_t words) {
@@ -379,15 +394,15 @@ int launch_transport(const KernelParams& p, int source, bool persistent, int blo
     }
     if (source == kSourceSynthetic) {
         if (persistent) {
-            hipLaunchKernelGGL((transport_kernel<kSourceSynthetic, true>), dim3(grid), dim3(block), lds, stream, p);
+            hipLaunchKernelGGL(transport_kernel_persistent<kSourceSynthetic>, dim3(grid), dim3(block), lds, stream, p);
         } else {
-            hipLaunchKernelGGL((transport_kernel<kSourceSynthetic, false>), dim3(grid), dim3(block), lds, stream, p);
+            hipLaunchKernelGGL(transport_kernel_plain<kSourceSynthetic>, dim3(grid), dim3(block), lds, stream, p);
         }
     } else {
         if (persistent) {
-            hipLaunchKernelGGL((transport_kernel<kSourceEmission, true>), dim3(grid), dim3(block), lds, stream, p);
+            hipLaunchKernelGGL(transport_kernel_persistent<kSourceEmission>, dim3(grid), dim3(block), lds, stream, p);
         } else {
-            hipLaunchKernelGGL((transport_kernel<kSourceEmission, false>), dim3(grid), dim3(block), lds, stream, p);
+            hipLaunchKernelGGL(transport_kernel_plain<kSourceEmission>, dim3(grid), dim3(block), lds, stream, p);
         }
     }
     return hipGetLastError() == hipSuccess ? 0 : 1;

@@ -285,72 +285,112 @@ struct D3 {
 G4GO_HD D3 dvec(V3 a) { return {static_cast<double>(a.x), static_cast<double>(a.y), static_cast<double>(a.z)}; }
 G4GO_HD double ddot(D3 a, D3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 G4GO_HD D3 dcross(D3 a, D3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
-G4GO_HD D3 dnormalize(D3 a) {
-    const double inv = 1.0 / sqrt(ddot(a, a));
-    return {a.x * inv, a.y * inv, a.z * inv};
-}
 
-// Fresnel equations with polarisation (OptiXBoundaryPhysics.cuh:122-178).
-// Directions and polarisations are FP32 with the reference's operation
-// order; the reflect/transmit probability and the total-internal-reflection
-// test are evaluated in FP64 from the same FP32 state (as Geant4's
-// G4OpBoundaryProcess does), which keeps them accurate near the critical
-// angle where 1 - sin^2(theta_t) cancels.
-G4GO_HD Fresnel compute_fresnel(V3 dir, V3 pol, V3 n, float n1, float n2) {
-    dir = normalize(dir);
-    n = normalize(n);
-    pol = project_polarization(pol, dir);
-    const D3 dd = dvec(dir), dn = dvec(n), dp = dvec(pol);
+// Fresnel equations with polarisation (OptiXBoundaryPhysics.cuh:122-178),
+// in two phases: fresnel_state() gives the reflect/transmit probability, then
+// fresnel_reflect() or fresnel_transmit() builds only the outgoing direction
+// and polarisation that the draw selects.  Directions and polarisations are
+// FP32 with the reference's operation order.  The probability and the
+// total-internal-reflection test are evaluated in FP64 from the same FP32
+// state (as Geant4's G4OpBoundaryProcess does), which keeps them accurate
+// near the critical angle where 1 - sin^2(theta_t) cancels.  The s/p split
+// of the polarisation uses |p.s|^2 = (p.(d x n))^2 / |d x n|^2 and
+// |p.s|^2 + |p.p|^2 = |p|^2 - (p.d)^2 / |d|^2 (the squared projections on the
+// unit s and p directions, both perpendicular to d).
+struct FresnelState {
+    V3 dir, n, pol;  // normalised direction and normal, polarisation projected perpendicular to dir
+    float cos_i, eta, cos_t;
+    double transmittance;
+    bool tir;
+    bool normal_incidence;  // |dir x n|^2 < 1e-12: no s/p split
+};
+
+G4GO_HD FresnelState fresnel_state(V3 dir, V3 pol, V3 n, float n1, float n2) {
+    FresnelState f;
+    f.dir = normalize(dir);
+    f.n = normalize(n);
+    f.pol = project_polarization(pol, f.dir);
+    const D3 dd = dvec(f.dir), dn = dvec(f.n);
     const double n1d = n1, n2d = n2;
     const double ci_d = fmin(fmax(-ddot(dd, dn), 0.0), 1.0);
     const double eta_d = n1d / n2d;
     const double st2_d = eta_d * eta_d * (1.0 - ci_d * ci_d);
-    const float cos_i = clamp01(-dot(dir, n));
-    const float eta = n1 / n2;
-    const float sin_t2 = eta * eta * (1.0f - cos_i * cos_i);
-    Fresnel out{};
-    out.reflected_dir = reflect_direction(dir, n);
-    out.tir = st2_d >= 1.0;
-    out.transmittance = 0.0;
-    if (out.tir) {
-        out.reflected_pol = reflect_polarization(pol, n, out.reflected_dir);
-        return out;
+    f.cos_i = clamp01(-dot(f.dir, f.n));
+    f.eta = n1 / n2;
+    const float sin_t2 = f.eta * f.eta * (1.0f - f.cos_i * f.cos_i);
+    f.tir = st2_d >= 1.0;
+    f.transmittance = 0.0;
+    f.cos_t = sqrtf(fmaxf(0.0f, 1.0f - sin_t2));
+    const V3 transverse = cross(f.dir, f.n);
+    f.normal_incidence = dot(transverse, transverse) < 1.0e-12f;
+    if (f.tir) {
+        return f;
+    }
+    if (f.normal_incidence) {
+        const double ra = (n1d - n2d) / (n1d + n2d);
+        f.transmittance = fmin(fmax(1.0 - ra * ra, 0.0), 1.0);
+        return f;
     }
     const double ct_d = sqrt(fmax(0.0, 1.0 - st2_d));
-    const float cos_t = sqrtf(fmaxf(0.0f, 1.0f - sin_t2));
-    out.transmitted_dir = normalize(add(scale(dir, eta), scale(n, eta * cos_i - cos_t)));
-    const V3 transverse = cross(dir, n);
-    if (dot(transverse, transverse) < 1.0e-12f) {
-        const double ra = (n1d - n2d) / (n1d + n2d);
-        out.transmittance = fmin(fmax(1.0 - ra * ra, 0.0), 1.0);
-        out.reflected_pol = n2 > n1 ? scale(pol, -1.0f) : pol;
-        out.transmitted_pol = pol;
-        return out;
+    const D3 dp = dvec(f.pol);
+    const D3 c = dcross(dd, dn);
+    const double pc = ddot(dp, c);
+    const double es2 = pc * pc / ddot(c, c);
+    const double pdd = ddot(dp, dd);
+    const double perp2 = ddot(dp, dp) - pdd * pdd / ddot(dd, dd);
+    const double rs = (n1d * ci_d - n2d * ct_d) / (n1d * ci_d + n2d * ct_d);
+    const double rp = (n2d * ci_d - n1d * ct_d) / (n2d * ci_d + n1d * ct_d);
+    f.transmittance = fmin(fmax(1.0 - (es2 * (rs * rs) + (perp2 - es2) * (rp * rp)), 0.0), 1.0);
+    return f;
+}
+
+G4GO_HD void fresnel_reflect(const FresnelState& f, float n1, float n2, V3& dir, V3& pol) {
+    dir = reflect_direction(f.dir, f.n);
+    if (f.tir) {
+        pol = reflect_polarization(f.pol, f.n, dir);
+        return;
     }
-    // FP64 probability
-    {
-        const D3 sd = dnormalize(dcross(dd, dn));
-        const D3 pd = dnormalize(dcross(dd, sd));
-        const double es = ddot(dp, sd), ep = ddot(dp, pd);
-        const double rs = (n1d * ci_d - n2d * ct_d) / (n1d * ci_d + n2d * ct_d);
-        const double rp = (n2d * ci_d - n1d * ct_d) / (n2d * ci_d + n1d * ct_d);
-        out.transmittance = fmin(fmax(1.0 - (es * es * rs * rs + ep * ep * rp * rp), 0.0), 1.0);
+    if (f.normal_incidence) {
+        pol = n2 > n1 ? scale(f.pol, -1.0f) : f.pol;
+        return;
     }
-    // FP32 outgoing polarisations
-    const V3 s_dir = normalize(transverse);
-    const V3 p_dir = normalize(cross(dir, s_dir));
-    const float in_s = dot(pol, s_dir);
-    const float in_p = dot(pol, p_dir);
-    const float s_den = n1 * cos_i + n2 * cos_t;
-    const float p_den = n2 * cos_i + n1 * cos_t;
-    const float rs = (n1 * cos_i - n2 * cos_t) / s_den;
-    const float rp = (n2 * cos_i - n1 * cos_t) / p_den;
-    const float ts = 2.0f * n1 * cos_i / s_den;
-    const float tp = 2.0f * n1 * cos_i / p_den;
-    const V3 rp_dir = normalize(cross(out.reflected_dir, s_dir));
-    const V3 tp_dir = normalize(cross(out.transmitted_dir, s_dir));
-    out.reflected_pol = normalize(add(scale(s_dir, in_s * rs), scale(rp_dir, in_p * rp)));
-    out.transmitted_pol = normalize(add(scale(s_dir, in_s * ts), scale(tp_dir, in_p * tp)));
+    const V3 s_dir = normalize(cross(f.dir, f.n));
+    const V3 p_dir = normalize(cross(f.dir, s_dir));
+    const float in_s = dot(f.pol, s_dir);
+    const float in_p = dot(f.pol, p_dir);
+    const float rs = (n1 * f.cos_i - n2 * f.cos_t) / (n1 * f.cos_i + n2 * f.cos_t);
+    const float rp = (n2 * f.cos_i - n1 * f.cos_t) / (n2 * f.cos_i + n1 * f.cos_t);
+    const V3 rp_dir = normalize(cross(dir, s_dir));
+    pol = normalize(add(scale(s_dir, in_s * rs), scale(rp_dir, in_p * rp)));
+}
+
+// Not defined under total internal reflection.
+G4GO_HD void fresnel_transmit(const FresnelState& f, float n1, float n2, V3& dir, V3& pol) {
+    dir = normalize(add(scale(f.dir, f.eta), scale(f.n, f.eta * f.cos_i - f.cos_t)));
+    if (f.normal_incidence) {
+        pol = f.pol;
+        return;
+    }
+    const V3 s_dir = normalize(cross(f.dir, f.n));
+    const V3 p_dir = normalize(cross(f.dir, s_dir));
+    const float in_s = dot(f.pol, s_dir);
+    const float in_p = dot(f.pol, p_dir);
+    const float ts = 2.0f * n1 * f.cos_i / (n1 * f.cos_i + n2 * f.cos_t);
+    const float tp = 2.0f * n1 * f.cos_i / (n2 * f.cos_i + n1 * f.cos_t);
+    const V3 tp_dir = normalize(cross(dir, s_dir));
+    pol = normalize(add(scale(s_dir, in_s * ts), scale(tp_dir, in_p * tp)));
+}
+
+// Both outcomes at once (for the evaluation entry point).
+G4GO_HD Fresnel compute_fresnel(V3 dir, V3 pol, V3 n, float n1, float n2) {
+    const FresnelState f = fresnel_state(dir, pol, n, n1, n2);
+    Fresnel out{};
+    out.tir = f.tir;
+    out.transmittance = f.transmittance;
+    fresnel_reflect(f, n1, n2, out.reflected_dir, out.reflected_pol);
+    if (!f.tir) {
+        fresnel_transmit(f, n1, n2, out.transmitted_dir, out.transmitted_pol);
+    }
     return out;
 }
 

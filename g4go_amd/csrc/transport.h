@@ -946,7 +946,7 @@ G4GO_HD int transport_step(Photon& ph, const SceneView& sc, const Rng& rng, cons
                     wr = rng.block(ph.photon_id, ph.event_id, ph.step, kProcRough, 0);
                     rough_pass = u01(wr.x) < g4_expf(-expo * expo);
                 }
-                const Fresnel f = compute_fresnel(dir, pol, inormal, n1, n2);
+                const FresnelState f = fresnel_state(dir, pol, inormal, n1, n2);
                 reflected = f.tir || static_cast<double>(u01(w.w)) > f.transmittance;
                 if (reflected) {
                     if (rough_pass) {
@@ -954,16 +954,14 @@ G4GO_HD int transport_step(Photon& ph, const SceneView& sc, const Rng& rng, cons
                             dir = sample_unified_reflection(sc, s, dir, normal, rng, ph);
                             pol = reflect_polarization(pol, normal, dir);
                         } else {
-                            dir = f.reflected_dir;
-                            pol = f.reflected_pol;
+                            fresnel_reflect(f, n1, n2, dir, pol);
                         }
                     } else {
                         dir = sample_lambertian(normal, u01(wr.y), u01(wr.z));
                         pol = project_polarization(pol, dir);
                     }
                 } else {
-                    dir = f.transmitted_dir;
-                    pol = f.transmitted_pol;
+                    fresnel_transmit(f, n1, n2, dir, pol);
                     transmitted = true;
                 }
             }
@@ -988,14 +986,12 @@ G4GO_HD int transport_step(Photon& ph, const SceneView& sc, const Rng& rng, cons
         if (!(n1 > 0.0f) || !finitef(n1) || !(n2 > 0.0f) || !finitef(n2)) {
             absorbed = true;
         } else {
-            const Fresnel f = compute_fresnel(dir, pol, normal, n1, n2);
+            const FresnelState f = fresnel_state(dir, pol, normal, n1, n2);
             reflected = static_cast<double>(u01(w.w)) > f.transmittance;
             if (reflected) {
-                dir = f.reflected_dir;
-                pol = f.reflected_pol;
+                fresnel_reflect(f, n1, n2, dir, pol);
             } else {
-                dir = f.transmitted_dir;
-                pol = f.transmitted_pol;
+                fresnel_transmit(f, n1, n2, dir, pol);
                 transmitted = true;
             }
         }

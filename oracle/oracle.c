@@ -349,11 +349,6 @@ static d3 dcross(d3 a, d3 b) {
     d3 r = {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
     return r;
 }
-static d3 dnorm(d3 a) {
-    const double inv = 1.0 / sqrt(ddot(a, a));
-    d3 r = {a.x * inv, a.y * inv, a.z * inv};
-    return r;
-}
 
 /* ComputeFresnel (OptiXBoundaryPhysics.cuh:122-178): vectors in the state
  * precision with the reference's operation order; the reflect/transmit
@@ -391,12 +386,17 @@ static fresnel_t or_fresnel(v3 d, v3 p, v3 n, real n1, real n2) {
         return o;
     }
     {
-        const d3 sd = dnorm(dcross(dd, dn));
-        const d3 pd = dnorm(dcross(dd, sd));
-        const double es = ddot(dp, sd), ep = ddot(dp, pd);
+        /* squared projections of p on the unit s and p directions (both
+         * perpendicular to d): |p.s|^2 = (p.(d x n))^2 / |d x n|^2 and
+         * |p.s|^2 + |p.p|^2 = |p|^2 - (p.d)^2 / |d|^2 */
+        const d3 c = dcross(dd, dn);
+        const double pc = ddot(dp, c);
+        const double es2 = pc * pc / ddot(c, c);
+        const double pdd = ddot(dp, dd);
+        const double perp2 = ddot(dp, dp) - pdd * pdd / ddot(dd, dd);
         const double rs = (n1d * ci_d - n2d * ct_d) / (n1d * ci_d + n2d * ct_d);
         const double rp = (n2d * ci_d - n1d * ct_d) / (n2d * ci_d + n1d * ct_d);
-        o.t = fmin(fmax(1.0 - (es * es * rs * rs + ep * ep * rp * rp), 0.0), 1.0);
+        o.t = fmin(fmax(1.0 - (es2 * (rs * rs) + (perp2 - es2) * (rp * rp)), 0.0), 1.0);
     }
     const v3 sd = or_normalize(tr);
     const v3 pd = or_normalize(vcross(d, sd));
