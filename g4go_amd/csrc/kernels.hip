@@ -519,8 +519,10 @@ __device__ __forceinline__ void add_material_grads(double* grad, const SceneView
 
 }  // namespace
 
+// Four waves per SIMD (at most 128 VGPRs), as for the persistent transport
+// kernel: the replay runs the same step code.
 template <int kSource>
-__global__ void __launch_bounds__(256) adjoint_kernel(AdjointParams a) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) adjoint_kernel(AdjointParams a) {
     extern __shared__ double s_grad[];
     const uint32_t n_params = a.map.n_params;
     const bool in_lds = n_params <= kAdjointLdsParams;

@@ -16,7 +16,9 @@ namespace g4go {
 
 constexpr uint32_t kFaceDynamic = 0xfffffffeu;  // neighbour found by a point query at run time
 
-struct Prop {
+// 16 bytes, 16-byte aligned in every record that holds one, so a lookup
+// reads the whole descriptor with one load (load_prop).
+struct alignas(16) Prop {
     uint32_t offset;  // energies at pool[offset], values at pool[offset + count]
     uint32_t count;   // 0 = property absent
     uint32_t flags;   // kPropConstant | kPropHotGrid
@@ -48,7 +50,7 @@ struct Surface {
     Prop reflectivity, efficiency, transmittance, lobe, spike, backscatter, roughness;
 };
 
-struct Volume {
+struct alignas(16) Volume {
     float lo[3];
     uint32_t material;
     float hi[3];

@@ -102,7 +102,16 @@ enum TraceEvent : int {
 // interpolation over ascending energies.  The bracketing node is found by a
 // binary search, which selects the same node as the reference's linear scan.
 // ---------------------------------------------------------------------------
-G4GO_HD float prop_eval(const float* pool, const Prop& p, float e, float fallback) {
+// One 16-byte read of a property descriptor (instead of one load per field
+// on each branch that needs it).
+G4GO_HD Prop load_prop(const Prop& p) {
+    Prop q;
+    __builtin_memcpy(&q, &p, sizeof(Prop));
+    return q;
+}
+
+G4GO_HD float prop_eval(const float* pool, const Prop& pref, float e, float fallback) {
+    const Prop p = load_prop(pref);
     if (p.count == 0) {
         return fallback;
     }
@@ -163,7 +172,8 @@ G4GO_HD void grid_locate(const float* en, uint32_t count, float e, uint32_t& i, 
 
 // prop_eval at the photon's energy, through its located hot-grid node when the
 // table lies on the hot grid (bit-identical to prop_eval).
-G4GO_HD float prop_eval_ph(const float* pool, const Prop& p, const Photon& ph, float fallback) {
+G4GO_HD float prop_eval_ph(const float* pool, const Prop& pref, const Photon& ph, float fallback) {
+    const Prop p = load_prop(pref);
     if (p.count == 0) {
         return fallback;
     }
