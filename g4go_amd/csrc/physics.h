@@ -68,7 +68,7 @@ G4GO_HD float u01(uint32_t word) { return fmaf(static_cast<float>(word), 0x1p-32
 // Process identifiers of this backend's draw map.
 enum Process : uint32_t {
     kProcStep = 1,       // per step: w0 distance, w1 bulk process, w2 surface class, w3 Fresnel/detect
-    kProcRayleigh = 2,   // Rayleigh angular sampling, block = attempt
+    kProcRayleigh = 2,   // Rayleigh angular sampling (one block: w0-w2 median and sign, w3 azimuth)
     kProcFacet = 3,      // ground-surface facet normal, block = attempt
     kProcLobe = 4,       // unified-model lobe choice and Lambertian draws
     kProcRough = 5,      // roughness test and Lambertian draws

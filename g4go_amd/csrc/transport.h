@@ -586,42 +586,43 @@ G4GO_HD V3 sample_surface_reflection(const SceneView& sc, const Surface& s, V3 d
 }
 
 // Rayleigh scattering with the dipole distribution relative to the
-// polarisation (Geant4 11 G4OpRayleigh::PostStepDoIt: uniform direction,
-// polarisation = projection of the old one, accept with cos^2 of the angle
-// between old and new polarisations).  One Philox block per attempt:
-// w0 |cos theta|, w1 sign bits, w2 azimuth, w3 acceptance.
+// polarisation (Geant4 11 G4OpRayleigh::PostStepDoIt): the new direction k'
+// has density proportional to 1 - (p.k')^2 about the old polarisation p, and
+// the new polarisation is the unit projection of p perpendicular to k', with
+// a random sign.  Geant4 reaches this by rejection (uniform k', accepted with
+// probability (p'.p)^2 = 1 - (p.k')^2); here it is sampled directly from one
+// Philox block, so a wave never loops on its slowest lane:
+//   * c = p.k' is the median of three uniforms on (-1, 1], whose density is
+//     3/4 (1 - c^2).  u01 is monotone, so that median is u01 of the median of
+//     the words w0, w1, w2;
+//   * the azimuth of k' about p is uniform (w3);
+//   * the sign of the new polarisation is the parity of the order of w0, w1,
+//     w2: a uniformly random permutation, independent of its median.
 G4GO_HD void rayleigh_scatter(Photon& ph, const Rng& rng) {
-    const V3 old_dir = ph.dir;
     const V3 old_pol = ph.pol;
-    for (uint32_t attempt = 0; attempt < 64u; ++attempt) {
-        const U4 w = rng.block(ph.photon_id, ph.event_id, ph.step, kProcRayleigh, attempt);
-        float ct = u01(w.x);
-        const float st = sqrtf(fmaxf(0.0f, 1.0f - ct * ct));
-        if ((w.y & 0x80000000u) != 0u) {
-            ct = -ct;
-        }
-        float sp, cp;
-        g4_sincos_turn(u01(w.z), sp, cp);
-        const V3 new_dir = rotate_uz(V3{st * cp, st * sp, ct}, old_dir);
-        V3 new_pol = sub(old_pol, scale(new_dir, dot(new_dir, old_pol)));
-        const float l2 = dot(new_pol, new_pol);
-        if (!(l2 > 0.0f)) {
-            float sq, cq;
-            g4_sincos_turn(static_cast<float>(w.y & 0x00ffffffu) * 0x1p-24f, sq, cq);
-            new_pol = rotate_uz(V3{cq, sq, 0.0f}, new_dir);
-        } else {
-            new_pol = scale(new_pol, 1.0f / sqrtf(l2));
-            if ((w.y & 0x40000000u) != 0u) {
-                new_pol = scale(new_pol, -1.0f);
-            }
-        }
-        const float c = dot(new_pol, old_pol);
-        ph.dir = new_dir;
-        ph.pol = new_pol;
-        if (!(c * c < u01(w.w))) {
-            return;
+    const U4 w = rng.block(ph.photon_id, ph.event_id, ph.step, kProcRayleigh, 0);
+    const uint32_t lo = min(w.x, w.y), hi = max(w.x, w.y);
+    const uint32_t med = max(lo, min(hi, w.z));
+    const bool flip = ((w.x < w.y) != (w.y < w.z)) != (w.x < w.z);
+    const float c = 2.0f * u01(med) - 1.0f;
+    const float s = sqrtf(fmaxf(0.0f, 1.0f - c * c));
+    float sp, cp;
+    g4_sincos_turn(u01(w.w), sp, cp);
+    const V3 new_dir = rotate_uz(V3{s * cp, s * sp, c}, old_pol);
+    V3 new_pol = sub(old_pol, scale(new_dir, dot(new_dir, old_pol)));
+    const float l2 = dot(new_pol, new_pol);
+    if (!(l2 > 0.0f)) {
+        float sq, cq;
+        g4_sincos_turn(static_cast<float>(w.w & 0x00ffffffu) * 0x1p-24f, sq, cq);
+        new_pol = rotate_uz(V3{cq, sq, 0.0f}, new_dir);
+    } else {
+        new_pol = scale(new_pol, 1.0f / sqrtf(l2));
+        if (flip) {
+            new_pol = scale(new_pol, -1.0f);
         }
     }
+    ph.dir = new_dir;
+    ph.pol = new_pol;
 }
 
 // ---------------------------------------------------------------------------

@@ -940,39 +940,39 @@ static v3 or_surface_reflection(const surface_t* sf, v3 d, v3 n, const rng_t* r,
     return or_unified(sf, d, n, r, ph);
 }
 
-/* Geant4 G4OpRayleigh dipole sampling, one block per attempt. */
+/* Dipole scattering about the old polarisation p (Geant4 G4OpRayleigh's
+ * distribution), sampled directly from one block: c = p.k' is the median of
+ * three uniforms on (-1, 1] (density 3/4 (1 - c^2)), taken as u01 of the
+ * median word; the azimuth about p is uniform; the polarisation sign is the
+ * parity of the order of the three words. */
 static void or_rayleigh(photon_t* ph, const rng_t* r) {
-    const v3 od = ph->dir, op = ph->pol;
+    const v3 op = ph->pol;
     uint32_t w[4];
-    for (uint32_t a = 0; a < 64u; ++a) {
-        block4(r, ph->photon, ph->event, ph->step, P_RAYLEIGH, a, w);
-        real ct = u01(w[0]);
-        const real st = R_SQRT(R_FMAX(RF(0.0), RF(1.0) - ct * ct));
-        if (w[1] & 0x80000000u) {
-            ct = -ct;
-        }
-        real sp, cp;
-        r_sincos_turn(u01(w[2]), &sp, &cp);
-        const v3 nd = or_rotate_uz(mk(st * cp, st * sp, ct), od);
-        v3 np = vsub(op, vscale(nd, vdot(nd, op)));
-        const real l2 = vdot(np, np);
-        if (!(l2 > RF(0.0))) {
-            real sq, cq;
-            r_sincos_turn((real)(w[1] & 0x00ffffffu) * RF(0x1p-24), &sq, &cq);
-            np = or_rotate_uz(mk(cq, sq, RF(0.0)), nd);
-        } else {
-            np = vscale(np, RF(1.0) / R_SQRT(l2));
-            if (w[1] & 0x40000000u) {
-                np = vscale(np, RF(-1.0));
-            }
-        }
-        const real c = vdot(np, op);
-        ph->dir = nd;
-        ph->pol = np;
-        if (!(c * c < u01(w[3]))) {
-            return;
+    block4(r, ph->photon, ph->event, ph->step, P_RAYLEIGH, 0, w);
+    const uint32_t lo = w[0] < w[1] ? w[0] : w[1];
+    const uint32_t hi = w[0] < w[1] ? w[1] : w[0];
+    const uint32_t hz = hi < w[2] ? hi : w[2];
+    const uint32_t med = lo < hz ? hz : lo;
+    const int flip = ((w[0] < w[1]) != (w[1] < w[2])) != (w[0] < w[2]);
+    const real c = RF(2.0) * u01(med) - RF(1.0);
+    const real s = R_SQRT(R_FMAX(RF(0.0), RF(1.0) - c * c));
+    real sp, cp;
+    r_sincos_turn(u01(w[3]), &sp, &cp);
+    const v3 nd = or_rotate_uz(mk(s * cp, s * sp, c), op);
+    v3 np = vsub(op, vscale(nd, vdot(nd, op)));
+    const real l2 = vdot(np, np);
+    if (!(l2 > RF(0.0))) {
+        real sq, cq;
+        r_sincos_turn((real)(w[3] & 0x00ffffffu) * RF(0x1p-24), &sq, &cq);
+        np = or_rotate_uz(mk(cq, sq, RF(0.0)), nd);
+    } else {
+        np = vscale(np, RF(1.0) / R_SQRT(l2));
+        if (flip) {
+            np = vscale(np, RF(-1.0));
         }
     }
+    ph->dir = nd;
+    ph->pol = np;
 }
 
 /* ------------------------------------------------------------------ */

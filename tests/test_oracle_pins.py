@@ -263,7 +263,7 @@ def test_rayleigh_dipole_distribution():
 
     rng = np.random.default_rng(123)
     n = 40000
-    c_vals, mu, perp = np.empty(n), np.empty(n), np.empty(n)
+    c_vals, mu, perp, sign = np.empty(n), np.empty(n), np.empty(n), np.empty(n)
     for i in range(n):
         d = rng.normal(size=3)
         d /= np.linalg.norm(d)
@@ -273,11 +273,18 @@ def test_rayleigh_dipole_distribution():
         c_vals[i] = np.dot(p, nd)
         mu[i] = np.dot(d, nd)
         perp[i] = abs(np.dot(nd, npol)) + abs(np.dot(npol, npol) - 1) + abs(np.dot(nd, nd) - 1)
+        # new polarisation = +-(unit projection of p perpendicular to k')
+        sign[i] = np.sign(np.dot(npol, p - np.dot(p, nd) * nd))
     ks = stats.kstest(c_vals, lambda c: (2 + 3 * c - c ** 3) / 4)
     assert ks.pvalue > 0.01, ks
     assert abs(np.mean(mu)) < 4 / math.sqrt(3 * n)  # <cos theta> = 0 (forward-backward symmetric)
     assert abs(np.mean(c_vals ** 2) - 0.2) < 0.01   # E[(p.k')^2] = 1/5
     assert np.max(perp) < 1e-12
+    # the sign is a fair coin, independent of p.k'
+    assert np.all(np.abs(sign) == 1)
+    assert abs(np.mean(sign)) < 4 / math.sqrt(n)
+    assert abs(np.corrcoef(sign, c_vals ** 2)[0, 1]) < 4 / math.sqrt(n)
+    assert abs(np.corrcoef(sign, c_vals)[0, 1]) < 4 / math.sqrt(n)
 
 
 def test_box_exit_against_brute_force():
