@@ -416,8 +416,8 @@ HostScene build_scene(const g4go_scene& in, float tolerance) {
         for (int axis = 0; axis < 3; ++axis) {
             for (int side = 0; side < 2; ++side) {
                 const int face = 2 * axis + side;
-                v.face_neighbor[face] = kInvalid;
-                v.face_surface[face] = kInvalid;
+                v.faces[face].neighbor = kInvalid;
+                v.faces[face].surface = kInvalid;
                 if (v.parent == kInvalid) {
                     continue;  // leaving the world: escape
                 }
@@ -451,11 +451,11 @@ HostScene build_scene(const g4go_scene& in, float tolerance) {
                 } else if (touching.size() == 1 && covered) {
                     next = touching[0];
                 } else {
-                    v.face_neighbor[face] = kFaceDynamic;
+                    v.faces[face].neighbor = kFaceDynamic;
                     continue;
                 }
-                v.face_neighbor[face] = next;
-                v.face_surface[face] = host_find_surface(s, i, next);
+                v.faces[face].neighbor = next;
+                v.faces[face].surface = host_find_surface(s, i, next);
             }
         }
     }
@@ -718,7 +718,7 @@ struct Core {
         p.max_steps = cfg.max_bounces_per_photon;
         p.refill_threshold = std::max<uint32_t>(1, std::min<uint32_t>(64, launch.refill_threshold));
         p.hist_lo_ns = 0.0;
-        p.hist_width_ns = 200.0 / kHistBins;
+        p.hist_bins_per_ns = kHistBins / 200.0;
         // scenes with a daughter BVH: every block stages the nodes and the
         // daughter boxes in LDS when they fit
         const uint64_t entries = static_cast<uint64_t>(view.n_bvh_nodes) + view.n_daughters;

@@ -62,7 +62,7 @@ struct KernelParams {
     unsigned long long* histogram;
     EventStats* event_stats;
     unsigned long long* counters;
-    double hist_lo_ns, hist_width_ns;
+    double hist_lo_ns, hist_bins_per_ns;  // bin = floor((t - lo) * bins_per_ns)
     // BVH nodes + daughter boxes staged into LDS by every block (32-byte
     // entries; 0 = walk them in global memory).  The launch allocates
     // lds_entries * 32 bytes of dynamic LDS.

@@ -21,6 +21,7 @@ namespace {
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
+// Number of lanes below this one whose bit is set in `mask`.
 __device__ __forceinline__ uint32_t rank_below(uint64_t mask) {
     return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),
                                      __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
@@ -91,8 +92,11 @@ __device__ __forceinline__ void flush_event_group(EventStats* es, uint32_t event
     }
 }
 
-__device__ __forceinline__ int hist_slot(double t, double lo, double width) {
-    const double x = (t - lo) / width;
+// Histogram slot of arrival time t: bin floor((t - lo) * bins_per_ns), then
+// underflow (and NaN) and overflow slots.  A product rather than a division by
+// the bin width keeps an FP64 divide out of every detection.
+__device__ __forceinline__ int hist_slot(double t, double lo, double bins_per_ns) {
+    const double x = (t - lo) * bins_per_ns;
     if (!(x >= 0.0)) {
         return kHistBins;  // underflow (and NaN)
     }
@@ -314,7 +318,7 @@ __device__ __forceinline__ void transport_body(const KernelParams& p) {
                 const uint32_t slot = staged + rank_below(det);
                 store_hit(stage + 3u * slot, ph, sensor);
                 stage_src[slot] = my_launch;
-                atomicAdd(&s_hist[hist_slot(ph.time, p.hist_lo_ns, p.hist_width_ns)], 1u);
+                atomicAdd(&s_hist[hist_slot(ph.time, p.hist_lo_ns, p.hist_bins_per_ns)], 1u);
             }
             staged += static_cast<uint32_t>(__popcll(det));
             if (staged >= kStageFlush) {
@@ -588,7 +592,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) a
         }
         ++replayed;
         const float e = ph.energy;
-        const double weight = a.slot_weight[hist_slot(hit.time, p.hist_lo_ns, p.hist_width_ns)];
+        const double weight = a.slot_weight[hist_slot(hit.time, p.hist_lo_ns, p.hist_bins_per_ns)];
         add_material_grads(grad, sc, a.map, m0, e, weight);
         add_material_grads(grad, sc, a.map, m1, e, weight);
         const SurfSlot rs[2] = {r0, r1};

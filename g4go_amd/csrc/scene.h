@@ -50,6 +50,11 @@ struct Surface {
     Prop reflectivity, efficiency, transmittance, lobe, spike, backscatter, roughness;
 };
 
+struct alignas(8) Face {
+    uint32_t neighbor;  // volume across the face, kInvalid (leave the world) or kFaceDynamic
+    uint32_t surface;   // surface for a static neighbour, kInvalid = none
+};
+
 struct alignas(16) Volume {
     float lo[3];
     uint32_t material;
@@ -60,8 +65,9 @@ struct alignas(16) Volume {
     uint32_t entry_surface;                   // surface crossed when entering from the parent
     uint32_t bvh_root;                        // root node of the daughter BVH, kInvalid = linear list
     uint32_t pad[2];
-    uint32_t face_neighbor[6];  // face = 2*axis + (outward sign > 0); volume, kInvalid (leave world) or kFaceDynamic
-    uint32_t face_surface[6];   // surface for a static face neighbour, kInvalid = none
+    // face = 2*axis + (outward sign > 0); the neighbour and its surface sit
+    // side by side so a step reads both with one 8-byte load
+    Face faces[6];
 };
 
 struct Binding {

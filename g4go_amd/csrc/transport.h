@@ -901,13 +901,15 @@ G4GO_HD int transport_step(Photon& ph, const SceneView& sc, const Rng& rng, cons
 
     if (!entering) {
         const int face = 2 * axis + (dsign > 0.0f ? 1 : 0);
-        next = vol.face_neighbor[face];
+        Face fr;
+        __builtin_memcpy(&fr, &vol.faces[face], sizeof(Face));
+        next = fr.neighbor;
         if (next == kFaceDynamic) {
             const V3 q = add(bpos, scale(normal, -cfg.eps));
             next = point_neighbor(sc, ph.volume, q);
             surface = next == kInvalid ? kInvalid : find_surface(sc, ph.volume, next);
         } else {
-            surface = vol.face_surface[face];
+            surface = fr.surface;
         }
         if (next == kInvalid) {
             return ph.volume == sc.world ? kEscaped : kInvalidState;

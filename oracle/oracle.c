@@ -1343,8 +1343,9 @@ typedef struct {
     uint64_t detected, absorbed, escaped, truncated, max_bounce, invalid, zero_step, steps, generated;
 } ostats_t;
 
+/* bin floor((t - 0) * 256 / 200), as the device computes it */
 static int hist_slot(double t) {
-    const double x = (t - 0.0) / (200.0 / 256.0);
+    const double x = (t - 0.0) * (256.0 / 200.0);
     if (!(x >= 0.0)) {
         return 256;
     }
