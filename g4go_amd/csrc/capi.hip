@@ -791,7 +791,10 @@ struct Core {
         const Layout& l = layout;
         d_hits.ensure(std::max<uint64_t>(n_photons, 1) * sizeof(Hit), "hipMalloc hits");
         d_hit_launch.ensure(std::max<uint64_t>(n_photons, 1) * sizeof(uint32_t), "hipMalloc hit launch indices");
-        d_event_stats.ensure(l.cap_ev * sizeof(EventStats), "hipMalloc event stats");
+        // kEventReplicas accumulation copies, then the reduced array
+        d_event_stats.ensure((kEventReplicas + 1) * l.cap_ev * sizeof(EventStats), "hipMalloc event stats");
+        const size_t replica_bytes = kEventReplicas * l.cap_ev * sizeof(EventStats);
+        EventStats* const d_events_out = d_event_stats.as<EventStats>() + kEventReplicas * l.cap_ev;
         d_meta.ensure(kMetaBytes, "hipMalloc counters");
         h_meta.ensure(kMetaBytes + n_events * sizeof(EventStats), "hipHostMalloc counters");
 
@@ -843,6 +846,7 @@ struct Core {
         p.hit_launch = d_hit_launch.as<uint32_t>();
         p.histogram = reinterpret_cast<unsigned long long*>(d_meta.as<char>() + kMetaHist);
         p.event_stats = d_event_stats.as<EventStats>();
+        p.event_stride = static_cast<uint32_t>(l.cap_ev);
         p.counters = reinterpret_cast<unsigned long long*>(d_meta.as<char>() + kMetaCounters);
         last_params = p;
         last_source = source;
@@ -852,8 +856,13 @@ struct Core {
         const uint32_t waves_per_block = launch.block_size / 64;
         const auto zero_outputs = [&] {
             if (launch_zero(d_meta.ptr, kMetaBytes, stream) != 0 ||
-                launch_zero(d_event_stats.ptr, d_event_stats.cap, stream) != 0) {
+                launch_zero(d_event_stats.ptr, replica_bytes, stream) != 0) {
                 fail(G4GO_ERROR_RUNTIME, "zero kernel launch failed");
+            }
+        };
+        const auto reduce_events = [&] {
+            if (launch_reduce_event_stats(p.event_stats, p.event_stride, p.header, d_events_out, stream) != 0) {
+                fail(G4GO_ERROR_RUNTIME, "event statistics reduction launch failed");
             }
         };
         if (launch.persistent) {
@@ -865,8 +874,10 @@ struct Core {
                     drop_graph();
                     hip_check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
                     bool ok = launch_zero(d_meta.ptr, kMetaBytes, stream) == 0 &&
-                              launch_zero(d_event_stats.ptr, d_event_stats.cap, stream) == 0 &&
-                              launch_transport(p, source, true, block, grid, stream) == 0;
+                              launch_zero(d_event_stats.ptr, replica_bytes, stream) == 0 &&
+                              launch_transport(p, source, true, block, grid, stream) == 0 &&
+                              launch_reduce_event_stats(p.event_stats, p.event_stride, p.header, d_events_out,
+                                                        stream) == 0;
                     const hipError_t end = hipStreamEndCapture(stream, &graph);
                     if (!ok || end != hipSuccess) {
                         drop_graph();
@@ -886,6 +897,7 @@ struct Core {
                 if (launch_transport(p, source, true, block, grid, stream) != 0) {
                     fail(G4GO_ERROR_RUNTIME, "transport kernel launch failed");
                 }
+                reduce_events();
             }
         } else {
             zero_outputs();
@@ -917,6 +929,7 @@ struct Core {
                     s0 = s1;
                 }
             }
+            reduce_events();
         }
         hip_check(hipEventRecord(ev_kernel1, stream), "hipEventRecord");
         ordered = cfg.sort_hits != 0 && n_photons != 0;
@@ -945,7 +958,7 @@ struct Core {
         hip_check(hipMemcpyAsync(h_meta.ptr, d_meta.ptr, kMetaBytes, hipMemcpyDeviceToHost, stream),
                   "hipMemcpyAsync counters");
         if (n_events) {
-            hip_check(hipMemcpyAsync(h_meta.as<char>() + kMetaBytes, d_event_stats.ptr, n_events * sizeof(EventStats),
+            hip_check(hipMemcpyAsync(h_meta.as<char>() + kMetaBytes, d_events_out, n_events * sizeof(EventStats),
                                      hipMemcpyDeviceToHost, stream),
                       "hipMemcpyAsync event stats");
         }

@@ -30,6 +30,14 @@ struct EventStats {
     unsigned long long detected, absorbed, escaped, truncated, max_bounce, invalid, zero_step;
 };
 
+// The transport kernels accumulate per-event counters into kEventReplicas
+// copies of the per-event array (block b writes copy b % kEventReplicas), so
+// that the waves of a batch dominated by a few large events do not all
+// serialise on the atomics of one record; reduce_event_stats sums the copies
+// (the maximum for max_bounce) afterwards.  Copy r of event e is
+// event_stats[r * event_stride + e].
+constexpr uint32_t kEventReplicas = 32;
+
 enum Counter : int {
     kCounterSteps = 0,      // transport steps executed (the reference's totalBounceCount)
     kCounterPhotons = 1,    // photons generated
@@ -60,7 +68,9 @@ struct KernelParams {
     uint32_t* hit_count;
     uint32_t* hit_launch;  // launch index of each hit's photon (source_photon_offset[source] + local index)
     unsigned long long* histogram;
-    EventStats* event_stats;
+    EventStats* event_stats;   // kEventReplicas copies, event_stride records apart
+    uint32_t event_stride;
+    uint32_t pad_es;
     unsigned long long* counters;
     double hist_lo_ns, hist_bins_per_ns;  // bin = floor((t - lo) * bins_per_ns)
     // BVH nodes + daughter boxes staged into LDS by every block (32-byte
@@ -95,6 +105,9 @@ struct AdjointParams {
 
 int launch_transport(const KernelParams& p, int source, bool persistent, int block, int grid, hipStream_t stream);
 int launch_zero(void* ptr, size_t bytes, hipStream_t stream);
+// Sums the kEventReplicas copies of the first header[2] events into out.
+int launch_reduce_event_stats(const EventStats* replicas, uint32_t stride, const uint32_t* header, EventStats* out,
+                              hipStream_t stream);
 int launch_adjoint(const AdjointParams& p, int source, int block, int grid, hipStream_t stream);
 
 // Launch-order output: reorders the hits of a batch into the order of their

@@ -218,6 +218,8 @@ __device__ __forceinline__ void transport_body(const KernelParams& p) {
     const uint32_t first_item = p.item_end != 0u ? p.item_begin : 0u;
     const uint32_t end_item = p.item_end != 0u ? p.item_end : uniform(p.item_offsets[n_sources]);
 
+    EventStats* const es = p.event_stats + static_cast<size_t>(blockIdx.x % kEventReplicas) * p.event_stride;
+
     Photon ph;
     ph.material = kInvalid;
     bool alive = false;
@@ -271,7 +273,7 @@ __device__ __forceinline__ void transport_body(const KernelParams& p) {
                 const uint32_t take = min(it_end - it_next, n_idle);
                 const uint32_t rank = rank_below(idle);
                 const bool fill = !alive && rank < take;
-                flush_event_group(p.event_stats, my_event, st, fill && my_event != it_event && my_event != kInvalid);
+                flush_event_group(es, my_event, st, fill && my_event != it_event && my_event != kInvalid);
                 if (fill) {
                     my_event = it_event;
                     generate<kSource>(ph, p, it_event, it_source, it_first + it_next + rank);
@@ -346,7 +348,7 @@ __device__ __forceinline__ void transport_body(const KernelParams& p) {
     if (staged != 0u) {
         flush_stage(p, stage, stage_src, staged, lane);
     }
-    flush_event_group(p.event_stats, my_event, st, my_event != kInvalid);
+    flush_event_group(es, my_event, st, my_event != kInvalid);
     if (lane == 0u) {
         if (wave_steps) atomicAdd(p.counters + kCounterSteps, wave_steps);
         if (wave_photons) atomicAdd(p.counters + kCounterPhotons, wave_photons);
@@ -379,6 +381,32 @@ __global__ void zero_kernel(uint32_t* ptr, size_t words) {
          i += static_cast<size_t>(gridDim.x) * blockDim.x) {
         ptr[i] = 0u;
     }
+}
+
+__global__ void reduce_event_stats_kernel(const EventStats* rep, uint32_t stride, const uint32_t* header,
+                                          EventStats* out) {
+    const uint32_t n_events = header[2];
+    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < n_events; e += gridDim.x * blockDim.x) {
+        EventStats s{0, 0, 0, 0, 0, 0, 0};
+        for (uint32_t r = 0; r < kEventReplicas; ++r) {
+            const EventStats& c = rep[static_cast<size_t>(r) * stride + e];
+            s.detected += c.detected;
+            s.absorbed += c.absorbed;
+            s.escaped += c.escaped;
+            s.truncated += c.truncated;
+            s.max_bounce = max(s.max_bounce, c.max_bounce);
+            s.invalid += c.invalid;
+            s.zero_step += c.zero_step;
+        }
+        out[e] = s;
+    }
+}
+
+int launch_reduce_event_stats(const EventStats* replicas, uint32_t stride, const uint32_t* header, EventStats* out,
+                              hipStream_t stream) {
+    const int grid = static_cast<int>(std::min<uint32_t>((stride + 255) / 256, 1024));
+    hipLaunchKernelGGL(reduce_event_stats_kernel, dim3(grid), dim3(256), 0, stream, replicas, stride, header, out);
+    return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 
 int launch_zero(void* ptr, size_t bytes, hipStream_t stream) {
