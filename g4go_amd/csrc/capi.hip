@@ -659,6 +659,7 @@ struct Core {
         char* base = d_scene.as<char>();
         view = SceneView{};
         view.pool = reinterpret_cast<const float*>(base + o_pool);
+        view.n_pool = static_cast<uint32_t>(s.pool.size());
         view.materials = reinterpret_cast<const Material*>(base + o_mat);
         view.surfaces = reinterpret_cast<const Surface*>(base + o_surf);
         view.volumes = reinterpret_cast<const Volume*>(base + o_vol);
@@ -719,10 +720,13 @@ struct Core {
         p.refill_threshold = std::max<uint32_t>(1, std::min<uint32_t>(64, launch.refill_threshold));
         p.hist_lo_ns = 0.0;
         p.hist_bins_per_ns = kHistBins / 200.0;
-        // scenes with a daughter BVH: every block stages the nodes and the
-        // daughter boxes in LDS when they fit
+        // every block stages in LDS, when they fit: the BVH nodes and daughter
+        // boxes of a scene with a daughter BVH, then the property pool
         const uint64_t entries = static_cast<uint64_t>(view.n_bvh_nodes) + view.n_daughters;
         p.lds_entries = view.n_bvh_nodes != 0 && entries * 32u <= kLdsSceneBytes ? static_cast<uint32_t>(entries) : 0u;
+        const uint64_t pool_bytes = (static_cast<uint64_t>(view.n_pool) + 3u) / 4u * 16u;
+        p.lds_pool_floats =
+            view.n_pool != 0 && p.lds_entries * 32ull + pool_bytes <= kLdsSceneBytes ? view.n_pool : 0u;
         return p;
     }
 

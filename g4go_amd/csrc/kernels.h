@@ -7,7 +7,10 @@
 
 namespace g4go {
 
-constexpr uint32_t kLdsSceneBytes = 32768;  // LDS budget of the staged BVH + daughter boxes per block
+// Dynamic LDS budget per block for the staged scene (BVH nodes, daughter
+// boxes, property pool).  With the ~21 KB of static LDS, four 256-thread
+// blocks (16 waves, the register-limited occupancy) still fit in 160 KB.
+constexpr uint32_t kLdsSceneBytes = 18432;
 constexpr int kHistBins = 256;
 constexpr int kHistSlots = kHistBins + 2;  // + underflow (256) + overflow (257)
 
@@ -74,10 +77,11 @@ struct KernelParams {
     unsigned long long* counters;
     double hist_lo_ns, hist_bins_per_ns;  // bin = floor((t - lo) * bins_per_ns)
     // BVH nodes + daughter boxes staged into LDS by every block (32-byte
-    // entries; 0 = walk them in global memory).  The launch allocates
-    // lds_entries * 32 bytes of dynamic LDS.
+    // entries; 0 = walk them in global memory), then the first lds_pool_floats
+    // floats of the property pool (0 = read it in global memory).  The launch
+    // allocates lds_entries * 32 + ceil(lds_pool_floats / 4) * 16 bytes.
     uint32_t lds_entries;
-    uint32_t pad_lds;
+    uint32_t lds_pool_floats;
 };
 
 // Parameterised property tables for the adjoint: theta offset of each
@@ -103,6 +107,10 @@ struct AdjointParams {
     unsigned long long* counters;  // [0] replayed, [1] path mismatches, [2] unsupported paths, [3] replay steps
 };
 
+// Dynamic LDS bytes of a transport launch (the staged scene).
+inline size_t transport_lds_bytes(const KernelParams& p) {
+    return static_cast<size_t>(p.lds_entries) * 32u + (static_cast<size_t>(p.lds_pool_floats) + 3u) / 4u * 16u;
+}
 int launch_transport(const KernelParams& p, int source, bool persistent, int block, int grid, hipStream_t stream);
 int launch_zero(void* ptr, size_t bytes, hipStream_t stream);
 // Sums the kEventReplicas copies of the first header[2] events into out.

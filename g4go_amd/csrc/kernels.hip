@@ -128,12 +128,12 @@ __device__ __forceinline__ ItemRef locate_item(const KernelParams& p, uint32_t c
 }
 
 template <int kSource>
-__device__ __forceinline__ void generate(Photon& ph, const KernelParams& p, uint32_t event_index, uint32_t source,
-                                         uint32_t local) {
+__device__ __forceinline__ void generate(Photon& ph, const KernelParams& p, const SceneView& sv, uint32_t event_index,
+                                         uint32_t source, uint32_t local) {
     if (kSource == kSourceSynthetic) {
         generate_synthetic(ph, p.synth, p.rng, p.event_ids[event_index], local);
     } else {
-        generate_emission(ph, p.emissions[source], p.scene, p.rng, local);
+        generate_emission(ph, p.emissions[source], sv, p.rng, local);
     }
 }
 
@@ -191,7 +191,7 @@ __device__ __forceinline__ void transport_body(const KernelParams& p) {
     __shared__ uint32_t s_hist[kHistSlots];
     __shared__ float4 s_stage[kMaxWavesPerBlock][3 * kStageCap];
     __shared__ uint32_t s_stage_src[kMaxWavesPerBlock][kStageCap];
-    extern __shared__ float4 s_scene[];  // p.lds_entries x 32 B: BVH nodes, then daughter boxes
+    extern __shared__ float4 s_scene[];  // BVH nodes, daughter boxes, property pool (transport_lds_bytes)
     for (int i = threadIdx.x; i < kHistSlots; i += blockDim.x) {
         s_hist[i] = 0u;
     }
@@ -206,6 +206,17 @@ __device__ __forceinline__ void transport_body(const KernelParams& p) {
         }
         sv.bvh = reinterpret_cast<const BvhNode*>(s_scene);
         sv.dbox = reinterpret_cast<const DaughterBox*>(s_scene + node_words);
+    }
+    if (p.lds_pool_floats != 0u) {
+        // the pool region of the scene upload is padded to 256 bytes, so whole
+        // float4 reads stay inside it
+        float4* const dst = s_scene + 2u * p.lds_entries;
+        const float4* src = reinterpret_cast<const float4*>(p.scene.pool);
+        const uint32_t quads = (p.lds_pool_floats + 3u) / 4u;
+        for (uint32_t i = threadIdx.x; i < quads; i += blockDim.x) {
+            dst[i] = src[i];
+        }
+        sv.pool = reinterpret_cast<const float*>(dst);
     }
     __syncthreads();
     float4* const stage = s_stage[threadIdx.x >> 6];
@@ -276,7 +287,7 @@ __device__ __forceinline__ void transport_body(const KernelParams& p) {
                 flush_event_group(es, my_event, st, fill && my_event != it_event && my_event != kInvalid);
                 if (fill) {
                     my_event = it_event;
-                    generate<kSource>(ph, p, it_event, it_source, it_first + it_next + rank);
+                    generate<kSource>(ph, p, sv, it_event, it_source, it_first + it_next + rank);
                     ph.material = kInvalid;
                     my_launch = p.source_photon_offset[it_source] + it_first + it_next + rank;
                     if (start_photon(ph)) {
@@ -420,7 +431,7 @@ int launch_zero(void* ptr, size_t bytes, hipStream_t stream) {
 }
 
 int launch_transport(const KernelParams& p, int source, bool persistent, int block, int grid, hipStream_t stream) {
-    const size_t lds = static_cast<size_t>(p.lds_entries) * 32u;
+    const size_t lds = transport_lds_bytes(p);
     if (lds > kLdsSceneBytes) {
         return 1;
     }

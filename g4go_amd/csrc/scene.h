@@ -110,7 +110,8 @@ struct SceneView {
     // pool[hot_offset], hot_count nodes; hot_count = 0: none).  A photon's
     // energy never changes, so its node on this grid is located once.
     uint32_t hot_offset, hot_count;
-    uint32_t pad[3];
+    uint32_t n_pool;  // floats in the pool
+    uint32_t pad[2];
 };
 
 }  // namespace g4go
