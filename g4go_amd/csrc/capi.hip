@@ -724,9 +724,23 @@ struct Core {
         // boxes of a scene with a daughter BVH, then the property pool
         const uint64_t entries = static_cast<uint64_t>(view.n_bvh_nodes) + view.n_daughters;
         p.lds_entries = view.n_bvh_nodes != 0 && entries * 32u <= kLdsSceneBytes ? static_cast<uint32_t>(entries) : 0u;
+        uint64_t used = p.lds_entries * 32ull;
         const uint64_t pool_bytes = (static_cast<uint64_t>(view.n_pool) + 3u) / 4u * 16u;
-        p.lds_pool_floats =
-            view.n_pool != 0 && p.lds_entries * 32ull + pool_bytes <= kLdsSceneBytes ? view.n_pool : 0u;
+        if (view.n_pool != 0 && used + pool_bytes <= kLdsSceneBytes) {
+            p.lds_pool_floats = view.n_pool;
+            used += pool_bytes;
+        }
+        // then the records a step reads: volumes, surfaces, materials
+        const std::pair<uint32_t, uint64_t> records[] = {
+            {kLdsVolumes, static_cast<uint64_t>(view.n_volumes) * sizeof(Volume)},
+            {kLdsSurfaces, static_cast<uint64_t>(view.n_surfaces) * sizeof(Surface)},
+            {kLdsMaterials, static_cast<uint64_t>(view.n_materials) * sizeof(Material)}};
+        for (const auto& r : records) {
+            if (r.second != 0 && used + r.second <= kLdsSceneBytes) {
+                p.lds_records |= r.first;
+                used += r.second;
+            }
+        }
         return p;
     }
 

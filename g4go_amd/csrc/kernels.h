@@ -76,12 +76,16 @@ struct KernelParams {
     uint32_t pad_es;
     unsigned long long* counters;
     double hist_lo_ns, hist_bins_per_ns;  // bin = floor((t - lo) * bins_per_ns)
-    // BVH nodes + daughter boxes staged into LDS by every block (32-byte
-    // entries; 0 = walk them in global memory), then the first lds_pool_floats
-    // floats of the property pool (0 = read it in global memory).  The launch
-    // allocates lds_entries * 32 + ceil(lds_pool_floats / 4) * 16 bytes.
+    // Scene parts staged into LDS by every block, in this order (each only
+    // when it fits; otherwise read in global memory): BVH nodes + daughter
+    // boxes (lds_entries 32-byte entries), the property pool (lds_pool_floats
+    // floats, padded to 16 bytes), then the volume, surface and material
+    // records selected by lds_records (kLdsVolumes | kLdsSurfaces |
+    // kLdsMaterials).  transport_lds_bytes gives the dynamic LDS size.
     uint32_t lds_entries;
     uint32_t lds_pool_floats;
+    uint32_t lds_records;
+    uint32_t pad_lds;
 };
 
 // Parameterised property tables for the adjoint: theta offset of each
@@ -107,9 +111,15 @@ struct AdjointParams {
     unsigned long long* counters;  // [0] replayed, [1] path mismatches, [2] unsupported paths, [3] replay steps
 };
 
+constexpr uint32_t kLdsVolumes = 1u, kLdsSurfaces = 2u, kLdsMaterials = 4u;
+
 // Dynamic LDS bytes of a transport launch (the staged scene).
 inline size_t transport_lds_bytes(const KernelParams& p) {
-    return static_cast<size_t>(p.lds_entries) * 32u + (static_cast<size_t>(p.lds_pool_floats) + 3u) / 4u * 16u;
+    size_t bytes = static_cast<size_t>(p.lds_entries) * 32u + (static_cast<size_t>(p.lds_pool_floats) + 3u) / 4u * 16u;
+    if (p.lds_records & kLdsVolumes) bytes += static_cast<size_t>(p.scene.n_volumes) * sizeof(Volume);
+    if (p.lds_records & kLdsSurfaces) bytes += static_cast<size_t>(p.scene.n_surfaces) * sizeof(Surface);
+    if (p.lds_records & kLdsMaterials) bytes += static_cast<size_t>(p.scene.n_materials) * sizeof(Material);
+    return bytes;
 }
 int launch_transport(const KernelParams& p, int source, bool persistent, int block, int grid, hipStream_t stream);
 int launch_zero(void* ptr, size_t bytes, hipStream_t stream);

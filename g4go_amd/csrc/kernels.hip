@@ -207,16 +207,32 @@ __device__ __forceinline__ void transport_body(const KernelParams& p) {
         sv.bvh = reinterpret_cast<const BvhNode*>(s_scene);
         sv.dbox = reinterpret_cast<const DaughterBox*>(s_scene + node_words);
     }
+    float4* next_quad = s_scene + 2u * p.lds_entries;
+    // copies `quads` 16-byte words from global memory to the next LDS slot
+    const auto stage_quads = [&](const void* src, uint32_t quads) {
+        float4* const dst = next_quad;
+        const float4* s = reinterpret_cast<const float4*>(src);
+        for (uint32_t i = threadIdx.x; i < quads; i += blockDim.x) {
+            dst[i] = s[i];
+        }
+        next_quad += quads;
+        return static_cast<const void*>(dst);
+    };
     if (p.lds_pool_floats != 0u) {
         // the pool region of the scene upload is padded to 256 bytes, so whole
         // float4 reads stay inside it
-        float4* const dst = s_scene + 2u * p.lds_entries;
-        const float4* src = reinterpret_cast<const float4*>(p.scene.pool);
-        const uint32_t quads = (p.lds_pool_floats + 3u) / 4u;
-        for (uint32_t i = threadIdx.x; i < quads; i += blockDim.x) {
-            dst[i] = src[i];
-        }
-        sv.pool = reinterpret_cast<const float*>(dst);
+        sv.pool = static_cast<const float*>(stage_quads(p.scene.pool, (p.lds_pool_floats + 3u) / 4u));
+    }
+    if (p.lds_records & kLdsVolumes) {
+        sv.volumes = static_cast<const Volume*>(stage_quads(p.scene.volumes, p.scene.n_volumes * (sizeof(Volume) / 16u)));
+    }
+    if (p.lds_records & kLdsSurfaces) {
+        sv.surfaces =
+            static_cast<const Surface*>(stage_quads(p.scene.surfaces, p.scene.n_surfaces * (sizeof(Surface) / 16u)));
+    }
+    if (p.lds_records & kLdsMaterials) {
+        sv.materials = static_cast<const Material*>(
+            stage_quads(p.scene.materials, p.scene.n_materials * (sizeof(Material) / 16u)));
     }
     __syncthreads();
     float4* const stage = s_stage[threadIdx.x >> 6];

@@ -70,6 +70,10 @@ struct alignas(16) Volume {
     Face faces[6];
 };
 
+// Records staged into LDS are copied in 16-byte words.
+static_assert(sizeof(Material) % 16 == 0 && sizeof(Surface) % 16 == 0 && sizeof(Volume) % 16 == 0,
+              "scene records are whole 16-byte words");
+
 struct Binding {
     uint32_t from, to, surface, pad;  // sorted by (from, to)
 };
