@@ -720,27 +720,25 @@ struct Core {
         p.refill_threshold = std::max<uint32_t>(1, std::min<uint32_t>(64, launch.refill_threshold));
         p.hist_lo_ns = 0.0;
         p.hist_bins_per_ns = kHistBins / 200.0;
-        // every block stages in LDS, when they fit: the BVH nodes and daughter
-        // boxes of a scene with a daughter BVH, then the property pool
-        const uint64_t entries = static_cast<uint64_t>(view.n_bvh_nodes) + view.n_daughters;
-        p.lds_entries = view.n_bvh_nodes != 0 && entries * 32u <= kLdsSceneBytes ? static_cast<uint32_t>(entries) : 0u;
-        uint64_t used = p.lds_entries * 32ull;
-        const uint64_t pool_bytes = (static_cast<uint64_t>(view.n_pool) + 3u) / 4u * 16u;
-        if (view.n_pool != 0 && used + pool_bytes <= kLdsSceneBytes) {
-            p.lds_pool_floats = view.n_pool;
-            used += pool_bytes;
-        }
-        // then the records a step reads: volumes, surfaces, materials
-        const std::pair<uint32_t, uint64_t> records[] = {
-            {kLdsVolumes, static_cast<uint64_t>(view.n_volumes) * sizeof(Volume)},
-            {kLdsSurfaces, static_cast<uint64_t>(view.n_surfaces) * sizeof(Surface)},
-            {kLdsMaterials, static_cast<uint64_t>(view.n_materials) * sizeof(Material)}};
-        for (const auto& r : records) {
-            if (r.second != 0 && used + r.second <= kLdsSceneBytes) {
-                p.lds_records |= r.first;
-                used += r.second;
+        // Every block stages scene parts in LDS, each when it still fits, in
+        // the order a step needs them: the volume and surface records (read
+        // every step), the BVH nodes and daughter boxes (read by photons in a
+        // volume with a daughter BVH), the property pool (read mostly when a
+        // photon starts), the material records (read when it changes volume).
+        uint64_t used = 0;
+        const auto fits = [&](uint64_t bytes) {
+            if (bytes == 0 || used + bytes > kLdsSceneBytes) {
+                return false;
             }
-        }
+            used += bytes;
+            return true;
+        };
+        if (fits(static_cast<uint64_t>(view.n_volumes) * sizeof(Volume))) p.lds_records |= kLdsVolumes;
+        if (fits(static_cast<uint64_t>(view.n_surfaces) * sizeof(Surface))) p.lds_records |= kLdsSurfaces;
+        const uint64_t entries = static_cast<uint64_t>(view.n_bvh_nodes) + view.n_daughters;
+        if (view.n_bvh_nodes != 0 && fits(entries * 32u)) p.lds_entries = static_cast<uint32_t>(entries);
+        if (fits((static_cast<uint64_t>(view.n_pool) + 3u) / 4u * 16u)) p.lds_pool_floats = view.n_pool;
+        if (fits(static_cast<uint64_t>(view.n_materials) * sizeof(Material))) p.lds_records |= kLdsMaterials;
         return p;
     }
 

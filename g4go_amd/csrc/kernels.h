@@ -8,9 +8,10 @@
 namespace g4go {
 
 // Dynamic LDS budget per block for the staged scene (BVH nodes, daughter
-// boxes, property pool).  With the ~21 KB of static LDS, four 256-thread
-// blocks (16 waves, the register-limited occupancy) still fit in 160 KB.
-constexpr uint32_t kLdsSceneBytes = 18432;
+// boxes, property pool, scene records).  With the ~14 KB of static LDS, four
+// 256-thread blocks (16 waves, the register-limited occupancy) still fit in
+// 160 KB.
+constexpr uint32_t kLdsSceneBytes = 26112;
 constexpr int kHistBins = 256;
 constexpr int kHistSlots = kHistBins + 2;  // + underflow (256) + overflow (257)
 

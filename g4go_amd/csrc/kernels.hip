@@ -140,8 +140,8 @@ __device__ __forceinline__ void generate(Photon& ph, const KernelParams& p, cons
 // Detected photons are staged per wave in LDS and moved to the global hit
 // list in groups: one atomic reservation per group instead of one per wave
 // step (a single global counter serialises at the L2 otherwise).
-constexpr uint32_t kStageCap = 96;    // staged records per wave
-constexpr uint32_t kStageFlush = 32;  // flush once this many are staged (<= 31 + 64 < kStageCap)
+constexpr uint32_t kStageCap = 64;    // staged records per wave (a step adds at most 64)
+constexpr uint32_t kStageFlush = 32;  // flush once this many are staged
 constexpr int kMaxWavesPerBlock = 4;
 
 __device__ __forceinline__ void flush_stage(const KernelParams& p, const float4* stage, const uint32_t* src,
@@ -343,13 +343,18 @@ __device__ __forceinline__ void transport_body(const KernelParams& p) {
         // ---- fused hit compaction: ballot + prefix into the wave's LDS stage ----
         const uint64_t det = __ballot(status == kDetected);
         if (det != 0ull) {
+            const uint32_t n_det = static_cast<uint32_t>(__popcll(det));
+            if (staged + n_det > kStageCap) {  // rare: more than 64 - 31 detections in one step
+                flush_stage(p, stage, stage_src, staged, lane);
+                staged = 0;
+            }
             if (status == kDetected) {
                 const uint32_t slot = staged + rank_below(det);
                 store_hit(stage + 3u * slot, ph, sensor);
                 stage_src[slot] = my_launch;
                 atomicAdd(&s_hist[hist_slot(ph.time, p.hist_lo_ns, p.hist_bins_per_ns)], 1u);
             }
-            staged += static_cast<uint32_t>(__popcll(det));
+            staged += n_det;
             if (staged >= kStageFlush) {
                 flush_stage(p, stage, stage_src, staged, lane);
                 staged = 0;
