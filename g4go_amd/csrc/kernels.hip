@@ -186,7 +186,10 @@ __device__ __forceinline__ void store_hit(float4* d, const Photon& ph, uint32_t 
 
 }  // namespace
 
-template <int kSource, bool kPersistent>
+// kHotLds: the volume and surface records are staged in LDS (the launch
+// checks lds_records).  Their pointers then come from s_scene on every path,
+// so the compiler reads them with LDS instructions instead of flat ones.
+template <int kSource, bool kPersistent, bool kHotLds>
 __device__ __forceinline__ void transport_body(const KernelParams& p) {
     __shared__ uint32_t s_hist[kHistSlots];
     __shared__ float4 s_stage[kMaxWavesPerBlock][3 * kStageCap];
@@ -223,10 +226,10 @@ __device__ __forceinline__ void transport_body(const KernelParams& p) {
         // float4 reads stay inside it
         sv.pool = static_cast<const float*>(stage_quads(p.scene.pool, (p.lds_pool_floats + 3u) / 4u));
     }
-    if (p.lds_records & kLdsVolumes) {
+    if (kHotLds || (p.lds_records & kLdsVolumes)) {
         sv.volumes = static_cast<const Volume*>(stage_quads(p.scene.volumes, p.scene.n_volumes * (sizeof(Volume) / 16u)));
     }
-    if (p.lds_records & kLdsSurfaces) {
+    if (kHotLds || (p.lds_records & kLdsSurfaces)) {
         sv.surfaces =
             static_cast<const Surface*>(stage_quads(p.scene.surfaces, p.scene.n_surfaces * (sizeof(Surface) / 16u)));
     }
@@ -397,15 +400,24 @@ __device__ __forceinline__ void transport_body(const KernelParams& p) {
 // the compiler schedules the body into ~140 registers (three waves); the
 // fourth wave hides more of the dependent scene-load latency of a step than
 // the few spills it forces cost.
-template <int kSource>
+template <int kSource, bool kHotLds>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
 transport_kernel_persistent(KernelParams p) {
-    transport_body<kSource, true>(p);
+    transport_body<kSource, true, kHotLds>(p);
 }
 
-template <int kSource>
+template <int kSource, bool kHotLds>
 __global__ void __launch_bounds__(256) transport_kernel_plain(KernelParams p) {
-    transport_body<kSource, false>(p);
+    transport_body<kSource, false, kHotLds>(p);
+}
+
+template <int kSource, bool kHotLds>
+void launch_transport_form(const KernelParams& p, bool persistent, int block, int grid, size_t lds, hipStream_t stream) {
+    if (persistent) {
+        hipLaunchKernelGGL((transport_kernel_persistent<kSource, kHotLds>), dim3(grid), dim3(block), lds, stream, p);
+    } else {
+        hipLaunchKernelGGL((transport_kernel_plain<kSource, kHotLds>), dim3(grid), dim3(block), lds, stream, p);
+    }
 }
 
 __global__ void zero_kernel(uint32_t* ptr, size_t words) {
@@ -456,17 +468,18 @@ int launch_transport(const KernelParams& p, int source, bool persistent, int blo
     if (lds > kLdsSceneBytes) {
         return 1;
     }
+    const bool hot = (p.lds_records & (kLdsVolumes | kLdsSurfaces)) == (kLdsVolumes | kLdsSurfaces);
     if (source == kSourceSynthetic) {
-        if (persistent) {
-            hipLaunchKernelGGL(transport_kernel_persistent<kSourceSynthetic>, dim3(grid), dim3(block), lds, stream, p);
+        if (hot) {
+            launch_transport_form<kSourceSynthetic, true>(p, persistent, block, grid, lds, stream);
         } else {
-            hipLaunchKernelGGL(transport_kernel_plain<kSourceSynthetic>, dim3(grid), dim3(block), lds, stream, p);
+            launch_transport_form<kSourceSynthetic, false>(p, persistent, block, grid, lds, stream);
         }
     } else {
-        if (persistent) {
-            hipLaunchKernelGGL(transport_kernel_persistent<kSourceEmission>, dim3(grid), dim3(block), lds, stream, p);
+        if (hot) {
+            launch_transport_form<kSourceEmission, true>(p, persistent, block, grid, lds, stream);
         } else {
-            hipLaunchKernelGGL(transport_kernel_plain<kSourceEmission>, dim3(grid), dim3(block), lds, stream, p);
+            launch_transport_form<kSourceEmission, false>(p, persistent, block, grid, lds, stream);
         }
     }
     return hipGetLastError() == hipSuccess ? 0 : 1;
