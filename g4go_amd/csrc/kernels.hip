@@ -186,10 +186,14 @@ __device__ __forceinline__ void store_hit(float4* d, const Photon& ph, uint32_t 
 
 }  // namespace
 
-// kHotLds: the volume and surface records are staged in LDS (the launch
-// checks lds_records).  Their pointers then come from s_scene on every path,
-// so the compiler reads them with LDS instructions instead of flat ones.
-template <int kSource, bool kPersistent, bool kHotLds>
+// kLdsMode: which scene parts the launch guarantees are staged in LDS (it
+// checks lds_records and lds_pool_floats): 0 none for certain, 1 the volume
+// and surface records, 2 those and the whole property pool and the material
+// records.  Their pointers then come from s_scene on every path, so the
+// compiler reads them with LDS instructions instead of flat ones.
+enum LdsMode : int { kLdsNone = 0, kLdsHot = 1, kLdsAll = 2 };
+
+template <int kSource, bool kPersistent, int kLdsMode>
 __device__ __forceinline__ void transport_body(const KernelParams& p) {
     __shared__ uint32_t s_hist[kHistSlots];
     __shared__ float4 s_stage[kMaxWavesPerBlock][3 * kStageCap];
@@ -221,19 +225,19 @@ __device__ __forceinline__ void transport_body(const KernelParams& p) {
         next_quad += quads;
         return static_cast<const void*>(dst);
     };
-    if (p.lds_pool_floats != 0u) {
+    if (kLdsMode >= kLdsAll || p.lds_pool_floats != 0u) {
         // the pool region of the scene upload is padded to 256 bytes, so whole
         // float4 reads stay inside it
         sv.pool = static_cast<const float*>(stage_quads(p.scene.pool, (p.lds_pool_floats + 3u) / 4u));
     }
-    if (kHotLds || (p.lds_records & kLdsVolumes)) {
+    if (kLdsMode >= kLdsHot || (p.lds_records & kLdsVolumes)) {
         sv.volumes = static_cast<const Volume*>(stage_quads(p.scene.volumes, p.scene.n_volumes * (sizeof(Volume) / 16u)));
     }
-    if (kHotLds || (p.lds_records & kLdsSurfaces)) {
+    if (kLdsMode >= kLdsHot || (p.lds_records & kLdsSurfaces)) {
         sv.surfaces =
             static_cast<const Surface*>(stage_quads(p.scene.surfaces, p.scene.n_surfaces * (sizeof(Surface) / 16u)));
     }
-    if (p.lds_records & kLdsMaterials) {
+    if (kLdsMode >= kLdsAll || (p.lds_records & kLdsMaterials)) {
         sv.materials = static_cast<const Material*>(
             stage_quads(p.scene.materials, p.scene.n_materials * (sizeof(Material) / 16u)));
     }
@@ -400,23 +404,37 @@ __device__ __forceinline__ void transport_body(const KernelParams& p) {
 // the compiler schedules the body into ~140 registers (three waves); the
 // fourth wave hides more of the dependent scene-load latency of a step than
 // the few spills it forces cost.
-template <int kSource, bool kHotLds>
+template <int kSource, int kLdsMode>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
 transport_kernel_persistent(KernelParams p) {
-    transport_body<kSource, true, kHotLds>(p);
+    transport_body<kSource, true, kLdsMode>(p);
 }
 
-template <int kSource, bool kHotLds>
+template <int kSource, int kLdsMode>
 __global__ void __launch_bounds__(256) transport_kernel_plain(KernelParams p) {
-    transport_body<kSource, false, kHotLds>(p);
+    transport_body<kSource, false, kLdsMode>(p);
 }
 
-template <int kSource, bool kHotLds>
+template <int kSource, int kLdsMode>
 void launch_transport_form(const KernelParams& p, bool persistent, int block, int grid, size_t lds, hipStream_t stream) {
     if (persistent) {
-        hipLaunchKernelGGL((transport_kernel_persistent<kSource, kHotLds>), dim3(grid), dim3(block), lds, stream, p);
+        hipLaunchKernelGGL((transport_kernel_persistent<kSource, kLdsMode>), dim3(grid), dim3(block), lds, stream, p);
     } else {
-        hipLaunchKernelGGL((transport_kernel_plain<kSource, kHotLds>), dim3(grid), dim3(block), lds, stream, p);
+        hipLaunchKernelGGL((transport_kernel_plain<kSource, kLdsMode>), dim3(grid), dim3(block), lds, stream, p);
+    }
+}
+
+template <int kSource>
+void launch_transport_source(const KernelParams& p, bool persistent, int block, int grid, size_t lds,
+                             hipStream_t stream) {
+    const uint32_t hot = kLdsVolumes | kLdsSurfaces;
+    const uint32_t all = hot | kLdsMaterials;
+    if ((p.lds_records & all) == all && p.scene.n_pool != 0 && p.lds_pool_floats == p.scene.n_pool) {
+        launch_transport_form<kSource, kLdsAll>(p, persistent, block, grid, lds, stream);
+    } else if ((p.lds_records & hot) == hot) {
+        launch_transport_form<kSource, kLdsHot>(p, persistent, block, grid, lds, stream);
+    } else {
+        launch_transport_form<kSource, kLdsNone>(p, persistent, block, grid, lds, stream);
     }
 }
 
@@ -468,19 +486,10 @@ int launch_transport(const KernelParams& p, int source, bool persistent, int blo
     if (lds > kLdsSceneBytes) {
         return 1;
     }
-    const bool hot = (p.lds_records & (kLdsVolumes | kLdsSurfaces)) == (kLdsVolumes | kLdsSurfaces);
     if (source == kSourceSynthetic) {
-        if (hot) {
-            launch_transport_form<kSourceSynthetic, true>(p, persistent, block, grid, lds, stream);
-        } else {
-            launch_transport_form<kSourceSynthetic, false>(p, persistent, block, grid, lds, stream);
-        }
+        launch_transport_source<kSourceSynthetic>(p, persistent, block, grid, lds, stream);
     } else {
-        if (hot) {
-            launch_transport_form<kSourceEmission, true>(p, persistent, block, grid, lds, stream);
-        } else {
-            launch_transport_form<kSourceEmission, false>(p, persistent, block, grid, lds, stream);
-        }
+        launch_transport_source<kSourceEmission>(p, persistent, block, grid, lds, stream);
     }
     return hipGetLastError() == hipSuccess ? 0 : 1;
 }
