@@ -186,35 +186,40 @@ __device__ __forceinline__ void store_hit(float4* d, const Photon& ph, uint32_t 
 
 }  // namespace
 
-// kLdsMode: which scene parts the launch guarantees are staged in LDS (it
-// checks lds_records and lds_pool_floats): 0 none for certain, 1 the volume
-// and surface records, 2 those and the whole property pool and the material
-// records.  Their pointers then come from s_scene on every path, so the
-// compiler reads them with LDS instructions instead of flat ones.
+// kLdsMode: which scene parts a launch guarantees are staged in LDS
+// (lds_mode): 0 none for certain, 1 the volume and surface records, 2 those
+// and the whole property pool and the material records.  Their pointers then
+// come from the LDS array on every path, so the compiler reads them with LDS
+// instructions instead of flat ones.
 enum LdsMode : int { kLdsNone = 0, kLdsHot = 1, kLdsAll = 2 };
 
-template <int kSource, bool kPersistent, int kLdsMode>
-__device__ __forceinline__ void transport_body(const KernelParams& p) {
-    __shared__ uint32_t s_hist[kHistSlots];
-    __shared__ float4 s_stage[kMaxWavesPerBlock][3 * kStageCap];
-    __shared__ uint32_t s_stage_src[kMaxWavesPerBlock][kStageCap];
-    extern __shared__ float4 s_scene[];  // BVH nodes, daughter boxes, property pool (transport_lds_bytes)
-    for (int i = threadIdx.x; i < kHistSlots; i += blockDim.x) {
-        s_hist[i] = 0u;
+inline int lds_mode(const KernelParams& p) {
+    const uint32_t hot = kLdsVolumes | kLdsSurfaces;
+    const uint32_t all = hot | kLdsMaterials;
+    if ((p.lds_records & all) == all && p.scene.n_pool != 0 && p.lds_pool_floats == p.scene.n_pool) {
+        return kLdsAll;
     }
-    SceneView sv = p.scene;
+    return (p.lds_records & hot) == hot ? kLdsHot : kLdsNone;
+}
+
+// Copies the scene parts that p selects (lds_entries, lds_pool_floats,
+// lds_records; the kLdsMode parts unconditionally) into `lds`, in the order
+// BVH nodes + daughter boxes, pool, volumes, surfaces, materials, and points sv
+// at the copies.  The caller synchronises the block before using them.
+template <int kLdsMode>
+__device__ __forceinline__ void stage_scene(const KernelParams& p, float4* lds, SceneView& sv) {
     if (p.lds_entries != 0u) {
         const uint32_t node_words = 2u * p.scene.n_bvh_nodes;
         const uint32_t words = 2u * p.lds_entries;
         const float4* nodes = reinterpret_cast<const float4*>(p.scene.bvh);
         const float4* boxes = reinterpret_cast<const float4*>(p.scene.dbox);
         for (uint32_t i = threadIdx.x; i < words; i += blockDim.x) {
-            s_scene[i] = i < node_words ? nodes[i] : boxes[i - node_words];
+            lds[i] = i < node_words ? nodes[i] : boxes[i - node_words];
         }
-        sv.bvh = reinterpret_cast<const BvhNode*>(s_scene);
-        sv.dbox = reinterpret_cast<const DaughterBox*>(s_scene + node_words);
+        sv.bvh = reinterpret_cast<const BvhNode*>(lds);
+        sv.dbox = reinterpret_cast<const DaughterBox*>(lds + node_words);
     }
-    float4* next_quad = s_scene + 2u * p.lds_entries;
+    float4* next_quad = lds + 2u * p.lds_entries;
     // copies `quads` 16-byte words from global memory to the next LDS slot
     const auto stage_quads = [&](const void* src, uint32_t quads) {
         float4* const dst = next_quad;
@@ -241,6 +246,19 @@ __device__ __forceinline__ void transport_body(const KernelParams& p) {
         sv.materials = static_cast<const Material*>(
             stage_quads(p.scene.materials, p.scene.n_materials * (sizeof(Material) / 16u)));
     }
+}
+
+template <int kSource, bool kPersistent, int kLdsMode>
+__device__ __forceinline__ void transport_body(const KernelParams& p) {
+    __shared__ uint32_t s_hist[kHistSlots];
+    __shared__ float4 s_stage[kMaxWavesPerBlock][3 * kStageCap];
+    __shared__ uint32_t s_stage_src[kMaxWavesPerBlock][kStageCap];
+    extern __shared__ float4 s_scene[];  // BVH nodes, daughter boxes, property pool (transport_lds_bytes)
+    for (int i = threadIdx.x; i < kHistSlots; i += blockDim.x) {
+        s_hist[i] = 0u;
+    }
+    SceneView sv = p.scene;
+    stage_scene<kLdsMode>(p, s_scene, sv);
     __syncthreads();
     float4* const stage = s_stage[threadIdx.x >> 6];
     uint32_t* const stage_src = s_stage_src[threadIdx.x >> 6];
@@ -427,14 +445,16 @@ void launch_transport_form(const KernelParams& p, bool persistent, int block, in
 template <int kSource>
 void launch_transport_source(const KernelParams& p, bool persistent, int block, int grid, size_t lds,
                              hipStream_t stream) {
-    const uint32_t hot = kLdsVolumes | kLdsSurfaces;
-    const uint32_t all = hot | kLdsMaterials;
-    if ((p.lds_records & all) == all && p.scene.n_pool != 0 && p.lds_pool_floats == p.scene.n_pool) {
-        launch_transport_form<kSource, kLdsAll>(p, persistent, block, grid, lds, stream);
-    } else if ((p.lds_records & hot) == hot) {
-        launch_transport_form<kSource, kLdsHot>(p, persistent, block, grid, lds, stream);
-    } else {
-        launch_transport_form<kSource, kLdsNone>(p, persistent, block, grid, lds, stream);
+    switch (lds_mode(p)) {
+        case kLdsAll:
+            launch_transport_form<kSource, kLdsAll>(p, persistent, block, grid, lds, stream);
+            break;
+        case kLdsHot:
+            launch_transport_form<kSource, kLdsHot>(p, persistent, block, grid, lds, stream);
+            break;
+        default:
+            launch_transport_form<kSource, kLdsNone>(p, persistent, block, grid, lds, stream);
+            break;
     }
 }
 
@@ -512,6 +532,12 @@ int launch_transport(const KernelParams& p, int source, bool persistent, int blo
 // per step.  A path may visit up to two materials and two parameterised
 // reflecting surfaces; longer ones are counted as unsupported.
 constexpr uint32_t kAdjointLdsParams = 4096;
+
+// 16-byte words of LDS that the adjoint's gradient accumulator takes (0: it
+// accumulates in global memory); the staged scene follows it.
+__host__ __device__ inline uint32_t adjoint_grad_quads(uint32_t n_params) {
+    return n_params <= kAdjointLdsParams ? (n_params + 1u) / 2u : 0u;
+}
 
 namespace {
 
@@ -607,9 +633,9 @@ __device__ __forceinline__ void add_material_grads(double* grad, const SceneView
 
 // Four waves per SIMD (at most 128 VGPRs), as for the persistent transport
 // kernel: the replay runs the same step code.
-template <int kSource>
+template <int kSource, int kLdsMode>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) adjoint_kernel(AdjointParams a) {
-    extern __shared__ double s_grad[];
+    extern __shared__ double s_grad[];  // gradient accumulator, then the staged scene
     const uint32_t n_params = a.map.n_params;
     const bool in_lds = n_params <= kAdjointLdsParams;
     double* grad = in_lds ? s_grad : a.grad;
@@ -617,10 +643,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) a
         for (uint32_t i = threadIdx.x; i < n_params; i += blockDim.x) {
             s_grad[i] = 0.0;
         }
-        __syncthreads();
     }
     const KernelParams& p = a.fwd;
-    const SceneView& sc = p.scene;
+    SceneView sc = p.scene;
+    stage_scene<kLdsMode>(p, reinterpret_cast<float4*>(s_grad) + adjoint_grad_quads(n_params), sc);
+    __syncthreads();
     const uint32_t n_sources = p.header != nullptr ? p.header[0] : 0u;
     uint32_t replayed = 0, mismatch = 0, unsupported = 0, steps = 0;
     for (uint32_t h = blockIdx.x * blockDim.x + threadIdx.x; h < a.n_hits; h += gridDim.x * blockDim.x) {
@@ -726,12 +753,28 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) a
     }
 }
 
+template <int kSource>
+void launch_adjoint_source(const AdjointParams& p, int block, int grid, size_t lds, hipStream_t stream) {
+    switch (lds_mode(p.fwd)) {
+        case kLdsAll:
+            hipLaunchKernelGGL((adjoint_kernel<kSource, kLdsAll>), dim3(grid), dim3(block), lds, stream, p);
+            break;
+        case kLdsHot:
+            hipLaunchKernelGGL((adjoint_kernel<kSource, kLdsHot>), dim3(grid), dim3(block), lds, stream, p);
+            break;
+        default:
+            hipLaunchKernelGGL((adjoint_kernel<kSource, kLdsNone>), dim3(grid), dim3(block), lds, stream, p);
+            break;
+    }
+}
+
 int launch_adjoint(const AdjointParams& p, int source, int block, int grid, hipStream_t stream) {
-    const size_t lds = p.map.n_params <= kAdjointLdsParams ? p.map.n_params * sizeof(double) : 0;
+    // the gradient accumulator (when it fits), then the scene the forward launch staged
+    const size_t lds = static_cast<size_t>(adjoint_grad_quads(p.map.n_params)) * 16u + transport_lds_bytes(p.fwd);
     if (source == kSourceSynthetic) {
-        hipLaunchKernelGGL((adjoint_kernel<kSourceSynthetic>), dim3(grid), dim3(block), lds, stream, p);
+        launch_adjoint_source<kSourceSynthetic>(p, block, grid, lds, stream);
     } else {
-        hipLaunchKernelGGL((adjoint_kernel<kSourceEmission>), dim3(grid), dim3(block), lds, stream, p);
+        launch_adjoint_source<kSourceEmission>(p, block, grid, lds, stream);
     }
     return hipGetLastError() == hipSuccess ? 0 : 1;
 }
