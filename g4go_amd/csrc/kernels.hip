@@ -9,10 +9,12 @@
 //     queue and refill idle lanes in place once at least `refill_threshold`
 //     of the 64 lanes are idle, so long photon histories do not hold a whole
 //     wave.
-// Detections are compacted in place with one ballot, one popcount and one
-// global atomic per wave step (replacing the flag array and
-// cub::DeviceSelect::Flagged pass of OptiXHitCompaction.cu:8-30), and binned
-// into an LDS arrival-time histogram flushed once per block.
+// Detections are compacted in place: a ballot and a lane rank place each
+// wave's hits in an LDS stage, which is written out with one global atomic per
+// group of at least 32 hits (replacing the flag array and
+// cub::DeviceSelect::Flagged pass of OptiXHitCompaction.cu:8-30); they are
+// binned into an LDS arrival-time histogram flushed once per block.  Every
+// block first copies the scene into LDS as far as it fits (stage_scene).
 #include "kernels.h"
 
 namespace g4go {
