@@ -248,7 +248,8 @@ g4go_status g4go_complete(g4go_transport* handle, g4go_batch_result* out);
 
 /* Same as g4go_complete, but the detections stay in device memory: out->hits
  * is NULL, out->n_hits is their count and *hits_device points at n_hits
- * g4go_hit records in HBM (unordered), valid until the next enqueue. */
+ * g4go_hit records in HBM (in launch order when config.sort_hits is set,
+ * otherwise in completion order), valid until the next enqueue. */
 g4go_status g4go_complete_device(g4go_transport* handle, g4go_batch_result* out, const void** hits_device);
 
 /* ---- differentiation (path-replay adjoint) ----
