@@ -20,8 +20,11 @@ from .transport import HipTransportHost, LaunchForm
 
 CACHE = Path(__file__).resolve().parent / "autotune_cache.json"
 
+# 256-thread blocks only: the kernels' static LDS (hit stages, histogram) is
+# laid out for four waves per block, so smaller blocks waste LDS per wave and,
+# next to a large staged scene, lose occupancy.
 DEFAULT_SPACE = {
-    "block_size": (128, 256),
+    "block_size": (256,),
     "blocks_per_cu": (2, 4, 6, 8, 12),
     "chunk": (256, 1024, 4096),
     "refill_threshold": (8, 16, 32),
