@@ -222,6 +222,17 @@ G4GO_HD V3 normalize(V3 a) {
 
 G4GO_HD float clamp01(float v) { return fminf(fmaxf(v, 0.0f), 1.0f); }
 
+// Specular reflection at a face normal to `axis` (0, 1, 2): d - 2 (d.n) n
+// negates the direction's axis component, and -p + 2 (p.n) n negates the
+// polarisation's other two components.  Both are exact sign flips, so norms
+// and p.d are kept and no renormalisation is needed.
+G4GO_HD V3 negate_axis(V3 a, int axis) {
+    return {axis == 0 ? -a.x : a.x, axis == 1 ? -a.y : a.y, axis == 2 ? -a.z : a.z};
+}
+G4GO_HD V3 negate_off_axis(V3 a, int axis) {
+    return {axis == 0 ? a.x : -a.x, axis == 1 ? a.y : -a.y, axis == 2 ? a.z : -a.z};
+}
+
 // OptiXBoundaryPhysics.cuh:57-67
 G4GO_HD V3 project_polarization(V3 pol, V3 dir) {
     V3 out = sub(pol, scale(dir, dot(pol, dir)));

@@ -981,14 +981,9 @@ G4GO_HD int transport_step(Photon& ph, const SceneView& sc, const Rng& rng, cons
         } else {
             reflected = true;
             if (s.finish == kPolished) {
-                // Specular reflection at an axis-aligned face without
-                // renormalisation: with a unit normal along one axis, every
-                // operation of d - 2 (d.n) n and -p + 2 (p.n) n is exact (the
-                // normal component changes sign), so norms and p.d are kept.
-                const float dn = dot(dir, normal);
-                const float pn = dot(pol, normal);
-                dir = sub(dir, scale(normal, 2.0f * dn));
-                pol = add(scale(pol, -1.0f), scale(normal, 2.0f * pn));
+                // specular reflection at the axis-aligned face
+                dir = negate_axis(dir, axis);
+                pol = negate_off_axis(pol, axis);
             } else {
                 dir = sample_surface_reflection(sc, s, dir, normal, rng, ph);
                 pol = reflect_polarization(pol, normal, dir);

@@ -1289,11 +1289,11 @@ static int or_step(photon_t* ph, const oscene_t* s, const rng_t* r, real eps, in
         } else {
             reflected = 1;
             if (sf->finish == 0) {
-                /* specular reflection at an axis-aligned face, exact without renormalisation */
-                const real dn = vdot(dir, normal);
-                const real pn = vdot(pol, normal);
-                dir = vsub(dir, vscale(normal, RF(2.0) * dn));
-                pol = vadd(vscale(pol, RF(-1.0)), vscale(normal, RF(2.0) * pn));
+                /* specular reflection at the face normal to `axis`: d - 2 (d.n) n
+                 * negates d's axis component and -p + 2 (p.n) n negates p's
+                 * other two, exactly (sign flips) */
+                dir = mk(axis == 0 ? -dir.x : dir.x, axis == 1 ? -dir.y : dir.y, axis == 2 ? -dir.z : dir.z);
+                pol = mk(axis == 0 ? pol.x : -pol.x, axis == 1 ? pol.y : -pol.y, axis == 2 ? pol.z : -pol.z);
             } else {
                 dir = or_surface_reflection(sf, dir, normal, r, ph);
                 pol = or_reflect_pol(pol, normal, dir);
