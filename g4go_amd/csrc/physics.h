@@ -29,7 +29,7 @@ constexpr float kHcEvNm = 1239.84198f;  // h*c in eV*nm
 
 G4GO_HD uint32_t f2u(float x) { return __builtin_bit_cast(uint32_t, x); }
 G4GO_HD float u2f(uint32_t x) { return __builtin_bit_cast(float, x); }
-G4GO_HD bool finitef(float x) { return (f2u(x) & 0x7f800000u) != 0x7f800000u; }
+G4GO_HD bool finitef(float x) { return __builtin_isfinite(x); }
 
 // ---------------------------------------------------------------------------
 // Philox4x32-10 (Salmon et al., SC'11).  Counter layout follows the reference

@@ -930,8 +930,9 @@ G4GO_HD int transport_step(Photon& ph, const SceneView& sc, const Rng& rng, cons
         if (kTrace) {
             trace->surface = surface;
         }
-        const float refl = clamp01(prop_eval_ph(sc.pool, s.reflectivity, ph, 0.0f));
-        const float trans = clamp01(prop_eval_ph(sc.pool, s.transmittance, ph, 0.0f));
+        // classify_surface clamps both to [0, 1]
+        const float refl = prop_eval_ph(sc.pool, s.reflectivity, ph, 0.0f);
+        const float trans = prop_eval_ph(sc.pool, s.transmittance, ph, 0.0f);
         const int outcome = classify_surface(refl, trans, s.reflectivity.count != 0, s.transmittance.count != 0, u01(w.z));
         if (kTrace) {
             trace->event = outcome == kInteraction ? kTraceInteraction
