@@ -6,7 +6,8 @@ own unit-test vectors (TestOpticalBoundaryPhysics.cxx:37-166), outputs of the
 reference's own boundary-physics header compiled from the reference tree
 (tests/golden/ref_boundary_outputs.npz, made by tests/golden/make_golden.py),
 libm in double precision, textbook closed forms (Fresnel amplitudes, the
-dipole distribution of Rayleigh scattering) and brute force (box exit).
+dipole distribution of Rayleigh scattering), brute force (box exit) and
+numpy's own binning (arrival-time histogram).
 """
 import json
 import math
@@ -307,3 +308,31 @@ def test_box_exit_against_brute_force():
             if srt[1] - srt[0] > 1e-4 * srt[0] + 1e-6:
                 assert a == axis
             assert abs(dist - t[axis]) <= 1e-5 * max(1.0, t[axis])
+
+
+def test_arrival_histogram_against_numpy_binning():
+    """The oracle's 258-slot arrival histogram equals numpy.histogram of its own
+    detections' FP64 times over 256 bins of [0, 200) ns plus underflow and
+    overflow (hits within rounding distance of a bin edge excluded), and counts
+    every detection once."""
+    from g4go_amd import abi, workloads
+
+    cs = workloads.crystal_scene()
+    src = workloads.synthetic_source(cs, decay_ns=60.0)  # a slow tail puts hits past 200 ns
+    r = O.run_synthetic(cs.scene, src, 21, [0], [20000], precision=32)
+    t = r.hits["time_ns"].astype(np.float64)
+    assert len(t) > 3000 and np.any(t >= 200.0)
+    assert int(r.hist.sum()) == len(t)
+    edges = np.arange(abi.HIST_BINS + 1) * (200.0 / abi.HIST_BINS)
+    near_edge = np.min(np.abs(t[:, None] - edges[None, :]), axis=1) < 1e-9 * np.maximum(t, 1.0)
+    tk = t[~near_edge]
+    want = np.zeros(abi.HIST_SLOTS, np.int64)
+    want[:abi.HIST_BINS] = np.histogram(tk, bins=edges)[0]
+    want[abi.HIST_BINS] = np.count_nonzero(tk < 0.0)
+    want[abi.HIST_BINS + 1] = np.count_nonzero(tk >= 200.0)
+    # remove the excluded hits from the oracle's histogram by its own rule
+    got = r.hist.astype(np.int64).copy()
+    for x in t[near_edge]:
+        s = int(x * (abi.HIST_BINS / 200.0))
+        got[abi.HIST_BINS + 1 if s >= abi.HIST_BINS else s] -= 1
+    np.testing.assert_array_equal(got, want)
