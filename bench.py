@@ -62,6 +62,8 @@ def parse():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-array", action="store_true")
     ap.add_argument("--no-c1", action="store_true")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="only the 1e8-photon batch: under a kernel trace, the transport kernel's average is that launch")
     ap.add_argument("--retune", action="store_true", help="ignore the autotuner cache")
     ap.add_argument("--launch", type=str, default=None,
                     help="explicit persistent form block,blocks_per_cu,chunk,refill (skips the autotuner)")
@@ -342,6 +344,8 @@ def main():
             with open(os.path.join(out_dir, "autotune_table.json"), "w") as f:
                 json.dump(table, f, indent=0)
     wall, kms, out = run_form(host, cs, src, args.photons, persistent, args.steps, args.warmup)
+    if args.headline_only:
+        args.no_plain = args.no_adjoint = args.no_array = args.no_c1 = True
     plain = None
     if not args.no_plain:
         p_wall, p_kms, p_out = run_form(host, cs, src, args.photons, PLAIN, args.steps, args.warmup)
@@ -351,10 +355,12 @@ def main():
     counts = workloads.lognormal_counts(4096, 5000, sigma=1.5, lo=10, hi=1_000_000, seed=args.seed)
     n_uneven = int(counts.astype(np.int64).sum())
     graphed = LaunchForm(**{**persistent.__dict__, "graph": True})
-    b_wall, b_kms, b_out = run_uneven(host, cs, src, counts, graphed, args.steps, args.warmup)
-    uneven = {"events": int(counts.size), "photons": n_uneven, "median": 5000, "sigma": 1.5,
-              "batched_photons_per_s": n_uneven * args.steps / b_wall, "batched_kernel_ms": b_kms,
-              "batched_launch": graphed.__dict__}
+    uneven = None
+    if not args.headline_only:
+        b_wall, b_kms, b_out = run_uneven(host, cs, src, counts, graphed, args.steps, args.warmup)
+        uneven = {"events": int(counts.size), "photons": n_uneven, "median": 5000, "sigma": 1.5,
+                  "batched_photons_per_s": n_uneven * args.steps / b_wall, "batched_kernel_ms": b_kms,
+                  "batched_launch": graphed.__dict__}
     if not args.no_plain:
         pe = LaunchForm(persistent=False, per_event=True)
         e_wall, e_kms, e_out = run_uneven(host, cs, src, counts, pe, args.steps, args.warmup)
