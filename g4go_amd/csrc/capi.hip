@@ -859,7 +859,9 @@ struct Core {
         p.hits = d_hits.as<Hit>();
         p.hit_capacity = static_cast<uint32_t>(std::min<size_t>(d_hits.cap / sizeof(Hit), 0xffffffffu));
         p.hit_count = reinterpret_cast<uint32_t*>(d_meta.as<char>() + kMetaHitCount);
-        p.hit_launch = d_hit_launch.as<uint32_t>();
+        // launch indices order the hits (sort_hits) and let the adjoint find an
+        // emission hit's record; a synthetic batch without ordering needs neither
+        p.hit_launch = cfg.sort_hits != 0 || source == kSourceEmission ? d_hit_launch.as<uint32_t>() : nullptr;
         p.histogram = reinterpret_cast<unsigned long long*>(d_meta.as<char>() + kMetaHist);
         p.event_stats = d_event_stats.as<EventStats>();
         p.event_stride = static_cast<uint32_t>(l.cap_ev);
