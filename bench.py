@@ -24,7 +24,11 @@ and the path-replay adjoint through torch.autograd) against a forward pass;
 times 5000 1 MeV electron events in the crystal, with the FP32 oracle at 6
 threads and at all host threads on a sample of them; `host_batches` times the
 drop-in path at the reference scheduler's batch size (~1e6 photons, hits read
-back to the host) with one and with two batches in flight.
+back to the host) with one and with two batches in flight; `mesh_crystals`
+times 1e8 photons in two crystals with mesh solids: the crystal tapered to a
+4x4 mm back face (a frustum mesh, source uniform in the 3x3x20 mm box inside
+it) and the 3x3x20 mm crystal turned by a rotation (box meshes, point source
+at its centre), each with its transport-step cost relative to the box crystal.
 """
 from __future__ import annotations
 
@@ -62,6 +66,7 @@ def parse():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-array", action="store_true")
     ap.add_argument("--no-c1", action="store_true")
+    ap.add_argument("--no-mesh", action="store_true")
     ap.add_argument("--headline-only", action="store_true",
                     help="only the 1e8-photon batch: under a kernel trace, the transport kernel's average is that launch")
     ap.add_argument("--retune", action="store_true", help="ignore the autotuner cache")
@@ -214,6 +219,31 @@ def run_array(seed, launch, steps, warmup, plain):
     return res
 
 
+def run_mesh(seed, launch, steps, warmup, photons, box_ms_per_step):
+    """1e8 photons in the tapered crystal and in the rotated crystal (mesh solids)."""
+    from g4go_amd import workloads
+    from g4go_amd.transport import HipTransportHost, PhotonTransportConfig
+
+    rot = workloads.rotation_matrix((1.0, 2.0, 3.0), 0.7)
+    cases = {"tapered_crystal": workloads.tapered_crystal_scene(), "rotated_crystal": workloads.rotated_crystal_scene(rot)}
+    res = {}
+    for name, cs in cases.items():
+        if name == "tapered_crystal":
+            src = workloads.synthetic_source(cs)
+        else:
+            src = workloads.point_source(cs.crystal_ids[0], (0.0, 0.0, 0.0))
+        host = HipTransportHost(PhotonTransportConfig(seed=seed, sort_hits=False))
+        wall, kms, out = run_form(host, cs, src, photons, launch, steps, warmup)
+        host.close()
+        st = out.statistics
+        k_ms = float(np.mean(kms))
+        res[name] = {"photons_per_s": photons * steps / wall, "kernel_ms": k_ms,
+                     "steps_per_photon": st["total_steps"] / max(1, st["generated"]),
+                     "detected_fraction": st["detected"] / max(1, st["generated"]),
+                     "step_cost_over_box": (k_ms / st["total_steps"]) / box_ms_per_step}
+    return res
+
+
 def run_c1(seed, launch, steps, warmup, cpu, cpu_events):
     """5000 events of 1 MeV electrons in the crystal (scintillation records, 30
     photons/keV): device photons/s, and the FP32 oracle on `cpu_events` events at
@@ -345,7 +375,7 @@ def main():
                 json.dump(table, f, indent=0)
     wall, kms, out = run_form(host, cs, src, args.photons, persistent, args.steps, args.warmup)
     if args.headline_only:
-        args.no_plain = args.no_adjoint = args.no_array = args.no_c1 = True
+        args.no_plain = args.no_adjoint = args.no_array = args.no_c1 = args.no_mesh = True
     plain = None
     if not args.no_plain:
         p_wall, p_kms, p_out = run_form(host, cs, src, args.photons, PLAIN, args.steps, args.warmup)
@@ -373,6 +403,8 @@ def main():
     array = None if args.no_array else run_array(args.seed, persistent, args.steps, args.warmup, not args.no_plain)
     c1 = None if args.no_c1 else run_c1(args.seed, persistent, args.steps, args.warmup, not args.no_cpu, 200)
     host_batches = None if args.no_c1 else run_host_batches(args.seed, persistent)
+    mesh = None if args.no_mesh else run_mesh(args.seed, persistent, args.steps, args.warmup, args.photons,
+                                              float(np.mean(kms)) / out.statistics["total_steps"])
 
     st = out.statistics
     P, S, D = st["generated"], st["total_steps"], st["n_hits"]
@@ -408,6 +440,7 @@ def main():
         "array_events": array,
         "electron_events": c1,
         "host_batches": host_batches,
+        "mesh_crystals": mesh,
         "batched_over_plain": (None if plain is None else (args.photons * args.steps / wall) / plain["photons_per_s"]),
         "counts": {"photons": P, "steps": S, "detected": D, "steps_per_photon": S / max(P, 1),
                    "detected_fraction": D / max(P, 1)},

@@ -96,6 +96,18 @@ class SurfaceBinding(C.Structure):
     _fields_ = [("from_volume_id", C.c_uint32), ("to_volume_id", C.c_uint32), ("surface_id", C.c_uint32)]
 
 
+class VolumeMesh(C.Structure):
+    _fields_ = [
+        ("volume_id", C.c_uint32),
+        ("n_vertices", C.c_uint32),
+        ("vertices_mm", C.POINTER(C.c_double)),
+        ("indices", C.POINTER(C.c_uint32)),
+        ("n_triangles", C.c_uint32),
+        ("reserved", C.c_uint32),
+        ("triangle_normals", C.POINTER(C.c_double)),
+    ]
+
+
 class Scene(C.Structure):
     _fields_ = [
         ("materials", C.POINTER(Material)),
@@ -107,6 +119,8 @@ class Scene(C.Structure):
         ("bindings", C.POINTER(SurfaceBinding)),
         ("n_bindings", C.c_uint32),
         ("world_volume_id", C.c_uint32),
+        ("meshes", C.POINTER(VolumeMesh)),
+        ("n_meshes", C.c_uint32),
     ]
 
 

@@ -58,6 +58,8 @@ struct HostScene {
     std::vector<uint32_t> daughters;
     std::vector<Binding> bindings;
     std::vector<BvhNode> bvh;
+    std::vector<Plane> planes;  // faces of the convex mesh solids
+    std::vector<std::vector<double>> face_points;  // per plane: its triangles' vertices (xyz), host only
     uint32_t world = kInvalid;
     uint32_t hot_offset = 0, hot_count = 0;  // hot energy grid (mark_hot_grid)
     std::unordered_map<uint32_t, uint32_t> index_of;  // public volume ID -> dense index
@@ -282,6 +284,201 @@ void mark_hot_grid(HostScene& s) {
     }
 }
 
+// Face planes and bounding box of a closed convex triangle mesh (world
+// frame), appended to `planes`.  All arithmetic is FP64 in a fixed order, the
+// results rounded once to FP32:
+//   * each triangle's plane: unit normal u = (v1 - v0) x (v2 - v0) / |...|,
+//     offset h = u.v0, turned so that the vertex centroid c lies inside
+//     (u.c < h);
+//   * triangles whose planes agree within 1e-9 (normal components, and
+//     offsets relative to the mesh size) and whose boundary normals agree
+//     form one face; faces keep the first triangle's values, in triangle
+//     order;
+//   * the mesh is closed when every edge (vertices welded by equal
+//     coordinates) belongs to exactly two triangles, convex when no vertex
+//     lies more than 1e-6 of the mesh size outside a face plane, and solid
+//     when c lies more than that inside every face plane.
+// The reference accepts any triangle mesh (OptiXTransportHost.cpp:1118-1200);
+// this backend's clipping tests need a closed convex one.
+void mesh_solid(const g4go_volume_mesh& m, std::vector<Plane>& planes, std::vector<std::vector<double>>& face_points,
+                float lo[3], float hi[3]) {
+    const std::string who = "volume " + std::to_string(m.volume_id) + " mesh";
+    if (m.vertices_mm == nullptr || m.indices == nullptr || m.n_vertices < 4 || m.n_triangles < 4) {
+        fail(G4GO_ERROR_INVALID_ARGUMENT, who + ": needs at least 4 vertices and 4 triangles");
+    }
+    if (m.reserved != 0) {
+        fail(G4GO_ERROR_INVALID_ARGUMENT, who + ": reserved field must be 0");
+    }
+    const uint32_t nv = m.n_vertices, nt = m.n_triangles;
+    const double* v = m.vertices_mm;
+    for (uint64_t i = 0; i < 3ull * nv; ++i) {
+        if (!std::isfinite(v[i])) {
+            fail(G4GO_ERROR_INVALID_ARGUMENT, who + ": non-finite vertex");
+        }
+    }
+    for (uint64_t i = 0; i < 3ull * nt; ++i) {
+        if (m.indices[i] >= nv) {
+            fail(G4GO_ERROR_INVALID_ARGUMENT, who + ": index exceeds vertex count");
+        }
+    }
+    double c[3] = {0.0, 0.0, 0.0};
+    for (uint32_t i = 0; i < nv; ++i) {
+        for (int k = 0; k < 3; ++k) {
+            c[k] += v[3 * i + k];
+        }
+    }
+    for (int k = 0; k < 3; ++k) {
+        c[k] /= static_cast<double>(nv);
+    }
+    double size = 0.0;
+    double vlo[3] = {v[0], v[1], v[2]}, vhi[3] = {v[0], v[1], v[2]};
+    for (uint32_t i = 0; i < nv; ++i) {
+        for (int k = 0; k < 3; ++k) {
+            size = std::max(size, std::fabs(v[3 * i + k] - c[k]));
+            vlo[k] = std::min(vlo[k], v[3 * i + k]);
+            vhi[k] = std::max(vhi[k], v[3 * i + k]);
+        }
+    }
+    const double tol = 1e-6 * size;
+    // closed: every edge of the welded mesh in exactly two triangles
+    {
+        std::vector<uint32_t> order(nv);
+        for (uint32_t i = 0; i < nv; ++i) {
+            order[i] = i;
+        }
+        const auto less = [&](uint32_t a, uint32_t b) {
+            for (int k = 0; k < 3; ++k) {
+                if (v[3 * a + k] != v[3 * b + k]) {
+                    return v[3 * a + k] < v[3 * b + k];
+                }
+            }
+            return a < b;
+        };
+        std::sort(order.begin(), order.end(), less);
+        std::vector<uint32_t> weld(nv);
+        for (uint32_t i = 0; i < nv; ++i) {
+            const uint32_t a = order[i];
+            const bool same = i > 0 && v[3 * a] == v[3 * order[i - 1]] && v[3 * a + 1] == v[3 * order[i - 1] + 1] &&
+                              v[3 * a + 2] == v[3 * order[i - 1] + 2];
+            weld[a] = same ? weld[order[i - 1]] : a;
+        }
+        std::vector<uint64_t> edges;
+        edges.reserve(3ull * nt);
+        for (uint32_t t = 0; t < nt; ++t) {
+            for (int e = 0; e < 3; ++e) {
+                const uint32_t a = weld[m.indices[3 * t + e]], b = weld[m.indices[3 * t + (e + 1) % 3]];
+                if (a == b) {
+                    fail(G4GO_ERROR_INVALID_ARGUMENT, who + ": degenerate triangle");
+                }
+                edges.push_back((static_cast<uint64_t>(std::min(a, b)) << 32) | std::max(a, b));
+            }
+        }
+        std::sort(edges.begin(), edges.end());
+        for (size_t i = 0; i < edges.size();) {
+            size_t j = i;
+            while (j < edges.size() && edges[j] == edges[i]) {
+                ++j;
+            }
+            if (j - i != 2) {
+                fail(G4GO_ERROR_INVALID_ARGUMENT, who + ": not closed (an edge is not shared by exactly two triangles)");
+            }
+            i = j;
+        }
+    }
+    struct Face64 {
+        double u[3], h, s[3];
+        std::vector<double> points;  // its triangles' vertices
+    };
+    std::vector<Face64> faces;
+    for (uint32_t t = 0; t < nt; ++t) {
+        const double* p0 = v + 3 * m.indices[3 * t];
+        const double* p1 = v + 3 * m.indices[3 * t + 1];
+        const double* p2 = v + 3 * m.indices[3 * t + 2];
+        const double a[3] = {p1[0] - p0[0], p1[1] - p0[1], p1[2] - p0[2]};
+        const double b[3] = {p2[0] - p0[0], p2[1] - p0[1], p2[2] - p0[2]};
+        const double g[3] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
+        const double len = std::sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
+        if (!(len > 0.0)) {
+            fail(G4GO_ERROR_INVALID_ARGUMENT, who + ": degenerate triangle");
+        }
+        Face64 f;
+        for (int k = 0; k < 3; ++k) {
+            f.u[k] = g[k] / len;
+        }
+        f.h = f.u[0] * p0[0] + f.u[1] * p0[1] + f.u[2] * p0[2];
+        if (f.u[0] * c[0] + f.u[1] * c[1] + f.u[2] * c[2] > f.h) {
+            for (int k = 0; k < 3; ++k) {
+                f.u[k] = -f.u[k];
+            }
+            f.h = -f.h;
+        }
+        if (m.triangle_normals != nullptr) {
+            const double* n = m.triangle_normals + 3 * t;
+            const double nl = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+            if (!(nl > 0.0) || !std::isfinite(nl)) {
+                fail(G4GO_ERROR_INVALID_ARGUMENT, who + ": invalid triangle normal");
+            }
+            for (int k = 0; k < 3; ++k) {
+                f.s[k] = n[k] / nl;
+            }
+        } else {
+            for (int k = 0; k < 3; ++k) {
+                f.s[k] = f.u[k];
+            }
+        }
+        Face64* into = nullptr;
+        for (Face64& o : faces) {
+            bool same = std::fabs(o.h - f.h) <= 1e-9 * size;
+            for (int k = 0; k < 3; ++k) {
+                same = same && std::fabs(o.u[k] - f.u[k]) <= 1e-9 && std::fabs(o.s[k] - f.s[k]) <= 1e-9;
+            }
+            if (same) {
+                into = &o;
+                break;
+            }
+        }
+        if (into == nullptr) {
+            if (faces.size() == kMaxMeshPlanes) {
+                fail(G4GO_ERROR_INVALID_ARGUMENT, who + ": more than " + std::to_string(kMaxMeshPlanes) + " faces");
+            }
+            faces.push_back(f);
+            into = &faces.back();
+        }
+        into->points.insert(into->points.end(), {p0[0], p0[1], p0[2], p1[0], p1[1], p1[2], p2[0], p2[1], p2[2]});
+    }
+    for (const Face64& f : faces) {
+        if (!(f.h - (f.u[0] * c[0] + f.u[1] * c[1] + f.u[2] * c[2]) > tol)) {
+            fail(G4GO_ERROR_INVALID_ARGUMENT, who + ": flat (the centroid lies on a face plane)");
+        }
+        for (uint32_t i = 0; i < nv; ++i) {
+            if (f.u[0] * v[3 * i] + f.u[1] * v[3 * i + 1] + f.u[2] * v[3 * i + 2] - f.h > tol) {
+                fail(G4GO_ERROR_INVALID_ARGUMENT, who + ": not convex");
+            }
+        }
+        Plane pl{};
+        for (int k = 0; k < 3; ++k) {
+            pl.n[k] = static_cast<float>(f.u[k]);
+            pl.shade[k] = static_cast<float>(f.s[k]);
+        }
+        pl.h = static_cast<float>(f.h);
+        pl.across = kFaceDynamic;
+        pl.surface_parent = pl.surface_across = kInvalid;
+        planes.push_back(pl);
+        face_points.push_back(f.points);
+    }
+    // bounding box rounded outwards
+    for (int k = 0; k < 3; ++k) {
+        lo[k] = static_cast<float>(vlo[k]);
+        if (static_cast<double>(lo[k]) > vlo[k]) {
+            lo[k] = std::nextafter(lo[k], -INFINITY);
+        }
+        hi[k] = static_cast<float>(vhi[k]);
+        if (static_cast<double>(hi[k]) < vhi[k]) {
+            hi[k] = std::nextafter(hi[k], INFINITY);
+        }
+    }
+}
+
 HostScene build_scene(const g4go_scene& in, float tolerance) {
     HostScene s;
     if (in.n_volumes == 0 || in.volumes == nullptr) {
@@ -345,15 +542,33 @@ HostScene build_scene(const g4go_scene& in, float tolerance) {
         return it->second;
     };
     s.world = map_id(in.world_volume_id, "world");
+    // mesh solids, by volume
+    if (in.n_meshes != 0 && in.meshes == nullptr) {
+        fail(G4GO_ERROR_INVALID_ARGUMENT, "scene array pointer is null");
+    }
+    std::vector<const g4go_volume_mesh*> mesh_of(in.n_volumes, nullptr);
+    for (uint32_t i = 0; i < in.n_meshes; ++i) {
+        const uint32_t vi = map_id(in.meshes[i].volume_id, "mesh");
+        if (mesh_of[vi] != nullptr) {
+            fail(G4GO_ERROR_INVALID_ARGUMENT, "volume " + std::to_string(in.meshes[i].volume_id) + " has two meshes");
+        }
+        mesh_of[vi] = &in.meshes[i];
+    }
     for (uint32_t i = 0; i < in.n_volumes; ++i) {
         const g4go_volume& v = in.volumes[i];
         Volume dv{};
-        for (int k = 0; k < 3; ++k) {
-            if (!std::isfinite(v.center_mm[k]) || !(v.half_mm[k] > 0.0f) || !std::isfinite(v.half_mm[k])) {
-                fail(G4GO_ERROR_INVALID_ARGUMENT, "volume " + std::to_string(v.volume_id) + ": invalid box");
+        if (mesh_of[i] != nullptr) {
+            dv.plane_begin = static_cast<uint32_t>(s.planes.size());
+            mesh_solid(*mesh_of[i], s.planes, s.face_points, dv.lo, dv.hi);
+            dv.plane_count = static_cast<uint32_t>(s.planes.size()) - dv.plane_begin;
+        } else {
+            for (int k = 0; k < 3; ++k) {
+                if (!std::isfinite(v.center_mm[k]) || !(v.half_mm[k] > 0.0f) || !std::isfinite(v.half_mm[k])) {
+                    fail(G4GO_ERROR_INVALID_ARGUMENT, "volume " + std::to_string(v.volume_id) + ": invalid box");
+                }
+                dv.lo[k] = static_cast<float>(static_cast<double>(v.center_mm[k]) - v.half_mm[k]);
+                dv.hi[k] = static_cast<float>(static_cast<double>(v.center_mm[k]) + v.half_mm[k]);
             }
-            dv.lo[k] = static_cast<float>(static_cast<double>(v.center_mm[k]) - v.half_mm[k]);
-            dv.hi[k] = static_cast<float>(static_cast<double>(v.center_mm[k]) + v.half_mm[k]);
         }
         if (v.material_id >= in.n_materials) {
             fail(G4GO_ERROR_INVALID_ARGUMENT, "volume " + std::to_string(v.volume_id) + ": unknown material");
@@ -418,6 +633,10 @@ HostScene build_scene(const g4go_scene& in, float tolerance) {
                 const int face = 2 * axis + side;
                 v.faces[face].neighbor = kInvalid;
                 v.faces[face].surface = kInvalid;
+                if (v.plane_count != 0) {
+                    v.faces[face].neighbor = kFaceDynamic;  // unused: a mesh solid's faces are its planes
+                    continue;
+                }
                 if (v.parent == kInvalid) {
                     continue;  // leaving the world: escape
                 }
@@ -425,6 +644,7 @@ HostScene build_scene(const g4go_scene& in, float tolerance) {
                 const int a1 = (axis + 1) % 3, a2 = (axis + 2) % 3;
                 std::vector<uint32_t> touching;
                 bool covered = false;
+                bool mesh_near = false;
                 const Volume& par = s.volumes[v.parent];
                 for (uint32_t k = 0; k < par.daughter_count; ++k) {
                     const uint32_t j = s.daughters[par.daughter_begin + k];
@@ -432,6 +652,14 @@ HostScene build_scene(const g4go_scene& in, float tolerance) {
                         continue;
                     }
                     const Volume& o = s.volumes[j];
+                    if (o.plane_count != 0) {
+                        // a mesh sibling whose bounding box reaches the face: found at run time
+                        mesh_near = mesh_near ||
+                                    (o.lo[axis] <= plane + tolerance && o.hi[axis] >= plane - tolerance &&
+                                     o.lo[a1] <= v.hi[a1] + tolerance && o.hi[a1] >= v.lo[a1] - tolerance &&
+                                     o.lo[a2] <= v.hi[a2] + tolerance && o.hi[a2] >= v.lo[a2] - tolerance);
+                        continue;
+                    }
                     const float oplane = side ? o.lo[axis] : o.hi[axis];
                     if (std::fabs(oplane - plane) > tolerance) {
                         continue;
@@ -446,6 +674,10 @@ HostScene build_scene(const g4go_scene& in, float tolerance) {
                               o.lo[a2] <= v.lo[a2] + tolerance && o.hi[a2] >= v.hi[a2] - tolerance;
                 }
                 uint32_t next;
+                if (mesh_near) {
+                    v.faces[face].neighbor = kFaceDynamic;
+                    continue;
+                }
                 if (touching.empty()) {
                     next = v.parent;
                 } else if (touching.size() == 1 && covered) {
@@ -466,6 +698,59 @@ HostScene build_scene(const g4go_scene& in, float tolerance) {
             build_bvh(s, v, v.daughter_begin, v.daughter_count);
         }
     }
+    // what lies across each mesh face: the siblings whose boxes meet the box
+    // of the face's points offset by the boundary tolerance along the outward
+    // normal, widened by a margin far above rounding (a superset of the
+    // siblings the run-time point query can find there)
+    for (uint32_t i = 0; i < in.n_volumes; ++i) {
+        const Volume& v = s.volumes[i];
+        for (uint32_t k = v.plane_begin; k < v.plane_begin + v.plane_count; ++k) {
+            Plane& pl = s.planes[k];
+            if (v.parent == kInvalid) {
+                pl.across = kInvalid;  // leaving the world: escape
+                continue;
+            }
+            const std::vector<double>& pts = s.face_points[k];
+            double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+            double scale = 0.0;
+            for (size_t j = 0; j < pts.size(); j += 3) {
+                for (int a = 0; a < 3; ++a) {
+                    const double x = pts[j + a] + 2.0 * tolerance * pl.n[a];
+                    lo[a] = std::min(lo[a], x);
+                    hi[a] = std::max(hi[a], x);
+                    scale = std::max(scale, std::fabs(x));
+                }
+            }
+            const double margin = 1e-3 + 1e-5 * scale + 2.0 * tolerance;
+            uint32_t found = kInvalid, count = 0;
+            const Volume& par = s.volumes[v.parent];
+            for (uint32_t slot = par.daughter_begin; slot < par.daughter_begin + par.daughter_count; ++slot) {
+                const uint32_t j = s.daughters[slot];
+                if (j == i) {
+                    continue;
+                }
+                const Volume& o = s.volumes[j];
+                bool meet = true;
+                for (int a = 0; a < 3; ++a) {
+                    meet = meet && o.lo[a] <= hi[a] + margin && o.hi[a] >= lo[a] - margin;
+                }
+                if (meet) {
+                    found = slot;
+                    ++count;
+                }
+            }
+            pl.surface_parent = host_find_surface(s, i, v.parent);
+            if (count == 0) {
+                pl.across = kInvalid;
+            } else if (count == 1) {
+                pl.across = found;
+                pl.surface_across = host_find_surface(s, i, s.daughters[found]);
+            } else {
+                pl.across = kFaceDynamic;
+            }
+        }
+    }
+    s.face_points.clear();
     if (s.pool.empty()) {
         s.pool.push_back(0.0f);
     }
@@ -632,26 +917,27 @@ struct Core {
         off = align256(off + std::max<size_t>(1, s.surfaces.size()) * sizeof(Surface));
         const size_t o_vol = off;
         off = align256(off + s.volumes.size() * sizeof(Volume));
-        const size_t o_dau = off;
-        off = align256(off + std::max<size_t>(1, s.daughters.size()) * sizeof(uint32_t));
         const size_t o_bind = off;
         off = align256(off + std::max<size_t>(1, s.bindings.size()) * sizeof(Binding));
         const size_t o_bvh = off;
         off = align256(off + std::max<size_t>(1, s.bvh.size()) * sizeof(BvhNode));
         const size_t o_dbox = off;
         off = align256(off + std::max<size_t>(1, s.daughters.size()) * sizeof(DaughterBox));
+        const size_t o_planes = off;
+        off = align256(off + std::max<size_t>(1, s.planes.size()) * sizeof(Plane));
         std::vector<char> blob(off, 0);
         // daughter boxes in daughter-list order (after the BVH build reordered the list)
         for (size_t k = 0; k < s.daughters.size(); ++k) {
             const Volume& d = s.volumes[s.daughters[k]];
-            DaughterBox b{{d.lo[0], d.lo[1], d.lo[2]}, s.daughters[k], {d.hi[0], d.hi[1], d.hi[2]}, 0u};
+            DaughterBox b{{d.lo[0], d.lo[1], d.lo[2]}, s.daughters[k], {d.hi[0], d.hi[1], d.hi[2]},
+                          d.plane_count != 0 ? 1u : 0u};
             std::memcpy(blob.data() + o_dbox + k * sizeof(DaughterBox), &b, sizeof(b));
         }
+        if (!s.planes.empty()) std::memcpy(blob.data() + o_planes, s.planes.data(), s.planes.size() * sizeof(Plane));
         std::memcpy(blob.data() + o_pool, s.pool.data(), s.pool.size() * sizeof(float));
         if (!s.materials.empty()) std::memcpy(blob.data() + o_mat, s.materials.data(), s.materials.size() * sizeof(Material));
         if (!s.surfaces.empty()) std::memcpy(blob.data() + o_surf, s.surfaces.data(), s.surfaces.size() * sizeof(Surface));
         std::memcpy(blob.data() + o_vol, s.volumes.data(), s.volumes.size() * sizeof(Volume));
-        if (!s.daughters.empty()) std::memcpy(blob.data() + o_dau, s.daughters.data(), s.daughters.size() * sizeof(uint32_t));
         if (!s.bindings.empty()) std::memcpy(blob.data() + o_bind, s.bindings.data(), s.bindings.size() * sizeof(Binding));
         if (!s.bvh.empty()) std::memcpy(blob.data() + o_bvh, s.bvh.data(), s.bvh.size() * sizeof(BvhNode));
         d_scene.ensure(blob.size(), "hipMalloc scene");
@@ -663,10 +949,11 @@ struct Core {
         view.materials = reinterpret_cast<const Material*>(base + o_mat);
         view.surfaces = reinterpret_cast<const Surface*>(base + o_surf);
         view.volumes = reinterpret_cast<const Volume*>(base + o_vol);
-        view.daughters = reinterpret_cast<const uint32_t*>(base + o_dau);
         view.bindings = reinterpret_cast<const Binding*>(base + o_bind);
         view.bvh = reinterpret_cast<const BvhNode*>(base + o_bvh);
         view.dbox = reinterpret_cast<const DaughterBox*>(base + o_dbox);
+        view.planes = reinterpret_cast<const Plane*>(base + o_planes);
+        view.n_planes = static_cast<uint32_t>(s.planes.size());
         view.n_daughters = static_cast<uint32_t>(s.daughters.size());
         view.hot_offset = s.hot_offset;
         view.hot_count = s.hot_count;
@@ -721,10 +1008,11 @@ struct Core {
         p.hist_lo_ns = 0.0;
         p.hist_bins_per_ns = kHistBins / 200.0;
         // Every block stages scene parts in LDS, each when it still fits, in
-        // the order a step needs them: the volume and surface records (read
-        // every step), the BVH nodes and daughter boxes (read by photons in a
-        // volume with a daughter BVH), the property pool (read mostly when a
-        // photon starts), the material records (read when it changes volume).
+        // the order a step needs them: the volume and surface records and the
+        // mesh planes (read every step), the BVH nodes and daughter boxes
+        // (read by photons in a volume with a daughter BVH), the property pool
+        // (read mostly when a photon starts), the material records (read when
+        // it changes volume).
         uint64_t used = 0;
         const auto fits = [&](uint64_t bytes) {
             if (bytes == 0 || used + bytes > kLdsSceneBytes) {
@@ -735,8 +1023,12 @@ struct Core {
         };
         if (fits(static_cast<uint64_t>(view.n_volumes) * sizeof(Volume))) p.lds_records |= kLdsVolumes;
         if (fits(static_cast<uint64_t>(view.n_surfaces) * sizeof(Surface))) p.lds_records |= kLdsSurfaces;
+        if (fits(static_cast<uint64_t>(view.n_planes) * sizeof(Plane))) p.lds_records |= kLdsPlanes;
+        // (a mesh face's sibling test reads the daughter boxes too)
         const uint64_t entries = static_cast<uint64_t>(view.n_bvh_nodes) + view.n_daughters;
-        if (view.n_bvh_nodes != 0 && fits(entries * 32u)) p.lds_entries = static_cast<uint32_t>(entries);
+        if ((view.n_bvh_nodes != 0 || view.n_planes != 0) && fits(entries * 32u)) {
+            p.lds_entries = static_cast<uint32_t>(entries);
+        }
         if (fits((static_cast<uint64_t>(view.n_pool) + 3u) / 4u * 16u)) p.lds_pool_floats = view.n_pool;
         if (fits(static_cast<uint64_t>(view.n_materials) * sizeof(Material))) p.lds_records |= kLdsMaterials;
         return p;

@@ -189,14 +189,15 @@ __device__ __forceinline__ void store_hit(float4* d, const Photon& ph, uint32_t 
 }  // namespace
 
 // kLdsMode: which scene parts a launch guarantees are staged in LDS
-// (lds_mode): 0 none for certain, 1 the volume and surface records, 2 those
-// and the whole property pool and the material records.  Their pointers then
-// come from the LDS array on every path, so the compiler reads them with LDS
-// instructions instead of flat ones.
+// (lds_mode): 0 none for certain, 1 the volume and surface records (and the
+// mesh planes of a scene with mesh solids), 2 those and the whole property
+// pool and the material records.  Their pointers then come from the LDS array
+// on every path, so the compiler reads them with LDS instructions instead of
+// flat ones.
 enum LdsMode : int { kLdsNone = 0, kLdsHot = 1, kLdsAll = 2 };
 
 inline int lds_mode(const KernelParams& p) {
-    const uint32_t hot = kLdsVolumes | kLdsSurfaces;
+    const uint32_t hot = kLdsVolumes | kLdsSurfaces | (p.scene.n_planes != 0u ? kLdsPlanes : 0u);
     const uint32_t all = hot | kLdsMaterials;
     if ((p.lds_records & all) == all && p.scene.n_pool != 0 && p.lds_pool_floats == p.scene.n_pool) {
         return kLdsAll;
@@ -206,9 +207,10 @@ inline int lds_mode(const KernelParams& p) {
 
 // Copies the scene parts that p selects (lds_entries, lds_pool_floats,
 // lds_records; the kLdsMode parts unconditionally) into `lds`, in the order
-// BVH nodes + daughter boxes, pool, volumes, surfaces, materials, and points sv
-// at the copies.  The caller synchronises the block before using them.
-template <int kLdsMode>
+// BVH nodes + daughter boxes, pool, volumes, surfaces, mesh planes,
+// materials, and points sv at the copies.  The caller synchronises the block
+// before using them.
+template <int kLdsMode, bool kMesh = false>
 __device__ __forceinline__ void stage_scene(const KernelParams& p, float4* lds, SceneView& sv) {
     if (p.lds_entries != 0u) {
         const uint32_t node_words = 2u * p.scene.n_bvh_nodes;
@@ -244,13 +246,16 @@ __device__ __forceinline__ void stage_scene(const KernelParams& p, float4* lds, 
         sv.surfaces =
             static_cast<const Surface*>(stage_quads(p.scene.surfaces, p.scene.n_surfaces * (sizeof(Surface) / 16u)));
     }
+    if (kMesh && (kLdsMode >= kLdsHot || (p.lds_records & kLdsPlanes))) {  // box scenes have no planes
+        sv.planes = static_cast<const Plane*>(stage_quads(p.scene.planes, p.scene.n_planes * (sizeof(Plane) / 16u)));
+    }
     if (kLdsMode >= kLdsAll || (p.lds_records & kLdsMaterials)) {
         sv.materials = static_cast<const Material*>(
             stage_quads(p.scene.materials, p.scene.n_materials * (sizeof(Material) / 16u)));
     }
 }
 
-template <int kSource, bool kPersistent, int kLdsMode>
+template <int kSource, bool kPersistent, int kLdsMode, bool kMesh>
 __device__ __forceinline__ void transport_body(const KernelParams& p) {
     __shared__ uint32_t s_hist[kHistSlots];
     __shared__ float4 s_stage[kMaxWavesPerBlock][3 * kStageCap];
@@ -260,7 +265,7 @@ __device__ __forceinline__ void transport_body(const KernelParams& p) {
         s_hist[i] = 0u;
     }
     SceneView sv = p.scene;
-    stage_scene<kLdsMode>(p, s_scene, sv);
+    stage_scene<kLdsMode, kMesh>(p, s_scene, sv);
     __syncthreads();
     float4* const stage = s_stage[threadIdx.x >> 6];
     uint32_t* const stage_src = s_stage_src[threadIdx.x >> 6];
@@ -359,7 +364,7 @@ __device__ __forceinline__ void transport_body(const KernelParams& p) {
         uint32_t sensor = kInvalid;
         if (alive) {
             bool zero = false;
-            status = transport_step<false>(ph, sv, p.rng, p.cfg, zero, sensor, nullptr);
+            status = transport_step<false, kMesh>(ph, sv, p.rng, p.cfg, zero, sensor, nullptr);
             ++ph.step;
             st.zero_step += zero ? 1u : 0u;
             if (status == kAlive && ph.step >= p.max_steps) {
@@ -424,29 +429,43 @@ __device__ __forceinline__ void transport_body(const KernelParams& p) {
 // the compiler schedules the body into ~140 registers (three waves); the
 // fourth wave hides more of the dependent scene-load latency of a step than
 // the few spills it forces cost.
-template <int kSource, int kLdsMode>
+template <int kSource, int kLdsMode, bool kMesh>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
 transport_kernel_persistent(KernelParams p) {
-    transport_body<kSource, true, kLdsMode>(p);
+    transport_body<kSource, true, kLdsMode, kMesh>(p);
 }
 
-template <int kSource, int kLdsMode>
+template <int kSource, int kLdsMode, bool kMesh>
 __global__ void __launch_bounds__(256) transport_kernel_plain(KernelParams p) {
-    transport_body<kSource, false, kLdsMode>(p);
+    transport_body<kSource, false, kLdsMode, kMesh>(p);
 }
 
-template <int kSource, int kLdsMode>
+template <int kSource, int kLdsMode, bool kMesh = false>
 void launch_transport_form(const KernelParams& p, bool persistent, int block, int grid, size_t lds, hipStream_t stream) {
     if (persistent) {
-        hipLaunchKernelGGL((transport_kernel_persistent<kSource, kLdsMode>), dim3(grid), dim3(block), lds, stream, p);
+        hipLaunchKernelGGL((transport_kernel_persistent<kSource, kLdsMode, kMesh>), dim3(grid), dim3(block), lds,
+                           stream, p);
     } else {
-        hipLaunchKernelGGL((transport_kernel_plain<kSource, kLdsMode>), dim3(grid), dim3(block), lds, stream, p);
+        hipLaunchKernelGGL((transport_kernel_plain<kSource, kLdsMode, kMesh>), dim3(grid), dim3(block), lds, stream,
+                           p);
     }
 }
 
+// Scenes with mesh solids run their own instances, so the box scenes' kernels
+// carry no mesh code.
 template <int kSource>
 void launch_transport_source(const KernelParams& p, bool persistent, int block, int grid, size_t lds,
                              hipStream_t stream) {
+    if (p.scene.n_planes != 0u) {
+        if (lds_mode(p) == kLdsAll) {
+            launch_transport_form<kSource, kLdsAll, true>(p, persistent, block, grid, lds, stream);
+        } else if (lds_mode(p) == kLdsHot) {
+            launch_transport_form<kSource, kLdsHot, true>(p, persistent, block, grid, lds, stream);
+        } else {
+            launch_transport_form<kSource, kLdsNone, true>(p, persistent, block, grid, lds, stream);
+        }
+        return;
+    }
     switch (lds_mode(p)) {
         case kLdsAll:
             launch_transport_form<kSource, kLdsAll>(p, persistent, block, grid, lds, stream);
@@ -635,7 +654,7 @@ __device__ __forceinline__ void add_material_grads(double* grad, const SceneView
 
 // Four waves per SIMD (at most 128 VGPRs), as for the persistent transport
 // kernel: the replay runs the same step code.
-template <int kSource, int kLdsMode>
+template <int kSource, int kLdsMode, bool kMesh>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) adjoint_kernel(AdjointParams a) {
     extern __shared__ double s_grad[];  // gradient accumulator, then the staged scene
     const uint32_t n_params = a.map.n_params;
@@ -648,7 +667,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) a
     }
     const KernelParams& p = a.fwd;
     SceneView sc = p.scene;
-    stage_scene<kLdsMode>(p, reinterpret_cast<float4*>(s_grad) + adjoint_grad_quads(n_params), sc);
+    stage_scene<kLdsMode, kMesh>(p, reinterpret_cast<float4*>(s_grad) + adjoint_grad_quads(n_params), sc);
     __syncthreads();
     const uint32_t n_sources = p.header != nullptr ? p.header[0] : 0u;
     uint32_t replayed = 0, mismatch = 0, unsupported = 0, steps = 0;
@@ -675,7 +694,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) a
         for (;;) {
             bool zero = false;
             StepTrace tr;
-            status = transport_step<true>(ph, sc, p.rng, p.cfg, zero, sensor, &tr);
+            status = transport_step<true, kMesh>(ph, sc, p.rng, p.cfg, zero, sensor, &tr);
             ++ph.step;
             ++steps;
             if (tr.segment > 0.0f || tr.event == kTraceRayleigh) {
@@ -757,15 +776,25 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) a
 
 template <int kSource>
 void launch_adjoint_source(const AdjointParams& p, int block, int grid, size_t lds, hipStream_t stream) {
+    if (p.fwd.scene.n_planes != 0u) {
+        if (lds_mode(p.fwd) == kLdsAll) {
+            hipLaunchKernelGGL((adjoint_kernel<kSource, kLdsAll, true>), dim3(grid), dim3(block), lds, stream, p);
+        } else if (lds_mode(p.fwd) == kLdsHot) {
+            hipLaunchKernelGGL((adjoint_kernel<kSource, kLdsHot, true>), dim3(grid), dim3(block), lds, stream, p);
+        } else {
+            hipLaunchKernelGGL((adjoint_kernel<kSource, kLdsNone, true>), dim3(grid), dim3(block), lds, stream, p);
+        }
+        return;
+    }
     switch (lds_mode(p.fwd)) {
         case kLdsAll:
-            hipLaunchKernelGGL((adjoint_kernel<kSource, kLdsAll>), dim3(grid), dim3(block), lds, stream, p);
+            hipLaunchKernelGGL((adjoint_kernel<kSource, kLdsAll, false>), dim3(grid), dim3(block), lds, stream, p);
             break;
         case kLdsHot:
-            hipLaunchKernelGGL((adjoint_kernel<kSource, kLdsHot>), dim3(grid), dim3(block), lds, stream, p);
+            hipLaunchKernelGGL((adjoint_kernel<kSource, kLdsHot, false>), dim3(grid), dim3(block), lds, stream, p);
             break;
         default:
-            hipLaunchKernelGGL((adjoint_kernel<kSource, kLdsNone>), dim3(grid), dim3(block), lds, stream, p);
+            hipLaunchKernelGGL((adjoint_kernel<kSource, kLdsNone, false>), dim3(grid), dim3(block), lds, stream, p);
             break;
     }
 }
@@ -904,7 +933,9 @@ __global__ void trace_kernel(KernelParams p, uint32_t event_id, uint32_t photon_
         for (; n < max_steps;) {
             bool zero = false;
             uint32_t sensor = kInvalid;
-            const int status = transport_step<false>(ph, p.scene, p.rng, p.cfg, zero, sensor, nullptr);
+            const int status = p.scene.n_planes != 0u
+                                   ? transport_step<false, true>(ph, p.scene, p.rng, p.cfg, zero, sensor, nullptr)
+                                   : transport_step<false, false>(ph, p.scene, p.rng, p.cfg, zero, sensor, nullptr);
             ++ph.step;
             float* o = out + 16 * n;
             o[0] = ph.pos.x;

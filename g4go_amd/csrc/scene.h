@@ -2,8 +2,10 @@
 //
 // Reference model: include/g4go/optical/Scene.hpp:61-164 and the device copy
 // in src/optical/optix/OptiXDeviceData.cuh:68-144.  Differences:
-//   * geometry is an axis-aligned box per volume (analytic slab tests replace
-//     the OptiX triangle traversal), with a parent relation for nesting;
+//   * a volume's solid is an axis-aligned box (analytic slab tests) or a
+//     closed convex triangle mesh held as the planes of its faces (clipping
+//     tests), in place of the OptiX triangle traversal, with a parent
+//     relation for nesting;
 //   * every property table lives in one float pool (energies then values),
 //     addressed by offset, so the whole scene is a handful of arrays;
 //   * the border/skin surface of every static box face, and of every
@@ -64,7 +66,9 @@ struct alignas(16) Volume {
     uint32_t daughter_begin, daughter_count;  // indices into the daughter list
     uint32_t entry_surface;                   // surface crossed when entering from the parent
     uint32_t bvh_root;                        // root node of the daughter BVH, kInvalid = linear list
-    uint32_t pad[2];
+    // convex mesh solid: planes [plane_begin, plane_begin + plane_count) of
+    // the scene's plane list, lo/hi its bounding box; plane_count 0: the box lo/hi
+    uint32_t plane_begin, plane_count;
     // face = 2*axis + (outward sign > 0); the neighbour and its surface sit
     // side by side so a step reads both with one 8-byte load
     Face faces[6];
@@ -73,6 +77,28 @@ struct alignas(16) Volume {
 // Records staged into LDS are copied in 16-byte words.
 static_assert(sizeof(Material) % 16 == 0 && sizeof(Surface) % 16 == 0 && sizeof(Volume) % 16 == 0,
               "scene records are whole 16-byte words");
+
+// Face plane of a convex mesh solid (world frame): the solid holds the points
+// x with dot(n, x) <= h for every plane of its volume.  `shade` is the unit
+// normal the boundary physics uses on the face (the caller's triangle normal,
+// else n), turned to face the photon at run time.  The first 16 bytes are all
+// a clipping test reads.
+//
+// Across the face: the host lists the siblings whose boxes reach the points
+// just outside the face (the face offset by the boundary tolerance, with a
+// margin).  None: the parent is across (surface_parent).  One: that sibling
+// (daughter-list slot `across`, surface_across) when it holds the point, else
+// the parent; the same answer as testing every sibling.  More: kFaceDynamic,
+// the full point query.
+struct alignas(16) Plane {
+    float n[3];
+    float h;
+    float shade[3];
+    uint32_t across;  // kInvalid, a daughter-list slot, or kFaceDynamic
+    uint32_t surface_parent, surface_across;
+    uint32_t pad[2];
+};
+constexpr uint32_t kMaxMeshPlanes = 1024;  // faces per mesh solid
 
 struct Binding {
     uint32_t from, to, surface, pad;  // sorted by (from, to)
@@ -96,7 +122,7 @@ struct DaughterBox {
     float lo[3];
     uint32_t volume;
     float hi[3];
-    uint32_t pad;
+    uint32_t mesh;  // nonzero: the daughter is a convex mesh within this box
 };
 
 struct SceneView {
@@ -104,18 +130,19 @@ struct SceneView {
     const Material* materials;
     const Surface* surfaces;
     const Volume* volumes;
-    const uint32_t* daughters;
+    const Plane* planes;  // faces of the convex mesh solids
     const Binding* bindings;
     const BvhNode* bvh;
-    const DaughterBox* dbox;  // one per daughter-list slot
+    const DaughterBox* dbox;  // one per daughter-list slot (volume index and box)
     uint32_t n_materials, n_surfaces, n_volumes, n_bindings;
     uint32_t n_bvh_nodes, world, n_daughters;
     // The energy grid shared by the most non-constant tables (energies at
     // pool[hot_offset], hot_count nodes; hot_count = 0: none).  A photon's
     // energy never changes, so its node on this grid is located once.
     uint32_t hot_offset, hot_count;
-    uint32_t n_pool;  // floats in the pool
-    uint32_t pad[2];
+    uint32_t n_pool;    // floats in the pool
+    uint32_t n_planes;  // 0: every solid is a box
+    uint32_t pad;
 };
 
 }  // namespace g4go

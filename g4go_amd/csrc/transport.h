@@ -373,6 +373,93 @@ G4GO_HD bool box_contains(const Volume& v, V3 q) {
     return q.x >= v.lo[0] && q.x <= v.hi[0] && q.y >= v.lo[1] && q.y <= v.hi[1] && q.z >= v.lo[2] && q.z <= v.hi[2];
 }
 
+// ---------------------------------------------------------------------------
+// Geometry: closed convex mesh solids, held as the planes of their faces.  A
+// ray from inside leaves through the plane with the smallest exit distance
+// among the planes it moves towards (n.d > 0); a ray from outside enters at
+// the largest entry distance among the planes it moves against (n.d < 0) when
+// that stays below the smallest exit distance (Cyrus-Beck clipping).  For a
+// closed convex mesh both are the closest triangle hit of the reference's
+// trace (OptiXTransportDevice.cu:105-196, 728-741) up to rounding at edges.
+// Distances are compared as cross products, with one division per call.
+// ---------------------------------------------------------------------------
+G4GO_HD float4 load_plane(const Plane& pl) {
+    float4 q;
+    __builtin_memcpy(&q, &pl, sizeof(q));
+    return q;
+}
+
+// Exit plane and distance from p inside the solid; plane = kInvalid and
+// dist = kInf when no plane faces away from d.
+G4GO_HD void mesh_exit(const Plane* planes, uint32_t begin, uint32_t count, V3 p, V3 d, float& dist,
+                       uint32_t& plane) {
+    float bn = 1.0f, ba = 0.0f;  // best distance bn / ba; ba = 0 until a plane qualifies
+    uint32_t best = kInvalid;
+    for (uint32_t k = begin; k < begin + count; ++k) {
+        const float4 q = load_plane(planes[k]);
+        const V3 n = {q.x, q.y, q.z};
+        const float nd = dot(n, d);
+        if (nd > 0.0f) {
+            const float num = fmaxf(q.w - dot(n, p), 0.0f);
+            if (num * ba < bn * nd) {
+                best = k;
+                bn = num;
+                ba = nd;
+            }
+        }
+    }
+    plane = best;
+    dist = best == kInvalid ? kInf : bn / ba;
+}
+
+// Entry distance into the solid from p outside it, kInf on a miss or when the
+// entry is closer than tmin.
+G4GO_HD float mesh_entry(const Plane* planes, uint32_t begin, uint32_t count, V3 p, V3 d, float tmin,
+                         uint32_t& plane) {
+    float a = -1.0f, b = 0.0f;  // entry a / b over n.d < 0 (b = -n.d > 0)
+    float c = 1.0f, e = 0.0f;   // exit c / e over n.d > 0
+    uint32_t best = kInvalid;
+    for (uint32_t k = begin; k < begin + count; ++k) {
+        const float4 q = load_plane(planes[k]);
+        const V3 n = {q.x, q.y, q.z};
+        const float nd = dot(n, d);
+        const float num = q.w - dot(n, p);
+        if (nd < 0.0f) {
+            if (-num * b > a * -nd) {
+                best = k;
+                a = -num;
+                b = -nd;
+            }
+        } else if (nd > 0.0f) {
+            if (num * e < c * nd) {
+                c = num;
+                e = nd;
+            }
+        } else if (num < 0.0f) {
+            return kInf;  // parallel to a face, outside its plane
+        }
+    }
+    if (best == kInvalid) {
+        return kInf;
+    }
+    const float t_near = a / b;
+    if (!(t_near >= tmin) || (e != 0.0f && t_near > c / e)) {
+        return kInf;
+    }
+    plane = best;
+    return t_near;
+}
+
+G4GO_HD bool mesh_contains(const Plane* planes, uint32_t begin, uint32_t count, V3 q) {
+    for (uint32_t k = begin; k < begin + count; ++k) {
+        const float4 pl = load_plane(planes[k]);
+        if (dot(V3{pl.x, pl.y, pl.z}, q) > pl.w) {
+            return false;
+        }
+    }
+    return true;
+}
+
 // Border binding lookup with skin precedence (OptiXTransportDevice.cu:218-252).
 G4GO_HD uint32_t find_surface(const SceneView& sc, uint32_t from, uint32_t to) {
     uint32_t first = 0, last = sc.n_bindings;
@@ -399,9 +486,24 @@ G4GO_HD uint32_t find_surface(const SceneView& sc, uint32_t from, uint32_t to) {
     return f.skin != kInvalid ? f.skin : t.skin;
 }
 
+// Entry distance into one daughter: its box, or with kMesh its mesh when it
+// has one (plane = the entered face, else kInvalid).
+template <bool kMesh>
+G4GO_HD float daughter_entry(const SceneView& sc, const DaughterBox& b, V3 p, V3 d, V3 inv, float tmin, int& axis,
+                             uint32_t& plane) {
+    if (kMesh && b.mesh != 0u) {
+        const Volume& v = sc.volumes[b.volume];
+        return mesh_entry(sc.planes, v.plane_begin, v.plane_count, p, d, tmin, plane);
+    }
+    plane = kInvalid;
+    return box_entry(b.lo, b.hi, p, inv, tmin, axis);
+}
+
 // Nearest daughter entry of volume `vol` (BVH walk when the volume has one).
+// `plane` (kMesh only) is the entered mesh face, kInvalid for a box face.
+template <bool kMesh>
 G4GO_HD float nearest_daughter(const SceneView& sc, const Volume& vol, V3 p, V3 d, float tmin, uint32_t& which,
-                               int& axis) {
+                               int& axis, uint32_t& plane) {
     const V3 inv = {1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
     float best = kInf;
     which = kInvalid;
@@ -410,11 +512,13 @@ G4GO_HD float nearest_daughter(const SceneView& sc, const Volume& vol, V3 p, V3 
             const DaughterBox& b = sc.dbox[vol.daughter_begin + k];
             const uint32_t j = b.volume;
             int a = 0;
-            const float t = box_entry(b.lo, b.hi, p, inv, tmin, a);
+            uint32_t f = kInvalid;
+            const float t = daughter_entry<kMesh>(sc, b, p, d, inv, tmin, a, f);
             if (t < best || (t == best && t < kInf && j < which)) {  // ties: lower volume index
                 best = t;
                 which = j;
                 axis = a;
+                plane = f;
             }
         }
         return best;
@@ -448,11 +552,13 @@ G4GO_HD float nearest_daughter(const SceneView& sc, const Volume& vol, V3 p, V3 
                 const DaughterBox& b = sc.dbox[node.left + k];
                 const uint32_t j = b.volume;
                 int a = 0;
-                const float t = box_entry(b.lo, b.hi, p, inv, tmin, a);
+                uint32_t f = kInvalid;
+                const float t = daughter_entry<kMesh>(sc, b, p, d, inv, tmin, a, f);
                 if (t < best || (t == best && t < kInf && j < which)) {  // ties: lower volume index
                     best = t;
                     which = j;
                     axis = a;
+                    plane = f;
                 }
             }
         } else {
@@ -479,8 +585,9 @@ G4GO_HD float nearest_daughter(const SceneView& sc, const Volume& vol, V3 p, V3 
     return best;
 }
 
-// Neighbour across a face found at run time: the first sibling box that holds
-// the point just outside the face, else the parent volume.
+// Neighbour across a face found at run time: the first sibling solid that
+// holds the point just outside the face, else the parent volume.
+template <bool kMesh>
 G4GO_HD uint32_t point_neighbor(const SceneView& sc, uint32_t current, V3 q) {
     const Volume& cur = sc.volumes[current];
     if (cur.parent == kInvalid) {
@@ -491,6 +598,12 @@ G4GO_HD uint32_t point_neighbor(const SceneView& sc, uint32_t current, V3 q) {
         const DaughterBox& b = sc.dbox[par.daughter_begin + k];
         if (b.volume != current && q.x >= b.lo[0] && q.x <= b.hi[0] && q.y >= b.lo[1] && q.y <= b.hi[1] &&
             q.z >= b.lo[2] && q.z <= b.hi[2]) {
+            if (kMesh && b.mesh != 0u) {
+                const Volume& v = sc.volumes[b.volume];
+                if (!mesh_contains(sc.planes, v.plane_begin, v.plane_count, q)) {
+                    continue;
+                }
+            }
             return b.volume;
         }
     }
@@ -808,7 +921,12 @@ struct StepConfig {
     float eps;  // boundary tolerance (mm)
 };
 
-template <bool kTrace>
+// kMesh: the scene has convex mesh solids (SceneView::n_planes != 0).  Their
+// faces are found by clipping, the photon is not snapped onto a mesh face,
+// and a daughter entered up to eps beyond the current volume's exit wins (a
+// daughter face coincident with the wall, which the reference settles by its
+// surface rank re-trace, OptiXTransportDevice.cu:105-127, 742-777).
+template <bool kTrace, bool kMesh = false>
 G4GO_HD int transport_step(Photon& ph, const SceneView& sc, const Rng& rng, const StepConfig& cfg, bool& zero_step,
                            uint32_t& sensor_out, StepTrace* trace) {
     zero_step = false;
@@ -829,20 +947,28 @@ G4GO_HD int transport_step(Photon& ph, const SceneView& sc, const Rng& rng, cons
         trace->event = 0;
     }
 
-    // nearest boundary: leave the box, or enter a daughter
+    // nearest boundary: leave the solid, or enter a daughter; `plane` is the
+    // mesh face crossed (kMesh only), kInvalid for a box face
     float dist;
-    int axis;
-    box_exit(vol, ph.pos, ph.dir, dist, axis);
+    int axis = 0;
+    uint32_t plane = kInvalid;
+    if (kMesh && vol.plane_count != 0u) {
+        mesh_exit(sc.planes, vol.plane_begin, vol.plane_count, ph.pos, ph.dir, dist, plane);
+    } else {
+        box_exit(vol, ph.pos, ph.dir, dist, axis);
+    }
     uint32_t next = kInvalid;
     uint32_t surface = kInvalid;
     bool entering = false;
     if (vol.daughter_count != 0) {
         uint32_t which;
         int a2 = 0;
-        const float t = nearest_daughter(sc, vol, ph.pos, ph.dir, cfg.eps, which, a2);
-        if (which != kInvalid && t <= dist) {
+        uint32_t p2 = kInvalid;
+        const float t = nearest_daughter<kMesh>(sc, vol, ph.pos, ph.dir, cfg.eps, which, a2, p2);
+        if (which != kInvalid && (kMesh ? t <= dist + cfg.eps : t <= dist)) {
             dist = t;
             axis = a2;
+            plane = p2;
             next = which;
             surface = sc.volumes[which].entry_surface;
             entering = true;
@@ -881,35 +1007,67 @@ G4GO_HD int transport_step(Photon& ph, const SceneView& sc, const Rng& rng, cons
         trace->segment = dist;
     }
     V3 bpos = add(ph.pos, scale(ph.dir, dist));
+    V3 normal;
+    const bool mesh_face = kMesh && plane != kInvalid;
     const float dsign = axis_of(ph.dir, axis) > 0.0f ? 1.0f : -1.0f;
-    // snap onto the face plane
-    {
+    if (mesh_face) {
+        // the face's boundary normal, turned to face the incoming photon (reference :789-791)
+        const Plane& pl = sc.planes[plane];
+        const V3 sn = {pl.shade[0], pl.shade[1], pl.shade[2]};
+        normal = dot(ph.dir, sn) > 0.0f ? scale(sn, -1.0f) : sn;
+    } else {
+        // snap onto the face plane
         const Volume& face_box = entering ? sc.volumes[next] : vol;
-        const float plane = entering ? (dsign > 0.0f ? face_box.lo[axis] : face_box.hi[axis])
-                                     : (dsign > 0.0f ? face_box.hi[axis] : face_box.lo[axis]);
+        const float fplane = entering ? (dsign > 0.0f ? face_box.lo[axis] : face_box.hi[axis])
+                                      : (dsign > 0.0f ? face_box.hi[axis] : face_box.lo[axis]);
         if (axis == 0) {
-            bpos.x = plane;
+            bpos.x = fplane;
         } else if (axis == 1) {
-            bpos.y = plane;
+            bpos.y = fplane;
         } else {
-            bpos.z = plane;
+            bpos.z = fplane;
         }
+        // normal facing the incoming photon (reference :789-791)
+        normal = axis == 0 ? V3{-dsign, 0.0f, 0.0f} : (axis == 1 ? V3{0.0f, -dsign, 0.0f} : V3{0.0f, 0.0f, -dsign});
     }
     ph.pos = bpos;
-    // normal facing the incoming photon (reference :789-791)
-    const V3 normal = axis == 0 ? V3{-dsign, 0.0f, 0.0f} : (axis == 1 ? V3{0.0f, -dsign, 0.0f} : V3{0.0f, 0.0f, -dsign});
 
     if (!entering) {
-        const int face = 2 * axis + (dsign > 0.0f ? 1 : 0);
-        Face fr;
-        __builtin_memcpy(&fr, &vol.faces[face], sizeof(Face));
-        next = fr.neighbor;
-        if (next == kFaceDynamic) {
-            const V3 q = add(bpos, scale(normal, -cfg.eps));
-            next = point_neighbor(sc, ph.volume, q);
-            surface = next == kInvalid ? kInvalid : find_surface(sc, ph.volume, next);
+        if (mesh_face) {
+            // the point just outside the face, along its outward plane normal
+            const Plane& pl = sc.planes[plane];
+            const V3 q = add(bpos, scale(V3{pl.n[0], pl.n[1], pl.n[2]}, cfg.eps));
+            const uint32_t across = pl.across;
+            if (across == kFaceDynamic) {
+                next = point_neighbor<kMesh>(sc, ph.volume, q);
+                surface = next == kInvalid ? kInvalid : find_surface(sc, ph.volume, next);
+            } else {
+                next = vol.parent;
+                surface = pl.surface_parent;
+                if (across != kInvalid) {
+                    const DaughterBox& b = sc.dbox[across];
+                    if (q.x >= b.lo[0] && q.x <= b.hi[0] && q.y >= b.lo[1] && q.y <= b.hi[1] && q.z >= b.lo[2] &&
+                        q.z <= b.hi[2]) {
+                        const Volume& v = sc.volumes[b.volume];
+                        if (b.mesh == 0u || mesh_contains(sc.planes, v.plane_begin, v.plane_count, q)) {
+                            next = b.volume;
+                            surface = pl.surface_across;
+                        }
+                    }
+                }
+            }
         } else {
-            surface = fr.surface;
+            const int face = 2 * axis + (dsign > 0.0f ? 1 : 0);
+            Face fr;
+            __builtin_memcpy(&fr, &vol.faces[face], sizeof(Face));
+            next = fr.neighbor;
+            if (next == kFaceDynamic) {
+                const V3 q = add(bpos, scale(normal, -cfg.eps));
+                next = point_neighbor<kMesh>(sc, ph.volume, q);
+                surface = next == kInvalid ? kInvalid : find_surface(sc, ph.volume, next);
+            } else {
+                surface = fr.surface;
+            }
         }
         if (next == kInvalid) {
             return ph.volume == sc.world ? kEscaped : kInvalidState;
@@ -981,7 +1139,10 @@ G4GO_HD int transport_step(Photon& ph, const SceneView& sc, const Rng& rng, cons
             }
         } else {
             reflected = true;
-            if (s.finish == kPolished) {
+            if (s.finish == kPolished && mesh_face) {
+                dir = reflect_direction(dir, normal);
+                pol = reflect_polarization(pol, normal, dir);
+            } else if (s.finish == kPolished) {
                 // specular reflection at the axis-aligned face
                 dir = negate_axis(dir, axis);
                 pol = negate_off_axis(pol, axis);

@@ -1,8 +1,10 @@
 """Scene model mirroring G4GO's backend-neutral optical scene.
 
 Reference: include/g4go/optical/Scene.hpp:61-164 and src/optical/Scene.cpp.
-Volumes carry an axis-aligned box solid (centre, half lengths in mm) instead
-of a triangle mesh, and materials may carry a Rayleigh scattering length.
+A volume's solid is an axis-aligned box (centre, half lengths in mm) or a
+closed convex triangle mesh in world coordinates (`Volume.mesh`, the
+reference's MeshGeometry placed by its Transform), and materials may carry a
+Rayleigh scattering length.
 """
 from __future__ import annotations
 
@@ -86,6 +88,64 @@ class Surface:
 
 
 @dataclass
+class Mesh:
+    """Closed convex triangle mesh in world coordinates (MeshGeometry, Scene.hpp:94-100).
+
+    vertices_mm: (n, 3) float64; indices: (m, 3) uint32; triangle_normals:
+    optional (m, 3) float64 boundary normals (else each triangle's plane
+    normal).  The winding does not matter.
+    """
+    vertices_mm: np.ndarray
+    indices: np.ndarray
+    triangle_normals: Optional[np.ndarray] = None
+
+    def __post_init__(self) -> None:
+        self.vertices_mm = np.ascontiguousarray(self.vertices_mm, dtype=np.float64).reshape(-1, 3)
+        self.indices = np.ascontiguousarray(self.indices, dtype=np.uint32).reshape(-1, 3)
+        if self.triangle_normals is not None:
+            self.triangle_normals = np.ascontiguousarray(self.triangle_normals, dtype=np.float64).reshape(-1, 3)
+
+    def transformed(self, rotation=None, translation_mm=(0.0, 0.0, 0.0)) -> "Mesh":
+        """The mesh placed by a Transform (Scene.hpp:42-59): x -> R x + t."""
+        r = np.eye(3) if rotation is None else np.asarray(rotation, dtype=np.float64).reshape(3, 3)
+        t = np.asarray(translation_mm, dtype=np.float64)
+        normals = None if self.triangle_normals is None else self.triangle_normals @ r.T
+        return Mesh(self.vertices_mm @ r.T + t, self.indices.copy(), normals)
+
+
+def frustum_mesh(front_half_xy: Sequence[float], back_half_xy: Sequence[float], half_z: float) -> Mesh:
+    """Frustum of a rectangular pyramid, 12 triangles (two per face): the
+    rectangle front_half_xy at z = -half_z, back_half_xy at z = +half_z."""
+    (fx, fy), (bx, by), hz = (float(x) for x in front_half_xy), (float(x) for x in back_half_xy), float(half_z)
+    v = np.array([[sx * (fx if z < 0 else bx), sy * (fy if z < 0 else by), z]
+                  for z in (-hz, hz) for sy in (-1.0, 1.0) for sx in (-1.0, 1.0)])
+    quads = [(0, 2, 3, 1), (4, 5, 7, 6), (0, 1, 5, 4), (2, 6, 7, 3), (0, 4, 6, 2), (1, 3, 7, 5)]
+    tri = [(a, b, c) for a, b, c, d in quads] + [(a, c, d) for a, b, c, d in quads]
+    return Mesh(v, np.array(tri))
+
+
+def box_mesh(half_mm: Sequence[float]) -> Mesh:
+    """The 12-triangle mesh of the box [-h, h] (two triangles per face)."""
+    return frustum_mesh(half_mm[:2], half_mm[:2], half_mm[2])
+
+
+def prism_mesh(polygon_xy: Sequence[Sequence[float]], z_lo: float, z_hi: float) -> Mesh:
+    """Right prism over a convex polygon (counter-clockwise xy vertices) between z_lo and z_hi."""
+    poly = np.asarray(polygon_xy, dtype=np.float64)
+    n = len(poly)
+    v = np.concatenate([np.c_[poly, np.full(n, z_lo)], np.c_[poly, np.full(n, z_hi)]])
+    tri = []
+    for i in range(1, n - 1):
+        tri.append((0, i + 1, i))          # bottom
+        tri.append((n, n + i, n + i + 1))  # top
+    for i in range(n):
+        j = (i + 1) % n
+        tri.append((i, j, n + j))
+        tri.append((i, n + j, n + i))
+    return Mesh(v, np.array(tri))
+
+
+@dataclass
 class Volume:
     name: str = ""
     volume_id: int = INVALID_ID
@@ -95,6 +155,7 @@ class Volume:
     sensor_id: int = INVALID_ID
     center_mm: Sequence[float] = (0.0, 0.0, 0.0)
     half_mm: Sequence[float] = (1.0, 1.0, 1.0)
+    mesh: Optional[Mesh] = None  # the solid when set (centre and half lengths are then ignored)
 
 
 @dataclass
@@ -200,7 +261,19 @@ class Scene:
             binds[i].from_volume_id = b.from_volume_id
             binds[i].to_volume_id = b.to_volume_id
             binds[i].surface_id = b.surface_id
-        keep.extend([mats, surfs, vols, binds])
+        meshed = [v for v in self.volumes if v.mesh is not None]
+        meshes = (abi.VolumeMesh * max(1, len(meshed)))()
+        for i, v in enumerate(meshed):
+            m = v.mesh
+            keep.extend([m.vertices_mm, m.indices, m.triangle_normals])
+            meshes[i].volume_id = v.volume_id
+            meshes[i].n_vertices = len(m.vertices_mm)
+            meshes[i].vertices_mm = m.vertices_mm.ctypes.data_as(C.POINTER(C.c_double))
+            meshes[i].indices = m.indices.ctypes.data_as(C.POINTER(C.c_uint32))
+            meshes[i].n_triangles = len(m.indices)
+            if m.triangle_normals is not None:
+                meshes[i].triangle_normals = m.triangle_normals.ctypes.data_as(C.POINTER(C.c_double))
+        keep.extend([mats, surfs, vols, binds, meshes])
         desc = abi.Scene(mats, len(self.materials), surfs, len(self.surfaces), vols, len(self.volumes), binds,
-                         len(self.surface_bindings), self.world_volume_id)
+                         len(self.surface_bindings), self.world_volume_id, meshes, len(meshed))
         return desc, keep

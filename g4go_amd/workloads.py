@@ -13,7 +13,11 @@
   polarisation, wavelength N(420, 20) nm, emission time Exp(40 ns));
 * emission records of 1 MeV electrons on the crystal axis, synthesised by a
   seeded generator (Geant4 is not available to produce them);
-* log-normal event sizes for the batched workloads.
+* log-normal event sizes for the batched workloads;
+* mesh solids: the crystal tapered from a 3x3 mm front face to a 4x4 mm back
+  face (a frustum given as a 12-triangle convex mesh, SiPM box on the back
+  face), and the single-crystal scene turned by a rotation (crystal and SiPM
+  as box meshes).
 """
 from __future__ import annotations
 
@@ -23,7 +27,7 @@ import numpy as np
 
 from . import abi
 from .scene import (INVALID_ID, Material, PropertyTable, Scene, Surface, SurfaceBinding, SurfaceFinish,
-                    SurfaceModel, SurfaceType, Volume)
+                    SurfaceModel, SurfaceType, Volume, box_mesh, frustum_mesh)
 
 HC_EV_NM = 1239.84198
 N_NODES = 32
@@ -160,6 +164,63 @@ def array_scene(p: CrystalParams | None = None, n: int = 8, pitch_mm: float = 3.
             sipms.append(s)
             mats.append(m)
     return CrystalScene(scene, world, crystals, sipms, mats, wrap, cathode, p)
+
+
+def tapered_crystal_scene(p: CrystalParams | None = None, back_half_xy=(2.0, 2.0), abs_nodes=None,
+                          pde_nodes=None) -> CrystalScene:
+    """The wrapped crystal tapered from its 3x3 mm front face (z = -10 mm) to a
+    back face of half sizes back_half_xy (z = +10 mm): a frustum mesh, its
+    four side faces tilted.  The SiPM is a box covering the back face.  The
+    crystal's half_mm stays the front face's (the synthetic source box, which
+    the frustum contains)."""
+    p = p or CrystalParams()
+    scene = Scene()
+    air, silicon, wrap, cathode = _common_materials_and_surfaces(scene, p, pde_nodes)
+    crystal_mat = scene.add_material(_crystal_material(p, abs_nodes))
+    world = scene.add_volume(Volume(name="World", material_id=air, center_mm=(0, 0, 0), half_mm=(100, 100, 100)))
+    scene.world_volume_id = world
+    hz = p.half_mm[2]
+    crystal = scene.add_volume(Volume(name="Crystal", material_id=crystal_mat, parent_volume_id=world,
+                                      center_mm=(0, 0, 0), half_mm=p.half_mm,
+                                      mesh=frustum_mesh(p.half_mm[:2], back_half_xy, hz)))
+    sipm = scene.add_volume(Volume(name="SiPM", material_id=silicon, parent_volume_id=world,
+                                   skin_surface_id=cathode, sensor_id=0,
+                                   center_mm=(0, 0, hz + 0.5 * p.sipm_thickness_mm),
+                                   half_mm=(back_half_xy[0], back_half_xy[1], 0.5 * p.sipm_thickness_mm)))
+    scene.add_surface_binding(SurfaceBinding(crystal, world, wrap))
+    return CrystalScene(scene, world, [crystal], [sipm], [crystal_mat], wrap, cathode, p)
+
+
+def rotated_crystal_scene(rotation, p: CrystalParams | None = None, abs_nodes=None, pde_nodes=None) -> CrystalScene:
+    """crystal_scene with the crystal and its SiPM as box meshes turned by the
+    3x3 `rotation` about the origin (the crystal's centre)."""
+    cs = crystal_scene(p, abs_nodes, pde_nodes)
+    r = np.asarray(rotation, dtype=np.float64).reshape(3, 3)
+    for vid in (cs.crystal_ids[0], cs.sipm_ids[0]):
+        v = cs.scene.volumes[vid]
+        v.mesh = box_mesh(v.half_mm).transformed(r, r @ np.asarray(v.center_mm, dtype=np.float64))
+    return cs
+
+
+def rotation_matrix(axis, angle_rad: float) -> np.ndarray:
+    """Rotation by angle_rad about `axis` (Rodrigues)."""
+    k = np.asarray(axis, dtype=np.float64)
+    k = k / np.linalg.norm(k)
+    kx = np.array([[0.0, -k[2], k[1]], [k[2], 0.0, -k[0]], [-k[1], k[0], 0.0]])
+    return np.eye(3) + np.sin(angle_rad) * kx + (1.0 - np.cos(angle_rad)) * (kx @ kx)
+
+
+def point_source(volume_id: int, point_mm, mean_nm: float = 420.0, sigma_nm: float = 20.0,
+                 decay_ns: float = 40.0) -> abi.SyntheticSource:
+    """Synthetic photons from one point (an empty source box)."""
+    src = abi.SyntheticSource()
+    src.lo_mm[:] = [float(x) for x in point_mm]
+    src.hi_mm[:] = [float(x) for x in point_mm]
+    src.wavelength_mean_nm = mean_nm
+    src.wavelength_sigma_nm = sigma_nm
+    src.decay_time_ns = decay_ns
+    src.volume_id = volume_id
+    return src
 
 
 def synthetic_source(cs: CrystalScene, crystal_index: int = 0, mean_nm: float = 420.0, sigma_nm: float = 20.0,

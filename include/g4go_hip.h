@@ -107,8 +107,10 @@ typedef struct g4go_surface {
     g4go_property specular_lobe, specular_spike, backscatter, surface_roughness;
 } g4go_surface;
 
-/* Volume (Scene.hpp:107-120) with an axis-aligned box solid in world
- * coordinates (centre and half lengths, mm) in place of a triangle mesh. */
+/* Volume (Scene.hpp:107-120).  Its solid is an axis-aligned box in world
+ * coordinates (centre and half lengths, mm), unless a g4go_volume_mesh of
+ * the scene names the volume: then the mesh is its solid and centre and half
+ * lengths are ignored. */
 typedef struct g4go_volume {
     uint32_t volume_id;
     uint32_t material_id;
@@ -123,6 +125,28 @@ typedef struct g4go_surface_binding { /* SurfaceBinding (Scene.hpp:122-126) */
     uint32_t from_volume_id, to_volume_id, surface_id;
 } g4go_surface_binding;
 
+/* Closed convex triangle mesh of one volume, in world coordinates: the
+ * volume's MeshGeometry (Scene.hpp:94-100) mapped through its Transform
+ * (Scene.hpp:42-59), as the reference places it with an instance transform
+ * (OptiXTransportHost.cpp:1221-1262).  The winding does not matter (the
+ * boundary normal is turned to face the photon, OptiXTransportDevice.cu:
+ * 789-791).  Triangles lying in one plane form one face.  Rejected with
+ * G4GO_ERROR_INVALID_ARGUMENT: fewer than four triangles, an index out of
+ * range, a degenerate triangle, an edge not shared by exactly two triangles
+ * (an open mesh), a vertex outside the plane of a triangle by more than
+ * 1e-6 of the mesh size (a concave mesh), more than 1024 faces. */
+typedef struct g4go_volume_mesh {
+    uint32_t volume_id;
+    uint32_t n_vertices;
+    const double* vertices_mm;      /* [n_vertices][3] */
+    const uint32_t* indices;        /* [n_triangles][3] */
+    uint32_t n_triangles;
+    uint32_t reserved;              /* 0 */
+    const double* triangle_normals; /* [n_triangles][3] boundary normals (triangleNormals,
+                                     * CreateTriangleNormals, OptiXTransportHost.cpp:1084-1116),
+                                     * or NULL: the normal of each triangle's plane */
+} g4go_volume_mesh;
+
 typedef struct g4go_scene {
     const g4go_material* materials;
     uint32_t n_materials;
@@ -133,6 +157,8 @@ typedef struct g4go_scene {
     const g4go_surface_binding* bindings;
     uint32_t n_bindings;
     uint32_t world_volume_id;
+    const g4go_volume_mesh* meshes; /* at most one per volume; NULL when n_meshes is 0 */
+    uint32_t n_meshes;
 } g4go_scene;
 
 /* OpticalEmission, 128 bytes, layout-identical to

@@ -22,6 +22,9 @@
  *   or_find_surface              OptiXTransportDevice.cu:218-252, src/optical/Scene.cpp:120-150
  *   or_facet / or_unified ...    OptiXTransportDevice.cu:492-595
  *   or_generate_emission         OptiXTransportDevice.cu:346-481
+ *   mesh_planes / mesh_exit ..   convex mesh solids as face planes, in place of the
+ *                                closest triangle hit, OptiXTransportDevice.cu:105-196,
+ *                                triangles from OptiXTransportHost.cpp:1084-1200
  *   or_step                      OptiXTransportDevice.cu:708-931 (bounce loop body)
  *   or_run                       OptiXTransportDevice.cu:660-941 + the batch
  *                                statistics of OptiXTransportHost.cpp:809-844
@@ -30,8 +33,9 @@
  *
  * Deliberate differences from the reference, shared with the HIP backend:
  * the draw map (four draws per Philox block, see physics.h), box solids with
- * analytic faces instead of OptiX triangles, no epsilon push-off (the photon
- * is snapped onto the face plane), the competing-exponential bulk step
+ * analytic faces and convex mesh solids as face planes instead of OptiX
+ * triangles, no epsilon push-off (the photon is snapped onto a box face
+ * plane), the competing-exponential bulk step
  * (absorption + Rayleigh), FP64 time, sin/cos of 2*pi*u by turns.
  */
 #include <math.h>
@@ -464,9 +468,15 @@ typedef struct {
 } surface_t;
 
 typedef struct {
-    float lo[3], hi[3]; /* box bounds are FP32 in both builds (as uploaded) */
+    float lo[3], hi[3]; /* box bounds (a mesh's bounding box) are FP32 in both builds (as uploaded) */
     uint32_t id, material, parent, skin, sensor;
+    uint32_t plane_begin, plane_count; /* convex mesh solid; plane_count 0: the box */
 } volume_t;
+
+/* Face plane of a convex mesh solid: inside where n.x <= h; s = boundary normal. */
+typedef struct {
+    real n[3], h, s[3];
+} plane_t;
 
 typedef struct {
     uint32_t n_mat, n_surf, n_vol, n_bind, world;
@@ -474,6 +484,8 @@ typedef struct {
     surface_t* surf;
     volume_t* vol;
     uint32_t* bind; /* triples (from, to, surface) in input order */
+    plane_t* planes;
+    uint32_t n_planes;
 } oscene_t;
 
 static prop_t make_prop(const g4go_property* p) {
@@ -552,6 +564,101 @@ static uint32_t vol_index(const oscene_t* s, uint32_t id) {
     return INVALID;
 }
 
+/* Face planes of a closed convex mesh (OptiXTransportHost.cpp:1084-1200 builds
+ * the reference's triangles; here their planes): per triangle, in FP64, the
+ * unit normal of (v1 - v0) x (v2 - v0) and offset u.v0, turned so that the
+ * vertex centroid lies inside; triangles whose normals, offsets (relative to
+ * the largest centroid-vertex coordinate distance) and boundary normals agree
+ * within 1e-9 share the first one's plane.  The mesh is assumed valid
+ * (closed, convex): the library rejects others.  Appends to s->planes and
+ * sets v's plane range and bounding box (rounded outwards to FP32). */
+static void mesh_planes(oscene_t* s, const g4go_volume_mesh* m, volume_t* v) {
+    const uint32_t nv = m->n_vertices, nt = m->n_triangles;
+    const double* x = m->vertices_mm;
+    double c[3] = {0.0, 0.0, 0.0}, lo[3], hi[3], size = 0.0;
+    for (uint32_t i = 0; i < nv; ++i) {
+        for (int k = 0; k < 3; ++k) {
+            c[k] += x[3 * i + k];
+        }
+    }
+    for (int k = 0; k < 3; ++k) {
+        c[k] /= (double)nv;
+        lo[k] = hi[k] = x[k];
+    }
+    for (uint32_t i = 0; i < nv; ++i) {
+        for (int k = 0; k < 3; ++k) {
+            const double d = fabs(x[3 * i + k] - c[k]);
+            size = d > size ? d : size;
+            lo[k] = x[3 * i + k] < lo[k] ? x[3 * i + k] : lo[k];
+            hi[k] = x[3 * i + k] > hi[k] ? x[3 * i + k] : hi[k];
+        }
+    }
+    double(*f64)[7] = (double(*)[7])calloc(nt, sizeof(*f64)); /* u[3], h, s[3] */
+    uint32_t nf = 0;
+    for (uint32_t t = 0; t < nt; ++t) {
+        const double* p0 = x + 3 * m->indices[3 * t];
+        const double* p1 = x + 3 * m->indices[3 * t + 1];
+        const double* p2 = x + 3 * m->indices[3 * t + 2];
+        const double a0 = p1[0] - p0[0], a1 = p1[1] - p0[1], a2 = p1[2] - p0[2];
+        const double b0 = p2[0] - p0[0], b1 = p2[1] - p0[1], b2 = p2[2] - p0[2];
+        const double g0 = a1 * b2 - a2 * b1, g1 = a2 * b0 - a0 * b2, g2 = a0 * b1 - a1 * b0;
+        const double len = sqrt(g0 * g0 + g1 * g1 + g2 * g2);
+        double f[7] = {g0 / len, g1 / len, g2 / len, 0.0, 0.0, 0.0, 0.0};
+        f[3] = f[0] * p0[0] + f[1] * p0[1] + f[2] * p0[2];
+        if (f[0] * c[0] + f[1] * c[1] + f[2] * c[2] > f[3]) {
+            f[0] = -f[0];
+            f[1] = -f[1];
+            f[2] = -f[2];
+            f[3] = -f[3];
+        }
+        if (m->triangle_normals) {
+            const double* n = m->triangle_normals + 3 * t;
+            const double nl = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+            f[4] = n[0] / nl;
+            f[5] = n[1] / nl;
+            f[6] = n[2] / nl;
+        } else {
+            f[4] = f[0];
+            f[5] = f[1];
+            f[6] = f[2];
+        }
+        int dup = 0;
+        for (uint32_t j = 0; j < nf && !dup; ++j) {
+            int same = fabs(f64[j][3] - f[3]) <= 1e-9 * size;
+            for (int k = 0; k < 3; ++k) {
+                same = same && fabs(f64[j][k] - f[k]) <= 1e-9 && fabs(f64[j][4 + k] - f[4 + k]) <= 1e-9;
+            }
+            dup = same;
+        }
+        if (!dup) {
+            memcpy(f64[nf++], f, sizeof(f));
+        }
+    }
+    s->planes = (plane_t*)realloc(s->planes, (s->n_planes + nf) * sizeof(plane_t));
+    v->plane_begin = s->n_planes;
+    v->plane_count = nf;
+    for (uint32_t j = 0; j < nf; ++j) {
+        plane_t* pl = &s->planes[s->n_planes + j];
+        for (int k = 0; k < 3; ++k) {
+            pl->n[k] = (real)f64[j][k];
+            pl->s[k] = (real)f64[j][4 + k];
+        }
+        pl->h = (real)f64[j][3];
+    }
+    s->n_planes += nf;
+    free(f64);
+    for (int k = 0; k < 3; ++k) {
+        v->lo[k] = (float)lo[k];
+        if ((double)v->lo[k] > lo[k]) {
+            v->lo[k] = nextafterf(v->lo[k], -INFINITY);
+        }
+        v->hi[k] = (float)hi[k];
+        if ((double)v->hi[k] < hi[k]) {
+            v->hi[k] = nextafterf(v->hi[k], INFINITY);
+        }
+    }
+}
+
 static oscene_t* build(const g4go_scene* in) {
     oscene_t* s = (oscene_t*)calloc(1, sizeof(oscene_t));
     s->n_mat = in->n_materials;
@@ -607,6 +714,11 @@ static oscene_t* build(const g4go_scene* in) {
         d->material = v->material_id;
         d->skin = v->skin_surface_id;
         d->sensor = v->sensor_id;
+        for (uint32_t j = 0; j < in->n_meshes; ++j) {
+            if (in->meshes[j].volume_id == v->volume_id) {
+                mesh_planes(s, &in->meshes[j], d);
+            }
+        }
     }
     for (uint32_t i = 0; i < s->n_vol; ++i) {
         s->vol[i].parent = in->volumes[i].parent_volume_id == INVALID ? INVALID : vol_index(s, in->volumes[i].parent_volume_id);
@@ -644,6 +756,7 @@ static void destroy(oscene_t* s) {
     free(s->surf);
     free(s->vol);
     free(s->bind);
+    free(s->planes);
     free(s);
 }
 
@@ -821,8 +934,80 @@ static int box_contains(const volume_t* v, v3 q) {
            q.z >= (real)v->lo[2] && q.z <= (real)v->hi[2];
 }
 
-/* Nearest daughter entry by brute force; ties go to the lower index. */
-static real nearest_daughter(const oscene_t* s, uint32_t cur, v3 p, v3 d, real tmin, uint32_t* which, int* axis) {
+/* Convex mesh solids by clipping against their face planes: the exit is the
+ * smallest (h - n.p)/(n.d) over the planes with n.d > 0, the entry the largest
+ * over n.d < 0 if it does not pass the smallest exit; candidates compared by
+ * cross products (first plane kept on ties), one division at the end.  For a
+ * closed convex mesh this is the closest triangle hit of the reference's
+ * trace (OptiXTransportDevice.cu:105-196). */
+static void mesh_exit(const oscene_t* s, const volume_t* v, v3 p, v3 d, real* dist, uint32_t* plane) {
+    real bn = RF(1.0), ba = RF(0.0);
+    uint32_t best = INVALID;
+    for (uint32_t k = v->plane_begin; k < v->plane_begin + v->plane_count; ++k) {
+        const plane_t* pl = &s->planes[k];
+        const v3 n = mk(pl->n[0], pl->n[1], pl->n[2]);
+        const real nd = vdot(n, d);
+        if (nd > RF(0.0)) {
+            const real num = R_FMAX(pl->h - vdot(n, p), RF(0.0));
+            if (num * ba < bn * nd) {
+                best = k;
+                bn = num;
+                ba = nd;
+            }
+        }
+    }
+    *plane = best;
+    *dist = best == INVALID ? R_INF : bn / ba;
+}
+
+static real mesh_entry(const oscene_t* s, const volume_t* v, v3 p, v3 d, real tmin, uint32_t* plane) {
+    real a = RF(-1.0), b = RF(0.0), c = RF(1.0), e = RF(0.0);
+    uint32_t best = INVALID;
+    for (uint32_t k = v->plane_begin; k < v->plane_begin + v->plane_count; ++k) {
+        const plane_t* pl = &s->planes[k];
+        const v3 n = mk(pl->n[0], pl->n[1], pl->n[2]);
+        const real nd = vdot(n, d);
+        const real num = pl->h - vdot(n, p);
+        if (nd < RF(0.0)) {
+            if (-num * b > a * -nd) {
+                best = k;
+                a = -num;
+                b = -nd;
+            }
+        } else if (nd > RF(0.0)) {
+            if (num * e < c * nd) {
+                c = num;
+                e = nd;
+            }
+        } else if (num < RF(0.0)) {
+            return R_INF;
+        }
+    }
+    if (best == INVALID) {
+        return R_INF;
+    }
+    const real t = a / b;
+    if (!(t >= tmin) || (e != RF(0.0) && t > c / e)) {
+        return R_INF;
+    }
+    *plane = best;
+    return t;
+}
+
+static int mesh_contains(const oscene_t* s, const volume_t* v, v3 q) {
+    for (uint32_t k = v->plane_begin; k < v->plane_begin + v->plane_count; ++k) {
+        const plane_t* pl = &s->planes[k];
+        if (vdot(mk(pl->n[0], pl->n[1], pl->n[2]), q) > pl->h) {
+            return 0;
+        }
+    }
+    return 1;
+}
+
+/* Nearest daughter entry by brute force; ties go to the lower index.  *plane
+ * is the entered mesh face, INVALID for a box face. */
+static real nearest_daughter(const oscene_t* s, uint32_t cur, v3 p, v3 d, real tmin, uint32_t* which, int* axis,
+                             uint32_t* plane) {
     const v3 inv = mk(RF(1.0) / d.x, RF(1.0) / d.y, RF(1.0) / d.z);
     real best = R_INF;
     *which = INVALID;
@@ -831,11 +1016,14 @@ static real nearest_daughter(const oscene_t* s, uint32_t cur, v3 p, v3 d, real t
             continue;
         }
         int a = 0;
-        const real t = box_entry(&s->vol[j], p, inv, tmin, &a);
+        uint32_t f = INVALID;
+        const real t = s->vol[j].plane_count ? mesh_entry(s, &s->vol[j], p, d, tmin, &f)
+                                             : box_entry(&s->vol[j], p, inv, tmin, &a);
         if (t < best || (t == best && t < R_INF && j < *which)) {
             best = t;
             *which = j;
             *axis = a;
+            *plane = f;
         }
     }
     return best;
@@ -847,7 +1035,8 @@ static uint32_t point_neighbor(const oscene_t* s, uint32_t cur, v3 q) {
         return INVALID;
     }
     for (uint32_t j = 0; j < s->n_vol; ++j) {
-        if (j != cur && s->vol[j].parent == parent && box_contains(&s->vol[j], q)) {
+        if (j != cur && s->vol[j].parent == parent && box_contains(&s->vol[j], q) &&
+            (s->vol[j].plane_count == 0 || mesh_contains(s, &s->vol[j], q))) {
             return j;
         }
     }
@@ -1162,18 +1351,28 @@ static int or_step(photon_t* ph, const oscene_t* s, const rng_t* r, real eps, in
     tr->material = ph->material;
     tr->surface = INVALID;
     tr->event = 0;
+    /* with mesh solids in the scene: faces by clipping, no snap onto a mesh
+     * face, and a daughter entered up to eps beyond the exit wins (the
+     * reference's coincident-surface rank, OptiXTransportDevice.cu:105-127) */
+    const int meshes = s->n_planes != 0;
     real dist;
-    int axis;
-    box_exit(vol, ph->pos, ph->dir, &dist, &axis);
+    int axis = 0;
+    uint32_t plane = INVALID;
+    if (vol->plane_count) {
+        mesh_exit(s, vol, ph->pos, ph->dir, &dist, &plane);
+    } else {
+        box_exit(vol, ph->pos, ph->dir, &dist, &axis);
+    }
     uint32_t next = INVALID, surface = INVALID;
     int entering = 0;
     {
-        uint32_t which;
+        uint32_t which, p2 = INVALID;
         int a2 = 0;
-        const real t = nearest_daughter(s, ph->volume, ph->pos, ph->dir, eps, &which, &a2);
-        if (which != INVALID && t <= dist) {
+        const real t = nearest_daughter(s, ph->volume, ph->pos, ph->dir, eps, &which, &a2, &p2);
+        if (which != INVALID && (meshes ? t <= dist + eps : t <= dist)) {
             dist = t;
             axis = a2;
+            plane = p2;
             next = which;
             surface = or_find_surface(s, ph->volume, which);
             entering = 1;
@@ -1204,23 +1403,31 @@ static int or_step(photon_t* ph, const oscene_t* s, const rng_t* r, real eps, in
     ph->time += (double)(dist * ph->inv_v);
     tr->segment = dist;
     v3 bpos = vadd(ph->pos, vscale(ph->dir, dist));
-    const real dsign = vaxis(ph->dir, axis) > RF(0.0) ? RF(1.0) : RF(-1.0);
-    {
+    v3 normal;
+    const int mesh_face = plane != INVALID;
+    if (mesh_face) {
+        const plane_t* pl = &s->planes[plane];
+        const v3 sn = mk(pl->s[0], pl->s[1], pl->s[2]);
+        normal = vdot(ph->dir, sn) > RF(0.0) ? vscale(sn, RF(-1.0)) : sn;
+    } else {
+        const real dsign = vaxis(ph->dir, axis) > RF(0.0) ? RF(1.0) : RF(-1.0);
         const volume_t* fb = entering ? &s->vol[next] : vol;
-        const real plane = entering ? (dsign > RF(0.0) ? (real)fb->lo[axis] : (real)fb->hi[axis])
-                                    : (dsign > RF(0.0) ? (real)fb->hi[axis] : (real)fb->lo[axis]);
+        const real fplane = entering ? (dsign > RF(0.0) ? (real)fb->lo[axis] : (real)fb->hi[axis])
+                                     : (dsign > RF(0.0) ? (real)fb->hi[axis] : (real)fb->lo[axis]);
         if (axis == 0) {
-            bpos.x = plane;
+            bpos.x = fplane;
         } else if (axis == 1) {
-            bpos.y = plane;
+            bpos.y = fplane;
         } else {
-            bpos.z = plane;
+            bpos.z = fplane;
         }
+        normal = axis == 0 ? mk(-dsign, RF(0.0), RF(0.0)) : (axis == 1 ? mk(RF(0.0), -dsign, RF(0.0)) : mk(RF(0.0), RF(0.0), -dsign));
     }
     ph->pos = bpos;
-    const v3 normal = axis == 0 ? mk(-dsign, RF(0.0), RF(0.0)) : (axis == 1 ? mk(RF(0.0), -dsign, RF(0.0)) : mk(RF(0.0), RF(0.0), -dsign));
     if (!entering) {
-        const v3 q = vadd(bpos, vscale(normal, -eps));
+        /* the point just outside the face */
+        const v3 q = mesh_face ? vadd(bpos, vscale(mk(s->planes[plane].n[0], s->planes[plane].n[1], s->planes[plane].n[2]), eps))
+                               : vadd(bpos, vscale(normal, -eps));
         next = point_neighbor(s, ph->volume, q);
         if (next == INVALID) {
             return ph->volume == s->world ? S_ESCAPED : S_INVALID;
@@ -1288,7 +1495,10 @@ static int or_step(photon_t* ph, const oscene_t* s, const rng_t* r, real eps, in
             }
         } else {
             reflected = 1;
-            if (sf->finish == 0) {
+            if (sf->finish == 0 && mesh_face) {
+                dir = or_reflect_dir(dir, normal);
+                pol = or_reflect_pol(pol, normal, dir);
+            } else if (sf->finish == 0) {
                 /* specular reflection at the face normal to `axis`: d - 2 (d.n) n
                  * negates d's axis component and -p + 2 (p.n) n negates p's
                  * other two, exactly (sign flips) */
@@ -1533,6 +1743,42 @@ void ORACLE_NAME(box_exit)(const float lo[3], const float hi[3], const real p[3]
         v.hi[k] = hi[k];
     }
     box_exit(&v, mk(p[0], p[1], p[2]), mk(d[0], d[1], d[2]), dist, axis);
+}
+
+/* Mesh solid queries on the volume `volume_id` of a scene, for n rays
+ * (p, d: [n][3]): entry = 0 the exit distance from inside, 1 the entry
+ * distance from outside (tmin as in the transport step); face[i] = the face
+ * (index among the volume's planes, INVALID for none). */
+void ORACLE_NAME(mesh_trace)(const g4go_scene* scene, uint32_t volume_id, const real* p, const real* d, uint64_t n,
+                             real tmin, int entry, real* dist, uint32_t* face) {
+    oscene_t* s = build(scene);
+    const uint32_t vi = vol_index(s, volume_id);
+    const volume_t* v = &s->vol[vi];
+    for (uint64_t i = 0; i < n; ++i) {
+        const v3 pp = mk(p[3 * i], p[3 * i + 1], p[3 * i + 2]);
+        const v3 dd = mk(d[3 * i], d[3 * i + 1], d[3 * i + 2]);
+        uint32_t pl = INVALID;
+        if (entry) {
+            dist[i] = mesh_entry(s, v, pp, dd, tmin, &pl);
+        } else {
+            mesh_exit(s, v, pp, dd, &dist[i], &pl);
+        }
+        face[i] = pl == INVALID ? INVALID : pl - v->plane_begin;
+    }
+    destroy(s);
+}
+
+/* The face planes of volume `volume_id` (n[3], h, s[3] each) into out
+ * (room for max planes); returns the plane count. */
+uint32_t ORACLE_NAME(mesh_planes)(const g4go_scene* scene, uint32_t volume_id, real* out, uint32_t max) {
+    oscene_t* s = build(scene);
+    const volume_t* v = &s->vol[vol_index(s, volume_id)];
+    const uint32_t n = v->plane_count;
+    for (uint32_t k = 0; k < n && k < max; ++k) {
+        memcpy(out + 7 * k, &s->planes[v->plane_begin + k], 7 * sizeof(real));
+    }
+    destroy(s);
+    return n;
 }
 
 void ORACLE_NAME(project_polarization)(const real pol[3], const real dir[3], real out[3]) {

@@ -62,6 +62,10 @@ def _load(precision: int) -> C.CDLL:
     getattr(lib, p + "adjoint_emissions").argtypes = [
         C.POINTER(abi.Scene), C.c_void_p, C.c_uint64, C.c_uint64, C.c_float, C.c_uint32, C.c_void_p, C.c_uint64,
         C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint64, C.c_void_p]
+    getattr(lib, p + "mesh_trace").argtypes = [C.POINTER(abi.Scene), C.c_uint32, C.c_void_p, C.c_void_p,
+                                               C.c_uint64, real, C.c_int, C.c_void_p, C.c_void_p]
+    getattr(lib, p + "mesh_planes").argtypes = [C.POINTER(abi.Scene), C.c_uint32, C.c_void_p, C.c_uint32]
+    getattr(lib, p + "mesh_planes").restype = C.c_uint32
     for name in ("project_polarization", "reflect_direction"):
         getattr(lib, p + name).argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
     getattr(lib, p + "normalize").argtypes = [C.c_void_p, C.c_void_p]
@@ -162,6 +166,28 @@ def box_exit(lo, hi, p, d, precision: int = 32):
     _fn(precision, "box_exit")(lo.ctypes.data, hi.ctypes.data, pp.ctypes.data, dd.ctypes.data, C.byref(dist),
                                C.byref(axis))
     return dist.value, axis.value
+
+
+def mesh_trace(scene, volume_id: int, p, d, tmin: float = 0.0, entry: bool = False, precision: int = 32):
+    """Exit (entry=False) or entry distances of rays into a mesh volume, and the face crossed."""
+    real = _real(precision)
+    p = np.ascontiguousarray(p, dtype=real).reshape(-1, 3)
+    d = np.ascontiguousarray(d, dtype=real).reshape(-1, 3)
+    dist = np.zeros(len(p), real)
+    face = np.zeros(len(p), np.uint32)
+    desc, keep = scene.to_abi()
+    _fn(precision, "mesh_trace")(C.byref(desc), volume_id, p.ctypes.data, d.ctypes.data, len(p), tmin, int(entry),
+                                 dist.ctypes.data, face.ctypes.data)
+    return dist, face
+
+
+def mesh_planes(scene, volume_id: int, precision: int = 32) -> np.ndarray:
+    """Face planes of a mesh volume as rows (n[3], h, s[3])."""
+    real = _real(precision)
+    out = np.zeros((1024, 7), real)
+    desc, keep = scene.to_abi()
+    n = _fn(precision, "mesh_planes")(C.byref(desc), volume_id, out.ctypes.data, 1024)
+    return out[:n].copy()
 
 
 def find_surface(scene, from_id: int, to_id: int) -> int:

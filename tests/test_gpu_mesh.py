@@ -1,0 +1,172 @@
+"""GPU parity on convex mesh solids: the HIP kernels against the FP32 oracle
+on the same Philox streams (hit lists, counters and histograms bit for bit),
+and the closed forms of the oracle's own tests at full size on the device.
+
+Scenes: the tapered crystal (frustum mesh, box SiPM on its back face), the
+crystal turned by a rotation (crystal and SiPM as box meshes sharing a face),
+a tetrahedron with a mesh sensor glued on one face, a detecting tetrahedron
+seen from the world (mesh entry), and a world of 40 random convex hulls (a
+daughter BVH over mesh solids, face planes read from global memory).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from g4go_amd import abi, workloads
+from g4go_amd.calibrate import crystal_layout
+from g4go_amd.scene import Material, Mesh, PropertyTable, Scene, Surface, SurfaceType, Volume
+from g4go_amd.transport import HipTransportHost, LaunchForm, PhotonTransportConfig
+from oracle import oracle as O
+
+from test_gpu_parity import assert_same_hits, assert_same_stats
+from test_mesh_oracle import (OUTSIDE_POINT, TETRA_POINT, binomial_check, detecting_tetra_scene, mean_pde,
+                              random_hull, tapered_check, tapered_limit_scene, tetra_face_fraction,
+                              tetra_sensor_scene, visible_fraction)
+
+pytestmark = pytest.mark.gpu
+
+PERSISTENT = LaunchForm(persistent=True, block_size=256, chunk=256, refill_threshold=16, blocks_per_cu=4)
+PLAIN = LaunchForm(persistent=False)
+
+
+def log(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def device_run(scene, src, seed, ev, counts, form=PERSISTENT, sort_hits=True):
+    with HipTransportHost(PhotonTransportConfig(seed=seed, sort_hits=sort_hits)) as h:
+        h.set_launch(form)
+        h.enqueue_synthetic(scene, src, np.asarray(ev, np.uint32), np.asarray(counts, np.uint32))
+        return h.complete_oldest_batch()
+
+
+def parity(scene, src, seed, ev, counts, form=PERSISTENT):
+    ref = O.run_synthetic(scene, src, seed, ev, counts, threads=8)
+    out = device_run(scene, src, seed, ev, counts, form)
+    assert out.statistics["invalid"] == 0
+    assert_same_hits(out.detections, ref.hits)
+    assert_same_stats(out, ref)
+    return out
+
+
+@pytest.mark.parametrize("form", ["plain", "persistent"])
+def test_tapered_crystal_bit_exact(form):
+    log(f"case: tapered crystal, synthetic, {form}")
+    cs = workloads.tapered_crystal_scene()
+    out = parity(cs.scene, workloads.synthetic_source(cs), 7, [0, 3], [30000, 4000],
+                 PLAIN if form == "plain" else PERSISTENT)
+    assert out.statistics["detected"] > 5000
+
+
+def test_rotated_crystal_bit_exact():
+    log("case: rotated crystal (box meshes with a shared face), point source")
+    rot = workloads.rotation_matrix((1.0, 2.0, 3.0), 0.7)
+    cs = workloads.rotated_crystal_scene(rot)
+    src = workloads.point_source(cs.crystal_ids[0], rot @ np.array([0.3, -0.2, 1.0]))
+    out = parity(cs.scene, src, 8, [0], [30000])
+    assert out.statistics["detected"] > 5000
+
+
+def test_tetrahedron_face_and_entry_bit_exact():
+    log("case: tetrahedron with a mesh sensor; detecting tetrahedron seen from outside")
+    sc, tet = tetra_sensor_scene()
+    parity(sc, workloads.point_source(tet, TETRA_POINT), 9, [0], [20000])
+    sc, world = detecting_tetra_scene()
+    parity(sc, workloads.point_source(world, OUTSIDE_POINT), 10, [0], [20000])
+
+
+def hull_forest(n=40, seed=6):
+    """Air world with n random convex glass hulls (every fourth a sensor with a
+    partly reflecting skin), absorbing and scattering, on a grid."""
+    rng = np.random.default_rng(seed)
+    sc = Scene()
+    air = sc.add_material(Material(name="air", rindex=PropertyTable.constant_value(1.0)))
+    glass = sc.add_material(Material(name="glass", rindex=PropertyTable.constant_value(1.5),
+                                     abs_length_mm=PropertyTable.constant_value(40.0),
+                                     rayleigh_length_mm=PropertyTable.constant_value(60.0)))
+    det = sc.add_surface(Surface(name="det", type=SurfaceType.DielectricMetal,
+                                 reflectivity=PropertyTable.constant_value(0.3),
+                                 efficiency=PropertyTable.constant_value(0.7)))
+    world = sc.add_volume(Volume(name="world", material_id=air, half_mm=(80.0, 80.0, 80.0)))
+    sc.world_volume_id = world
+    for k in range(n):
+        m = random_hull(rng, 24)
+        centre = np.array([(k % 4) * 30.0 - 45.0, ((k // 4) % 4) * 30.0 - 45.0, (k // 16) * 30.0 - 30.0])
+        m = Mesh(1.2 * (m.vertices_mm - m.vertices_mm.mean(axis=0)) + centre, m.indices)  # radius < 15 mm
+        sensor = k % 4 == 0
+        sc.add_volume(Volume(name=f"h{k}", material_id=glass, parent_volume_id=world,
+                             skin_surface_id=det if sensor else abi.INVALID_ID,
+                             sensor_id=k if sensor else abi.INVALID_ID, mesh=m))
+    src = abi.SyntheticSource()
+    src.lo_mm[:] = [-3.0, -3.0, -18.0]  # a gap between the hulls
+    src.hi_mm[:] = [3.0, 3.0, -12.0]
+    src.wavelength_mean_nm, src.wavelength_sigma_nm, src.decay_time_ns = 420.0, 20.0, 5.0
+    src.volume_id = world
+    return sc, src
+
+
+def test_hull_forest_bvh_bit_exact():
+    log("case: 40 random convex hulls, daughter BVH over mesh solids")
+    sc, src = hull_forest()
+    out = parity(sc, src, 11, [0, 1], [60000, 9000])
+    assert out.statistics["detected"] > 800 and out.statistics["escaped"] > 10000
+
+
+def test_tapered_crystal_emissions_bit_exact():
+    log("case: tapered crystal, electron emission records")
+    cs = workloads.tapered_crystal_scene()
+    em, event_ids, idx = workloads.electron_emissions(cs, 4, seed=3)
+    ref = O.run_emissions(cs.scene, em, event_ids, idx, 12)
+    with HipTransportHost(PhotonTransportConfig(seed=12, sort_hits=True)) as h:
+        h.enqueue_emissions(cs.scene, em, event_ids, idx)
+        out = h.complete_oldest_batch()
+    assert_same_hits(out.detections, ref.hits)
+    assert_same_stats(out, ref)
+
+
+@pytest.mark.parametrize("reflectivity", [1.0, 0.0])
+def test_device_tapered_crystal_limits(reflectivity):
+    log(f"case: tapered crystal closed form R={reflectivity}, 2e7 photons")
+    cs = tapered_limit_scene(reflectivity)
+    out = device_run(cs.scene, workloads.synthetic_source(cs), 33, [0], [20_000_000], sort_hits=False)
+    ev = out.event_statistics[0]
+    st = {k: int(ev[k]) for k in ("detected", "absorbed", "escaped", "truncated", "invalid")}
+    st["generated"] = int(out.statistics["generated"])
+    tapered_check(st, reflectivity)
+
+
+def test_device_tetrahedron_solid_angles():
+    log("case: tetrahedron solid angles, 1e7 photons each")
+    sc, tet = tetra_sensor_scene()
+    n = 10_000_000
+    out = device_run(sc, workloads.point_source(tet, TETRA_POINT), 34, [0], [n], sort_hits=False)
+    assert out.statistics["invalid"] == 0 and out.statistics["escaped"] == 0
+    binomial_check(int(out.statistics["detected"]), n, tetra_face_fraction(), "tetrahedron face")
+    sc, world = detecting_tetra_scene()
+    out = device_run(sc, workloads.point_source(world, OUTSIDE_POINT), 35, [0], [n], sort_hits=False)
+    assert out.statistics["invalid"] == 0 and out.statistics["absorbed"] == 0
+    binomial_check(int(out.statistics["detected"]), n, visible_fraction(), "tetrahedron seen from outside")
+
+
+def test_mesh_adjoint_matches_oracle_replay():
+    """The path-replay adjoint on the tapered crystal replays every hit and
+    matches the oracle's replay."""
+    log("case: adjoint on the tapered crystal vs oracle replay")
+    cs = workloads.tapered_crystal_scene()
+    src = workloads.synthetic_source(cs)
+    ev = np.array([2, 5], np.uint32)
+    counts = np.array([40000, 900], np.uint32)
+    w = np.random.default_rng(13).normal(size=abi.HIST_SLOTS)
+    with HipTransportHost(PhotonTransportConfig(seed=36, sort_hits=True)) as h:
+        h.set_launch(PERSISTENT)
+        h.prepare_scene(cs.scene)
+        n_params = h.set_parameter_layout(crystal_layout(cs))
+        h.enqueue_synthetic(cs.scene, src, ev, counts)
+        out = h.complete_oldest_batch()
+        grad, stats = h.adjoint(w)
+    assert stats["replayed"] == len(out.detections) and stats["mismatched"] == 0 and stats["unsupported"] == 0
+    ref, counters = O.adjoint_synthetic(cs.scene, src, 36, out.detections, w, crystal_layout(cs), n_params)
+    assert int(counters[0]) == len(out.detections) and int(counters[1]) == 0
+    np.testing.assert_allclose(grad, ref, rtol=1e-9, atol=1e-9 * np.max(np.abs(ref)))
