@@ -19,11 +19,13 @@
 // own types instead.  The shim includes no HIP header: it reaches the device
 // only through g4go_hip.h.
 //
-// Geometry: the backend transports axis-aligned boxes.  A volume's triangle
-// mesh (Scene.hpp:94-105) is mapped to world space with its transform
-// (Scene.hpp:48-59) and must be an axis-aligned box there (all eight corners
-// of its bounding box present, every vertex on a corner); anything else is
-// rejected with std::invalid_argument.
+// Geometry: a volume's triangle mesh (Scene.hpp:94-105) is mapped to world
+// space with its transform (Scene.hpp:48-59).  A mesh that is an axis-aligned
+// box there (all eight corners of its bounding box present, every vertex on a
+// corner) is passed as a box solid, the backend's fastest geometry; any other
+// mesh is passed as a world-space triangle mesh (g4go_volume_mesh, triangle
+// normals turned with it), which the backend accepts when it is closed and
+// convex (PrepareScene throws std::invalid_argument otherwise).
 #pragma once
 
 #include "g4go_hip.h"
@@ -349,11 +351,21 @@ public:
             out.parent_volume_id = v.parentVolumeID;
             out.skin_surface_id = v.skinSurfaceID;
             out.sensor_id = v.sensorID;
-            world_box(scene, v, tolerance_mm, out.center_mm, out.half_mm);
+            world_solid(scene, v, tolerance_mm, out.center_mm, out.half_mm);
             volumes_.push_back(out);
         }
         for (const auto& b : scene.SurfaceBindings()) {
             bindings_.push_back(g4go_surface_binding{b.fromVolumeID, b.toVolumeID, b.surfaceID});
+        }
+        for (const MeshStore& m : mesh_store_) {  // pointers taken once the stores stop growing
+            g4go_volume_mesh out{};
+            out.volume_id = m.volume_id;
+            out.n_vertices = static_cast<std::uint32_t>(m.vertices.size() / 3);
+            out.vertices_mm = m.vertices.data();
+            out.indices = m.indices.data();
+            out.n_triangles = static_cast<std::uint32_t>(m.indices.size() / 3);
+            out.triangle_normals = m.normals.empty() ? nullptr : m.normals.data();
+            meshes_.push_back(out);
         }
         view_.materials = materials_.data();
         view_.n_materials = static_cast<std::uint32_t>(materials_.size());
@@ -364,10 +376,15 @@ public:
         view_.bindings = bindings_.data();
         view_.n_bindings = static_cast<std::uint32_t>(bindings_.size());
         view_.world_volume_id = scene.WorldVolumeID();
+        view_.meshes = meshes_.empty() ? nullptr : meshes_.data();
+        view_.n_meshes = static_cast<std::uint32_t>(meshes_.size());
     }
+    FlatScene(const FlatScene&) = delete;  // the view points into this object
+    FlatScene& operator=(const FlatScene&) = delete;
 
     const g4go_scene& view() const { return view_; }
     const std::vector<g4go_volume>& volumes() const { return volumes_; }
+    const std::vector<g4go_volume_mesh>& meshes() const { return meshes_; }
 
 private:
     template <class Table>
@@ -381,9 +398,10 @@ private:
         return g4go_property{static_cast<std::uint32_t>(t.energyEv.size()), t.energyEv.data(), t.values.data()};
     }
 
-    // World-space axis-aligned box of a volume's mesh.
+    // A volume's mesh in world space: as a box (centre, half lengths) when it
+    // is an axis-aligned box there, else as a mesh store.
     template <class Volume>
-    static void world_box(const Scene& scene, const Volume& v, float tolerance_mm, float* center, float* half) {
+    void world_solid(const Scene& scene, const Volume& v, float tolerance_mm, float* center, float* half) {
         const auto* geometry = scene.FindGeometry(v.geometryID);
         const std::string who = "volume " + std::to_string(v.volumeID);
         if (geometry == nullptr || geometry->mesh.verticesMm.empty()) {
@@ -391,49 +409,73 @@ private:
         }
         const auto& r = v.transform.rotation;
         const auto& t = v.transform.translationMm;
-        std::vector<std::array<double, 3>> world;
-        world.reserve(geometry->mesh.verticesMm.size());
+        MeshStore m;
+        m.volume_id = v.volumeID;
+        m.vertices.reserve(3 * geometry->mesh.verticesMm.size());
         double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
         for (const auto& p : geometry->mesh.verticesMm) {
             const double x = p.x(), y = p.y(), z = p.z();
-            const std::array<double, 3> w = {r.xx * x + r.xy * y + r.xz * z + t.x(),
-                                             r.yx * x + r.yy * y + r.yz * z + t.y(),
-                                             r.zx * x + r.zy * y + r.zz * z + t.z()};
+            const double w[3] = {r.xx * x + r.xy * y + r.xz * z + t.x(), r.yx * x + r.yy * y + r.yz * z + t.y(),
+                                 r.zx * x + r.zy * y + r.zz * z + t.z()};
             for (int a = 0; a < 3; ++a) {
                 lo[a] = std::fmin(lo[a], w[a]);
                 hi[a] = std::fmax(hi[a], w[a]);
+                m.vertices.push_back(w[a]);
             }
-            world.push_back(w);
         }
         const double tol = std::max(static_cast<double>(tolerance_mm), 1e-6);
         unsigned corners = 0;
-        for (const auto& w : world) {
+        bool box = true;
+        for (size_t i = 0; box && i < m.vertices.size(); i += 3) {
             unsigned corner = 0;
             for (int a = 0; a < 3; ++a) {
-                if (std::fabs(w[a] - lo[a]) <= tol) {
+                if (std::fabs(m.vertices[i + a] - lo[a]) <= tol) {
                     continue;
                 }
-                if (std::fabs(w[a] - hi[a]) <= tol) {
+                if (std::fabs(m.vertices[i + a] - hi[a]) <= tol) {
                     corner |= 1u << a;
                     continue;
                 }
-                throw std::invalid_argument(who + ": mesh is not an axis-aligned box in world space");
+                box = false;
             }
             corners |= 1u << corner;
         }
-        if (corners != 0xffu) {
-            throw std::invalid_argument(who + ": mesh is not an axis-aligned box in world space");
+        if (box && corners == 0xffu) {
+            for (int a = 0; a < 3; ++a) {
+                center[a] = static_cast<float>(0.5 * (lo[a] + hi[a]));
+                half[a] = static_cast<float>(0.5 * (hi[a] - lo[a]));
+            }
+            return;
         }
-        for (int a = 0; a < 3; ++a) {
-            center[a] = static_cast<float>(0.5 * (lo[a] + hi[a]));
-            half[a] = static_cast<float>(0.5 * (hi[a] - lo[a]));
+        if (geometry->mesh.indices.size() % 3 != 0 ||
+            geometry->mesh.indices.size() / 3 > std::numeric_limits<std::uint32_t>::max()) {
+            throw std::invalid_argument(who + ": invalid mesh");
         }
+        m.indices.assign(geometry->mesh.indices.begin(), geometry->mesh.indices.end());
+        for (const auto& n : geometry->mesh.triangleNormals) {
+            const double x = n.x(), y = n.y(), z = n.z();
+            m.normals.insert(m.normals.end(), {r.xx * x + r.xy * y + r.xz * z, r.yx * x + r.yy * y + r.yz * z,
+                                               r.zx * x + r.zy * y + r.zz * z});
+        }
+        if (!m.normals.empty() && m.normals.size() != m.indices.size()) {
+            throw std::invalid_argument(who + ": triangle normal count mismatch");
+        }
+        mesh_store_.push_back(std::move(m));
     }
+
+    struct MeshStore {
+        std::uint32_t volume_id{};
+        std::vector<double> vertices;  // world frame, xyz
+        std::vector<std::uint32_t> indices;
+        std::vector<double> normals;  // world frame, or empty
+    };
 
     std::vector<g4go_material> materials_;
     std::vector<g4go_surface> surfaces_;
     std::vector<g4go_volume> volumes_;
     std::vector<g4go_surface_binding> bindings_;
+    std::vector<MeshStore> mesh_store_;
+    std::vector<g4go_volume_mesh> meshes_;
     g4go_scene view_{};
 };
 

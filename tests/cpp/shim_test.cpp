@@ -198,27 +198,49 @@ int run_cpu() {
         expect(flat.view().world_volume_id == c.world && flat.view().n_bindings == 1, "world and binding");
         expect(flat.view().materials[1].rayleigh_length_mm.count == 2, "Rayleigh table passed through");
     }
-    // non-box meshes are rejected
+    // other meshes pass as world-space meshes; the backend checks them at PrepareScene
     {
         CrystalScene c = crystal_scene();
         Geometry tetra;
-        tetra.mesh.verticesMm = {ThreeVector{0, 0, 0}, ThreeVector{1, 0, 0}, ThreeVector{0, 1, 0}, ThreeVector{0, 0, 1}};
+        tetra.mesh.verticesMm = {ThreeVector{30, 0, 0}, ThreeVector{31, 0, 0}, ThreeVector{30, 1, 0},
+                                 ThreeVector{30, 0, 1}};
+        tetra.mesh.indices = {0, 1, 2, 0, 1, 3, 1, 2, 3, 0, 2, 3};
         const std::uint32_t g = c.scene.AddGeometry(tetra);
         Volume v;
         v.geometryID = g;
         v.materialID = 0;
         v.parentVolumeID = c.world;
-        c.scene.AddVolume(v);
-        expect_throws<std::invalid_argument>([&] { g4go_hip::FlatScene<Scene> f(c.scene, 1e-4F); }, "tetrahedron mesh");
+        v.transform.translationMm = ThreeVector{0, 5, 0};
+        const std::uint32_t tv = c.scene.AddVolume(v);
+        {
+            g4go_hip::FlatScene<Scene> f(c.scene, 1e-4F);
+            expect(f.meshes().size() == 1 && f.meshes()[0].volume_id == tv && f.meshes()[0].n_triangles == 4,
+                   "tetrahedron passed as a mesh");
+            expect(f.meshes()[0].vertices_mm[1] == 5.0 && f.view().n_meshes == 1, "mesh translated to world space");
+        }
+        g4go_hip::HipTransportHost host{PhotonTransportConfig{}};
+        host.PrepareScene(c.scene);
         CrystalScene d = crystal_scene();
         Volume turned;
         turned.geometryID = 2;
         turned.materialID = 0;
         turned.parentVolumeID = d.world;
+        turned.transform.translationMm = ThreeVector{0, 40, 0};
         const float s = 0.70710678F;
         turned.transform.rotation = Rotation{s, -s, 0, s, s, 0, 0, 0, 1};  // 45 degrees about z
         d.scene.AddVolume(turned);
-        expect_throws<std::invalid_argument>([&] { g4go_hip::FlatScene<Scene> f(d.scene, 1e-4F); }, "box turned 45 deg");
+        {
+            g4go_hip::FlatScene<Scene> f(d.scene, 1e-4F);
+            expect(f.meshes().size() == 1 && f.meshes()[0].n_triangles == 12, "box turned 45 deg passed as a mesh");
+        }
+        host.PrepareScene(d.scene);
+        Geometry open = tetra;
+        open.mesh.indices.resize(9);  // three triangles: not closed
+        CrystalScene e = crystal_scene();
+        Volume ov = v;
+        ov.geometryID = e.scene.AddGeometry(open);
+        e.scene.AddVolume(ov);
+        expect_throws<std::invalid_argument>([&] { host.PrepareScene(e.scene); }, "open mesh rejected");
     }
     // exception types
     {
