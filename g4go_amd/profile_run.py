@@ -2,6 +2,8 @@
 
     python -m g4go_amd.profile_run --workload crystal --photons 100000000
     python -m g4go_amd.profile_run --workload uneven
+    python -m g4go_amd.profile_run --workload tapered   # mesh solids: tapered crystal
+    python -m g4go_amd.profile_run --workload rotated   # mesh solids: crystal turned by a rotation
     python -m g4go_amd.profile_run --copy-bytes 1073741824   # calibration copy
 
 The launch form is the cached autotuner result for the device when present,
@@ -20,7 +22,8 @@ import numpy as np
 
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", choices=("crystal", "uneven", "array", "none"), default="crystal")
+    ap.add_argument("--workload", choices=("crystal", "uneven", "array", "tapered", "rotated", "none"),
+                    default="crystal")
     ap.add_argument("--photons", type=int, default=100_000_000)
     ap.add_argument("--events", type=int, default=4096)
     ap.add_argument("--seed", type=int, default=2024)
@@ -44,8 +47,18 @@ def main(argv=None) -> int:
         print(json.dumps({"copy_bytes": args.copy_bytes}), flush=True)
     if args.workload == "none":
         return 0
-    cs = workloads.array_scene() if args.workload == "array" else workloads.crystal_scene()
-    src = workloads.synthetic_source(cs, crystal_index=27 if args.workload == "array" else 0)
+    if args.workload == "array":
+        cs = workloads.array_scene()
+        src = workloads.synthetic_source(cs, crystal_index=27)
+    elif args.workload == "tapered":
+        cs = workloads.tapered_crystal_scene()
+        src = workloads.synthetic_source(cs)
+    elif args.workload == "rotated":
+        cs = workloads.rotated_crystal_scene(workloads.rotation_matrix((1.0, 2.0, 3.0), 0.7))
+        src = workloads.point_source(cs.crystal_ids[0], (0.0, 0.0, 0.0))
+    else:
+        cs = workloads.crystal_scene()
+        src = workloads.synthetic_source(cs)
     host = HipTransportHost(PhotonTransportConfig(seed=args.seed, sort_hits=False))
     form = autotune.cached(autotune.device_key(host), "crystal-synthetic") or LaunchForm(persistent=True, block_size=256, chunk=256,
                                                                       refill_threshold=16, blocks_per_cu=4)
