@@ -28,7 +28,9 @@ back to the host) with one and with two batches in flight; `mesh_crystals`
 times 1e8 photons in two crystals with mesh solids: the crystal tapered to a
 4x4 mm back face (a frustum mesh, source uniform in the 3x3x20 mm box inside
 it) and the 3x3x20 mm crystal turned by a rotation (box meshes, point source
-at its centre), each with its transport-step cost relative to the box crystal.
+at its centre), both convex, and an L-shaped light guide (a non-convex prism
+held as triangles), each with its transport-step cost relative to the box
+crystal.
 """
 from __future__ import annotations
 
@@ -220,18 +222,24 @@ def run_array(seed, launch, steps, warmup, plain):
 
 
 def run_mesh(seed, launch, steps, warmup, photons, box_ms_per_step):
-    """1e8 photons in the tapered crystal and in the rotated crystal (mesh solids)."""
+    """1e8 photons in the tapered crystal, the rotated crystal and the L-shaped light guide (mesh solids)."""
     from g4go_amd import workloads
     from g4go_amd.transport import HipTransportHost, PhotonTransportConfig
 
     rot = workloads.rotation_matrix((1.0, 2.0, 3.0), 0.7)
-    cases = {"tapered_crystal": workloads.tapered_crystal_scene(), "rotated_crystal": workloads.rotated_crystal_scene(rot)}
+    tapered = workloads.tapered_crystal_scene()
+    rotated = workloads.rotated_crystal_scene(rot)
+    guide, guide_src, _, _ = workloads.l_guide_scene()
+
+    class Holder:  # run_form reads .scene
+        def __init__(self, scene):
+            self.scene = scene
+
+    cases = {"tapered_crystal": (tapered, workloads.synthetic_source(tapered)),
+             "rotated_crystal": (rotated, workloads.point_source(rotated.crystal_ids[0], (0.0, 0.0, 0.0))),
+             "l_guide_triangles": (Holder(guide), guide_src)}
     res = {}
-    for name, cs in cases.items():
-        if name == "tapered_crystal":
-            src = workloads.synthetic_source(cs)
-        else:
-            src = workloads.point_source(cs.crystal_ids[0], (0.0, 0.0, 0.0))
+    for name, (cs, src) in cases.items():
         host = HipTransportHost(PhotonTransportConfig(seed=seed, sort_hits=False))
         wall, kms, out = run_form(host, cs, src, photons, launch, steps, warmup)
         host.close()

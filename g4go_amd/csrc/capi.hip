@@ -60,6 +60,8 @@ struct HostScene {
     std::vector<BvhNode> bvh;
     std::vector<Plane> planes;  // faces of the convex mesh solids
     std::vector<std::vector<double>> face_points;  // per plane: its triangles' vertices (xyz), host only
+    std::vector<Tri> tris;                         // triangles of the non-convex mesh solids
+    std::vector<std::vector<double>> tri_points;   // per triangle: its vertices (xyz), host only
     uint32_t world = kInvalid;
     uint32_t hot_offset = 0, hot_count = 0;  // hot energy grid (mark_hot_grid)
     std::unordered_map<uint32_t, uint32_t> index_of;  // public volume ID -> dense index
@@ -284,9 +286,9 @@ void mark_hot_grid(HostScene& s) {
     }
 }
 
-// Face planes and bounding box of a closed convex triangle mesh (world
-// frame), appended to `planes`.  All arithmetic is FP64 in a fixed order, the
-// results rounded once to FP32:
+// Solid of a closed triangle mesh (world frame): face planes when it is
+// convex, else its triangles, plus the bounding box.  All arithmetic is FP64
+// in a fixed order, the results rounded once to FP32:
 //   * each triangle's plane: unit normal u = (v1 - v0) x (v2 - v0) / |...|,
 //     offset h = u.v0, turned so that the vertex centroid c lies inside
 //     (u.c < h);
@@ -295,13 +297,16 @@ void mark_hot_grid(HostScene& s) {
 //     form one face; faces keep the first triangle's values, in triangle
 //     order;
 //   * the mesh is closed when every edge (vertices welded by equal
-//     coordinates) belongs to exactly two triangles, convex when no vertex
-//     lies more than 1e-6 of the mesh size outside a face plane, and solid
-//     when c lies more than that inside every face plane.
-// The reference accepts any triangle mesh (OptiXTransportHost.cpp:1118-1200);
-// this backend's clipping tests need a closed convex one.
-void mesh_solid(const g4go_volume_mesh& m, std::vector<Plane>& planes, std::vector<std::vector<double>>& face_points,
-                float lo[3], float hi[3]) {
+//     coordinates) belongs to exactly two triangles; it is convex when no
+//     vertex lies more than 1e-6 of the mesh size outside a face plane and c
+//     lies more than that inside every face plane.
+// A convex mesh becomes planes (clipping tests).  Any other closed mesh must
+// be wound consistently (each edge traversed once each way); it becomes
+// triangles, all turned outwards when the signed volume is negative, with
+// keys = their index in the mesh.  The reference accepts any triangle mesh
+// (OptiXTransportHost.cpp:1118-1200) and ignores the winding; these tests
+// need closed meshes.
+void mesh_solid(const g4go_volume_mesh& m, HostScene& s, Volume& dv) {
     const std::string who = "volume " + std::to_string(m.volume_id) + " mesh";
     if (m.vertices_mm == nullptr || m.indices == nullptr || m.n_vertices < 4 || m.n_triangles < 4) {
         fail(G4GO_ERROR_INVALID_ARGUMENT, who + ": needs at least 4 vertices and 4 triangles");
@@ -340,7 +345,9 @@ void mesh_solid(const g4go_volume_mesh& m, std::vector<Plane>& planes, std::vect
         }
     }
     const double tol = 1e-6 * size;
-    // closed: every edge of the welded mesh in exactly two triangles
+    // closed: every edge of the welded mesh in exactly two triangles;
+    // consistently wound: traversed once each way
+    bool consistent = true;
     {
         std::vector<uint32_t> order(nv);
         for (uint32_t i = 0; i < nv; ++i) {
@@ -362,7 +369,7 @@ void mesh_solid(const g4go_volume_mesh& m, std::vector<Plane>& planes, std::vect
                               v[3 * a + 2] == v[3 * order[i - 1] + 2];
             weld[a] = same ? weld[order[i - 1]] : a;
         }
-        std::vector<uint64_t> edges;
+        std::vector<uint64_t> edges;  // (min, max) << 1 | (traversed max -> min)
         edges.reserve(3ull * nt);
         for (uint32_t t = 0; t < nt; ++t) {
             for (int e = 0; e < 3; ++e) {
@@ -370,18 +377,20 @@ void mesh_solid(const g4go_volume_mesh& m, std::vector<Plane>& planes, std::vect
                 if (a == b) {
                     fail(G4GO_ERROR_INVALID_ARGUMENT, who + ": degenerate triangle");
                 }
-                edges.push_back((static_cast<uint64_t>(std::min(a, b)) << 32) | std::max(a, b));
+                edges.push_back((((static_cast<uint64_t>(std::min(a, b)) << 32) | std::max(a, b)) << 1) |
+                                (a > b ? 1u : 0u));
             }
         }
         std::sort(edges.begin(), edges.end());
         for (size_t i = 0; i < edges.size();) {
             size_t j = i;
-            while (j < edges.size() && edges[j] == edges[i]) {
+            while (j < edges.size() && (edges[j] >> 1) == (edges[i] >> 1)) {
                 ++j;
             }
             if (j - i != 2) {
                 fail(G4GO_ERROR_INVALID_ARGUMENT, who + ": not closed (an edge is not shared by exactly two triangles)");
             }
+            consistent = consistent && (edges[i] & 1u) != (edges[i + 1] & 1u);
             i = j;
         }
     }
@@ -389,7 +398,25 @@ void mesh_solid(const g4go_volume_mesh& m, std::vector<Plane>& planes, std::vect
         double u[3], h, s[3];
         std::vector<double> points;  // its triangles' vertices
     };
+    // boundary normal of triangle t: the caller's, else u
+    const auto shade_of = [&](uint32_t t, const double* u, double* out) {
+        if (m.triangle_normals == nullptr) {
+            for (int k = 0; k < 3; ++k) {
+                out[k] = u[k];
+            }
+            return;
+        }
+        const double* n = m.triangle_normals + 3 * t;
+        const double nl = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+        if (!(nl > 0.0) || !std::isfinite(nl)) {
+            fail(G4GO_ERROR_INVALID_ARGUMENT, who + ": invalid triangle normal");
+        }
+        for (int k = 0; k < 3; ++k) {
+            out[k] = n[k] / nl;
+        }
+    };
     std::vector<Face64> faces;
+    bool too_many = false;
     for (uint32_t t = 0; t < nt; ++t) {
         const double* p0 = v + 3 * m.indices[3 * t];
         const double* p1 = v + 3 * m.indices[3 * t + 1];
@@ -412,20 +439,7 @@ void mesh_solid(const g4go_volume_mesh& m, std::vector<Plane>& planes, std::vect
             }
             f.h = -f.h;
         }
-        if (m.triangle_normals != nullptr) {
-            const double* n = m.triangle_normals + 3 * t;
-            const double nl = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
-            if (!(nl > 0.0) || !std::isfinite(nl)) {
-                fail(G4GO_ERROR_INVALID_ARGUMENT, who + ": invalid triangle normal");
-            }
-            for (int k = 0; k < 3; ++k) {
-                f.s[k] = n[k] / nl;
-            }
-        } else {
-            for (int k = 0; k < 3; ++k) {
-                f.s[k] = f.u[k];
-            }
-        }
+        shade_of(t, f.u, f.s);
         Face64* into = nullptr;
         for (Face64& o : faces) {
             bool same = std::fabs(o.h - f.h) <= 1e-9 * size;
@@ -439,44 +453,187 @@ void mesh_solid(const g4go_volume_mesh& m, std::vector<Plane>& planes, std::vect
         }
         if (into == nullptr) {
             if (faces.size() == kMaxMeshPlanes) {
-                fail(G4GO_ERROR_INVALID_ARGUMENT, who + ": more than " + std::to_string(kMaxMeshPlanes) + " faces");
+                too_many = true;  // not a convex solid this backend holds as planes
+                break;
             }
             faces.push_back(f);
             into = &faces.back();
         }
         into->points.insert(into->points.end(), {p0[0], p0[1], p0[2], p1[0], p1[1], p1[2], p2[0], p2[1], p2[2]});
     }
-    for (const Face64& f : faces) {
-        if (!(f.h - (f.u[0] * c[0] + f.u[1] * c[1] + f.u[2] * c[2]) > tol)) {
-            fail(G4GO_ERROR_INVALID_ARGUMENT, who + ": flat (the centroid lies on a face plane)");
+    bool convex = !too_many;
+    for (size_t j = 0; convex && j < faces.size(); ++j) {
+        const Face64& f = faces[j];
+        convex = f.h - (f.u[0] * c[0] + f.u[1] * c[1] + f.u[2] * c[2]) > tol;
+        for (uint32_t i = 0; convex && i < nv; ++i) {
+            convex = !(f.u[0] * v[3 * i] + f.u[1] * v[3 * i + 1] + f.u[2] * v[3 * i + 2] - f.h > tol);
         }
-        for (uint32_t i = 0; i < nv; ++i) {
-            if (f.u[0] * v[3 * i] + f.u[1] * v[3 * i + 1] + f.u[2] * v[3 * i + 2] - f.h > tol) {
-                fail(G4GO_ERROR_INVALID_ARGUMENT, who + ": not convex");
+    }
+    if (convex) {
+        dv.plane_begin = static_cast<uint32_t>(s.planes.size());
+        for (const Face64& f : faces) {
+            Plane pl{};
+            for (int k = 0; k < 3; ++k) {
+                pl.n[k] = static_cast<float>(f.u[k]);
+                pl.shade[k] = static_cast<float>(f.s[k]);
             }
+            pl.h = static_cast<float>(f.h);
+            pl.across = kFaceDynamic;
+            pl.surface_parent = pl.surface_across = kInvalid;
+            s.planes.push_back(pl);
+            s.face_points.push_back(f.points);
         }
-        Plane pl{};
-        for (int k = 0; k < 3; ++k) {
-            pl.n[k] = static_cast<float>(f.u[k]);
-            pl.shade[k] = static_cast<float>(f.s[k]);
+        dv.plane_count = static_cast<uint32_t>(faces.size());
+    } else {
+        if (!consistent) {
+            fail(G4GO_ERROR_INVALID_ARGUMENT, who + ": not convex, and its triangles are not wound consistently");
         }
-        pl.h = static_cast<float>(f.h);
-        pl.across = kFaceDynamic;
-        pl.surface_parent = pl.surface_across = kInvalid;
-        planes.push_back(pl);
-        face_points.push_back(f.points);
+        if (nt > kMaxSolidTriangles) {
+            fail(G4GO_ERROR_INVALID_ARGUMENT,
+                 who + ": more than " + std::to_string(kMaxSolidTriangles) + " triangles in a non-convex mesh");
+        }
+        // signed volume about the centroid: negative when the winding points inwards
+        double vol6 = 0.0;
+        for (uint32_t t = 0; t < nt; ++t) {
+            double q[3][3];
+            for (int j = 0; j < 3; ++j) {
+                for (int k = 0; k < 3; ++k) {
+                    q[j][k] = v[3 * m.indices[3 * t + j] + k] - c[k];
+                }
+            }
+            vol6 += q[0][0] * (q[1][1] * q[2][2] - q[1][2] * q[2][1]) + q[0][1] * (q[1][2] * q[2][0] - q[1][0] * q[2][2]) +
+                    q[0][2] * (q[1][0] * q[2][1] - q[1][1] * q[2][0]);
+        }
+        if (!(std::fabs(vol6) > tol * size * size)) {
+            fail(G4GO_ERROR_INVALID_ARGUMENT, who + ": flat (no enclosed volume)");
+        }
+        const bool flip = vol6 < 0.0;
+        dv.plane_begin = static_cast<uint32_t>(s.tris.size());
+        for (uint32_t t = 0; t < nt; ++t) {
+            const double* p0 = v + 3 * m.indices[3 * t];
+            const double* p1 = v + 3 * m.indices[3 * t + (flip ? 2 : 1)];
+            const double* p2 = v + 3 * m.indices[3 * t + (flip ? 1 : 2)];
+            const double a[3] = {p1[0] - p0[0], p1[1] - p0[1], p1[2] - p0[2]};
+            const double b[3] = {p2[0] - p0[0], p2[1] - p0[1], p2[2] - p0[2]};
+            const double g[3] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
+            const double len = std::sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
+            double u[3], sh[3];
+            for (int k = 0; k < 3; ++k) {
+                u[k] = g[k] / len;
+            }
+            shade_of(t, u, sh);
+            Tri tr{};
+            for (int k = 0; k < 3; ++k) {
+                tr.face.n[k] = static_cast<float>(u[k]);
+                tr.face.shade[k] = static_cast<float>(sh[k]);
+                tr.v0[k] = static_cast<float>(p0[k]);
+                tr.v1[k] = static_cast<float>(p1[k]);
+                tr.v2[k] = static_cast<float>(p2[k]);
+            }
+            tr.face.h = static_cast<float>(u[0] * p0[0] + u[1] * p0[1] + u[2] * p0[2]);
+            tr.face.across = kFaceDynamic;
+            tr.face.surface_parent = tr.face.surface_across = kInvalid;
+            tr.key = t;
+            s.tris.push_back(tr);
+            s.tri_points.push_back({p0[0], p0[1], p0[2], p1[0], p1[1], p1[2], p2[0], p2[1], p2[2]});
+        }
+        dv.plane_count = nt | kSolidTriangles;
     }
     // bounding box rounded outwards
     for (int k = 0; k < 3; ++k) {
-        lo[k] = static_cast<float>(vlo[k]);
-        if (static_cast<double>(lo[k]) > vlo[k]) {
-            lo[k] = std::nextafter(lo[k], -INFINITY);
+        dv.lo[k] = static_cast<float>(vlo[k]);
+        if (static_cast<double>(dv.lo[k]) > vlo[k]) {
+            dv.lo[k] = std::nextafter(dv.lo[k], -INFINITY);
         }
-        hi[k] = static_cast<float>(vhi[k]);
-        if (static_cast<double>(hi[k]) < vhi[k]) {
-            hi[k] = std::nextafter(hi[k], INFINITY);
+        dv.hi[k] = static_cast<float>(vhi[k]);
+        if (static_cast<double>(dv.hi[k]) < vhi[k]) {
+            dv.hi[k] = std::nextafter(dv.hi[k], INFINITY);
         }
     }
+}
+
+// BVH over the triangles [begin, begin + count) of a triangle solid (median
+// split of the centroids on the longest axis, leaves of <= 4; the triangles
+// and their points are reordered in place).  Node boxes are widened by 1e-6
+// of their coordinates so that a hit on a triangle is never culled by the
+// rounding of the slab test.  Returns the root node.
+uint32_t build_tri_bvh(HostScene& s, uint32_t begin, uint32_t count) {
+    struct Task {
+        uint32_t node, first, count, depth;
+    };
+    const auto tri_lo = [&](const Tri& t, int k) { return std::min({t.v0[k], t.v1[k], t.v2[k]}); };
+    const auto tri_hi = [&](const Tri& t, int k) { return std::max({t.v0[k], t.v1[k], t.v2[k]}); };
+    const uint32_t root = static_cast<uint32_t>(s.bvh.size());
+    s.bvh.push_back(BvhNode{});
+    std::vector<Task> stack{{root, begin, count, 0u}};
+    std::vector<uint32_t> perm;
+    while (!stack.empty()) {
+        const Task t = stack.back();
+        stack.pop_back();
+        if (t.depth > static_cast<uint32_t>(kBvhMaxDepth)) {
+            fail(G4GO_ERROR_INVALID_ARGUMENT, "triangle BVH deeper than the device traversal stack");
+        }
+        BvhNode node{};
+        float clo[3], chi[3];
+        for (int k = 0; k < 3; ++k) {
+            node.lo[k] = clo[k] = std::numeric_limits<float>::infinity();
+            node.hi[k] = chi[k] = -std::numeric_limits<float>::infinity();
+        }
+        for (uint32_t i = t.first; i < t.first + t.count; ++i) {
+            for (int k = 0; k < 3; ++k) {
+                node.lo[k] = std::min(node.lo[k], tri_lo(s.tris[i], k));
+                node.hi[k] = std::max(node.hi[k], tri_hi(s.tris[i], k));
+                const float cc = tri_lo(s.tris[i], k) + tri_hi(s.tris[i], k);
+                clo[k] = std::min(clo[k], cc);
+                chi[k] = std::max(chi[k], cc);
+            }
+        }
+        for (int k = 0; k < 3; ++k) {
+            const float pad = 1e-6f * std::max(std::fabs(node.lo[k]), std::fabs(node.hi[k])) + 1e-6f;
+            node.lo[k] -= pad;
+            node.hi[k] += pad;
+        }
+        if (t.count <= 4) {
+            node.left = t.first;
+            node.right_or_count = t.count | kBvhLeaf;
+            s.bvh[t.node] = node;
+            continue;
+        }
+        int axis = 0;
+        for (int k = 1; k < 3; ++k) {
+            if (chi[k] - clo[k] > chi[axis] - clo[axis]) {
+                axis = k;
+            }
+        }
+        perm.resize(t.count);
+        for (uint32_t i = 0; i < t.count; ++i) {
+            perm[i] = t.first + i;
+        }
+        const uint32_t half = t.count / 2;
+        std::nth_element(perm.begin(), perm.begin() + half, perm.end(), [&](uint32_t a, uint32_t b) {
+            const float ca = tri_lo(s.tris[a], axis) + tri_hi(s.tris[a], axis);
+            const float cb = tri_lo(s.tris[b], axis) + tri_hi(s.tris[b], axis);
+            return ca < cb || (ca == cb && s.tris[a].key < s.tris[b].key);
+        });
+        std::vector<Tri> tris(t.count);
+        std::vector<std::vector<double>> pts(t.count);
+        for (uint32_t i = 0; i < t.count; ++i) {
+            tris[i] = s.tris[perm[i]];
+            pts[i] = std::move(s.tri_points[perm[i]]);
+        }
+        for (uint32_t i = 0; i < t.count; ++i) {
+            s.tris[t.first + i] = tris[i];
+            s.tri_points[t.first + i] = std::move(pts[i]);
+        }
+        node.left = static_cast<uint32_t>(s.bvh.size());
+        node.right_or_count = node.left + 1;
+        s.bvh.push_back(BvhNode{});
+        s.bvh.push_back(BvhNode{});
+        s.bvh[t.node] = node;
+        stack.push_back({node.left, t.first, half, t.depth + 1});
+        stack.push_back({node.right_or_count, t.first + half, t.count - half, t.depth + 1});
+    }
+    return root;
 }
 
 HostScene build_scene(const g4go_scene& in, float tolerance) {
@@ -558,9 +715,7 @@ HostScene build_scene(const g4go_scene& in, float tolerance) {
         const g4go_volume& v = in.volumes[i];
         Volume dv{};
         if (mesh_of[i] != nullptr) {
-            dv.plane_begin = static_cast<uint32_t>(s.planes.size());
-            mesh_solid(*mesh_of[i], s.planes, s.face_points, dv.lo, dv.hi);
-            dv.plane_count = static_cast<uint32_t>(s.planes.size()) - dv.plane_begin;
+            mesh_solid(*mesh_of[i], s, dv);
         } else {
             for (int k = 0; k < 3; ++k) {
                 if (!std::isfinite(v.center_mm[k]) || !(v.half_mm[k] > 0.0f) || !std::isfinite(v.half_mm[k])) {
@@ -698,19 +853,29 @@ HostScene build_scene(const g4go_scene& in, float tolerance) {
             build_bvh(s, v, v.daughter_begin, v.daughter_count);
         }
     }
+    // triangle BVHs (the root sits in the otherwise unused first face record)
+    for (uint32_t i = 0; i < in.n_volumes; ++i) {
+        Volume& v = s.volumes[i];
+        if ((v.plane_count & kSolidTriangles) != 0) {
+            const uint32_t n = v.plane_count & ~kSolidTriangles;
+            v.faces[0].neighbor = n > 8 ? build_tri_bvh(s, v.plane_begin, n) : kInvalid;
+        }
+    }
     // what lies across each mesh face: the siblings whose boxes meet the box
     // of the face's points offset by the boundary tolerance along the outward
     // normal, widened by a margin far above rounding (a superset of the
     // siblings the run-time point query can find there)
     for (uint32_t i = 0; i < in.n_volumes; ++i) {
         const Volume& v = s.volumes[i];
-        for (uint32_t k = v.plane_begin; k < v.plane_begin + v.plane_count; ++k) {
-            Plane& pl = s.planes[k];
+        const bool tri = (v.plane_count & kSolidTriangles) != 0;
+        const uint32_t n_faces = v.plane_count & ~kSolidTriangles;
+        for (uint32_t k = v.plane_begin; k < v.plane_begin + n_faces; ++k) {
+            Plane& pl = tri ? s.tris[k].face : s.planes[k];
             if (v.parent == kInvalid) {
                 pl.across = kInvalid;  // leaving the world: escape
                 continue;
             }
-            const std::vector<double>& pts = s.face_points[k];
+            const std::vector<double>& pts = tri ? s.tri_points[k] : s.face_points[k];
             double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
             double scale = 0.0;
             for (size_t j = 0; j < pts.size(); j += 3) {
@@ -751,6 +916,7 @@ HostScene build_scene(const g4go_scene& in, float tolerance) {
         }
     }
     s.face_points.clear();
+    s.tri_points.clear();
     if (s.pool.empty()) {
         s.pool.push_back(0.0f);
     }
@@ -923,8 +1089,8 @@ struct Core {
         off = align256(off + std::max<size_t>(1, s.bvh.size()) * sizeof(BvhNode));
         const size_t o_dbox = off;
         off = align256(off + std::max<size_t>(1, s.daughters.size()) * sizeof(DaughterBox));
-        const size_t o_planes = off;
-        off = align256(off + std::max<size_t>(1, s.planes.size()) * sizeof(Plane));
+        const size_t o_planes = off;  // planes, then triangles
+        off = align256(off + std::max<size_t>(1, s.planes.size() * sizeof(Plane) + s.tris.size() * sizeof(Tri)));
         std::vector<char> blob(off, 0);
         // daughter boxes in daughter-list order (after the BVH build reordered the list)
         for (size_t k = 0; k < s.daughters.size(); ++k) {
@@ -934,6 +1100,10 @@ struct Core {
             std::memcpy(blob.data() + o_dbox + k * sizeof(DaughterBox), &b, sizeof(b));
         }
         if (!s.planes.empty()) std::memcpy(blob.data() + o_planes, s.planes.data(), s.planes.size() * sizeof(Plane));
+        if (!s.tris.empty()) {
+            std::memcpy(blob.data() + o_planes + s.planes.size() * sizeof(Plane), s.tris.data(),
+                        s.tris.size() * sizeof(Tri));
+        }
         std::memcpy(blob.data() + o_pool, s.pool.data(), s.pool.size() * sizeof(float));
         if (!s.materials.empty()) std::memcpy(blob.data() + o_mat, s.materials.data(), s.materials.size() * sizeof(Material));
         if (!s.surfaces.empty()) std::memcpy(blob.data() + o_surf, s.surfaces.data(), s.surfaces.size() * sizeof(Surface));
@@ -954,6 +1124,7 @@ struct Core {
         view.dbox = reinterpret_cast<const DaughterBox*>(base + o_dbox);
         view.planes = reinterpret_cast<const Plane*>(base + o_planes);
         view.n_planes = static_cast<uint32_t>(s.planes.size());
+        view.n_tris = static_cast<uint32_t>(s.tris.size());
         view.n_daughters = static_cast<uint32_t>(s.daughters.size());
         view.hot_offset = s.hot_offset;
         view.hot_count = s.hot_count;
@@ -1023,10 +1194,10 @@ struct Core {
         };
         if (fits(static_cast<uint64_t>(view.n_volumes) * sizeof(Volume))) p.lds_records |= kLdsVolumes;
         if (fits(static_cast<uint64_t>(view.n_surfaces) * sizeof(Surface))) p.lds_records |= kLdsSurfaces;
-        if (fits(static_cast<uint64_t>(view.n_planes) * sizeof(Plane))) p.lds_records |= kLdsPlanes;
+        if (fits(mesh_record_bytes(view))) p.lds_records |= kLdsPlanes;
         // (a mesh face's sibling test reads the daughter boxes too)
         const uint64_t entries = static_cast<uint64_t>(view.n_bvh_nodes) + view.n_daughters;
-        if ((view.n_bvh_nodes != 0 || view.n_planes != 0) && fits(entries * 32u)) {
+        if ((view.n_bvh_nodes != 0 || has_meshes(view)) && fits(entries * 32u)) {
             p.lds_entries = static_cast<uint32_t>(entries);
         }
         if (fits((static_cast<uint64_t>(view.n_pool) + 3u) / 4u * 16u)) p.lds_pool_floats = view.n_pool;

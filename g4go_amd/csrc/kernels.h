@@ -81,7 +81,7 @@ struct KernelParams {
     // when it fits; otherwise read in global memory): BVH nodes + daughter
     // boxes (lds_entries 32-byte entries), the property pool (lds_pool_floats
     // floats, padded to 16 bytes), then the volume, surface, mesh plane and
-    // material records selected by lds_records (kLdsVolumes | kLdsSurfaces |
+    // triangle, and material records selected by lds_records (kLdsVolumes | kLdsSurfaces |
     // kLdsPlanes | kLdsMaterials).  transport_lds_bytes gives the dynamic LDS
     // size.
     uint32_t lds_entries;
@@ -120,7 +120,7 @@ inline size_t transport_lds_bytes(const KernelParams& p) {
     size_t bytes = static_cast<size_t>(p.lds_entries) * 32u + (static_cast<size_t>(p.lds_pool_floats) + 3u) / 4u * 16u;
     if (p.lds_records & kLdsVolumes) bytes += static_cast<size_t>(p.scene.n_volumes) * sizeof(Volume);
     if (p.lds_records & kLdsSurfaces) bytes += static_cast<size_t>(p.scene.n_surfaces) * sizeof(Surface);
-    if (p.lds_records & kLdsPlanes) bytes += static_cast<size_t>(p.scene.n_planes) * sizeof(Plane);
+    if (p.lds_records & kLdsPlanes) bytes += mesh_record_bytes(p.scene);
     if (p.lds_records & kLdsMaterials) bytes += static_cast<size_t>(p.scene.n_materials) * sizeof(Material);
     return bytes;
 }

@@ -197,7 +197,7 @@ __device__ __forceinline__ void store_hit(float4* d, const Photon& ph, uint32_t 
 enum LdsMode : int { kLdsNone = 0, kLdsHot = 1, kLdsAll = 2 };
 
 inline int lds_mode(const KernelParams& p) {
-    const uint32_t hot = kLdsVolumes | kLdsSurfaces | (p.scene.n_planes != 0u ? kLdsPlanes : 0u);
+    const uint32_t hot = kLdsVolumes | kLdsSurfaces | (has_meshes(p.scene) ? kLdsPlanes : 0u);
     const uint32_t all = hot | kLdsMaterials;
     if ((p.lds_records & all) == all && p.scene.n_pool != 0 && p.lds_pool_floats == p.scene.n_pool) {
         return kLdsAll;
@@ -210,7 +210,7 @@ inline int lds_mode(const KernelParams& p) {
 // BVH nodes + daughter boxes, pool, volumes, surfaces, mesh planes,
 // materials, and points sv at the copies.  The caller synchronises the block
 // before using them.
-template <int kLdsMode, bool kMesh = false>
+template <int kLdsMode, int kGeom = kGeomBoxes>
 __device__ __forceinline__ void stage_scene(const KernelParams& p, float4* lds, SceneView& sv) {
     if (p.lds_entries != 0u) {
         const uint32_t node_words = 2u * p.scene.n_bvh_nodes;
@@ -246,8 +246,11 @@ __device__ __forceinline__ void stage_scene(const KernelParams& p, float4* lds, 
         sv.surfaces =
             static_cast<const Surface*>(stage_quads(p.scene.surfaces, p.scene.n_surfaces * (sizeof(Surface) / 16u)));
     }
-    if (kMesh && (kLdsMode >= kLdsHot || (p.lds_records & kLdsPlanes))) {  // box scenes have no planes
-        sv.planes = static_cast<const Plane*>(stage_quads(p.scene.planes, p.scene.n_planes * (sizeof(Plane) / 16u)));
+    if (kGeom != kGeomBoxes && (kLdsMode >= kLdsHot || (p.lds_records & kLdsPlanes))) {  // box scenes have none
+        // planes then triangles, whole 16-byte words (triangles only at kGeomTriangles)
+        const uint32_t quads = p.scene.n_planes * (sizeof(Plane) / 16u) +
+                               (kGeom == kGeomTriangles ? p.scene.n_tris * (sizeof(Tri) / 16u) : 0u);
+        sv.planes = static_cast<const Plane*>(stage_quads(p.scene.planes, quads));
     }
     if (kLdsMode >= kLdsAll || (p.lds_records & kLdsMaterials)) {
         sv.materials = static_cast<const Material*>(
@@ -255,7 +258,7 @@ __device__ __forceinline__ void stage_scene(const KernelParams& p, float4* lds, 
     }
 }
 
-template <int kSource, bool kPersistent, int kLdsMode, bool kMesh>
+template <int kSource, bool kPersistent, int kLdsMode, int kGeom>
 __device__ __forceinline__ void transport_body(const KernelParams& p) {
     __shared__ uint32_t s_hist[kHistSlots];
     __shared__ float4 s_stage[kMaxWavesPerBlock][3 * kStageCap];
@@ -265,7 +268,7 @@ __device__ __forceinline__ void transport_body(const KernelParams& p) {
         s_hist[i] = 0u;
     }
     SceneView sv = p.scene;
-    stage_scene<kLdsMode, kMesh>(p, s_scene, sv);
+    stage_scene<kLdsMode, kGeom>(p, s_scene, sv);
     __syncthreads();
     float4* const stage = s_stage[threadIdx.x >> 6];
     uint32_t* const stage_src = s_stage_src[threadIdx.x >> 6];
@@ -364,7 +367,7 @@ __device__ __forceinline__ void transport_body(const KernelParams& p) {
         uint32_t sensor = kInvalid;
         if (alive) {
             bool zero = false;
-            status = transport_step<false, kMesh>(ph, sv, p.rng, p.cfg, zero, sensor, nullptr);
+            status = transport_step<false, kGeom>(ph, sv, p.rng, p.cfg, zero, sensor, nullptr);
             ++ph.step;
             st.zero_step += zero ? 1u : 0u;
             if (status == kAlive && ph.step >= p.max_steps) {
@@ -429,40 +432,54 @@ __device__ __forceinline__ void transport_body(const KernelParams& p) {
 // the compiler schedules the body into ~140 registers (three waves); the
 // fourth wave hides more of the dependent scene-load latency of a step than
 // the few spills it forces cost.
-template <int kSource, int kLdsMode, bool kMesh>
+template <int kSource, int kLdsMode, int kGeom>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
 transport_kernel_persistent(KernelParams p) {
-    transport_body<kSource, true, kLdsMode, kMesh>(p);
+    transport_body<kSource, true, kLdsMode, kGeom>(p);
 }
 
-template <int kSource, int kLdsMode, bool kMesh>
+template <int kSource, int kLdsMode, int kGeom>
 __global__ void __launch_bounds__(256) transport_kernel_plain(KernelParams p) {
-    transport_body<kSource, false, kLdsMode, kMesh>(p);
+    transport_body<kSource, false, kLdsMode, kGeom>(p);
 }
 
-template <int kSource, int kLdsMode, bool kMesh = false>
+template <int kSource, int kLdsMode, int kGeom = kGeomBoxes>
 void launch_transport_form(const KernelParams& p, bool persistent, int block, int grid, size_t lds, hipStream_t stream) {
     if (persistent) {
-        hipLaunchKernelGGL((transport_kernel_persistent<kSource, kLdsMode, kMesh>), dim3(grid), dim3(block), lds,
+        hipLaunchKernelGGL((transport_kernel_persistent<kSource, kLdsMode, kGeom>), dim3(grid), dim3(block), lds,
                            stream, p);
     } else {
-        hipLaunchKernelGGL((transport_kernel_plain<kSource, kLdsMode, kMesh>), dim3(grid), dim3(block), lds, stream,
+        hipLaunchKernelGGL((transport_kernel_plain<kSource, kLdsMode, kGeom>), dim3(grid), dim3(block), lds, stream,
                            p);
     }
 }
 
-// Scenes with mesh solids run their own instances, so the box scenes' kernels
-// carry no mesh code.
+// Scenes run the instances of the lowest geometry level that holds them, so
+// box scenes carry no mesh code and convex-mesh scenes no triangle code.
+// Triangle scenes have two LDS modes (their triangles rarely fit).
+__host__ __device__ inline int geom_level(const SceneView& sc) {
+    return sc.n_tris != 0u ? kGeomTriangles : (sc.n_planes != 0u ? kGeomConvex : kGeomBoxes);
+}
+
 template <int kSource>
 void launch_transport_source(const KernelParams& p, bool persistent, int block, int grid, size_t lds,
                              hipStream_t stream) {
-    if (p.scene.n_planes != 0u) {
-        if (lds_mode(p) == kLdsAll) {
-            launch_transport_form<kSource, kLdsAll, true>(p, persistent, block, grid, lds, stream);
-        } else if (lds_mode(p) == kLdsHot) {
-            launch_transport_form<kSource, kLdsHot, true>(p, persistent, block, grid, lds, stream);
+    const int geom = geom_level(p.scene);
+    if (geom == kGeomTriangles) {
+        if (lds_mode(p) >= kLdsHot) {
+            launch_transport_form<kSource, kLdsHot, kGeomTriangles>(p, persistent, block, grid, lds, stream);
         } else {
-            launch_transport_form<kSource, kLdsNone, true>(p, persistent, block, grid, lds, stream);
+            launch_transport_form<kSource, kLdsNone, kGeomTriangles>(p, persistent, block, grid, lds, stream);
+        }
+        return;
+    }
+    if (geom == kGeomConvex) {
+        if (lds_mode(p) == kLdsAll) {
+            launch_transport_form<kSource, kLdsAll, kGeomConvex>(p, persistent, block, grid, lds, stream);
+        } else if (lds_mode(p) == kLdsHot) {
+            launch_transport_form<kSource, kLdsHot, kGeomConvex>(p, persistent, block, grid, lds, stream);
+        } else {
+            launch_transport_form<kSource, kLdsNone, kGeomConvex>(p, persistent, block, grid, lds, stream);
         }
         return;
     }
@@ -654,7 +671,7 @@ __device__ __forceinline__ void add_material_grads(double* grad, const SceneView
 
 // Four waves per SIMD (at most 128 VGPRs), as for the persistent transport
 // kernel: the replay runs the same step code.
-template <int kSource, int kLdsMode, bool kMesh>
+template <int kSource, int kLdsMode, int kGeom>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) adjoint_kernel(AdjointParams a) {
     extern __shared__ double s_grad[];  // gradient accumulator, then the staged scene
     const uint32_t n_params = a.map.n_params;
@@ -667,7 +684,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) a
     }
     const KernelParams& p = a.fwd;
     SceneView sc = p.scene;
-    stage_scene<kLdsMode, kMesh>(p, reinterpret_cast<float4*>(s_grad) + adjoint_grad_quads(n_params), sc);
+    stage_scene<kLdsMode, kGeom>(p, reinterpret_cast<float4*>(s_grad) + adjoint_grad_quads(n_params), sc);
     __syncthreads();
     const uint32_t n_sources = p.header != nullptr ? p.header[0] : 0u;
     uint32_t replayed = 0, mismatch = 0, unsupported = 0, steps = 0;
@@ -694,7 +711,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) a
         for (;;) {
             bool zero = false;
             StepTrace tr;
-            status = transport_step<true, kMesh>(ph, sc, p.rng, p.cfg, zero, sensor, &tr);
+            status = transport_step<true, kGeom>(ph, sc, p.rng, p.cfg, zero, sensor, &tr);
             ++ph.step;
             ++steps;
             if (tr.segment > 0.0f || tr.event == kTraceRayleigh) {
@@ -776,25 +793,37 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) a
 
 template <int kSource>
 void launch_adjoint_source(const AdjointParams& p, int block, int grid, size_t lds, hipStream_t stream) {
-    if (p.fwd.scene.n_planes != 0u) {
-        if (lds_mode(p.fwd) == kLdsAll) {
-            hipLaunchKernelGGL((adjoint_kernel<kSource, kLdsAll, true>), dim3(grid), dim3(block), lds, stream, p);
-        } else if (lds_mode(p.fwd) == kLdsHot) {
-            hipLaunchKernelGGL((adjoint_kernel<kSource, kLdsHot, true>), dim3(grid), dim3(block), lds, stream, p);
+    const int geom = geom_level(p.fwd.scene);
+    if (geom == kGeomTriangles) {
+        if (lds_mode(p.fwd) >= kLdsHot) {
+            hipLaunchKernelGGL((adjoint_kernel<kSource, kLdsHot, kGeomTriangles>), dim3(grid), dim3(block), lds, stream,
+                               p);
         } else {
-            hipLaunchKernelGGL((adjoint_kernel<kSource, kLdsNone, true>), dim3(grid), dim3(block), lds, stream, p);
+            hipLaunchKernelGGL((adjoint_kernel<kSource, kLdsNone, kGeomTriangles>), dim3(grid), dim3(block), lds,
+                               stream, p);
+        }
+        return;
+    }
+    if (geom == kGeomConvex) {
+        if (lds_mode(p.fwd) == kLdsAll) {
+            hipLaunchKernelGGL((adjoint_kernel<kSource, kLdsAll, kGeomConvex>), dim3(grid), dim3(block), lds, stream, p);
+        } else if (lds_mode(p.fwd) == kLdsHot) {
+            hipLaunchKernelGGL((adjoint_kernel<kSource, kLdsHot, kGeomConvex>), dim3(grid), dim3(block), lds, stream, p);
+        } else {
+            hipLaunchKernelGGL((adjoint_kernel<kSource, kLdsNone, kGeomConvex>), dim3(grid), dim3(block), lds, stream,
+                               p);
         }
         return;
     }
     switch (lds_mode(p.fwd)) {
         case kLdsAll:
-            hipLaunchKernelGGL((adjoint_kernel<kSource, kLdsAll, false>), dim3(grid), dim3(block), lds, stream, p);
+            hipLaunchKernelGGL((adjoint_kernel<kSource, kLdsAll, kGeomBoxes>), dim3(grid), dim3(block), lds, stream, p);
             break;
         case kLdsHot:
-            hipLaunchKernelGGL((adjoint_kernel<kSource, kLdsHot, false>), dim3(grid), dim3(block), lds, stream, p);
+            hipLaunchKernelGGL((adjoint_kernel<kSource, kLdsHot, kGeomBoxes>), dim3(grid), dim3(block), lds, stream, p);
             break;
         default:
-            hipLaunchKernelGGL((adjoint_kernel<kSource, kLdsNone, false>), dim3(grid), dim3(block), lds, stream, p);
+            hipLaunchKernelGGL((adjoint_kernel<kSource, kLdsNone, kGeomBoxes>), dim3(grid), dim3(block), lds, stream, p);
             break;
     }
 }
@@ -933,9 +962,13 @@ __global__ void trace_kernel(KernelParams p, uint32_t event_id, uint32_t photon_
         for (; n < max_steps;) {
             bool zero = false;
             uint32_t sensor = kInvalid;
-            const int status = p.scene.n_planes != 0u
-                                   ? transport_step<false, true>(ph, p.scene, p.rng, p.cfg, zero, sensor, nullptr)
-                                   : transport_step<false, false>(ph, p.scene, p.rng, p.cfg, zero, sensor, nullptr);
+            const int geom = geom_level(p.scene);
+            const int status =
+                geom == kGeomTriangles
+                    ? transport_step<false, kGeomTriangles>(ph, p.scene, p.rng, p.cfg, zero, sensor, nullptr)
+                    : (geom == kGeomConvex
+                           ? transport_step<false, kGeomConvex>(ph, p.scene, p.rng, p.cfg, zero, sensor, nullptr)
+                           : transport_step<false, kGeomBoxes>(ph, p.scene, p.rng, p.cfg, zero, sensor, nullptr));
             ++ph.step;
             float* o = out + 16 * n;
             o[0] = ph.pos.x;

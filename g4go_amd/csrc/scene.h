@@ -2,16 +2,18 @@
 //
 // Reference model: include/g4go/optical/Scene.hpp:61-164 and the device copy
 // in src/optical/optix/OptiXDeviceData.cuh:68-144.  Differences:
-//   * a volume's solid is an axis-aligned box (analytic slab tests) or a
-//     closed convex triangle mesh held as the planes of its faces (clipping
-//     tests), in place of the OptiX triangle traversal, with a parent
-//     relation for nesting;
+//   * a volume's solid is an axis-aligned box (analytic slab tests), a closed
+//     convex triangle mesh held as the planes of its faces (clipping tests),
+//     or any other closed triangle mesh held as its triangles under a BVH
+//     (watertight ray/triangle tests), in place of the OptiX traversal, with
+//     a parent relation for nesting;
 //   * every property table lives in one float pool (energies then values),
 //     addressed by offset, so the whole scene is a handful of arrays;
 //   * the border/skin surface of every static box face, and of every
 //     parent->daughter entry, is resolved on the host (Scene.cpp:120-150).
 #pragma once
 
+#include <stddef.h>
 #include <stdint.h>
 
 namespace g4go {
@@ -67,7 +69,10 @@ struct alignas(16) Volume {
     uint32_t entry_surface;                   // surface crossed when entering from the parent
     uint32_t bvh_root;                        // root node of the daughter BVH, kInvalid = linear list
     // convex mesh solid: planes [plane_begin, plane_begin + plane_count) of
-    // the scene's plane list, lo/hi its bounding box; plane_count 0: the box lo/hi
+    // the scene's plane list; triangle solid (plane_count has kSolidTriangles
+    // set): triangles [plane_begin, plane_begin + count) of the triangle
+    // list, BVH root in faces[0].neighbor (kInvalid: no BVH); in both, lo/hi
+    // is the solid's bounding box.  plane_count 0: the box lo/hi.
     uint32_t plane_begin, plane_count;
     // face = 2*axis + (outward sign > 0); the neighbour and its surface sit
     // side by side so a step reads both with one 8-byte load
@@ -98,7 +103,26 @@ struct alignas(16) Plane {
     uint32_t surface_parent, surface_across;
     uint32_t pad[2];
 };
-constexpr uint32_t kMaxMeshPlanes = 1024;  // faces per mesh solid
+constexpr uint32_t kMaxMeshPlanes = 1024;  // faces per convex mesh solid
+
+constexpr uint32_t kSolidTriangles = 0x80000000u;  // Volume::plane_count flag
+constexpr uint32_t kFaceTri = 0x80000000u;         // face reference: a triangle (else a plane)
+constexpr uint32_t kMaxSolidTriangles = 65536;     // triangles per triangle solid
+
+// Triangle of a non-convex mesh solid, wound so that (v1 - v0) x (v2 - v0)
+// points out of the solid.  `face` holds its plane (outward), boundary normal
+// and what lies across it, as for a convex solid's face; `key` is the
+// triangle's index in the caller's mesh (ties between equal hit distances go
+// to the lower key, whatever order the BVH stores the triangles in).
+struct alignas(16) Tri {
+    Plane face;
+    float v0[3];
+    uint32_t key;
+    float v1[3];
+    uint32_t pad1;
+    float v2[3];
+    uint32_t pad2;
+};
 
 struct Binding {
     uint32_t from, to, surface, pad;  // sorted by (from, to)
@@ -130,7 +154,7 @@ struct SceneView {
     const Material* materials;
     const Surface* surfaces;
     const Volume* volumes;
-    const Plane* planes;  // faces of the convex mesh solids
+    const Plane* planes;  // faces of the convex mesh solids, then the triangles of the others (scene_tris)
     const Binding* bindings;
     const BvhNode* bvh;
     const DaughterBox* dbox;  // one per daughter-list slot (volume index and box)
@@ -141,8 +165,22 @@ struct SceneView {
     // energy never changes, so its node on this grid is located once.
     uint32_t hot_offset, hot_count;
     uint32_t n_pool;    // floats in the pool
-    uint32_t n_planes;  // 0: every solid is a box
-    uint32_t pad;
+    uint32_t n_planes;  // 0 (and n_tris 0): every solid is a box
+    uint32_t n_tris;
 };
+
+// Scenes with mesh solids (convex or triangle).
+inline __host__ __device__ bool has_meshes(const SceneView& sc) { return (sc.n_planes | sc.n_tris) != 0u; }
+
+// Bytes of the plane and triangle records (one array).
+inline __host__ __device__ size_t mesh_record_bytes(const SceneView& sc) {
+    return static_cast<size_t>(sc.n_planes) * sizeof(Plane) + static_cast<size_t>(sc.n_tris) * sizeof(Tri);
+}
+
+// The triangle records follow the plane records in one array (in global
+// memory and in the LDS copy alike).
+inline __host__ __device__ const Tri* scene_tris(const SceneView& sc) {
+    return reinterpret_cast<const Tri*>(sc.planes + sc.n_planes);
+}
 
 }  // namespace g4go

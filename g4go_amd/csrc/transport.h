@@ -460,6 +460,186 @@ G4GO_HD bool mesh_contains(const Plane* planes, uint32_t begin, uint32_t count, 
     return true;
 }
 
+// ---------------------------------------------------------------------------
+// Geometry: closed non-convex mesh solids, held as their triangles (wound so
+// that (v1 - v0) x (v2 - v0) points outwards) under a BVH per solid.  Hits use
+// the watertight ray/triangle test of Woop, Benthin and Wald (Journal of
+// Computer Graphics Techniques 2(1), 2013): the ray is sheared onto its
+// dominant axis, the triangle's three edge functions are evaluated in 2D (in
+// FP64 when one is exactly zero), so a ray through a shared edge or vertex
+// never passes between the triangles there.  With outward winding the sign of
+// the determinant separates leaving hits (det < 0) from entering ones
+// (det > 0).  A point lies inside a solid when the first triangle along
+// kContainDir is a leaving one.
+// ---------------------------------------------------------------------------
+struct WtRay {
+    V3 o;
+    int kx, ky, kz;
+    float sx, sy, sz;
+};
+
+G4GO_HD WtRay wt_ray(V3 o, V3 d) {
+    const float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+    WtRay r;
+    r.o = o;
+    r.kz = (ax >= ay && ax >= az) ? 0 : (ay >= az ? 1 : 2);
+    r.kx = r.kz == 2 ? 0 : r.kz + 1;
+    r.ky = r.kx == 2 ? 0 : r.kx + 1;
+    const float dz = axis_of(d, r.kz);
+    if (dz < 0.0f) {  // keep the winding
+        const int k = r.kx;
+        r.kx = r.ky;
+        r.ky = k;
+    }
+    r.sx = axis_of(d, r.kx) / dz;
+    r.sy = axis_of(d, r.ky) / dz;
+    r.sz = 1.0f / dz;
+    return r;
+}
+
+// Hit distance of the ray on triangle tr and whether it leaves the solid;
+// false on a miss.
+G4GO_HD bool wt_hit(const WtRay& r, const Tri& tr, float& t, bool& leaving) {
+    const V3 a = sub(V3{tr.v0[0], tr.v0[1], tr.v0[2]}, r.o);
+    const V3 b = sub(V3{tr.v1[0], tr.v1[1], tr.v1[2]}, r.o);
+    const V3 c = sub(V3{tr.v2[0], tr.v2[1], tr.v2[2]}, r.o);
+    const float az = axis_of(a, r.kz), bz = axis_of(b, r.kz), cz = axis_of(c, r.kz);
+    const float ax = axis_of(a, r.kx) - r.sx * az, ay = axis_of(a, r.ky) - r.sy * az;
+    const float bx = axis_of(b, r.kx) - r.sx * bz, by = axis_of(b, r.ky) - r.sy * bz;
+    const float cx = axis_of(c, r.kx) - r.sx * cz, cy = axis_of(c, r.ky) - r.sy * cz;
+    float u = cx * by - cy * bx;
+    float v = ax * cy - ay * cx;
+    float w = bx * ay - by * ax;
+    if (u == 0.0f || v == 0.0f || w == 0.0f) {
+        u = static_cast<float>(static_cast<double>(cx) * by - static_cast<double>(cy) * bx);
+        v = static_cast<float>(static_cast<double>(ax) * cy - static_cast<double>(ay) * cx);
+        w = static_cast<float>(static_cast<double>(bx) * ay - static_cast<double>(by) * ax);
+    }
+    if ((u < 0.0f || v < 0.0f || w < 0.0f) && (u > 0.0f || v > 0.0f || w > 0.0f)) {
+        return false;
+    }
+    const float det = u + v + w;
+    if (det == 0.0f) {
+        return false;
+    }
+    t = (u * (r.sz * az) + v * (r.sz * bz) + w * (r.sz * cz)) / det;
+    leaving = det < 0.0f;
+    return true;
+}
+
+enum TriMode : int { kTriLeaving = 0, kTriEntering = 1, kTriAny = 2 };
+
+// Closest hit at t >= tmin on the triangles of triangle solid `vol` with the
+// orientation `mode` asks for: distance (kInf on none), face (kFaceTri | the
+// triangle's index, kInvalid on none) and whether it leaves the solid.  Ties
+// go to the lower key.  Out of line: inlined at its four call sites its
+// traversal stack costs the triangle-level kernels more in spills than the
+// calls do (13% on the L-shaped light guide).
+static __host__ __device__ __attribute__((noinline)) float tri_closest(const SceneView& sc, const Volume& vol, V3 p, V3 d, float tmin, int mode, uint32_t& face,
+                          bool& leaving) {
+    const Tri* tris = scene_tris(sc);
+    const WtRay ray = wt_ray(p, d);
+    float best = kInf;
+    uint32_t best_i = kInvalid, best_key = kInvalid;
+    bool best_leave = false;
+    const auto test = [&](uint32_t i) {
+        float t;
+        bool leave;
+        const Tri& tr = tris[i];
+        if (wt_hit(ray, tr, t, leave) && t >= tmin && (mode == kTriAny || leave == (mode == kTriLeaving)) &&
+            (t < best || (t == best && tr.key < best_key))) {
+            best = t;
+            best_i = i;
+            best_key = tr.key;
+            best_leave = leave;
+        }
+    };
+    const uint32_t begin = vol.plane_begin, count = vol.plane_count & ~kSolidTriangles;
+    const uint32_t root = vol.faces[0].neighbor;
+    if (root == kInvalid) {
+        for (uint32_t i = begin; i < begin + count; ++i) {
+            test(i);
+        }
+    } else {
+        // near-first walk, as for the daughter BVH; nodes beyond the best hit are skipped
+        const V3 inv = {1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
+        uint32_t stack[kBvhMaxDepth + 1];
+        float stack_t[kBvhMaxDepth + 1];
+        int top = 0;
+        {
+            const BvhNode& n0 = sc.bvh[root];
+            const float t = node_entry(n0.lo, n0.hi, p, inv, tmin);
+            if (t < kInf) {
+                stack[0] = root;
+                stack_t[0] = t;
+                top = 1;
+            }
+        }
+        while (top > 0) {
+            --top;
+            if (!(stack_t[top] <= best)) {
+                continue;
+            }
+            const BvhNode& node = sc.bvh[stack[top]];
+            if ((node.right_or_count & kBvhLeaf) != 0) {
+                const uint32_t n = node.right_or_count & ~kBvhLeaf;
+                for (uint32_t k = 0; k < n; ++k) {
+                    test(node.left + k);
+                }
+            } else {
+                const uint32_t l = node.left, rr = node.right_or_count;
+                const BvhNode& nl = sc.bvh[l];
+                const BvhNode& nr = sc.bvh[rr];
+                const float tl = node_entry(nl.lo, nl.hi, p, inv, tmin);
+                const float tr = node_entry(nr.lo, nr.hi, p, inv, tmin);
+                const bool left_near = tl <= tr;
+                const float t_far = left_near ? tr : tl;
+                const float t_near = left_near ? tl : tr;
+                if (t_far < kInf && t_far <= best && top <= kBvhMaxDepth) {
+                    stack[top] = left_near ? rr : l;
+                    stack_t[top] = t_far;
+                    ++top;
+                }
+                if (t_near < kInf && t_near <= best && top <= kBvhMaxDepth) {
+                    stack[top] = left_near ? l : rr;
+                    stack_t[top] = t_near;
+                    ++top;
+                }
+            }
+        }
+    }
+    face = best_i == kInvalid ? kInvalid : (best_i | kFaceTri);
+    leaving = best_leave;
+    return best;
+}
+
+// A direction that meets no axis-aligned edge or face of a mesh edge-on:
+// (1, 2, 3) / sqrt(14).
+constexpr float kContainDir[3] = {0.26726124f, 0.53452248f, 0.80178373f};
+
+G4GO_HD bool tri_contains(const SceneView& sc, const Volume& vol, V3 q) {
+    uint32_t face;
+    bool leave;
+    const float t = tri_closest(sc, vol, q, V3{kContainDir[0], kContainDir[1], kContainDir[2]}, 0.0f, kTriAny, face,
+                                leave);
+    return t < kInf && leave;
+}
+
+// Point inside a mesh solid (convex, or a triangle solid at kGeomTriangles).
+template <int kGeom>
+G4GO_HD bool solid_contains(const SceneView& sc, const Volume& v, V3 q) {
+    if (kGeom == 2 && (v.plane_count & kSolidTriangles) != 0u) {
+        return tri_contains(sc, v, q);
+    }
+    return mesh_contains(sc.planes, v.plane_begin, v.plane_count, q);
+}
+
+// The face record of a mesh face reference (a plane or a triangle).
+template <int kGeom>
+G4GO_HD const Plane& face_record(const SceneView& sc, uint32_t face) {
+    return kGeom == 2 && (face & kFaceTri) != 0u ? scene_tris(sc)[face & ~kFaceTri].face : sc.planes[face];
+}
+
 // Border binding lookup with skin precedence (OptiXTransportDevice.cu:218-252).
 G4GO_HD uint32_t find_surface(const SceneView& sc, uint32_t from, uint32_t to) {
     uint32_t first = 0, last = sc.n_bindings;
@@ -486,13 +666,22 @@ G4GO_HD uint32_t find_surface(const SceneView& sc, uint32_t from, uint32_t to) {
     return f.skin != kInvalid ? f.skin : t.skin;
 }
 
-// Entry distance into one daughter: its box, or with kMesh its mesh when it
-// has one (plane = the entered face, else kInvalid).
-template <bool kMesh>
+// Geometry levels of the step code: kGeomBoxes (box solids only),
+// kGeomConvex (also convex mesh solids), kGeomTriangles (also triangle
+// solids).  A scene runs the lowest level that holds it.
+enum GeomLevel : int { kGeomBoxes = 0, kGeomConvex = 1, kGeomTriangles = 2 };
+
+// Entry distance into one daughter: its box, or (kGeom >= kGeomConvex) its
+// mesh when it has one (plane = the entered face, else kInvalid).
+template <int kGeom>
 G4GO_HD float daughter_entry(const SceneView& sc, const DaughterBox& b, V3 p, V3 d, V3 inv, float tmin, int& axis,
                              uint32_t& plane) {
-    if (kMesh && b.mesh != 0u) {
+    if (kGeom != kGeomBoxes && b.mesh != 0u) {
         const Volume& v = sc.volumes[b.volume];
+        if (kGeom == kGeomTriangles && (v.plane_count & kSolidTriangles) != 0u) {
+            bool leave;
+            return tri_closest(sc, v, p, d, tmin, kTriEntering, plane, leave);
+        }
         return mesh_entry(sc.planes, v.plane_begin, v.plane_count, p, d, tmin, plane);
     }
     plane = kInvalid;
@@ -500,8 +689,8 @@ G4GO_HD float daughter_entry(const SceneView& sc, const DaughterBox& b, V3 p, V3
 }
 
 // Nearest daughter entry of volume `vol` (BVH walk when the volume has one).
-// `plane` (kMesh only) is the entered mesh face, kInvalid for a box face.
-template <bool kMesh>
+// `plane` (mesh levels only) is the entered mesh face, kInvalid for a box face.
+template <int kGeom>
 G4GO_HD float nearest_daughter(const SceneView& sc, const Volume& vol, V3 p, V3 d, float tmin, uint32_t& which,
                                int& axis, uint32_t& plane) {
     const V3 inv = {1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
@@ -513,7 +702,7 @@ G4GO_HD float nearest_daughter(const SceneView& sc, const Volume& vol, V3 p, V3 
             const uint32_t j = b.volume;
             int a = 0;
             uint32_t f = kInvalid;
-            const float t = daughter_entry<kMesh>(sc, b, p, d, inv, tmin, a, f);
+            const float t = daughter_entry<kGeom>(sc, b, p, d, inv, tmin, a, f);
             if (t < best || (t == best && t < kInf && j < which)) {  // ties: lower volume index
                 best = t;
                 which = j;
@@ -553,7 +742,7 @@ G4GO_HD float nearest_daughter(const SceneView& sc, const Volume& vol, V3 p, V3 
                 const uint32_t j = b.volume;
                 int a = 0;
                 uint32_t f = kInvalid;
-                const float t = daughter_entry<kMesh>(sc, b, p, d, inv, tmin, a, f);
+                const float t = daughter_entry<kGeom>(sc, b, p, d, inv, tmin, a, f);
                 if (t < best || (t == best && t < kInf && j < which)) {  // ties: lower volume index
                     best = t;
                     which = j;
@@ -587,7 +776,7 @@ G4GO_HD float nearest_daughter(const SceneView& sc, const Volume& vol, V3 p, V3 
 
 // Neighbour across a face found at run time: the first sibling solid that
 // holds the point just outside the face, else the parent volume.
-template <bool kMesh>
+template <int kGeom>
 G4GO_HD uint32_t point_neighbor(const SceneView& sc, uint32_t current, V3 q) {
     const Volume& cur = sc.volumes[current];
     if (cur.parent == kInvalid) {
@@ -598,11 +787,8 @@ G4GO_HD uint32_t point_neighbor(const SceneView& sc, uint32_t current, V3 q) {
         const DaughterBox& b = sc.dbox[par.daughter_begin + k];
         if (b.volume != current && q.x >= b.lo[0] && q.x <= b.hi[0] && q.y >= b.lo[1] && q.y <= b.hi[1] &&
             q.z >= b.lo[2] && q.z <= b.hi[2]) {
-            if (kMesh && b.mesh != 0u) {
-                const Volume& v = sc.volumes[b.volume];
-                if (!mesh_contains(sc.planes, v.plane_begin, v.plane_count, q)) {
-                    continue;
-                }
+            if (kGeom != kGeomBoxes && b.mesh != 0u && !solid_contains<kGeom>(sc, sc.volumes[b.volume], q)) {
+                continue;
             }
             return b.volume;
         }
@@ -921,12 +1107,13 @@ struct StepConfig {
     float eps;  // boundary tolerance (mm)
 };
 
-// kMesh: the scene has convex mesh solids (SceneView::n_planes != 0).  Their
-// faces are found by clipping, the photon is not snapped onto a mesh face,
+// kGeom: the geometry level (kGeomConvex: convex mesh solids, whose faces
+// are found by clipping; kGeomTriangles: also triangle solids, whose faces
+// are found by the watertight triangle test).  On mesh faces the photon is not snapped onto a mesh face,
 // and a daughter entered up to eps beyond the current volume's exit wins (a
 // daughter face coincident with the wall, which the reference settles by its
 // surface rank re-trace, OptiXTransportDevice.cu:105-127, 742-777).
-template <bool kTrace, bool kMesh = false>
+template <bool kTrace, int kGeom = kGeomBoxes>
 G4GO_HD int transport_step(Photon& ph, const SceneView& sc, const Rng& rng, const StepConfig& cfg, bool& zero_step,
                            uint32_t& sensor_out, StepTrace* trace) {
     zero_step = false;
@@ -948,11 +1135,18 @@ G4GO_HD int transport_step(Photon& ph, const SceneView& sc, const Rng& rng, cons
     }
 
     // nearest boundary: leave the solid, or enter a daughter; `plane` is the
-    // mesh face crossed (kMesh only), kInvalid for a box face
+    // mesh face crossed (mesh levels only), kInvalid for a box face
     float dist;
     int axis = 0;
     uint32_t plane = kInvalid;
-    if (kMesh && vol.plane_count != 0u) {
+    constexpr bool kMesh = kGeom != kGeomBoxes;
+    if (kGeom == kGeomTriangles && (vol.plane_count & kSolidTriangles) != 0u) {
+        bool leave;
+        dist = tri_closest(sc, vol, ph.pos, ph.dir, 0.0f, kTriLeaving, plane, leave);
+        if (!finitef(dist)) {
+            return kInvalidState;  // no triangle ahead: the photon is not inside its solid
+        }
+    } else if (kMesh && vol.plane_count != 0u) {
         mesh_exit(sc.planes, vol.plane_begin, vol.plane_count, ph.pos, ph.dir, dist, plane);
     } else {
         box_exit(vol, ph.pos, ph.dir, dist, axis);
@@ -964,7 +1158,7 @@ G4GO_HD int transport_step(Photon& ph, const SceneView& sc, const Rng& rng, cons
         uint32_t which;
         int a2 = 0;
         uint32_t p2 = kInvalid;
-        const float t = nearest_daughter<kMesh>(sc, vol, ph.pos, ph.dir, cfg.eps, which, a2, p2);
+        const float t = nearest_daughter<kGeom>(sc, vol, ph.pos, ph.dir, cfg.eps, which, a2, p2);
         if (which != kInvalid && (kMesh ? t <= dist + cfg.eps : t <= dist)) {
             dist = t;
             axis = a2;
@@ -1012,7 +1206,7 @@ G4GO_HD int transport_step(Photon& ph, const SceneView& sc, const Rng& rng, cons
     const float dsign = axis_of(ph.dir, axis) > 0.0f ? 1.0f : -1.0f;
     if (mesh_face) {
         // the face's boundary normal, turned to face the incoming photon (reference :789-791)
-        const Plane& pl = sc.planes[plane];
+        const Plane& pl = face_record<kGeom>(sc, plane);
         const V3 sn = {pl.shade[0], pl.shade[1], pl.shade[2]};
         normal = dot(ph.dir, sn) > 0.0f ? scale(sn, -1.0f) : sn;
     } else {
@@ -1035,11 +1229,11 @@ G4GO_HD int transport_step(Photon& ph, const SceneView& sc, const Rng& rng, cons
     if (!entering) {
         if (mesh_face) {
             // the point just outside the face, along its outward plane normal
-            const Plane& pl = sc.planes[plane];
+            const Plane& pl = face_record<kGeom>(sc, plane);
             const V3 q = add(bpos, scale(V3{pl.n[0], pl.n[1], pl.n[2]}, cfg.eps));
             const uint32_t across = pl.across;
             if (across == kFaceDynamic) {
-                next = point_neighbor<kMesh>(sc, ph.volume, q);
+                next = point_neighbor<kGeom>(sc, ph.volume, q);
                 surface = next == kInvalid ? kInvalid : find_surface(sc, ph.volume, next);
             } else {
                 next = vol.parent;
@@ -1048,8 +1242,7 @@ G4GO_HD int transport_step(Photon& ph, const SceneView& sc, const Rng& rng, cons
                     const DaughterBox& b = sc.dbox[across];
                     if (q.x >= b.lo[0] && q.x <= b.hi[0] && q.y >= b.lo[1] && q.y <= b.hi[1] && q.z >= b.lo[2] &&
                         q.z <= b.hi[2]) {
-                        const Volume& v = sc.volumes[b.volume];
-                        if (b.mesh == 0u || mesh_contains(sc.planes, v.plane_begin, v.plane_count, q)) {
+                        if (b.mesh == 0u || solid_contains<kGeom>(sc, sc.volumes[b.volume], q)) {
                             next = b.volume;
                             surface = pl.surface_across;
                         }
@@ -1063,7 +1256,7 @@ G4GO_HD int transport_step(Photon& ph, const SceneView& sc, const Rng& rng, cons
             next = fr.neighbor;
             if (next == kFaceDynamic) {
                 const V3 q = add(bpos, scale(normal, -cfg.eps));
-                next = point_neighbor<kMesh>(sc, ph.volume, q);
+                next = point_neighbor<kGeom>(sc, ph.volume, q);
                 surface = next == kInvalid ? kInvalid : find_surface(sc, ph.volume, next);
             } else {
                 surface = fr.surface;

@@ -129,19 +129,63 @@ def box_mesh(half_mm: Sequence[float]) -> Mesh:
     return frustum_mesh(half_mm[:2], half_mm[:2], half_mm[2])
 
 
+def _ear_clip(poly: np.ndarray) -> list:
+    """Triangles (counter-clockwise index triples) of a simple counter-clockwise polygon."""
+    def cross(o, a, b):
+        return (a[0] - o[0]) * (b[1] - o[1]) - (a[1] - o[1]) * (b[0] - o[0])
+
+    idx = list(range(len(poly)))
+    tris = []
+    while len(idx) > 3:
+        for k in range(len(idx)):
+            i0, i1, i2 = idx[k - 1], idx[k], idx[(k + 1) % len(idx)]
+            p0, p1, p2 = poly[i0], poly[i1], poly[i2]
+            if cross(p0, p1, p2) <= 0:
+                continue  # reflex corner
+            if any(cross(p0, p1, poly[j]) >= 0 and cross(p1, p2, poly[j]) >= 0 and cross(p2, p0, poly[j]) >= 0
+                   for j in idx if j not in (i0, i1, i2)):
+                continue  # another vertex inside the ear
+            tris.append((i0, i1, i2))
+            idx.pop(k)
+            break
+        else:
+            raise ValueError("polygon is not simple and counter-clockwise")
+    tris.append(tuple(idx))
+    return tris
+
+
 def prism_mesh(polygon_xy: Sequence[Sequence[float]], z_lo: float, z_hi: float) -> Mesh:
-    """Right prism over a convex polygon (counter-clockwise xy vertices) between z_lo and z_hi."""
+    """Right prism over a simple polygon (counter-clockwise xy vertices, convex
+    or not) between z_lo and z_hi, wound outwards; caps by ear clipping."""
     poly = np.asarray(polygon_xy, dtype=np.float64)
     n = len(poly)
     v = np.concatenate([np.c_[poly, np.full(n, z_lo)], np.c_[poly, np.full(n, z_hi)]])
     tri = []
-    for i in range(1, n - 1):
-        tri.append((0, i + 1, i))          # bottom
-        tri.append((n, n + i, n + i + 1))  # top
+    for a, b, c in _ear_clip(poly):
+        tri.append((a, c, b))              # bottom, facing -z
+        tri.append((n + a, n + b, n + c))  # top, facing +z
     for i in range(n):
         j = (i + 1) % n
         tri.append((i, j, n + j))
         tri.append((i, n + j, n + i))
+    return Mesh(v, np.array(tri))
+
+
+def tube_mesh(r_in: float, r_out: float, half_z: float, segments: int = 24) -> Mesh:
+    """Hollow cylinder (a non-convex solid) about the z axis: regular polygons
+    of `segments` sides inscribed in r_in and r_out, |z| <= half_z, wound outwards."""
+    ang = 2.0 * np.pi * np.arange(segments) / segments
+    ring = np.c_[np.cos(ang), np.sin(ang)]
+    v = np.concatenate([np.c_[r_out * ring, np.full(segments, -half_z)], np.c_[r_out * ring, np.full(segments, half_z)],
+                        np.c_[r_in * ring, np.full(segments, -half_z)], np.c_[r_in * ring, np.full(segments, half_z)]])
+    ob, ot, ib, it = 0, segments, 2 * segments, 3 * segments
+    tri = []
+    for k in range(segments):
+        m = (k + 1) % segments
+        tri += [(ob + k, ob + m, ot + m), (ob + k, ot + m, ot + k)]   # outer wall, facing out
+        tri += [(ib + m, ib + k, it + k), (ib + m, it + k, it + m)]   # inner wall, facing the axis
+        tri += [(ob + k, ib + k, ib + m), (ob + k, ib + m, ob + m)]   # bottom annulus, facing -z
+        tri += [(ot + k, ot + m, it + m), (ot + k, it + m, it + k)]   # top annulus, facing +z
     return Mesh(v, np.array(tri))
 
 

@@ -16,8 +16,9 @@
 * log-normal event sizes for the batched workloads;
 * mesh solids: the crystal tapered from a 3x3 mm front face to a 4x4 mm back
   face (a frustum given as a 12-triangle convex mesh, SiPM box on the back
-  face), and the single-crystal scene turned by a rotation (crystal and SiPM
-  as box meshes).
+  face), the single-crystal scene turned by a rotation (crystal and SiPM as
+  box meshes), and an L-shaped light guide (a non-convex prism, 20
+  triangles) read out at the end of its long arm.
 """
 from __future__ import annotations
 
@@ -27,7 +28,7 @@ import numpy as np
 
 from . import abi
 from .scene import (INVALID_ID, Material, PropertyTable, Scene, Surface, SurfaceBinding, SurfaceFinish,
-                    SurfaceModel, SurfaceType, Volume, box_mesh, frustum_mesh)
+                    SurfaceModel, SurfaceType, Volume, box_mesh, frustum_mesh, prism_mesh)
 
 HC_EV_NM = 1239.84198
 N_NODES = 32
@@ -200,6 +201,42 @@ def rotated_crystal_scene(rotation, p: CrystalParams | None = None, abs_nodes=No
         v = cs.scene.volumes[vid]
         v.mesh = box_mesh(v.half_mm).transformed(r, r @ np.asarray(v.center_mm, dtype=np.float64))
     return cs
+
+
+L_GUIDE_POLYGON = ((0.0, 0.0), (20.0, 0.0), (20.0, 5.0), (5.0, 5.0), (5.0, 15.0), (0.0, 15.0))
+
+
+def l_guide_scene(abs_length_mm: float = 30.0):
+    """L-shaped light guide of glass (n = 1.5, absorption length abs_length_mm
+    in a two-node table) in air: the L polygon L_GUIDE_POLYGON extruded over
+    z in [0, 4] mm, a non-convex prism.  Its faces are bare (Fresnel and total
+    internal reflection at glass/air, so photons can leave one arm and enter
+    the other); a silicon box on the end face x = 20 mm detects what reaches
+    it (R = 0, efficiency 0.8).  Photons start uniform in the box
+    [6, 19] x [0.5, 4.5] x [0.5, 3.5] mm of the long arm.  Returns (scene,
+    source, glass material, detector surface)."""
+    scene = Scene()
+    air = scene.add_material(Material(name="air", rindex=PropertyTable.constant_value(1.0)))
+    glass = scene.add_material(Material(name="glass", rindex=PropertyTable.constant_value(1.5),
+                                        abs_length_mm=PropertyTable(np.array([1.5, 4.5], np.float32),
+                                                                    np.array([abs_length_mm] * 2, np.float32))))
+    si = scene.add_material(Material(name="si", rindex=PropertyTable.constant_value(3.5)))
+    detect = scene.add_surface(Surface(name="detect", type=SurfaceType.DielectricMetal,
+                                       reflectivity=PropertyTable.constant_value(0.0),
+                                       efficiency=PropertyTable.constant_value(0.8)))
+    world = scene.add_volume(Volume(name="World", material_id=air, center_mm=(0, 0, 0), half_mm=(60, 60, 60)))
+    scene.world_volume_id = world
+    guide = scene.add_volume(Volume(name="Guide", material_id=glass, parent_volume_id=world,
+                                    mesh=prism_mesh(L_GUIDE_POLYGON, 0.0, 4.0)))
+    sensor = scene.add_volume(Volume(name="Sensor", material_id=si, parent_volume_id=world, sensor_id=0,
+                                     center_mm=(20.25, 2.5, 2.0), half_mm=(0.25, 2.5, 2.0)))
+    scene.add_surface_binding(SurfaceBinding(guide, sensor, detect))
+    src = abi.SyntheticSource()
+    src.lo_mm[:] = [6.0, 0.5, 0.5]
+    src.hi_mm[:] = [19.0, 4.5, 3.5]
+    src.wavelength_mean_nm, src.wavelength_sigma_nm, src.decay_time_ns = 420.0, 20.0, 40.0
+    src.volume_id = guide
+    return scene, src, glass, detect
 
 
 def rotation_matrix(axis, angle_rad: float) -> np.ndarray:

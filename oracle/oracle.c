@@ -22,9 +22,11 @@
  *   or_find_surface              OptiXTransportDevice.cu:218-252, src/optical/Scene.cpp:120-150
  *   or_facet / or_unified ...    OptiXTransportDevice.cu:492-595
  *   or_generate_emission         OptiXTransportDevice.cu:346-481
- *   mesh_planes / mesh_exit ..   convex mesh solids as face planes, in place of the
- *                                closest triangle hit, OptiXTransportDevice.cu:105-196,
- *                                triangles from OptiXTransportHost.cpp:1084-1200
+ *   mesh_solid / mesh_exit ..    convex mesh solids as face planes, other closed meshes
+ *   wt_hit / tri_closest         as triangles (watertight test, Woop et al. 2013), in
+ *                                place of the closest triangle hit,
+ *                                OptiXTransportDevice.cu:105-196, triangles from
+ *                                OptiXTransportHost.cpp:1084-1200
  *   or_step                      OptiXTransportDevice.cu:708-931 (bounce loop body)
  *   or_run                       OptiXTransportDevice.cu:660-941 + the batch
  *                                statistics of OptiXTransportHost.cpp:809-844
@@ -470,13 +472,20 @@ typedef struct {
 typedef struct {
     float lo[3], hi[3]; /* box bounds (a mesh's bounding box) are FP32 in both builds (as uploaded) */
     uint32_t id, material, parent, skin, sensor;
-    uint32_t plane_begin, plane_count; /* convex mesh solid; plane_count 0: the box */
+    uint32_t plane_begin, plane_count; /* mesh solid: planes (or triangles when tri); plane_count 0: the box */
+    int tri;
 } volume_t;
 
 /* Face plane of a convex mesh solid: inside where n.x <= h; s = boundary normal. */
 typedef struct {
     real n[3], h, s[3];
 } plane_t;
+
+/* Triangle of a non-convex mesh solid, wound outwards (its plane in face). */
+typedef struct {
+    plane_t face;
+    real v[3][3];
+} otri_t;
 
 typedef struct {
     uint32_t n_mat, n_surf, n_vol, n_bind, world;
@@ -486,6 +495,8 @@ typedef struct {
     uint32_t* bind; /* triples (from, to, surface) in input order */
     plane_t* planes;
     uint32_t n_planes;
+    otri_t* tris;
+    uint32_t n_tris;
 } oscene_t;
 
 static prop_t make_prop(const g4go_property* p) {
@@ -564,15 +575,20 @@ static uint32_t vol_index(const oscene_t* s, uint32_t id) {
     return INVALID;
 }
 
-/* Face planes of a closed convex mesh (OptiXTransportHost.cpp:1084-1200 builds
- * the reference's triangles; here their planes): per triangle, in FP64, the
- * unit normal of (v1 - v0) x (v2 - v0) and offset u.v0, turned so that the
- * vertex centroid lies inside; triangles whose normals, offsets (relative to
- * the largest centroid-vertex coordinate distance) and boundary normals agree
- * within 1e-9 share the first one's plane.  The mesh is assumed valid
- * (closed, convex): the library rejects others.  Appends to s->planes and
- * sets v's plane range and bounding box (rounded outwards to FP32). */
-static void mesh_planes(oscene_t* s, const g4go_volume_mesh* m, volume_t* v) {
+/* Solid of a closed mesh (OptiXTransportHost.cpp:1084-1200 builds the
+ * reference's triangles).  In FP64: per triangle the unit normal of
+ * (v1 - v0) x (v2 - v0) and offset u.v0, turned so that the vertex centroid
+ * lies inside; triangles whose normals, offsets (relative to the largest
+ * centroid-vertex coordinate distance) and boundary normals agree within 1e-9
+ * share the first one's plane (at most 1024 planes).  The mesh is convex when
+ * the centroid lies more than 1e-6 of that size inside every plane and no
+ * vertex more than that outside one: it becomes planes.  Otherwise it becomes
+ * its triangles in mesh order (key = index), all reversed when the signed
+ * volume about the centroid is negative.  The mesh is assumed valid (closed,
+ * consistently wound when not convex): the library rejects others.  Sets v's
+ * range and bounding box (rounded outwards to FP32). */
+#define MAX_PLANES 1024
+static void mesh_solid(oscene_t* s, const g4go_volume_mesh* m, volume_t* v) {
     const uint32_t nv = m->n_vertices, nt = m->n_triangles;
     const double* x = m->vertices_mm;
     double c[3] = {0.0, 0.0, 0.0}, lo[3], hi[3], size = 0.0;
@@ -593,8 +609,10 @@ static void mesh_planes(oscene_t* s, const g4go_volume_mesh* m, volume_t* v) {
             hi[k] = x[3 * i + k] > hi[k] ? x[3 * i + k] : hi[k];
         }
     }
+    const double tol = 1e-6 * size;
     double(*f64)[7] = (double(*)[7])calloc(nt, sizeof(*f64)); /* u[3], h, s[3] */
     uint32_t nf = 0;
+    int convex = 1;
     for (uint32_t t = 0; t < nt; ++t) {
         const double* p0 = x + 3 * m->indices[3 * t];
         const double* p1 = x + 3 * m->indices[3 * t + 1];
@@ -631,21 +649,76 @@ static void mesh_planes(oscene_t* s, const g4go_volume_mesh* m, volume_t* v) {
             dup = same;
         }
         if (!dup) {
+            if (nf == MAX_PLANES) {
+                convex = 0;
+                break;
+            }
             memcpy(f64[nf++], f, sizeof(f));
         }
     }
-    s->planes = (plane_t*)realloc(s->planes, (s->n_planes + nf) * sizeof(plane_t));
-    v->plane_begin = s->n_planes;
-    v->plane_count = nf;
-    for (uint32_t j = 0; j < nf; ++j) {
-        plane_t* pl = &s->planes[s->n_planes + j];
-        for (int k = 0; k < 3; ++k) {
-            pl->n[k] = (real)f64[j][k];
-            pl->s[k] = (real)f64[j][4 + k];
+    for (uint32_t j = 0; convex && j < nf; ++j) {
+        convex = f64[j][3] - (f64[j][0] * c[0] + f64[j][1] * c[1] + f64[j][2] * c[2]) > tol;
+        for (uint32_t i = 0; convex && i < nv; ++i) {
+            convex = !(f64[j][0] * x[3 * i] + f64[j][1] * x[3 * i + 1] + f64[j][2] * x[3 * i + 2] - f64[j][3] > tol);
         }
-        pl->h = (real)f64[j][3];
     }
-    s->n_planes += nf;
+    if (convex) {
+        s->planes = (plane_t*)realloc(s->planes, (s->n_planes + nf) * sizeof(plane_t));
+        v->plane_begin = s->n_planes;
+        v->plane_count = nf;
+        for (uint32_t j = 0; j < nf; ++j) {
+            plane_t* pl = &s->planes[s->n_planes + j];
+            for (int k = 0; k < 3; ++k) {
+                pl->n[k] = (real)f64[j][k];
+                pl->s[k] = (real)f64[j][4 + k];
+            }
+            pl->h = (real)f64[j][3];
+        }
+        s->n_planes += nf;
+    } else {
+        double vol6 = 0.0;
+        for (uint32_t t = 0; t < nt; ++t) {
+            double q[3][3];
+            for (int j = 0; j < 3; ++j) {
+                for (int k = 0; k < 3; ++k) {
+                    q[j][k] = x[3 * m->indices[3 * t + j] + k] - c[k];
+                }
+            }
+            vol6 += q[0][0] * (q[1][1] * q[2][2] - q[1][2] * q[2][1]) + q[0][1] * (q[1][2] * q[2][0] - q[1][0] * q[2][2]) +
+                    q[0][2] * (q[1][0] * q[2][1] - q[1][1] * q[2][0]);
+        }
+        const int flip = vol6 < 0.0;
+        s->tris = (otri_t*)realloc(s->tris, (s->n_tris + nt) * sizeof(otri_t));
+        v->plane_begin = s->n_tris;
+        v->plane_count = nt;
+        v->tri = 1;
+        for (uint32_t t = 0; t < nt; ++t) {
+            const double* p0 = x + 3 * m->indices[3 * t];
+            const double* p1 = x + 3 * m->indices[3 * t + (flip ? 2 : 1)];
+            const double* p2 = x + 3 * m->indices[3 * t + (flip ? 1 : 2)];
+            const double a0 = p1[0] - p0[0], a1 = p1[1] - p0[1], a2 = p1[2] - p0[2];
+            const double b0 = p2[0] - p0[0], b1 = p2[1] - p0[1], b2 = p2[2] - p0[2];
+            const double g0 = a1 * b2 - a2 * b1, g1 = a2 * b0 - a0 * b2, g2 = a0 * b1 - a1 * b0;
+            const double len = sqrt(g0 * g0 + g1 * g1 + g2 * g2);
+            const double u[3] = {g0 / len, g1 / len, g2 / len};
+            otri_t* tr = &s->tris[s->n_tris + t];
+            for (int k = 0; k < 3; ++k) {
+                tr->face.n[k] = (real)u[k];
+                if (m->triangle_normals) {
+                    const double* n = m->triangle_normals + 3 * t;
+                    const double nl = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+                    tr->face.s[k] = (real)(n[k] / nl);
+                } else {
+                    tr->face.s[k] = (real)u[k];
+                }
+                tr->v[0][k] = (real)p0[k];
+                tr->v[1][k] = (real)p1[k];
+                tr->v[2][k] = (real)p2[k];
+            }
+            tr->face.h = (real)(u[0] * p0[0] + u[1] * p0[1] + u[2] * p0[2]);
+        }
+        s->n_tris += nt;
+    }
     free(f64);
     for (int k = 0; k < 3; ++k) {
         v->lo[k] = (float)lo[k];
@@ -716,7 +789,7 @@ static oscene_t* build(const g4go_scene* in) {
         d->sensor = v->sensor_id;
         for (uint32_t j = 0; j < in->n_meshes; ++j) {
             if (in->meshes[j].volume_id == v->volume_id) {
-                mesh_planes(s, &in->meshes[j], d);
+                mesh_solid(s, &in->meshes[j], d);
             }
         }
     }
@@ -757,6 +830,7 @@ static void destroy(oscene_t* s) {
     free(s->vol);
     free(s->bind);
     free(s->planes);
+    free(s->tris);
     free(s);
 }
 
@@ -1004,6 +1078,92 @@ static int mesh_contains(const oscene_t* s, const volume_t* v, v3 q) {
     return 1;
 }
 
+/* Non-convex mesh solids as their triangles: the watertight ray/triangle test
+ * of Woop, Benthin and Wald (JCGT 2(1), 2013): the ray sheared onto its
+ * dominant axis, the edge functions in 2D (in FP64 when one is exactly
+ * zero); det < 0 marks a hit leaving the solid (outward winding).  Closest
+ * hits by brute force in mesh order (ties: the lower index).  A point is
+ * inside when the first triangle along (1, 2, 3)/sqrt(14) is a leaving one.
+ * The reference traces the triangles themselves (OptiXTransportDevice.cu:
+ * 105-196). */
+#define FACE_TRI 0x80000000u
+static real ax3(v3 a, int k) { return k == 0 ? a.x : (k == 1 ? a.y : a.z); }
+
+static int wt_hit(v3 o, v3 d, const otri_t* tr, real* t, int* leaving) {
+    const real adx = R_FABS(d.x), ady = R_FABS(d.y), adz = R_FABS(d.z);
+    const int kz = (adx >= ady && adx >= adz) ? 0 : (ady >= adz ? 1 : 2);
+    int kx = kz == 2 ? 0 : kz + 1;
+    int ky = kx == 2 ? 0 : kx + 1;
+    const real dz = ax3(d, kz);
+    if (dz < RF(0.0)) {
+        const int k = kx;
+        kx = ky;
+        ky = k;
+    }
+    const real sx = ax3(d, kx) / dz, sy = ax3(d, ky) / dz, sz = RF(1.0) / dz;
+    const v3 a = vsub(mk(tr->v[0][0], tr->v[0][1], tr->v[0][2]), o);
+    const v3 b = vsub(mk(tr->v[1][0], tr->v[1][1], tr->v[1][2]), o);
+    const v3 c = vsub(mk(tr->v[2][0], tr->v[2][1], tr->v[2][2]), o);
+    const real az = ax3(a, kz), bz = ax3(b, kz), cz = ax3(c, kz);
+    const real ax_ = ax3(a, kx) - sx * az, ay_ = ax3(a, ky) - sy * az;
+    const real bx = ax3(b, kx) - sx * bz, by = ax3(b, ky) - sy * bz;
+    const real cx = ax3(c, kx) - sx * cz, cy = ax3(c, ky) - sy * cz;
+    real u = cx * by - cy * bx;
+    real vv = ax_ * cy - ay_ * cx;
+    real w = bx * ay_ - by * ax_;
+    if (u == RF(0.0) || vv == RF(0.0) || w == RF(0.0)) {
+        u = (real)((double)cx * (double)by - (double)cy * (double)bx);
+        vv = (real)((double)ax_ * (double)cy - (double)ay_ * (double)cx);
+        w = (real)((double)bx * (double)ay_ - (double)by * (double)ax_);
+    }
+    if ((u < RF(0.0) || vv < RF(0.0) || w < RF(0.0)) && (u > RF(0.0) || vv > RF(0.0) || w > RF(0.0))) {
+        return 0;
+    }
+    const real det = u + vv + w;
+    if (det == RF(0.0)) {
+        return 0;
+    }
+    *t = (u * (sz * az) + vv * (sz * bz) + w * (sz * cz)) / det;
+    *leaving = det < RF(0.0);
+    return 1;
+}
+
+/* mode: 0 leaving, 1 entering, 2 either; returns R_INF and *face INVALID on none */
+static real tri_closest(const oscene_t* s, const volume_t* v, v3 p, v3 d, real tmin, int mode, uint32_t* face,
+                        int* leaving) {
+    real best = R_INF;
+    uint32_t bi = INVALID;
+    int bl = 0;
+    for (uint32_t k = v->plane_begin; k < v->plane_begin + v->plane_count; ++k) {
+        real t;
+        int l;
+        if (wt_hit(p, d, &s->tris[k], &t, &l) && t >= tmin && (mode == 2 || l == (mode == 0)) && t < best) {
+            best = t;
+            bi = k;
+            bl = l;
+        }
+    }
+    *face = bi == INVALID ? INVALID : (bi | FACE_TRI);
+    *leaving = bl;
+    return best;
+}
+
+static int tri_contains(const oscene_t* s, const volume_t* v, v3 q) {
+    uint32_t f;
+    int l;
+    const v3 u = mk((real)0.26726124f, (real)0.53452248f, (real)0.80178373f);
+    const real t = tri_closest(s, v, q, u, RF(0.0), 2, &f, &l);
+    return t < R_INF && l;
+}
+
+static int solid_contains(const oscene_t* s, const volume_t* v, v3 q) {
+    return v->tri ? tri_contains(s, v, q) : mesh_contains(s, v, q);
+}
+
+static const plane_t* face_record(const oscene_t* s, uint32_t face) {
+    return (face & FACE_TRI) ? &s->tris[face & ~FACE_TRI].face : &s->planes[face];
+}
+
 /* Nearest daughter entry by brute force; ties go to the lower index.  *plane
  * is the entered mesh face, INVALID for a box face. */
 static real nearest_daughter(const oscene_t* s, uint32_t cur, v3 p, v3 d, real tmin, uint32_t* which, int* axis,
@@ -1017,8 +1177,10 @@ static real nearest_daughter(const oscene_t* s, uint32_t cur, v3 p, v3 d, real t
         }
         int a = 0;
         uint32_t f = INVALID;
-        const real t = s->vol[j].plane_count ? mesh_entry(s, &s->vol[j], p, d, tmin, &f)
-                                             : box_entry(&s->vol[j], p, inv, tmin, &a);
+        int l;
+        const real t = s->vol[j].tri ? tri_closest(s, &s->vol[j], p, d, tmin, 1, &f, &l)
+                       : s->vol[j].plane_count ? mesh_entry(s, &s->vol[j], p, d, tmin, &f)
+                                               : box_entry(&s->vol[j], p, inv, tmin, &a);
         if (t < best || (t == best && t < R_INF && j < *which)) {
             best = t;
             *which = j;
@@ -1036,7 +1198,7 @@ static uint32_t point_neighbor(const oscene_t* s, uint32_t cur, v3 q) {
     }
     for (uint32_t j = 0; j < s->n_vol; ++j) {
         if (j != cur && s->vol[j].parent == parent && box_contains(&s->vol[j], q) &&
-            (s->vol[j].plane_count == 0 || mesh_contains(s, &s->vol[j], q))) {
+            (s->vol[j].plane_count == 0 || solid_contains(s, &s->vol[j], q))) {
             return j;
         }
     }
@@ -1354,11 +1516,17 @@ static int or_step(photon_t* ph, const oscene_t* s, const rng_t* r, real eps, in
     /* with mesh solids in the scene: faces by clipping, no snap onto a mesh
      * face, and a daughter entered up to eps beyond the exit wins (the
      * reference's coincident-surface rank, OptiXTransportDevice.cu:105-127) */
-    const int meshes = s->n_planes != 0;
+    const int meshes = s->n_planes != 0 || s->n_tris != 0;
     real dist;
     int axis = 0;
     uint32_t plane = INVALID;
-    if (vol->plane_count) {
+    if (vol->tri) {
+        int l;
+        dist = tri_closest(s, vol, ph->pos, ph->dir, RF(0.0), 0, &plane, &l);
+        if (!finite_r(dist)) {
+            return S_INVALID;
+        }
+    } else if (vol->plane_count) {
         mesh_exit(s, vol, ph->pos, ph->dir, &dist, &plane);
     } else {
         box_exit(vol, ph->pos, ph->dir, &dist, &axis);
@@ -1406,7 +1574,7 @@ static int or_step(photon_t* ph, const oscene_t* s, const rng_t* r, real eps, in
     v3 normal;
     const int mesh_face = plane != INVALID;
     if (mesh_face) {
-        const plane_t* pl = &s->planes[plane];
+        const plane_t* pl = face_record(s, plane);
         const v3 sn = mk(pl->s[0], pl->s[1], pl->s[2]);
         normal = vdot(ph->dir, sn) > RF(0.0) ? vscale(sn, RF(-1.0)) : sn;
     } else {
@@ -1426,7 +1594,7 @@ static int or_step(photon_t* ph, const oscene_t* s, const rng_t* r, real eps, in
     ph->pos = bpos;
     if (!entering) {
         /* the point just outside the face */
-        const v3 q = mesh_face ? vadd(bpos, vscale(mk(s->planes[plane].n[0], s->planes[plane].n[1], s->planes[plane].n[2]), eps))
+        const v3 q = mesh_face ? vadd(bpos, vscale(mk(face_record(s, plane)->n[0], face_record(s, plane)->n[1], face_record(s, plane)->n[2]), eps))
                                : vadd(bpos, vscale(normal, -eps));
         next = point_neighbor(s, ph->volume, q);
         if (next == INVALID) {
@@ -1758,12 +1926,36 @@ void ORACLE_NAME(mesh_trace)(const g4go_scene* scene, uint32_t volume_id, const 
         const v3 pp = mk(p[3 * i], p[3 * i + 1], p[3 * i + 2]);
         const v3 dd = mk(d[3 * i], d[3 * i + 1], d[3 * i + 2]);
         uint32_t pl = INVALID;
-        if (entry) {
+        int l;
+        if (v->tri) {
+            dist[i] = tri_closest(s, v, pp, dd, entry ? tmin : RF(0.0), entry ? 1 : 0, &pl, &l);
+            pl = pl == INVALID ? INVALID : (pl & ~FACE_TRI);
+        } else if (entry) {
             dist[i] = mesh_entry(s, v, pp, dd, tmin, &pl);
         } else {
             mesh_exit(s, v, pp, dd, &dist[i], &pl);
         }
         face[i] = pl == INVALID ? INVALID : pl - v->plane_begin;
+    }
+    destroy(s);
+}
+
+/* Solid kind of volume `volume_id`: 0 box, 1 convex mesh (planes), 2
+ * triangles; and whether point q lies inside it. */
+int ORACLE_NAME(solid_kind)(const g4go_scene* scene, uint32_t volume_id) {
+    oscene_t* s = build(scene);
+    const volume_t* v = &s->vol[vol_index(s, volume_id)];
+    const int kind = v->tri ? 2 : (v->plane_count ? 1 : 0);
+    destroy(s);
+    return kind;
+}
+
+void ORACLE_NAME(solid_contains)(const g4go_scene* scene, uint32_t volume_id, const real* q, uint64_t n, int* out) {
+    oscene_t* s = build(scene);
+    const volume_t* v = &s->vol[vol_index(s, volume_id)];
+    for (uint64_t i = 0; i < n; ++i) {
+        const v3 p = mk(q[3 * i], q[3 * i + 1], q[3 * i + 2]);
+        out[i] = v->plane_count ? solid_contains(s, v, p) : box_contains(v, p);
     }
     destroy(s);
 }
@@ -1775,7 +1967,8 @@ uint32_t ORACLE_NAME(mesh_planes)(const g4go_scene* scene, uint32_t volume_id, r
     const volume_t* v = &s->vol[vol_index(s, volume_id)];
     const uint32_t n = v->plane_count;
     for (uint32_t k = 0; k < n && k < max; ++k) {
-        memcpy(out + 7 * k, &s->planes[v->plane_begin + k], 7 * sizeof(real));
+        memcpy(out + 7 * k, v->tri ? &s->tris[v->plane_begin + k].face : &s->planes[v->plane_begin + k],
+               7 * sizeof(real));
     }
     destroy(s);
     return n;

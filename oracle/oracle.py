@@ -66,6 +66,8 @@ def _load(precision: int) -> C.CDLL:
                                                C.c_uint64, real, C.c_int, C.c_void_p, C.c_void_p]
     getattr(lib, p + "mesh_planes").argtypes = [C.POINTER(abi.Scene), C.c_uint32, C.c_void_p, C.c_uint32]
     getattr(lib, p + "mesh_planes").restype = C.c_uint32
+    getattr(lib, p + "solid_kind").argtypes = [C.POINTER(abi.Scene), C.c_uint32]
+    getattr(lib, p + "solid_contains").argtypes = [C.POINTER(abi.Scene), C.c_uint32, C.c_void_p, C.c_uint64, C.c_void_p]
     for name in ("project_polarization", "reflect_direction"):
         getattr(lib, p + name).argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
     getattr(lib, p + "normalize").argtypes = [C.c_void_p, C.c_void_p]
@@ -188,6 +190,22 @@ def mesh_planes(scene, volume_id: int, precision: int = 32) -> np.ndarray:
     desc, keep = scene.to_abi()
     n = _fn(precision, "mesh_planes")(C.byref(desc), volume_id, out.ctypes.data, 1024)
     return out[:n].copy()
+
+
+def solid_kind(scene, volume_id: int) -> int:
+    """0 box, 1 convex mesh (face planes), 2 triangle solid."""
+    desc, keep = scene.to_abi()
+    return _fn(32, "solid_kind")(C.byref(desc), volume_id)
+
+
+def solid_contains(scene, volume_id: int, q, precision: int = 32) -> np.ndarray:
+    """Whether each point lies inside the volume's solid."""
+    real = _real(precision)
+    q = np.ascontiguousarray(q, dtype=real).reshape(-1, 3)
+    out = np.zeros(len(q), np.int32)
+    desc, keep = scene.to_abi()
+    _fn(precision, "solid_contains")(C.byref(desc), volume_id, q.ctypes.data, len(q), out.ctypes.data)
+    return out.astype(bool)
 
 
 def find_surface(scene, from_id: int, to_id: int) -> int:

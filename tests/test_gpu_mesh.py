@@ -7,6 +7,9 @@ crystal turned by a rotation (crystal and SiPM as box meshes sharing a face),
 a tetrahedron with a mesh sensor glued on one face, a detecting tetrahedron
 seen from the world (mesh entry), and a world of 40 random convex hulls (a
 daughter BVH over mesh solids, face planes read from global memory).
+Triangle solids (non-convex meshes): the L-shaped light guide (Fresnel and
+TIR on its triangles, photons crossing between its arms through the air),
+the L-prism with a sensor on its end face, and the tube seen from its hole.
 """
 import sys
 
@@ -20,9 +23,10 @@ from g4go_amd.transport import HipTransportHost, LaunchForm, PhotonTransportConf
 from oracle import oracle as O
 
 from test_gpu_parity import assert_same_hits, assert_same_stats
-from test_mesh_oracle import (OUTSIDE_POINT, TETRA_POINT, binomial_check, detecting_tetra_scene, mean_pde,
-                              random_hull, tapered_check, tapered_limit_scene, tetra_face_fraction,
-                              tetra_sensor_scene, visible_fraction)
+from test_mesh_oracle import (HOLE_POINT, L_POINT, OUTSIDE_POINT, TETRA_POINT, binomial_check, detecting_tetra_scene,
+                              l_end_face_fraction, l_sensor_scene, mean_pde, random_hull, tapered_check,
+                              tapered_limit_scene, tetra_face_fraction, tetra_sensor_scene, tube_hole_scene,
+                              tube_wall_fraction, visible_fraction)
 
 pytestmark = pytest.mark.gpu
 
@@ -168,5 +172,56 @@ def test_mesh_adjoint_matches_oracle_replay():
         grad, stats = h.adjoint(w)
     assert stats["replayed"] == len(out.detections) and stats["mismatched"] == 0 and stats["unsupported"] == 0
     ref, counters = O.adjoint_synthetic(cs.scene, src, 36, out.detections, w, crystal_layout(cs), n_params)
+    assert int(counters[0]) == len(out.detections) and int(counters[1]) == 0
+    np.testing.assert_allclose(grad, ref, rtol=1e-9, atol=1e-9 * np.max(np.abs(ref)))
+
+
+@pytest.mark.parametrize("form", ["plain", "persistent"])
+def test_l_guide_triangle_solid_bit_exact(form):
+    log(f"case: L-shaped light guide (triangle solid, Fresnel), {form}")
+    sc, src, glass, detect = workloads.l_guide_scene()
+    out = parity(sc, src, 21, [0, 4], [30000, 3000], PLAIN if form == "plain" else PERSISTENT)
+    st = out.statistics
+    assert st["detected"] > 1000 and st["escaped"] > 1000
+
+
+def test_triangle_solid_closed_forms_bit_exact():
+    log("case: L-prism end face and tube hole (triangle solids), oracle parity")
+    sc, lv = l_sensor_scene()
+    parity(sc, workloads.point_source(lv, L_POINT), 22, [0], [20000])
+    sc, world = tube_hole_scene()
+    parity(sc, workloads.point_source(world, HOLE_POINT), 23, [0], [20000])
+
+
+def test_device_triangle_solid_solid_angles():
+    log("case: triangle-solid solid angles on the device, 1e7 photons each")
+    n = 10_000_000
+    sc, lv = l_sensor_scene()
+    out = device_run(sc, workloads.point_source(lv, L_POINT), 24, [0], [n], sort_hits=False)
+    assert out.statistics["invalid"] == 0 and out.statistics["escaped"] == 0
+    binomial_check(int(out.statistics["detected"]), n, l_end_face_fraction(), "L-prism end face")
+    sc, world = tube_hole_scene()
+    out = device_run(sc, workloads.point_source(world, HOLE_POINT), 25, [0], [n], sort_hits=False)
+    assert out.statistics["invalid"] == 0 and out.statistics["absorbed"] == 0
+    binomial_check(int(out.statistics["detected"]), n, tube_wall_fraction(), "tube wall seen from its hole")
+
+
+def test_triangle_solid_adjoint_matches_oracle_replay():
+    log("case: adjoint on the L-shaped light guide vs oracle replay")
+    sc, src, glass, detect = workloads.l_guide_scene()
+    layout = [(abi.PARAM_ABS_LENGTH, glass), (abi.PARAM_EFFICIENCY, detect)]
+    ev = np.array([1, 2], np.uint32)
+    counts = np.array([30000, 2000], np.uint32)
+    w = np.random.default_rng(14).normal(size=abi.HIST_SLOTS)
+    with HipTransportHost(PhotonTransportConfig(seed=37, sort_hits=True)) as h:
+        h.set_launch(PERSISTENT)
+        h.prepare_scene(sc)
+        n_params = h.set_parameter_layout(layout)
+        h.enqueue_synthetic(sc, src, ev, counts)
+        out = h.complete_oldest_batch()
+        grad, stats = h.adjoint(w)
+    assert len(out.detections) > 1000
+    assert stats["replayed"] == len(out.detections) and stats["mismatched"] == 0 and stats["unsupported"] == 0
+    ref, counters = O.adjoint_synthetic(sc, src, 37, out.detections, w, layout, n_params)
     assert int(counters[0]) == len(out.detections) and int(counters[1]) == 0
     np.testing.assert_allclose(grad, ref, rtol=1e-9, atol=1e-9 * np.max(np.abs(ref)))

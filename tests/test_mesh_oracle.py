@@ -394,21 +394,25 @@ def test_library_accepts_and_rejects_meshes():
     m = box_mesh((1.0, 1.0, 1.0))
     bad = {
         "not closed": Mesh(m.vertices_mm, m.indices[:-1]),
-        "not convex": Mesh(np.r_[m.vertices_mm[:7], [[0.2, 0.2, 0.2]]], m.indices),
         "degenerate": Mesh(np.r_[m.vertices_mm, m.vertices_mm[:1]], np.r_[m.indices, [[0, 0, 1]]]),
         "index": Mesh(m.vertices_mm, np.r_[m.indices[:-1], [[0, 1, 99]]]),
         "at least 4": Mesh(TETRA[:3], TETRA_TRIS[:1]),
-        "flat": Mesh(np.c_[TETRA[:, :2], np.zeros(4)], TETRA_TRIS),
+        "flat|wound": Mesh(np.c_[TETRA[:, :2], np.zeros(4)], TETRA_TRIS),
     }
     for what, mesh in bad.items():
         with pytest.raises(ValueError, match=what):
             prepare(mesh_scene([mesh]))
+    # a dented box (one corner pushed inside) is not convex: a triangle solid
+    dented = mesh_scene([Mesh(np.r_[m.vertices_mm[:7], [[0.2, 0.2, 0.2]]], m.indices)])
+    prepare(dented)
+    assert O.solid_kind(dented, 1) == 2
     normals = np.tile([0.0, 0.0, 0.0], (12, 1))
     with pytest.raises(ValueError, match="normal"):
         prepare(mesh_scene([Mesh(m.vertices_mm, m.indices, normals)]))
-    big = Mesh(*_sphere_mesh(48, 40))
-    with pytest.raises(ValueError, match="faces"):
-        prepare(mesh_scene([big]))
+    # a convex mesh of more than 1024 faces is held as triangles
+    big = mesh_scene([Mesh(*_sphere_mesh(48, 40))])
+    prepare(big)
+    assert O.solid_kind(big, 1) == 2
 
 
 def _sphere_mesh(n_lon, n_lat):
@@ -429,3 +433,179 @@ def _sphere_mesh(n_lon, n_lat):
             tri.append((ring(i, j), ring(i + 1, j), ring(i + 1, j + 1)))
             tri.append((ring(i, j), ring(i + 1, j + 1), ring(i, j + 1)))
     return np.array(v), np.array(tri)
+
+
+# ---------------------------------------------------------------------------
+# Non-convex closed meshes (triangle solids): an L-shaped prism and a hollow
+# tube.  Pinned by an analytic inside test, a brute-force closest triangle hit
+# and two solid-angle closed forms.
+L_POLYGON = [(0.0, 0.0), (20.0, 0.0), (20.0, 5.0), (5.0, 5.0), (5.0, 15.0), (0.0, 15.0)]
+TUBE = dict(r_in=3.0, r_out=5.0, half_z=10.0, segments=24)
+
+
+def l_inside(q):
+    x, y, z = q.T
+    return (z > 0) & (z < 4) & (((x > 0) & (x < 20) & (y > 0) & (y < 5)) | ((x > 0) & (x < 5) & (y > 0) & (y < 15)))
+
+
+def tube_inside(q, centre=(0.0, 0.0)):
+    """Inside the tube: between the inscribed 24-gons of r_in and r_out, |z| < half_z."""
+    n = TUBE["segments"]
+    r = np.hypot(q[:, 0] - centre[0], q[:, 1] - centre[1])
+    a = np.mod(np.arctan2(q[:, 1] - centre[1], q[:, 0] - centre[0]), 2 * np.pi / n) - np.pi / n
+    fac = np.cos(np.pi / n) / np.cos(a)
+    return (np.abs(q[:, 2]) < TUBE["half_z"]) & (r < TUBE["r_out"] * fac) & (r > TUBE["r_in"] * fac)
+
+
+def test_non_convex_meshes_become_triangle_solids():
+    from g4go_amd.scene import tube_mesh
+
+    sc = mesh_scene([prism_mesh(L_POLYGON, 0.0, 4.0), tube_mesh(**TUBE), box_mesh((1.0, 1.0, 1.0))])
+    assert [O.solid_kind(sc, i) for i in range(4)] == [0, 2, 2, 1]
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_triangle_solid_containment_and_hits(precision):
+    """Inside test against the analytic shapes; exit and entry distances
+    against a brute-force closest triangle hit (FP64 Moller-Trumbore)."""
+    from g4go_amd.scene import tube_mesh
+
+    rng = np.random.default_rng(8)
+    lm = prism_mesh(L_POLYGON, 0.0, 4.0)
+    tm = tube_mesh(**TUBE).transformed(None, (40.0, 0.0, 0.0))
+    sc = mesh_scene([lm, tm])
+    q = rng.uniform([-2, -2, -1], [22, 17, 5], size=(20000, 3))
+    gap = np.min(np.abs(np.c_[q[:, 2], q[:, 2] - 4, q[:, 0], q[:, 0] - 20, q[:, 0] - 5, q[:, 1], q[:, 1] - 5,
+                              q[:, 1] - 15]), axis=1)
+    ok = gap > 1e-4
+    np.testing.assert_array_equal(O.solid_contains(sc, 1, q, precision)[ok], l_inside(q)[ok])
+    q2 = rng.uniform([34, -6, -11], [46, 6, 11], size=(20000, 3))
+    np.testing.assert_array_equal(O.solid_contains(sc, 2, q2, precision), tube_inside(q2, (40.0, 0.0)))
+    rtol = 1e-9 if precision == 64 else 2e-5
+    for vid, mesh, inside in ((1, lm, q[l_inside(q) & ok][:400]), (2, tm, q2[tube_inside(q2, (40.0, 0.0))][:400])):
+        d = rng.normal(size=inside.shape)
+        d /= np.linalg.norm(d, axis=1, keepdims=True)
+        dist, face = O.mesh_trace(sc, vid, inside, d, entry=False, precision=precision)
+        tris = mesh.indices.astype(np.int64)
+        for i in range(len(inside)):
+            t, k, edge = closest_triangle_hit(mesh.vertices_mm, tris, inside[i], d[i], 0.0)
+            assert dist[i] == pytest.approx(t, rel=rtol, abs=rtol * 10)
+            if edge > 1e-4:
+                assert face[i] == k  # the triangle, in mesh order
+    po = rng.uniform([30, -10, -15], [50, 10, 15], size=(3000, 3))
+    po = po[~tube_inside(po, (40.0, 0.0))]
+    aim = rng.uniform([35, -5, -10], [45, 5, 10], size=po.shape) - po
+    aim /= np.linalg.norm(aim, axis=1, keepdims=True)
+    dist, face = O.mesh_trace(sc, 2, po, aim, tmin=1e-4, entry=True, precision=precision)
+    hits = 0
+    for i in range(len(po)):
+        t, k, edge = closest_triangle_hit(tm.vertices_mm, tm.indices.astype(np.int64), po[i], aim[i], 1e-4)
+        if edge < 1e-6 and np.isfinite(t):
+            continue
+        assert np.isfinite(dist[i]) == np.isfinite(t)
+        if np.isfinite(t):
+            hits += 1
+            assert dist[i] == pytest.approx(t, rel=rtol, abs=rtol * 10)
+    assert hits > 1000
+
+
+def l_sensor_scene():
+    """Glass L-prism in air; a silicon box on the end face x = 20 of its long
+    arm detects every photon reaching it; the other faces absorb."""
+    sc = Scene()
+    air = sc.add_material(Material(name="air", rindex=PropertyTable.constant_value(1.0)))
+    glass = sc.add_material(Material(name="glass", rindex=PropertyTable.constant_value(1.5)))
+    si = sc.add_material(Material(name="si", rindex=PropertyTable.constant_value(3.5)))
+    detect = sc.add_surface(Surface(name="detect", type=SurfaceType.DielectricMetal,
+                                    reflectivity=PropertyTable.constant_value(0.0),
+                                    efficiency=PropertyTable.constant_value(1.0)))
+    absorb = sc.add_surface(Surface(name="absorb", type=SurfaceType.DielectricMetal,
+                                    reflectivity=PropertyTable.constant_value(0.0)))
+    world = sc.add_volume(Volume(name="world", material_id=air, half_mm=(60.0, 60.0, 60.0)))
+    sc.world_volume_id = world
+    lv = sc.add_volume(Volume(name="L", material_id=glass, parent_volume_id=world,
+                              mesh=prism_mesh(L_POLYGON, 0.0, 4.0)))
+    sensor = sc.add_volume(Volume(name="sensor", material_id=si, parent_volume_id=world, sensor_id=0,
+                                  center_mm=(20.25, 2.5, 2.0), half_mm=(0.25, 2.5, 2.0)))
+    sc.add_surface_binding(SurfaceBinding(lv, sensor, detect))
+    sc.add_surface_binding(SurfaceBinding(lv, world, absorb))
+    return sc, lv
+
+
+L_POINT = np.array([12.0, 2.5, 2.0])
+
+
+def l_end_face_fraction(p=L_POINT):
+    a, b, c, d = (np.array(v) for v in ((20.0, 0.0, 0.0), (20.0, 5.0, 0.0), (20.0, 5.0, 4.0), (20.0, 0.0, 4.0)))
+    return (triangle_solid_angle(p, a, b, c) + triangle_solid_angle(p, a, c, d)) / (4 * np.pi)
+
+
+def test_oracle_l_prism_end_face_solid_angle():
+    """From a point of the long arm the whole end face is visible: detected /
+    generated = its solid angle / 4 pi."""
+    sc, lv = l_sensor_scene()
+    n = 400_000
+    r = O.run_synthetic(sc, workloads.point_source(lv, L_POINT), SEED, [0], [n], threads=8)
+    st = dict(zip(O.OracleResult.STAT_NAMES, r.stats[0].tolist()))
+    assert st["escaped"] == 0 and st["invalid"] == 0 and st["detected"] + st["absorbed"] == n
+    binomial_check(st["detected"], n, l_end_face_fraction(), "L-prism end face")
+
+
+def tube_hole_scene():
+    """A detecting glass tube (skin R = 0, efficiency 1) in an air world."""
+    from g4go_amd.scene import tube_mesh
+
+    sc = Scene()
+    air = sc.add_material(Material(name="air", rindex=PropertyTable.constant_value(1.0)))
+    glass = sc.add_material(Material(name="glass", rindex=PropertyTable.constant_value(1.5)))
+    detect = sc.add_surface(Surface(name="detect", type=SurfaceType.DielectricMetal,
+                                    reflectivity=PropertyTable.constant_value(0.0),
+                                    efficiency=PropertyTable.constant_value(1.0)))
+    world = sc.add_volume(Volume(name="world", material_id=air, half_mm=(60.0, 60.0, 60.0)))
+    sc.world_volume_id = world
+    sc.add_volume(Volume(name="tube", material_id=glass, parent_volume_id=world, skin_surface_id=detect, sensor_id=0,
+                         mesh=tube_mesh(**TUBE)))
+    return sc, world
+
+
+HOLE_POINT = np.array([0.5, 0.3, 2.0])
+
+
+def tube_wall_fraction(p=HOLE_POINT):
+    """From a point in the hole, 1 - (solid angles of the two 24-gon openings) / 4 pi."""
+    n, r, hz = TUBE["segments"], TUBE["r_in"], TUBE["half_z"]
+    ang = 2 * np.pi * np.arange(n) / n
+    omega = 0.0
+    for z in (-hz, hz):
+        ring = np.c_[r * np.cos(ang), r * np.sin(ang), np.full(n, z)]
+        centre = np.array([0.0, 0.0, z])
+        omega += sum(triangle_solid_angle(p, centre, ring[k], ring[(k + 1) % n]) for k in range(n))
+    return 1.0 - omega / (4 * np.pi)
+
+
+def test_oracle_tube_hole_wall_fraction():
+    """Photons from inside the tube's hole (a point of the world, inside the
+    tube's bounding box) enter the non-convex tube unless they leave by an end."""
+    sc, world = tube_hole_scene()
+    n = 400_000
+    r = O.run_synthetic(sc, workloads.point_source(world, HOLE_POINT), SEED, [0], [n], threads=8)
+    st = dict(zip(O.OracleResult.STAT_NAMES, r.stats[0].tolist()))
+    assert st["invalid"] == 0 and st["absorbed"] == 0 and st["detected"] + st["escaped"] == n
+    binomial_check(st["detected"], n, tube_wall_fraction(), "tube wall seen from its hole")
+
+
+def test_library_triangle_solid_checks():
+    """The library accepts the L-prism and the tube, and rejects a non-convex
+    mesh whose triangles are not wound consistently."""
+    from g4go_amd.scene import tube_mesh
+
+    prepare(l_sensor_scene()[0])
+    prepare(tube_hole_scene()[0])
+    m = prism_mesh(L_POLYGON, 0.0, 4.0)
+    idx = m.indices.copy()
+    idx[3] = idx[3][::-1]
+    with pytest.raises(ValueError, match="wound"):
+        prepare(mesh_scene([Mesh(m.vertices_mm, idx)]))
+    t = tube_mesh(**TUBE)
+    with pytest.raises(ValueError, match="not closed"):
+        prepare(mesh_scene([Mesh(t.vertices_mm, t.indices[:-2])]))
