@@ -125,16 +125,21 @@ typedef struct g4go_surface_binding { /* SurfaceBinding (Scene.hpp:122-126) */
     uint32_t from_volume_id, to_volume_id, surface_id;
 } g4go_surface_binding;
 
-/* Closed convex triangle mesh of one volume, in world coordinates: the
- * volume's MeshGeometry (Scene.hpp:94-100) mapped through its Transform
+/* Closed triangle mesh of one volume, in world coordinates: the volume's
+ * MeshGeometry (Scene.hpp:94-100) mapped through its Transform
  * (Scene.hpp:42-59), as the reference places it with an instance transform
- * (OptiXTransportHost.cpp:1221-1262).  The winding does not matter (the
- * boundary normal is turned to face the photon, OptiXTransportDevice.cu:
- * 789-791).  Triangles lying in one plane form one face.  Rejected with
+ * (OptiXTransportHost.cpp:1221-1262).  A convex mesh (no vertex outside the
+ * plane of a triangle by more than 1e-6 of the mesh size, at most 1024
+ * distinct face planes) is held as its face planes and may be wound either
+ * way; any other mesh is held as its triangles and must be wound
+ * consistently (each edge traversed once each way; inwards or outwards).
+ * The boundary normal is turned to face the photon, as in the reference
+ * (OptiXTransportDevice.cu:789-791).  Rejected with
  * G4GO_ERROR_INVALID_ARGUMENT: fewer than four triangles, an index out of
  * range, a degenerate triangle, an edge not shared by exactly two triangles
- * (an open mesh), a vertex outside the plane of a triangle by more than
- * 1e-6 of the mesh size (a concave mesh), more than 1024 faces. */
+ * (an open mesh; vertices are welded by equal coordinates), an inconsistent
+ * winding of a non-convex mesh, no enclosed volume, more than 65536
+ * triangles in a non-convex mesh. */
 typedef struct g4go_volume_mesh {
     uint32_t volume_id;
     uint32_t n_vertices;
