@@ -230,14 +230,10 @@ def run_mesh(seed, launch, steps, warmup, photons, box_ms_per_step):
     tapered = workloads.tapered_crystal_scene()
     rotated = workloads.rotated_crystal_scene(rot)
     guide, guide_src, _, _ = workloads.l_guide_scene()
-
-    class Holder:  # run_form reads .scene
-        def __init__(self, scene):
-            self.scene = scene
-
+    guide_cs = workloads.CrystalScene(guide, guide.world_volume_id, [], [], [], -1, -1, workloads.CrystalParams())
     cases = {"tapered_crystal": (tapered, workloads.synthetic_source(tapered)),
              "rotated_crystal": (rotated, workloads.point_source(rotated.crystal_ids[0], (0.0, 0.0, 0.0))),
-             "l_guide_triangles": (Holder(guide), guide_src)}
+             "l_guide_triangles": (guide_cs, guide_src)}
     res = {}
     for name, (cs, src) in cases.items():
         host = HipTransportHost(PhotonTransportConfig(seed=seed, sort_hits=False))

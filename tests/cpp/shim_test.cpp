@@ -7,6 +7,12 @@
 //                            Cerenkov and direct emission records through
 //                            the three-method interface and writes the
 //                            records and the output to OUT.bin (layout below)
+//   shim_test gpu-mesh OUT.bin  the same with the crystal a tapered mesh
+//                            (3x3 mm front face, 4x4 mm back face, turned
+//                            into place by its transform), passed on as a
+//                            world-space mesh; OUT.bin then ends with that
+//                            mesh: u64 n_vertices, n_triangles; vertices
+//                            (3 f64 each); indices (3 u32 each)
 //
 // OUT.bin: u64 n_emissions, n_events, n_detections; emissions (128 B each);
 // event IDs (u32); emission event indices (u32); detections (48 B each);
@@ -66,6 +72,19 @@ Geometry box_mesh(double hx, double hy, double hz) {
     return g;
 }
 
+// Frustum along local x from the 2*hy x 2*hz face at x = -hx to the
+// 2*by x 2*bz face at x = +hx (12 triangles, wound outwards).
+Geometry frustum_x_mesh(double hx, double hy, double hz, double by, double bz) {
+    Geometry g = box_mesh(hx, hy, hz);
+    for (auto& v : g.mesh.verticesMm) {
+        if (v.px > 0) {
+            v.py = v.py > 0 ? by : -by;
+            v.pz = v.pz > 0 ? bz : -bz;
+        }
+    }
+    return g;
+}
+
 PropertyTable table(std::initializer_list<float> e, std::initializer_list<float> v) {
     return PropertyTable{std::vector<float>(e), std::vector<float>(v)};
 }
@@ -77,7 +96,7 @@ struct CrystalScene {
     std::uint32_t world, crystal, sipm;
 };
 
-CrystalScene crystal_scene() {
+CrystalScene crystal_scene(bool tapered = false) {
     CrystalScene c;
     Scene& s = c.scene;
     Material air;
@@ -106,8 +125,8 @@ CrystalScene crystal_scene() {
     cathode.efficiency = table({2.5F, 3.5F}, {0.3F, 0.5F});
     const std::uint32_t s_wrap = s.AddSurface(wrap), s_cathode = s.AddSurface(cathode);
     const std::uint32_t g_world = s.AddGeometry(box_mesh(100, 100, 100));
-    const std::uint32_t g_crystal = s.AddGeometry(box_mesh(10, 1.5, 1.5));
-    const std::uint32_t g_sipm = s.AddGeometry(box_mesh(1.5, 1.5, 0.05));
+    const std::uint32_t g_crystal = s.AddGeometry(tapered ? frustum_x_mesh(10, 1.5, 1.5, 2.0, 2.0) : box_mesh(10, 1.5, 1.5));
+    const std::uint32_t g_sipm = s.AddGeometry(tapered ? box_mesh(2.0, 2.0, 0.05) : box_mesh(1.5, 1.5, 0.05));
     Volume world;
     world.name = "World";
     world.geometryID = g_world;
@@ -312,8 +331,8 @@ void put(std::FILE* f, const T& x) {
     std::fwrite(&x, sizeof(T), 1, f);
 }
 
-int run_gpu(const char* path) {
-    CrystalScene c = crystal_scene();
+int run_gpu(const char* path, bool tapered) {
+    CrystalScene c = crystal_scene(tapered);
     std::vector<OpticalEmission> em;
     std::vector<std::uint32_t> events, index;
     make_batch(c, em, events, index);
@@ -348,6 +367,19 @@ int run_gpu(const char* path) {
     }
     put_stats(0u, out.statistics);
     put<std::uint64_t>(f, out.performance.totalBounceCount);
+    if (tapered) {
+        g4go_hip::FlatScene<Scene> flat(c.scene, cfg.boundaryToleranceMm);
+        if (flat.meshes().size() != 1) {
+            std::printf("FAIL: expected one mesh solid, got %zu\n", flat.meshes().size());
+            std::fclose(f);
+            return 1;
+        }
+        const g4go_volume_mesh& m = flat.meshes()[0];
+        put<std::uint64_t>(f, m.n_vertices);
+        put<std::uint64_t>(f, m.n_triangles);
+        std::fwrite(m.vertices_mm, sizeof(double), 3u * m.n_vertices, f);
+        std::fwrite(m.indices, sizeof(std::uint32_t), 3u * m.n_triangles, f);
+    }
     std::fclose(f);
     std::printf("transported %zu records, %zu detections, kernel %.3f ms\n", em.size(), out.detections.size(),
                 out.performance.optiXKernelMs);
@@ -362,12 +394,15 @@ int main(int argc, char** argv) {
             return run_cpu();
         }
         if (argc >= 3 && std::string(argv[1]) == "gpu") {
-            return run_gpu(argv[2]);
+            return run_gpu(argv[2], false);
+        }
+        if (argc == 3 && std::string(argv[1]) == "gpu-mesh") {
+            return run_gpu(argv[2], true);
         }
     } catch (const std::exception& e) {
         std::printf("FAIL: uncaught %s: %s\n", typeid(e).name(), e.what());
         return 1;
     }
-    std::printf("usage: shim_test cpu | shim_test gpu OUT.bin\n");
+    std::printf("usage: shim_test cpu | shim_test gpu OUT.bin | shim_test gpu-mesh OUT.bin\n");
     return 2;
 }

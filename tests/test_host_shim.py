@@ -6,7 +6,9 @@ exception type of each failure class and an empty batch.
 GPU: the shim transports scintillation, Cerenkov and direct emission records
 in a crystal given as rotated/translated box meshes; its detections and
 per-event counters equal the FP32 oracle's on the same records and an
-equivalent box scene.
+equivalent box scene.  In gpu-mesh mode the crystal is a tapered mesh, which
+the shim passes on as a world-space mesh solid; the oracle transports the
+same mesh.
 """
 import ctypes
 import re
@@ -17,7 +19,7 @@ import numpy as np
 import pytest
 
 from g4go_amd import abi
-from g4go_amd.scene import (Material, PropertyTable, Scene, Surface, SurfaceBinding, SurfaceFinish, SurfaceModel,
+from g4go_amd.scene import (Material, Mesh, PropertyTable, Scene, Surface, SurfaceBinding, SurfaceFinish, SurfaceModel,
                             SurfaceType, Volume)
 
 ROOT = Path(__file__).resolve().parent.parent
@@ -61,8 +63,10 @@ def test_shim_host_checks():
     assert proc.returncode == 0 and "PASS" in proc.stdout, proc.stdout + proc.stderr
 
 
-def shim_crystal_scene():
-    """The box-scene equivalent of the shim test's mesh scene (same IDs and tables)."""
+def shim_crystal_scene(tapered_mesh=None):
+    """The box-scene equivalent of the shim test's mesh scene (same IDs and
+    tables); with tapered_mesh (world-space vertices, indices) the crystal is
+    that mesh and the SiPM covers its 4x4 mm back face, as in gpu-mesh mode."""
     def t(e, v):
         return PropertyTable(np.float32(e), np.float32(v))
 
@@ -81,9 +85,12 @@ def shim_crystal_scene():
                                      efficiency=t([2.5, 3.5], [0.3, 0.5])))
     world = sc.add_volume(Volume(name="World", material_id=air, half_mm=(100, 100, 100)))
     sc.world_volume_id = world
-    xtal = sc.add_volume(Volume(name="Crystal", material_id=crystal, parent_volume_id=world, half_mm=(1.5, 1.5, 10)))
+    mesh = None if tapered_mesh is None else Mesh(*tapered_mesh)
+    xtal = sc.add_volume(Volume(name="Crystal", material_id=crystal, parent_volume_id=world, half_mm=(1.5, 1.5, 10),
+                                mesh=mesh))
+    sipm_half = (1.5, 1.5, 0.05) if tapered_mesh is None else (2.0, 2.0, 0.05)
     sc.add_volume(Volume(name="SiPM", material_id=si, parent_volume_id=world, skin_surface_id=cathode, sensor_id=0,
-                         center_mm=(0, 0, 10.05), half_mm=(1.5, 1.5, 0.05)))
+                         center_mm=(0, 0, 10.05), half_mm=sipm_half))
     sc.add_surface_binding(SurfaceBinding(xtal, world, wrap))
     return sc
 
@@ -105,20 +112,27 @@ def read_shim_output(path):
     batch = np.frombuffer(raw, STATS_DTYPE, 1, off)[0]
     off += STATS_DTYPE.itemsize
     bounces = int(np.frombuffer(raw, "<u8", 1, off)[0])
-    return em, events, idx, det, ev_stats, batch, bounces
+    off += 8
+    mesh = None
+    if off < len(raw):  # gpu-mesh: the world-space mesh the shim passed on
+        n_v, n_t = (int(x) for x in np.frombuffer(raw, "<u8", 2, off))
+        off += 16
+        verts = np.frombuffer(raw, "<f8", 3 * n_v, off).reshape(-1, 3)
+        off += verts.nbytes
+        mesh = (verts.copy(), np.frombuffer(raw, "<u4", 3 * n_t, off).reshape(-1, 3).copy())
+    return em, events, idx, det, ev_stats, batch, bounces, mesh
 
 
-@pytest.mark.gpu
-def test_shim_transport_matches_oracle(tmp_path):
+def check_shim_run(tmp_path, mode):
     from oracle import oracle as O
 
     out = tmp_path / "shim.bin"
-    print("case: C++ shim, crystal mesh scene, emission records", flush=True)
-    proc = subprocess.run([str(shim_binary()), "gpu", str(out)], capture_output=True, text=True, timeout=100)
+    proc = subprocess.run([str(shim_binary()), mode, str(out)], capture_output=True, text=True, timeout=100)
     assert proc.returncode == 0, proc.stdout + proc.stderr
-    em, events, idx, det, ev_stats, batch, bounces = read_shim_output(out)
+    em, events, idx, det, ev_stats, batch, bounces, mesh = read_shim_output(out)
+    assert (mesh is not None) == (mode == "gpu-mesh")
     assert set(em["type"]) == {0, 1, 2}
-    ref = O.run_emissions(shim_crystal_scene(), em, events, idx, 4242)
+    ref = O.run_emissions(shim_crystal_scene(mesh), em, events, idx, 4242)
     assert len(det) == len(ref.hits) > 1000
     for name in ("position_mm", "direction", "energy_ev", "event_id", "photon_id", "sensor_id"):
         np.testing.assert_array_equal(det[name], ref.hits[name], err_msg=name)
@@ -136,3 +150,19 @@ def test_shim_transport_matches_oracle(tmp_path):
     full = (1 << 2) | (1 << 3) | (1 << 4) | (1 << 5) | (1 << 6) | (1 << 7) | (1 << 8)
     assert int(batch["valid"]) == full | (1 << 1) | (1 << 9)
     assert all(int(v) == full | (1 << 1) for v in ev_stats["valid"])
+    return mesh
+
+
+@pytest.mark.gpu
+def test_shim_transport_matches_oracle(tmp_path):
+    print("case: C++ shim, crystal mesh scene, emission records", flush=True)
+    check_shim_run(tmp_path, "gpu")
+
+
+@pytest.mark.gpu
+def test_shim_tapered_mesh_transport_matches_oracle(tmp_path):
+    """A non-box mesh through the three-method shim: the tapered crystal, turned
+    into place by its transform and passed on as a world-space mesh."""
+    print("case: C++ shim, tapered crystal mesh (non-box solid), emission records", flush=True)
+    verts, tris = check_shim_run(tmp_path, "gpu-mesh")
+    assert len(tris) == 12 and np.isclose(verts[:, 2].max(), 10.0) and np.isclose(verts[:, 0].max(), 2.0)
