@@ -267,7 +267,7 @@ EXPORTED_SYMBOLS = (
     "g4go_create", "g4go_destroy", "g4go_last_error", "g4go_set_launch", "g4go_get_launch",
     "g4go_prepare_scene", "g4go_enqueue_emissions", "g4go_enqueue_synthetic", "g4go_complete",
     "g4go_complete_device", "g4go_eval_math", "g4go_eval_fresnel", "g4go_device_info", "g4go_trace_synthetic", "g4go_set_parameter_layout", "g4go_get_parameters",
-    "g4go_set_parameters", "g4go_adjoint", "g4go_version",
+    "g4go_set_parameters", "g4go_adjoint", "g4go_version", "g4go_set_stream", "g4go_enqueue_emissions_device",
 )
 
 _lib = None
@@ -291,6 +291,8 @@ def load() -> C.CDLL:
     lib.g4go_get_launch.argtypes = [H, C.POINTER(Launch)]
     lib.g4go_prepare_scene.argtypes = [H, C.POINTER(Scene)]
     lib.g4go_enqueue_emissions.argtypes = [H, C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint64, C.c_void_p]
+    lib.g4go_enqueue_emissions_device.argtypes = [H, C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint64, C.c_void_p]
+    lib.g4go_set_stream.argtypes = [H, C.c_void_p]
     lib.g4go_enqueue_synthetic.argtypes = [H, C.POINTER(SyntheticSource), C.c_void_p, C.c_void_p, C.c_uint64]
     lib.g4go_complete.argtypes = [H, C.POINTER(BatchResult)]
     lib.g4go_complete_device.argtypes = [H, C.POINTER(BatchResult), C.POINTER(C.c_void_p)]
@@ -306,7 +308,8 @@ def load() -> C.CDLL:
     lib.g4go_version.restype = C.c_char_p
     for name in ("g4go_create", "g4go_set_launch", "g4go_get_launch", "g4go_prepare_scene", "g4go_enqueue_emissions",
                  "g4go_enqueue_synthetic", "g4go_complete", "g4go_complete_device", "g4go_eval_math", "g4go_eval_fresnel", "g4go_device_info", "g4go_trace_synthetic",
-                 "g4go_set_parameter_layout", "g4go_get_parameters", "g4go_set_parameters", "g4go_adjoint"):
+                 "g4go_set_parameter_layout", "g4go_get_parameters", "g4go_set_parameters", "g4go_adjoint",
+                 "g4go_set_stream", "g4go_enqueue_emissions_device"):
         getattr(lib, name).restype = C.c_int
     _lib = lib
     return lib

@@ -998,6 +998,10 @@ struct Core {
     int cu_count = 256;
     hipStream_t stream = nullptr;
     hipEvent_t ev_start = nullptr, ev_kernel0 = nullptr, ev_kernel1 = nullptr, ev_meta = nullptr;
+    // the caller's stream (g4go_set_stream): each batch waits for the work
+    // queued on it before the batch was enqueued
+    hipStream_t user_stream = nullptr;
+    hipEvent_t ev_user = nullptr;
 
     HostScene scene;
     bool scene_built = false;
@@ -1066,6 +1070,7 @@ struct Core {
         hip_check(hipEventCreate(&ev_kernel0), "hipEventCreate");
         hip_check(hipEventCreate(&ev_kernel1), "hipEventCreate");
         hip_check(hipEventCreate(&ev_meta), "hipEventCreate");
+        hip_check(hipEventCreateWithFlags(&ev_user, hipEventDisableTiming), "hipEventCreate");
         hip_ready = true;
     }
 
@@ -1156,7 +1161,7 @@ struct Core {
         h_stage.release();
         h_meta.release();
         h_hits.release();
-        for (hipEvent_t* e : {&ev_start, &ev_kernel0, &ev_kernel1, &ev_meta}) {
+        for (hipEvent_t* e : {&ev_start, &ev_kernel0, &ev_kernel1, &ev_meta, &ev_user}) {
             if (*e != nullptr) {
                 (void)hipEventDestroy(*e);
                 *e = nullptr;
@@ -1257,6 +1262,10 @@ struct Core {
     void run_batch(const Emission* emissions, uint64_t n_emissions, const std::vector<uint32_t>& source_counts,
                    const std::vector<uint32_t>& source_event, const SyntheticSource* synth) {
         init_hip();
+        if (user_stream != nullptr) {
+            hip_check(hipEventRecord(ev_user, user_stream), "hipEventRecord caller stream");
+            hip_check(hipStreamWaitEvent(stream, ev_user, 0), "hipStreamWaitEvent caller stream");
+        }
         upload_scene();
         const uint64_t n_events = event_ids.size();
         const uint64_t n_sources = source_counts.size();
@@ -2172,6 +2181,56 @@ g4go_status g4go_enqueue_emissions(g4go_transport* t, const g4go_emission* emiss
         ++t->count;
     }
     return s;
+}
+
+g4go_status g4go_set_stream(g4go_transport* t, void* stream) {
+    if (t == nullptr) {
+        return G4GO_ERROR_INVALID_ARGUMENT;
+    }
+    return t->each([&](Core* c) {
+        c->user_stream = static_cast<hipStream_t>(stream);
+        return G4GO_OK;
+    });
+}
+
+g4go_status g4go_enqueue_emissions_device(g4go_transport* t, const g4go_emission* emissions_device,
+                                          uint64_t n_emissions, const uint32_t* event_ids, uint64_t n_events,
+                                          const uint32_t* emission_event_indices_device) {
+    if (t == nullptr) {
+        return G4GO_ERROR_INVALID_ARGUMENT;
+    }
+    if (t->count == t->cores.size()) {
+        return t->logic("transport queue is full");
+    }
+    Core* c = t->slot(t->head + t->count);
+    std::vector<g4go_emission> em;
+    std::vector<uint32_t> idx;
+    const g4go_status copied = t->pass(c, guarded(c, [&] {
+        if (n_emissions != 0 && (emissions_device == nullptr || emission_event_indices_device == nullptr)) {
+            fail(G4GO_ERROR_INVALID_ARGUMENT, "null batch array");
+        }
+        if (n_emissions > 0xffffffffull) {
+            fail(G4GO_ERROR_OVERFLOW, "emission batch exceeds the supported launch size");
+        }
+        c->init_hip();
+        em.resize(n_emissions);
+        idx.resize(n_emissions);
+        if (n_emissions != 0) {
+            // after the caller's work on its stream (the records' producer)
+            hipStream_t s = c->user_stream != nullptr ? c->user_stream : c->stream;
+            hip_check(hipMemcpyAsync(em.data(), emissions_device, n_emissions * sizeof(g4go_emission),
+                                     hipMemcpyDeviceToHost, s),
+                      "hipMemcpyAsync emission records");
+            hip_check(hipMemcpyAsync(idx.data(), emission_event_indices_device, n_emissions * sizeof(uint32_t),
+                                     hipMemcpyDeviceToHost, s),
+                      "hipMemcpyAsync emission event indices");
+            hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+        }
+    }));
+    if (copied != G4GO_OK) {
+        return copied;
+    }
+    return g4go_enqueue_emissions(t, em.data(), n_emissions, event_ids, n_events, idx.data());
 }
 
 g4go_status g4go_enqueue_synthetic(g4go_transport* t, const g4go_synthetic_source* source, const uint32_t* event_ids,

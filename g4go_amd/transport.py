@@ -167,6 +167,44 @@ class HipTransportHost:
                                                      ev.ctypes.data if ev.size else None, ev.size,
                                                      idx.ctypes.data if idx.size else None))
 
+    # ---- device-memory interop ----
+    def set_stream(self, stream) -> None:
+        """Orders every later batch after the work queued on `stream` (a
+        torch.cuda.Stream, a raw hipStream_t as int, or None to stop)."""
+        handle = getattr(stream, "cuda_stream", stream)
+        self._check(self._lib.g4go_set_stream(self._h, C.c_void_p(int(handle)) if handle else None))
+
+    def enqueue_emissions_device(self, scene: Scene, emissions, event_ids: np.ndarray, emission_event_indices) -> None:
+        """EnqueueEmissions with the records (a device tensor of n x 128 bytes,
+        abi.EMISSION_DTYPE layout) and the emission event indices (a device
+        tensor of n uint32/int32) in device memory; event IDs on the host."""
+        if emissions.numel() and (not emissions.is_cuda or not emission_event_indices.is_cuda):
+            raise ValueError("records and indices must be device tensors")
+        if not emissions.is_contiguous() or not emission_event_indices.is_contiguous():
+            raise ValueError("device tensors must be contiguous")
+        n = emissions.numel() * emissions.element_size() // abi.EMISSION_DTYPE.itemsize
+        if emission_event_indices.numel() != n or emission_event_indices.element_size() != 4:
+            raise ValueError("one 4-byte event index per record")
+        self.prepare_scene(scene)
+        ev = np.ascontiguousarray(event_ids, dtype=np.uint32)
+        self._check(self._lib.g4go_enqueue_emissions_device(
+            self._h, emissions.data_ptr() if n else None, n, ev.ctypes.data if ev.size else None, ev.size,
+            emission_event_indices.data_ptr() if n else None))
+
+    @staticmethod
+    def hits_tensor(hits_device: int, n_hits: int):
+        """A torch view (uint8, n_hits x 48, abi.HIT_DTYPE records) of the hits
+        complete_device left in HBM; valid until the next enqueue."""
+        import torch
+
+        class _View:
+            __cuda_array_interface__ = {"shape": (n_hits, abi.HIT_DTYPE.itemsize), "typestr": "|u1",
+                                        "data": (hits_device, False), "version": 2, "strides": None}
+
+        if n_hits == 0:
+            return torch.empty((0, abi.HIT_DTYPE.itemsize), dtype=torch.uint8, device="cuda")
+        return torch.as_tensor(_View(), device="cuda")
+
     def enqueue_synthetic(self, scene: Scene, source: abi.SyntheticSource, event_ids: np.ndarray,
                           photon_counts: np.ndarray) -> None:
         self.prepare_scene(scene)

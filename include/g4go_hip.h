@@ -274,6 +274,25 @@ g4go_status g4go_enqueue_emissions(g4go_transport* handle, const g4go_emission* 
 g4go_status g4go_enqueue_synthetic(g4go_transport* handle, const g4go_synthetic_source* source,
                                    const uint32_t* event_ids, const uint32_t* photon_counts, uint64_t n_events);
 
+/* ---- device-memory interop (torch and other stream-ordered callers) ----
+ * No counterpart in the reference's interface; the reference's caller hands
+ * over host spans (OpticalBatchScheduler.cpp:521, 551).
+ *
+ * g4go_set_stream: every batch enqueued afterwards first waits for the work
+ * queued on `stream` (a hipStream_t, e.g. torch.cuda.current_stream()) when
+ * it was enqueued; NULL stops waiting.  The backend's own streams keep the
+ * default priority.
+ *
+ * g4go_enqueue_emissions_device: g4go_enqueue_emissions with the records and
+ * the emission event indices in device memory (read in order after the work
+ * on the stream set by g4go_set_stream, else on the backend's stream); the
+ * event IDs are a host array.  The records are validated as on the host
+ * path. */
+g4go_status g4go_set_stream(g4go_transport* handle, void* stream);
+g4go_status g4go_enqueue_emissions_device(g4go_transport* handle, const g4go_emission* emissions_device,
+                                          uint64_t n_emissions, const uint32_t* event_ids, uint64_t n_events,
+                                          const uint32_t* emission_event_indices_device);
+
 /* ---- CompleteOldestBatch (OptiXTransportHost.cpp:779-883) ---- */
 g4go_status g4go_complete(g4go_transport* handle, g4go_batch_result* out);
 

@@ -285,3 +285,30 @@ def test_emission_launch_order_with_shuffled_records(crystal):
                  < f + em[k]["photon_count"]]
         rec.append(cands[0])
     assert all(a <= b for a, b in zip(rec, rec[1:]))
+
+
+def test_device_memory_interop(crystal):
+    """Records and indices from torch device tensors, batches ordered after the
+    caller's torch stream, hits viewed as a torch tensor: the same detections
+    as the host path."""
+    import torch
+
+    log("case: device-memory interop (torch tensors, caller stream)")
+    em, event_ids, idx = workloads.electron_emissions(crystal, 5, seed=9)
+    with host(27) as h:
+        h.enqueue_emissions(crystal.scene, em, event_ids, idx)
+        want = h.complete_oldest_batch()
+    stream = torch.cuda.Stream()
+    with torch.cuda.stream(stream):
+        d_em = torch.from_numpy(em.view(np.uint8).copy()).to("cuda", non_blocking=True)
+        d_idx = torch.from_numpy(idx.astype(np.int32)).to("cuda", non_blocking=True)
+    with host(27) as h:
+        h.set_stream(stream)
+        h.enqueue_emissions_device(crystal.scene, d_em, event_ids, d_idx)
+        out, ptr = h.complete_device()
+        hits = h.hits_tensor(ptr, int(out.statistics["n_hits"])).cpu().numpy()
+        h.set_stream(None)
+    got = np.frombuffer(hits.tobytes(), dtype=abi.HIT_DTYPE)
+    assert len(got) == len(want.detections) > 1000
+    assert_same_hits(got, want.detections)
+    assert out.event_statistics.tobytes() == want.event_statistics.tobytes()
