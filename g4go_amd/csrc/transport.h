@@ -613,6 +613,36 @@ static __host__ __device__ __attribute__((noinline)) float tri_closest(const Sce
     return best;
 }
 
+// Fallback exit from a triangle solid when no triangle ahead is a leaving
+// hit: among the triangles the ray moves away from (n.d > 0), the one whose
+// plane lies nearest the point (ties: lower key); the distance to that plane
+// (0 when the point is beyond it).  kInf when none.
+static __host__ __device__ __attribute__((noinline)) float tri_nearest_leaving(const SceneView& sc, const Volume& vol,
+                                                                            V3 p, V3 d, uint32_t& face) {
+    const Tri* tris = scene_tris(sc);
+    const uint32_t begin = vol.plane_begin, count = vol.plane_count & ~kSolidTriangles;
+    float best_gap = kInf, best_num = 0.0f, best_nd = 1.0f;
+    uint32_t best_i = kInvalid, best_key = kInvalid;
+    for (uint32_t i = begin; i < begin + count; ++i) {
+        const Plane& f = tris[i].face;
+        const V3 n = {f.n[0], f.n[1], f.n[2]};
+        const float nd = dot(n, d);
+        if (nd > 0.0f) {
+            const float num = f.h - dot(n, p);
+            const float gap = fabsf(num);
+            if (gap < best_gap || (gap == best_gap && tris[i].key < best_key)) {
+                best_gap = gap;
+                best_num = num;
+                best_nd = nd;
+                best_i = i;
+                best_key = tris[i].key;
+            }
+        }
+    }
+    face = best_i == kInvalid ? kInvalid : (best_i | kFaceTri);
+    return best_i == kInvalid ? kInf : fmaxf(best_num, 0.0f) / best_nd;
+}
+
 // A direction that meets no axis-aligned edge or face of a mesh edge-on:
 // (1, 2, 3) / sqrt(14).
 constexpr float kContainDir[3] = {0.26726124f, 0.53452248f, 0.80178373f};
@@ -1144,7 +1174,14 @@ G4GO_HD int transport_step(Photon& ph, const SceneView& sc, const Rng& rng, cons
         bool leave;
         dist = tri_closest(sc, vol, ph.pos, ph.dir, 0.0f, kTriLeaving, plane, leave);
         if (!finitef(dist)) {
-            return kInvalidState;  // no triangle ahead: the photon is not inside its solid
+            // No leaving triangle ahead: rounding left the photon a hair
+            // outside its solid at an edge (after a reflection there), moving
+            // out past the edge.  It leaves through the nearest face it moves
+            // away from, as the clipping of a convex solid does.
+            dist = tri_nearest_leaving(sc, vol, ph.pos, ph.dir, plane);
+            if (!finitef(dist)) {
+                return kInvalidState;
+            }
         }
     } else if (kMesh && vol.plane_count != 0u) {
         mesh_exit(sc.planes, vol.plane_begin, vol.plane_count, ph.pos, ph.dir, dist, plane);

@@ -1148,6 +1148,31 @@ static real tri_closest(const oscene_t* s, const volume_t* v, v3 p, v3 d, real t
     return best;
 }
 
+/* fallback exit when no triangle ahead is a leaving hit: among the triangles
+ * with n.d > 0 the one whose plane is nearest the point (ties: lower index),
+ * at max(h - n.p, 0) / n.d */
+static real tri_nearest_leaving(const oscene_t* s, const volume_t* v, v3 p, v3 d, uint32_t* face) {
+    real best_gap = R_INF, best_num = RF(0.0), best_nd = RF(1.0);
+    uint32_t bi = INVALID;
+    for (uint32_t k = v->plane_begin; k < v->plane_begin + v->plane_count; ++k) {
+        const plane_t* f = &s->tris[k].face;
+        const v3 n = mk(f->n[0], f->n[1], f->n[2]);
+        const real nd = vdot(n, d);
+        if (nd > RF(0.0)) {
+            const real num = f->h - vdot(n, p);
+            const real gap = R_FABS(num);
+            if (gap < best_gap) {
+                best_gap = gap;
+                best_num = num;
+                best_nd = nd;
+                bi = k;
+            }
+        }
+    }
+    *face = bi == INVALID ? INVALID : (bi | FACE_TRI);
+    return bi == INVALID ? R_INF : R_FMAX(best_num, RF(0.0)) / best_nd;
+}
+
 static int tri_contains(const oscene_t* s, const volume_t* v, v3 q) {
     uint32_t f;
     int l;
@@ -1524,7 +1549,13 @@ static int or_step(photon_t* ph, const oscene_t* s, const rng_t* r, real eps, in
         int l;
         dist = tri_closest(s, vol, ph->pos, ph->dir, RF(0.0), 0, &plane, &l);
         if (!finite_r(dist)) {
-            return S_INVALID;
+            /* rounding left the photon a hair outside its solid at an edge,
+             * moving out past it: it leaves through the nearest face it moves
+             * away from */
+            dist = tri_nearest_leaving(s, vol, ph->pos, ph->dir, &plane);
+            if (!finite_r(dist)) {
+                return S_INVALID;
+            }
         }
     } else if (vol->plane_count) {
         mesh_exit(s, vol, ph->pos, ph->dir, &dist, &plane);

@@ -225,3 +225,82 @@ def test_triangle_solid_adjoint_matches_oracle_replay():
     ref, counters = O.adjoint_synthetic(sc, src, 37, out.detections, w, layout, n_params)
     assert int(counters[0]) == len(out.detections) and int(counters[1]) == 0
     np.testing.assert_allclose(grad, ref, rtol=1e-9, atol=1e-9 * np.max(np.abs(ref)))
+
+
+def test_mesh_graph_replays_and_full_size_invariants():
+    """Mesh instances under hipGraph replay (several batches, eager work in
+    between) and per-event launches equal an eager launch; at 1e8 photons in
+    the L-shaped guide every photon ends in one category and hits =
+    detections = histogram total."""
+    log("case: tapered crystal graph replays and per-event launches; L guide at 1e8 photons")
+    cs = workloads.tapered_crystal_scene()
+    src = workloads.synthetic_source(cs)
+    batches = [workloads.lognormal_counts(n, 300, sigma=1.5, lo=1, hi=20000, seed=s) for n, s in ((30, 1), (20, 2))]
+    with HipTransportHost(PhotonTransportConfig(seed=6, sort_hits=True)) as eager, \
+            HipTransportHost(PhotonTransportConfig(seed=6, sort_hits=True)) as graphed, \
+            HipTransportHost(PhotonTransportConfig(seed=6, sort_hits=True)) as per_event:
+        graphed.set_launch(LaunchForm(persistent=True, block_size=256, chunk=256, refill_threshold=16, blocks_per_cu=4,
+                                      graph=True))
+        per_event.set_launch(LaunchForm(persistent=False, per_event=True))
+        for k, counts in enumerate(batches):
+            ev = np.arange(100 * k, 100 * k + counts.size, dtype=np.uint32)
+            eager.enqueue_synthetic(cs.scene, src, ev, counts)
+            want = eager.complete_oldest_batch()
+            for h in (graphed, per_event):
+                h.enqueue_synthetic(cs.scene, src, ev, counts)
+                got = h.complete_oldest_batch()
+                assert_same_hits(got.detections, want.detections)
+                assert got.event_statistics.tobytes() == want.event_statistics.tobytes()
+                np.testing.assert_array_equal(got.histogram, want.histogram)
+    sc, gsrc, _, _ = workloads.l_guide_scene()
+    out = device_run(sc, gsrc, 38, [0], [100_000_000], sort_hits=False)
+    st = out.statistics
+    assert st["generated"] == 100_000_000 and st["invalid"] == 0
+    assert st["detected"] + st["absorbed"] + st["escaped"] + st["truncated"] == st["generated"]
+    assert st["n_hits"] == st["detected"] == int(out.histogram.sum())
+
+
+def nested_mesh_scene():
+    """Mesh parents with mesh or box daughters: a glass housing (a box mesh
+    turned by a rotation) holding the turned crystal and its SiPM (box meshes,
+    wrap and cathode as in the crystal scene), and beside it a glass tube
+    (a triangle solid; its border to the world detects, reflects or lets
+    through, so photons also cross the hole and re-enter) holding a glass
+    bead (a box) in its wall."""
+    from g4go_amd.scene import Surface as Surf, SurfaceBinding as Bind, box_mesh, tube_mesh
+
+    rot = workloads.rotation_matrix((1.0, -2.0, 0.5), 1.1)
+    cs = workloads.rotated_crystal_scene(rot)
+    sc = cs.scene
+    glass = sc.add_material(Material(name="glass", rindex=PropertyTable.constant_value(1.5),
+                                     abs_length_mm=PropertyTable.constant_value(80.0)))
+    housing = sc.add_volume(Volume(name="Housing", material_id=glass, parent_volume_id=cs.world_id,
+                                   mesh=box_mesh((3.0, 3.0, 12.0)).transformed(rot, (0.0, 0.0, 0.0))))
+    for vid in (cs.crystal_ids[0], cs.sipm_ids[0]):
+        sc.volumes[vid].parent_volume_id = housing
+    sc.surface_bindings[0] = Bind(cs.crystal_ids[0], housing, cs.wrap_surface)
+    det = sc.add_surface(Surf(name="det", type=SurfaceType.DielectricMetal,
+                              reflectivity=PropertyTable.constant_value(0.1),
+                              transmittance=PropertyTable.constant_value(0.3),
+                              efficiency=PropertyTable.constant_value(0.9)))
+    tube = sc.add_volume(Volume(name="Tube", material_id=glass, parent_volume_id=cs.world_id, sensor_id=7,
+                                mesh=tube_mesh(3.0, 6.0, 10.0, 16).transformed(None, (30.0, 0.0, 0.0))))
+    sc.add_surface_binding(Bind(tube, cs.world_id, det))
+    sc.add_volume(Volume(name="Bead", material_id=cs.crystal_materials[0], parent_volume_id=tube,
+                         center_mm=(34.5, 0.0, 0.0), half_mm=(0.8, 0.8, 3.0)))
+    src = abi.SyntheticSource()
+    src.lo_mm[:] = [33.6, -0.5, -5.0]   # in the tube wall, around the bead
+    src.hi_mm[:] = [35.4, 0.5, 5.0]
+    src.wavelength_mean_nm, src.wavelength_sigma_nm, src.decay_time_ns = 420.0, 20.0, 10.0
+    src.volume_id = tube
+    return cs, src
+
+
+def test_nested_meshes_bit_exact():
+    log("case: nested mesh solids (housing > crystal + SiPM; tube > bead)")
+    cs, tube_src = nested_mesh_scene()
+    rot_src = workloads.point_source(cs.crystal_ids[0], (0.0, 0.0, 0.0))
+    out = parity(cs.scene, rot_src, 39, [0], [20000])
+    assert out.statistics["detected"] > 2000
+    out = parity(cs.scene, tube_src, 40, [0, 1], [20000, 3000])
+    assert out.statistics["detected"] > 1000 and out.statistics["escaped"] > 100
