@@ -244,6 +244,7 @@ def run_mesh(seed, launch, steps, warmup, photons, box_ms_per_step):
         res[name] = {"photons_per_s": photons * steps / wall, "kernel_ms": k_ms,
                      "steps_per_photon": st["total_steps"] / max(1, st["generated"]),
                      "detected_fraction": st["detected"] / max(1, st["generated"]),
+                     "invalid": st["invalid"],
                      "step_cost_over_box": (k_ms / st["total_steps"]) / box_ms_per_step}
     return res
 

@@ -253,11 +253,16 @@ def test_mesh_graph_replays_and_full_size_invariants():
                 assert got.event_statistics.tobytes() == want.event_statistics.tobytes()
                 np.testing.assert_array_equal(got.histogram, want.histogram)
     sc, gsrc, _, _ = workloads.l_guide_scene()
-    out = device_run(sc, gsrc, 38, [0], [100_000_000], sort_hits=False)
-    st = out.statistics
-    assert st["generated"] == 100_000_000 and st["invalid"] == 0
-    assert st["detected"] + st["absorbed"] + st["escaped"] + st["truncated"] == st["generated"]
-    assert st["n_hits"] == st["detected"] == int(out.histogram.sum())
+    rot = workloads.rotation_matrix((1.0, 2.0, 3.0), 0.7)
+    rc = workloads.rotated_crystal_scene(rot)
+    for name, scene, source in (("L guide", sc, gsrc), ("tapered crystal", cs.scene, src),
+                                ("rotated crystal", rc.scene, workloads.point_source(rc.crystal_ids[0], (0, 0, 0)))):
+        log(f"  {name}: 1e8 photons")
+        out = device_run(scene, source, 38, [0], [100_000_000], sort_hits=False)
+        st = out.statistics
+        assert st["generated"] == 100_000_000 and st["invalid"] == 0, (name, st["invalid"])
+        assert st["detected"] + st["absorbed"] + st["escaped"] + st["truncated"] == st["generated"]
+        assert st["n_hits"] == st["detected"] == int(out.histogram.sum())
 
 
 def nested_mesh_scene():
