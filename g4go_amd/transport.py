@@ -182,7 +182,10 @@ class HipTransportHost:
             raise ValueError("records and indices must be device tensors")
         if not emissions.is_contiguous() or not emission_event_indices.is_contiguous():
             raise ValueError("device tensors must be contiguous")
-        n = emissions.numel() * emissions.element_size() // abi.EMISSION_DTYPE.itemsize
+        nbytes = emissions.numel() * emissions.element_size()
+        if nbytes % abi.EMISSION_DTYPE.itemsize:
+            raise ValueError("the records tensor must hold whole 128-byte records")
+        n = nbytes // abi.EMISSION_DTYPE.itemsize
         if emission_event_indices.numel() != n or emission_event_indices.element_size() != 4:
             raise ValueError("one 4-byte event index per record")
         self.prepare_scene(scene)
