@@ -529,88 +529,102 @@ G4GO_HD bool wt_hit(const WtRay& r, const Tri& tr, float& t, bool& leaving) {
 
 enum TriMode : int { kTriLeaving = 0, kTriEntering = 1, kTriAny = 2 };
 
-// Closest hit at t >= tmin on the triangles of triangle solid `vol` with the
-// orientation `mode` asks for: distance (kInf on none), face (kFaceTri | the
-// triangle's index, kInvalid on none) and whether it leaves the solid.  Ties
-// go to the lower key.  Out of line: inlined at its four call sites its
-// traversal stack costs the triangle-level kernels more in spills than the
-// calls do (13% on the L-shaped light guide).
-static __host__ __device__ __attribute__((noinline)) float tri_closest(const SceneView& sc, const Volume& vol, V3 p, V3 d, float tmin, int mode, uint32_t& face,
-                          bool& leaving) {
+// Closest hit so far of a triangle search.
+struct TriBest {
+    float t;
+    uint32_t index, key;
+    bool leaving;
+};
+
+G4GO_HD void tri_test(const Tri* tris, const WtRay& ray, uint32_t i, float tmin, int mode, TriBest& b) {
+    float t;
+    bool leave;
+    const Tri& tr = tris[i];
+    if (wt_hit(ray, tr, t, leave) && t >= tmin && (mode == kTriAny || leave == (mode == kTriLeaving)) &&
+        (t < b.t || (t == b.t && tr.key < b.key))) {
+        b.t = t;
+        b.index = i;
+        b.key = tr.key;
+        b.leaving = leave;
+    }
+}
+
+// Near-first walk of a triangle solid's BVH, as for the daughter BVH: nodes
+// beyond the best hit so far are skipped.  Out of line: inlined at the call
+// sites its traversal stack costs the triangle-level kernels more in spills
+// than the calls do.
+static __host__ __device__ __attribute__((noinline)) void tri_walk(const SceneView& sc, uint32_t root, V3 p, V3 d,
+                                                                  const WtRay& ray, float tmin, int mode, TriBest& b) {
     const Tri* tris = scene_tris(sc);
-    const WtRay ray = wt_ray(p, d);
-    float best = kInf;
-    uint32_t best_i = kInvalid, best_key = kInvalid;
-    bool best_leave = false;
-    const auto test = [&](uint32_t i) {
-        float t;
-        bool leave;
-        const Tri& tr = tris[i];
-        if (wt_hit(ray, tr, t, leave) && t >= tmin && (mode == kTriAny || leave == (mode == kTriLeaving)) &&
-            (t < best || (t == best && tr.key < best_key))) {
-            best = t;
-            best_i = i;
-            best_key = tr.key;
-            best_leave = leave;
+    const V3 inv = {1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
+    uint32_t stack[kBvhMaxDepth + 1];
+    float stack_t[kBvhMaxDepth + 1];
+    int top = 0;
+    {
+        const BvhNode& n0 = sc.bvh[root];
+        const float t = node_entry(n0.lo, n0.hi, p, inv, tmin);
+        if (t < kInf) {
+            stack[0] = root;
+            stack_t[0] = t;
+            top = 1;
         }
-    };
-    const uint32_t begin = vol.plane_begin, count = vol.plane_count & ~kSolidTriangles;
-    const uint32_t root = vol.faces[0].neighbor;
-    if (root == kInvalid) {
-        for (uint32_t i = begin; i < begin + count; ++i) {
-            test(i);
+    }
+    while (top > 0) {
+        --top;
+        if (!(stack_t[top] <= b.t)) {
+            continue;
         }
-    } else {
-        // near-first walk, as for the daughter BVH; nodes beyond the best hit are skipped
-        const V3 inv = {1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
-        uint32_t stack[kBvhMaxDepth + 1];
-        float stack_t[kBvhMaxDepth + 1];
-        int top = 0;
-        {
-            const BvhNode& n0 = sc.bvh[root];
-            const float t = node_entry(n0.lo, n0.hi, p, inv, tmin);
-            if (t < kInf) {
-                stack[0] = root;
-                stack_t[0] = t;
-                top = 1;
+        const BvhNode& node = sc.bvh[stack[top]];
+        if ((node.right_or_count & kBvhLeaf) != 0) {
+            const uint32_t n = node.right_or_count & ~kBvhLeaf;
+            for (uint32_t k = 0; k < n; ++k) {
+                tri_test(tris, ray, node.left + k, tmin, mode, b);
             }
-        }
-        while (top > 0) {
-            --top;
-            if (!(stack_t[top] <= best)) {
-                continue;
+        } else {
+            const uint32_t l = node.left, rr = node.right_or_count;
+            const BvhNode& nl = sc.bvh[l];
+            const BvhNode& nr = sc.bvh[rr];
+            const float tl = node_entry(nl.lo, nl.hi, p, inv, tmin);
+            const float tr = node_entry(nr.lo, nr.hi, p, inv, tmin);
+            const bool left_near = tl <= tr;
+            const float t_far = left_near ? tr : tl;
+            const float t_near = left_near ? tl : tr;
+            if (t_far < kInf && t_far <= b.t && top <= kBvhMaxDepth) {
+                stack[top] = left_near ? rr : l;
+                stack_t[top] = t_far;
+                ++top;
             }
-            const BvhNode& node = sc.bvh[stack[top]];
-            if ((node.right_or_count & kBvhLeaf) != 0) {
-                const uint32_t n = node.right_or_count & ~kBvhLeaf;
-                for (uint32_t k = 0; k < n; ++k) {
-                    test(node.left + k);
-                }
-            } else {
-                const uint32_t l = node.left, rr = node.right_or_count;
-                const BvhNode& nl = sc.bvh[l];
-                const BvhNode& nr = sc.bvh[rr];
-                const float tl = node_entry(nl.lo, nl.hi, p, inv, tmin);
-                const float tr = node_entry(nr.lo, nr.hi, p, inv, tmin);
-                const bool left_near = tl <= tr;
-                const float t_far = left_near ? tr : tl;
-                const float t_near = left_near ? tl : tr;
-                if (t_far < kInf && t_far <= best && top <= kBvhMaxDepth) {
-                    stack[top] = left_near ? rr : l;
-                    stack_t[top] = t_far;
-                    ++top;
-                }
-                if (t_near < kInf && t_near <= best && top <= kBvhMaxDepth) {
-                    stack[top] = left_near ? l : rr;
-                    stack_t[top] = t_near;
-                    ++top;
-                }
+            if (t_near < kInf && t_near <= b.t && top <= kBvhMaxDepth) {
+                stack[top] = left_near ? l : rr;
+                stack_t[top] = t_near;
+                ++top;
             }
         }
     }
-    face = best_i == kInvalid ? kInvalid : (best_i | kFaceTri);
-    leaving = best_leave;
-    return best;
+}
+
+// Closest hit at t >= tmin on the triangles of triangle solid `vol` with the
+// orientation `mode` asks for: distance (kInf on none), face (kFaceTri | the
+// triangle's index, kInvalid on none) and whether it leaves the solid.  Ties
+// go to the lower key.  A solid without a BVH (at most 64 triangles) is
+// tested in a loop, inline.
+G4GO_HD float tri_closest(const SceneView& sc, const Volume& vol, V3 p, V3 d, float tmin, int mode, uint32_t& face,
+                          bool& leaving) {
+    const Tri* tris = scene_tris(sc);
+    const WtRay ray = wt_ray(p, d);
+    TriBest b{kInf, kInvalid, kInvalid, false};
+    const uint32_t root = vol.faces[0].neighbor;
+    if (root == kInvalid) {
+        const uint32_t begin = vol.plane_begin, count = vol.plane_count & ~kSolidTriangles;
+        for (uint32_t i = begin; i < begin + count; ++i) {
+            tri_test(tris, ray, i, tmin, mode, b);
+        }
+    } else {
+        tri_walk(sc, root, p, d, ray, tmin, mode, b);
+    }
+    face = b.index == kInvalid ? kInvalid : (b.index | kFaceTri);
+    leaving = b.leaving;
+    return b.t;
 }
 
 // Fallback exit from a triangle solid when no triangle ahead is a leaving
