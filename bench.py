@@ -28,9 +28,10 @@ back to the host) with one and with two batches in flight; `mesh_crystals`
 times 1e8 photons in two crystals with mesh solids: the crystal tapered to a
 4x4 mm back face (a frustum mesh, source uniform in the 3x3x20 mm box inside
 it) and the 3x3x20 mm crystal turned by a rotation (box meshes, point source
-at its centre), both convex, and an L-shaped light guide (a non-convex prism
-held as triangles), each with its transport-step cost relative to the box
-crystal.
+at its centre), both convex, a ring of 48 bare crystals turned to face its
+centre (96 box meshes under a BVH; photons from crystal 0), and an L-shaped
+light guide (a non-convex prism held as triangles), each with its
+transport-step cost relative to the box crystal.
 """
 from __future__ import annotations
 
@@ -231,8 +232,10 @@ def run_mesh(seed, launch, steps, warmup, photons, box_ms_per_step):
     rotated = workloads.rotated_crystal_scene(rot)
     guide, guide_src, _, _ = workloads.l_guide_scene()
     guide_cs = workloads.CrystalScene(guide, guide.world_volume_id, [], [], [], -1, -1, workloads.CrystalParams())
+    ring, ring_src = workloads.ring_scene()
     cases = {"tapered_crystal": (tapered, workloads.synthetic_source(tapered)),
              "rotated_crystal": (rotated, workloads.point_source(rotated.crystal_ids[0], (0.0, 0.0, 0.0))),
+             "crystal_ring_48": (ring, ring_src),
              "l_guide_triangles": (guide_cs, guide_src)}
     res = {}
     for name, (cs, src) in cases.items():

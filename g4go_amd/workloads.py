@@ -17,8 +17,9 @@
 * mesh solids: the crystal tapered from a 3x3 mm front face to a 4x4 mm back
   face (a frustum given as a 12-triangle convex mesh, SiPM box on the back
   face), the single-crystal scene turned by a rotation (crystal and SiPM as
-  box meshes), and an L-shaped light guide (a non-convex prism, 20
-  triangles) read out at the end of its long arm.
+  box meshes), an L-shaped light guide (a non-convex prism, 20 triangles)
+  read out at the end of its long arm, and a ring of 48 crystals turned to
+  face its centre (box meshes), read out on their outer faces.
 """
 from __future__ import annotations
 
@@ -237,6 +238,46 @@ def l_guide_scene(abs_length_mm: float = 30.0):
     src.wavelength_mean_nm, src.wavelength_sigma_nm, src.decay_time_ns = 420.0, 20.0, 40.0
     src.volume_id = guide
     return scene, src, glass, detect
+
+
+def ring_scene(n: int = 48, radius_mm: float = 40.0, p: CrystalParams | None = None, wrapped: bool = False):
+    """A ring of n crystals (3x3x20 mm box meshes, the long axis radial, centres
+    at radius_mm in the z = 0 plane), each turned to face the centre and read
+    out by a SiPM (box mesh) on its outer face; bare crystal faces (Fresnel
+    with the air between them) unless `wrapped`.  Returns the CrystalScene
+    and a synthetic source uniform in a box inside crystal 0 (the one on +x,
+    whose faces are axis-aligned)."""
+    p = p or CrystalParams()
+    scene = Scene()
+    air, silicon, wrap, cathode = _common_materials_and_surfaces(scene, p)
+    world = scene.add_volume(Volume(name="World", material_id=air, center_mm=(0, 0, 0), half_mm=(100, 100, 100)))
+    scene.world_volume_id = world
+    hx, hy, hz = p.half_mm
+    crystals, sipms, mats = [], [], []
+    for k in range(n):
+        phi = 2.0 * np.pi * k / n
+        radial = np.array([np.cos(phi), np.sin(phi), 0.0])
+        tangential = np.array([-np.sin(phi), np.cos(phi), 0.0])
+        rot = np.c_[tangential, [0.0, 0.0, 1.0], radial]  # local x, y, z -> tangential, axial, radial
+        m = scene.add_material(_crystal_material(p, name=f"crystal{k}"))
+        c = scene.add_volume(Volume(name=f"Crystal{k}", material_id=m, parent_volume_id=world, half_mm=p.half_mm,
+                                    mesh=box_mesh((hx, hy, hz)).transformed(rot, radius_mm * radial)))
+        sp = scene.add_volume(Volume(name=f"SiPM{k}", material_id=silicon, parent_volume_id=world,
+                                     skin_surface_id=cathode, sensor_id=k, half_mm=(hx, hy, 0.5 * p.sipm_thickness_mm),
+                                     mesh=box_mesh((hx, hy, 0.5 * p.sipm_thickness_mm)).transformed(
+                                         rot, (radius_mm + hz + 0.5 * p.sipm_thickness_mm) * radial)))
+        if wrapped:
+            scene.add_surface_binding(SurfaceBinding(c, world, wrap))
+        crystals.append(c)
+        sipms.append(sp)
+        mats.append(m)
+    cs = CrystalScene(scene, world, crystals, sipms, mats, wrap, cathode, p)
+    src = abi.SyntheticSource()
+    src.lo_mm[:] = [radius_mm - hz + 0.5, -hx + 0.1, -hy + 0.1]
+    src.hi_mm[:] = [radius_mm + hz - 0.5, hx - 0.1, hy - 0.1]
+    src.wavelength_mean_nm, src.wavelength_sigma_nm, src.decay_time_ns = 420.0, 20.0, 40.0
+    src.volume_id = crystals[0]
+    return cs, src
 
 
 def rotation_matrix(axis, angle_rad: float) -> np.ndarray:

@@ -309,3 +309,14 @@ def test_nested_meshes_bit_exact():
     assert out.statistics["detected"] > 2000
     out = parity(cs.scene, tube_src, 40, [0, 1], [20000, 3000])
     assert out.statistics["detected"] > 1000 and out.statistics["escaped"] > 100
+
+
+def test_crystal_ring_bit_exact():
+    """48 bare crystals turned to face the ring's centre, with SiPMs (96 box
+    meshes under the world's daughter BVH): photons leave crystal 0 through
+    Fresnel faces and reach its neighbours' sensors."""
+    log("case: ring of 48 turned crystals (convex meshes, daughter BVH)")
+    cs, src = workloads.ring_scene()
+    out = parity(cs.scene, src, 5, [0, 1], [20000, 3000])
+    assert out.statistics["detected"] > 5000 and out.statistics["escaped"] > 5000
+    assert np.unique(out.detections["sensor_id"]).size > 10
