@@ -5,6 +5,7 @@
     python -m g4go_amd.profile_run --workload tapered   # mesh solids: tapered crystal
     python -m g4go_amd.profile_run --workload rotated   # mesh solids: crystal turned by a rotation
     python -m g4go_amd.profile_run --workload lguide    # triangle solid: L-shaped light guide
+    python -m g4go_amd.profile_run --workload ring      # 48 turned crystals (convex meshes)
     python -m g4go_amd.profile_run --copy-bytes 1073741824   # calibration copy
 
 The launch form is the cached autotuner result for the device when present,
@@ -23,7 +24,8 @@ import numpy as np
 
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", choices=("crystal", "uneven", "array", "tapered", "rotated", "lguide", "none"),
+    ap.add_argument("--workload", choices=("crystal", "uneven", "array", "tapered", "rotated", "lguide", "ring",
+                                              "none"),
                     default="crystal")
     ap.add_argument("--photons", type=int, default=100_000_000)
     ap.add_argument("--events", type=int, default=4096)
@@ -57,6 +59,8 @@ def main(argv=None) -> int:
     elif args.workload == "rotated":
         cs = workloads.rotated_crystal_scene(workloads.rotation_matrix((1.0, 2.0, 3.0), 0.7))
         src = workloads.point_source(cs.crystal_ids[0], (0.0, 0.0, 0.0))
+    elif args.workload == "ring":
+        cs, src = workloads.ring_scene()
     elif args.workload == "lguide":
         scene, src, _, _ = workloads.l_guide_scene()
         cs = workloads.CrystalScene(scene, scene.world_volume_id, [], [], [], -1, -1, workloads.CrystalParams())
