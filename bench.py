@@ -63,6 +63,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--photons", type=int, default=100_000_000)
     ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--tune-table-dir", default=None,
+                    help="directory that receives the autotuner's cache and sweep table when it sweeps")
     ap.add_argument("--cpu-sample", type=int, default=40_000_000, help="photons in the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
@@ -376,8 +378,8 @@ def main():
     else:
         persistent, table = autotune.tuned_launch(host, tune_key, "crystal-synthetic", tune_batch,
                                                   retune=args.retune)
-        out_dir = os.path.join(ROOT, "gpurun_out")
-        if table is not None and os.path.isdir(out_dir):
+        out_dir = args.tune_table_dir
+        if table is not None and out_dir and os.path.isdir(out_dir):
             shutil.copy(autotune.CACHE, os.path.join(out_dir, "autotune_cache.json"))
             with open(os.path.join(out_dir, "autotune_table.json"), "w") as f:
                 json.dump(table, f, indent=0)

@@ -1,7 +1,7 @@
 /*
  * oracle.c -- CPU restatement of G4GO's optical-photon transport, used ONLY
- * as the checker of the HIP backend (tests, smoke check, CPU baseline leg of
- * bench.py).  It is never linked into or called by the product.
+ * as the reference the HIP backend is tested against (tests, smoke check, CPU
+ * baseline leg of bench.py).  It is never linked into or called by the product.
  *
  * The same source builds twice:
  *   liboracle32.so  FP32 state, FP64 time: bit-exact replay of the HIP

@@ -5,7 +5,7 @@ scintillation emission records).
 Same Philox streams (seed 77, 500 events): the FP64 oracle replays every
 photon with the device's draws, so the two differ only where FP32 rounding
 changes a decision.  Per-event detected mean and width agree within 0.5%
-(north-star gate), and the arrival-time distributions pass a two-sample KS
+(the accuracy gate of the path), and the arrival-time distributions pass a two-sample KS
 test at p > 0.01.
 
 Independent streams (device seed 78 on all 5000 events, oracle seed 77 on
