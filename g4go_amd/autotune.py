@@ -27,7 +27,7 @@ DEFAULT_SPACE = {
     "block_size": (256,),
     "blocks_per_cu": (2, 4, 6, 8, 12),
     "chunk": (256, 1024, 4096),
-    "refill_threshold": (8, 16, 32),
+    "refill_threshold": (8, 16, 24, 32),
 }
 
 
