@@ -64,7 +64,7 @@ def parse():
     ap.add_argument("--photons", type=int, default=100_000_000)
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--tune-table-dir", default=None,
-                    help="directory that receives the autotuner's cache and sweep table when it sweeps")
+                    help="directory that receives the autotuner's cache (and the crystal sweep table when it sweeps)")
     ap.add_argument("--cpu-sample", type=int, default=40_000_000, help="photons in the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
@@ -192,11 +192,12 @@ def run_emission_batches(host, scene, em, ev, idx, launch, steps, warmup):
     return (time.perf_counter() - t0) / steps, float(np.mean(kms)), out
 
 
-def run_array(seed, launch, steps, warmup, plain):
+def run_array(seed, steps, warmup, plain, retune=False):
     """8x8 array with septa (128 boxes), 1024 uneven electron events (log-normal
-    median 2e4 photons): batched (persistent + hipGraph) vs per-event plain
-    launches, and the 2048-parameter per-crystal absorption gradient."""
-    from g4go_amd import abi, workloads
+    median 2e4 photons): batched (persistent + hipGraph, launch form autotuned
+    on this batch and cached under its own tag) vs per-event plain launches,
+    and the 2048-parameter per-crystal absorption gradient."""
+    from g4go_amd import abi, autotune, workloads
     from g4go_amd.transport import HipTransportHost, LaunchForm, PhotonTransportConfig
 
     cs = workloads.array_scene()
@@ -205,6 +206,13 @@ def run_array(seed, launch, steps, warmup, plain):
     host = HipTransportHost(PhotonTransportConfig(seed=seed, sort_hits=False))
     host.prepare_scene(cs.scene)
     host.set_parameter_layout([(abi.PARAM_ABS_LENGTH, m) for m in cs.crystal_materials])
+
+    def tune_batch():
+        host.enqueue_emissions(cs.scene, em, ev, idx)
+        out, _ = host.complete_device()
+        return out.statistics["kernel_ms"]
+
+    launch, _ = autotune.tuned_launch(host, autotune.device_key(host), "array-emission", tune_batch, retune=retune)
     graphed = LaunchForm(**{**launch.__dict__, "graph": True})
     b_wall, b_kms, b_out = run_emission_batches(host, cs.scene, em, ev, idx, graphed, steps, warmup)
     _, adj = host.adjoint(np.ones(abi.HIST_SLOTS))
@@ -213,7 +221,7 @@ def run_array(seed, launch, steps, warmup, plain):
            "steps_per_photon": b_out.statistics["total_steps"] / max(1, b_out.statistics["generated"]),
            "detected_fraction": b_out.statistics["detected"] / max(1, b_out.statistics["generated"]),
            "adjoint_kernel_ms": adj["kernel_ms"], "adjoint_over_forward_kernel": adj["kernel_ms"] / b_kms,
-           "replay_mismatches": adj["mismatched"]}
+           "replay_mismatches": adj["mismatched"], "launch": launch.__dict__}
     if plain:
         e_wall, e_kms, e_out = run_emission_batches(host, cs.scene, em, ev, idx,
                                                     LaunchForm(persistent=False, per_event=True), steps, warmup)
@@ -380,7 +388,6 @@ def main():
                                                   retune=args.retune)
         out_dir = args.tune_table_dir
         if table is not None and out_dir and os.path.isdir(out_dir):
-            shutil.copy(autotune.CACHE, os.path.join(out_dir, "autotune_cache.json"))
             with open(os.path.join(out_dir, "autotune_table.json"), "w") as f:
                 json.dump(table, f, indent=0)
     wall, kms, out = run_form(host, cs, src, args.photons, persistent, args.steps, args.warmup)
@@ -410,7 +417,8 @@ def main():
                        "batched_over_plain": e_wall / b_wall})
     host.close()
     adjoint = None if args.no_adjoint else run_adjoint(cs, src, args.seed, persistent, args.steps, args.warmup)
-    array = None if args.no_array else run_array(args.seed, persistent, args.steps, args.warmup, not args.no_plain)
+    array = None if args.no_array else run_array(args.seed, args.steps, args.warmup, not args.no_plain,
+                                                     retune=args.retune)
     c1 = None if args.no_c1 else run_c1(args.seed, persistent, args.steps, args.warmup, not args.no_cpu, 200)
     host_batches = None if args.no_c1 else run_host_batches(args.seed, persistent)
     mesh = None if args.no_mesh else run_mesh(args.seed, persistent, args.steps, args.warmup, args.photons,
@@ -468,6 +476,8 @@ def main():
     }
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(cs, src, args.seed, args.cpu_sample)
+    if args.tune_table_dir and os.path.isdir(args.tune_table_dir) and autotune.CACHE.is_file():
+        shutil.copy(autotune.CACHE, os.path.join(args.tune_table_dir, "autotune_cache.json"))
     print(json.dumps(result))
 
 
