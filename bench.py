@@ -401,9 +401,17 @@ def main():
     # uneven events: 4096 log-normal events per batch, batched (persistent + hipGraph) vs plain per event
     counts = workloads.lognormal_counts(4096, 5000, sigma=1.5, lo=10, hi=1_000_000, seed=args.seed)
     n_uneven = int(counts.astype(np.int64).sum())
-    graphed = LaunchForm(**{**persistent.__dict__, "graph": True})
     uneven = None
     if not args.headline_only:
+        ev_ids = np.arange(counts.size, dtype=np.uint32)
+
+        def tune_uneven():
+            host.enqueue_synthetic(cs.scene, src, ev_ids, counts)
+            o, _ = host.complete_device()
+            return o.statistics["kernel_ms"]
+
+        uneven_form, _ = autotune.tuned_launch(host, tune_key, "crystal-uneven", tune_uneven, retune=args.retune)
+        graphed = LaunchForm(**{**uneven_form.__dict__, "graph": True})
         b_wall, b_kms, b_out = run_uneven(host, cs, src, counts, graphed, args.steps, args.warmup)
         uneven = {"events": int(counts.size), "photons": n_uneven, "median": 5000, "sigma": 1.5,
                   "batched_photons_per_s": n_uneven * args.steps / b_wall, "batched_kernel_ms": b_kms,
