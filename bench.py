@@ -262,11 +262,12 @@ def run_mesh(seed, launch, steps, warmup, photons, box_ms_per_step):
     return res
 
 
-def run_c1(seed, launch, steps, warmup, cpu, cpu_events):
+def run_c1(seed, steps, warmup, cpu, cpu_events, retune=False):
     """5000 events of 1 MeV electrons in the crystal (scintillation records, 30
-    photons/keV): device photons/s, and the FP32 oracle on `cpu_events` events at
-    6 threads and at all (up to 16) host threads."""
-    from g4go_amd import workloads
+    photons/keV): device photons/s (launch form autotuned on this batch and
+    cached under its own tag), and the FP32 oracle on `cpu_events` events at 6
+    threads and at all (up to 16) host threads."""
+    from g4go_amd import autotune, workloads
     from g4go_amd.transport import HipTransportHost, PhotonTransportConfig
 
     cs = workloads.crystal_scene()
@@ -274,10 +275,18 @@ def run_c1(seed, launch, steps, warmup, cpu, cpu_events):
     photons = int(em["photon_count"].astype(np.int64).sum())
     host = HipTransportHost(PhotonTransportConfig(seed=seed, sort_hits=False))
     host.prepare_scene(cs.scene)
+
+    def tune_batch():
+        host.enqueue_emissions(cs.scene, em, ev, idx)
+        o, _ = host.complete_device()
+        return o.statistics["kernel_ms"]
+
+    launch, _ = autotune.tuned_launch(host, autotune.device_key(host), "crystal-electrons", tune_batch, retune=retune)
     wall, kms, out = run_emission_batches(host, cs.scene, em, ev, idx, launch, steps, warmup)
     host.close()
     res = {"events": 5000, "photons": photons, "photons_per_s": photons / wall, "kernel_ms": kms,
-           "h2d_ms": out.statistics["host_to_device_ms"], "record_bytes": int(em.nbytes)}
+           "h2d_ms": out.statistics["host_to_device_ms"], "record_bytes": int(em.nbytes),
+           "launch": launch.__dict__}
     if cpu:
         from oracle import oracle as O
 
@@ -427,7 +436,8 @@ def main():
     adjoint = None if args.no_adjoint else run_adjoint(cs, src, args.seed, persistent, args.steps, args.warmup)
     array = None if args.no_array else run_array(args.seed, args.steps, args.warmup, not args.no_plain,
                                                      retune=args.retune)
-    c1 = None if args.no_c1 else run_c1(args.seed, persistent, args.steps, args.warmup, not args.no_cpu, 200)
+    c1 = None if args.no_c1 else run_c1(args.seed, args.steps, args.warmup, not args.no_cpu, 200,
+                                               retune=args.retune)
     host_batches = None if args.no_c1 else run_host_batches(args.seed, persistent)
     mesh = None if args.no_mesh else run_mesh(args.seed, persistent, args.steps, args.warmup, args.photons,
                                               float(np.mean(kms)) / out.statistics["total_steps"])
