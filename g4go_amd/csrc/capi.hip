@@ -2116,7 +2116,7 @@ g4go_status g4go_create(const g4go_config* config, g4go_transport** out) {
         for (uint32_t k = 0; k < n; ++k) {
             auto c = std::make_unique<Core>();
             c->cfg = *config;
-            c->launch = g4go_launch{0, 256, 64, 64, 8, 0, 0};
+            c->launch = g4go_launch{1, 256, 256, 24, 8, 0, 0};  // persistent, tuned on crystal batches
             t->cores.push_back(std::move(c));
         }
         *out = t.release();

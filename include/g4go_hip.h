@@ -257,6 +257,9 @@ void g4go_destroy(g4go_transport* handle);
 /* Message of the last failure on this handle (or of the last g4go_create
  * failure when handle is NULL). */
 const char* g4go_last_error(const g4go_transport* handle);
+/* Launch form of the handle's batches.  A new handle uses the persistent form
+ * {persistent 1, block 256, chunk 256, refill 24, 8 blocks per CU, per_event
+ * 0, graph 0}, the form tuned on MI355X for crystal batches. */
 g4go_status g4go_set_launch(g4go_transport* handle, const g4go_launch* launch);
 g4go_status g4go_get_launch(const g4go_transport* handle, g4go_launch* launch);
 
