@@ -232,9 +232,11 @@ def run_array(seed, steps, warmup, plain, retune=False):
     return res
 
 
-def run_mesh(seed, launch, steps, warmup, photons, box_ms_per_step):
-    """1e8 photons in the tapered crystal, the rotated crystal and the L-shaped light guide (mesh solids)."""
-    from g4go_amd import workloads
+def run_mesh(seed, steps, warmup, photons, box_ms_per_step, retune=False):
+    """1e8 photons in the tapered crystal, the rotated crystal, the ring of 48
+    turned crystals and the L-shaped light guide (mesh solids), each in the
+    launch form autotuned on a 2e7-photon batch of it (cached per workload)."""
+    from g4go_amd import autotune, workloads
     from g4go_amd.transport import HipTransportHost, PhotonTransportConfig
 
     rot = workloads.rotation_matrix((1.0, 2.0, 3.0), 0.7)
@@ -250,6 +252,14 @@ def run_mesh(seed, launch, steps, warmup, photons, box_ms_per_step):
     res = {}
     for name, (cs, src) in cases.items():
         host = HipTransportHost(PhotonTransportConfig(seed=seed, sort_hits=False))
+        one = np.array([0], np.uint32)
+
+        def tune_batch():
+            host.enqueue_synthetic(cs.scene, src, one, np.array([20_000_000], np.uint32))
+            o, _ = host.complete_device()
+            return o.statistics["kernel_ms"]
+
+        launch, _ = autotune.tuned_launch(host, autotune.device_key(host), "mesh-" + name, tune_batch, retune=retune)
         wall, kms, out = run_form(host, cs, src, photons, launch, steps, warmup)
         host.close()
         st = out.statistics
@@ -258,7 +268,8 @@ def run_mesh(seed, launch, steps, warmup, photons, box_ms_per_step):
                      "steps_per_photon": st["total_steps"] / max(1, st["generated"]),
                      "detected_fraction": st["detected"] / max(1, st["generated"]),
                      "invalid": st["invalid"],
-                     "step_cost_over_box": (k_ms / st["total_steps"]) / box_ms_per_step}
+                     "step_cost_over_box": (k_ms / st["total_steps"]) / box_ms_per_step,
+                     "launch": launch.__dict__}
     return res
 
 
@@ -439,8 +450,9 @@ def main():
     c1 = None if args.no_c1 else run_c1(args.seed, args.steps, args.warmup, not args.no_cpu, 200,
                                                retune=args.retune)
     host_batches = None if args.no_c1 else run_host_batches(args.seed, persistent)
-    mesh = None if args.no_mesh else run_mesh(args.seed, persistent, args.steps, args.warmup, args.photons,
-                                              float(np.mean(kms)) / out.statistics["total_steps"])
+    mesh = None if args.no_mesh else run_mesh(args.seed, args.steps, args.warmup, args.photons,
+                                              float(np.mean(kms)) / out.statistics["total_steps"],
+                                              retune=args.retune)
 
     st = out.statistics
     P, S, D = st["generated"], st["total_steps"], st["n_hits"]
