@@ -1592,15 +1592,17 @@ static g4go_status core_enqueue_emissions(Core* h, const g4go_emission* emission
         if (photons > 0xffffffffull) {
             fail(G4GO_ERROR_OVERFLOW, "emission batch exceeds the supported photon count");
         }
-        std::vector<Emission> staged(n_emissions);
-        std::vector<uint32_t> counts(n_emissions), source_event(n_emissions);
-        for (uint64_t i = 0; i < n_emissions; ++i) {
+        std::vector<Emission> staged(photons != 0 ? n_emissions : 0);
+        std::vector<uint32_t> counts(staged.size()), source_event(staged.size());
+        // a batch without photons is not validated further (OptiXTransportHost.cpp:609-627); otherwise
+        // every record's volume and material must exist, whatever its type (:650-660)
+        for (uint64_t i = 0; i < staged.size(); ++i) {
             std::memcpy(&staged[i], &emissions[i], sizeof(Emission));
             const auto it = h->scene.index_of.find(emissions[i].volume_id);
             if (it == h->scene.index_of.end()) {
                 fail(G4GO_ERROR_INVALID_ARGUMENT, "emission references an unknown scene volume ID");
             }
-            if (emissions[i].type != 0 && emissions[i].material_id >= h->scene.materials.size()) {
+            if (emissions[i].material_id >= h->scene.materials.size()) {
                 fail(G4GO_ERROR_INVALID_ARGUMENT, "emission references an unknown scene material ID");
             }
             staged[i].volume = it->second;

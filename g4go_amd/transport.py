@@ -78,10 +78,16 @@ class PhotonTransportConfig:
 
 @dataclass
 class LaunchForm:
-    persistent: bool = False
+    """Launch form of a handle (g4go_launch).  The defaults are those of a new
+    handle: the persistent kernel with 256-thread blocks, 256-photon work
+    items, refill at 24 idle lanes and 8 blocks per CU.  The plain form
+    (`LaunchForm(persistent=False)`) runs one wave per item of at most 64
+    photons and ignores chunk and refill_threshold."""
+
+    persistent: bool = True
     block_size: int = 256
-    chunk: int = 64
-    refill_threshold: int = 64
+    chunk: int = 256
+    refill_threshold: int = 24
     blocks_per_cu: int = 8
     per_event: bool = False
     graph: bool = False
@@ -112,6 +118,7 @@ class HipTransportHost:
         if status != abi.G4GO_OK:
             raise _ERRORS.get(status, RuntimeError)(self._lib.g4go_last_error(None).decode())
         self._h = handle
+        self.batches_completed = 0  # batches completed so far (a serial for callers such as autograd)
         self._scene: Optional[Scene] = None
         self._scene_keep: list = []
         self._n_params = 0
@@ -221,6 +228,7 @@ class HipTransportHost:
     def complete_oldest_batch(self) -> PhotonTransportOutput:
         res = abi.BatchResult()
         self._check(self._lib.g4go_complete(self._h, C.byref(res)))
+        self.batches_completed += 1
         return self._output(res, copy_hits=True)
 
     def complete_device(self) -> tuple[PhotonTransportOutput, int]:
@@ -228,6 +236,7 @@ class HipTransportHost:
         res = abi.BatchResult()
         ptr = C.c_void_p()
         self._check(self._lib.g4go_complete_device(self._h, C.byref(res), C.byref(ptr)))
+        self.batches_completed += 1
         return self._output(res, copy_hits=False), int(ptr.value or 0)
 
     def _output(self, res: abi.BatchResult, copy_hits: bool) -> PhotonTransportOutput:

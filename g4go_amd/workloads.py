@@ -133,6 +133,44 @@ def crystal_scene(p: CrystalParams | None = None, abs_nodes=None, pde_nodes=None
     return CrystalScene(scene, world, [crystal], [sipm], [crystal_mat], wrap, cathode, p)
 
 
+GROUND_FINISHES = ("glisur_metal", "unified_metal", "glisur_dielectric", "unified_dielectric")
+
+
+def ground_wrap(kind: str) -> Surface:
+    """The crystal's border surface with a ground finish (OptiXTransportDevice.cu:492-595, 843-876):
+    * glisur_metal: dielectric_metal reflector, R = 0.95, glisur polish 0.7 (smeared facets);
+    * unified_metal: dielectric_metal reflector, R = 0.95, unified sigma_alpha 0.2 rad with
+      spike 0.2 / lobe 0.5 / backscatter 0.1 (Lambertian 0.2);
+    * glisur_dielectric: bare ground crystal/air interface, glisur polish 0.8, surface roughness
+      20 nm (Lambertian reflections from the dense side);
+    * unified_dielectric: bare ground interface, unified sigma_alpha 0.15 rad, spike 0.3 / lobe 0.5 /
+      backscatter 0.05, roughness 20 nm.
+    """
+    c = PropertyTable.constant_value
+    if kind == "glisur_metal":
+        return Surface(name="GroundGlisur", type=SurfaceType.DielectricMetal, model=SurfaceModel.Glisur,
+                       finish=SurfaceFinish.Ground, model_value=0.7, reflectivity=c(0.95))
+    if kind == "unified_metal":
+        return Surface(name="GroundUnified", type=SurfaceType.DielectricMetal, model=SurfaceModel.Unified,
+                       finish=SurfaceFinish.Ground, model_value=0.2, reflectivity=c(0.95),
+                       specular_spike=c(0.2), specular_lobe=c(0.5), backscatter=c(0.1))
+    if kind == "glisur_dielectric":
+        return Surface(name="GroundGlisurBare", type=SurfaceType.DielectricDielectric, model=SurfaceModel.Glisur,
+                       finish=SurfaceFinish.Ground, model_value=0.8, surface_roughness=c(2.0e-5))
+    if kind == "unified_dielectric":
+        return Surface(name="GroundUnifiedBare", type=SurfaceType.DielectricDielectric, model=SurfaceModel.Unified,
+                       finish=SurfaceFinish.Ground, model_value=0.15, specular_spike=c(0.3),
+                       specular_lobe=c(0.5), backscatter=c(0.05), surface_roughness=c(2.0e-5))
+    raise ValueError(f"unknown ground finish {kind!r}")
+
+
+def ground_crystal_scene(kind: str, p: CrystalParams | None = None) -> CrystalScene:
+    """crystal_scene with its polished wrap replaced by the ground border surface ground_wrap(kind)."""
+    cs = crystal_scene(p)
+    cs.scene.surfaces[cs.wrap_surface] = ground_wrap(kind)
+    return cs
+
+
 def array_scene(p: CrystalParams | None = None, n: int = 8, pitch_mm: float = 3.2, abs_nodes=None,
                 pde_nodes=None, septa: bool = True) -> CrystalScene:
     """n x n crystals with reflective septa, one SiPM per crystal (2 n^2 boxes).

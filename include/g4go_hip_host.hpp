@@ -570,12 +570,16 @@ public:
     // CompleteOldestBatch (OptiXTransportHost.cpp:779-883).
     Output CompleteOldestBatch() {
         g4go_batch_result r{};
-        check(g4go_complete(handle_, &r), handle_);
+        const g4go_status status = g4go_complete(handle_, &r);
+        // g4go_complete retires the oldest batch even when it fails (only an
+        // empty queue, where nothing is pending here either, leaves it), so
+        // its bookkeeping goes before the status can throw
         Pending batch;
         if (!pending_.empty()) {
             batch = std::move(pending_.front());
             pending_.pop_front();
         }
+        check(status, handle_);
         Output out{};
         const std::uint32_t counts = bit(Field::Detected) | bit(Field::Absorbed) | bit(Field::Escaped) |
                                      bit(Field::Truncated) | bit(Field::MaxBounce) | bit(Field::InvalidState) |

@@ -1285,7 +1285,8 @@ static v3 or_facet(const surface_t* sf, v3 n, v3 incoming, const rng_t* r, const
     return n;
 }
 
-static v3 or_unified(const surface_t* sf, v3 d, v3 n, const rng_t* r, const photon_t* ph) {
+/* branch (when not NULL): 0 spike, 1 lobe, 2 backscatter, 3 Lambertian. */
+static v3 or_unified(const surface_t* sf, v3 d, v3 n, const rng_t* r, const photon_t* ph, int* branch) {
     const int ground = sf->finish == 1;
     const v3 facet = or_facet(sf, n, d, r, ph);
     const real spike = clamp01(or_prop(&sf->spike, ph->energy, ground ? RF(0.0) : RF(1.0)));
@@ -1294,26 +1295,30 @@ static v3 or_unified(const surface_t* sf, v3 d, v3 n, const rng_t* r, const phot
     uint32_t w[4];
     block4(r, ph->photon, ph->event, ph->step, P_LOBE, 0, w);
     const real u = u01(w[0]);
-    if (u < spike) {
+    const int b = u < spike ? 0 : (u < spike + lobe ? 1 : (u < spike + lobe + back ? 2 : 3));
+    if (branch != NULL) {
+        *branch = b;
+    }
+    if (b == 0) {
         return or_reflect_dir(d, n);
     }
-    if (u < spike + lobe) {
+    if (b == 1) {
         return or_reflect_dir(d, facet);
     }
-    if (u < spike + lobe + back) {
+    if (b == 2) {
         return vscale(d, RF(-1.0));
     }
     return or_lambertian(n, u01(w[1]), u01(w[2]));
 }
 
-static v3 or_surface_reflection(const surface_t* sf, v3 d, v3 n, const rng_t* r, const photon_t* ph) {
+static v3 or_surface_reflection(const surface_t* sf, v3 d, v3 n, const rng_t* r, const photon_t* ph, int* branch) {
     if (sf->finish == 0) {
         return or_reflect_dir(d, n);
     }
     if (sf->model == 0) {
         return or_reflect_dir(d, or_facet(sf, n, d, r, ph));
     }
-    return or_unified(sf, d, n, r, ph);
+    return or_unified(sf, d, n, r, ph, branch);
 }
 
 /* Dipole scattering about the old polarisation p (Geant4 G4OpRayleigh's
@@ -1676,7 +1681,7 @@ static int or_step(photon_t* ph, const oscene_t* s, const rng_t* r, real eps, in
                 if (reflected) {
                     if (rough_pass) {
                         if (sf->model == 1 && sf->finish != 0) {
-                            dir = or_unified(sf, dir, normal, r, ph);
+                            dir = or_unified(sf, dir, normal, r, ph, NULL);
                             pol = or_reflect_pol(pol, normal, dir);
                         } else {
                             dir = f.rdir;
@@ -1704,7 +1709,7 @@ static int or_step(photon_t* ph, const oscene_t* s, const rng_t* r, real eps, in
                 dir = mk(axis == 0 ? -dir.x : dir.x, axis == 1 ? -dir.y : dir.y, axis == 2 ? -dir.z : dir.z);
                 pol = mk(axis == 0 ? pol.x : -pol.x, axis == 1 ? pol.y : -pol.y, axis == 2 ? pol.z : -pol.z);
             } else {
-                dir = or_surface_reflection(sf, dir, normal, r, ph);
+                dir = or_surface_reflection(sf, dir, normal, r, ph, NULL);
                 pol = or_reflect_pol(pol, normal, dir);
             }
         }
@@ -2316,4 +2321,120 @@ int ORACLE_NAME(adjoint_emissions)(const g4go_scene* scene, const g4go_emission*
     free(keys);
     destroy(s);
     return rc;
+}
+
+/* ------------------------------------------------------------------ */
+/* Exports for the distribution tests of generation and surfaces      */
+/* ------------------------------------------------------------------ */
+/* Photon state as 12 doubles: pos[3], dir[3], pol[3], energy, time, valid
+ * (valid = 0 when start_photon rejects the photon, OptiXTransportDevice.cu:693-702). */
+static void put_state(photon_t* ph, double* o) {
+    const int ok = start_photon(ph);
+    o[0] = ph->pos.x; o[1] = ph->pos.y; o[2] = ph->pos.z;
+    o[3] = ph->dir.x; o[4] = ph->dir.y; o[5] = ph->dir.z;
+    o[6] = ph->pol.x; o[7] = ph->pol.y; o[8] = ph->pol.z;
+    o[9] = ph->energy;
+    o[10] = ph->time;
+    o[11] = ok;
+}
+
+/* Photons first .. first + n - 1 of one emission record (GeneratePhoton,
+ * OptiXTransportDevice.cu:346-481), as generated before transport. */
+void ORACLE_NAME(generate_emission)(const g4go_scene* scene, const g4go_emission* em, uint64_t seed, uint32_t first,
+                                    uint32_t n, double* out) {
+    oscene_t* s = build(scene);
+    const rng_t r = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    const uint32_t volume = vol_index(s, em->volume_id);
+    for (uint32_t k = 0; k < n; ++k) {
+        photon_t ph;
+        gen_emission(&ph, em, volume, s, &r, first + k);
+        put_state(&ph, out + 12 * (uint64_t)k);
+    }
+    destroy(s);
+}
+
+/* Photons first .. first + n - 1 of one synthetic event. */
+void ORACLE_NAME(generate_synthetic)(const g4go_scene* scene, const g4go_synthetic_source* src, uint64_t seed,
+                                     uint32_t event, uint32_t first, uint32_t n, double* out) {
+    oscene_t* s = build(scene);
+    const rng_t r = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    const uint32_t volume = vol_index(s, src->volume_id);
+    for (uint32_t k = 0; k < n; ++k) {
+        photon_t ph;
+        gen_synthetic(&ph, src, volume, &r, event, first + k);
+        put_state(&ph, out + 12 * (uint64_t)k);
+    }
+    destroy(s);
+}
+
+/* Reflection off surface `surface` (its index in the scene's surface list)
+ * for photons photon_first .. + n - 1 of `event` at step `step`, arriving
+ * along d at a face whose normal nrm faces the photon
+ * (SampleSurfaceReflection, OptiXTransportDevice.cu:582-595): out_dir the
+ * reflected direction, out_facet the facet normal SampleFacetNormal draws
+ * (:492-556), branch the unified lobe (0 spike, 1 lobe, 2 backscatter, 3
+ * Lambertian; -1 when no lobe is chosen). */
+void ORACLE_NAME(surface_reflect)(const g4go_scene* scene, uint32_t surface, const real d[3], const real nrm[3],
+                                  real energy, uint64_t seed, uint32_t event, uint32_t step, uint32_t photon_first,
+                                  uint32_t n, real* out_dir, real* out_facet, int* branch) {
+    oscene_t* s = build(scene);
+    const rng_t r = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    const surface_t* sf = &s->surf[surface];
+    const v3 dd = mk(d[0], d[1], d[2]);
+    const v3 nn = mk(nrm[0], nrm[1], nrm[2]);
+    for (uint32_t k = 0; k < n; ++k) {
+        photon_t ph;
+        memset(&ph, 0, sizeof(ph));
+        ph.photon = photon_first + k;
+        ph.event = event;
+        ph.step = step;
+        ph.energy = energy;
+        int b = -1;
+        const v3 o = or_surface_reflection(sf, dd, nn, &r, &ph, &b);
+        const v3 f = or_facet(sf, nn, dd, &r, &ph);
+        out_dir[3 * k] = o.x; out_dir[3 * k + 1] = o.y; out_dir[3 * k + 2] = o.z;
+        out_facet[3 * k] = f.x; out_facet[3 * k + 1] = f.y; out_facet[3 * k + 2] = f.z;
+        branch[k] = b;
+    }
+    destroy(s);
+}
+
+/* One transport step (the bounce loop body, OptiXTransportDevice.cu:708-931)
+ * from a given state, for photons photon_first .. + n - 1 of `event` at step
+ * `step`, all starting from state[12] (pos, dir, pol, energy, time, -) in
+ * volume `volume_id`.  out: the state after the step (12 doubles; [11] = the
+ * volume index after the step), status: S_* (0 alive, 1 detected, 2 bulk
+ * absorbed, 3 surface absorbed, 4 escaped, 5 invalid). */
+void ORACLE_NAME(step_from)(const g4go_scene* scene, uint32_t volume_id, const double* state, uint64_t seed,
+                            uint32_t event, uint32_t step, uint32_t photon_first, uint32_t n, float eps, double* out,
+                            int* status) {
+    oscene_t* s = build(scene);
+    const rng_t r = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    const uint32_t volume = vol_index(s, volume_id);
+    for (uint32_t k = 0; k < n; ++k) {
+        photon_t ph;
+        memset(&ph, 0, sizeof(ph));
+        ph.pos = mk((real)state[0], (real)state[1], (real)state[2]);
+        ph.dir = mk((real)state[3], (real)state[4], (real)state[5]);
+        ph.pol = mk((real)state[6], (real)state[7], (real)state[8]);
+        ph.energy = (real)state[9];
+        ph.time = state[10];
+        ph.volume = volume;
+        ph.material = INVALID;
+        ph.photon = photon_first + k;
+        ph.event = event;
+        ph.step = step;
+        int zero = 0;
+        uint32_t sensor = INVALID;
+        trace_t tr;
+        status[k] = or_step(&ph, s, &r, (real)eps, &zero, &sensor, &tr);
+        double* o = out + 12 * (uint64_t)k;
+        o[0] = ph.pos.x; o[1] = ph.pos.y; o[2] = ph.pos.z;
+        o[3] = ph.dir.x; o[4] = ph.dir.y; o[5] = ph.dir.z;
+        o[6] = ph.pol.x; o[7] = ph.pol.y; o[8] = ph.pol.z;
+        o[9] = ph.energy;
+        o[10] = ph.time;
+        o[11] = ph.volume;
+    }
+    destroy(s);
 }

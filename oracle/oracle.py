@@ -68,6 +68,15 @@ def _load(precision: int) -> C.CDLL:
     getattr(lib, p + "mesh_planes").restype = C.c_uint32
     getattr(lib, p + "solid_kind").argtypes = [C.POINTER(abi.Scene), C.c_uint32]
     getattr(lib, p + "solid_contains").argtypes = [C.POINTER(abi.Scene), C.c_uint32, C.c_void_p, C.c_uint64, C.c_void_p]
+    getattr(lib, p + "generate_emission").argtypes = [C.POINTER(abi.Scene), C.c_void_p, C.c_uint64, C.c_uint32,
+                                                      C.c_uint32, C.c_void_p]
+    getattr(lib, p + "generate_synthetic").argtypes = [C.POINTER(abi.Scene), C.POINTER(abi.SyntheticSource),
+                                                       C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p]
+    getattr(lib, p + "surface_reflect").argtypes = [C.POINTER(abi.Scene), C.c_uint32, C.c_void_p, C.c_void_p, real,
+                                                    C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                                    C.c_void_p, C.c_void_p, C.c_void_p]
+    getattr(lib, p + "step_from").argtypes = [C.POINTER(abi.Scene), C.c_uint32, C.c_void_p, C.c_uint64, C.c_uint32,
+                                              C.c_uint32, C.c_uint32, C.c_uint32, C.c_float, C.c_void_p, C.c_void_p]
     for name in ("project_polarization", "reflect_direction"):
         getattr(lib, p + name).argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
     getattr(lib, p + "normalize").argtypes = [C.c_void_p, C.c_void_p]
@@ -213,6 +222,58 @@ def find_surface(scene, from_id: int, to_id: int) -> int:
     r = _fn(32, "find_surface")(C.byref(desc), from_id, to_id)
     del keep
     return r
+
+
+# ---- generation, surfaces and single steps (distribution tests) ----
+STATE_FIELDS = ("x", "y", "z", "dx", "dy", "dz", "px", "py", "pz", "energy", "time", "valid")
+
+
+def generate_emission(scene, record: np.ndarray, seed: int, n: int, first: int = 0, precision: int = 32) -> np.ndarray:
+    """Photons first .. first + n - 1 of one emission record, as (n, 12) rows of STATE_FIELDS."""
+    desc, keep = scene.to_abi()
+    rec = np.ascontiguousarray(np.asarray(record, dtype=abi.EMISSION_DTYPE).reshape(1))
+    out = np.zeros((n, 12), np.float64)
+    _fn(precision, "generate_emission")(C.byref(desc), rec.ctypes.data, seed, first, n, out.ctypes.data)
+    return out
+
+
+def generate_synthetic(scene, source: abi.SyntheticSource, seed: int, event: int, n: int, first: int = 0,
+                       precision: int = 32) -> np.ndarray:
+    """Photons first .. first + n - 1 of one synthetic event, as (n, 12) rows of STATE_FIELDS."""
+    desc, keep = scene.to_abi()
+    out = np.zeros((n, 12), np.float64)
+    _fn(precision, "generate_synthetic")(C.byref(desc), C.byref(source), seed, event, first, n, out.ctypes.data)
+    return out
+
+
+def surface_reflect(scene, surface: int, direction, normal, energy_ev: float, seed: int, n: int, event: int = 0,
+                    step: int = 0, first: int = 0, precision: int = 32):
+    """Reflections off surface index `surface` of n photons arriving along `direction` at a face with
+    `normal` (facing the photon): (directions (n, 3), facet normals (n, 3), unified branch (n,))."""
+    real = _real(precision)
+    desc, keep = scene.to_abi()
+    d = np.ascontiguousarray(direction, dtype=real)
+    nn = np.ascontiguousarray(normal, dtype=real)
+    out_d = np.zeros((n, 3), real)
+    out_f = np.zeros((n, 3), real)
+    branch = np.zeros(n, np.int32)
+    _fn(precision, "surface_reflect")(C.byref(desc), surface, d.ctypes.data, nn.ctypes.data, energy_ev, seed, event,
+                                      step, first, n, out_d.ctypes.data, out_f.ctypes.data, branch.ctypes.data)
+    return out_d, out_f, branch
+
+
+def step_from(scene, volume_id: int, state, seed: int, n: int, event: int = 0, step: int = 0, first: int = 0,
+              eps: float = 1e-4, precision: int = 32):
+    """One transport step of photons first .. + n - 1 from one state (pos, dir, pol, energy, time): the
+    states after the step ((n, 12); column 11 = volume index) and the step statuses."""
+    desc, keep = scene.to_abi()
+    st = np.zeros(12, np.float64)
+    st[:11] = np.asarray(state, dtype=np.float64)[:11]
+    out = np.zeros((n, 12), np.float64)
+    status = np.zeros(n, np.int32)
+    _fn(precision, "step_from")(C.byref(desc), volume_id, st.ctypes.data, seed, event, step, first, n, eps,
+                                out.ctypes.data, status.ctypes.data)
+    return out, status
 
 
 # ---- whole-path runs ----

@@ -294,6 +294,35 @@ int run_cpu() {
         std::vector<std::uint32_t> no_idx;
         expect_throws<std::invalid_argument>(
             [&] { host.EnqueueEmissions(c.scene, PhotonTransportBatch{em, ev, no_idx}); }, "index count mismatch");
+        // more than 2^32 - 1 photons in a batch: overflow_error (OptiXTransportHost.cpp:621-623),
+        // raised by the host-side validation before anything reaches the device
+        {
+            std::vector<OpticalEmission> big(2);
+            for (auto& r : big) {
+                r.eventID = 6;
+                r.volumeID = c.crystal;
+                r.materialID = 1;
+                r.type = OpticalEmissionType::Direct;
+                r.photonCount = 0x80000000u;
+            }
+            std::vector<std::uint32_t> big_ev = {6}, big_idx = {0, 0};
+            expect_throws<std::overflow_error>(
+                [&] { host.EnqueueEmissions(c.scene, PhotonTransportBatch{big, big_ev, big_idx}); },
+                "photon count beyond 2^32 - 1");
+            // an unknown material is rejected on every record type, Direct included (:658-660)
+            big.resize(1);
+            big[0].photonCount = 10;
+            big[0].materialID = 99;
+            big_idx.resize(1);
+            expect_throws<std::invalid_argument>(
+                [&] { host.EnqueueEmissions(c.scene, PhotonTransportBatch{big, big_ev, big_idx}); },
+                "direct record with an unknown material");
+            big[0].materialID = 1;
+            big[0].volumeID = 999;
+            expect_throws<std::invalid_argument>(
+                [&] { host.EnqueueEmissions(c.scene, PhotonTransportBatch{big, big_ev, big_idx}); },
+                "record with an unknown volume");
+        }
         // empty batch (zero photons) completes without touching the device
         em[0].eventID = 6;
         host.EnqueueEmissions(c.scene, PhotonTransportBatch{em, ev, idx});

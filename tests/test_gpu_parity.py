@@ -104,8 +104,12 @@ def test_synthetic_crystal_bit_exact(crystal, form):
     counts = np.array([40000, 1000, 7], np.uint32)
     ref = O.run_synthetic(crystal.scene, src, 42, event_ids, counts, threads=4)
     with host(42) as h:
+        assert h.launch() == LaunchForm()  # a new handle's form
         if form == "persistent":
             h.set_launch(LaunchForm(persistent=True, block_size=256, chunk=256, refill_threshold=16, blocks_per_cu=2))
+        else:
+            h.set_launch(LaunchForm(persistent=False))
+        assert h.launch().persistent == (form == "persistent")
         h.enqueue_synthetic(crystal.scene, src, event_ids, counts)
         out = h.complete_oldest_batch()
     assert out.statistics["dropped_hits"] == 0

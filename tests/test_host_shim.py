@@ -63,6 +63,23 @@ def test_shim_host_checks():
     assert proc.returncode == 0 and "PASS" in proc.stdout, proc.stdout + proc.stderr
 
 
+REFERENCE_INCLUDE = Path(__import__("os").environ.get("G4GO_REFERENCE_ROOT", "/root/reference")) / "include"
+
+
+@pytest.mark.skipif(not (REFERENCE_INCLUDE / "g4go" / "optical" / "Scene.hpp").is_file(),
+                    reason="the reference tree is not present")
+def test_shim_compiles_against_reference_types():
+    """BasicHipTransportHost<HipTypes>, the bundle INTEGRATION.md has a maintainer add, type-checks
+    (-fsyntax-only, every member function instantiated) against the reference's own
+    PhotonTransport.hpp, PhotonTransportConfig.hpp and Scene.hpp; tests/cpp/refcheck holds a
+    declaration-only G4ThreeVector, the one header the image lacks."""
+    proc = subprocess.run(["g++", "-std=c++20", "-fsyntax-only", "-Wall", "-Wextra", "-Werror",
+                           f"-I{ROOT / 'tests' / 'cpp' / 'refcheck'}", f"-I{REFERENCE_INCLUDE}",
+                           f"-I{ROOT / 'include'}", str(ROOT / "tests" / "cpp" / "reference_types_check.cpp")],
+                          capture_output=True, text=True, timeout=120)
+    assert proc.returncode == 0, proc.stderr[-4000:]
+
+
 def shim_crystal_scene(tapered_mesh=None):
     """The box-scene equivalent of the shim test's mesh scene (same IDs and
     tables); with tapered_mesh (world-space vertices, indices) the crystal is

@@ -1,7 +1,7 @@
 """Batch slots: up to max_in_flight batches enqueued before the oldest is
 completed, completed in enqueue order (CPU checks use photon-less batches, which
-never reach the device; the device check compares pipelined with one-at-a-time
-batches bit for bit)."""
+never reach the device; the device check compares every pipelined batch with
+the oracle's run of that batch, bit for bit)."""
 import numpy as np
 import pytest
 
@@ -48,32 +48,51 @@ def test_in_flight_limit():
 
 
 @pytest.mark.gpu
-def test_pipelined_batches_equal_sequential(crystal):
-    """Three slots with up to three batches in flight give the same hits and
-    counters as one slot, batch by batch."""
-    print("case: pipelined emission batches (3 slots) vs one slot", flush=True)
-    em, ev, idx = workloads.electron_emissions(crystal, 12, seed=6)
-    batches = []
-    for b in range(4):
-        sel = (idx >= 3 * b) & (idx < 3 * b + 3)
-        batches.append((em[sel], ev[3 * b:3 * b + 3], (idx[sel] - 3 * b).astype(np.uint32)))
+@pytest.mark.parametrize("slots", [2, 3, 4])
+def test_pipelined_batches_match_oracle(crystal, slots):
+    """Up to `slots` batches of different sizes in flight (scintillation
+    records, and one synthetic batch between them): every batch, completed in
+    enqueue order, equals the FP32 oracle's run of that batch alone, bit for
+    bit (hits in launch order, per-event counters, histogram)."""
+    from oracle import oracle as O
+    from tests.test_gpu_parity import assert_same_hits, assert_same_stats
+
+    print(f"case: pipelined batches, {slots} slots, vs the oracle", flush=True)
+    em, ev, idx = workloads.electron_emissions(crystal, 16, seed=6)
+    src = workloads.synthetic_source(crystal)
+    batches, refs = [], []
+    first = 0
+    for n in (1, 4, 2, None, 5, 1, 3):
+        if n is None:  # a synthetic batch between the record batches
+            sev, scounts = np.array([90, 91], np.uint32), np.array([20000, 700], np.uint32)
+            batches.append(("synthetic", (crystal.scene, src, sev, scounts)))
+            refs.append(O.run_synthetic(crystal.scene, src, 21, sev, scounts, threads=4))
+            continue
+        sel = (idx >= first) & (idx < first + n)
+        b = (em[sel], ev[first:first + n], (idx[sel] - first).astype(np.uint32))
+        batches.append(("records", (crystal.scene, *b)))
+        refs.append(O.run_emissions(crystal.scene, *b, 21, threads=4))
+        first += n
     form = LaunchForm(persistent=True, block_size=256, chunk=256, refill_threshold=16, blocks_per_cu=4, graph=True)
-    seq = []
-    with HipTransportHost(PhotonTransportConfig(seed=21)) as h:
-        h.set_launch(form)
-        for b in batches:
-            h.enqueue_emissions(crystal.scene, *b)
-            seq.append(h.complete_oldest_batch())
-    pipe = []
-    with HipTransportHost(PhotonTransportConfig(seed=21, max_in_flight=3)) as h:
+    outs = []
+    with HipTransportHost(PhotonTransportConfig(seed=21, max_in_flight=slots)) as h:
         h.set_launch(form)
         h.prepare_scene(crystal.scene)
-        for b in batches[:3]:
-            h.enqueue_emissions(crystal.scene, *b)
-        pipe.append(h.complete_oldest_batch())
-        h.enqueue_emissions(crystal.scene, *batches[3])
-        pipe += [h.complete_oldest_batch() for _ in range(3)]
-    for a, b in zip(seq, pipe):
-        assert a.detections.tobytes() == b.detections.tobytes()
-        assert a.event_statistics.tobytes() == b.event_statistics.tobytes()
-        np.testing.assert_array_equal(a.histogram, b.histogram)
+        pending = 0
+        for kind, args in batches:
+            if pending == slots:
+                outs.append(h.complete_oldest_batch())
+                pending -= 1
+            if kind == "records":
+                h.enqueue_emissions(*args)
+            else:
+                h.enqueue_synthetic(*args)
+            pending += 1
+        while pending:
+            outs.append(h.complete_oldest_batch())
+            pending -= 1
+    assert len(outs) == len(refs)
+    for k, (out, ref) in enumerate(zip(outs, refs)):
+        print(f"  batch {k}: {len(out.detections)} hits", flush=True)
+        assert_same_hits(out.detections, ref.hits)
+        assert_same_stats(out, ref)
