@@ -234,6 +234,22 @@ class BatchResult(C.Structure):
     ]
 
 
+NOALLOC_MAX_EVENTS = 1024
+NOALLOC_SCRATCH_BYTES = 1844224
+
+
+class DeviceOutputs(C.Structure):
+    """g4go_device_outputs: device pointers of an allocation-free batch's outputs."""
+    _fields_ = [
+        ("hits", C.c_void_p),
+        ("hit_capacity", C.c_uint64),
+        ("hit_count", C.c_void_p),
+        ("histogram", C.c_void_p),
+        ("event_counters", C.c_void_p),
+        ("batch_counters", C.c_void_p),
+    ]
+
+
 HIT_DTYPE = np.dtype([
     ("position_mm", "<f4", (3,)),
     ("direction", "<f4", (3,)),
@@ -268,6 +284,7 @@ EXPORTED_SYMBOLS = (
     "g4go_prepare_scene", "g4go_enqueue_emissions", "g4go_enqueue_synthetic", "g4go_complete",
     "g4go_complete_device", "g4go_eval_math", "g4go_eval_fresnel", "g4go_device_info", "g4go_trace_synthetic", "g4go_set_parameter_layout", "g4go_get_parameters",
     "g4go_set_parameters", "g4go_adjoint", "g4go_version", "g4go_set_stream", "g4go_enqueue_emissions_device",
+    "g4go_transport_synthetic_noalloc",
 )
 
 _lib = None
@@ -305,11 +322,13 @@ def load() -> C.CDLL:
     lib.g4go_get_parameters.argtypes = [H, C.c_void_p, C.c_uint64]
     lib.g4go_set_parameters.argtypes = [H, C.c_void_p, C.c_uint64]
     lib.g4go_adjoint.argtypes = [H, C.c_void_p, C.c_void_p, C.c_uint64, C.POINTER(AdjointStats)]
+    lib.g4go_transport_synthetic_noalloc.argtypes = [H, C.POINTER(SyntheticSource), C.c_void_p, C.c_void_p,
+                                                     C.c_uint32, C.POINTER(DeviceOutputs), C.c_void_p]
     lib.g4go_version.restype = C.c_char_p
     for name in ("g4go_create", "g4go_set_launch", "g4go_get_launch", "g4go_prepare_scene", "g4go_enqueue_emissions",
                  "g4go_enqueue_synthetic", "g4go_complete", "g4go_complete_device", "g4go_eval_math", "g4go_eval_fresnel", "g4go_device_info", "g4go_trace_synthetic",
                  "g4go_set_parameter_layout", "g4go_get_parameters", "g4go_set_parameters", "g4go_adjoint",
-                 "g4go_set_stream", "g4go_enqueue_emissions_device"):
+                 "g4go_set_stream", "g4go_enqueue_emissions_device", "g4go_transport_synthetic_noalloc"):
         getattr(lib, name).restype = C.c_int
     _lib = lib
     return lib

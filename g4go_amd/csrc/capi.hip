@@ -17,6 +17,7 @@
 #include <unordered_map>
 #include <unordered_set>
 #include <memory>
+#include <mutex>
 #include <vector>
 
 #include "../../include/g4go_hip.h"
@@ -46,6 +47,54 @@ void hip_check(hipError_t err, const char* what) {
 }
 
 thread_local std::string g_create_error;
+
+// ---------------------------------------------------------------------------
+// Device scratch of the allocation-free entry point
+// (g4go_transport_synthetic_noalloc): a fixed array in every GPU's memory,
+// laid out as
+//   [0]            batch header {n_sources, chunk, n_events, 0}
+//   [256]          counters (kCounterCount u64: steps, photons, dropped, queue head)
+//   [512]          item offsets (kPrefixMaxSources + 1 u32)
+//   [4864]         photon offsets (kPrefixMaxSources + 1 u32)
+//   [9216]         kEventReplicas x kPrefixMaxSources per-event accumulation copies
+// Host code never passes `noalloc_scratch` itself to a kernel (in host code the
+// name is a host-side stand-in, not a device address): it uses the device
+// address noalloc_scratch_address() looked up once per device.
+// ---------------------------------------------------------------------------
+constexpr size_t scratch_round(size_t x) { return (x + 255) & ~static_cast<size_t>(255); }
+constexpr size_t kScratchHeader = 0;
+constexpr size_t kScratchCounters = 256;
+constexpr size_t kScratchItems = 512;
+constexpr size_t kScratchPhotons = scratch_round(kScratchItems + (kPrefixMaxSources + 1) * sizeof(uint32_t));
+constexpr size_t kScratchReplicas = scratch_round(kScratchPhotons + (kPrefixMaxSources + 1) * sizeof(uint32_t));
+constexpr size_t kScratchBytes =
+    kScratchReplicas + static_cast<size_t>(kEventReplicas) * kPrefixMaxSources * sizeof(EventStats);
+static_assert(kScratchBytes == G4GO_NOALLOC_SCRATCH_BYTES, "scratch size of the C ABI");
+static_assert(kPrefixMaxSources == G4GO_NOALLOC_MAX_EVENTS, "event limit of the C ABI");
+
+static __device__ __align__(256) unsigned char noalloc_scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch_address[kMaxDevices] = {};
+
+// Device address of noalloc_scratch on the current device `device`, looked up
+// once per device (each GPU holds its own copy of the array).
+void* noalloc_scratch_address(int device) {
+    if (device < 0 || device >= kMaxDevices) {
+        fail(G4GO_ERROR_RUNTIME, "device index beyond the scratch table");
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (g_scratch_address[device] == nullptr) {
+        void* ptr = nullptr;
+        hip_check(hipGetSymbolAddress(&ptr, HIP_SYMBOL(noalloc_scratch)), "hipGetSymbolAddress scratch");
+        if (ptr == nullptr) {
+            fail(G4GO_ERROR_RUNTIME, "hipGetSymbolAddress returned no scratch address");
+        }
+        g_scratch_address[device] = ptr;
+    }
+    return g_scratch_address[device];
+}
 
 // ---------------------------------------------------------------------------
 // Host scene (flattened Scene, see scene.h)
@@ -1004,6 +1053,7 @@ struct Core {
     // queued on it before the batch was enqueued
     hipStream_t user_stream = nullptr;
     hipEvent_t ev_user = nullptr;
+    char* scratch = nullptr;  // device address of this device's noalloc_scratch
 
     HostScene scene;
     bool scene_built = false;
@@ -1073,6 +1123,7 @@ struct Core {
         hip_check(hipEventCreate(&ev_kernel1), "hipEventCreate");
         hip_check(hipEventCreate(&ev_meta), "hipEventCreate");
         hip_check(hipEventCreateWithFlags(&ev_user, hipEventDisableTiming), "hipEventCreate");
+        scratch = static_cast<char*>(noalloc_scratch_address(device));
         hip_ready = true;
     }
 
@@ -1688,6 +1739,119 @@ static g4go_status core_enqueue_synthetic(Core* h, const g4go_synthetic_source* 
     });
 }
 
+// The allocation-free synthetic batch (g4go_transport_synthetic_noalloc):
+// the scratch layout is described at noalloc_scratch.  Validation mirrors
+// core_enqueue_synthetic as far as host data allows (event IDs and counts are
+// in device memory).
+static g4go_status core_transport_synthetic_noalloc(Core* h, const g4go_synthetic_source* source,
+                                                    const uint32_t* event_ids, const uint32_t* photon_counts,
+                                                    uint32_t n_events, const g4go_device_outputs* outputs,
+                                                    void* stream_handle) {
+    if (h == nullptr) {
+        return G4GO_ERROR_INVALID_ARGUMENT;
+    }
+    return guarded(h, [&] {
+        if (source == nullptr || outputs == nullptr) {
+            fail(G4GO_ERROR_INVALID_ARGUMENT, "null synthetic source or outputs");
+        }
+        if (n_events > kPrefixMaxSources) {
+            fail(G4GO_ERROR_INVALID_ARGUMENT, "the allocation-free entry point takes at most " +
+                                                  std::to_string(kPrefixMaxSources) + " events per call");
+        }
+        if ((n_events != 0 && (event_ids == nullptr || photon_counts == nullptr)) ||
+            (outputs->hit_capacity != 0 && outputs->hits == nullptr) || outputs->hit_count == nullptr ||
+            outputs->histogram == nullptr || (n_events != 0 && outputs->event_counters == nullptr) ||
+            outputs->batch_counters == nullptr) {
+            fail(G4GO_ERROR_INVALID_ARGUMENT, "null device array");
+        }
+        if (outputs->hit_capacity > 0xffffffffull) {
+            fail(G4GO_ERROR_INVALID_ARGUMENT, "hit capacity must be below 2^32");
+        }
+        if (!h->scene_built) {
+            fail(G4GO_ERROR_LOGIC, "no scene has been prepared");
+        }
+        if (!h->hip_ready || !h->scene_uploaded) {
+            fail(G4GO_ERROR_LOGIC,
+                 "the scene is not resident on the device: transport one batch with the allocating entry points first");
+        }
+        const auto it = h->scene.index_of.find(source->volume_id);
+        if (it == h->scene.index_of.end()) {
+            fail(G4GO_ERROR_INVALID_ARGUMENT, "synthetic source references an unknown volume ID");
+        }
+        for (int k = 0; k < 3; ++k) {
+            if (!std::isfinite(source->lo_mm[k]) || !std::isfinite(source->hi_mm[k]) || source->hi_mm[k] < source->lo_mm[k]) {
+                fail(G4GO_ERROR_INVALID_ARGUMENT, "synthetic source box is invalid");
+            }
+        }
+        if (!(source->wavelength_mean_nm > 0.0f) || !(source->wavelength_sigma_nm >= 0.0f) ||
+            !(source->decay_time_ns >= 0.0f)) {
+            fail(G4GO_ERROR_INVALID_ARGUMENT, "synthetic source distribution is invalid");
+        }
+        hipStream_t stream = stream_handle != nullptr ? static_cast<hipStream_t>(stream_handle) : h->stream;
+        char* const scratch = h->scratch;
+        auto* header = reinterpret_cast<uint32_t*>(scratch + kScratchHeader);
+        auto* counters = reinterpret_cast<unsigned long long*>(scratch + kScratchCounters);
+        auto* items = reinterpret_cast<uint32_t*>(scratch + kScratchItems);
+        auto* photon_offsets = reinterpret_cast<uint32_t*>(scratch + kScratchPhotons);
+        auto* replicas = reinterpret_cast<EventStats*>(scratch + kScratchReplicas);
+        auto* batch_counters = reinterpret_cast<unsigned long long*>(outputs->batch_counters);
+        // everything that accumulates starts at zero: fill kernels, no memset
+        const size_t replica_bytes = static_cast<size_t>(kEventReplicas) * n_events * sizeof(EventStats);
+        if (launch_zero(counters, kCounterCount * sizeof(unsigned long long), stream) != 0 ||
+            launch_zero(replicas, replica_bytes, stream) != 0 ||
+            launch_zero(outputs->hit_count, sizeof(uint32_t), stream) != 0 ||
+            launch_zero(outputs->histogram, kHistSlots * sizeof(uint64_t), stream) != 0) {
+            fail(G4GO_ERROR_RUNTIME, "zero kernel launch failed");
+        }
+        if (n_events == 0) {
+            if (launch_export_counters(counters, batch_counters, stream) != 0) {
+                fail(G4GO_ERROR_RUNTIME, "counter export launch failed");
+            }
+            return;
+        }
+        const uint32_t chunk = h->launch.chunk >= 1u && h->launch.chunk <= (1u << 20) ? h->launch.chunk : 256u;
+        if (launch_synthetic_prefix(photon_counts, n_events, chunk, header, items, photon_offsets, stream) != 0) {
+            fail(G4GO_ERROR_RUNTIME, "batch prefix launch failed");
+        }
+        KernelParams p = h->base_params();
+        p.synth.lo[0] = source->lo_mm[0];
+        p.synth.lo[1] = source->lo_mm[1];
+        p.synth.lo[2] = source->lo_mm[2];
+        p.synth.hi[0] = source->hi_mm[0];
+        p.synth.hi[1] = source->hi_mm[1];
+        p.synth.hi[2] = source->hi_mm[2];
+        p.synth.wavelength_mean_nm = source->wavelength_mean_nm;
+        p.synth.wavelength_sigma_nm = source->wavelength_sigma_nm;
+        p.synth.decay_ns = source->decay_time_ns;
+        p.synth.volume = it->second;
+        p.header = header;
+        p.event_ids = event_ids;
+        p.item_offsets = items;
+        p.source_counts = photon_counts;
+        p.source_event = nullptr;
+        p.source_photon_offset = photon_offsets;
+        p.hits = reinterpret_cast<Hit*>(outputs->hits);
+        p.hit_capacity = static_cast<uint32_t>(outputs->hit_capacity);
+        p.hit_count = outputs->hit_count;
+        p.hit_launch = nullptr;
+        p.histogram = reinterpret_cast<unsigned long long*>(outputs->histogram);
+        p.event_stats = replicas;
+        p.event_stride = n_events;
+        p.counters = counters;
+        const int block = static_cast<int>(h->launch.block_size);
+        const int grid =
+            static_cast<int>(std::max<uint32_t>(1, h->launch.blocks_per_cu) * static_cast<uint32_t>(h->cu_count));
+        if (launch_transport(p, kSourceSynthetic, true, block, grid, stream) != 0) {
+            fail(G4GO_ERROR_RUNTIME, "transport kernel launch failed");
+        }
+        if (launch_reduce_event_stats(replicas, n_events, header, reinterpret_cast<EventStats*>(outputs->event_counters),
+                                      stream) != 0 ||
+            launch_export_counters(counters, batch_counters, stream) != 0) {
+            fail(G4GO_ERROR_RUNTIME, "counter reduction launch failed");
+        }
+    });
+}
+
 static g4go_status complete_batch(Core* h, g4go_batch_result* out, bool copy_hits,
                                   const void** hits_device) {
     if (h == nullptr || out == nullptr) {
@@ -2280,6 +2444,24 @@ g4go_status g4go_complete_device(g4go_transport* t, g4go_batch_result* out, cons
         return complete_oldest(t, out, &ignored);
     }
     return complete_oldest(t, out, hits_device);
+}
+
+g4go_status g4go_transport_synthetic_noalloc(g4go_transport* t, const g4go_synthetic_source* source,
+                                             const uint32_t* event_ids_device, const uint32_t* photon_counts_device,
+                                             uint32_t n_events, const g4go_device_outputs* outputs, void* stream) {
+    if (t == nullptr) {
+        return G4GO_ERROR_INVALID_ARGUMENT;
+    }
+    // any slot whose scene copy is resident serves (the slots hold the same scene)
+    Core* c = t->cores[0].get();
+    for (auto& k : t->cores) {
+        if (k->scene_uploaded) {
+            c = k.get();
+            break;
+        }
+    }
+    return t->pass(c, core_transport_synthetic_noalloc(c, source, event_ids_device, photon_counts_device, n_events,
+                                                       outputs, stream));
 }
 
 g4go_status g4go_device_info(g4go_transport* t, char* name, uint64_t name_len, int32_t* cu_count) {

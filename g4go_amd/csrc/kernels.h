@@ -149,6 +149,16 @@ struct OrderParams {
     uint32_t* out_launch;
 };
 int launch_order_hits(const OrderParams& p, hipStream_t stream);
+// Work-item and photon offsets of a synthetic batch computed on the device
+// (the batch's photon counts are in device memory): one block scans the
+// n <= kPrefixMaxSources counts into item_offsets[n + 1] (exclusive prefix of
+// ceil(count / chunk)) and photon_offsets[n + 1], and writes the batch header
+// {n, chunk, n, 0}.
+constexpr uint32_t kPrefixMaxSources = 1024;
+int launch_synthetic_prefix(const uint32_t* counts, uint32_t n, uint32_t chunk, uint32_t* header,
+                            uint32_t* item_offsets, uint32_t* photon_offsets, hipStream_t stream);
+// out[0..2] = counters[kCounterSteps], counters[kCounterPhotons], counters[kCounterDropped].
+int launch_export_counters(const unsigned long long* counters, unsigned long long* out, hipStream_t stream);
 int launch_trace(const KernelParams& p, uint32_t event_id, uint32_t photon_id, uint32_t max_steps, float* out,
                  uint32_t* n_out, hipStream_t stream);
 int launch_eval_math(const float* in, float* out, int n, int which, hipStream_t stream);

@@ -930,6 +930,63 @@ __global__ void order_scatter_kernel(OrderParams p) {
     }
 }
 
+__global__ void __launch_bounds__(kPrefixMaxSources) synthetic_prefix_kernel(const uint32_t* counts, uint32_t n,
+                                                                               uint32_t chunk, uint32_t* header,
+                                                                               uint32_t* item_offsets,
+                                                                               uint32_t* photon_offsets) {
+    __shared__ uint32_t s_items[kPrefixMaxSources];
+    __shared__ uint32_t s_photons[kPrefixMaxSources];
+    const uint32_t t = threadIdx.x;
+    const uint32_t c = t < n ? counts[t] : 0u;
+    s_items[t] = static_cast<uint32_t>((static_cast<uint64_t>(c) + chunk - 1u) / chunk);
+    s_photons[t] = c;
+    __syncthreads();
+    // inclusive Hillis-Steele scan over the block
+    for (uint32_t off = 1; off < kPrefixMaxSources; off <<= 1) {
+        const uint32_t a = t >= off ? s_items[t - off] : 0u;
+        const uint32_t b = t >= off ? s_photons[t - off] : 0u;
+        __syncthreads();
+        s_items[t] += a;
+        s_photons[t] += b;
+        __syncthreads();
+    }
+    if (t < n) {
+        item_offsets[t + 1] = s_items[t];
+        photon_offsets[t + 1] = s_photons[t];
+    }
+    if (t == 0) {
+        item_offsets[0] = 0u;
+        photon_offsets[0] = 0u;
+        header[0] = n;
+        header[1] = chunk;
+        header[2] = n;
+        header[3] = 0u;
+    }
+}
+
+int launch_synthetic_prefix(const uint32_t* counts, uint32_t n, uint32_t chunk, uint32_t* header,
+                            uint32_t* item_offsets, uint32_t* photon_offsets, hipStream_t stream) {
+    if (n > kPrefixMaxSources || chunk == 0u) {
+        return 1;
+    }
+    hipLaunchKernelGGL(synthetic_prefix_kernel, dim3(1), dim3(kPrefixMaxSources), 0, stream, counts, n, chunk, header,
+                       item_offsets, photon_offsets);
+    return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
+__global__ void export_counters_kernel(const unsigned long long* counters, unsigned long long* out) {
+    if (threadIdx.x == 0) {
+        out[0] = counters[kCounterSteps];
+        out[1] = counters[kCounterPhotons];
+        out[2] = counters[kCounterDropped];
+    }
+}
+
+int launch_export_counters(const unsigned long long* counters, unsigned long long* out, hipStream_t stream) {
+    hipLaunchKernelGGL(export_counters_kernel, dim3(1), dim3(64), 0, stream, counters, out);
+    return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
 int launch_order_hits(const OrderParams& p, hipStream_t stream) {
     const uint32_t n_words = (p.n_photons + 31u) / 32u;
     const uint32_t n_blocks = (n_words + kOrderWordsPerBlock - 1) / kOrderWordsPerBlock;

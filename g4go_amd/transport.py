@@ -225,6 +225,23 @@ class HipTransportHost:
         self._check(self._lib.g4go_enqueue_synthetic(self._h, C.byref(source), ev.ctypes.data if ev.size else None,
                                                      counts.ctypes.data if counts.size else None, ev.size))
 
+    def transport_synthetic_noalloc(self, source: abi.SyntheticSource, event_ids, photon_counts,
+                                    outputs: abi.DeviceOutputs, stream=None) -> None:
+        """g4go_transport_synthetic_noalloc: a synthetic batch that allocates nothing.  event_ids and
+        photon_counts are device tensors of n_events 4-byte values; `outputs` holds device pointers
+        (abi.DeviceOutputs).  The work is queued on `stream` (a torch.cuda.Stream or raw handle; None:
+        the handle's stream) and complete when the stream reaches it."""
+        n = int(event_ids.numel())
+        if int(photon_counts.numel()) != n:
+            raise ValueError("event_ids and photon_counts must have the same length")
+        for t in (event_ids, photon_counts):
+            if n and (not t.is_cuda or t.element_size() != 4 or not t.is_contiguous()):
+                raise ValueError("event_ids and photon_counts must be contiguous 4-byte device tensors")
+        handle = getattr(stream, "cuda_stream", stream)
+        self._check(self._lib.g4go_transport_synthetic_noalloc(
+            self._h, C.byref(source), event_ids.data_ptr() if n else None, photon_counts.data_ptr() if n else None,
+            n, C.byref(outputs), C.c_void_p(int(handle)) if handle else None))
+
     def complete_oldest_batch(self) -> PhotonTransportOutput:
         res = abi.BatchResult()
         self._check(self._lib.g4go_complete(self._h, C.byref(res)))

@@ -305,6 +305,49 @@ g4go_status g4go_complete(g4go_transport* handle, g4go_batch_result* out);
  * otherwise in completion order), valid until the next enqueue. */
 g4go_status g4go_complete_device(g4go_transport* handle, g4go_batch_result* out, const void** hits_device);
 
+/* ---- allocation-free synthetic batch ----
+ * No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The variant of g4go_enqueue_synthetic +
+ * g4go_complete_device that allocates nothing: the batch description is in
+ * device memory, the caller passes the output buffers, and the work queue,
+ * batch header, item offsets and per-event accumulation copies live in the
+ * library's fixed device scratch array (G4GO_NOALLOC_SCRATCH_BYTES per GPU,
+ * held until the process ends).  At most G4GO_NOALLOC_MAX_EVENTS events per
+ * call; larger batches are rejected with G4GO_ERROR_INVALID_ARGUMENT.
+ *
+ * The call only enqueues device work on `stream` (a hipStream_t; NULL: the
+ * handle's own stream) and returns: no allocation, no host synchronisation,
+ * no hipMemset (fill kernels clear what starts at zero), so it can be
+ * captured into a graph.  The outputs are complete when the stream reaches
+ * that point.  The handle's scene must already be resident on the device
+ * (prepared and transported at least once by the ordinary entry points), and
+ * must not change while such work is in flight.  Every call on one GPU shares
+ * that GPU's scratch array: calls whose work overlaps on one GPU (on
+ * different streams, or from different handles) race and give wrong results.
+ * The transport is the handle's persistent form (block size, chunk, refill
+ * threshold, blocks per CU of g4go_get_launch), with the handle's seed; the
+ * hits are the same records as g4go_complete_device's, in completion order.
+ * The caller guarantees distinct event IDs and fewer than 2^32 photons. */
+#define G4GO_NOALLOC_MAX_EVENTS 1024
+#define G4GO_NOALLOC_SCRATCH_BYTES 1844224 /* header, counters, offsets, 32 x 1024 event counters */
+
+typedef struct g4go_device_outputs {
+    g4go_hit* hits;              /* hit_capacity records; detections beyond it are counted as dropped */
+    uint64_t hit_capacity;       /* at most 2^32 - 1 */
+    uint32_t* hit_count;         /* 1: detections (may exceed hit_capacity) */
+    uint64_t* histogram;         /* G4GO_HIST_SLOTS arrival-time counts */
+    uint64_t* event_counters;    /* 7 per event, in batch order: detected, absorbed, escaped,
+                                    truncated, max_bounce, invalid, zero_step */
+    uint64_t* batch_counters;    /* 3: transport steps, generated photons, dropped hits */
+} g4go_device_outputs;
+
+/* event_ids_device[n_events], photon_counts_device[n_events]: device arrays.
+ * All pointers of `outputs` are device pointers. */
+g4go_status g4go_transport_synthetic_noalloc(g4go_transport* handle, const g4go_synthetic_source* source,
+                                             const uint32_t* event_ids_device, const uint32_t* photon_counts_device,
+                                             uint32_t n_events, const g4go_device_outputs* outputs, void* stream);
+
 /* ---- differentiation (path-replay adjoint) ----
  * A parameter layout binds property tables of the prepared scene to a flat
  * parameter vector theta, table after table in the order given:
