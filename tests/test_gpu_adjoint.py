@@ -258,7 +258,7 @@ def test_autograd_emission_records(crystal):
         loss.backward()
         grad = theta.grad.numpy()
         w = np.zeros(abi.HIST_SLOTS)
-        w[:] = 2.0 * (float(n) - float(n_t))
+        w[:] = 2.0 * (float(n.detach()) - float(n_t))
         w[:abi.HIST_BINS] += 2.0 * (hist.detach().numpy() - h_t.numpy())
         direct, stats = h.adjoint(w)
         assert stats["mismatched"] == 0 and stats["unsupported"] == 0
