@@ -47,6 +47,7 @@ class Launch(C.Structure):
         ("blocks_per_cu", C.c_uint32),
         ("per_event", C.c_uint32),
         ("graph", C.c_uint32),
+        ("pregenerate", C.c_uint32),
     ]
 
 
@@ -220,6 +221,12 @@ class BatchStats(C.Structure):
         ("device_to_host_ms", C.c_double),
         ("host_to_device_bytes", C.c_uint64),
         ("device_to_host_bytes", C.c_uint64),
+        ("device_memset_ms", C.c_double),
+        ("device_order_ms", C.c_double),
+        ("metadata_to_host_ms", C.c_double),
+        ("metadata_to_host_bytes", C.c_uint64),
+        ("hits_to_host_ms", C.c_double),
+        ("hits_to_host_bytes", C.c_uint64),
     ]
 
 
@@ -277,7 +284,7 @@ EMISSION_DTYPE = np.dtype([
 ])
 assert EMISSION_DTYPE.itemsize == C.sizeof(Emission) == 128
 
-assert C.sizeof(Config) == 48 and C.sizeof(Launch) == 28
+assert C.sizeof(Config) == 48 and C.sizeof(Launch) == 32
 
 EXPORTED_SYMBOLS = (
     "g4go_create", "g4go_destroy", "g4go_last_error", "g4go_set_launch", "g4go_get_launch",

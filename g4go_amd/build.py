@@ -65,6 +65,27 @@ def build(verbose: bool = True) -> Path:
     return LIBRARY
 
 
+PROFILE_LIBRARY = PACKAGE_DIR / "_prof" / "libg4go_hip.so"
+
+
+def build_profile(verbose: bool = True) -> Path:
+    """The region-profile counter build (-DG4GO_REGION_PROFILE, csrc/region_prof.h) in _prof/:
+    the same kernels with cycle counters at region markers; never the product library."""
+    prof_dir = PROFILE_LIBRARY.parent
+    prof_dir.mkdir(exist_ok=True)
+    objects = []
+    for src in SOURCES:
+        obj = prof_dir / (Path(src).stem + ".o")
+        _run([HIPCC, *HIP_FLAGS, "-DG4GO_REGION_PROFILE", "-c", str(CSRC / src), "-o", str(obj)])
+        objects.append(str(obj))
+    tmp = PROFILE_LIBRARY.with_suffix(".so.tmp")
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objects, "-o", str(tmp)])
+    os.replace(tmp, PROFILE_LIBRARY)
+    if verbose:
+        print(f"built {PROFILE_LIBRARY}")
+    return PROFILE_LIBRARY
+
+
 def build_shim_test(verbose: bool = True) -> Path:
     """Host-only C++20 test program of include/g4go_hip_host.hpp, linked to the library."""
     tmp = SHIM_TEST.with_name(SHIM_TEST.name + ".tmp")
@@ -77,4 +98,7 @@ def build_shim_test(verbose: bool = True) -> Path:
 
 
 if __name__ == "__main__":
-    build()
+    if len(sys.argv) > 1 and sys.argv[1] == "profile":
+        build_profile()
+    else:
+        build()

@@ -1049,6 +1049,8 @@ struct Core {
     int cu_count = 256;
     hipStream_t stream = nullptr;
     hipEvent_t ev_start = nullptr, ev_kernel0 = nullptr, ev_kernel1 = nullptr, ev_meta = nullptr;
+    // per-stage timers: H2D copies end, launch-order pass end, hit readback
+    hipEvent_t ev_h2d = nullptr, ev_order = nullptr, ev_hits0 = nullptr, ev_hits1 = nullptr;
     // the caller's stream (g4go_set_stream): each batch waits for the work
     // queued on it before the batch was enqueued
     hipStream_t user_stream = nullptr;
@@ -1075,6 +1077,7 @@ struct Core {
     uint64_t h2d_bytes = 0;
 
     DevBuf d_inputs, d_hits, d_event_stats, d_meta;
+    DevBuf d_starts;  // start states of the pre-generation pass (kStartQuads float4 per photon)
     // launch index of each hit, and the launch-order copies (cfg.sort_hits)
     DevBuf d_hit_launch, d_hits_ordered, d_launch_ordered, d_order;
     bool ordered = false;  // the last batch's final hits are the launch-order copies
@@ -1122,6 +1125,9 @@ struct Core {
         hip_check(hipEventCreate(&ev_kernel0), "hipEventCreate");
         hip_check(hipEventCreate(&ev_kernel1), "hipEventCreate");
         hip_check(hipEventCreate(&ev_meta), "hipEventCreate");
+        for (hipEvent_t* e : {&ev_h2d, &ev_order, &ev_hits0, &ev_hits1}) {
+            hip_check(hipEventCreate(e), "hipEventCreate");
+        }
         hip_check(hipEventCreateWithFlags(&ev_user, hipEventDisableTiming), "hipEventCreate");
         scratch = static_cast<char*>(noalloc_scratch_address(device));
         hip_ready = true;
@@ -1208,13 +1214,15 @@ struct Core {
         d_adjoint.release();
         d_scene.release();
         d_inputs.release();
+        d_starts.release();
         d_hits.release();
         d_event_stats.release();
         d_meta.release();
         h_stage.release();
         h_meta.release();
         h_hits.release();
-        for (hipEvent_t* e : {&ev_start, &ev_kernel0, &ev_kernel1, &ev_meta, &ev_user}) {
+        for (hipEvent_t* e : {&ev_start, &ev_kernel0, &ev_kernel1, &ev_meta, &ev_user, &ev_h2d, &ev_order, &ev_hits0,
+                              &ev_hits1}) {
             if (*e != nullptr) {
                 (void)hipEventDestroy(*e);
                 *e = nullptr;
@@ -1369,6 +1377,7 @@ struct Core {
             }
         }
         h2d_bytes = 16 + em_bytes + ev_bytes + off_bytes + cnt_bytes + sev_bytes + poff_bytes;
+        hip_check(hipEventRecord(ev_h2d, stream), "hipEventRecord");
 
         KernelParams p = base_params();
         if (synth != nullptr) {
@@ -1410,14 +1419,25 @@ struct Core {
         };
         if (launch.persistent) {
             const int grid = static_cast<int>(std::max<uint32_t>(1, launch.blocks_per_cu) * static_cast<uint32_t>(cu_count));
+            // pre-generation: the start states of every photon by a full-wave pass
+            // before the transport (not for triangle solids, whose kernels generate)
+            if (launch.pregenerate != 0 && view.n_tris == 0) {
+                d_starts.ensure(std::max<uint64_t>(n_photons, 1) * kStartQuads * sizeof(float4), "hipMalloc start states");
+                p.starts = d_starts.as<float4>();
+                last_params.starts = nullptr;  // the adjoint replays by generating
+            }
+            const int gen_grid = 8 * cu_count;
+            const auto generate = [&] {
+                return p.starts == nullptr || launch_generate(p, source, const_cast<float4*>(p.starts), gen_grid, stream) == 0;
+            };
             if (launch.graph) {
-                // zero kernels + transport kernel, captured once per parameter set
+                // zero kernels + generation + transport kernel, captured once per parameter set
                 if (graph_exec == nullptr || std::memcmp(&graph_params, &p, sizeof(p)) != 0 || graph_grid != grid ||
                     graph_block != block || graph_source != source) {
                     drop_graph();
                     hip_check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
                     bool ok = launch_zero(d_meta.ptr, kMetaBytes, stream) == 0 &&
-                              launch_zero(d_event_stats.ptr, replica_bytes, stream) == 0 &&
+                              launch_zero(d_event_stats.ptr, replica_bytes, stream) == 0 && generate() &&
                               launch_transport(p, source, true, block, grid, stream) == 0 &&
                               launch_reduce_event_stats(p.event_stats, p.event_stride, p.header, d_events_out,
                                                         stream) == 0;
@@ -1437,6 +1457,9 @@ struct Core {
             } else {
                 zero_outputs();
                 hip_check(hipEventRecord(ev_kernel0, stream), "hipEventRecord");
+                if (!generate()) {
+                    fail(G4GO_ERROR_RUNTIME, "generation kernel launch failed");
+                }
                 if (launch_transport(p, source, true, block, grid, stream) != 0) {
                     fail(G4GO_ERROR_RUNTIME, "transport kernel launch failed");
                 }
@@ -1498,6 +1521,7 @@ struct Core {
                 fail(G4GO_ERROR_RUNTIME, "hit ordering launch failed");
             }
         }
+        hip_check(hipEventRecord(ev_order, stream), "hipEventRecord");
         hip_check(hipMemcpyAsync(h_meta.ptr, d_meta.ptr, kMetaBytes, hipMemcpyDeviceToHost, stream),
                   "hipMemcpyAsync counters");
         if (n_events) {
@@ -1549,6 +1573,9 @@ void check_launch(const g4go_launch& l) {
     }
     if (l.persistent && (l.blocks_per_cu == 0 || l.blocks_per_cu > 64)) {
         fail(G4GO_ERROR_INVALID_ARGUMENT, "blocks_per_cu must be in [1, 64]");
+    }
+    if (l.pregenerate > 1) {
+        fail(G4GO_ERROR_INVALID_ARGUMENT, "pregenerate must be 0 or 1");
     }
 }
 
@@ -1884,9 +1911,12 @@ static g4go_status complete_batch(Core* h, g4go_batch_result* out, bool copy_hit
             const auto* hist = reinterpret_cast<const uint64_t*>(meta + Core::kMetaHist);
             const auto* es = reinterpret_cast<const EventStats*>(meta + Core::kMetaBytes);
             const uint64_t n_hits = std::min<uint64_t>(hit_count, h->n_photons);
-            float kernel_ms = 0.0f, h2d_ms = 0.0f;
+            float kernel_ms = 0.0f, h2d_ms = 0.0f, memset_ms = 0.0f, order_ms = 0.0f, meta_ms = 0.0f, hits_ms = 0.0f;
             hip_check(hipEventElapsedTime(&kernel_ms, h->ev_kernel0, h->ev_kernel1), "hipEventElapsedTime");
-            hip_check(hipEventElapsedTime(&h2d_ms, h->ev_start, h->ev_kernel0), "hipEventElapsedTime");
+            hip_check(hipEventElapsedTime(&h2d_ms, h->ev_start, h->ev_h2d), "hipEventElapsedTime");
+            hip_check(hipEventElapsedTime(&memset_ms, h->ev_h2d, h->ev_kernel0), "hipEventElapsedTime");
+            hip_check(hipEventElapsedTime(&order_ms, h->ev_kernel1, h->ev_order), "hipEventElapsedTime");
+            hip_check(hipEventElapsedTime(&meta_ms, h->ev_order, h->ev_meta), "hipEventElapsedTime");
             const auto d2h0 = std::chrono::steady_clock::now();
             h->device_hits = n_hits;
             if (!copy_hits) {
@@ -1895,10 +1925,13 @@ static g4go_status complete_batch(Core* h, g4go_batch_result* out, bool copy_hit
                 }
             } else if (n_hits != 0) {
                 h->h_hits.ensure(n_hits * sizeof(Hit), "hipHostMalloc hits");
+                hip_check(hipEventRecord(h->ev_hits0, h->stream), "hipEventRecord");
                 hip_check(hipMemcpyAsync(h->h_hits.ptr, h->final_hits(), n_hits * sizeof(Hit), hipMemcpyDeviceToHost,
                                          h->stream),
                           "hipMemcpyAsync hits");
+                hip_check(hipEventRecord(h->ev_hits1, h->stream), "hipEventRecord");
                 hip_check(hipStreamSynchronize(h->stream), "hipStreamSynchronize");
+                hip_check(hipEventElapsedTime(&hits_ms, h->ev_hits0, h->ev_hits1), "hipEventElapsedTime");
                 h->out_n_hits = n_hits;
             }
             const auto d2h1 = std::chrono::steady_clock::now();
@@ -1927,7 +1960,13 @@ static g4go_status complete_batch(Core* h, g4go_batch_result* out, bool copy_hit
             bs.host_to_device_ms = h2d_ms;
             bs.device_to_host_ms = std::chrono::duration<double, std::milli>(d2h1 - d2h0).count();
             bs.host_to_device_bytes = h->h2d_bytes;
-            bs.device_to_host_bytes = Core::kMetaBytes + n_events * sizeof(EventStats) + n_hits * sizeof(Hit);
+            bs.device_memset_ms = memset_ms;
+            bs.device_order_ms = order_ms;
+            bs.metadata_to_host_ms = meta_ms;
+            bs.metadata_to_host_bytes = Core::kMetaBytes + n_events * sizeof(EventStats);
+            bs.hits_to_host_ms = hits_ms;
+            bs.hits_to_host_bytes = h->out_n_hits * sizeof(Hit);
+            bs.device_to_host_bytes = bs.metadata_to_host_bytes + bs.hits_to_host_bytes;
         }
         bs.transport_time_ms =
             std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - h->started).count();
@@ -2282,7 +2321,7 @@ g4go_status g4go_create(const g4go_config* config, g4go_transport** out) {
         for (uint32_t k = 0; k < n; ++k) {
             auto c = std::make_unique<Core>();
             c->cfg = *config;
-            c->launch = g4go_launch{1, 256, 256, 24, 8, 0, 0};  // persistent, tuned on crystal batches
+            c->launch = g4go_launch{1, 256, 256, 24, 8, 0, 0, 0};  // persistent, tuned on crystal batches
             t->cores.push_back(std::move(c));
         }
         *out = t.release();

@@ -88,7 +88,17 @@ struct KernelParams {
     uint32_t lds_pool_floats;
     uint32_t lds_records;
     uint32_t pad_lds;
+    // Persistent form: the start states of the batch's photons, generated by
+    // generate_kernel before the transport launch, three float4 per photon in
+    // launch order (StartState); null: idle lanes generate their photons at
+    // refill.
+    const float4* starts;
 };
+
+// Start state of one photon as generate_kernel writes it: position and
+// energy (NaN when start_photon rejects the photon), direction and the low
+// word of the FP64 time, polarisation and the high word.
+constexpr uint32_t kStartQuads = 3;
 
 // Parameterised property tables for the adjoint: theta offset of each
 // material's absorption / Rayleigh table and each surface's reflectivity /
@@ -116,7 +126,7 @@ struct AdjointParams {
 constexpr uint32_t kLdsVolumes = 1u, kLdsSurfaces = 2u, kLdsMaterials = 4u, kLdsPlanes = 8u;
 
 // Dynamic LDS bytes of a transport launch (the staged scene).
-inline size_t transport_lds_bytes(const KernelParams& p) {
+__host__ __device__ inline size_t transport_lds_bytes(const KernelParams& p) {
     size_t bytes = static_cast<size_t>(p.lds_entries) * 32u + (static_cast<size_t>(p.lds_pool_floats) + 3u) / 4u * 16u;
     if (p.lds_records & kLdsVolumes) bytes += static_cast<size_t>(p.scene.n_volumes) * sizeof(Volume);
     if (p.lds_records & kLdsSurfaces) bytes += static_cast<size_t>(p.scene.n_surfaces) * sizeof(Surface);
@@ -125,6 +135,10 @@ inline size_t transport_lds_bytes(const KernelParams& p) {
     return bytes;
 }
 int launch_transport(const KernelParams& p, int source, bool persistent, int block, int grid, hipStream_t stream);
+// Writes the start states of all photons of the batch (items of p, launch
+// order) to starts[kStartQuads * n_photons]: the generation pre-pass of the
+// persistent form.
+int launch_generate(const KernelParams& p, int source, float4* starts, int grid, hipStream_t stream);
 int launch_zero(void* ptr, size_t bytes, hipStream_t stream);
 // Sums the kEventReplicas copies of the first header[2] events into out.
 int launch_reduce_event_stats(const EventStats* replicas, uint32_t stride, const uint32_t* header, EventStats* out,

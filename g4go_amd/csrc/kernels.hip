@@ -186,6 +186,46 @@ __device__ __forceinline__ void store_hit(float4* d, const Photon& ph, uint32_t 
                        __uint_as_float(static_cast<uint32_t>(tb)), __uint_as_float(static_cast<uint32_t>(tb >> 32)));
 }
 
+// Start state of a generated photon (kStartQuads float4 at out), checked by
+// start_photon; a rejected start is marked by a NaN energy.
+__device__ __forceinline__ void write_start(float4* out, Photon& g) {
+    const bool ok = start_photon(g);
+    const uint64_t tb = static_cast<uint64_t>(__double_as_longlong(g.time));
+    out[0] = make_float4(g.pos.x, g.pos.y, g.pos.z, ok ? g.energy : u2f(0x7fc00000u));
+    out[1] = make_float4(g.dir.x, g.dir.y, g.dir.z, __uint_as_float(static_cast<uint32_t>(tb)));
+    out[2] = make_float4(g.pol.x, g.pol.y, g.pol.z, __uint_as_float(static_cast<uint32_t>(tb >> 32)));
+}
+
+// Photon `local` of source `source` (event index event_index) from its start
+// state at launch index `launch`: the identifiers as generate() sets them.
+// Returns false when the start was rejected.
+template <int kSource>
+__device__ __forceinline__ bool load_start(Photon& ph, const KernelParams& p, uint32_t event_index, uint32_t source,
+                                           uint32_t local, uint32_t launch) {
+    const float4* r = p.starts + static_cast<size_t>(kStartQuads) * launch;
+    const float4 a = r[0], b = r[1], c = r[2];
+    ph.pos = V3{a.x, a.y, a.z};
+    ph.energy = a.w;
+    ph.dir = V3{b.x, b.y, b.z};
+    ph.pol = V3{c.x, c.y, c.z};
+    ph.time = __longlong_as_double(
+        static_cast<long long>((static_cast<uint64_t>(__float_as_uint(c.w)) << 32) | __float_as_uint(b.w)));
+    if (kSource == kSourceSynthetic) {
+        ph.photon_id = local;
+        ph.event_id = p.event_ids[event_index];
+        ph.volume = p.synth.volume;
+    } else {
+        const Emission& em = p.emissions[source];
+        ph.photon_id = em.first_photon_id + local;
+        ph.event_id = em.event_id;
+        ph.volume = em.volume;
+    }
+    ph.step = 0;
+    ph.material = kInvalid;
+    ph.grid_i = kInvalid;
+    return a.w == a.w;
+}
+
 }  // namespace
 
 // kLdsMode: which scene parts a launch guarantees are staged in LDS
@@ -258,7 +298,10 @@ __device__ __forceinline__ void stage_scene(const KernelParams& p, float4* lds, 
     }
 }
 
-template <int kSource, bool kPersistent, int kLdsMode, int kGeom>
+// kPregen (persistent form): idle lanes load the start states that
+// generate_kernel wrote for their photons (KernelParams::starts) instead of
+// generating them at refill.
+template <int kSource, bool kPersistent, int kLdsMode, int kGeom, bool kPregen = false>
 __device__ __forceinline__ void transport_body(const KernelParams& p) {
     __shared__ uint32_t s_hist[kHistSlots];
     __shared__ float4 s_stage[kMaxWavesPerBlock][3 * kStageCap];
@@ -269,6 +312,7 @@ __device__ __forceinline__ void transport_body(const KernelParams& p) {
     }
     SceneView sv = p.scene;
     stage_scene<kLdsMode, kGeom>(p, s_scene, sv);
+    G4GO_REGION_ZERO();
     __syncthreads();
     float4* const stage = s_stage[threadIdx.x >> 6];
     uint32_t* const stage_src = s_stage_src[threadIdx.x >> 6];
@@ -310,6 +354,7 @@ __device__ __forceinline__ void transport_body(const KernelParams& p) {
         uint64_t idle = __ballot(!alive);
         uint32_t n_idle = static_cast<uint32_t>(__popcll(idle));
         if (!kPersistent || n_idle >= p.refill_threshold || n_idle == 64u) {
+            G4GO_REGION_BEGIN(kRegionRefill);
             while (n_idle > 0u) {
                 if (it_next >= it_end) {
                     if (done) {
@@ -337,21 +382,30 @@ __device__ __forceinline__ void transport_body(const KernelParams& p) {
                 const bool fill = !alive && rank < take;
                 flush_event_group(es, my_event, st, fill && my_event != it_event && my_event != kInvalid);
                 if (fill) {
+                    G4GO_REGION_BEGIN(kRegionGenerate);
                     my_event = it_event;
-                    generate<kSource>(ph, p, sv, it_event, it_source, it_first + it_next + rank);
-                    ph.material = kInvalid;
                     my_launch = p.source_photon_offset[it_source] + it_first + it_next + rank;
-                    if (start_photon(ph)) {
+                    bool ok;
+                    if (kPregen) {
+                        ok = load_start<kSource>(ph, p, it_event, it_source, it_first + it_next + rank, my_launch);
+                    } else {
+                        generate<kSource>(ph, p, sv, it_event, it_source, it_first + it_next + rank);
+                        ph.material = kInvalid;
+                        ok = start_photon(ph);
+                    }
+                    if (ok) {
                         alive = true;
                     } else {
                         ++st.invalid;
                     }
+                    G4GO_REGION_END(kRegionGenerate);
                 }
                 it_next += take;
                 wave_photons += take;
                 idle = __ballot(!alive);
                 n_idle = static_cast<uint32_t>(__popcll(idle));
             }
+            G4GO_REGION_END(kRegionRefill);
         }
         const uint64_t alive_lanes = __ballot(alive);
         if (alive_lanes == 0ull) {
@@ -365,6 +419,7 @@ __device__ __forceinline__ void transport_body(const KernelParams& p) {
         // ---- one transport step ----
         int status = kAlive;
         uint32_t sensor = kInvalid;
+        G4GO_REGION_BEGIN(kRegionStep);
         if (alive) {
             bool zero = false;
             status = transport_step<false, kGeom>(ph, sv, p.rng, p.cfg, zero, sensor, nullptr);
@@ -374,8 +429,10 @@ __device__ __forceinline__ void transport_body(const KernelParams& p) {
                 status = -1;  // truncated
             }
         }
+        G4GO_REGION_END(kRegionStep);
 
         // ---- fused hit compaction: ballot + prefix into the wave's LDS stage ----
+        G4GO_REGION_BEGIN(kRegionCompact);
         const uint64_t det = __ballot(status == kDetected);
         if (det != 0ull) {
             const uint32_t n_det = static_cast<uint32_t>(__popcll(det));
@@ -410,6 +467,7 @@ __device__ __forceinline__ void transport_body(const KernelParams& p) {
                 ++st.truncated;
             }
         }
+        G4GO_REGION_END(kRegionCompact);
     }
 
     if (staged != 0u) {
@@ -426,16 +484,40 @@ __device__ __forceinline__ void transport_body(const KernelParams& p) {
             atomicAdd(p.histogram + i, static_cast<unsigned long long>(s_hist[i]));
         }
     }
+    G4GO_REGION_FLUSH();
 }
 
 // Persistent form: four waves per SIMD, i.e. at most 128 VGPRs.  Left alone
 // the compiler schedules the body into ~140 registers (three waves); the
 // fourth wave hides more of the dependent scene-load latency of a step than
 // the few spills it forces cost.
-template <int kSource, int kLdsMode, int kGeom>
+template <int kSource, int kLdsMode, int kGeom, bool kPregen>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
 transport_kernel_persistent(KernelParams p) {
-    transport_body<kSource, true, kLdsMode, kGeom>(p);
+    transport_body<kSource, true, kLdsMode, kGeom, kPregen>(p);
+}
+
+// Generation pre-pass of the persistent form: wave w generates the photons of
+// work items w, w + waves, ... (64 photons at a time, one per lane) and
+// writes their start states in launch order.  Generation reads only the
+// source and, for emission records, the material tables in global memory.
+template <int kSource>
+__global__ void __launch_bounds__(256) generate_kernel(KernelParams p, float4* starts) {
+    const SceneView sv = p.scene;
+    const uint32_t lane = lane_id();
+    const uint32_t n_sources = uniform(p.header[0]);
+    const uint32_t chunk = uniform(p.header[1]);
+    const uint32_t n_items = uniform(p.item_offsets[n_sources]);
+    const uint32_t waves = gridDim.x * (blockDim.x >> 6);
+    for (uint32_t c = uniform(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)); c < n_items; c += waves) {
+        const ItemRef item = locate_item<kSource>(p, c, n_sources, chunk);
+        const uint32_t base = uniform(p.source_photon_offset[item.source_index]);
+        for (uint32_t k = lane; k < item.count; k += 64u) {
+            Photon g;
+            generate<kSource>(g, p, sv, item.event_index, item.source_index, item.first + k);
+            write_start(starts + static_cast<size_t>(kStartQuads) * (base + item.first + k), g);
+        }
+    }
 }
 
 template <int kSource, int kLdsMode, int kGeom>
@@ -445,8 +527,12 @@ __global__ void __launch_bounds__(256) transport_kernel_plain(KernelParams p) {
 
 template <int kSource, int kLdsMode, int kGeom = kGeomBoxes>
 void launch_transport_form(const KernelParams& p, bool persistent, int block, int grid, size_t lds, hipStream_t stream) {
-    if (persistent) {
-        hipLaunchKernelGGL((transport_kernel_persistent<kSource, kLdsMode, kGeom>), dim3(grid), dim3(block), lds,
+    // the pre-generated instances exist for box and convex-mesh scenes
+    if (persistent && p.starts != nullptr && kGeom != kGeomTriangles) {
+        hipLaunchKernelGGL((transport_kernel_persistent<kSource, kLdsMode, kGeom, kGeom != kGeomTriangles>), dim3(grid),
+                           dim3(block), lds, stream, p);
+    } else if (persistent) {
+        hipLaunchKernelGGL((transport_kernel_persistent<kSource, kLdsMode, kGeom, false>), dim3(grid), dim3(block), lds,
                            stream, p);
     } else {
         hipLaunchKernelGGL((transport_kernel_plain<kSource, kLdsMode, kGeom>), dim3(grid), dim3(block), lds, stream,
@@ -496,6 +582,15 @@ void launch_transport_source(const KernelParams& p, bool persistent, int block, 
     }
 }
 
+int launch_generate(const KernelParams& p, int source, float4* starts, int grid, hipStream_t stream) {
+    if (source == kSourceSynthetic) {
+        hipLaunchKernelGGL(generate_kernel<kSourceSynthetic>, dim3(grid), dim3(256), 0, stream, p, starts);
+    } else {
+        hipLaunchKernelGGL(generate_kernel<kSourceEmission>, dim3(grid), dim3(256), 0, stream, p, starts);
+    }
+    return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
 __global__ void zero_kernel(uint32_t* ptr, size_t words) {
     for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; i < words;
          i += static_cast<size_t>(gridDim.x) * blockDim.x) {
@@ -541,7 +636,7 @@ int launch_zero(void* ptr, size_t bytes, hipStream_t stream) {
 
 int launch_transport(const KernelParams& p, int source, bool persistent, int block, int grid, hipStream_t stream) {
     const size_t lds = transport_lds_bytes(p);
-    if (lds > kLdsSceneBytes) {
+    if (lds > kLdsSceneBytes || (p.starts != nullptr && geom_level(p.scene) == kGeomTriangles)) {
         return 1;
     }
     if (source == kSourceSynthetic) {
@@ -986,6 +1081,17 @@ int launch_export_counters(const unsigned long long* counters, unsigned long lon
     hipLaunchKernelGGL(export_counters_kernel, dim3(1), dim3(64), 0, stream, counters, out);
     return hipGetLastError() == hipSuccess ? 0 : 1;
 }
+
+#if defined(G4GO_REGION_PROFILE)
+// Counter build only: read and clear the region profile (kRegionCount x 3 u64).
+extern "C" int g4go_region_profile_read(unsigned long long* out) {
+    return hipMemcpyFromSymbol(out, HIP_SYMBOL(g4go_region_global), sizeof(g4go_region_global)) == hipSuccess ? 0 : 1;
+}
+extern "C" int g4go_region_profile_clear() {
+    static const unsigned long long zeros[kRegionCount][3] = {};
+    return hipMemcpyToSymbol(HIP_SYMBOL(g4go_region_global), zeros, sizeof(zeros)) == hipSuccess ? 0 : 1;
+}
+#endif
 
 int launch_order_hits(const OrderParams& p, hipStream_t stream) {
     const uint32_t n_words = (p.n_photons + 31u) / 32u;

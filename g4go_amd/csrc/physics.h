@@ -112,12 +112,10 @@ G4GO_HD float g4_logf(float x) {
     }
     e += static_cast<int>(b >> 23) - 126;
     float m = u2f((b & 0x007fffffu) | 0x3f000000u);  // [0.5, 1)
-    if (m < 0.70710678118654752f) {
-        e -= 1;
-        m = (m + m) - 1.0f;
-    } else {
-        m = m - 1.0f;
-    }
+    // branch-free: the lanes of a wave fall on both sides
+    const bool low = m < 0.70710678118654752f;
+    e -= low ? 1 : 0;
+    m = low ? (m + m) - 1.0f : m - 1.0f;
     const float z = m * m;
     float y = 7.0376836292e-2f;
     y = fmaf(y, m, -1.1514610310e-1f);
@@ -181,20 +179,14 @@ G4GO_HD void g4_sincos_turn(float u, float& s, float& c) {
     const float sn = fmaf(sp * z, x, x);
     const float cp = fmaf(fmaf(2.443315711809948e-5f, z, -1.388731625493765e-3f), z, 4.166664568298827e-2f);
     const float cs = fmaf(cp * z, z, fmaf(-0.5f, z, 1.0f));
+    // quadrant by selects, not branches (the lanes of a wave fall in every
+    // quadrant): (s, c) = (sn, cs), (cs, -sn), (-sn, -cs), (-cs, sn)
     const int quadrant = static_cast<int>(q) & 3;
-    if (quadrant == 0) {
-        s = sn;
-        c = cs;
-    } else if (quadrant == 1) {
-        s = cs;
-        c = -sn;
-    } else if (quadrant == 2) {
-        s = -sn;
-        c = -cs;
-    } else {
-        s = -cs;
-        c = sn;
-    }
+    const bool swap = (quadrant & 1) != 0;
+    const float a = swap ? cs : sn;
+    const float b = swap ? sn : cs;
+    s = (quadrant & 2) != 0 ? -a : a;
+    c = ((quadrant + 1) & 2) != 0 ? -b : b;
 }
 
 // ---------------------------------------------------------------------------

@@ -6,6 +6,7 @@
 #pragma once
 
 #include "physics.h"
+#include "region_prof.h"
 #include "scene.h"
 
 namespace g4go {
@@ -1164,6 +1165,7 @@ G4GO_HD int transport_step(Photon& ph, const SceneView& sc, const Rng& rng, cons
     if (ph.volume >= sc.n_volumes) {
         return kInvalidState;
     }
+    G4GO_REGION_BEGIN(kRegionGeometry);
     if (ph.grid_i == kInvalid && sc.hot_count != 0) {
         grid_locate(sc.pool + sc.hot_offset, sc.hot_count, ph.energy, ph.grid_i, ph.grid_f);
     }
@@ -1225,6 +1227,7 @@ G4GO_HD int transport_step(Photon& ph, const SceneView& sc, const Rng& rng, cons
     if (dist <= cfg.eps) {
         zero_step = true;
     }
+    G4GO_REGION_END(kRegionGeometry);
 
     const U4 w = rng.block(ph.photon_id, ph.event_id, ph.step, kProcStep, 0);
 
@@ -1240,13 +1243,16 @@ G4GO_HD int transport_step(Photon& ph, const SceneView& sc, const Rng& rng, cons
                 return kAbsorbedBulk;
             }
             ph.pos = add(ph.pos, scale(ph.dir, d_int));
+            G4GO_REGION_BEGIN(kRegionRayleigh);
             rayleigh_scatter(ph, rng);
+            G4GO_REGION_END(kRegionRayleigh);
             if (kTrace) {
                 trace->event = kTraceRayleigh;
             }
             return kAlive;
         }
     }
+    G4GO_REGION_BEGIN(kRegionSurface);
     ph.time += static_cast<double>(dist * ph.inv_velocity);
     if (kTrace) {
         trace->segment = dist;
@@ -1341,9 +1347,11 @@ G4GO_HD int transport_step(Photon& ph, const SceneView& sc, const Rng& rng, cons
                                                    : (outcome == kDirectTransmit ? kTraceTransmit : kTraceNone);
         }
         if (outcome == kAbsorb) {
+            G4GO_REGION_BEGIN(kRegionAbsorbDetect);
             absorbed = true;
             const float eff = clamp01(prop_eval_ph(sc.pool, s.efficiency, ph, 0.0f));
             detected = evaluate_absorption(eff, nvol.sensor != kInvalid || vol.sensor != kInvalid, u01(w.w)) == kDetect;
+            G4GO_REGION_END(kRegionAbsorbDetect);
         } else if (outcome == kDirectTransmit) {
             transmitted = true;
         } else if (s.type == kDielectricDielectric) {
@@ -1411,6 +1419,7 @@ G4GO_HD int transport_step(Photon& ph, const SceneView& sc, const Rng& rng, cons
         }
     }
 
+    G4GO_REGION_END(kRegionSurface);
     if (detected) {
         sensor_out = nvol.sensor != kInvalid ? nvol.sensor : vol.sensor;
         if (sensor_out == kInvalid) {

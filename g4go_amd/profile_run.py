@@ -7,6 +7,12 @@
     python -m g4go_amd.profile_run --workload lguide    # triangle solid: L-shaped light guide
     python -m g4go_amd.profile_run --workload ring      # 48 turned crystals (convex meshes)
     python -m g4go_amd.profile_run --copy-bytes 1073741824   # calibration copy
+    python -m g4go_amd.profile_run --workload crystal --regions   # region profile (counter build)
+
+--regions loads the region-profile counter build (g4go_amd/_prof, made by
+`python -m g4go_amd.build profile`) instead of the product library and
+prints, per region of the transport kernel (csrc/region_prof.h), its share of
+the wave cycles and its lane utilisation.
 
 The launch form is the cached autotuner result for the device when present,
 else the default persistent form.  --copy-bytes additionally runs one device
@@ -33,8 +39,16 @@ def main(argv=None) -> int:
     ap.add_argument("--form", choices=("tuned", "plain"), default="tuned")
     ap.add_argument("--copy-bytes", type=int, default=0)
     ap.add_argument("--repeat", type=int, default=1)
+    ap.add_argument("--regions", action="store_true")
+    ap.add_argument("--launch", default="",
+                    help="persistent form block,blocks_per_cu,chunk,refill[,pregenerate] instead of the cached one")
     args = ap.parse_args(argv)
 
+    from . import abi
+    if args.regions:
+        from .build import PROFILE_LIBRARY
+
+        abi.LIBRARY_PATH = PROFILE_LIBRARY
     from . import autotune, workloads
     from .transport import HipTransportHost, LaunchForm, PhotonTransportConfig
 
@@ -70,6 +84,10 @@ def main(argv=None) -> int:
     host = HipTransportHost(PhotonTransportConfig(seed=args.seed, sort_hits=False))
     form = autotune.cached(autotune.device_key(host), "crystal-synthetic") or LaunchForm(persistent=True, block_size=256, chunk=256,
                                                                       refill_threshold=16, blocks_per_cu=4)
+    if args.launch:
+        v = [int(x) for x in args.launch.split(",")]
+        form = LaunchForm(persistent=True, block_size=v[0], blocks_per_cu=v[1], chunk=v[2], refill_threshold=v[3],
+                          pregenerate=bool(v[4]) if len(v) > 4 else False)
     if args.form == "plain":
         form = LaunchForm(persistent=False)
     if args.workload == "uneven":
@@ -79,15 +97,44 @@ def main(argv=None) -> int:
         counts = np.array([args.photons], np.uint32)
     host.set_launch(form)
     ev = np.arange(counts.size, dtype=np.uint32)
+    lib = abi.load()
     for _ in range(args.repeat):
+        if args.regions:
+            lib.g4go_region_profile_clear()
         host.enqueue_synthetic(cs.scene, src, ev, counts)
         out, _ = host.complete_device()
         st = out.statistics
-        print(json.dumps({"workload": args.workload, "photons": int(counts.astype(np.int64).sum()),
-                          "kernel_ms": st["kernel_ms"], "steps": st["total_steps"], "detected": st["detected"],
-                          "launch": form.__dict__}), flush=True)
+        line = {"workload": args.workload, "photons": int(counts.astype(np.int64).sum()),
+                "kernel_ms": st["kernel_ms"], "steps": st["total_steps"], "detected": st["detected"],
+                "launch": form.__dict__}
+        if args.regions:
+            line["regions"] = region_table(lib, st["total_steps"])
+        print(json.dumps(line), flush=True)
     host.close()
     return 0
+
+
+REGIONS = ("refill", "generate", "step", "compact", "geometry", "rayleigh", "surface", "absorb_detect")
+
+
+def region_table(lib, steps: int) -> dict:
+    """Per region: share of the wave cycles of the four top-level regions, wave entries per 1000
+    lane-steps, and lane utilisation (active lanes / 64 per entry)."""
+    import ctypes as C
+
+    raw = (C.c_ulonglong * (3 * len(REGIONS)))()
+    if lib.g4go_region_profile_read(raw) != 0:
+        raise RuntimeError("region profile read failed")
+    v = np.array(raw[:], dtype=np.float64).reshape(len(REGIONS), 3)
+    top = v[[0, 2, 3], 0].sum()
+    out = {}
+    for k, name in enumerate(REGIONS):
+        cycles, entries, lanes = v[k]
+        out[name] = {"cycle_share": cycles / top if top else 0.0,
+                     "entries_per_1000_steps": 1000.0 * entries / max(steps, 1),
+                     "lane_utilisation": lanes / (64.0 * entries) if entries else 0.0,
+                     "cycles_per_entry": cycles / entries if entries else 0.0}
+    return out
 
 
 if __name__ == "__main__":

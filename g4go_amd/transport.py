@@ -80,7 +80,8 @@ class PhotonTransportConfig:
 class LaunchForm:
     """Launch form of a handle (g4go_launch).  The defaults are those of a new
     handle: the persistent kernel with 256-thread blocks, 256-photon work
-    items, refill at 24 idle lanes and 8 blocks per CU.  The plain form
+    items, refill at 24 idle lanes, 8 blocks per CU, photons generated at
+    refill (pregenerate off).  The plain form
     (`LaunchForm(persistent=False)`) runs one wave per item of at most 64
     photons and ignores chunk and refill_threshold."""
 
@@ -91,10 +92,11 @@ class LaunchForm:
     blocks_per_cu: int = 8
     per_event: bool = False
     graph: bool = False
+    pregenerate: bool = False
 
     def to_abi(self) -> abi.Launch:
         return abi.Launch(int(self.persistent), self.block_size, self.chunk, self.refill_threshold,
-                          self.blocks_per_cu, int(self.per_event), int(self.graph))
+                          self.blocks_per_cu, int(self.per_event), int(self.graph), int(self.pregenerate))
 
 
 @dataclass
@@ -151,7 +153,7 @@ class HipTransportHost:
         form = abi.Launch()
         self._check(self._lib.g4go_get_launch(self._h, C.byref(form)))
         return LaunchForm(bool(form.persistent), form.block_size, form.chunk, form.refill_threshold,
-                          form.blocks_per_cu, bool(form.per_event), bool(form.graph))
+                          form.blocks_per_cu, bool(form.per_event), bool(form.graph), bool(form.pregenerate))
 
     # ---- the reference interface ----
     def prepare_scene(self, scene: Scene) -> None:
@@ -269,7 +271,9 @@ class HipTransportHost:
         stats = {name: getattr(st, name) for name, _ in abi.BatchStats._fields_}
         stats["n_hits"] = n_hits
         perf = {k: stats[k] for k in ("kernel_ms", "host_to_device_ms", "device_to_host_ms", "host_to_device_bytes",
-                                      "device_to_host_bytes", "total_steps")}
+                                      "device_to_host_bytes", "total_steps", "device_memset_ms", "device_order_ms",
+                                      "metadata_to_host_ms", "metadata_to_host_bytes", "hits_to_host_ms",
+                                      "hits_to_host_bytes")}
         return PhotonTransportOutput(hits, stats, events, hist, perf)
 
     def device_info(self) -> tuple[str, int]:

@@ -77,6 +77,10 @@ typedef struct g4go_launch {
     uint32_t blocks_per_cu;    /* persistent grid = blocks_per_cu * CU count */
     uint32_t per_event;        /* plain form: one launch per event */
     uint32_t graph;            /* persistent form: replay a captured hipGraph */
+    uint32_t pregenerate;      /* persistent form: 1 = a generation pass on full waves writes every
+                                  photon's start state (48 B in HBM) before the transport, whose idle
+                                  lanes load them at refill; 0 = idle lanes generate their photons at
+                                  refill (always so in scenes with triangle solids) */
 } g4go_launch;
 
 /* A property table: `count` (energy eV, value) pairs with ascending energies;
@@ -238,6 +242,13 @@ typedef struct g4go_batch_stats {
     double device_to_host_ms;
     uint64_t host_to_device_bytes;
     uint64_t device_to_host_bytes;
+    /* per-stage timers of PhotonTransportPerformance (OptiXTransportHost.cpp:845-876), HIP events */
+    double device_memset_ms;         /* clearing the counters and event accumulators (0 when a graph replays them) */
+    double device_order_ms;          /* launch-order pass over the hits (deviceHitCompactionMs) */
+    double metadata_to_host_ms;      /* counters, histogram and event statistics to the host */
+    uint64_t metadata_to_host_bytes;
+    double hits_to_host_ms;          /* the hit records to the host (0 for g4go_complete_device) */
+    uint64_t hits_to_host_bytes;
 } g4go_batch_stats;
 
 /* Results of one completed batch.  Arrays are owned by the handle and stay
@@ -259,7 +270,7 @@ void g4go_destroy(g4go_transport* handle);
 const char* g4go_last_error(const g4go_transport* handle);
 /* Launch form of the handle's batches.  A new handle uses the persistent form
  * {persistent 1, block 256, chunk 256, refill 24, 8 blocks per CU, per_event
- * 0, graph 0}, the form tuned on MI355X for crystal batches. */
+ * 0, graph 0, pregenerate 0}, the form tuned on MI355X for crystal batches. */
 g4go_status g4go_set_launch(g4go_transport* handle, const g4go_launch* launch);
 g4go_status g4go_get_launch(const g4go_transport* handle, g4go_launch* launch);
 

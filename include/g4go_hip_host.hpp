@@ -32,6 +32,7 @@
 
 #include <algorithm>
 #include <array>
+#include <chrono>
 #include <cmath>
 #include <cstddef>
 #include <cstdint>
@@ -238,14 +239,21 @@ struct PhotonTransportEventStatistics {
     PhotonTransportStatistics statistics{};
 };
 
-struct PhotonTransportPerformance {  // the members this backend fills
+struct PhotonTransportPerformance {  // the members the backend fills (OptiXTransportHost.cpp:845-876)
     double hostToDeviceMs{};
     std::uint64_t hostToDeviceBytes{};
+    double deviceMemsetMs{};
     double optiXKernelMs{};
+    double deviceHitCompactionMs{};
     double deviceToHostMs{};
+    double deviceMetadataToHostMs{};
+    std::uint64_t deviceMetadataToHostBytes{};
+    double deviceHitsToHostMs{};
     std::uint64_t deviceHitsToHostBytes{};
     double hostHitCompactionMs{};
     std::uint64_t totalBounceCount{};
+    std::uint64_t coincidentCandidateTraceCount{};
+    std::uint64_t coincidentCandidateHitCount{};
 };
 
 struct PhotonTransportOutput {
@@ -584,6 +592,7 @@ public:
         const std::uint32_t counts = bit(Field::Detected) | bit(Field::Absorbed) | bit(Field::Escaped) |
                                      bit(Field::Truncated) | bit(Field::MaxBounce) | bit(Field::InvalidState) |
                                      bit(Field::ZeroStep);
+        const auto conversion0 = std::chrono::steady_clock::now();
         out.detections.reserve(r.n_hits);
         for (std::uint64_t i = 0; i < r.n_hits; ++i) {
             const g4go_hit& h = r.hits[i];
@@ -597,6 +606,8 @@ public:
             d.sensorID = h.sensor_id;
             out.detections.push_back(d);
         }
+        const double conversion_ms =
+            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - conversion0).count();
         auto& st = out.statistics;
         st.capturedCount = r.stats.captured;
         st.detectedCount = r.stats.detected;
@@ -628,12 +639,20 @@ public:
             es.statistics.validFields = (owns ? bit(Field::Captured) : 0u) | (batch.any() ? counts : 0u);
             out.eventStatistics.push_back(es);
         }
+        // the reference's stage timers; this backend has no coincident-face re-trace, so its
+        // coincidentCandidate counts stay 0
         auto& perf = out.performance;
         perf.hostToDeviceMs = r.stats.host_to_device_ms;
         perf.hostToDeviceBytes = r.stats.host_to_device_bytes;
+        perf.deviceMemsetMs = r.stats.device_memset_ms;
         perf.optiXKernelMs = r.stats.kernel_ms;
-        perf.deviceToHostMs = r.stats.device_to_host_ms;
-        perf.deviceHitsToHostBytes = r.n_hits * sizeof(g4go_hit);
+        perf.deviceHitCompactionMs = r.stats.device_order_ms;
+        perf.deviceMetadataToHostMs = r.stats.metadata_to_host_ms;
+        perf.deviceMetadataToHostBytes = r.stats.metadata_to_host_bytes;
+        perf.deviceHitsToHostMs = r.stats.hits_to_host_ms;
+        perf.deviceHitsToHostBytes = r.stats.hits_to_host_bytes;
+        perf.deviceToHostMs = perf.deviceMetadataToHostMs + perf.deviceHitsToHostMs;
+        perf.hostHitCompactionMs = conversion_ms;
         perf.totalBounceCount = r.stats.total_steps;
         return out;
     }

@@ -7,6 +7,8 @@
 //                            Cerenkov and direct emission records through
 //                            the three-method interface and writes the
 //                            records and the output to OUT.bin (layout below)
+//   shim_test bench SLOTS BATCHES [CHUNK BLOCKS_PER_CU REFILL PREGENERATE]
+//                            throughput of the drop-in path (JSON line)
 //   shim_test gpu-mesh OUT.bin  the same with the crystal a tapered mesh
 //                            (3x3 mm front face, 4x4 mm back face, turned
 //                            into place by its transform), passed on as a
@@ -22,6 +24,7 @@
 // maxBounce, invalidState, zeroStep, totalBounceCount.
 #include "g4go_hip_host.hpp"
 
+#include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <functional>
@@ -415,6 +418,130 @@ int run_gpu(const char* path, bool tapered) {
     return 0;
 }
 
+// Electron-like scintillation records (the shape of the 1 MeV electron events of
+// the single-crystal workload): per event 20 records along the crystal axis,
+// ~1500 photons each, decay 40 ns.
+void electron_batch(const CrystalScene& c, std::uint32_t first_event, std::uint32_t n_events,
+                    std::vector<OpticalEmission>& em, std::vector<std::uint32_t>& events,
+                    std::vector<std::uint32_t>& index) {
+    for (std::uint32_t e = 0; e < n_events; ++e) {
+        const std::uint32_t event_id = first_event + e;
+        events.push_back(event_id);
+        std::uint32_t next_photon = 0;
+        float x = 0.0F, y = 0.0F, z = -9.999F;
+        for (int k = 0; k < 20; ++k) {
+            OpticalEmission r{};
+            const float dx = 0.2F * (lcg() - 0.5F), dy = 0.2F * (lcg() - 0.5F);
+            r.positionMm = {x, y, z};
+            r.timeNs = 0.0002F * k;
+            r.direction = {dx, dy, 1.0F};
+            r.stepLengthMm = 0.065F;
+            r.stepDeltaMm = {0.065F * dx, 0.065F * dy, 0.065F};
+            r.preVelocityMmPerNs = 281.8F;
+            r.deltaVelocityMmPerNs = -2.8F;
+            r.eventID = event_id;
+            r.firstPhotonID = next_photon;
+            r.volumeID = c.crystal;
+            r.materialID = 1;
+            r.type = OpticalEmissionType::Scintillation;
+            r.decayTimeNs = 40.0F;
+            r.charge = -1.0F;
+            r.photonCount = 1400 + static_cast<std::uint32_t>(200.0F * lcg());
+            next_photon += r.photonCount;
+            em.push_back(r);
+            index.push_back(e);
+            x += r.stepDeltaMm[0];
+            y += r.stepDeltaMm[1];
+            z += r.stepDeltaMm[2];
+        }
+    }
+}
+
+void accumulate(PhotonTransportPerformance& sum, const PhotonTransportPerformance& p) {
+    sum.hostToDeviceMs += p.hostToDeviceMs;
+    sum.deviceMemsetMs += p.deviceMemsetMs;
+    sum.optiXKernelMs += p.optiXKernelMs;
+    sum.deviceHitCompactionMs += p.deviceHitCompactionMs;
+    sum.deviceMetadataToHostMs += p.deviceMetadataToHostMs;
+    sum.deviceHitsToHostMs += p.deviceHitsToHostMs;
+    sum.deviceToHostMs += p.deviceToHostMs;
+    sum.hostHitCompactionMs += p.hostHitCompactionMs;
+}
+
+// Throughput of the drop-in path: batches of 33 electron events (~1e6
+// photons) through the three-method interface, hits returned to the host in
+// launch order and converted to PhotonDetection records, with up to `slots`
+// batches in flight.  Prints one JSON line: photons/s and the mean per-batch
+// performance fields.
+int run_bench(int slots, int batches, const g4go_launch* launch) {
+    CrystalScene c = crystal_scene();
+    const int warm = 3;
+    std::vector<std::vector<OpticalEmission>> em(batches + warm);
+    std::vector<std::vector<std::uint32_t>> ev(batches + warm), idx(batches + warm);
+    std::uint64_t photons = 0;
+    for (int b = 0; b < batches + warm; ++b) {
+        electron_batch(c, 1000u * static_cast<std::uint32_t>(b), 33, em[b], ev[b], idx[b]);
+        if (b >= warm) {
+            for (const auto& r : em[b]) {
+                photons += r.photonCount;
+            }
+        }
+    }
+    PhotonTransportConfig cfg;
+    cfg.seed = 99;
+    g4go_hip::HipTransportHost host{cfg, launch, static_cast<std::uint32_t>(slots)};
+    host.PrepareScene(c.scene);
+    PhotonTransportPerformance sum{};
+    std::uint64_t hits = 0;
+    double enqueue_ms = 0.0, complete_ms = 0.0;
+    const auto ms_since = [](std::chrono::steady_clock::time_point t) {
+        return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t).count();
+    };
+    auto complete = [&](bool record) {
+        const auto t = std::chrono::steady_clock::now();
+        PhotonTransportOutput out = host.CompleteOldestBatch();
+        if (record) {
+            complete_ms += ms_since(t);
+            hits += out.detections.size();
+            accumulate(sum, out.performance);
+        }
+    };
+    auto run = [&](int b0, int b1, bool record) {
+        int in_flight = 0;
+        for (int b = b0; b < b1; ++b) {
+            if (in_flight == slots) {
+                complete(record);
+                --in_flight;
+            }
+            const auto t = std::chrono::steady_clock::now();
+            host.EnqueueEmissions(c.scene, PhotonTransportBatch{em[b], ev[b], idx[b]});
+            if (record) {
+                enqueue_ms += ms_since(t);
+            }
+            ++in_flight;
+        }
+        while (in_flight > 0) {
+            complete(record);
+            --in_flight;
+        }
+    };
+    run(0, warm, false);
+    const auto t0 = std::chrono::steady_clock::now();
+    run(warm, warm + batches, true);
+    const double seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    const double n = batches;
+    std::printf("{\"slots\": %d, \"batches\": %d, \"photons_per_batch\": %.1f, \"photons_per_s\": %.6g, "
+                "\"ms_per_batch\": %.4f, \"hits_per_batch\": %.1f, \"enqueue_ms\": %.4f, \"complete_ms\": %.4f, "
+                "\"host_to_device_ms\": %.4f, \"device_memset_ms\": %.4f, \"kernel_ms\": %.4f, "
+                "\"device_hit_order_ms\": %.4f, \"metadata_to_host_ms\": %.4f, \"hits_to_host_ms\": %.4f, "
+                "\"host_hit_conversion_ms\": %.4f}\n",
+                slots, batches, static_cast<double>(photons) / n, static_cast<double>(photons) / seconds,
+                1e3 * seconds / n, static_cast<double>(hits) / n, enqueue_ms / n, complete_ms / n,
+                sum.hostToDeviceMs / n, sum.deviceMemsetMs / n, sum.optiXKernelMs / n, sum.deviceHitCompactionMs / n,
+                sum.deviceMetadataToHostMs / n, sum.deviceHitsToHostMs / n, sum.hostHitCompactionMs / n);
+    return 0;
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -428,10 +555,21 @@ int main(int argc, char** argv) {
         if (argc == 3 && std::string(argv[1]) == "gpu-mesh") {
             return run_gpu(argv[2], true);
         }
+        if ((argc == 4 || argc == 8) && std::string(argv[1]) == "bench") {
+            // optional persistent form: CHUNK BLOCKS_PER_CU REFILL PREGENERATE (else the handle's default)
+            g4go_launch form{1, 256, 256, 24, 8, 0, 0, 0};
+            if (argc == 8) {
+                form.chunk = static_cast<std::uint32_t>(std::atoi(argv[4]));
+                form.blocks_per_cu = static_cast<std::uint32_t>(std::atoi(argv[5]));
+                form.refill_threshold = static_cast<std::uint32_t>(std::atoi(argv[6]));
+                form.pregenerate = static_cast<std::uint32_t>(std::atoi(argv[7]));
+            }
+            return run_bench(std::atoi(argv[2]), std::atoi(argv[3]), argc == 8 ? &form : nullptr);
+        }
     } catch (const std::exception& e) {
         std::printf("FAIL: uncaught %s: %s\n", typeid(e).name(), e.what());
         return 1;
     }
-    std::printf("usage: shim_test cpu | shim_test gpu OUT.bin | shim_test gpu-mesh OUT.bin\n");
+    std::printf("usage: shim_test cpu | shim_test gpu OUT.bin | shim_test gpu-mesh OUT.bin | shim_test bench SLOTS BATCHES [CHUNK BLOCKS_PER_CU REFILL PREGENERATE]\n");
     return 2;
 }
