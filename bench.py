@@ -223,11 +223,15 @@ def run_array(seed, steps, warmup, plain, retune=False):
            "adjoint_kernel_ms": adj["kernel_ms"], "adjoint_over_forward_kernel": adj["kernel_ms"] / b_kms,
            "replay_mismatches": adj["mismatched"], "launch": launch.__dict__}
     if plain:
+        p_wall, p_kms, p_out = run_emission_batches(host, cs.scene, em, ev, idx, LaunchForm(persistent=False),
+                                                    steps, warmup)
         e_wall, e_kms, e_out = run_emission_batches(host, cs.scene, em, ev, idx,
                                                     LaunchForm(persistent=False, per_event=True), steps, warmup)
-        assert e_out.statistics["detected"] == b_out.statistics["detected"]
-        res.update({"plain_per_event_photons_per_s": photons / e_wall, "plain_per_event_kernel_ms": e_kms,
-                    "batched_over_plain": e_wall / b_wall})
+        assert e_out.statistics["detected"] == b_out.statistics["detected"] == p_out.statistics["detected"]
+        res.update({"plain_batch_photons_per_s": photons / p_wall, "plain_batch_kernel_ms": p_kms,
+                    "batched_over_plain_batch": p_wall / b_wall,
+                    "plain_per_event_photons_per_s": photons / e_wall, "plain_per_event_kernel_ms": e_kms,
+                    "batched_over_plain_per_event": e_wall / b_wall})
     host.close()
     return res
 
@@ -312,6 +316,25 @@ def run_c1(seed, steps, warmup, cpu, cpu_events, retune=False):
     return res
 
 
+def run_dropin_cpp(batches=40):
+    """The drop-in path through the C++ shim (tests/cpp/shim_test bench): batches
+    of 33 electron events (~1e6 photons) through PrepareScene / EnqueueEmissions /
+    CompleteOldestBatch, hits returned as PhotonDetection records in launch order,
+    with one batch in flight (the reference's single slot) and with two."""
+    import subprocess
+
+    exe = os.path.join(ROOT, "tests", "cpp", "shim_test")
+    if not os.path.isfile(exe):
+        return None
+    res = {}
+    for slots in (1, 2):
+        proc = subprocess.run([exe, "bench", str(slots), str(batches)], capture_output=True, text=True, timeout=120)
+        if proc.returncode != 0:
+            return {"error": proc.stdout[-500:] + proc.stderr[-500:]}
+        res[f"slots_{slots}"] = json.loads(proc.stdout.strip().splitlines()[-1])
+    return res
+
+
 def run_host_batches(seed, launch, n_batches=40, events_per_batch=33):
     """The drop-in path at the reference scheduler's batch size (~1e6 photons,
     PhotonTransportConfig::targetPhotonsPerBatch): 1 MeV electron events in the
@@ -351,16 +374,43 @@ def run_host_batches(seed, launch, n_batches=40, events_per_batch=33):
     return res
 
 
+# Host cores one GPU's job may use on the GPU host (its CPU share); the
+# oracle's thread pool is sized to it, whatever the machine's core count is.
+HOST_CORE_SHARE = 16
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(cs, src, seed, sample):
+    """The FP32 oracle on the host: the first `sample` photons of the headline
+    workload at all the cores of the job's host share and at 6 threads (the
+    reference's Geant4 CPU backend configuration)."""
     from oracle import oracle as O
 
-    threads = max(1, min(16, os.cpu_count() or 1))
-    t0 = time.perf_counter()
-    O.run_synthetic(cs.scene, src, seed, [0], [sample], precision=32, threads=threads)
-    dt = time.perf_counter() - t0
-    return {"value": sample / dt, "unit": "photons/s", "cores": threads, "kind": "port",
-            "sample": f"photons 0..{sample - 1} of the same 1e8-photon workload, FP32 oracle (oracle/oracle.c), "
-                      f"{threads} threads, {dt:.2f} s wall"}
+    cores = max(1, min(HOST_CORE_SHARE, os.cpu_count() or 1))
+    rates = {}
+    for threads in (cores, 6):
+        n = sample if threads == cores else sample * 6 // cores
+        t0 = time.perf_counter()
+        O.run_synthetic(cs.scene, src, seed, [0], [n], precision=32, threads=threads)
+        rates[threads] = (n, time.perf_counter() - t0)
+    n, dt = rates[cores]
+    n6, dt6 = rates[6]
+    return {"value": n / dt, "unit": "photons/s", "cores": cores, "kind": "port",
+            "sample": f"photons 0..{n - 1} of the same 1e8-photon workload, FP32 oracle (oracle/oracle.c), "
+                      f"{cores} threads, {dt:.2f} s wall",
+            "value_6_threads": n6 / dt6, "sample_6_threads": f"photons 0..{n6 - 1}, 6 threads, {dt6:.2f} s wall",
+            "cpu_model": cpu_model(), "host_cpus_visible": os.cpu_count(),
+            "host_core_share": HOST_CORE_SHARE}
 
 
 def profiled_counters(photons):
@@ -437,12 +487,19 @@ def main():
                   "batched_photons_per_s": n_uneven * args.steps / b_wall, "batched_kernel_ms": b_kms,
                   "batched_launch": graphed.__dict__}
     if not args.no_plain:
+        # the plain form as one launch over the whole batch (one wave per 64
+        # photons of an event), and as one launch per event
+        p_wall, p_kms, p_out = run_uneven(host, cs, src, counts, LaunchForm(persistent=False), args.steps,
+                                          args.warmup)
         pe = LaunchForm(persistent=False, per_event=True)
         e_wall, e_kms, e_out = run_uneven(host, cs, src, counts, pe, args.steps, args.warmup)
-        assert e_out.statistics["detected"] == b_out.statistics["detected"]
-        uneven.update({"plain_per_event_photons_per_s": n_uneven * args.steps / e_wall,
+        assert e_out.statistics["detected"] == b_out.statistics["detected"] == p_out.statistics["detected"]
+        uneven.update({"plain_batch_photons_per_s": n_uneven * args.steps / p_wall,
+                       "plain_batch_kernel_ms": p_kms,
+                       "batched_over_plain_batch": p_wall / b_wall,
+                       "plain_per_event_photons_per_s": n_uneven * args.steps / e_wall,
                        "plain_per_event_kernel_ms": e_kms,
-                       "batched_over_plain": e_wall / b_wall})
+                       "batched_over_plain_per_event": e_wall / b_wall})
     host.close()
     adjoint = None if args.no_adjoint else run_adjoint(cs, src, args.seed, persistent, args.steps, args.warmup)
     array = None if args.no_array else run_array(args.seed, args.steps, args.warmup, not args.no_plain,
@@ -450,6 +507,7 @@ def main():
     c1 = None if args.no_c1 else run_c1(args.seed, args.steps, args.warmup, not args.no_cpu, 200,
                                                retune=args.retune)
     host_batches = None if args.no_c1 else run_host_batches(args.seed, persistent)
+    dropin_cpp = None if args.no_c1 else run_dropin_cpp()
     mesh = None if args.no_mesh else run_mesh(args.seed, args.steps, args.warmup, args.photons,
                                               float(np.mean(kms)) / out.statistics["total_steps"],
                                               retune=args.retune)
@@ -488,6 +546,7 @@ def main():
         "array_events": array,
         "electron_events": c1,
         "host_batches": host_batches,
+        "dropin_cpp": dropin_cpp,
         "mesh_crystals": mesh,
         "batched_over_plain": (None if plain is None else (args.photons * args.steps / wall) / plain["photons_per_s"]),
         "counts": {"photons": P, "steps": S, "detected": D, "steps_per_photon": S / max(P, 1),

@@ -520,8 +520,11 @@ __global__ void __launch_bounds__(256) generate_kernel(KernelParams p, float4* s
     }
 }
 
+// The plain form gets the persistent form's register cap (four waves per
+// SIMD), so that the two forms differ only in the work queue and refill.
 template <int kSource, int kLdsMode, int kGeom>
-__global__ void __launch_bounds__(256) transport_kernel_plain(KernelParams p) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
+transport_kernel_plain(KernelParams p) {
     transport_body<kSource, false, kLdsMode, kGeom>(p);
 }
 
