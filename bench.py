@@ -319,8 +319,9 @@ def run_c1(seed, steps, warmup, cpu, cpu_events, retune=False):
 def run_dropin_cpp(batches=40):
     """The drop-in path through the C++ shim (tests/cpp/shim_test bench): batches
     of 33 electron events (~1e6 photons) through PrepareScene / EnqueueEmissions /
-    CompleteOldestBatch, hits returned as PhotonDetection records in launch order,
-    with one batch in flight (the reference's single slot) and with two."""
+    CompleteOldestBatch, hits returned as PhotonDetection records in launch order
+    (converted from the library's hit records by four host threads), with one
+    batch in flight (the reference's single slot) and with two."""
     import subprocess
 
     exe = os.path.join(ROOT, "tests", "cpp", "shim_test")
@@ -328,7 +329,7 @@ def run_dropin_cpp(batches=40):
         return None
     res = {}
     for slots in (1, 2):
-        proc = subprocess.run([exe, "bench", str(slots), str(batches)], capture_output=True, text=True, timeout=120)
+        proc = subprocess.run([exe, "bench", str(slots), str(batches), "4"], capture_output=True, text=True, timeout=120)
         if proc.returncode != 0:
             return {"error": proc.stdout[-500:] + proc.stderr[-500:]}
         res[f"slots_{slots}"] = json.loads(proc.stdout.strip().splitlines()[-1])

@@ -89,7 +89,7 @@ def build_profile(verbose: bool = True) -> Path:
 def build_shim_test(verbose: bool = True) -> Path:
     """Host-only C++20 test program of include/g4go_hip_host.hpp, linked to the library."""
     tmp = SHIM_TEST.with_name(SHIM_TEST.name + ".tmp")
-    _run(["g++", "-std=c++20", "-O2", "-Wall", "-Wextra", f"-I{INCLUDE}", str(SHIM_TEST_SRC), f"-L{PACKAGE_DIR}",
+    _run(["g++", "-std=c++20", "-O2", "-pthread", "-Wall", "-Wextra", f"-I{INCLUDE}", str(SHIM_TEST_SRC), f"-L{PACKAGE_DIR}",
           "-lg4go_hip", "-Wl,-rpath,$ORIGIN/../../g4go_amd", "-o", str(tmp)])
     os.replace(tmp, SHIM_TEST)
     if verbose:

@@ -34,15 +34,19 @@
 #include <array>
 #include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <cstddef>
 #include <cstdint>
 #include <cstring>
 #include <deque>
 #include <limits>
+#include <mutex>
 #include <span>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <type_traits>
+#include <utility>
 #include <vector>
 
 namespace g4go_hip {
@@ -488,6 +492,102 @@ private:
 };
 
 // ---------------------------------------------------------------------------
+// A fixed set of helper threads that split an index range with the calling
+// thread.  CompleteOldestBatch uses it to turn a batch's hit records into the
+// caller's detection records: that loop streams the freshly copied hits from
+// host memory and is bound by one core's memory bandwidth.
+class RangePool {
+public:
+    explicit RangePool(unsigned helpers) {
+        threads_.reserve(helpers);
+        for (unsigned k = 0; k < helpers; ++k) {
+            threads_.emplace_back([this, k] { work(k + 1); });
+        }
+    }
+
+    ~RangePool() {
+        {
+            std::lock_guard<std::mutex> lock(mutex_);
+            stop_ = true;
+        }
+        wake_.notify_all();
+        for (std::thread& t : threads_) {
+            t.join();
+        }
+    }
+
+    RangePool(const RangePool&) = delete;
+    RangePool& operator=(const RangePool&) = delete;
+
+    // Calls body(lo, hi) over disjoint slices covering [0, n), one slice per
+    // thread, and returns when all are done.  body must not throw.
+    template <class F>
+    void run(std::uint64_t n, F& body) {
+        if (threads_.empty() || n < kMinParallel) {
+            body(std::uint64_t{0}, n);
+            return;
+        }
+        {
+            std::lock_guard<std::mutex> lock(mutex_);
+            job_ = Job{n, &body, [](void* f, std::uint64_t lo, std::uint64_t hi) { (*static_cast<F*>(f))(lo, hi); }};
+            outstanding_ = static_cast<unsigned>(threads_.size());
+            ++generation_;
+        }
+        wake_.notify_all();
+        const auto [lo, hi] = slice(n, 0);
+        body(lo, hi);
+        std::unique_lock<std::mutex> lock(mutex_);
+        done_.wait(lock, [this] { return outstanding_ == 0; });
+    }
+
+private:
+    static constexpr std::uint64_t kMinParallel = 16384;
+
+    struct Job {
+        std::uint64_t n = 0;
+        void* body = nullptr;
+        void (*call)(void*, std::uint64_t, std::uint64_t) = nullptr;
+    };
+
+    std::pair<std::uint64_t, std::uint64_t> slice(std::uint64_t n, unsigned part) const {
+        const std::uint64_t parts = threads_.size() + 1;
+        return {n * part / parts, n * (part + 1) / parts};
+    }
+
+    void work(unsigned part) {
+        std::uint64_t seen = 0;
+        for (;;) {
+            Job job;
+            {
+                std::unique_lock<std::mutex> lock(mutex_);
+                wake_.wait(lock, [&] { return stop_ || generation_ != seen; });
+                if (stop_) {
+                    return;
+                }
+                seen = generation_;
+                job = job_;
+            }
+            const auto [lo, hi] = slice(job.n, part);
+            job.call(job.body, lo, hi);
+            {
+                std::lock_guard<std::mutex> lock(mutex_);
+                if (--outstanding_ == 0) {
+                    done_.notify_one();
+                }
+            }
+        }
+    }
+
+    std::vector<std::thread> threads_;
+    std::mutex mutex_;
+    std::condition_variable wake_, done_;
+    Job job_{};
+    std::uint64_t generation_ = 0;
+    unsigned outstanding_ = 0;
+    bool stop_ = false;
+};
+
+// ---------------------------------------------------------------------------
 template <class Api = api::Types>
 class BasicHipTransportHost final {
 public:
@@ -511,9 +611,13 @@ public:
     // OptiXTransportHost(PhotonTransportConfig) (OptiXTransportHost.hpp:12).
     // in_flight > 1 lets that many batches be enqueued before the oldest is
     // completed (the reference keeps one, OptiXTransportHost.cpp:523-525).
+    // conversion_threads is the number of host threads, the caller's
+    // included, that turn a completed batch's hits into detection records
+    // (1: the calling thread alone).
     explicit BasicHipTransportHost(Config configuration, const g4go_launch* launch = nullptr,
-                                   std::uint32_t in_flight = 1)
-        : tolerance_mm_(configuration.boundaryToleranceMm) {
+                                   std::uint32_t in_flight = 1, std::uint32_t conversion_threads = 4)
+        : tolerance_mm_(configuration.boundaryToleranceMm),
+          pool_(conversion_threads > 1 ? conversion_threads - 1 : 0) {
         g4go_config c{};
         c.seed = configuration.seed;
         c.max_photons_per_event = configuration.maxPhotonsPerEvent;
@@ -593,19 +697,23 @@ public:
                                      bit(Field::Truncated) | bit(Field::MaxBounce) | bit(Field::InvalidState) |
                                      bit(Field::ZeroStep);
         const auto conversion0 = std::chrono::steady_clock::now();
-        out.detections.reserve(r.n_hits);
-        for (std::uint64_t i = 0; i < r.n_hits; ++i) {
-            const g4go_hit& h = r.hits[i];
-            Detection d{};
-            d.positionMm = {h.position_mm[0], h.position_mm[1], h.position_mm[2]};
-            d.timeNs = static_cast<float>(h.time_ns);
-            d.direction = {h.direction[0], h.direction[1], h.direction[2]};
-            d.energyEv = h.energy_ev;
-            d.eventID = h.event_id;
-            d.photonID = h.photon_id;
-            d.sensorID = h.sensor_id;
-            out.detections.push_back(d);
-        }
+        out.detections.resize(r.n_hits);
+        Detection* detections = out.detections.data();
+        const g4go_hit* hits = r.hits;
+        auto convert = [detections, hits](std::uint64_t lo, std::uint64_t hi) {
+            for (std::uint64_t i = lo; i < hi; ++i) {
+                const g4go_hit& h = hits[i];
+                Detection& d = detections[i];
+                d.positionMm = {h.position_mm[0], h.position_mm[1], h.position_mm[2]};
+                d.timeNs = static_cast<float>(h.time_ns);
+                d.direction = {h.direction[0], h.direction[1], h.direction[2]};
+                d.energyEv = h.energy_ev;
+                d.eventID = h.event_id;
+                d.photonID = h.photon_id;
+                d.sensorID = h.sensor_id;
+            }
+        };
+        pool_.run(r.n_hits, convert);
         const double conversion_ms =
             std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - conversion0).count();
         auto& st = out.statistics;
@@ -680,6 +788,7 @@ private:
     const Scene* prepared_ = nullptr;
     float tolerance_mm_;
     std::deque<Pending> pending_;
+    RangePool pool_;
 };
 
 using HipTransportHost = BasicHipTransportHost<>;

@@ -7,7 +7,7 @@
 //                            Cerenkov and direct emission records through
 //                            the three-method interface and writes the
 //                            records and the output to OUT.bin (layout below)
-//   shim_test bench SLOTS BATCHES [CHUNK BLOCKS_PER_CU REFILL PREGENERATE]
+//   shim_test bench SLOTS BATCHES [THREADS [CHUNK BLOCKS_PER_CU REFILL PREGENERATE]]
 //                            throughput of the drop-in path (JSON line)
 //   shim_test gpu-mesh OUT.bin  the same with the crystal a tapered mesh
 //                            (3x3 mm front face, 4x4 mm back face, turned
@@ -473,7 +473,7 @@ void accumulate(PhotonTransportPerformance& sum, const PhotonTransportPerformanc
 // launch order and converted to PhotonDetection records, with up to `slots`
 // batches in flight.  Prints one JSON line: photons/s and the mean per-batch
 // performance fields.
-int run_bench(int slots, int batches, const g4go_launch* launch) {
+int run_bench(int slots, int batches, int threads, const g4go_launch* launch) {
     CrystalScene c = crystal_scene();
     const int warm = 3;
     std::vector<std::vector<OpticalEmission>> em(batches + warm);
@@ -489,7 +489,8 @@ int run_bench(int slots, int batches, const g4go_launch* launch) {
     }
     PhotonTransportConfig cfg;
     cfg.seed = 99;
-    g4go_hip::HipTransportHost host{cfg, launch, static_cast<std::uint32_t>(slots)};
+    g4go_hip::HipTransportHost host{cfg, launch, static_cast<std::uint32_t>(slots),
+                                    static_cast<std::uint32_t>(std::max(1, threads))};
     host.PrepareScene(c.scene);
     PhotonTransportPerformance sum{};
     std::uint64_t hits = 0;
@@ -530,12 +531,13 @@ int run_bench(int slots, int batches, const g4go_launch* launch) {
     run(warm, warm + batches, true);
     const double seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     const double n = batches;
-    std::printf("{\"slots\": %d, \"batches\": %d, \"photons_per_batch\": %.1f, \"photons_per_s\": %.6g, "
+    std::printf("{\"slots\": %d, \"conversion_threads\": %d, \"batches\": %d, \"photons_per_batch\": %.1f, "
+                "\"photons_per_s\": %.6g, "
                 "\"ms_per_batch\": %.4f, \"hits_per_batch\": %.1f, \"enqueue_ms\": %.4f, \"complete_ms\": %.4f, "
                 "\"host_to_device_ms\": %.4f, \"device_memset_ms\": %.4f, \"kernel_ms\": %.4f, "
                 "\"device_hit_order_ms\": %.4f, \"metadata_to_host_ms\": %.4f, \"hits_to_host_ms\": %.4f, "
                 "\"host_hit_conversion_ms\": %.4f}\n",
-                slots, batches, static_cast<double>(photons) / n, static_cast<double>(photons) / seconds,
+                slots, std::max(1, threads), batches, static_cast<double>(photons) / n, static_cast<double>(photons) / seconds,
                 1e3 * seconds / n, static_cast<double>(hits) / n, enqueue_ms / n, complete_ms / n,
                 sum.hostToDeviceMs / n, sum.deviceMemsetMs / n, sum.optiXKernelMs / n, sum.deviceHitCompactionMs / n,
                 sum.deviceMetadataToHostMs / n, sum.deviceHitsToHostMs / n, sum.hostHitCompactionMs / n);
@@ -555,21 +557,23 @@ int main(int argc, char** argv) {
         if (argc == 3 && std::string(argv[1]) == "gpu-mesh") {
             return run_gpu(argv[2], true);
         }
-        if ((argc == 4 || argc == 8) && std::string(argv[1]) == "bench") {
-            // optional persistent form: CHUNK BLOCKS_PER_CU REFILL PREGENERATE (else the handle's default)
+        if ((argc == 4 || argc == 5 || argc == 9) && std::string(argv[1]) == "bench") {
+            // optional: the conversion thread count, then a persistent form
+            // CHUNK BLOCKS_PER_CU REFILL PREGENERATE (else the handle's default)
             g4go_launch form{1, 256, 256, 24, 8, 0, 0, 0};
-            if (argc == 8) {
-                form.chunk = static_cast<std::uint32_t>(std::atoi(argv[4]));
-                form.blocks_per_cu = static_cast<std::uint32_t>(std::atoi(argv[5]));
-                form.refill_threshold = static_cast<std::uint32_t>(std::atoi(argv[6]));
-                form.pregenerate = static_cast<std::uint32_t>(std::atoi(argv[7]));
+            const int threads = argc >= 5 ? std::atoi(argv[4]) : 4;
+            if (argc == 9) {
+                form.chunk = static_cast<std::uint32_t>(std::atoi(argv[5]));
+                form.blocks_per_cu = static_cast<std::uint32_t>(std::atoi(argv[6]));
+                form.refill_threshold = static_cast<std::uint32_t>(std::atoi(argv[7]));
+                form.pregenerate = static_cast<std::uint32_t>(std::atoi(argv[8]));
             }
-            return run_bench(std::atoi(argv[2]), std::atoi(argv[3]), argc == 8 ? &form : nullptr);
+            return run_bench(std::atoi(argv[2]), std::atoi(argv[3]), threads, argc == 9 ? &form : nullptr);
         }
     } catch (const std::exception& e) {
         std::printf("FAIL: uncaught %s: %s\n", typeid(e).name(), e.what());
         return 1;
     }
-    std::printf("usage: shim_test cpu | shim_test gpu OUT.bin | shim_test gpu-mesh OUT.bin | shim_test bench SLOTS BATCHES [CHUNK BLOCKS_PER_CU REFILL PREGENERATE]\n");
+    std::printf("usage: shim_test cpu | shim_test gpu OUT.bin | shim_test gpu-mesh OUT.bin | shim_test bench SLOTS BATCHES [THREADS [CHUNK BLOCKS_PER_CU REFILL PREGENERATE]]\n");
     return 2;
 }
