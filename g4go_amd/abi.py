@@ -168,7 +168,7 @@ class ParamTable(C.Structure):
 
 
 class AdjointStats(C.Structure):
-    _fields_ = [("replayed", C.c_uint64), ("mismatched", C.c_uint64), ("unsupported", C.c_uint64),
+    _fields_ = [("replayed", C.c_uint64), ("mismatched", C.c_uint64), ("rescored", C.c_uint64),
                 ("steps", C.c_uint64), ("kernel_ms", C.c_double)]
 
 

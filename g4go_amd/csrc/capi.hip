@@ -2127,7 +2127,7 @@ static g4go_status core_adjoint(Core* h, const double* slot_weight, double* grad
         }
         h->init_hip();
         const size_t map_bytes = h->param_map.size() * sizeof(uint32_t);
-        // [map][slot weights 258 f64][grad n f64][counters 4 u64]
+        // [map][slot weights 258 f64][grad n f64][counters 8 u64: 4 reported, then the replay's queue head]
         const size_t o_w = align256(map_bytes), o_g = o_w + align256(kHistSlots * sizeof(double));
         const size_t o_c = o_g + align256(n * sizeof(double)), bytes = o_c + 64;
         h->d_adjoint.ensure(bytes, "hipMalloc adjoint");
@@ -2150,9 +2150,11 @@ static g4go_status core_adjoint(Core* h, const double* slot_weight, double* grad
         const auto* map = reinterpret_cast<const uint32_t*>(d);
         a.map = ParamMap{map, map + n_mat, map + 2 * n_mat, map + 2 * n_mat + n_surf, h->n_params};
         a.counters = reinterpret_cast<unsigned long long*>(d + o_c);
+        // persistent replay: four 256-thread blocks per CU (four waves per
+        // SIMD), fewer for a short hit list
         const int block = 256;
         const int grid = static_cast<int>(std::max<uint64_t>(
-            1, std::min<uint64_t>((a.n_hits + block - 1) / block, static_cast<uint64_t>(h->cu_count) * 8)));
+            1, std::min<uint64_t>((a.n_hits + block - 1) / block, static_cast<uint64_t>(h->cu_count) * 4)));
         hip_check(hipEventRecord(h->ev_kernel0, h->stream), "hipEventRecord");
         if (a.n_hits != 0 && launch_adjoint(a, h->last_source, block, grid, h->stream) != 0) {
             fail(G4GO_ERROR_RUNTIME, "adjoint kernel launch failed");

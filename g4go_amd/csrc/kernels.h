@@ -120,7 +120,9 @@ struct AdjointParams {
     const double* slot_weight;  // dLoss/dN + dLoss/dh_b per histogram slot (kHistSlots)
     double* grad;               // n_params, accumulated
     ParamMap map;
-    unsigned long long* counters;  // [0] replayed, [1] path mismatches, [2] unsupported paths, [3] replay steps
+    // [0] replayed, [1] path mismatches, [2] replays scored step by step on a
+    // second pass, [3] replay steps, [4] hit-queue head (zeroed before launch)
+    unsigned long long* counters;
 };
 
 constexpr uint32_t kLdsVolumes = 1u, kLdsSurfaces = 2u, kLdsMaterials = 4u, kLdsPlanes = 8u;

@@ -664,10 +664,19 @@ int launch_transport(const KernelParams& p, int source, bool persistent, int blo
 //     d/d eff = 1 / eff;
 // each chained to the two interpolation nodes of the table at the photon
 // energy and weighted by dLoss/dN + dLoss/dh_b of the photon's arrival-time
-// slot b.  The loss weight is known before the replay, so nothing is stored
-// per step.  A path may visit up to two materials and two parameterised
-// reflecting surfaces; longer ones are counted as unsupported.
+// slot b, read from the recorded hit before the replay.
+//
+// A replay first sums its path into two material and two surface slots and
+// scores them once it has matched its recorded end.  A path that visits more
+// materials or parameterised surfaces than that is replayed a second time
+// (it is known to match by then) and scored segment by segment as it goes.
+//
+// The kernel is persistent like the forward one: each wave takes chunks of
+// the hit list from a device counter and refills lanes whose replay ended, so
+// a wave's lanes stay busy whatever the spread of path lengths.
 constexpr uint32_t kAdjointLdsParams = 4096;
+constexpr uint32_t kAdjointChunk = 64;   // hits a wave takes from the queue at a time
+constexpr uint32_t kAdjointRefill = 24;  // idle lanes that trigger a refill
 
 // 16-byte words of LDS that the adjoint's gradient accumulator takes (0: it
 // accumulates in global memory); the staged scene follows it.
@@ -765,6 +774,43 @@ __device__ __forceinline__ void add_material_grads(double* grad, const SceneView
     }
 }
 
+__device__ __forceinline__ void add_reflection_grads(double* grad, const SceneView& sc, const ParamMap& map,
+                                                     const SurfSlot& r, float e, double weight) {
+    if (r.surface == kInvalid) {
+        return;
+    }
+    const Surface& sf = sc.surfaces[r.surface];
+    const float refl = prop_eval(sc.pool, sf.reflectivity, e, 0.0f);
+    if (refl > 0.0f && refl < 1.0f) {
+        add_table_grad(grad, sc.pool, sf.reflectivity, map.surface_reflectivity[r.surface], e,
+                       weight * static_cast<double>(r.count) / static_cast<double>(refl));
+    }
+}
+
+__device__ __forceinline__ void add_detection_grads(double* grad, const SceneView& sc, const ParamMap& map,
+                                                    uint32_t surface, float e, double weight) {
+    if (surface >= sc.n_surfaces) {
+        return;
+    }
+    const Surface& sf = sc.surfaces[surface];
+    const uint32_t oe = map.surface_efficiency[surface];
+    if (oe != kInvalid && sf.efficiency.count != 0) {
+        const float eff = prop_eval(sc.pool, sf.efficiency, e, 0.0f);
+        if (eff > 0.0f && eff < 1.0f) {
+            add_table_grad(grad, sc.pool, sf.efficiency, oe, e, weight / static_cast<double>(eff));
+        }
+    }
+    const uint32_t orf = map.surface_reflectivity[surface];
+    if (orf != kInvalid && sf.reflectivity.count != 0) {
+        const float r = prop_eval(sc.pool, sf.reflectivity, e, 0.0f);
+        const float t = sf.transmittance.count != 0 ? clamp01(prop_eval(sc.pool, sf.transmittance, e, 0.0f)) : 0.0f;
+        const double absorb = 1.0 - static_cast<double>(r) - static_cast<double>(t);
+        if (r > 0.0f && r < 1.0f && absorb > 0.0) {
+            add_table_grad(grad, sc.pool, sf.reflectivity, orf, e, -weight / absorb);
+        }
+    }
+}
+
 }  // namespace
 
 // Four waves per SIMD (at most 128 VGPRs), as for the persistent transport
@@ -785,95 +831,143 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) a
     stage_scene<kLdsMode, kGeom>(p, reinterpret_cast<float4*>(s_grad) + adjoint_grad_quads(n_params), sc);
     __syncthreads();
     const uint32_t n_sources = p.header != nullptr ? p.header[0] : 0u;
-    uint32_t replayed = 0, mismatch = 0, unsupported = 0, steps = 0;
-    for (uint32_t h = blockIdx.x * blockDim.x + threadIdx.x; h < a.n_hits; h += gridDim.x * blockDim.x) {
-        const Hit hit = a.hits[h];
-        Photon ph;
-        if (kSource == kSourceSynthetic) {
-            generate_synthetic(ph, p.synth, p.rng, hit.event_id, hit.photon_id);
-        } else {
-            const uint32_t src = launch_source(p, a.hit_launch[h], n_sources);
-            const Emission& em = p.emissions[src];
-            generate_emission(ph, em, sc, p.rng, hit.photon_id - em.first_photon_id);
+    const uint32_t lane = lane_id();
+    uint32_t replayed = 0, mismatch = 0, rescored = 0, steps = 0;
+
+    // per-lane replay: hit index, pass (direct: scored step by step), path sums
+    Photon ph;
+    ph.material = kInvalid;
+    bool alive = false, pending = false, direct = false, ok = true;
+    uint32_t h = 0, det_surface = kInvalid;
+    double weight = 0.0;
+    MatSlot m0{kInvalid, 0u, 0.0}, m1{kInvalid, 0u, 0.0};
+    SurfSlot r0{kInvalid, 0u}, r1{kInvalid, 0u};
+    // wave-uniform cursor over the hit list
+    uint32_t it_next = 0, it_end = 0;
+    bool done = false;
+
+    for (;;) {
+        // ---- idle lanes take the next hits of the wave's chunk ----
+        uint64_t idle = __ballot(!alive && !pending);
+        uint32_t n_idle = static_cast<uint32_t>(__popcll(idle));
+        if (n_idle >= kAdjointRefill || n_idle == 64u) {
+            while (n_idle > 0u) {
+                if (it_next >= it_end) {
+                    if (done) {
+                        break;
+                    }
+                    uint32_t c = 0;
+                    if (lane == 0u) {
+                        c = atomicAdd(reinterpret_cast<uint32_t*>(a.counters + 4), kAdjointChunk);
+                    }
+                    c = uniform(static_cast<uint32_t>(__shfl(static_cast<int>(c), 0)));
+                    if (c >= a.n_hits) {
+                        done = true;
+                        break;
+                    }
+                    it_next = c;
+                    it_end = min(a.n_hits, c + kAdjointChunk);
+                    continue;
+                }
+                const uint32_t take = min(it_end - it_next, n_idle);
+                const uint32_t rank = rank_below(idle);
+                if (!alive && !pending && rank < take) {
+                    h = it_next + rank;
+                    pending = true;
+                    direct = false;
+                }
+                it_next += take;
+                idle = __ballot(!alive && !pending);
+                n_idle = static_cast<uint32_t>(__popcll(idle));
+            }
         }
-        ph.material = kInvalid;
-        if (!start_photon(ph)) {
-            ++mismatch;
+        // ---- start the replays of new hits and second passes ----
+        if (pending) {
+            pending = false;
+            const Hit& hit = a.hits[h];
+            if (kSource == kSourceSynthetic) {
+                generate_synthetic(ph, p.synth, p.rng, hit.event_id, hit.photon_id);
+            } else {
+                const uint32_t src = launch_source(p, a.hit_launch[h], n_sources);
+                const Emission& em = p.emissions[src];
+                generate_emission(ph, em, sc, p.rng, hit.photon_id - em.first_photon_id);
+            }
+            ph.material = kInvalid;
+            if (start_photon(ph)) {
+                alive = true;
+                ok = true;
+                det_surface = kInvalid;
+                m0 = MatSlot{kInvalid, 0u, 0.0};
+                m1 = MatSlot{kInvalid, 0u, 0.0};
+                r0 = SurfSlot{kInvalid, 0u};
+                r1 = SurfSlot{kInvalid, 0u};
+                weight = a.slot_weight[hist_slot(hit.time, p.hist_lo_ns, p.hist_bins_per_ns)];
+            } else {
+                ++mismatch;
+            }
+        }
+        if (__ballot(alive) == 0ull) {
+            if (done) {
+                break;
+            }
             continue;
         }
-        MatSlot m0{kInvalid, 0u, 0.0}, m1{kInvalid, 0u, 0.0};
-        SurfSlot r0{kInvalid, 0u}, r1{kInvalid, 0u};
-        bool ok = true;
-        int status = kAlive;
-        uint32_t sensor = kInvalid, det_surface = kInvalid;
-        for (;;) {
+
+        // ---- one replayed step ----
+        if (alive) {
             bool zero = false;
+            uint32_t sensor = kInvalid;
             StepTrace tr;
-            status = transport_step<true, kGeom>(ph, sc, p.rng, p.cfg, zero, sensor, &tr);
+            const int status = transport_step<true, kGeom>(ph, sc, p.rng, p.cfg, zero, sensor, &tr);
             ++ph.step;
             ++steps;
-            if (tr.segment > 0.0f || tr.event == kTraceRayleigh) {
-                ok = add_segment(m0, m1, tr.material, tr.segment, tr.event == kTraceRayleigh) && ok;
+            const float e = ph.energy;
+            if ((tr.segment > 0.0f || tr.event == kTraceRayleigh) && tr.material < sc.n_materials) {
+                if (direct) {
+                    add_material_grads(grad, sc, a.map,
+                                       MatSlot{tr.material, tr.event == kTraceRayleigh ? 1u : 0u,
+                                               static_cast<double>(tr.segment)},
+                                       e, weight);
+                } else {
+                    ok = add_segment(m0, m1, tr.material, tr.segment, tr.event == kTraceRayleigh) && ok;
+                }
             }
             if (tr.event == kTraceInteraction && tr.surface < sc.n_surfaces &&
                 a.map.surface_reflectivity[tr.surface] != kInvalid) {
-                ok = add_reflection(r0, r1, tr.surface) && ok;
+                if (direct) {
+                    add_reflection_grads(grad, sc, a.map, SurfSlot{tr.surface, 1u}, e, weight);
+                } else {
+                    ok = add_reflection(r0, r1, tr.surface) && ok;
+                }
             }
             if (status == kDetected) {
                 det_surface = tr.surface;
             }
             if (status != kAlive || ph.step >= p.max_steps) {
-                break;
-            }
-        }
-        if (status != kDetected || ph.time != hit.time || ph.pos.x != hit.pos[0] || ph.pos.y != hit.pos[1] ||
-            ph.pos.z != hit.pos[2]) {
-            ++mismatch;
-            continue;
-        }
-        if (!ok) {
-            ++unsupported;
-            continue;
-        }
-        ++replayed;
-        const float e = ph.energy;
-        const double weight = a.slot_weight[hist_slot(hit.time, p.hist_lo_ns, p.hist_bins_per_ns)];
-        add_material_grads(grad, sc, a.map, m0, e, weight);
-        add_material_grads(grad, sc, a.map, m1, e, weight);
-        const SurfSlot rs[2] = {r0, r1};
-        for (int k = 0; k < 2; ++k) {
-            if (rs[k].surface == kInvalid) {
-                continue;
-            }
-            const Surface& sf = sc.surfaces[rs[k].surface];
-            const float r = prop_eval(sc.pool, sf.reflectivity, e, 0.0f);
-            if (r > 0.0f && r < 1.0f) {
-                add_table_grad(grad, sc.pool, sf.reflectivity, a.map.surface_reflectivity[rs[k].surface], e,
-                               weight * static_cast<double>(rs[k].count) / static_cast<double>(r));
-            }
-        }
-        if (det_surface < sc.n_surfaces) {
-            const Surface& sf = sc.surfaces[det_surface];
-            const uint32_t oe = a.map.surface_efficiency[det_surface];
-            if (oe != kInvalid && sf.efficiency.count != 0) {
-                const float eff = prop_eval(sc.pool, sf.efficiency, e, 0.0f);
-                if (eff > 0.0f && eff < 1.0f) {
-                    add_table_grad(grad, sc.pool, sf.efficiency, oe, e, weight / static_cast<double>(eff));
-                }
-            }
-            const uint32_t orf = a.map.surface_reflectivity[det_surface];
-            if (orf != kInvalid && sf.reflectivity.count != 0) {
-                const float r = prop_eval(sc.pool, sf.reflectivity, e, 0.0f);
-                const float t = sf.transmittance.count != 0 ? clamp01(prop_eval(sc.pool, sf.transmittance, e, 0.0f)) : 0.0f;
-                const double absorb = 1.0 - static_cast<double>(r) - static_cast<double>(t);
-                if (r > 0.0f && r < 1.0f && absorb > 0.0) {
-                    add_table_grad(grad, sc.pool, sf.reflectivity, orf, e, -weight / absorb);
+                alive = false;
+                const Hit& hit = a.hits[h];
+                if (status != kDetected || ph.time != hit.time || ph.pos.x != hit.pos[0] || ph.pos.y != hit.pos[1] ||
+                    ph.pos.z != hit.pos[2]) {
+                    ++mismatch;
+                } else if (!ok) {
+                    pending = true;  // replay again, scoring each segment as it goes
+                    direct = true;
+                } else {
+                    if (!direct) {
+                        add_material_grads(grad, sc, a.map, m0, e, weight);
+                        add_material_grads(grad, sc, a.map, m1, e, weight);
+                        add_reflection_grads(grad, sc, a.map, r0, e, weight);
+                        add_reflection_grads(grad, sc, a.map, r1, e, weight);
+                    }
+                    add_detection_grads(grad, sc, a.map, det_surface, e, weight);
+                    ++replayed;
+                    rescored += direct ? 1u : 0u;
                 }
             }
         }
     }
-    const uint32_t wr = wave_sum(replayed), wm = wave_sum(mismatch), wu = wave_sum(unsupported), ws = wave_sum(steps);
-    if (lane_id() == 0u) {
+    const uint32_t wr = wave_sum(replayed), wm = wave_sum(mismatch), wu = wave_sum(rescored), ws = wave_sum(steps);
+    if (lane == 0u) {
         if (wr) atomicAdd(a.counters + 0, static_cast<unsigned long long>(wr));
         if (wm) atomicAdd(a.counters + 1, static_cast<unsigned long long>(wm));
         if (wu) atomicAdd(a.counters + 2, static_cast<unsigned long long>(wu));

@@ -374,7 +374,8 @@ typedef struct g4go_param_table {
 typedef struct g4go_adjoint_stats {
     uint64_t replayed;     /* detected photons replayed and scored */
     uint64_t mismatched;   /* replays that did not end in their recorded detection (0 expected) */
-    uint64_t unsupported;  /* paths visiting more than two materials or parameterised surfaces */
+    uint64_t rescored;     /* of those, paths through more than two materials or two parameterised
+                              surfaces, scored step by step on a second replay */
     uint64_t steps;        /* transport steps replayed */
     double kernel_ms;
 } g4go_adjoint_stats;

@@ -2071,40 +2071,41 @@ static void or_add_table(double* grad, const prop_t* p, uint32_t offset, real e,
     }
 }
 
+/* A replayed path's sums, per material (bulk length, Rayleigh scatters) and
+ * per parameterised surface (reflections), kept sparse: the arrays span the
+ * scene and the lists name the entries this path touched, in first-touch
+ * order, so that they are cleared in time proportional to the path. */
 typedef struct {
-    uint32_t material, rayleigh;
-    double length;
-} mslot_t;
+    double* length;
+    uint32_t* rayleigh;
+    uint32_t* reflections;
+    uint32_t *mats, *surfs;
+    uint32_t n_mats, n_surfs;
+} path_sums_t;
 
-typedef struct {
-    uint32_t surface, count;
-} sslot_t;
-
-static int or_add_segment(mslot_t* a, mslot_t* b, uint32_t m, real len, int rayleigh) {
-    mslot_t* s = a->material == m ? a : (b->material == m ? b : NULL);
-    if (!s) {
-        s = a->material == INVALID ? a : (b->material == INVALID ? b : NULL);
-        if (!s) {
-            return 0;
-        }
-        s->material = m;
+static void path_add_segment(path_sums_t* p, uint32_t m, real len, int rayleigh) {
+    if (p->length[m] == 0.0 && p->rayleigh[m] == 0u) {
+        p->mats[p->n_mats++] = m;
     }
-    s->length += (double)len;
-    s->rayleigh += rayleigh ? 1u : 0u;
-    return 1;
+    p->length[m] += (double)len;
+    p->rayleigh[m] += rayleigh ? 1u : 0u;
 }
 
-static int or_add_reflection(sslot_t* a, sslot_t* b, uint32_t sf) {
-    sslot_t* s = a->surface == sf ? a : (b->surface == sf ? b : NULL);
-    if (!s) {
-        s = a->surface == INVALID ? a : (b->surface == INVALID ? b : NULL);
-        if (!s) {
-            return 0;
-        }
-        s->surface = sf;
+static void path_add_reflection(path_sums_t* p, uint32_t sf) {
+    if (p->reflections[sf]++ == 0u) {
+        p->surfs[p->n_surfs++] = sf;
     }
-    ++s->count;
-    return 1;
+}
+
+static void path_clear(path_sums_t* p) {
+    for (uint32_t k = 0; k < p->n_mats; ++k) {
+        p->length[p->mats[k]] = 0.0;
+        p->rayleigh[p->mats[k]] = 0u;
+    }
+    for (uint32_t k = 0; k < p->n_surfs; ++k) {
+        p->reflections[p->surfs[k]] = 0u;
+    }
+    p->n_mats = p->n_surfs = 0;
 }
 
 /* Whether a replayed path ends at its recorded detection.  The FP32 replay
@@ -2182,7 +2183,8 @@ static int regen_emission(photon_t* ph, const void* c, const oscene_t* s, const 
 
 /* tables: n_tables (kind, index) pairs bound to theta in order (kind 0 absorption,
  * 1 Rayleigh of a material; 2 reflectivity, 3 efficiency of a surface).
- * counters: replayed, mismatched, unsupported, steps.  Returns 0, or -1 when the
+ * counters: replayed, mismatched, replayed paths beyond two materials or two
+ * parameterised surfaces, steps.  Returns 0, or -1 when the
  * layout does not fit n_params. */
 static int adjoint_core(oscene_t* s, regen_fn regen, const void* ctx, uint64_t seed, float eps, uint32_t max_steps,
                         const g4go_hit* hits, uint64_t n_hits, const double* slot_weight, const uint32_t* tables,
@@ -2209,24 +2211,30 @@ static int adjoint_core(oscene_t* s, regen_fn regen, const void* ctx, uint64_t s
         return -1;
     }
     const rng_t r = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    path_sums_t ps;
+    ps.length = (double*)calloc(s->n_mat + 1, sizeof(double));
+    ps.rayleigh = (uint32_t*)calloc(s->n_mat + 1, sizeof(uint32_t));
+    ps.reflections = (uint32_t*)calloc(s->n_surf + 1, sizeof(uint32_t));
+    ps.mats = (uint32_t*)malloc(sizeof(uint32_t) * (s->n_mat + 1));
+    ps.surfs = (uint32_t*)malloc(sizeof(uint32_t) * (s->n_surf + 1));
+    ps.n_mats = ps.n_surfs = 0;
     for (uint64_t h = 0; h < n_hits; ++h) {
         const g4go_hit* hit = &hits[h];
         photon_t ph;
         if (!regen(&ph, ctx, s, &r, hit) || !start_photon(&ph)) { counters[1]++; continue; }
-        mslot_t m[2] = {{INVALID, 0, 0.0}, {INVALID, 0, 0.0}};
-        sslot_t rs[2] = {{INVALID, 0}, {INVALID, 0}};
-        int ok = 1, status = S_ALIVE, zero = 0;
+        path_clear(&ps);
+        int status = S_ALIVE, zero = 0;
         uint32_t sensor = INVALID, det_surface = INVALID;
         for (;;) {
             trace_t tr;
             status = or_step(&ph, s, &r, (real)eps, &zero, &sensor, &tr);
             ph.step++;
             counters[3]++;
-            if (tr.segment > RF(0.0) || tr.event == 1) {
-                ok = or_add_segment(&m[0], &m[1], tr.material, tr.segment, tr.event == 1) && ok;
+            if ((tr.segment > RF(0.0) || tr.event == 1) && tr.material < s->n_mat) {
+                path_add_segment(&ps, tr.material, tr.segment, tr.event == 1);
             }
             if (tr.event == 2 && tr.surface < s->n_surf && srefl[tr.surface] != INVALID) {
-                ok = or_add_reflection(&rs[0], &rs[1], tr.surface) && ok;
+                path_add_reflection(&ps, tr.surface);
             }
             if (status == S_DETECTED) det_surface = tr.surface;
             if (status != S_ALIVE || ph.step >= max_steps) break;
@@ -2235,35 +2243,37 @@ static int adjoint_core(oscene_t* s, regen_fn regen, const void* ctx, uint64_t s
             counters[1]++;
             continue;
         }
-        if (!ok) { counters[2]++; continue; }
         counters[0]++;
+        /* paths beyond two materials or two parameterised surfaces (the device
+         * scores those on a second replay) */
+        if (ps.n_mats > 2 || ps.n_surfs > 2) counters[2]++;
         const real e = ph.energy;
         const double wgt = slot_weight[hist_slot(hit->time_ns)];
-        for (int k = 0; k < 2; ++k) {
-            if (m[k].material == INVALID || m[k].material >= s->n_mat) continue;
-            const material_t* mt = &s->mat[m[k].material];
-            if (mabs[m[k].material] != INVALID && mt->abslen.n) {
+        for (uint32_t k = 0; k < ps.n_mats; ++k) {
+            const uint32_t mi = ps.mats[k];
+            const material_t* mt = &s->mat[mi];
+            if (mabs[mi] != INVALID && mt->abslen.n) {
                 const real la = or_prop(&mt->abslen, e, R_INF);
                 if (finite_r(la) && la > RF(0.0)) {
                     const double l = (double)la;
-                    or_add_table(grad, &mt->abslen, mabs[m[k].material], e, wgt * m[k].length / (l * l));
+                    or_add_table(grad, &mt->abslen, mabs[mi], e, wgt * ps.length[mi] / (l * l));
                 }
             }
-            if (mray[m[k].material] != INVALID && mt->rayleigh.n) {
+            if (mray[mi] != INVALID && mt->rayleigh.n) {
                 const real lr = or_prop(&mt->rayleigh, e, R_INF);
                 if (finite_r(lr) && lr > RF(0.0)) {
                     const double l = (double)lr;
-                    or_add_table(grad, &mt->rayleigh, mray[m[k].material], e,
-                                 wgt * (m[k].length / (l * l) - (double)m[k].rayleigh / l));
+                    or_add_table(grad, &mt->rayleigh, mray[mi], e,
+                                 wgt * (ps.length[mi] / (l * l) - (double)ps.rayleigh[mi] / l));
                 }
             }
         }
-        for (int k = 0; k < 2; ++k) {
-            if (rs[k].surface == INVALID) continue;
-            const surface_t* sf = &s->surf[rs[k].surface];
+        for (uint32_t k = 0; k < ps.n_surfs; ++k) {
+            const uint32_t si = ps.surfs[k];
+            const surface_t* sf = &s->surf[si];
             const real rr = or_prop(&sf->refl, e, RF(0.0));
             if (rr > RF(0.0) && rr < RF(1.0)) {
-                or_add_table(grad, &sf->refl, srefl[rs[k].surface], e, wgt * (double)rs[k].count / (double)rr);
+                or_add_table(grad, &sf->refl, srefl[si], e, wgt * (double)ps.reflections[si] / (double)rr);
             }
         }
         if (det_surface < s->n_surf) {
@@ -2284,6 +2294,7 @@ static int adjoint_core(oscene_t* s, regen_fn regen, const void* ctx, uint64_t s
             }
         }
     }
+    free(ps.length); free(ps.rayleigh); free(ps.reflections); free(ps.mats); free(ps.surfs);
     free(mabs); free(mray); free(srefl); free(seff);
     return 0;
 }

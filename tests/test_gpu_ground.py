@@ -81,7 +81,7 @@ def test_ground_wrap_adjoint_matches_oracle_replay(kind):
         out = h.complete_oldest_batch()
         grad, stats = h.adjoint(w)
     assert stats["replayed"] == len(out.detections) > 0
-    assert stats["mismatched"] == 0 and stats["unsupported"] == 0
+    assert stats["mismatched"] == 0 and stats["rescored"] == 0
     ref, counters = O.adjoint_synthetic(cs.scene, src, 29, out.detections, w, layout, n_params)
     assert int(counters[0]) == len(out.detections) and int(counters[1]) == 0
     np.testing.assert_allclose(grad, ref, rtol=1e-9, atol=1e-9 * np.max(np.abs(ref)))

@@ -170,7 +170,7 @@ def test_mesh_adjoint_matches_oracle_replay():
         h.enqueue_synthetic(cs.scene, src, ev, counts)
         out = h.complete_oldest_batch()
         grad, stats = h.adjoint(w)
-    assert stats["replayed"] == len(out.detections) and stats["mismatched"] == 0 and stats["unsupported"] == 0
+    assert stats["replayed"] == len(out.detections) and stats["mismatched"] == 0 and stats["rescored"] == 0
     ref, counters = O.adjoint_synthetic(cs.scene, src, 36, out.detections, w, crystal_layout(cs), n_params)
     assert int(counters[0]) == len(out.detections) and int(counters[1]) == 0
     np.testing.assert_allclose(grad, ref, rtol=1e-9, atol=1e-9 * np.max(np.abs(ref)))
@@ -221,7 +221,7 @@ def test_triangle_solid_adjoint_matches_oracle_replay():
         out = h.complete_oldest_batch()
         grad, stats = h.adjoint(w)
     assert len(out.detections) > 1000
-    assert stats["replayed"] == len(out.detections) and stats["mismatched"] == 0 and stats["unsupported"] == 0
+    assert stats["replayed"] == len(out.detections) and stats["mismatched"] == 0 and stats["rescored"] == 0
     ref, counters = O.adjoint_synthetic(sc, src, 37, out.detections, w, layout, n_params)
     assert int(counters[0]) == len(out.detections) and int(counters[1]) == 0
     np.testing.assert_allclose(grad, ref, rtol=1e-9, atol=1e-9 * np.max(np.abs(ref)))

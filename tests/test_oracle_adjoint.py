@@ -177,3 +177,65 @@ def test_emission_replay_closed_form():
     bad["photon_id"][0] = len(mus) * per_record + 7
     _, c2 = O.adjoint_emissions(sc, em, 41, bad, np.ones(abi.HIST_SLOTS), tables, 5)
     assert c2[1] == 1 and c2[0] == 2
+
+
+STACK_D = (2.0, 3.0, 2.5, 2.5)      # slab thicknesses along z, mm
+STACK_L = (30.0, 50.0, 80.0, 40.0)  # their absorption lengths, mm
+
+
+@pytest.mark.parametrize("precision", [32, 64])
+def test_replay_gradient_across_four_materials(precision):
+    """Four touching slabs of equal refractive index (no reflection at the
+    internal faces), each its own material with absorption length L_k and
+    thickness d_k, photons isotropic on the bottom face, a SiPM of
+    efficiency p on top and black elsewhere.  With a = sum_k d_k / L_k,
+      E[N] = N0/2 p E_2(a),   dE[N]/dL_k = N0/2 p E_1(a) d_k / L_k^2.
+    Every detected path crosses all four materials, more than a replay's two
+    path slots hold, so this pins the per-material sums of long paths; a
+    length credited to the wrong material moves the four slopes apart."""
+    sc = Scene()
+    air = sc.add_material(Material(name="air", rindex=PropertyTable.constant_value(1.0)))
+    si = sc.add_material(Material(name="si", rindex=PropertyTable.constant_value(3.5)))
+    black = sc.add_surface(Surface(name="black", type=SurfaceType.DielectricMetal,
+                                   reflectivity=PropertyTable.constant_value(0.0)))
+    cathode = sc.add_surface(Surface(name="cathode", type=SurfaceType.DielectricMetal,
+                                     reflectivity=PropertyTable.constant_value(0.0),
+                                     efficiency=PropertyTable.constant_value(PDE)))
+    world = sc.add_volume(Volume(name="world", material_id=air, half_mm=(3000, 3000, 3000)))
+    sc.world_volume_id = world
+    total = sum(STACK_D)
+    z = -total / 2
+    mats, slabs = [], []
+    for k, (d, ln) in enumerate(zip(STACK_D, STACK_L)):
+        m = sc.add_material(Material(name=f"slab{k}", rindex=PropertyTable.constant_value(1.5),
+                                     abs_length_mm=PropertyTable(np.float32([2.0, 4.0]), np.float32([ln, ln]))))
+        v = sc.add_volume(Volume(name=f"slab{k}", material_id=m, parent_volume_id=world,
+                                 center_mm=(0, 0, z + d / 2), half_mm=(1000, 1000, d / 2)))
+        sc.add_surface_binding(SurfaceBinding(v, world, black))
+        mats.append(m)
+        slabs.append(v)
+        z += d
+    sc.add_volume(Volume(name="sipm", material_id=si, parent_volume_id=world, skin_surface_id=cathode, sensor_id=0,
+                         center_mm=(0, 0, total / 2 + 0.5), half_mm=(1000, 1000, 0.5)))
+    bottom = sc.add_volume(Volume(name="floor", material_id=si, parent_volume_id=world,
+                                  center_mm=(0, 0, -total / 2 - 0.5), half_mm=(1000, 1000, 0.5)))
+    sc.add_surface_binding(SurfaceBinding(slabs[0], bottom, black))
+    src = abi.SyntheticSource()
+    src.lo_mm[:] = [-1.0, -1.0, -total / 2]
+    src.hi_mm[:] = [1.0, 1.0, -total / 2]
+    src.wavelength_mean_nm, src.wavelength_sigma_nm, src.decay_time_ns = 420.0, 5.0, 0.0
+    src.volume_id = slabs[0]
+    n0 = 400_000
+    run = O.run_synthetic(sc, src, 19, [0], [n0], precision=precision, threads=8)
+    a = sum(d / ln for d, ln in zip(STACK_D, STACK_L))
+    expect_n = n0 / 2 * PDE * special.expn(2, a)
+    assert abs(len(run.hits) - expect_n) < 5 * np.sqrt(expect_n)
+    tables = [(abi.PARAM_ABS_LENGTH, m) for m in mats]
+    grad, counters = O.adjoint_synthetic(sc, src, 19, run.hits, np.ones(abi.HIST_SLOTS), tables, 8,
+                                         precision=precision)
+    assert counters[0] == len(run.hits) and counters[1] == 0
+    assert counters[2] == len(run.hits)  # every path is longer than two materials
+    e1 = special.expn(1, a)
+    for k, (d, ln) in enumerate(zip(STACK_D, STACK_L)):
+        np.testing.assert_allclose(grad[2 * k] + grad[2 * k + 1], n0 / 2 * PDE * e1 * d / ln**2, rtol=0.03,
+                                   err_msg=f"slab {k}")
