@@ -80,6 +80,11 @@ def parse():
     return ap.parse_args()
 
 
+def progress(msg: str) -> None:
+    """A line on stderr as each part of the bench ends (stdout holds only the result)."""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
 def run_form(host, cs, src, photons, launch, steps, warmup):
     from g4go_amd.transport import LaunchForm
 
@@ -462,6 +467,7 @@ def main():
             with open(os.path.join(out_dir, "autotune_table.json"), "w") as f:
                 json.dump(table, f, indent=0)
     wall, kms, out = run_form(host, cs, src, args.photons, persistent, args.steps, args.warmup)
+    progress(f"headline: {args.photons * args.steps / wall:.4g} photons/s, kernel {float(np.mean(kms)):.3f} ms")
     if args.headline_only:
         args.no_plain = args.no_adjoint = args.no_array = args.no_c1 = args.no_mesh = True
     plain = None
@@ -469,6 +475,7 @@ def main():
         p_wall, p_kms, p_out = run_form(host, cs, src, args.photons, PLAIN, args.steps, args.warmup)
         plain = {"photons_per_s": args.photons * args.steps / p_wall, "ms_per_step": 1e3 * p_wall / args.steps,
                  "kernel_ms": float(np.mean(p_kms))}
+        progress(f"plain: kernel {plain['kernel_ms']:.3f} ms")
     # uneven events: 4096 log-normal events per batch, batched (persistent + hipGraph) vs plain per event
     counts = workloads.lognormal_counts(4096, 5000, sigma=1.5, lo=10, hi=1_000_000, seed=args.seed)
     n_uneven = int(counts.astype(np.int64).sum())
@@ -501,17 +508,23 @@ def main():
                        "plain_per_event_photons_per_s": n_uneven * args.steps / e_wall,
                        "plain_per_event_kernel_ms": e_kms,
                        "batched_over_plain_per_event": e_wall / b_wall})
+    progress("uneven events done")
     host.close()
     adjoint = None if args.no_adjoint else run_adjoint(cs, src, args.seed, persistent, args.steps, args.warmup)
+    progress("adjoint done")
     array = None if args.no_array else run_array(args.seed, args.steps, args.warmup, not args.no_plain,
                                                      retune=args.retune)
+    progress("array done")
     c1 = None if args.no_c1 else run_c1(args.seed, args.steps, args.warmup, not args.no_cpu, 200,
                                                retune=args.retune)
+    progress("electron events done")
     host_batches = None if args.no_c1 else run_host_batches(args.seed, persistent)
     dropin_cpp = None if args.no_c1 else run_dropin_cpp()
+    progress("drop-in batches done")
     mesh = None if args.no_mesh else run_mesh(args.seed, args.steps, args.warmup, args.photons,
                                               float(np.mean(kms)) / out.statistics["total_steps"],
                                               retune=args.retune)
+    progress("mesh crystals done")
 
     st = out.statistics
     P, S, D = st["generated"], st["total_steps"], st["n_hits"]

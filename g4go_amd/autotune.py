@@ -20,12 +20,14 @@ from .transport import HipTransportHost, LaunchForm
 
 CACHE = Path(__file__).resolve().parent / "autotune_cache.json"
 
-# 256-thread blocks only: the kernels' static LDS (hit stages, histogram) is
-# laid out for four waves per block, so smaller blocks waste LDS per wave and,
-# next to a large staged scene, lose occupancy.
+# Block sizes of 2, 4 and 8 waves: each wave's hit stage is sized with the
+# block in dynamic LDS, so the block size trades the per-block scene copy
+# (staged once per block) against the granularity of residency on a CU.
+# Candidates ask for 8 to 32 waves per CU (the kernels run at most 16,
+# four per SIMD, so deeper grids only queue blocks behind the resident ones).
 DEFAULT_SPACE = {
-    "block_size": (256,),
-    "blocks_per_cu": (2, 4, 6, 8, 12),
+    "block_size": (128, 256, 512),
+    "blocks_per_cu": (2, 4, 6, 8, 12, 16),
     "chunk": (256, 1024, 4096),
     "refill_threshold": (8, 16, 24, 32),
 }
@@ -35,7 +37,7 @@ def candidates(space: Optional[dict] = None) -> Iterable[LaunchForm]:
     space = space or DEFAULT_SPACE
     for bs, bpc, chunk, refill in itertools.product(space["block_size"], space["blocks_per_cu"], space["chunk"],
                                                     space["refill_threshold"]):
-        if bs * bpc > 2048:  # at most 32 waves per CU
+        if not 512 <= bs * bpc <= 2048:  # 8 to 32 waves per CU
             continue
         yield LaunchForm(persistent=True, block_size=bs, chunk=chunk, refill_threshold=refill, blocks_per_cu=bpc)
 

@@ -1559,8 +1559,8 @@ g4go_status guarded(Core* h, F&& f) {
 }
 
 void check_launch(const g4go_launch& l) {
-    if (l.block_size != 64 && l.block_size != 128 && l.block_size != 256) {
-        fail(G4GO_ERROR_INVALID_ARGUMENT, "block_size must be 64, 128 or 256");
+    if (l.block_size != 64 && l.block_size != 128 && l.block_size != 256 && l.block_size != 512) {
+        fail(G4GO_ERROR_INVALID_ARGUMENT, "block_size must be 64, 128, 256 or 512");
     }
     if (l.persistent && (l.chunk == 0 || l.chunk > (1u << 20))) {
         fail(G4GO_ERROR_INVALID_ARGUMENT, "chunk must be in [1, 2^20]");

@@ -8,9 +8,9 @@
 namespace g4go {
 
 // Dynamic LDS budget per block for the staged scene (BVH nodes, daughter
-// boxes, property pool, scene records).  With the ~14 KB of static LDS, four
-// 256-thread blocks (16 waves, the register-limited occupancy) still fit in
-// 160 KB.
+// boxes, property pool, scene records).  With the histogram (1 KB) and the
+// waves' hit stages (3.25 KB per wave), four 256-thread blocks (16 waves, the
+// register-limited occupancy) still fit in 160 KB.
 constexpr uint32_t kLdsSceneBytes = 26112;
 constexpr int kHistBins = 256;
 constexpr int kHistSlots = kHistBins + 2;  // + underflow (256) + overflow (257)
@@ -87,7 +87,7 @@ struct KernelParams {
     uint32_t lds_entries;
     uint32_t lds_pool_floats;
     uint32_t lds_records;
-    uint32_t pad_lds;
+    uint32_t lds_stage_quads;  // 16-byte word of dynamic LDS where the hit stages begin (set at launch)
     // Persistent form: the start states of the batch's photons, generated by
     // generate_kernel before the transport launch, three float4 per photon in
     // launch order (StartState); null: idle lanes generate their photons at
@@ -126,6 +126,12 @@ struct AdjointParams {
 };
 
 constexpr uint32_t kLdsVolumes = 1u, kLdsSurfaces = 2u, kLdsMaterials = 4u, kLdsPlanes = 8u;
+
+// Threads per block of the transport kernels: 64 to kMaxBlock, whole waves.
+// Each wave of a block has its own hit stage in dynamic LDS, ahead of the
+// staged scene: 64 hit records of 48 bytes and their 64 launch indices.
+constexpr uint32_t kMaxBlock = 512;
+constexpr uint32_t kStageBytesPerWave = 64u * 48u + 64u * 4u;
 
 // Dynamic LDS bytes of a transport launch (the staged scene).
 __host__ __device__ inline size_t transport_lds_bytes(const KernelParams& p) {

@@ -144,7 +144,7 @@ __device__ __forceinline__ void generate(Photon& ph, const KernelParams& p, cons
 // step (a single global counter serialises at the L2 otherwise).
 constexpr uint32_t kStageCap = 64;    // staged records per wave (a step adds at most 64)
 constexpr uint32_t kStageFlush = 32;  // flush once this many are staged
-constexpr int kMaxWavesPerBlock = 4;
+static_assert(kStageBytesPerWave == 3u * kStageCap * 16u + kStageCap * 4u, "stage layout");
 
 __device__ __forceinline__ void flush_stage(const KernelParams& p, const float4* stage, const uint32_t* src,
                                             uint32_t staged, uint32_t lane) {
@@ -304,18 +304,19 @@ __device__ __forceinline__ void stage_scene(const KernelParams& p, float4* lds, 
 template <int kSource, bool kPersistent, int kLdsMode, int kGeom, bool kPregen = false>
 __device__ __forceinline__ void transport_body(const KernelParams& p) {
     __shared__ uint32_t s_hist[kHistSlots];
-    __shared__ float4 s_stage[kMaxWavesPerBlock][3 * kStageCap];
-    __shared__ uint32_t s_stage_src[kMaxWavesPerBlock][kStageCap];
-    extern __shared__ float4 s_scene[];  // BVH nodes, daughter boxes, property pool (transport_lds_bytes)
+    // dynamic LDS: the staged scene (transport_lds_bytes), then from word
+    // lds_stage_quads each wave's hit stage (kStageCap records of three
+    // 16-byte words, then their kStageCap launch indices)
+    extern __shared__ float4 s_dynamic[];
     for (int i = threadIdx.x; i < kHistSlots; i += blockDim.x) {
         s_hist[i] = 0u;
     }
     SceneView sv = p.scene;
-    stage_scene<kLdsMode, kGeom>(p, s_scene, sv);
+    stage_scene<kLdsMode, kGeom>(p, s_dynamic, sv);
     G4GO_REGION_ZERO();
     __syncthreads();
-    float4* const stage = s_stage[threadIdx.x >> 6];
-    uint32_t* const stage_src = s_stage_src[threadIdx.x >> 6];
+    float4* const stage = s_dynamic + p.lds_stage_quads + uniform(threadIdx.x >> 6) * (kStageBytesPerWave / 16u);
+    uint32_t* const stage_src = reinterpret_cast<uint32_t*>(stage + 3u * kStageCap);
     uint32_t staged = 0;  // wave-uniform
 
     const uint32_t lane = lane_id();
@@ -492,7 +493,7 @@ __device__ __forceinline__ void transport_body(const KernelParams& p) {
 // fourth wave hides more of the dependent scene-load latency of a step than
 // the few spills it forces cost.
 template <int kSource, int kLdsMode, int kGeom, bool kPregen>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
+__global__ void __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(4)))
 transport_kernel_persistent(KernelParams p) {
     transport_body<kSource, true, kLdsMode, kGeom, kPregen>(p);
 }
@@ -523,7 +524,7 @@ __global__ void __launch_bounds__(256) generate_kernel(KernelParams p, float4* s
 // The plain form gets the persistent form's register cap (four waves per
 // SIMD), so that the two forms differ only in the work queue and refill.
 template <int kSource, int kLdsMode, int kGeom>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
+__global__ void __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(4)))
 transport_kernel_plain(KernelParams p) {
     transport_body<kSource, false, kLdsMode, kGeom>(p);
 }
@@ -638,14 +639,18 @@ int launch_zero(void* ptr, size_t bytes, hipStream_t stream) {
 }
 
 int launch_transport(const KernelParams& p, int source, bool persistent, int block, int grid, hipStream_t stream) {
-    const size_t lds = transport_lds_bytes(p);
-    if (lds > kLdsSceneBytes || (p.starts != nullptr && geom_level(p.scene) == kGeomTriangles)) {
+    const size_t scene_lds = transport_lds_bytes(p);
+    if (scene_lds > kLdsSceneBytes || (p.starts != nullptr && geom_level(p.scene) == kGeomTriangles) || block < 64 ||
+        block > static_cast<int>(kMaxBlock) || block % 64 != 0) {
         return 1;
     }
+    KernelParams q = p;
+    q.lds_stage_quads = static_cast<uint32_t>((scene_lds + 15u) / 16u);
+    const size_t lds = static_cast<size_t>(q.lds_stage_quads) * 16u + static_cast<size_t>(block / 64) * kStageBytesPerWave;
     if (source == kSourceSynthetic) {
-        launch_transport_source<kSourceSynthetic>(p, persistent, block, grid, lds, stream);
+        launch_transport_source<kSourceSynthetic>(q, persistent, block, grid, lds, stream);
     } else {
-        launch_transport_source<kSourceEmission>(p, persistent, block, grid, lds, stream);
+        launch_transport_source<kSourceEmission>(q, persistent, block, grid, lds, stream);
     }
     return hipGetLastError() == hipSuccess ? 0 : 1;
 }

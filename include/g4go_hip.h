@@ -71,7 +71,7 @@ typedef struct g4go_config {
  *     still fit. */
 typedef struct g4go_launch {
     uint32_t persistent;
-    uint32_t block_size;       /* 64, 128, 256 */
+    uint32_t block_size;       /* 64, 128, 256, 512 */
     uint32_t chunk;            /* photons per work item (persistent form) */
     uint32_t refill_threshold; /* 1..64 */
     uint32_t blocks_per_cu;    /* persistent grid = blocks_per_cu * CU count */

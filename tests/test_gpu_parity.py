@@ -117,6 +117,26 @@ def test_synthetic_crystal_bit_exact(crystal, form):
     assert_same_stats(out, ref)
 
 
+@pytest.mark.parametrize("persistent", [False, True])
+@pytest.mark.parametrize("block", [64, 128, 512])
+def test_block_sizes_bit_exact(persistent, block):
+    """Blocks of 1, 2 and 8 waves on the septa array (a staged BVH and
+    property pool ahead of the waves' hit stages in dynamic LDS)."""
+    log(f"case: array, block {block}, {'persistent' if persistent else 'plain'}")
+    cs = workloads.array_scene()
+    em, ev, idx, _ = workloads.array_events(cs, 6, median=3000, seed=3)
+    ref = O.run_emissions(cs.scene, em, ev, idx, 61, threads=4)
+    with host(61) as h:
+        h.set_launch(LaunchForm(persistent=persistent, block_size=block, chunk=256, refill_threshold=16,
+                                blocks_per_cu=max(1, 1024 // block)))
+        assert h.launch().block_size == block
+        h.enqueue_emissions(cs.scene, em, ev, idx)
+        out = h.complete_oldest_batch()
+    assert out.statistics["dropped_hits"] == 0
+    assert_same_hits(out.detections, ref.hits)
+    assert_same_stats(out, ref)
+
+
 def test_emission_crystal_bit_exact(crystal):
     log("case: scintillation emission records")
     em, event_ids, idx = workloads.electron_emissions(crystal, 4, seed=5)
