@@ -29,7 +29,7 @@ DEFAULT_SPACE = {
     "block_size": (128, 256, 512),
     "blocks_per_cu": (2, 4, 6, 8, 12, 16),
     "chunk": (256, 1024, 4096),
-    "refill_threshold": (8, 16, 24, 32),
+    "refill_threshold": (2, 4, 8, 16, 24),
 }
 
 

@@ -1242,8 +1242,6 @@ struct Core {
         p.cfg = StepConfig{cfg.boundary_tolerance_mm};
         p.max_steps = cfg.max_bounces_per_photon;
         p.refill_threshold = std::max<uint32_t>(1, std::min<uint32_t>(64, launch.refill_threshold));
-        p.hist_lo_ns = 0.0;
-        p.hist_bins_per_ns = kHistBins / 200.0;
         // Every block stages scene parts in LDS, each when it still fits, in
         // the order a step needs them: the volume and surface records and the
         // mesh planes (read every step), the BVH nodes and daughter boxes
@@ -2323,7 +2321,7 @@ g4go_status g4go_create(const g4go_config* config, g4go_transport** out) {
         for (uint32_t k = 0; k < n; ++k) {
             auto c = std::make_unique<Core>();
             c->cfg = *config;
-            c->launch = g4go_launch{1, 256, 256, 24, 8, 0, 0, 0};  // persistent, tuned on crystal batches
+            c->launch = g4go_launch{1, 512, 256, 4, 2, 0, 0, 0};  // persistent, tuned on crystal batches
             t->cores.push_back(std::move(c));
         }
         *out = t.release();

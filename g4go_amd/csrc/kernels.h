@@ -14,6 +14,9 @@ namespace g4go {
 constexpr uint32_t kLdsSceneBytes = 26112;
 constexpr int kHistBins = 256;
 constexpr int kHistSlots = kHistBins + 2;  // + underflow (256) + overflow (257)
+// The arrival-time histogram: kHistBins bins of 200 ns / kHistBins from 0 ns.
+constexpr double kHistLoNs = 0.0;
+constexpr double kHistBinsPerNs = kHistBins / 200.0;
 
 // Work is cut into items of `chunk` consecutive photons of one source record
 // (one event of a synthetic batch, or one emission record).  Items are not
@@ -76,7 +79,6 @@ struct KernelParams {
     uint32_t event_stride;
     uint32_t pad_es;
     unsigned long long* counters;
-    double hist_lo_ns, hist_bins_per_ns;  // bin = floor((t - lo) * bins_per_ns)
     // Scene parts staged into LDS by every block, in this order (each only
     // when it fits; otherwise read in global memory): BVH nodes + daughter
     // boxes (lds_entries 32-byte entries), the property pool (lds_pool_floats
@@ -88,6 +90,10 @@ struct KernelParams {
     uint32_t lds_pool_floats;
     uint32_t lds_records;
     uint32_t lds_stage_quads;  // 16-byte word of dynamic LDS where the hit stages begin (set at launch)
+    // Persistent form: ready-photon ring entries per wave, in dynamic LDS
+    // after the hit stages (set at launch from the LDS left over, at most
+    // kRingMaxSlots; 0: idle lanes only get photons from generation passes)
+    uint32_t ring_slots;
     // Persistent form: the start states of the batch's photons, generated by
     // generate_kernel before the transport launch, three float4 per photon in
     // launch order (StartState); null: idle lanes generate their photons at
@@ -132,6 +138,16 @@ constexpr uint32_t kLdsVolumes = 1u, kLdsSurfaces = 2u, kLdsMaterials = 4u, kLds
 // staged scene: 64 hit records of 48 bytes and their 64 launch indices.
 constexpr uint32_t kMaxBlock = 512;
 constexpr uint32_t kStageBytesPerWave = 64u * 48u + 64u * 4u;
+
+// Ready-photon ring of the persistent form: 16-byte words per entry, entries
+// per wave at most.  A launch gives the ring what its blocks' LDS share leaves
+// after the static LDS (at most kLdsStaticReserve), the staged scene and the hit
+// stages, with kResidentWaves waves resident per CU (the register-limited
+// occupancy of the persistent kernels).
+constexpr uint32_t kRingQuads = 6;
+constexpr uint32_t kRingMaxSlots = 64;
+constexpr uint32_t kResidentWaves = 16;
+constexpr uint32_t kLdsStaticReserve = 4096;
 
 // Dynamic LDS bytes of a transport launch (the staged scene).
 __host__ __device__ inline size_t transport_lds_bytes(const KernelParams& p) {

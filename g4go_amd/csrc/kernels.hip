@@ -49,6 +49,26 @@ __device__ __forceinline__ uint32_t wave_max(uint32_t v) {
     return v;
 }
 
+// The block's copy of the per-event counters (computed where a flush needs
+// it rather than held through the transport loop).
+__device__ __forceinline__ EventStats* event_copy(const KernelParams& p) {
+    return p.event_stats + static_cast<size_t>(blockIdx.x % kEventReplicas) * p.event_stride;
+}
+
+// The launch's parameters, re-derived opaquely (an empty asm statement on the
+// kernel-argument pointer, no instruction): loads through the result are
+// scalar loads of the kernel-argument segment where the transport loop uses
+// them, not values the compiler hoists into SGPRs for the whole loop and then
+// spills into VGPR lanes (every reload of a spilled value is a VALU
+// instruction; the queue, hit-list and source fields are used only at
+// refills and flushes).
+__device__ __forceinline__ const KernelParams& fresh_params() {
+    typedef const KernelParams __attribute__((address_space(4))) ArgParams;
+    ArgParams* kp = (ArgParams*)__builtin_amdgcn_kernarg_segment_ptr();
+    asm volatile("" : "+s"(kp));
+    return *(const KernelParams*)kp;
+}
+
 struct LaneStats {
     uint32_t detected, absorbed, escaped, truncated, invalid, zero_step, max_bounce;
 };
@@ -94,11 +114,12 @@ __device__ __forceinline__ void flush_event_group(EventStats* es, uint32_t event
     }
 }
 
-// Histogram slot of arrival time t: bin floor((t - lo) * bins_per_ns), then
-// underflow (and NaN) and overflow slots.  A product rather than a division by
-// the bin width keeps an FP64 divide out of every detection.
-__device__ __forceinline__ int hist_slot(double t, double lo, double bins_per_ns) {
-    const double x = (t - lo) * bins_per_ns;
+// Histogram slot of arrival time t: bin floor((t - kHistLoNs) *
+// kHistBinsPerNs), then underflow (and NaN) and overflow slots.  A product
+// rather than a division by the bin width keeps an FP64 divide out of every
+// detection.
+__device__ __forceinline__ int hist_slot(double t) {
+    const double x = (t - kHistLoNs) * kHistBinsPerNs;
     if (!(x >= 0.0)) {
         return kHistBins;  // underflow (and NaN)
     }
@@ -226,6 +247,65 @@ __device__ __forceinline__ bool load_start(Photon& ph, const KernelParams& p, ui
     return a.w == a.w;
 }
 
+// Ring entry of a ready photon (kRingQuads 16-byte words): position and
+// energy (NaN: start rejected); direction and the time's low word;
+// polarisation and its high word; photon id, event id, volume, material; the
+// cached bulk properties; hot-grid node and weight, launch index (a parked
+// photon: its step) and event index.
+__device__ __forceinline__ void ring_put(float4* s, const Photon& ph, float energy, uint32_t word, uint32_t event) {
+    const uint64_t tb = static_cast<uint64_t>(__double_as_longlong(ph.time));
+    s[0] = make_float4(ph.pos.x, ph.pos.y, ph.pos.z, energy);
+    s[1] = make_float4(ph.dir.x, ph.dir.y, ph.dir.z, __uint_as_float(static_cast<uint32_t>(tb)));
+    s[2] = make_float4(ph.pol.x, ph.pol.y, ph.pol.z, __uint_as_float(static_cast<uint32_t>(tb >> 32)));
+    s[3] = make_float4(__uint_as_float(ph.photon_id), __uint_as_float(ph.event_id), __uint_as_float(ph.volume),
+                       __uint_as_float(ph.material));
+    s[4] = make_float4(ph.rindex, ph.inv_velocity, ph.interaction_length, ph.absorb_fraction);
+    s[5] = make_float4(__uint_as_float(ph.grid_i), ph.grid_f, __uint_as_float(word), __uint_as_float(event));
+}
+
+__device__ __forceinline__ void ring_get(const float4* s, Photon& ph, uint32_t& word, uint32_t& event) {
+    const float4 a = s[0], b = s[1], c = s[2], d = s[3], e = s[4], f = s[5];
+    ph.pos = V3{a.x, a.y, a.z};
+    ph.energy = a.w;
+    ph.dir = V3{b.x, b.y, b.z};
+    ph.pol = V3{c.x, c.y, c.z};
+    ph.time = __longlong_as_double(
+        static_cast<long long>((static_cast<uint64_t>(__float_as_uint(c.w)) << 32) | __float_as_uint(b.w)));
+    ph.photon_id = __float_as_uint(d.x);
+    ph.event_id = __float_as_uint(d.y);
+    ph.volume = __float_as_uint(d.z);
+    ph.material = __float_as_uint(d.w);
+    ph.rindex = e.x;
+    ph.inv_velocity = e.y;
+    ph.interaction_length = e.z;
+    ph.absorb_fraction = e.w;
+    ph.grid_i = __float_as_uint(f.x);
+    ph.grid_f = f.y;
+    word = __float_as_uint(f.z);
+    event = __float_as_uint(f.w);
+}
+
+// A lane's transported photon, parked in its ring slot during a generation pass.
+__device__ __forceinline__ void ring_park(float4* s, const Photon& ph) { ring_put(s, ph, ph.energy, ph.step, kInvalid); }
+
+// Swaps the freshly generated photon (launch index, event index; ok: its
+// start was accepted) into the slot for the photon parked there.
+__device__ __forceinline__ void ring_swap(float4* s, Photon& ph, bool ok, uint32_t launch, uint32_t event) {
+    Photon parked;
+    uint32_t step, unused;
+    ring_get(s, parked, step, unused);
+    ring_put(s, ph, ok ? ph.energy : u2f(0x7fc00000u), launch, event);
+    ph = parked;
+    ph.step = step;
+}
+
+// Takes a ready photon from a ring slot; false for a rejected start.
+__device__ __forceinline__ bool ring_take(const float4* s, Photon& ph, uint32_t& launch, uint32_t& event) {
+    ring_get(s, ph, launch, event);
+    ph.step = 0;
+    return ph.energy == ph.energy;
+}
+
 }  // namespace
 
 // kLdsMode: which scene parts a launch guarantees are staged in LDS
@@ -325,7 +405,6 @@ __device__ __forceinline__ void transport_body(const KernelParams& p) {
     const uint32_t first_item = p.item_end != 0u ? p.item_begin : 0u;
     const uint32_t end_item = p.item_end != 0u ? p.item_end : uniform(p.item_offsets[n_sources]);
 
-    EventStats* const es = p.event_stats + static_cast<size_t>(blockIdx.x % kEventReplicas) * p.event_stride;
 
     Photon ph;
     ph.material = kInvalid;
@@ -335,27 +414,60 @@ __device__ __forceinline__ void transport_body(const KernelParams& p) {
     LaneStats st{0, 0, 0, 0, 0, 0, 0};
     unsigned long long wave_steps = 0, wave_photons = 0;
 
-    // wave-uniform queue cursor
+    // wave-uniform queue cursor (persistent form)
     uint32_t it_event = 0, it_source = 0, it_first = 0, it_next = 0, it_end = 0;
     bool done = false;
-    if (!kPersistent) {
+    if constexpr (!kPersistent) {
+        // plain form: the wave's one work item, a photon per lane, generated
+        // before the transport loop
         const uint32_t wave = first_item + blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
         if (wave < end_item) {
             const ItemRef item = locate_item<kSource>(p, uniform(wave), n_sources, chunk);
-            it_event = item.event_index;
-            it_source = item.source_index;
-            it_first = item.first;
-            it_end = item.count;
+            wave_photons = item.count;
+            if (lane < item.count) {
+                const uint32_t local = item.first + lane;
+                my_event = item.event_index;
+                my_launch = p.source_photon_offset[item.source_index] + local;
+                bool ok;
+                if (kPregen) {
+                    ok = load_start<kSource>(ph, p, item.event_index, item.source_index, local, my_launch);
+                } else {
+                    generate<kSource>(ph, p, sv, item.event_index, item.source_index, local);
+                    ph.material = kInvalid;
+                    ok = start_photon(ph);
+                }
+                if (ok) {
+                    prepare_start(ph, sv);
+                    alive = true;
+                } else {
+                    ++st.invalid;
+                }
+            }
         }
         done = true;
     }
 
+    // Persistent form: the wave's ring of ready photons in dynamic LDS, after
+    // the block's hit stages (ring_slots entries of kRingQuads words).  A
+    // generation pass fills it through the alive lanes; lanes that go idle
+    // later take its entries [0, ring_count) from the top.
+    constexpr bool kInLaneRefill = kPersistent && kGeom == kGeomTriangles;
+    const uint32_t ring_slots = kPersistent && !kInLaneRefill ? uniform(p.ring_slots) : 0u;
+    float4* const ring = s_dynamic + p.lds_stage_quads + (blockDim.x >> 6) * (kStageBytesPerWave / 16u) +
+                         uniform(threadIdx.x >> 6) * ring_slots * kRingQuads;
+    uint32_t ring_count = 0;  // wave-uniform
+
+    const KernelParams& p_arg = p;
     for (;;) {
-        // ---- refill idle lanes ----
+        // (the kernel's only argument is the KernelParams that p refers to)
+        const KernelParams& p = kPersistent ? fresh_params() : p_arg;
+        // ---- refill idle lanes (persistent form) ----
         uint64_t idle = __ballot(!alive);
         uint32_t n_idle = static_cast<uint32_t>(__popcll(idle));
-        if (!kPersistent || n_idle >= p.refill_threshold || n_idle == 64u) {
-            G4GO_REGION_BEGIN(kRegionRefill);
+        if (kInLaneRefill && (n_idle >= p.refill_threshold || n_idle == 64u)) {
+            // triangle-solid instances: idle lanes generate their photons in
+            // place, item by item (the ring's LDS traffic per photon costs more
+            // than it saves at their few steps per photon)
             while (n_idle > 0u) {
                 if (it_next >= it_end) {
                     if (done) {
@@ -381,36 +493,156 @@ __device__ __forceinline__ void transport_body(const KernelParams& p) {
                 const uint32_t take = min(it_end - it_next, n_idle);
                 const uint32_t rank = rank_below(idle);
                 const bool fill = !alive && rank < take;
-                flush_event_group(es, my_event, st, fill && my_event != it_event && my_event != kInvalid);
+                flush_event_group(event_copy(p), my_event, st, fill && my_event != it_event && my_event != kInvalid);
                 if (fill) {
-                    G4GO_REGION_BEGIN(kRegionGenerate);
                     my_event = it_event;
                     my_launch = p.source_photon_offset[it_source] + it_first + it_next + rank;
-                    bool ok;
-                    if (kPregen) {
-                        ok = load_start<kSource>(ph, p, it_event, it_source, it_first + it_next + rank, my_launch);
-                    } else {
-                        generate<kSource>(ph, p, sv, it_event, it_source, it_first + it_next + rank);
-                        ph.material = kInvalid;
-                        ok = start_photon(ph);
-                    }
-                    if (ok) {
+                    generate<kSource>(ph, p, sv, it_event, it_source, it_first + it_next + rank);
+                    ph.material = kInvalid;
+                    if (start_photon(ph)) {
                         alive = true;
                     } else {
                         ++st.invalid;
                     }
-                    G4GO_REGION_END(kRegionGenerate);
                 }
                 it_next += take;
                 wave_photons += take;
                 idle = __ballot(!alive);
                 n_idle = static_cast<uint32_t>(__popcll(idle));
             }
+        } else if (kPersistent && !kInLaneRefill && (n_idle >= p.refill_threshold || n_idle == 64u)) {
+            G4GO_REGION_BEGIN(kRegionRefill);
+            // (1) idle lanes take ready photons from the top of the ring
+            if (ring_count != 0u) {
+                const uint32_t take = min(ring_count, n_idle);
+                const uint32_t rank = rank_below(idle);
+                const bool fill = !alive && rank < take;
+                uint32_t event = kInvalid, launch = kInvalid;
+                bool ok = false;
+                if (fill) {
+                    ok = ring_take(ring + (ring_count - 1u - rank) * kRingQuads, ph, launch, event);
+                }
+                flush_event_group(event_copy(p), my_event, st, fill && my_event != event && my_event != kInvalid);
+                if (fill) {
+                    my_event = event;
+                    my_launch = launch;
+                    if (ok) {
+                        alive = true;
+                    } else {
+                        ++st.invalid;
+                    }
+                }
+                ring_count -= take;
+                idle = __ballot(!alive);
+                n_idle = static_cast<uint32_t>(__popcll(idle));
+            }
+            // (2) generation pass, once the ring is empty: the photons of the
+            // idle lanes and, through up to ring_slots alive lanes, the ring's
+            // next photons are generated together on the wave
+            if (n_idle != 0u && ring_count == 0u && !(done && it_next >= it_end)) {
+                G4GO_REGION_BEGIN(kRegionGenerate);
+                const uint32_t to_ring = min(64u - n_idle, ring_slots);
+                const uint32_t n_want = n_idle + to_ring;
+                const uint32_t rank = alive ? rank_below(~idle) : rank_below(idle);
+                const bool ring_lane = alive && rank < to_ring;
+                const uint32_t want = alive ? n_idle + rank : rank;  // the lane's place in the pass
+                // (a) claim photons from the queue in pass order
+                uint32_t g_event = kInvalid, g_source = 0, g_local = 0;
+                bool got = false;
+                uint32_t claimed = 0;  // wave-uniform
+                uint32_t items_used = 0;  // wave-uniform: work items the pass draws from
+                while (claimed < n_want) {
+                    if (it_next >= it_end) {
+                        if (done) {
+                            break;
+                        }
+                        uint32_t c = 0;
+                        if (lane == 0u) {
+                            c = atomicAdd(reinterpret_cast<uint32_t*>(p.counters + kCounterQueueHead), 1u);
+                        }
+                        c = first_item + uniform(static_cast<uint32_t>(__shfl(static_cast<int>(c), 0)));
+                        if (c >= end_item) {
+                            done = true;
+                            break;
+                        }
+                        const ItemRef item = locate_item<kSource>(p, c, n_sources, chunk);
+                        it_event = item.event_index;
+                        it_source = item.source_index;
+                        it_first = item.first;
+                        it_next = 0;
+                        it_end = item.count;
+                        continue;
+                    }
+                    const uint32_t take = min(it_end - it_next, n_want - claimed);
+                    ++items_used;
+                    if ((!alive || ring_lane) && want >= claimed && want - claimed < take) {
+                        got = true;
+                        g_event = it_event;
+                        g_source = it_source;
+                        g_local = it_first + it_next + (want - claimed);
+                    }
+                    it_next += take;
+                    claimed += take;
+                }
+                wave_photons += claimed;
+                // (b) a ring lane parks its own photon in its slot while it
+                // generates the new one
+                float4* const slot = ring + rank * kRingQuads;
+                const bool swap = ring_lane && got;
+                if (swap) {
+                    ring_park(slot, ph);
+                }
+                // (c) generation and the first step's property setup
+                bool ok = false;
+                uint32_t g_launch = kInvalid;
+                if (items_used == 1u) {
+                    // one work item: its event and source are wave-uniform
+                    if (got) {
+                        g_launch = p.source_photon_offset[it_source] + g_local;
+                        if (kPregen) {
+                            ok = load_start<kSource>(ph, p, it_event, it_source, g_local, g_launch);
+                        } else {
+                            generate<kSource>(ph, p, sv, it_event, it_source, g_local);
+                            ph.material = kInvalid;
+                            ok = start_photon(ph);
+                        }
+                    }
+                } else if (got) {
+                    g_launch = p.source_photon_offset[g_source] + g_local;
+                    if (kPregen) {
+                        ok = load_start<kSource>(ph, p, g_event, g_source, g_local, g_launch);
+                    } else {
+                        generate<kSource>(ph, p, sv, g_event, g_source, g_local);
+                        ph.material = kInvalid;
+                        ok = start_photon(ph);
+                    }
+                }
+                if (got && ok) {
+                    prepare_start(ph, sv);
+                }
+                // (d) the new photon into the slot, the parked one back
+                if (swap) {
+                    ring_swap(slot, ph, ok, g_launch, g_event);
+                }
+                const bool fresh = !alive && got;
+                flush_event_group(event_copy(p), my_event, st, fresh && my_event != g_event && my_event != kInvalid);
+                if (fresh) {
+                    my_event = g_event;
+                    my_launch = g_launch;
+                    if (ok) {
+                        alive = true;
+                    } else {
+                        ++st.invalid;
+                    }
+                }
+                ring_count = claimed > n_idle ? claimed - n_idle : 0u;
+                G4GO_REGION_END(kRegionGenerate);
+            }
             G4GO_REGION_END(kRegionRefill);
         }
         const uint64_t alive_lanes = __ballot(alive);
         if (alive_lanes == 0ull) {
-            if (done) {
+            if (done && it_next >= it_end && ring_count == 0u) {
                 break;
             }
             continue;
@@ -445,7 +677,7 @@ __device__ __forceinline__ void transport_body(const KernelParams& p) {
                 const uint32_t slot = staged + rank_below(det);
                 store_hit(stage + 3u * slot, ph, sensor);
                 stage_src[slot] = my_launch;
-                atomicAdd(&s_hist[hist_slot(ph.time, p.hist_lo_ns, p.hist_bins_per_ns)], 1u);
+                atomicAdd(&s_hist[hist_slot(ph.time)], 1u);
             }
             staged += n_det;
             if (staged >= kStageFlush) {
@@ -474,10 +706,10 @@ __device__ __forceinline__ void transport_body(const KernelParams& p) {
     if (staged != 0u) {
         flush_stage(p, stage, stage_src, staged, lane);
     }
-    flush_event_group(es, my_event, st, my_event != kInvalid);
+    flush_event_group(event_copy(p), my_event, st, my_event != kInvalid);
     if (lane == 0u) {
-        if (wave_steps) atomicAdd(p.counters + kCounterSteps, wave_steps);
-        if (wave_photons) atomicAdd(p.counters + kCounterPhotons, wave_photons);
+        if (wave_steps) atomicAdd(p.counters + kCounterSteps, static_cast<unsigned long long>(wave_steps));
+        if (wave_photons) atomicAdd(p.counters + kCounterPhotons, static_cast<unsigned long long>(wave_photons));
     }
     __syncthreads();
     for (int i = threadIdx.x; i < kHistSlots; i += blockDim.x) {
@@ -646,7 +878,26 @@ int launch_transport(const KernelParams& p, int source, bool persistent, int blo
     }
     KernelParams q = p;
     q.lds_stage_quads = static_cast<uint32_t>((scene_lds + 15u) / 16u);
-    const size_t lds = static_cast<size_t>(q.lds_stage_quads) * 16u + static_cast<size_t>(block / 64) * kStageBytesPerWave;
+    const size_t waves = static_cast<size_t>(block / 64);
+    size_t lds = static_cast<size_t>(q.lds_stage_quads) * 16u + waves * kStageBytesPerWave;
+    q.ring_slots = 0;
+    if (persistent && geom_level(p.scene) != kGeomTriangles) {
+        // the ring takes what the block's share of the CU's LDS leaves (the
+        // triangle-solid instances refill in place and have no ring)
+        int device = 0, per_cu = 0, per_block = 0;
+        if (hipGetDevice(&device) != hipSuccess ||
+            hipDeviceGetAttribute(&per_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, device) != hipSuccess ||
+            hipDeviceGetAttribute(&per_block, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess) {
+            return 1;
+        }
+        const size_t resident = std::max<size_t>(1, kResidentWaves / waves);
+        const size_t share = std::min(static_cast<size_t>(per_cu) / resident, static_cast<size_t>(per_block));
+        if (share > lds + kLdsStaticReserve) {
+            const size_t slots = (share - lds - kLdsStaticReserve) / (waves * kRingQuads * 16u);
+            q.ring_slots = static_cast<uint32_t>(std::min<size_t>(kRingMaxSlots, slots));
+        }
+        lds += waves * q.ring_slots * kRingQuads * 16u;
+    }
     if (source == kSourceSynthetic) {
         launch_transport_source<kSourceSynthetic>(q, persistent, block, grid, lds, stream);
     } else {
@@ -906,7 +1157,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) a
                 m1 = MatSlot{kInvalid, 0u, 0.0};
                 r0 = SurfSlot{kInvalid, 0u};
                 r1 = SurfSlot{kInvalid, 0u};
-                weight = a.slot_weight[hist_slot(hit.time, p.hist_lo_ns, p.hist_bins_per_ns)];
+                weight = a.slot_weight[hist_slot(hit.time)];
             } else {
                 ++mismatch;
             }

@@ -8,7 +8,7 @@
 //       waited for the SIMD while other waves issued;
 //   [1] wave entries (executions of the region by a wave);
 //   [2] active lanes at the end marker, summed over the entries.
-// With four waves sharing each SIMD and the kernel VALU-issue bound, a
+// With several waves sharing each SIMD and the kernel VALU-issue bound, a
 // region's share of the wave cycles approximates its share of the issued
 // VALU instructions, and [2] / (64 [1]) is the lane utilisation there.
 // Markers sit at single-entry points; an end marker inside divergent code
@@ -34,7 +34,9 @@ enum Region : int {
 };
 
 #if defined(G4GO_REGION_PROFILE)
-static __shared__ unsigned long long g4go_region_lds[4][kRegionCount][3];
+// one row per wave of the largest transport block (512 threads)
+constexpr int kRegionWaves = 8;
+static __shared__ unsigned long long g4go_region_lds[kRegionWaves][kRegionCount][3];
 static __device__ unsigned long long g4go_region_global[kRegionCount][3];
 #endif
 
@@ -55,7 +57,7 @@ __device__ __forceinline__ void region_end(int r, unsigned long long t0) {
 
 __device__ __forceinline__ void region_zero() {
     unsigned long long* s = &g4go_region_lds[0][0][0];
-    for (uint32_t i = threadIdx.x; i < 4u * kRegionCount * 3u; i += blockDim.x) {
+    for (uint32_t i = threadIdx.x; i < kRegionWaves * kRegionCount * 3u; i += blockDim.x) {
         s[i] = 0ull;
     }
 }
@@ -63,7 +65,7 @@ __device__ __forceinline__ void region_zero() {
 __device__ __forceinline__ void region_flush() {
     for (uint32_t i = threadIdx.x; i < kRegionCount * 3u; i += blockDim.x) {
         unsigned long long v = 0;
-        for (int w = 0; w < 4; ++w) {
+        for (int w = 0; w < kRegionWaves; ++w) {
             v += (&g4go_region_lds[w][0][0])[i];
         }
         if (v != 0ull) {

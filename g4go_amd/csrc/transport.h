@@ -1145,6 +1145,20 @@ G4GO_HD bool start_photon(Photon& ph) {
     return true;
 }
 
+// The first step's property setup, done when the photon is generated: its
+// node on the hot grid and the bulk properties of its start volume's material
+// (transport_step computes the same values when it finds them missing; a
+// photon whose volume or material cannot transport is left to fail there).
+G4GO_HD void prepare_start(Photon& ph, const SceneView& sc) {
+    if (ph.volume >= sc.n_volumes) {
+        return;
+    }
+    if (sc.hot_count != 0) {
+        grid_locate(sc.pool + sc.hot_offset, sc.hot_count, ph.energy, ph.grid_i, ph.grid_f);
+    }
+    cache_material(ph, sc, sc.volumes[ph.volume].material);
+}
+
 // ---------------------------------------------------------------------------
 // One transport step (one pass of the bounce loop, OptiXTransportDevice.cu:708-931).
 // ---------------------------------------------------------------------------

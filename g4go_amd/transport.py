@@ -79,17 +79,17 @@ class PhotonTransportConfig:
 @dataclass
 class LaunchForm:
     """Launch form of a handle (g4go_launch).  The defaults are those of a new
-    handle: the persistent kernel with 256-thread blocks, 256-photon work
-    items, refill at 24 idle lanes, 8 blocks per CU, photons generated at
+    handle: the persistent kernel with 512-thread blocks, 256-photon work
+    items, refill at 4 idle lanes, 2 blocks per CU, photons generated at
     refill (pregenerate off).  The plain form
     (`LaunchForm(persistent=False)`) runs one wave per item of at most 64
     photons and ignores chunk and refill_threshold."""
 
     persistent: bool = True
-    block_size: int = 256
+    block_size: int = 512
     chunk: int = 256
-    refill_threshold: int = 24
-    blocks_per_cu: int = 8
+    refill_threshold: int = 4
+    blocks_per_cu: int = 2
     per_event: bool = False
     graph: bool = False
     pregenerate: bool = False

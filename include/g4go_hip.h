@@ -66,9 +66,12 @@ typedef struct g4go_config {
  *     (the per-event launches a caller without batching would make).
  *   persistent=1: blocks_per_cu x CU-count blocks pop work items of `chunk`
  *     photons from a device queue and refill idle lanes once at least
- *     refill_threshold of the 64 lanes are idle; with graph=1 the launch is
- *     captured once in a hipGraph and replayed for every batch whose buffers
- *     still fit. */
+ *     refill_threshold of the 64 lanes are idle: from the wave's ring of
+ *     ready photons in LDS while it holds any, else by a generation pass in
+ *     which the whole wave generates the idle lanes' photons and the ring's
+ *     next ones (scenes with triangle solids: the idle lanes generate their
+ *     own); with graph=1 the launch is captured once in a hipGraph and
+ *     replayed for every batch whose buffers still fit. */
 typedef struct g4go_launch {
     uint32_t persistent;
     uint32_t block_size;       /* 64, 128, 256, 512 */

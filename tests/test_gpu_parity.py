@@ -117,6 +117,29 @@ def test_synthetic_crystal_bit_exact(crystal, form):
     assert_same_stats(out, ref)
 
 
+@pytest.mark.parametrize("block,refill", [(64, 1), (256, 2), (512, 1), (512, 24)])
+def test_ring_generation_passes_bit_exact(crystal, block, refill):
+    """The persistent form's ready-photon ring: 8-photon work items of 300
+    small events, so that one generation pass draws from many items (photons
+    of several events and sources, per lane), refilled from one idle lane on;
+    block sizes of 1 to 8 waves give rings of different sizes."""
+    log(f"case: ring passes, block {block}, refill {refill}")
+    src = workloads.synthetic_source(crystal)
+    rng = np.random.default_rng(5)
+    event_ids = rng.permutation(1000)[:300].astype(np.uint32)
+    counts = rng.integers(0, 60, 300).astype(np.uint32)
+    ref = O.run_synthetic(crystal.scene, src, 19, event_ids, counts, threads=4)
+    with host(19) as h:
+        h.set_launch(LaunchForm(persistent=True, block_size=block, chunk=8, refill_threshold=refill,
+                                blocks_per_cu=max(1, 1024 // block)))
+        h.enqueue_synthetic(crystal.scene, src, event_ids, counts)
+        out = h.complete_oldest_batch()
+    assert out.statistics["dropped_hits"] == 0
+    assert out.statistics["generated"] == int(counts.sum())
+    assert_same_hits(out.detections, ref.hits)
+    assert_same_stats(out, ref)
+
+
 @pytest.mark.parametrize("persistent", [False, True])
 @pytest.mark.parametrize("block", [64, 128, 512])
 def test_block_sizes_bit_exact(persistent, block):
