@@ -587,6 +587,85 @@ private:
     bool stop_ = false;
 };
 
+// Sized output vectors made ahead of need on a thread of their own: the
+// zero-filling construction of n detection records (the value-initialisation
+// of std::vector::resize, 0.17-0.21 ms for 1.5e5 records on the test host)
+// runs while the GPU transports the next batch instead of after its hits have
+// arrived.  request(n) starts one; take() returns it (waiting if it is still
+// being made) and leaves none prepared.
+template <class T>
+class SpareVector {
+public:
+    SpareVector() : thread_([this] { work(); }) {}
+
+    ~SpareVector() {
+        {
+            std::lock_guard<std::mutex> lock(mutex_);
+            stop_ = true;
+        }
+        wake_.notify_all();
+        thread_.join();
+    }
+
+    SpareVector(const SpareVector&) = delete;
+    SpareVector& operator=(const SpareVector&) = delete;
+
+    void request(std::size_t n) {
+        {
+            std::lock_guard<std::mutex> lock(mutex_);
+            if (requested_ || ready_) {
+                return;
+            }
+            requested_ = true;
+            size_ = n;
+        }
+        wake_.notify_all();
+    }
+
+    // The prepared vector (empty when none was requested).
+    std::vector<T> take() {
+        std::unique_lock<std::mutex> lock(mutex_);
+        done_.wait(lock, [this] { return !requested_; });
+        ready_ = false;
+        return std::move(vector_);
+    }
+
+private:
+    void work() {
+        for (;;) {
+            std::size_t n = 0;
+            {
+                std::unique_lock<std::mutex> lock(mutex_);
+                wake_.wait(lock, [this] { return stop_ || requested_; });
+                if (stop_) {
+                    return;
+                }
+                n = size_;
+            }
+            std::vector<T> v;
+            try {
+                v.resize(n);
+            } catch (...) {
+                v = std::vector<T>();  // out of memory: the caller sizes its own
+            }
+            {
+                std::lock_guard<std::mutex> lock(mutex_);
+                vector_ = std::move(v);
+                requested_ = false;
+                ready_ = true;
+            }
+            done_.notify_all();
+        }
+    }
+
+    std::mutex mutex_;
+    std::condition_variable wake_, done_;
+    bool stop_ = false, requested_ = false, ready_ = false;
+    std::size_t size_ = 0;
+    std::vector<T> vector_;
+    std::thread thread_;
+};
+
 // ---------------------------------------------------------------------------
 template <class Api = api::Types>
 class BasicHipTransportHost final {
@@ -697,6 +776,9 @@ public:
                                      bit(Field::Truncated) | bit(Field::MaxBounce) | bit(Field::InvalidState) |
                                      bit(Field::ZeroStep);
         const auto conversion0 = std::chrono::steady_clock::now();
+        // the vector prepared while this batch ran, trimmed (no work for
+        // trivially destructible records) or grown to the hit count
+        out.detections = spare_.take();
         out.detections.resize(r.n_hits);
         Detection* detections = out.detections.data();
         const g4go_hit* hits = r.hits;
@@ -762,6 +844,8 @@ public:
         perf.deviceToHostMs = perf.deviceMetadataToHostMs + perf.deviceHitsToHostMs;
         perf.hostHitCompactionMs = conversion_ms;
         perf.totalBounceCount = r.stats.total_steps;
+        // the next batch's records, sized by this one's hit count with a margin
+        spare_.request(r.n_hits + r.n_hits / 8 + 1024);
         return out;
     }
 
@@ -789,6 +873,7 @@ private:
     float tolerance_mm_;
     std::deque<Pending> pending_;
     RangePool pool_;
+    SpareVector<Detection> spare_;
 };
 
 using HipTransportHost = BasicHipTransportHost<>;

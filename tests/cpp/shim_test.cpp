@@ -7,7 +7,7 @@
 //                            Cerenkov and direct emission records through
 //                            the three-method interface and writes the
 //                            records and the output to OUT.bin (layout below)
-//   shim_test bench SLOTS BATCHES [THREADS [CHUNK BLOCKS_PER_CU REFILL PREGENERATE]]
+//   shim_test bench SLOTS BATCHES [THREADS [BLOCK CHUNK BLOCKS_PER_CU REFILL GRAPH PREGENERATE]]
 //                            throughput of the drop-in path (JSON line)
 //   shim_test gpu-mesh OUT.bin  the same with the crystal a tapered mesh
 //                            (3x3 mm front face, 4x4 mm back face, turned
@@ -557,23 +557,25 @@ int main(int argc, char** argv) {
         if (argc == 3 && std::string(argv[1]) == "gpu-mesh") {
             return run_gpu(argv[2], true);
         }
-        if ((argc == 4 || argc == 5 || argc == 9) && std::string(argv[1]) == "bench") {
+        if ((argc == 4 || argc == 5 || argc == 11) && std::string(argv[1]) == "bench") {
             // optional: the conversion thread count, then a persistent form
-            // CHUNK BLOCKS_PER_CU REFILL PREGENERATE (else the handle's default)
-            g4go_launch form{1, 256, 256, 24, 8, 0, 0, 0};
+            // BLOCK CHUNK BLOCKS_PER_CU REFILL GRAPH PREGENERATE (else the handle's default)
+            g4go_launch form{1, 512, 256, 4, 2, 0, 0, 0};
             const int threads = argc >= 5 ? std::atoi(argv[4]) : 4;
-            if (argc == 9) {
-                form.chunk = static_cast<std::uint32_t>(std::atoi(argv[5]));
-                form.blocks_per_cu = static_cast<std::uint32_t>(std::atoi(argv[6]));
-                form.refill_threshold = static_cast<std::uint32_t>(std::atoi(argv[7]));
-                form.pregenerate = static_cast<std::uint32_t>(std::atoi(argv[8]));
+            if (argc == 11) {
+                form.block_size = static_cast<std::uint32_t>(std::atoi(argv[5]));
+                form.chunk = static_cast<std::uint32_t>(std::atoi(argv[6]));
+                form.blocks_per_cu = static_cast<std::uint32_t>(std::atoi(argv[7]));
+                form.refill_threshold = static_cast<std::uint32_t>(std::atoi(argv[8]));
+                form.graph = static_cast<std::uint32_t>(std::atoi(argv[9]));
+                form.pregenerate = static_cast<std::uint32_t>(std::atoi(argv[10]));
             }
-            return run_bench(std::atoi(argv[2]), std::atoi(argv[3]), threads, argc == 9 ? &form : nullptr);
+            return run_bench(std::atoi(argv[2]), std::atoi(argv[3]), threads, argc == 11 ? &form : nullptr);
         }
     } catch (const std::exception& e) {
         std::printf("FAIL: uncaught %s: %s\n", typeid(e).name(), e.what());
         return 1;
     }
-    std::printf("usage: shim_test cpu | shim_test gpu OUT.bin | shim_test gpu-mesh OUT.bin | shim_test bench SLOTS BATCHES [THREADS [CHUNK BLOCKS_PER_CU REFILL PREGENERATE]]\n");
+    std::printf("usage: shim_test cpu | shim_test gpu OUT.bin | shim_test gpu-mesh OUT.bin | shim_test bench SLOTS BATCHES [THREADS [BLOCK CHUNK BLOCKS_PER_CU REFILL GRAPH PREGENERATE]]\n");
     return 2;
 }
