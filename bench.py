@@ -341,11 +341,13 @@ def run_dropin_cpp(batches=40):
     return res
 
 
-def run_host_batches(seed, launch, n_batches=40, events_per_batch=33):
+def run_host_batches(seed, n_batches=40, events_per_batch=33):
     """The drop-in path at the reference scheduler's batch size (~1e6 photons,
     PhotonTransportConfig::targetPhotonsPerBatch): 1 MeV electron events in the
     crystal, enqueue + completion with the hits read back to the host in launch
-    order, with one batch in flight (the reference's single slot) and with two."""
+    order, with one batch in flight (the reference's single slot) and with two.
+    The launch form is a new handle's default (what a drop-in caller gets, as
+    in the C++ shim bench), replayed as a graph."""
     from g4go_amd import workloads
     from g4go_amd.transport import HipTransportHost, LaunchForm, PhotonTransportConfig
 
@@ -360,7 +362,7 @@ def run_host_batches(seed, launch, n_batches=40, events_per_batch=33):
     res = {"batches": n_batches, "photons_per_batch": photons / n_batches, "readback": "host, launch order"}
     for slots in (1, 2):
         host = HipTransportHost(PhotonTransportConfig(seed=seed, max_in_flight=slots))
-        host.set_launch(LaunchForm(**{**launch.__dict__, "graph": True}))
+        host.set_launch(LaunchForm(graph=True))
         host.prepare_scene(cs.scene)
         for rep in range(2):  # the first pass warms the graphs and buffers
             t0 = time.perf_counter()
@@ -518,7 +520,7 @@ def main():
     c1 = None if args.no_c1 else run_c1(args.seed, args.steps, args.warmup, not args.no_cpu, 200,
                                                retune=args.retune)
     progress("electron events done")
-    host_batches = None if args.no_c1 else run_host_batches(args.seed, persistent)
+    host_batches = None if args.no_c1 else run_host_batches(args.seed)
     dropin_cpp = None if args.no_c1 else run_dropin_cpp()
     progress("drop-in batches done")
     mesh = None if args.no_mesh else run_mesh(args.seed, args.steps, args.warmup, args.photons,

@@ -895,6 +895,27 @@ HostScene build_scene(const g4go_scene& in, float tolerance) {
             }
         }
     }
+    // faces that the lean box step serves (Volume::box_step_faces)
+    for (Volume& v : s.volumes) {
+        if (v.plane_count != 0) {
+            continue;  // a mesh solid: the word holds plane_begin
+        }
+        v.box_step_faces = 0;
+        if (v.daughter_count != 0) {
+            continue;
+        }
+        for (int face = 0; face < 6; ++face) {
+            const Face& f = v.faces[face];
+            if (f.neighbor == kInvalid || f.neighbor == kFaceDynamic || f.surface >= s.surfaces.size() ||
+                s.volumes[f.neighbor].material >= in.n_materials) {
+                continue;
+            }
+            const Surface& sf = s.surfaces[f.surface];
+            if (sf.type == kDielectricMetal && sf.finish == kPolished) {
+                v.box_step_faces |= 1u << face;
+            }
+        }
+    }
     // daughter BVHs for volumes with many daughters
     for (uint32_t i = 0; i < in.n_volumes; ++i) {
         Volume& v = s.volumes[i];

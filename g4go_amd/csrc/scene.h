@@ -73,7 +73,15 @@ struct alignas(16) Volume {
     // set): triangles [plane_begin, plane_begin + count) of the triangle
     // list, BVH root in faces[0].neighbor (kInvalid: no BVH); in both, lo/hi
     // is the solid's bounding box.  plane_count 0: the box lo/hi.
-    uint32_t plane_begin, plane_count;
+    union {
+        uint32_t plane_begin;  // mesh solid (plane_count != 0): its first plane or triangle
+        // box solid (plane_count == 0), bit f set: a step leaving through face
+        // f can take the lean box step of transport_step (no daughters, a
+        // static neighbour, a polished dielectric_metal surface; resolved on
+        // the host)
+        uint32_t box_step_faces;
+    };
+    uint32_t plane_count;
     // face = 2*axis + (outward sign > 0); the neighbour and its surface sit
     // side by side so a step reads both with one 8-byte load
     Face faces[6];

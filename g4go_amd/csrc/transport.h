@@ -1166,6 +1166,78 @@ struct StepConfig {
     float eps;  // boundary tolerance (mm)
 };
 
+// The lean box step: a photon in a box volume with its material cached and
+// its node on the hot grid located, leaving through a face that
+// Volume::box_step_faces marks (no daughters, a static neighbour, a polished
+// dielectric_metal surface).  It is transport_step's arithmetic for that case
+// in the same order, so it gives the same bits, without the tests and
+// branches of every other case, which lengthen the dependent chain of a step
+// and the register pressure of the loop.  Returns kStepFallback, having
+// changed nothing, when the step is not such a step.
+constexpr int kStepFallback = -2;
+
+G4GO_HD int box_step(Photon& ph, const SceneView& sc, const Rng& rng, const StepConfig& cfg, bool& zero_step,
+                     uint32_t& sensor_out) {
+    if (ph.volume >= sc.n_volumes || (ph.grid_i == kInvalid && sc.hot_count != 0)) {
+        return kStepFallback;
+    }
+    const Volume& vol = sc.volumes[ph.volume];
+    float dist;
+    int axis = 0;
+    box_exit(vol, ph.pos, ph.dir, dist, axis);
+    const bool up = axis_of(ph.dir, axis) > 0.0f;
+    const int face = 2 * axis + (up ? 1 : 0);
+    if (vol.material != ph.material || ((vol.box_step_faces >> face) & 1u) == 0u || !finitef(dist)) {
+        return kStepFallback;
+    }
+    zero_step = dist <= cfg.eps;
+
+    const U4 w = rng.block(ph.photon_id, ph.event_id, ph.step, kProcStep, 0);
+    // bulk interaction (an infinite interaction length gives an infinite or
+    // NaN distance, which never wins)
+    const float d_int = -g4_logf(u01(w.x)) * ph.interaction_length;
+    if (d_int < dist) {
+        ph.time += static_cast<double>(d_int * ph.inv_velocity);
+        if (u01(w.y) <= ph.absorb_fraction) {
+            return kAbsorbedBulk;
+        }
+        ph.pos = add(ph.pos, scale(ph.dir, d_int));
+        rayleigh_scatter(ph, rng);
+        return kAlive;
+    }
+    ph.time += static_cast<double>(dist * ph.inv_velocity);
+    // onto the face plane
+    const V3 bpos = add(ph.pos, scale(ph.dir, dist));
+    const float fplane = up ? (axis == 0 ? vol.hi[0] : (axis == 1 ? vol.hi[1] : vol.hi[2]))
+                            : (axis == 0 ? vol.lo[0] : (axis == 1 ? vol.lo[1] : vol.lo[2]));
+    ph.pos = V3{axis == 0 ? fplane : bpos.x, axis == 1 ? fplane : bpos.y, axis == 2 ? fplane : bpos.z};
+
+    Face fr;
+    __builtin_memcpy(&fr, &vol.faces[face], sizeof(Face));
+    const Volume& nvol = sc.volumes[fr.neighbor];
+    const Surface& srf = sc.surfaces[fr.surface];
+    const float refl = prop_eval_ph(sc.pool, srf.reflectivity, ph, 0.0f);
+    const float trans = prop_eval_ph(sc.pool, srf.transmittance, ph, 0.0f);
+    const int outcome =
+        classify_surface(refl, trans, srf.reflectivity.count != 0, srf.transmittance.count != 0, u01(w.z));
+    if (outcome == kAbsorb) {
+        const float eff = clamp01(prop_eval_ph(sc.pool, srf.efficiency, ph, 0.0f));
+        if (evaluate_absorption(eff, nvol.sensor != kInvalid || vol.sensor != kInvalid, u01(w.w)) == kDetect) {
+            sensor_out = nvol.sensor != kInvalid ? nvol.sensor : vol.sensor;
+            return kDetected;
+        }
+        return kAbsorbedSurface;
+    }
+    if (outcome == kDirectTransmit) {
+        ph.volume = fr.neighbor;
+        return kAlive;
+    }
+    // specular reflection at the axis-aligned face
+    ph.dir = negate_axis(ph.dir, axis);
+    ph.pol = negate_off_axis(ph.pol, axis);
+    return kAlive;
+}
+
 // kGeom: the geometry level (kGeomConvex: convex mesh solids, whose faces
 // are found by clipping; kGeomTriangles: also triangle solids, whose faces
 // are found by the watertight triangle test).  On mesh faces the photon is not snapped onto a mesh face,
@@ -1175,6 +1247,12 @@ struct StepConfig {
 template <bool kTrace, int kGeom = kGeomBoxes>
 G4GO_HD int transport_step(Photon& ph, const SceneView& sc, const Rng& rng, const StepConfig& cfg, bool& zero_step,
                            uint32_t& sensor_out, StepTrace* trace) {
+    if constexpr (!kTrace && kGeom == kGeomBoxes) {
+        const int lean = box_step(ph, sc, rng, cfg, zero_step, sensor_out);
+        if (lean != kStepFallback) {
+            return lean;
+        }
+    }
     zero_step = false;
     if (ph.volume >= sc.n_volumes) {
         return kInvalidState;
