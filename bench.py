@@ -321,11 +321,11 @@ def run_c1(seed, steps, warmup, cpu, cpu_events, retune=False):
     return res
 
 
-def run_dropin_cpp(batches=40):
+def run_dropin_cpp(batches=200):
     """The drop-in path through the C++ shim (tests/cpp/shim_test bench): batches
     of 33 electron events (~1e6 photons) through PrepareScene / EnqueueEmissions /
     CompleteOldestBatch, hits returned as PhotonDetection records in launch order
-    (converted from the library's hit records by four host threads), with one
+    (converted from the library's hit records by eight host threads), with one
     batch in flight (the reference's single slot) and with two."""
     import subprocess
 
@@ -334,7 +334,7 @@ def run_dropin_cpp(batches=40):
         return None
     res = {}
     for slots in (1, 2):
-        proc = subprocess.run([exe, "bench", str(slots), str(batches), "4"], capture_output=True, text=True, timeout=120)
+        proc = subprocess.run([exe, "bench", str(slots), str(batches), "8"], capture_output=True, text=True, timeout=120)
         if proc.returncode != 0:
             return {"error": proc.stdout[-500:] + proc.stderr[-500:]}
         res[f"slots_{slots}"] = json.loads(proc.stdout.strip().splitlines()[-1])

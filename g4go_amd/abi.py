@@ -304,6 +304,11 @@ def load() -> C.CDLL:
         return _lib
     if not LIBRARY_PATH.is_file():
         raise RuntimeError(f"{LIBRARY_PATH} is missing: build it with `python -m g4go_amd.build`")
+    # torch ships its own HIP runtime under the same soname (libamdhip64.so.7)
+    # as the system one this library links.  Loaded first, torch's copy serves
+    # both; loaded after a process has initialised the system copy, torch
+    # finds no device.  So torch is imported before the library.
+    import torch  # noqa: F401
     lib = C.CDLL(str(LIBRARY_PATH))
     H = C.c_void_p
     lib.g4go_create.argtypes = [C.POINTER(Config), C.POINTER(H)]

@@ -1092,6 +1092,9 @@ struct Core {
     uint64_t device_hits = 0;  // detections of the last completed batch left in d_hits
     uint32_t chunk = 64;
     std::vector<uint32_t> event_ids;
+    // validated records of an emission batch (reused between batches)
+    std::vector<Emission> staged;
+    std::vector<uint32_t> staged_counts, staged_events;
     std::vector<uint64_t> captured;
     std::vector<uint32_t> captured_u32() const { return std::vector<uint32_t>(captured.begin(), captured.end()); }
     std::chrono::steady_clock::time_point started;
@@ -1311,13 +1314,15 @@ struct Core {
         l.cap_em += l.cap_em / 2;
         l.cap_ev += l.cap_ev / 2;
         l.cap_src += l.cap_src / 2;
-        l.o_em = 256;
-        l.o_ev = l.o_em + align256(l.cap_em * sizeof(Emission));
+        // the small arrays first and the records last, so that one copy of
+        // [0, o_em + record bytes) moves a batch's inputs
+        l.o_ev = 256;
         l.o_off = l.o_ev + align256(l.cap_ev * sizeof(uint32_t));
         l.o_cnt = l.o_off + align256((l.cap_src + 1) * sizeof(uint32_t));
         l.o_sev = l.o_cnt + align256(l.cap_src * sizeof(uint32_t));
         l.o_poff = l.o_sev + align256(l.cap_src * sizeof(uint32_t));
-        l.bytes = l.o_poff + align256((l.cap_src + 1) * sizeof(uint32_t));
+        l.o_em = l.o_poff + align256((l.cap_src + 1) * sizeof(uint32_t));
+        l.bytes = l.o_em + align256(l.cap_em * sizeof(Emission));
         d_inputs.release();
         d_inputs.ensure(l.bytes, "hipMalloc batch inputs");
         h_stage.ensure(l.bytes, "hipHostMalloc staging");
@@ -1386,16 +1391,9 @@ struct Core {
         std::memcpy(st + l.o_poff, photon_offsets.data(), poff_bytes);
         char* in = d_inputs.as<char>();
         hip_check(hipEventRecord(ev_start, stream), "hipEventRecord");
-        const std::pair<size_t, size_t> regions[] = {
-            {0, 16}, {l.o_em, em_bytes}, {l.o_ev, ev_bytes}, {l.o_off, off_bytes}, {l.o_cnt, cnt_bytes},
-            {l.o_sev, sev_bytes}, {l.o_poff, poff_bytes}};
-        for (const auto& r : regions) {
-            if (r.second != 0) {
-                hip_check(hipMemcpyAsync(in + r.first, st + r.first, r.second, hipMemcpyHostToDevice, stream),
-                          "hipMemcpyAsync batch inputs");
-            }
-        }
-        h2d_bytes = 16 + em_bytes + ev_bytes + off_bytes + cnt_bytes + sev_bytes + poff_bytes;
+        // one copy (the unused tails of the small arrays ride along)
+        h2d_bytes = l.o_em + em_bytes;
+        hip_check(hipMemcpyAsync(in, st, h2d_bytes, hipMemcpyHostToDevice, stream), "hipMemcpyAsync batch inputs");
         hip_check(hipEventRecord(ev_h2d, stream), "hipEventRecord");
 
         KernelParams p = base_params();
@@ -1689,20 +1687,31 @@ static g4go_status core_enqueue_emissions(Core* h, const g4go_emission* emission
         if (photons > 0xffffffffull) {
             fail(G4GO_ERROR_OVERFLOW, "emission batch exceeds the supported photon count");
         }
-        std::vector<Emission> staged(photons != 0 ? n_emissions : 0);
-        std::vector<uint32_t> counts(staged.size()), source_event(staged.size());
+        // the handle's staging vectors, kept between batches
+        std::vector<Emission>& staged = h->staged;
+        std::vector<uint32_t>& counts = h->staged_counts;
+        std::vector<uint32_t>& source_event = h->staged_events;
+        staged.resize(photons != 0 ? n_emissions : 0);
+        counts.resize(staged.size());
+        source_event.resize(staged.size());
         // a batch without photons is not validated further (OptiXTransportHost.cpp:609-627); otherwise
         // every record's volume and material must exist, whatever its type (:650-660)
+        uint32_t last_id = kInvalid, last_index = kInvalid;  // consecutive records mostly share a volume
         for (uint64_t i = 0; i < staged.size(); ++i) {
             std::memcpy(&staged[i], &emissions[i], sizeof(Emission));
-            const auto it = h->scene.index_of.find(emissions[i].volume_id);
-            if (it == h->scene.index_of.end()) {
-                fail(G4GO_ERROR_INVALID_ARGUMENT, "emission references an unknown scene volume ID");
+            const uint32_t id = emissions[i].volume_id;
+            if (id != last_id || last_index == kInvalid) {
+                const auto it = h->scene.index_of.find(id);
+                if (it == h->scene.index_of.end()) {
+                    fail(G4GO_ERROR_INVALID_ARGUMENT, "emission references an unknown scene volume ID");
+                }
+                last_id = id;
+                last_index = it->second;
             }
             if (emissions[i].material_id >= h->scene.materials.size()) {
                 fail(G4GO_ERROR_INVALID_ARGUMENT, "emission references an unknown scene material ID");
             }
-            staged[i].volume = it->second;
+            staged[i].volume = last_index;
             counts[i] = emissions[i].photon_count;
             source_event[i] = emission_event_indices[i];
         }

@@ -32,6 +32,7 @@
 
 #include <algorithm>
 #include <array>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
@@ -508,7 +509,7 @@ public:
     ~RangePool() {
         {
             std::lock_guard<std::mutex> lock(mutex_);
-            stop_ = true;
+            stop_.store(true);
         }
         wake_.notify_all();
         for (std::thread& t : threads_) {
@@ -519,6 +520,19 @@ public:
     RangePool(const RangePool&) = delete;
     RangePool& operator=(const RangePool&) = delete;
 
+    // From arm() to disarm() the helpers wait for a job by spinning (with
+    // yields) instead of sleeping, so run() starts them without a futex
+    // wake-up, which costs tens of microseconds; CompleteOldestBatch arms the
+    // pool while it waits for the GPU.
+    void arm() {
+        {
+            std::lock_guard<std::mutex> lock(mutex_);
+            armed_.store(true);
+        }
+        wake_.notify_all();
+    }
+    void disarm() { armed_.store(false); }
+
     // Calls body(lo, hi) over disjoint slices covering [0, n), one slice per
     // thread, and returns when all are done.  body must not throw.
     template <class F>
@@ -527,17 +541,18 @@ public:
             body(std::uint64_t{0}, n);
             return;
         }
+        job_ = Job{n, &body, [](void* f, std::uint64_t lo, std::uint64_t hi) { (*static_cast<F*>(f))(lo, hi); }};
+        outstanding_.store(static_cast<unsigned>(threads_.size()), std::memory_order_relaxed);
+        generation_.fetch_add(1, std::memory_order_release);
         {
-            std::lock_guard<std::mutex> lock(mutex_);
-            job_ = Job{n, &body, [](void* f, std::uint64_t lo, std::uint64_t hi) { (*static_cast<F*>(f))(lo, hi); }};
-            outstanding_ = static_cast<unsigned>(threads_.size());
-            ++generation_;
+            std::lock_guard<std::mutex> lock(mutex_);  // a sleeping helper re-checks under the lock
         }
         wake_.notify_all();
         const auto [lo, hi] = slice(n, 0);
         body(lo, hi);
-        std::unique_lock<std::mutex> lock(mutex_);
-        done_.wait(lock, [this] { return outstanding_ == 0; });
+        while (outstanding_.load(std::memory_order_acquire) != 0) {
+            std::this_thread::yield();
+        }
     }
 
 private:
@@ -557,34 +572,39 @@ private:
     void work(unsigned part) {
         std::uint64_t seen = 0;
         for (;;) {
-            Job job;
-            {
-                std::unique_lock<std::mutex> lock(mutex_);
-                wake_.wait(lock, [&] { return stop_ || generation_ != seen; });
-                if (stop_) {
+            // the next job: spin while armed, else sleep until one is posted
+            for (;;) {
+                if (stop_.load()) {
                     return;
                 }
-                seen = generation_;
-                job = job_;
+                const std::uint64_t g = generation_.load(std::memory_order_acquire);
+                if (g != seen) {
+                    seen = g;
+                    break;
+                }
+                if (armed_.load(std::memory_order_relaxed)) {
+                    std::this_thread::yield();
+                    continue;
+                }
+                std::unique_lock<std::mutex> lock(mutex_);
+                wake_.wait(lock, [&] {
+                    return stop_.load() || armed_.load() || generation_.load(std::memory_order_acquire) != seen;
+                });
             }
+            const Job job = job_;
             const auto [lo, hi] = slice(job.n, part);
             job.call(job.body, lo, hi);
-            {
-                std::lock_guard<std::mutex> lock(mutex_);
-                if (--outstanding_ == 0) {
-                    done_.notify_one();
-                }
-            }
+            outstanding_.fetch_sub(1, std::memory_order_release);
         }
     }
 
     std::vector<std::thread> threads_;
     std::mutex mutex_;
-    std::condition_variable wake_, done_;
+    std::condition_variable wake_;
     Job job_{};
-    std::uint64_t generation_ = 0;
-    unsigned outstanding_ = 0;
-    bool stop_ = false;
+    std::atomic<std::uint64_t> generation_{0};
+    std::atomic<unsigned> outstanding_{0};
+    std::atomic<bool> stop_{false}, armed_{false};
 };
 
 // Sized output vectors made ahead of need on a thread of their own: the
@@ -694,7 +714,7 @@ public:
     // included, that turn a completed batch's hits into detection records
     // (1: the calling thread alone).
     explicit BasicHipTransportHost(Config configuration, const g4go_launch* launch = nullptr,
-                                   std::uint32_t in_flight = 1, std::uint32_t conversion_threads = 4)
+                                   std::uint32_t in_flight = 1, std::uint32_t conversion_threads = 8)
         : tolerance_mm_(configuration.boundaryToleranceMm),
           pool_(conversion_threads > 1 ? conversion_threads - 1 : 0) {
         g4go_config c{};
@@ -761,6 +781,8 @@ public:
     // CompleteOldestBatch (OptiXTransportHost.cpp:779-883).
     Output CompleteOldestBatch() {
         g4go_batch_result r{};
+        // the conversion helpers spin while the GPU finishes the batch
+        pool_.arm();
         const g4go_status status = g4go_complete(handle_, &r);
         // g4go_complete retires the oldest batch even when it fails (only an
         // empty queue, where nothing is pending here either, leaves it), so
@@ -769,6 +791,9 @@ public:
         if (!pending_.empty()) {
             batch = std::move(pending_.front());
             pending_.pop_front();
+        }
+        if (status != G4GO_OK) {
+            pool_.disarm();
         }
         check(status, handle_);
         Output out{};
@@ -796,6 +821,7 @@ public:
             }
         };
         pool_.run(r.n_hits, convert);
+        pool_.disarm();
         const double conversion_ms =
             std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - conversion0).count();
         auto& st = out.statistics;
