@@ -111,6 +111,7 @@ struct HostScene {
     std::vector<std::vector<double>> face_points;  // per plane: its triangles' vertices (xyz), host only
     std::vector<Tri> tris;                         // triangles of the non-convex mesh solids
     std::vector<std::vector<double>> tri_points;   // per triangle: its vertices (xyz), host only
+    std::vector<uint32_t> face_sibs;               // sibling lists of kFaceDynamic faces (Plane::sib_first)
     uint32_t world = kInvalid;
     uint32_t hot_offset = 0, hot_count = 0;  // hot energy grid (mark_hot_grid)
     std::unordered_map<uint32_t, uint32_t> index_of;  // public volume ID -> dense index
@@ -959,7 +960,7 @@ HostScene build_scene(const g4go_scene& in, float tolerance) {
                 }
             }
             const double margin = 1e-3 + 1e-5 * scale + 2.0 * tolerance;
-            uint32_t found = kInvalid, count = 0;
+            std::vector<uint32_t> meets;  // daughter-list slots, ascending
             const Volume& par = s.volumes[v.parent];
             for (uint32_t slot = par.daughter_begin; slot < par.daughter_begin + par.daughter_count; ++slot) {
                 const uint32_t j = s.daughters[slot];
@@ -972,18 +973,23 @@ HostScene build_scene(const g4go_scene& in, float tolerance) {
                     meet = meet && o.lo[a] <= hi[a] + margin && o.hi[a] >= lo[a] - margin;
                 }
                 if (meet) {
-                    found = slot;
-                    ++count;
+                    meets.push_back(slot);
                 }
             }
             pl.surface_parent = host_find_surface(s, i, v.parent);
-            if (count == 0) {
+            pl.sib_first = pl.sib_count = 0;
+            if (meets.empty()) {
                 pl.across = kInvalid;
-            } else if (count == 1) {
-                pl.across = found;
-                pl.surface_across = host_find_surface(s, i, s.daughters[found]);
+            } else if (meets.size() == 1) {
+                pl.across = meets[0];
+                pl.surface_across = host_find_surface(s, i, s.daughters[meets[0]]);
             } else {
                 pl.across = kFaceDynamic;
+                if (meets.size() <= kMaxFaceSiblings) {
+                    pl.sib_first = static_cast<uint32_t>(s.face_sibs.size());
+                    pl.sib_count = static_cast<uint32_t>(meets.size());
+                    s.face_sibs.insert(s.face_sibs.end(), meets.begin(), meets.end());
+                }
             }
         }
     }
@@ -1068,6 +1074,7 @@ struct Core {
     bool hip_ready = false;
     int device = 0;
     int cu_count = 256;
+    int lds_per_cu = 0;  // bytes of LDS per CU (0 until the device is set up: the smallest scene budget)
     hipStream_t stream = nullptr;
     hipEvent_t ev_start = nullptr, ev_kernel0 = nullptr, ev_kernel1 = nullptr, ev_meta = nullptr;
     // per-stage timers: H2D copies end, launch-order pass end, hit readback
@@ -1144,6 +1151,8 @@ struct Core {
         hip_check(hipGetDevice(&device), "hipGetDevice");
         hip_check(hipDeviceGetAttribute(&cu_count, hipDeviceAttributeMultiprocessorCount, device),
                   "hipDeviceGetAttribute");
+        hip_check(hipDeviceGetAttribute(&lds_per_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, device),
+                  "hipDeviceGetAttribute");
         hip_check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
         hip_check(hipEventCreate(&ev_start), "hipEventCreate");
         hip_check(hipEventCreate(&ev_kernel0), "hipEventCreate");
@@ -1177,20 +1186,31 @@ struct Core {
         off = align256(off + std::max<size_t>(1, s.bvh.size()) * sizeof(BvhNode));
         const size_t o_dbox = off;
         off = align256(off + std::max<size_t>(1, s.daughters.size()) * sizeof(DaughterBox));
-        const size_t o_planes = off;  // planes, then triangles
-        off = align256(off + std::max<size_t>(1, s.planes.size() * sizeof(Plane) + s.tris.size() * sizeof(Tri)));
+        const size_t o_planes = off;  // planes, then triangles, then the face sibling lists
+        const size_t sib_quads = (s.face_sibs.size() + 3) / 4;
+        off = align256(off + std::max<size_t>(1, s.planes.size() * sizeof(Plane) + s.tris.size() * sizeof(Tri) +
+                                                     sib_quads * 16));
         std::vector<char> blob(off, 0);
         // daughter boxes in daughter-list order (after the BVH build reordered the list)
         for (size_t k = 0; k < s.daughters.size(); ++k) {
             const Volume& d = s.volumes[s.daughters[k]];
-            DaughterBox b{{d.lo[0], d.lo[1], d.lo[2]}, s.daughters[k], {d.hi[0], d.hi[1], d.hi[2]},
-                          d.plane_count != 0 ? 1u : 0u};
+            uint32_t mesh = 0;
+            if (d.plane_count != 0) {
+                const bool encodable = (d.plane_count & kSolidTriangles) == 0 &&
+                                       d.plane_begin < (1u << (32 - kMeshBeginShift)) && d.plane_count < kMeshInVolume;
+                mesh = encodable ? d.plane_count | d.plane_begin << kMeshBeginShift : kMeshInVolume;
+            }
+            DaughterBox b{{d.lo[0], d.lo[1], d.lo[2]}, s.daughters[k], {d.hi[0], d.hi[1], d.hi[2]}, mesh};
             std::memcpy(blob.data() + o_dbox + k * sizeof(DaughterBox), &b, sizeof(b));
         }
         if (!s.planes.empty()) std::memcpy(blob.data() + o_planes, s.planes.data(), s.planes.size() * sizeof(Plane));
         if (!s.tris.empty()) {
             std::memcpy(blob.data() + o_planes + s.planes.size() * sizeof(Plane), s.tris.data(),
                         s.tris.size() * sizeof(Tri));
+        }
+        if (!s.face_sibs.empty()) {
+            std::memcpy(blob.data() + o_planes + s.planes.size() * sizeof(Plane) + s.tris.size() * sizeof(Tri),
+                        s.face_sibs.data(), s.face_sibs.size() * sizeof(uint32_t));
         }
         std::memcpy(blob.data() + o_pool, s.pool.data(), s.pool.size() * sizeof(float));
         if (!s.materials.empty()) std::memcpy(blob.data() + o_mat, s.materials.data(), s.materials.size() * sizeof(Material));
@@ -1213,6 +1233,10 @@ struct Core {
         view.planes = reinterpret_cast<const Plane*>(base + o_planes);
         view.n_planes = static_cast<uint32_t>(s.planes.size());
         view.n_tris = static_cast<uint32_t>(s.tris.size());
+        view.n_sib_quads = static_cast<uint32_t>(sib_quads);
+        for (const Volume& v : s.volumes) {
+            view.max_daughters = std::max(view.max_daughters, v.daughter_count);
+        }
         view.n_daughters = static_cast<uint32_t>(s.daughters.size());
         view.hot_offset = s.hot_offset;
         view.hot_count = s.hot_count;
@@ -1266,15 +1290,16 @@ struct Core {
         p.cfg = StepConfig{cfg.boundary_tolerance_mm};
         p.max_steps = cfg.max_bounces_per_photon;
         p.refill_threshold = std::max<uint32_t>(1, std::min<uint32_t>(64, launch.refill_threshold));
-        // Every block stages scene parts in LDS, each when it still fits, in
-        // the order a step needs them: the volume and surface records and the
-        // mesh planes (read every step), the BVH nodes and daughter boxes
-        // (read by photons in a volume with a daughter BVH), the property pool
-        // (read mostly when a photon starts), the material records (read when
-        // it changes volume).
+        // Every block stages scene parts in LDS, each when it still fits the
+        // launch form's budget, in the order a step needs them: the volume
+        // and surface records and the mesh planes (read every step), the BVH
+        // nodes and daughter boxes (read by photons in a volume with a
+        // daughter BVH), the property pool (read mostly when a photon
+        // starts), the material records (read when it changes volume).
+        const uint64_t budget = lds_scene_budget(launch.block_size, static_cast<size_t>(lds_per_cu));
         uint64_t used = 0;
         const auto fits = [&](uint64_t bytes) {
-            if (bytes == 0 || used + bytes > kLdsSceneBytes) {
+            if (bytes == 0 || used + bytes > budget) {
                 return false;
             }
             used += bytes;
@@ -1288,8 +1313,13 @@ struct Core {
         if ((view.n_bvh_nodes != 0 || has_meshes(view)) && fits(entries * 32u)) {
             p.lds_entries = static_cast<uint32_t>(entries);
         }
-        if (fits((static_cast<uint64_t>(view.n_pool) + 3u) / 4u * 16u)) p.lds_pool_floats = view.n_pool;
-        if (fits(static_cast<uint64_t>(view.n_materials) * sizeof(Material))) p.lds_records |= kLdsMaterials;
+        // The pool and the material records only within the smallest budget:
+        // past it they would take the LDS of the ready-photon ring, which
+        // serves a scene better (the 8x8 array: 6% slower with its 17 KB pool
+        // staged in 512-thread blocks).
+        const auto fits_small = [&](uint64_t bytes) { return used + bytes <= kLdsSceneBytes && fits(bytes); };
+        if (fits_small((static_cast<uint64_t>(view.n_pool) + 3u) / 4u * 16u)) p.lds_pool_floats = view.n_pool;
+        if (fits_small(static_cast<uint64_t>(view.n_materials) * sizeof(Material))) p.lds_records |= kLdsMaterials;
         return p;
     }
 

@@ -7,10 +7,11 @@
 
 namespace g4go {
 
-// Dynamic LDS budget per block for the staged scene (BVH nodes, daughter
-// boxes, property pool, scene records).  With the histogram (1 KB) and the
-// waves' hit stages (3.25 KB per wave), four 256-thread blocks (16 waves, the
-// register-limited occupancy) still fit in 160 KB.
+// Smallest dynamic LDS budget per block for the staged scene (BVH nodes,
+// daughter boxes, property pool, scene records): with the histogram (1 KB) and
+// the waves' hit stages (3.25 KB per wave), four 256-thread blocks (16 waves,
+// the register-limited occupancy) still fit in 160 KB.  Larger blocks get
+// more (lds_scene_budget).
 constexpr uint32_t kLdsSceneBytes = 26112;
 constexpr int kHistBins = 256;
 constexpr int kHistSlots = kHistBins + 2;  // + underflow (256) + overflow (257)
@@ -148,6 +149,23 @@ constexpr uint32_t kRingQuads = 6;
 constexpr uint32_t kRingMaxSlots = 64;
 constexpr uint32_t kResidentWaves = 16;
 constexpr uint32_t kLdsStaticReserve = 4096;
+
+// Ready-photon ring entries per wave that a block's scene budget leaves free.
+constexpr uint32_t kRingMinSlots = 8;
+
+// Bytes of dynamic LDS the staged scene may take in a block of `block`
+// threads on a device with lds_per_cu bytes of LDS per CU: the block's share
+// at kResidentWaves resident waves, less the static LDS, the hit stages and
+// kRingMinSlots ring entries per wave; at least kLdsSceneBytes.  512-thread
+// blocks on a 160 KB CU get 44 KB, enough for a ring of 48 mesh crystals
+// (volumes, face planes, daughter boxes and BVH).
+inline size_t lds_scene_budget(uint32_t block, size_t lds_per_cu) {
+    const size_t waves = block < 64u ? 1u : block / 64u;
+    const size_t resident = waves >= kResidentWaves ? 1u : kResidentWaves / waves;
+    const size_t share = lds_per_cu / resident;
+    const size_t taken = kLdsStaticReserve + waves * (kStageBytesPerWave + kRingMinSlots * kRingQuads * 16u);
+    return share > taken + kLdsSceneBytes ? share - taken : kLdsSceneBytes;
+}
 
 // Dynamic LDS bytes of a transport launch (the staged scene).
 __host__ __device__ inline size_t transport_lds_bytes(const KernelParams& p) {

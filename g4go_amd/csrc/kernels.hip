@@ -49,6 +49,107 @@ __device__ __forceinline__ uint32_t wave_max(uint32_t v) {
     return v;
 }
 
+__device__ __forceinline__ float lane_float(float v, int lane) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
+// Minimum of v over the wave's 64 lanes (every lane active), by DPP moves:
+// butterflies within each row of 16 lanes, then the row minima carried up by
+// row broadcasts; lane 63 ends with the minimum.
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
+    const auto step = [](uint32_t x, uint32_t y) { return y < x ? y : x; };
+    int x = static_cast<int>(v);
+    x = static_cast<int>(step(x, __builtin_amdgcn_update_dpp(x, x, 0xB1, 0xF, 0xF, false)));   // quad_perm 1,0,3,2
+    x = static_cast<int>(step(x, __builtin_amdgcn_update_dpp(x, x, 0x4E, 0xF, 0xF, false)));   // quad_perm 2,3,0,1
+    x = static_cast<int>(step(x, __builtin_amdgcn_update_dpp(x, x, 0x141, 0xF, 0xF, false)));  // row_half_mirror
+    x = static_cast<int>(step(x, __builtin_amdgcn_update_dpp(x, x, 0x140, 0xF, 0xF, false)));  // row_mirror
+    x = static_cast<int>(step(x, __builtin_amdgcn_update_dpp(x, x, 0x142, 0xA, 0xF, false)));  // row_bcast:15
+    x = static_cast<int>(step(x, __builtin_amdgcn_update_dpp(x, x, 0x143, 0xC, 0xF, false)));  // row_bcast:31
+    return static_cast<uint32_t>(__builtin_amdgcn_readlane(x, 63));
+}
+
+// Lanes of a wave whose daughter search the whole wave makes, at most.
+constexpr int kWaveSearchLanes = 16;
+
+// Daughter searches of this step for the alive lanes in a volume of more than
+// eight and at most kScanDaughters daughters (scenes with convex mesh solids,
+// where such a volume's daughters are all tested: scans_daughters).  A photon in such a
+// volume (the air around a ring of crystals) makes a long serial search on a
+// few lanes while the rest of the wave waits.  When at most kWaveSearchLanes
+// lanes need one, the wave serves them in turn: for each, lane k tests
+// daughters k, k + 64, ... of its volume with daughter_entry and keeps its
+// smallest entry (ties to the lower volume index), and a wave minimum of
+// (distance bits, volume index) picks the answer, the same one as a single
+// lane testing every daughter.  Every lane of the wave must call this.
+// Returns the searched lanes' answers; `which` is kDaughterUnsearched for the
+// others (transport_step then searches itself).
+template <int kGeom>
+__device__ __forceinline__ DaughterHit wave_daughter_search(const SceneView& sc, const Photon& ph, bool alive,
+                                                            float eps) {
+    DaughterHit mine{kInf, kDaughterUnsearched, 0, kInvalid};
+    if (sc.max_daughters <= 8u) {
+        return mine;
+    }
+    bool want = false;
+    if (alive && ph.volume < sc.n_volumes) {
+        const uint32_t n = sc.volumes[ph.volume].daughter_count;
+        want = n > 8u && n <= kScanDaughters;
+    }
+    uint64_t pending = __ballot(want);
+    if (pending == 0ull || __popcll(pending) > kWaveSearchLanes) {
+        return mine;
+    }
+    const uint32_t lane = lane_id();
+    const V3 own_inv = {1.0f / ph.dir.x, 1.0f / ph.dir.y, 1.0f / ph.dir.z};
+    while (pending != 0ull) {
+        const int l = __builtin_ctzll(pending);
+        pending &= pending - 1ull;
+        const V3 p = {lane_float(ph.pos.x, l), lane_float(ph.pos.y, l), lane_float(ph.pos.z, l)};
+        const V3 d = {lane_float(ph.dir.x, l), lane_float(ph.dir.y, l), lane_float(ph.dir.z, l)};
+        const V3 inv = {lane_float(own_inv.x, l), lane_float(own_inv.y, l), lane_float(own_inv.z, l)};
+        const Volume& v = sc.volumes[static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(ph.volume), l))];
+        const uint32_t begin = uniform(v.daughter_begin), count = uniform(v.daughter_count);
+        float bt = kInf;
+        uint32_t bj = kInvalid, bf = kInvalid;
+        int ba = 0;
+        for (uint32_t k = lane; k < count; k += 64u) {
+            const DaughterBox b = load_dbox(sc.dbox[begin + k]);
+            int a = 0;
+            uint32_t f = kInvalid;
+            const float t = daughter_entry<kGeom, true>(sc, b, p, d, inv, eps, a, f);
+            const uint32_t j = b.volume;
+            if (t < bt || (t == bt && t < kInf && j < bj)) {
+                bt = t;
+                bj = j;
+                ba = a;
+                bf = f;
+            }
+        }
+        // a finite entry is >= eps > 0 (+ 0.0f turns a -0 into +0), so its
+        // bits order like the distance; equal distances go to the lower
+        // volume index
+        const uint32_t tkey = bt < kInf ? __float_as_uint(bt + 0.0f) : 0xffffffffu;
+        const uint32_t tmin = wave_min_u32(tkey);
+        DaughterHit h{kInf, kInvalid, 0, kInvalid};
+        if (tmin != 0xffffffffu) {
+            uint64_t at = __ballot(tkey == tmin);
+            if ((at & (at - 1ull)) != 0ull) {
+                const uint32_t jmin = wave_min_u32(tkey == tmin ? bj : 0xffffffffu);
+                at = __ballot(tkey == tmin && bj == jmin);
+            }
+            const int w = __builtin_ctzll(at);
+            h.t = lane_float(bt, w);
+            h.which = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(bj), w));
+            h.axis = __builtin_amdgcn_readlane(ba, w);
+            h.plane = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(bf), w));
+        }
+        if (lane == static_cast<uint32_t>(l)) {
+            mine = h;
+        }
+    }
+    return mine;
+}
+
 // The block's copy of the per-event counters (computed where a flush needs
 // it rather than held through the transport loop).
 __device__ __forceinline__ EventStats* event_copy(const KernelParams& p) {
@@ -367,9 +468,11 @@ __device__ __forceinline__ void stage_scene(const KernelParams& p, float4* lds, 
             static_cast<const Surface*>(stage_quads(p.scene.surfaces, p.scene.n_surfaces * (sizeof(Surface) / 16u)));
     }
     if (kGeom != kGeomBoxes && (kLdsMode >= kLdsHot || (p.lds_records & kLdsPlanes))) {  // box scenes have none
-        // planes then triangles, whole 16-byte words (triangles only at kGeomTriangles)
+        // planes, triangles and face sibling lists, whole 16-byte words (a
+        // scene with triangle solids runs kGeomTriangles; below it n_tris is 0)
         const uint32_t quads = p.scene.n_planes * (sizeof(Plane) / 16u) +
-                               (kGeom == kGeomTriangles ? p.scene.n_tris * (sizeof(Tri) / 16u) : 0u);
+                               (kGeom == kGeomTriangles ? p.scene.n_tris * (sizeof(Tri) / 16u) : 0u) +
+                               p.scene.n_sib_quads;
         sv.planes = static_cast<const Plane*>(stage_quads(p.scene.planes, quads));
     }
     if (kLdsMode >= kLdsAll || (p.lds_records & kLdsMaterials)) {
@@ -653,9 +756,13 @@ __device__ __forceinline__ void transport_body(const KernelParams& p) {
         int status = kAlive;
         uint32_t sensor = kInvalid;
         G4GO_REGION_BEGIN(kRegionStep);
+        DaughterHit searched{kInf, kDaughterUnsearched, 0, kInvalid};
+        if constexpr (kGeom == kGeomConvex) {
+            searched = wave_daughter_search<kGeom>(sv, ph, alive, p.cfg.eps);
+        }
         if (alive) {
             bool zero = false;
-            status = transport_step<false, kGeom>(ph, sv, p.rng, p.cfg, zero, sensor, nullptr);
+            status = transport_step<false, kGeom>(ph, sv, p.rng, p.cfg, zero, sensor, nullptr, searched);
             ++ph.step;
             st.zero_step += zero ? 1u : 0u;
             if (status == kAlive && ph.step >= p.max_steps) {
@@ -872,24 +979,27 @@ int launch_zero(void* ptr, size_t bytes, hipStream_t stream) {
 
 int launch_transport(const KernelParams& p, int source, bool persistent, int block, int grid, hipStream_t stream) {
     const size_t scene_lds = transport_lds_bytes(p);
-    if (scene_lds > kLdsSceneBytes || (p.starts != nullptr && geom_level(p.scene) == kGeomTriangles) || block < 64 ||
+    if ((p.starts != nullptr && geom_level(p.scene) == kGeomTriangles) || block < 64 ||
         block > static_cast<int>(kMaxBlock) || block % 64 != 0) {
+        return 1;
+    }
+    int device = 0, per_cu = 0, per_block = 0;
+    if (hipGetDevice(&device) != hipSuccess ||
+        hipDeviceGetAttribute(&per_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, device) != hipSuccess ||
+        hipDeviceGetAttribute(&per_block, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess) {
         return 1;
     }
     KernelParams q = p;
     q.lds_stage_quads = static_cast<uint32_t>((scene_lds + 15u) / 16u);
     const size_t waves = static_cast<size_t>(block / 64);
     size_t lds = static_cast<size_t>(q.lds_stage_quads) * 16u + waves * kStageBytesPerWave;
+    if (lds + kLdsStaticReserve > static_cast<size_t>(per_block)) {
+        return 1;  // the staged scene was planned for a larger block
+    }
     q.ring_slots = 0;
     if (persistent && geom_level(p.scene) != kGeomTriangles) {
         // the ring takes what the block's share of the CU's LDS leaves (the
         // triangle-solid instances refill in place and have no ring)
-        int device = 0, per_cu = 0, per_block = 0;
-        if (hipGetDevice(&device) != hipSuccess ||
-            hipDeviceGetAttribute(&per_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, device) != hipSuccess ||
-            hipDeviceGetAttribute(&per_block, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess) {
-            return 1;
-        }
         const size_t resident = std::max<size_t>(1, kResidentWaves / waves);
         const size_t share = std::min(static_cast<size_t>(per_cu) / resident, static_cast<size_t>(per_block));
         if (share > lds + kLdsStaticReserve) {

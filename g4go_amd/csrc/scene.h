@@ -102,15 +102,18 @@ static_assert(sizeof(Material) % 16 == 0 && sizeof(Surface) % 16 == 0 && sizeof(
 // margin).  None: the parent is across (surface_parent).  One: that sibling
 // (daughter-list slot `across`, surface_across) when it holds the point, else
 // the parent; the same answer as testing every sibling.  More: kFaceDynamic,
-// the full point query.
+// and the point query tests only the listed siblings, in daughter-list order
+// (sib_count of them from entry sib_first of the scene's sibling lists), or
+// every sibling when the list would exceed kMaxFaceSiblings (sib_count 0).
 struct alignas(16) Plane {
     float n[3];
     float h;
     float shade[3];
     uint32_t across;  // kInvalid, a daughter-list slot, or kFaceDynamic
     uint32_t surface_parent, surface_across;
-    uint32_t pad[2];
+    uint32_t sib_first, sib_count;
 };
+constexpr uint32_t kMaxFaceSiblings = 64;
 constexpr uint32_t kMaxMeshPlanes = 1024;  // faces per convex mesh solid
 
 constexpr uint32_t kSolidTriangles = 0x80000000u;  // Volume::plane_count flag
@@ -150,12 +153,18 @@ constexpr int kBvhMaxDepth = 15;  // root at depth 0; the host rejects deeper tr
 // Box of the daughter in one slot of the daughter list (slot order = the
 // list's order, so BVH leaves address it directly).  32 bytes: the BVH walk
 // reads two aligned 16-byte words per candidate instead of a Volume record.
-struct DaughterBox {
+struct alignas(16) DaughterBox {
     float lo[3];
     uint32_t volume;
     float hi[3];
-    uint32_t mesh;  // nonzero: the daughter is a convex mesh within this box
+    // 0: a box solid.  Else a mesh solid within this box: a convex one's planes
+    // as plane_count | plane_begin << kMeshBeginShift (no volume record read
+    // to find them), or kMeshInVolume (a triangle solid, or a plane range
+    // beyond the encoding): read the volume record.
+    uint32_t mesh;
 };
+constexpr uint32_t kMeshBeginShift = 11;  // plane_count <= kMaxMeshPlanes < 2^11
+constexpr uint32_t kMeshInVolume = (1u << kMeshBeginShift) - 1u;
 
 struct SceneView {
     const float* pool;
@@ -175,20 +184,27 @@ struct SceneView {
     uint32_t n_pool;    // floats in the pool
     uint32_t n_planes;  // 0 (and n_tris 0): every solid is a box
     uint32_t n_tris;
+    uint32_t n_sib_quads;  // 16-byte words of face sibling lists after the triangles
+    uint32_t max_daughters;  // most daughters of any volume
 };
 
 // Scenes with mesh solids (convex or triangle).
 inline __host__ __device__ bool has_meshes(const SceneView& sc) { return (sc.n_planes | sc.n_tris) != 0u; }
 
-// Bytes of the plane and triangle records (one array).
+// Bytes of the plane and triangle records and the face sibling lists (one array).
 inline __host__ __device__ size_t mesh_record_bytes(const SceneView& sc) {
-    return static_cast<size_t>(sc.n_planes) * sizeof(Plane) + static_cast<size_t>(sc.n_tris) * sizeof(Tri);
+    return static_cast<size_t>(sc.n_planes) * sizeof(Plane) + static_cast<size_t>(sc.n_tris) * sizeof(Tri) +
+           static_cast<size_t>(sc.n_sib_quads) * 16u;
 }
 
 // The triangle records follow the plane records in one array (in global
-// memory and in the LDS copy alike).
+// memory and in the LDS copy alike), and the face sibling lists (daughter-list
+// slots, four per 16-byte word) follow the triangles.
 inline __host__ __device__ const Tri* scene_tris(const SceneView& sc) {
     return reinterpret_cast<const Tri*>(sc.planes + sc.n_planes);
+}
+inline __host__ __device__ const uint32_t* scene_face_sibs(const SceneView& sc) {
+    return reinterpret_cast<const uint32_t*>(scene_tris(sc) + sc.n_tris);
 }
 
 }  // namespace g4go

@@ -415,13 +415,14 @@ G4GO_HD void mesh_exit(const Plane* planes, uint32_t begin, uint32_t count, V3 p
 
 // Entry distance into the solid from p outside it, kInf on a miss or when the
 // entry is closer than tmin.
+template <bool kBatch>
 G4GO_HD float mesh_entry(const Plane* planes, uint32_t begin, uint32_t count, V3 p, V3 d, float tmin,
                          uint32_t& plane) {
     float a = -1.0f, b = 0.0f;  // entry a / b over n.d < 0 (b = -n.d > 0)
     float c = 1.0f, e = 0.0f;   // exit c / e over n.d > 0
     uint32_t best = kInvalid;
-    for (uint32_t k = begin; k < begin + count; ++k) {
-        const float4 q = load_plane(planes[k]);
+    bool outside = false;  // parallel to a face, outside its plane: a miss
+    const auto test = [&](const float4 q, uint32_t k) {
         const V3 n = {q.x, q.y, q.z};
         const float nd = dot(n, d);
         const float num = q.w - dot(n, p);
@@ -436,11 +437,36 @@ G4GO_HD float mesh_entry(const Plane* planes, uint32_t begin, uint32_t count, V3
                 c = num;
                 e = nd;
             }
-        } else if (num < 0.0f) {
-            return kInf;  // parallel to a face, outside its plane
+        } else {
+            outside = outside || num < 0.0f;
+        }
+    };
+    // The planes in order.  kBatch (a search spread over a wave, where the
+    // latency of each load is the wave's): three loads issued together, and
+    // no early exit.  Else one by one, leaving at the first outside parallel
+    // face (three at a time made the single-lane steps of a mesh crystal 20%
+    // slower).
+    const uint32_t end = begin + count;
+    uint32_t k = begin;
+    if constexpr (kBatch) {
+        for (; k + 2u < end; k += 3u) {
+            const float4 q0 = load_plane(planes[k]), q1 = load_plane(planes[k + 1u]), q2 = load_plane(planes[k + 2u]);
+            test(q0, k);
+            test(q1, k + 1u);
+            test(q2, k + 2u);
+        }
+        for (; k < end; ++k) {
+            test(load_plane(planes[k]), k);
+        }
+    } else {
+        for (; k < end; ++k) {
+            test(load_plane(planes[k]), k);
+            if (outside) {
+                return kInf;
+            }
         }
     }
-    if (best == kInvalid) {
+    if (best == kInvalid || outside) {
         return kInf;
     }
     const float t_near = a / b;
@@ -716,34 +742,71 @@ G4GO_HD uint32_t find_surface(const SceneView& sc, uint32_t from, uint32_t to) {
 // solids).  A scene runs the lowest level that holds it.
 enum GeomLevel : int { kGeomBoxes = 0, kGeomConvex = 1, kGeomTriangles = 2 };
 
+// A daughter box read with two 16-byte loads (its fields one by one would be
+// eight loads, each waited for where a test reads it).
+G4GO_HD DaughterBox load_dbox(const DaughterBox& src) {
+    DaughterBox b;
+    __builtin_memcpy(&b, &src, sizeof(b));
+    return b;
+}
+
 // Entry distance into one daughter: its box, or (kGeom >= kGeomConvex) its
 // mesh when it has one (plane = the entered face, else kInvalid).
-template <int kGeom>
+template <int kGeom, bool kBatch = false>
 G4GO_HD float daughter_entry(const SceneView& sc, const DaughterBox& b, V3 p, V3 d, V3 inv, float tmin, int& axis,
                              uint32_t& plane) {
     if (kGeom != kGeomBoxes && b.mesh != 0u) {
+        if (b.mesh != kMeshInVolume) {
+            return mesh_entry<kBatch>(sc.planes, b.mesh >> kMeshBeginShift, b.mesh & kMeshInVolume, p, d, tmin, plane);
+        }
         const Volume& v = sc.volumes[b.volume];
         if (kGeom == kGeomTriangles && (v.plane_count & kSolidTriangles) != 0u) {
             bool leave;
             return tri_closest(sc, v, p, d, tmin, kTriEntering, plane, leave);
         }
-        return mesh_entry(sc.planes, v.plane_begin, v.plane_count, p, d, tmin, plane);
+        return mesh_entry<kBatch>(sc.planes, v.plane_begin, v.plane_count, p, d, tmin, plane);
     }
     plane = kInvalid;
     return box_entry(b.lo, b.hi, p, inv, tmin, axis);
 }
 
-// Nearest daughter entry of volume `vol` (BVH walk when the volume has one).
-// `plane` (mesh levels only) is the entered mesh face, kInvalid for a box face.
+// In scenes with convex mesh solids (kGeomConvex) a volume with at most
+// kScanDaughters daughters is searched by testing every daughter, BVH or not:
+// the smallest entry distance, ties to the lower volume index.  The
+// transport kernels split that test over the lanes of a wave for the few
+// lanes that need it (wave_daughter_search); every other caller runs it per
+// lane.  Both give the same answer bit for bit, whatever order the daughters
+// are tested in.  Larger volumes, and volumes of the other geometry levels
+// with more than eight daughters, take the BVH walk everywhere.
+constexpr uint32_t kScanDaughters = 128;
+
+G4GO_HD bool scans_daughters(int geom, const Volume& vol) {
+    return vol.bvh_root == kInvalid || (geom == kGeomConvex && vol.daughter_count <= kScanDaughters);
+}
+
+// A daughter search's answer: entry distance (kInf: none), the daughter
+// entered (kInvalid: none) and the box axis or mesh face it enters through.
+// which == kDaughterUnsearched: not searched yet.
+constexpr uint32_t kDaughterUnsearched = 0xfffffffdu;
+struct DaughterHit {
+    float t;
+    uint32_t which;
+    int axis;
+    uint32_t plane;
+};
+
+// Nearest daughter entry of volume `vol` (BVH walk when the volume has one and
+// scans_daughters is false).  `plane` (mesh levels only) is the entered mesh
+// face, kInvalid for a box face.
 template <int kGeom>
 G4GO_HD float nearest_daughter(const SceneView& sc, const Volume& vol, V3 p, V3 d, float tmin, uint32_t& which,
                                int& axis, uint32_t& plane) {
     const V3 inv = {1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
     float best = kInf;
     which = kInvalid;
-    if (vol.bvh_root == kInvalid) {
+    if (scans_daughters(kGeom, vol)) {
         for (uint32_t k = 0; k < vol.daughter_count; ++k) {
-            const DaughterBox& b = sc.dbox[vol.daughter_begin + k];
+            const DaughterBox b = load_dbox(sc.dbox[vol.daughter_begin + k]);
             const uint32_t j = b.volume;
             int a = 0;
             uint32_t f = kInvalid;
@@ -783,7 +846,7 @@ G4GO_HD float nearest_daughter(const SceneView& sc, const Volume& vol, V3 p, V3 
         if ((node.right_or_count & kBvhLeaf) != 0) {
             const uint32_t count = node.right_or_count & ~kBvhLeaf;
             for (uint32_t k = 0; k < count; ++k) {
-                const DaughterBox& b = sc.dbox[node.left + k];
+                const DaughterBox b = load_dbox(sc.dbox[node.left + k]);
                 const uint32_t j = b.volume;
                 int a = 0;
                 uint32_t f = kInvalid;
@@ -829,7 +892,7 @@ G4GO_HD uint32_t point_neighbor(const SceneView& sc, uint32_t current, V3 q) {
     }
     const Volume& par = sc.volumes[cur.parent];
     for (uint32_t k = 0; k < par.daughter_count; ++k) {
-        const DaughterBox& b = sc.dbox[par.daughter_begin + k];
+        const DaughterBox b = load_dbox(sc.dbox[par.daughter_begin + k]);
         if (b.volume != current && q.x >= b.lo[0] && q.x <= b.hi[0] && q.y >= b.lo[1] && q.y <= b.hi[1] &&
             q.z >= b.lo[2] && q.z <= b.hi[2]) {
             if (kGeom != kGeomBoxes && b.mesh != 0u && !solid_contains<kGeom>(sc, sc.volumes[b.volume], q)) {
@@ -839,6 +902,25 @@ G4GO_HD uint32_t point_neighbor(const SceneView& sc, uint32_t current, V3 q) {
         }
     }
     return cur.parent;
+}
+
+// point_neighbor over a face's listed siblings (daughter-list slots in
+// ascending order, the current volume not among them): every sibling left out
+// has a box that cannot hold q, so the first listed one that holds it is the
+// first of all.
+template <int kGeom>
+G4GO_HD uint32_t listed_neighbor(const SceneView& sc, const uint32_t* slots, uint32_t count, uint32_t parent, V3 q) {
+    for (uint32_t k = 0; k < count; ++k) {
+        const DaughterBox b = load_dbox(sc.dbox[slots[k]]);
+        if (q.x >= b.lo[0] && q.x <= b.hi[0] && q.y >= b.lo[1] && q.y <= b.hi[1] && q.z >= b.lo[2] &&
+            q.z <= b.hi[2]) {
+            if (kGeom != kGeomBoxes && b.mesh != 0u && !solid_contains<kGeom>(sc, sc.volumes[b.volume], q)) {
+                continue;
+            }
+            return b.volume;
+        }
+    }
+    return parent;
 }
 
 // ---------------------------------------------------------------------------
@@ -1244,9 +1326,13 @@ G4GO_HD int box_step(Photon& ph, const SceneView& sc, const Rng& rng, const Step
 // and a daughter entered up to eps beyond the current volume's exit wins (a
 // daughter face coincident with the wall, which the reference settles by its
 // surface rank re-trace, OptiXTransportDevice.cu:105-127, 742-777).
+//
+// `pre` (kGeomConvex) is the daughter search of this step when the caller has
+// made it already (wave_daughter_search), else which == kDaughterUnsearched.
 template <bool kTrace, int kGeom = kGeomBoxes>
 G4GO_HD int transport_step(Photon& ph, const SceneView& sc, const Rng& rng, const StepConfig& cfg, bool& zero_step,
-                           uint32_t& sensor_out, StepTrace* trace) {
+                           uint32_t& sensor_out, StepTrace* trace,
+                           DaughterHit pre = DaughterHit{kInf, kDaughterUnsearched, 0, kInvalid}) {
     if constexpr (!kTrace && kGeom == kGeomBoxes) {
         const int lean = box_step(ph, sc, rng, cfg, zero_step, sensor_out);
         if (lean != kStepFallback) {
@@ -1303,7 +1389,15 @@ G4GO_HD int transport_step(Photon& ph, const SceneView& sc, const Rng& rng, cons
         uint32_t which;
         int a2 = 0;
         uint32_t p2 = kInvalid;
-        const float t = nearest_daughter<kGeom>(sc, vol, ph.pos, ph.dir, cfg.eps, which, a2, p2);
+        float t;
+        if (kGeom == kGeomConvex && pre.which != kDaughterUnsearched) {
+            t = pre.t;
+            which = pre.which;
+            a2 = pre.axis;
+            p2 = pre.plane;
+        } else {
+            t = nearest_daughter<kGeom>(sc, vol, ph.pos, ph.dir, cfg.eps, which, a2, p2);
+        }
         if (which != kInvalid && (kMesh ? t <= dist + cfg.eps : t <= dist)) {
             dist = t;
             axis = a2;
@@ -1382,13 +1476,15 @@ G4GO_HD int transport_step(Photon& ph, const SceneView& sc, const Rng& rng, cons
             const V3 q = add(bpos, scale(V3{pl.n[0], pl.n[1], pl.n[2]}, cfg.eps));
             const uint32_t across = pl.across;
             if (across == kFaceDynamic) {
-                next = point_neighbor<kGeom>(sc, ph.volume, q);
+                next = pl.sib_count != 0u
+                           ? listed_neighbor<kGeom>(sc, scene_face_sibs(sc) + pl.sib_first, pl.sib_count, vol.parent, q)
+                           : point_neighbor<kGeom>(sc, ph.volume, q);
                 surface = next == kInvalid ? kInvalid : find_surface(sc, ph.volume, next);
             } else {
                 next = vol.parent;
                 surface = pl.surface_parent;
                 if (across != kInvalid) {
-                    const DaughterBox& b = sc.dbox[across];
+                    const DaughterBox b = load_dbox(sc.dbox[across]);
                     if (q.x >= b.lo[0] && q.x <= b.hi[0] && q.y >= b.lo[1] && q.y <= b.hi[1] && q.z >= b.lo[2] &&
                         q.z <= b.hi[2]) {
                         if (b.mesh == 0u || solid_contains<kGeom>(sc, sc.volumes[b.volume], q)) {

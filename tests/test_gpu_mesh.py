@@ -320,3 +320,32 @@ def test_crystal_ring_bit_exact():
     out = parity(cs.scene, src, 5, [0, 1], [20000, 3000])
     assert out.statistics["detected"] > 5000 and out.statistics["escaped"] > 5000
     assert np.unique(out.detections["sensor_id"]).size > 10
+
+
+@pytest.mark.parametrize("form", ["plain", "persistent-512"])
+def test_crystal_ring_world_source_bit_exact(form):
+    """Photons from the air inside the ring: most lanes of a wave search the
+    world's 96 mesh daughters at once (each lane testing all of them), later
+    only a few (the wave sharing each search); faces whose box meets several
+    siblings resolve the neighbour over their sibling lists."""
+    log(f"case: ring of 48 turned crystals, source in the world, {form}")
+    cs, _ = workloads.ring_scene()
+    src = abi.SyntheticSource()
+    src.lo_mm[:] = [-20.0, -20.0, -1.0]
+    src.hi_mm[:] = [20.0, 20.0, 1.0]
+    src.wavelength_mean_nm, src.wavelength_sigma_nm, src.decay_time_ns = 420.0, 20.0, 40.0
+    src.volume_id = cs.scene.world_volume_id
+    form_ = PLAIN if form == "plain" else LaunchForm(persistent=True, block_size=512, chunk=256, refill_threshold=4,
+                                                     blocks_per_cu=2)
+    out = parity(cs.scene, src, 21, [0, 2], [60000, 10000], form_)
+    assert out.statistics["detected"] > 1000 and out.statistics["escaped"] > 20000
+    assert np.unique(out.detections["sensor_id"]).size > 40
+
+
+def test_large_crystal_ring_bvh_walk_bit_exact():
+    """80 crystals and SiPMs (160 daughters, beyond the all-daughter test):
+    the world's daughters are searched by the BVH walk."""
+    log("case: ring of 80 turned crystals (BVH walk over 160 mesh daughters)")
+    cs, src = workloads.ring_scene(n=80, radius_mm=80.0)
+    out = parity(cs.scene, src, 23, [0], [20000])
+    assert out.statistics["detected"] > 3000 and out.statistics["escaped"] > 3000
