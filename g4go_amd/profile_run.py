@@ -14,8 +14,9 @@
 prints, per region of the transport kernel (csrc/region_prof.h), its share of
 the wave cycles and its lane utilisation.
 
-The launch form is the cached autotuner result for the device when present,
-else the default persistent form.  --copy-bytes additionally runs one device
+The launch form is the autotuner result cached for the workload on the
+device (for the mesh workloads their own tags, as bench.py tunes them), else
+the crystal's, else the default persistent form.  --copy-bytes additionally runs one device
 to device copy of that many bytes (hipMemcpy through torch), whose FETCH_SIZE
 and WRITE_SIZE calibrate the traffic counters against a known byte count.
 """
@@ -82,8 +83,12 @@ def main(argv=None) -> int:
         cs = workloads.crystal_scene()
         src = workloads.synthetic_source(cs)
     host = HipTransportHost(PhotonTransportConfig(seed=args.seed, sort_hits=False))
-    form = autotune.cached(autotune.device_key(host), "crystal-synthetic") or LaunchForm(persistent=True, block_size=256, chunk=256,
-                                                                      refill_threshold=16, blocks_per_cu=4)
+    # each workload in the launch form cached for it (the crystal's otherwise)
+    tag = {"uneven": "crystal-uneven", "tapered": "mesh-tapered_crystal", "rotated": "mesh-rotated_crystal",
+           "ring": "mesh-crystal_ring_48", "lguide": "mesh-l_guide_triangles"}.get(args.workload, "crystal-synthetic")
+    key = autotune.device_key(host)
+    form = (autotune.cached(key, tag) or autotune.cached(key, "crystal-synthetic")
+            or LaunchForm(persistent=True, block_size=512, chunk=256, refill_threshold=4, blocks_per_cu=2))
     if args.launch:
         v = [int(x) for x in args.launch.split(",")]
         form = LaunchForm(persistent=True, block_size=v[0], blocks_per_cu=v[1], chunk=v[2], refill_threshold=v[3],
