@@ -336,7 +336,7 @@ void mark_hot_grid(HostScene& s) {
     }
 }
 
-// Solid of a closed triangle mesh (world frame): face planes when it is
+// Solid of a triangle mesh (world frame): face planes when it is closed and
 // convex, else its triangles, plus the bounding box.  All arithmetic is FP64
 // in a fixed order, the results rounded once to FP32:
 //   * each triangle's plane: unit normal u = (v1 - v0) x (v2 - v0) / |...|,
@@ -350,16 +350,20 @@ void mark_hot_grid(HostScene& s) {
 //     coordinates) belongs to exactly two triangles; it is convex when no
 //     vertex lies more than 1e-6 of the mesh size outside a face plane and c
 //     lies more than that inside every face plane.
-// A convex mesh becomes planes (clipping tests).  Any other closed mesh must
-// be wound consistently (each edge traversed once each way); it becomes
-// triangles, all turned outwards when the signed volume is negative, with
-// keys = their index in the mesh.  The reference accepts any triangle mesh
-// (OptiXTransportHost.cpp:1118-1200) and ignores the winding; these tests
-// need closed meshes.
+// A closed convex mesh becomes planes (clipping tests).  Any other mesh
+// becomes triangles with keys = their index in the mesh.  A closed one's
+// winding is made consistent (each edge traversed once each way) when it is
+// not: from triangle 0, over shared welded edges, each triangle is reversed
+// when it traverses an edge in the same direction as the neighbour it is
+// reached from; then all are turned outwards when the signed volume is
+// negative.  A mesh that is open, flat, not orientable, or in several parts
+// and not wound consistently is an open solid (kSolidOpen): its triangles as
+// given, hit from either side, holding no point.  The reference accepts any
+// triangle mesh (OptiXTransportHost.cpp:1118-1200) and ignores the winding.
 void mesh_solid(const g4go_volume_mesh& m, HostScene& s, Volume& dv) {
     const std::string who = "volume " + std::to_string(m.volume_id) + " mesh";
-    if (m.vertices_mm == nullptr || m.indices == nullptr || m.n_vertices < 4 || m.n_triangles < 4) {
-        fail(G4GO_ERROR_INVALID_ARGUMENT, who + ": needs at least 4 vertices and 4 triangles");
+    if (m.vertices_mm == nullptr || m.indices == nullptr || m.n_vertices < 3 || m.n_triangles < 1) {
+        fail(G4GO_ERROR_INVALID_ARGUMENT, who + ": needs at least 3 vertices and 1 triangle");
     }
     if (m.reserved != 0) {
         fail(G4GO_ERROR_INVALID_ARGUMENT, who + ": reserved field must be 0");
@@ -397,7 +401,13 @@ void mesh_solid(const g4go_volume_mesh& m, HostScene& s, Volume& dv) {
     const double tol = 1e-6 * size;
     // closed: every edge of the welded mesh in exactly two triangles;
     // consistently wound: traversed once each way
-    bool consistent = true;
+    bool closed = true, consistent = true;
+    struct MeshEdge {
+        uint64_t key;  // welded (min, max) vertex pair
+        uint32_t tri;
+        uint32_t dir;  // 1: traversed max -> min
+    };
+    std::vector<MeshEdge> edges;  // sorted by (key, tri): the two triangles of an edge side by side
     {
         std::vector<uint32_t> order(nv);
         for (uint32_t i = 0; i < nv; ++i) {
@@ -419,7 +429,6 @@ void mesh_solid(const g4go_volume_mesh& m, HostScene& s, Volume& dv) {
                               v[3 * a + 2] == v[3 * order[i - 1] + 2];
             weld[a] = same ? weld[order[i - 1]] : a;
         }
-        std::vector<uint64_t> edges;  // (min, max) << 1 | (traversed max -> min)
         edges.reserve(3ull * nt);
         for (uint32_t t = 0; t < nt; ++t) {
             for (int e = 0; e < 3; ++e) {
@@ -427,20 +436,22 @@ void mesh_solid(const g4go_volume_mesh& m, HostScene& s, Volume& dv) {
                 if (a == b) {
                     fail(G4GO_ERROR_INVALID_ARGUMENT, who + ": degenerate triangle");
                 }
-                edges.push_back((((static_cast<uint64_t>(std::min(a, b)) << 32) | std::max(a, b)) << 1) |
-                                (a > b ? 1u : 0u));
+                edges.push_back({(static_cast<uint64_t>(std::min(a, b)) << 32) | std::max(a, b), t, a > b ? 1u : 0u});
             }
         }
-        std::sort(edges.begin(), edges.end());
+        std::sort(edges.begin(), edges.end(), [](const MeshEdge& x, const MeshEdge& y) {
+            return x.key != y.key ? x.key < y.key : x.tri < y.tri;
+        });
         for (size_t i = 0; i < edges.size();) {
             size_t j = i;
-            while (j < edges.size() && (edges[j] >> 1) == (edges[i] >> 1)) {
+            while (j < edges.size() && edges[j].key == edges[i].key) {
                 ++j;
             }
             if (j - i != 2) {
-                fail(G4GO_ERROR_INVALID_ARGUMENT, who + ": not closed (an edge is not shared by exactly two triangles)");
+                closed = false;
+            } else {
+                consistent = consistent && edges[i].dir != edges[i + 1].dir;
             }
-            consistent = consistent && (edges[i] & 1u) != (edges[i + 1] & 1u);
             i = j;
         }
     }
@@ -511,7 +522,7 @@ void mesh_solid(const g4go_volume_mesh& m, HostScene& s, Volume& dv) {
         }
         into->points.insert(into->points.end(), {p0[0], p0[1], p0[2], p1[0], p1[1], p1[2], p2[0], p2[1], p2[2]});
     }
-    bool convex = !too_many;
+    bool convex = closed && !too_many;
     for (size_t j = 0; convex && j < faces.size(); ++j) {
         const Face64& f = faces[j];
         convex = f.h - (f.u[0] * c[0] + f.u[1] * c[1] + f.u[2] * c[2]) > tol;
@@ -535,9 +546,47 @@ void mesh_solid(const g4go_volume_mesh& m, HostScene& s, Volume& dv) {
         }
         dv.plane_count = static_cast<uint32_t>(faces.size());
     } else {
-        if (!consistent) {
-            fail(G4GO_ERROR_INVALID_ARGUMENT, who + ": not convex, and its triangles are not wound consistently");
+        // An open mesh, or one whose winding cannot be made consistent, or a
+        // flat one, is an open solid: its triangles as given (kSolidOpen).
+        bool open = !closed;
+        // reversed[t]: triangle t's winding is turned to make the mesh consistent
+        std::vector<uint8_t> reversed(nt, 0);
+        if (!open && !consistent) {
+            std::vector<uint32_t> first(nt * 3u);  // per triangle edge: its entry in `edges`
+            std::vector<uint32_t> seen(nt, 0);
+            for (uint32_t i = 0; i < edges.size(); ++i) {
+                first[3u * edges[i].tri + seen[edges[i].tri]++] = i;
+            }
+            std::vector<uint8_t> done(nt, 0);
+            std::vector<uint32_t> queue{0u};
+            done[0] = 1;
+            for (size_t q = 0; q < queue.size(); ++q) {
+                const uint32_t t = queue[q];
+                for (int e = 0; e < 3; ++e) {
+                    const uint32_t i = first[3u * t + e];
+                    const MeshEdge& mine = edges[i];
+                    const MeshEdge& other = edges[(i & 1u) == 0u ? i + 1u : i - 1u];
+                    // opposite directions once both windings are applied
+                    const uint8_t want = static_cast<uint8_t>((mine.dir ^ reversed[t] ^ 1u) ^ other.dir);
+                    if (!done[other.tri]) {
+                        done[other.tri] = 1;
+                        reversed[other.tri] = want;
+                        queue.push_back(other.tri);
+                    } else if (reversed[other.tri] != want) {
+                        open = true;  // not orientable
+                    }
+                }
+            }
+            open = open || queue.size() != nt;  // in several parts, not wound consistently
+            if (open) {
+                std::fill(reversed.begin(), reversed.end(), 0);
+            }
         }
+        // triangle t's corners in its consistent winding
+        const auto corner = [&](uint32_t t, int j) {
+            const int k = reversed[t] != 0u && j != 0 ? 3 - j : j;
+            return m.indices[3 * t + k];
+        };
         if (nt > kMaxSolidTriangles) {
             fail(G4GO_ERROR_INVALID_ARGUMENT,
                  who + ": more than " + std::to_string(kMaxSolidTriangles) + " triangles in a non-convex mesh");
@@ -548,21 +597,19 @@ void mesh_solid(const g4go_volume_mesh& m, HostScene& s, Volume& dv) {
             double q[3][3];
             for (int j = 0; j < 3; ++j) {
                 for (int k = 0; k < 3; ++k) {
-                    q[j][k] = v[3 * m.indices[3 * t + j] + k] - c[k];
+                    q[j][k] = v[3 * corner(t, j) + k] - c[k];
                 }
             }
             vol6 += q[0][0] * (q[1][1] * q[2][2] - q[1][2] * q[2][1]) + q[0][1] * (q[1][2] * q[2][0] - q[1][0] * q[2][2]) +
                     q[0][2] * (q[1][0] * q[2][1] - q[1][1] * q[2][0]);
         }
-        if (!(std::fabs(vol6) > tol * size * size)) {
-            fail(G4GO_ERROR_INVALID_ARGUMENT, who + ": flat (no enclosed volume)");
-        }
-        const bool flip = vol6 < 0.0;
+        open = open || !(std::fabs(vol6) > tol * size * size);  // flat: no enclosed volume
+        const bool flip = !open && vol6 < 0.0;
         dv.plane_begin = static_cast<uint32_t>(s.tris.size());
         for (uint32_t t = 0; t < nt; ++t) {
-            const double* p0 = v + 3 * m.indices[3 * t];
-            const double* p1 = v + 3 * m.indices[3 * t + (flip ? 2 : 1)];
-            const double* p2 = v + 3 * m.indices[3 * t + (flip ? 1 : 2)];
+            const double* p0 = v + 3 * corner(t, 0);
+            const double* p1 = v + 3 * corner(t, flip ? 2 : 1);
+            const double* p2 = v + 3 * corner(t, flip ? 1 : 2);
             const double a[3] = {p1[0] - p0[0], p1[1] - p0[1], p1[2] - p0[2]};
             const double b[3] = {p2[0] - p0[0], p2[1] - p0[1], p2[2] - p0[2]};
             const double g[3] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
@@ -587,7 +634,7 @@ void mesh_solid(const g4go_volume_mesh& m, HostScene& s, Volume& dv) {
             s.tris.push_back(tr);
             s.tri_points.push_back({p0[0], p0[1], p0[2], p1[0], p1[1], p1[2], p2[0], p2[1], p2[2]});
         }
-        dv.plane_count = nt | kSolidTriangles;
+        dv.plane_count = nt | kSolidTriangles | (open ? kSolidOpen : 0u);
     }
     // bounding box rounded outwards
     for (int k = 0; k < 3; ++k) {
@@ -930,7 +977,7 @@ HostScene build_scene(const g4go_scene& in, float tolerance) {
     for (uint32_t i = 0; i < in.n_volumes; ++i) {
         Volume& v = s.volumes[i];
         if ((v.plane_count & kSolidTriangles) != 0) {
-            const uint32_t n = v.plane_count & ~kSolidTriangles;
+            const uint32_t n = v.plane_count & kSolidCountMask;
             v.faces[0].neighbor = n > 64 ? build_tri_bvh(s, v.plane_begin, n) : kInvalid;
         }
     }
@@ -941,7 +988,7 @@ HostScene build_scene(const g4go_scene& in, float tolerance) {
     for (uint32_t i = 0; i < in.n_volumes; ++i) {
         const Volume& v = s.volumes[i];
         const bool tri = (v.plane_count & kSolidTriangles) != 0;
-        const uint32_t n_faces = v.plane_count & ~kSolidTriangles;
+        const uint32_t n_faces = v.plane_count & kSolidCountMask;
         for (uint32_t k = v.plane_begin; k < v.plane_begin + n_faces; ++k) {
             Plane& pl = tri ? s.tris[k].face : s.planes[k];
             if (v.parent == kInvalid) {
@@ -1236,6 +1283,7 @@ struct Core {
         view.n_sib_quads = static_cast<uint32_t>(sib_quads);
         for (const Volume& v : s.volumes) {
             view.max_daughters = std::max(view.max_daughters, v.daughter_count);
+            view.open_solids += (v.plane_count & kSolidOpen) != 0u ? 1u : 0u;
         }
         view.n_daughters = static_cast<uint32_t>(s.daughters.size());
         view.hot_offset = s.hot_offset;

@@ -469,9 +469,10 @@ __device__ __forceinline__ void stage_scene(const KernelParams& p, float4* lds, 
     }
     if (kGeom != kGeomBoxes && (kLdsMode >= kLdsHot || (p.lds_records & kLdsPlanes))) {  // box scenes have none
         // planes, triangles and face sibling lists, whole 16-byte words (a
-        // scene with triangle solids runs kGeomTriangles; below it n_tris is 0)
+        // scene with triangle solids runs kGeomTriangles or kGeomOpen; below
+        // them n_tris is 0)
         const uint32_t quads = p.scene.n_planes * (sizeof(Plane) / 16u) +
-                               (kGeom == kGeomTriangles ? p.scene.n_tris * (sizeof(Tri) / 16u) : 0u) +
+                               (kGeom >= kGeomTriangles ? p.scene.n_tris * (sizeof(Tri) / 16u) : 0u) +
                                p.scene.n_sib_quads;
         sv.planes = static_cast<const Plane*>(stage_quads(p.scene.planes, quads));
     }
@@ -554,7 +555,7 @@ __device__ __forceinline__ void transport_body(const KernelParams& p) {
     // the block's hit stages (ring_slots entries of kRingQuads words).  A
     // generation pass fills it through the alive lanes; lanes that go idle
     // later take its entries [0, ring_count) from the top.
-    constexpr bool kInLaneRefill = kPersistent && kGeom == kGeomTriangles;
+    constexpr bool kInLaneRefill = kPersistent && kGeom >= kGeomTriangles;
     const uint32_t ring_slots = kPersistent && !kInLaneRefill ? uniform(p.ring_slots) : 0u;
     float4* const ring = s_dynamic + p.lds_stage_quads + (blockDim.x >> 6) * (kStageBytesPerWave / 16u) +
                          uniform(threadIdx.x >> 6) * ring_slots * kRingQuads;
@@ -871,8 +872,8 @@ transport_kernel_plain(KernelParams p) {
 template <int kSource, int kLdsMode, int kGeom = kGeomBoxes>
 void launch_transport_form(const KernelParams& p, bool persistent, int block, int grid, size_t lds, hipStream_t stream) {
     // the pre-generated instances exist for box and convex-mesh scenes
-    if (persistent && p.starts != nullptr && kGeom != kGeomTriangles) {
-        hipLaunchKernelGGL((transport_kernel_persistent<kSource, kLdsMode, kGeom, kGeom != kGeomTriangles>), dim3(grid),
+    if (persistent && p.starts != nullptr && kGeom < kGeomTriangles) {
+        hipLaunchKernelGGL((transport_kernel_persistent<kSource, kLdsMode, kGeom, (kGeom < kGeomTriangles)>), dim3(grid),
                            dim3(block), lds, stream, p);
     } else if (persistent) {
         hipLaunchKernelGGL((transport_kernel_persistent<kSource, kLdsMode, kGeom, false>), dim3(grid), dim3(block), lds,
@@ -887,13 +888,22 @@ void launch_transport_form(const KernelParams& p, bool persistent, int block, in
 // box scenes carry no mesh code and convex-mesh scenes no triangle code.
 // Triangle scenes have two LDS modes (their triangles rarely fit).
 __host__ __device__ inline int geom_level(const SceneView& sc) {
-    return sc.n_tris != 0u ? kGeomTriangles : (sc.n_planes != 0u ? kGeomConvex : kGeomBoxes);
+    return sc.n_tris != 0u ? (sc.open_solids != 0u ? kGeomOpen : kGeomTriangles)
+                           : (sc.n_planes != 0u ? kGeomConvex : kGeomBoxes);
 }
 
 template <int kSource>
 void launch_transport_source(const KernelParams& p, bool persistent, int block, int grid, size_t lds,
                              hipStream_t stream) {
     const int geom = geom_level(p.scene);
+    if (geom == kGeomOpen) {
+        if (lds_mode(p) >= kLdsHot) {
+            launch_transport_form<kSource, kLdsHot, kGeomOpen>(p, persistent, block, grid, lds, stream);
+        } else {
+            launch_transport_form<kSource, kLdsNone, kGeomOpen>(p, persistent, block, grid, lds, stream);
+        }
+        return;
+    }
     if (geom == kGeomTriangles) {
         if (lds_mode(p) >= kLdsHot) {
             launch_transport_form<kSource, kLdsHot, kGeomTriangles>(p, persistent, block, grid, lds, stream);
@@ -979,7 +989,7 @@ int launch_zero(void* ptr, size_t bytes, hipStream_t stream) {
 
 int launch_transport(const KernelParams& p, int source, bool persistent, int block, int grid, hipStream_t stream) {
     const size_t scene_lds = transport_lds_bytes(p);
-    if ((p.starts != nullptr && geom_level(p.scene) == kGeomTriangles) || block < 64 ||
+    if ((p.starts != nullptr && geom_level(p.scene) >= kGeomTriangles) || block < 64 ||
         block > static_cast<int>(kMaxBlock) || block % 64 != 0) {
         return 1;
     }
@@ -997,7 +1007,7 @@ int launch_transport(const KernelParams& p, int source, bool persistent, int blo
         return 1;  // the staged scene was planned for a larger block
     }
     q.ring_slots = 0;
-    if (persistent && geom_level(p.scene) != kGeomTriangles) {
+    if (persistent && geom_level(p.scene) < kGeomTriangles) {
         // the ring takes what the block's share of the CU's LDS leaves (the
         // triangle-solid instances refill in place and have no ring)
         const size_t resident = std::max<size_t>(1, kResidentWaves / waves);
@@ -1352,6 +1362,14 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) a
 template <int kSource>
 void launch_adjoint_source(const AdjointParams& p, int block, int grid, size_t lds, hipStream_t stream) {
     const int geom = geom_level(p.fwd.scene);
+    if (geom == kGeomOpen) {
+        if (lds_mode(p.fwd) >= kLdsHot) {
+            hipLaunchKernelGGL((adjoint_kernel<kSource, kLdsHot, kGeomOpen>), dim3(grid), dim3(block), lds, stream, p);
+        } else {
+            hipLaunchKernelGGL((adjoint_kernel<kSource, kLdsNone, kGeomOpen>), dim3(grid), dim3(block), lds, stream, p);
+        }
+        return;
+    }
     if (geom == kGeomTriangles) {
         if (lds_mode(p.fwd) >= kLdsHot) {
             hipLaunchKernelGGL((adjoint_kernel<kSource, kLdsHot, kGeomTriangles>), dim3(grid), dim3(block), lds, stream,
@@ -1590,7 +1608,9 @@ __global__ void trace_kernel(KernelParams p, uint32_t event_id, uint32_t photon_
             uint32_t sensor = kInvalid;
             const int geom = geom_level(p.scene);
             const int status =
-                geom == kGeomTriangles
+                geom == kGeomOpen
+                    ? transport_step<false, kGeomOpen>(ph, p.scene, p.rng, p.cfg, zero, sensor, nullptr)
+                : geom == kGeomTriangles
                     ? transport_step<false, kGeomTriangles>(ph, p.scene, p.rng, p.cfg, zero, sensor, nullptr)
                     : (geom == kGeomConvex
                            ? transport_step<false, kGeomConvex>(ph, p.scene, p.rng, p.cfg, zero, sensor, nullptr)

@@ -117,6 +117,12 @@ constexpr uint32_t kMaxFaceSiblings = 64;
 constexpr uint32_t kMaxMeshPlanes = 1024;  // faces per convex mesh solid
 
 constexpr uint32_t kSolidTriangles = 0x80000000u;  // Volume::plane_count flag
+// Volume::plane_count flag of a triangle solid whose mesh is open (or cannot
+// be wound consistently): it has no inside.  Its triangles are hit from either
+// side, it holds no point (a photon inside it leaves only through a triangle,
+// or into the parent where no triangle lies ahead).
+constexpr uint32_t kSolidOpen = 0x40000000u;
+constexpr uint32_t kSolidCountMask = 0x3fffffffu;  // Volume::plane_count without the flags
 constexpr uint32_t kFaceTri = 0x80000000u;         // face reference: a triangle (else a plane)
 constexpr uint32_t kMaxSolidTriangles = 65536;     // triangles per triangle solid
 
@@ -186,6 +192,7 @@ struct SceneView {
     uint32_t n_tris;
     uint32_t n_sib_quads;  // 16-byte words of face sibling lists after the triangles
     uint32_t max_daughters;  // most daughters of any volume
+    uint32_t open_solids;    // volumes that are open triangle solids
 };
 
 // Scenes with mesh solids (convex or triangle).

@@ -642,7 +642,7 @@ G4GO_HD float tri_closest(const SceneView& sc, const Volume& vol, V3 p, V3 d, fl
     TriBest b{kInf, kInvalid, kInvalid, false};
     const uint32_t root = vol.faces[0].neighbor;
     if (root == kInvalid) {
-        const uint32_t begin = vol.plane_begin, count = vol.plane_count & ~kSolidTriangles;
+        const uint32_t begin = vol.plane_begin, count = vol.plane_count & kSolidCountMask;
         for (uint32_t i = begin; i < begin + count; ++i) {
             tri_test(tris, ray, i, tmin, mode, b);
         }
@@ -654,6 +654,15 @@ G4GO_HD float tri_closest(const SceneView& sc, const Volume& vol, V3 p, V3 d, fl
     return b.t;
 }
 
+// Closest hit at t >= tmin on the triangles of an open triangle solid, from
+// either side.  Out of line, so that the closed solids' searches (inlined,
+// with their orientation fixed) keep their code and registers.
+static __host__ __device__ __attribute__((noinline)) float tri_closest_open(const SceneView& sc, const Volume& vol,
+                                                                         V3 p, V3 d, float tmin, uint32_t& face) {
+    bool leaving;
+    return tri_closest(sc, vol, p, d, tmin, kTriAny, face, leaving);
+}
+
 // Fallback exit from a triangle solid when no triangle ahead is a leaving
 // hit: among the triangles the ray moves away from (n.d > 0), the one whose
 // plane lies nearest the point (ties: lower key); the distance to that plane
@@ -661,7 +670,7 @@ G4GO_HD float tri_closest(const SceneView& sc, const Volume& vol, V3 p, V3 d, fl
 static __host__ __device__ __attribute__((noinline)) float tri_nearest_leaving(const SceneView& sc, const Volume& vol,
                                                                             V3 p, V3 d, uint32_t& face) {
     const Tri* tris = scene_tris(sc);
-    const uint32_t begin = vol.plane_begin, count = vol.plane_count & ~kSolidTriangles;
+    const uint32_t begin = vol.plane_begin, count = vol.plane_count & kSolidCountMask;
     float best_gap = kInf, best_num = 0.0f, best_nd = 1.0f;
     uint32_t best_i = kInvalid, best_key = kInvalid;
     for (uint32_t i = begin; i < begin + count; ++i) {
@@ -696,11 +705,19 @@ G4GO_HD bool tri_contains(const SceneView& sc, const Volume& vol, V3 q) {
     return t < kInf && leave;
 }
 
-// Point inside a mesh solid (convex, or a triangle solid at kGeomTriangles).
+// Geometry levels of the step code: kGeomBoxes (box solids only),
+// kGeomConvex (also convex mesh solids), kGeomTriangles (also closed
+// triangle solids), kGeomOpen (also open triangle solids).  A scene runs the
+// lowest level that holds it, so the closed-solid kernels carry no code of
+// open solids.
+enum GeomLevel : int { kGeomBoxes = 0, kGeomConvex = 1, kGeomTriangles = 2, kGeomOpen = 3 };
+
+// Point inside a mesh solid (convex, or a triangle solid at kGeomTriangles
+// and kGeomOpen; an open solid holds no point).
 template <int kGeom>
 G4GO_HD bool solid_contains(const SceneView& sc, const Volume& v, V3 q) {
-    if (kGeom == 2 && (v.plane_count & kSolidTriangles) != 0u) {
-        return tri_contains(sc, v, q);
+    if (kGeom >= 2 && (v.plane_count & kSolidTriangles) != 0u) {
+        return !(kGeom == kGeomOpen && (v.plane_count & kSolidOpen) != 0u) && tri_contains(sc, v, q);
     }
     return mesh_contains(sc.planes, v.plane_begin, v.plane_count, q);
 }
@@ -708,7 +725,7 @@ G4GO_HD bool solid_contains(const SceneView& sc, const Volume& v, V3 q) {
 // The face record of a mesh face reference (a plane or a triangle).
 template <int kGeom>
 G4GO_HD const Plane& face_record(const SceneView& sc, uint32_t face) {
-    return kGeom == 2 && (face & kFaceTri) != 0u ? scene_tris(sc)[face & ~kFaceTri].face : sc.planes[face];
+    return kGeom >= 2 && (face & kFaceTri) != 0u ? scene_tris(sc)[face & ~kFaceTri].face : sc.planes[face];
 }
 
 // Border binding lookup with skin precedence (OptiXTransportDevice.cu:218-252).
@@ -737,10 +754,6 @@ G4GO_HD uint32_t find_surface(const SceneView& sc, uint32_t from, uint32_t to) {
     return f.skin != kInvalid ? f.skin : t.skin;
 }
 
-// Geometry levels of the step code: kGeomBoxes (box solids only),
-// kGeomConvex (also convex mesh solids), kGeomTriangles (also triangle
-// solids).  A scene runs the lowest level that holds it.
-enum GeomLevel : int { kGeomBoxes = 0, kGeomConvex = 1, kGeomTriangles = 2 };
 
 // A daughter box read with two 16-byte loads (its fields one by one would be
 // eight loads, each waited for where a test reads it).
@@ -760,7 +773,10 @@ G4GO_HD float daughter_entry(const SceneView& sc, const DaughterBox& b, V3 p, V3
             return mesh_entry<kBatch>(sc.planes, b.mesh >> kMeshBeginShift, b.mesh & kMeshInVolume, p, d, tmin, plane);
         }
         const Volume& v = sc.volumes[b.volume];
-        if (kGeom == kGeomTriangles && (v.plane_count & kSolidTriangles) != 0u) {
+        if (kGeom >= kGeomTriangles && (v.plane_count & kSolidTriangles) != 0u) {
+            if (kGeom == kGeomOpen && (v.plane_count & kSolidOpen) != 0u) {
+                return tri_closest_open(sc, v, p, d, tmin, plane);
+            }
             bool leave;
             return tri_closest(sc, v, p, d, tmin, kTriEntering, plane, leave);
         }
@@ -1364,10 +1380,20 @@ G4GO_HD int transport_step(Photon& ph, const SceneView& sc, const Rng& rng, cons
     int axis = 0;
     uint32_t plane = kInvalid;
     constexpr bool kMesh = kGeom != kGeomBoxes;
-    if (kGeom == kGeomTriangles && (vol.plane_count & kSolidTriangles) != 0u) {
-        bool leave;
-        dist = tri_closest(sc, vol, ph.pos, ph.dir, 0.0f, kTriLeaving, plane, leave);
-        if (!finitef(dist)) {
+    bool open_solid = false;  // in an open triangle solid, with no triangle of it ahead
+    if (kGeom >= kGeomTriangles && (vol.plane_count & kSolidTriangles) != 0u) {
+        // A closed solid: its closest leaving hit.  An open one: any of its
+        // triangles beyond the boundary tolerance, from either side (the
+        // reference's trace from the photon, tmin = eps).
+        const bool open = kGeom == kGeomOpen && (vol.plane_count & kSolidOpen) != 0u;
+        if (open) {
+            dist = tri_closest_open(sc, vol, ph.pos, ph.dir, cfg.eps, plane);
+            open_solid = !finitef(dist);
+        } else {
+            bool leave;
+            dist = tri_closest(sc, vol, ph.pos, ph.dir, 0.0f, kTriLeaving, plane, leave);
+        }
+        if (!open && !finitef(dist)) {
             // No leaving triangle ahead: rounding left the photon a hair
             // outside its solid at an edge (after a reflection there), moving
             // out past the edge.  It leaves through the nearest face it moves
@@ -1408,6 +1434,13 @@ G4GO_HD int transport_step(Photon& ph, const SceneView& sc, const Rng& rng, cons
         }
     }
     if (!finitef(dist)) {
+        if (kGeom == kGeomOpen && open_solid && vol.parent != kInvalid) {
+            // out of an open solid where none of its triangles lies ahead:
+            // into the parent where the photon is, without a boundary
+            ph.volume = vol.parent;
+            zero_step = true;
+            return kAlive;
+        }
         return kEscaped;
     }
     if (dist <= cfg.eps) {
@@ -1472,8 +1505,11 @@ G4GO_HD int transport_step(Photon& ph, const SceneView& sc, const Rng& rng, cons
     if (!entering) {
         if (mesh_face) {
             // the point just outside the face, along its outward plane normal
+            // (an open solid's triangle: its normal's side the photon moves to)
             const Plane& pl = face_record<kGeom>(sc, plane);
-            const V3 q = add(bpos, scale(V3{pl.n[0], pl.n[1], pl.n[2]}, cfg.eps));
+            const V3 pn = {pl.n[0], pl.n[1], pl.n[2]};
+            const bool back = kGeom == kGeomOpen && (vol.plane_count & kSolidOpen) != 0u && dot(pn, ph.dir) < 0.0f;
+            const V3 q = add(bpos, scale(pn, back ? -cfg.eps : cfg.eps));
             const uint32_t across = pl.across;
             if (across == kFaceDynamic) {
                 next = pl.sib_count != 0u

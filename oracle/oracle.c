@@ -22,11 +22,13 @@
  *   or_find_surface              OptiXTransportDevice.cu:218-252, src/optical/Scene.cpp:120-150
  *   or_facet / or_unified ...    OptiXTransportDevice.cu:492-595
  *   or_generate_emission         OptiXTransportDevice.cu:346-481
- *   mesh_solid / mesh_exit ..    convex mesh solids as face planes, other closed meshes
+ *   mesh_solid / mesh_exit ..    closed convex mesh solids as face planes, other meshes
  *   wt_hit / tri_closest         as triangles (watertight test, Woop et al. 2013), in
  *                                place of the closest triangle hit,
  *                                OptiXTransportDevice.cu:105-196, triangles from
- *                                OptiXTransportHost.cpp:1084-1200
+ *                                OptiXTransportHost.cpp:1084-1200; open meshes hit
+ *                                from either side beyond eps as the reference's
+ *                                trace (OptiXTransportDevice.cu:742-746)
  *   or_step                      OptiXTransportDevice.cu:708-931 (bounce loop body)
  *   or_run                       OptiXTransportDevice.cu:660-941 + the batch
  *                                statistics of OptiXTransportHost.cpp:809-844
@@ -474,6 +476,7 @@ typedef struct {
     uint32_t id, material, parent, skin, sensor;
     uint32_t plane_begin, plane_count; /* mesh solid: planes (or triangles when tri); plane_count 0: the box */
     int tri;
+    int open; /* triangle solid without an inside (open mesh): hit from either side, holds no point */
 } volume_t;
 
 /* Face plane of a convex mesh solid: inside where n.x <= h; s = boundary normal. */
@@ -583,11 +586,107 @@ static uint32_t vol_index(const oscene_t* s, uint32_t id) {
  * share the first one's plane (at most 1024 planes).  The mesh is convex when
  * the centroid lies more than 1e-6 of that size inside every plane and no
  * vertex more than that outside one: it becomes planes.  Otherwise it becomes
- * its triangles in mesh order (key = index), all reversed when the signed
- * volume about the centroid is negative.  The mesh is assumed valid (closed,
- * consistently wound when not convex): the library rejects others.  Sets v's
- * range and bounding box (rounded outwards to FP32). */
+ * its triangles in mesh order (key = index).  A closed one is wound
+ * consistently (mesh_topology) and all reversed when the signed volume about
+ * the centroid is negative; an open, non-orientable or flat one is an open
+ * solid, its triangles as given.  Sets v's range and bounding box (rounded
+ * outwards to FP32). */
 #define MAX_PLANES 1024
+
+/* Winding of a closed mesh made consistent (OptiX ignores the winding,
+ * OptiXTransportHost.cpp:1118-1200; the library's closed triangle solids
+ * need it): vertices are welded by equal coordinates; two triangles sharing a
+ * welded edge must traverse it in opposite directions, so rev[u] = rev[t] ^
+ * dir_t ^ dir_u ^ 1 along every shared edge, with rev[0] = 0.  On a connected
+ * orientable surface that fixes every triangle whatever the visiting order;
+ * here a depth-first walk (the library walks breadth-first). */
+typedef struct {
+    uint64_t key; /* welded (min, max) vertex pair */
+    uint32_t tri, dir;
+} oedge_t;
+static int oedge_cmp(const void* a, const void* b) {
+    const oedge_t* x = (const oedge_t*)a;
+    const oedge_t* y = (const oedge_t*)b;
+    if (x->key != y->key) return x->key < y->key ? -1 : 1;
+    return x->tri < y->tri ? -1 : (x->tri > y->tri);
+}
+static uint32_t weld_of(const double* x, uint32_t nv, uint32_t i) {
+    for (uint32_t j = 0; j < nv; ++j) { /* the first vertex with the same coordinates */
+        if (x[3 * j] == x[3 * i] && x[3 * j + 1] == x[3 * i + 1] && x[3 * j + 2] == x[3 * i + 2]) return j;
+    }
+    return i;
+}
+/* Topology of a mesh: returns 1 when it is closed (every welded edge in
+ * exactly two triangles), else 0.  For a closed mesh, rev[t] = 1 when
+ * triangle t is reversed to wind the mesh consistently, and *orientable = 0
+ * when no winding is (a triangle forced both ways, or a part the walk from
+ * triangle 0 does not reach while the given winding is inconsistent). */
+static int mesh_topology(const g4go_volume_mesh* m, uint8_t* rev, int* orientable) {
+    const uint32_t nt = m->n_triangles, nv = m->n_vertices;
+    uint32_t* w = (uint32_t*)malloc(nv * sizeof(uint32_t));
+    for (uint32_t i = 0; i < nv; ++i) w[i] = weld_of(m->vertices_mm, nv, i);
+    oedge_t* e = (oedge_t*)malloc(3u * nt * sizeof(oedge_t));
+    for (uint32_t t = 0; t < nt; ++t) {
+        for (int k = 0; k < 3; ++k) {
+            const uint32_t a = w[m->indices[3 * t + k]], b = w[m->indices[3 * t + (k + 1) % 3]];
+            e[3 * t + k].key = ((uint64_t)(a < b ? a : b) << 32) | (a < b ? b : a);
+            e[3 * t + k].tri = t;
+            e[3 * t + k].dir = a > b;
+        }
+    }
+    qsort(e, 3u * nt, sizeof(oedge_t), oedge_cmp);
+    int closed = 1, consistent = 1;
+    for (uint32_t i = 0; i < 3u * nt;) {
+        uint32_t j = i;
+        while (j < 3u * nt && e[j].key == e[i].key) ++j;
+        if (j - i != 2) {
+            closed = 0;
+        } else if (e[i].dir == e[i + 1].dir) {
+            consistent = 0;
+        }
+        i = j;
+    }
+    memset(rev, 0, nt);
+    *orientable = 1;
+    if (closed && !consistent) {
+        /* per triangle, the positions of its three edges in e (an edge's two
+         * entries are e[2i], e[2i + 1]) */
+        uint32_t* at = (uint32_t*)malloc(3u * nt * sizeof(uint32_t));
+        uint32_t* cnt = (uint32_t*)calloc(nt, sizeof(uint32_t));
+        for (uint32_t i = 0; i < 3u * nt; ++i) at[3 * e[i].tri + cnt[e[i].tri]++] = i;
+        uint8_t* seen = (uint8_t*)calloc(nt, 1);
+        uint32_t* stack = (uint32_t*)malloc(nt * sizeof(uint32_t));
+        uint32_t top = 0, reached = 1;
+        stack[top++] = 0;
+        seen[0] = 1;
+        while (top > 0) {
+            const uint32_t t = stack[--top];
+            for (int k = 0; k < 3; ++k) {
+                const uint32_t i = at[3 * t + k];
+                const oedge_t* o = &e[i ^ 1u];
+                const uint8_t want = (uint8_t)(rev[t] ^ e[i].dir ^ o->dir ^ 1u);
+                if (!seen[o->tri]) {
+                    seen[o->tri] = 1;
+                    rev[o->tri] = want;
+                    stack[top++] = o->tri;
+                    ++reached;
+                } else if (rev[o->tri] != want) {
+                    *orientable = 0;
+                }
+            }
+        }
+        if (reached != nt) *orientable = 0;
+        if (!*orientable) memset(rev, 0, nt);
+        free(stack);
+        free(seen);
+        free(cnt);
+        free(at);
+    }
+    free(e);
+    free(w);
+    return closed;
+}
+
 static void mesh_solid(oscene_t* s, const g4go_volume_mesh* m, volume_t* v) {
     const uint32_t nv = m->n_vertices, nt = m->n_triangles;
     const double* x = m->vertices_mm;
@@ -656,6 +755,10 @@ static void mesh_solid(oscene_t* s, const g4go_volume_mesh* m, volume_t* v) {
             memcpy(f64[nf++], f, sizeof(f));
         }
     }
+    uint8_t* rev = (uint8_t*)malloc(nt);
+    int orientable;
+    const int closed = mesh_topology(m, rev, &orientable);
+    convex = convex && closed;
     for (uint32_t j = 0; convex && j < nf; ++j) {
         convex = f64[j][3] - (f64[j][0] * c[0] + f64[j][1] * c[1] + f64[j][2] * c[2]) > tol;
         for (uint32_t i = 0; convex && i < nv; ++i) {
@@ -676,26 +779,33 @@ static void mesh_solid(oscene_t* s, const g4go_volume_mesh* m, volume_t* v) {
         }
         s->n_planes += nf;
     } else {
+        /* an open mesh, one without a consistent winding, or a flat one is an
+         * open solid: its triangles as given */
+        int open = !closed || !orientable;
+        /* corner j of triangle t in its consistent winding */
+#define CORNER(t, j) m->indices[3 * (t) + (rev[t] && (j) ? 3 - (j) : (j))]
         double vol6 = 0.0;
         for (uint32_t t = 0; t < nt; ++t) {
             double q[3][3];
             for (int j = 0; j < 3; ++j) {
                 for (int k = 0; k < 3; ++k) {
-                    q[j][k] = x[3 * m->indices[3 * t + j] + k] - c[k];
+                    q[j][k] = x[3 * CORNER(t, j) + k] - c[k];
                 }
             }
             vol6 += q[0][0] * (q[1][1] * q[2][2] - q[1][2] * q[2][1]) + q[0][1] * (q[1][2] * q[2][0] - q[1][0] * q[2][2]) +
                     q[0][2] * (q[1][0] * q[2][1] - q[1][1] * q[2][0]);
         }
-        const int flip = vol6 < 0.0;
+        open = open || !(fabs(vol6) > tol * size * size);
+        const int flip = !open && vol6 < 0.0;
         s->tris = (otri_t*)realloc(s->tris, (s->n_tris + nt) * sizeof(otri_t));
         v->plane_begin = s->n_tris;
         v->plane_count = nt;
         v->tri = 1;
+        v->open = open;
         for (uint32_t t = 0; t < nt; ++t) {
-            const double* p0 = x + 3 * m->indices[3 * t];
-            const double* p1 = x + 3 * m->indices[3 * t + (flip ? 2 : 1)];
-            const double* p2 = x + 3 * m->indices[3 * t + (flip ? 1 : 2)];
+            const double* p0 = x + 3 * CORNER(t, 0);
+            const double* p1 = x + 3 * CORNER(t, flip ? 2 : 1);
+            const double* p2 = x + 3 * CORNER(t, flip ? 1 : 2);
             const double a0 = p1[0] - p0[0], a1 = p1[1] - p0[1], a2 = p1[2] - p0[2];
             const double b0 = p2[0] - p0[0], b1 = p2[1] - p0[1], b2 = p2[2] - p0[2];
             const double g0 = a1 * b2 - a2 * b1, g1 = a2 * b0 - a0 * b2, g2 = a0 * b1 - a1 * b0;
@@ -717,8 +827,10 @@ static void mesh_solid(oscene_t* s, const g4go_volume_mesh* m, volume_t* v) {
             }
             tr->face.h = (real)(u[0] * p0[0] + u[1] * p0[1] + u[2] * p0[2]);
         }
+#undef CORNER
         s->n_tris += nt;
     }
+    free(rev);
     free(f64);
     for (int k = 0; k < 3; ++k) {
         v->lo[k] = (float)lo[k];
@@ -1174,6 +1286,9 @@ static real tri_nearest_leaving(const oscene_t* s, const volume_t* v, v3 p, v3 d
 }
 
 static int tri_contains(const oscene_t* s, const volume_t* v, v3 q) {
+    if (v->open) {
+        return 0; /* an open solid holds no point */
+    }
     uint32_t f;
     int l;
     const v3 u = mk((real)0.26726124f, (real)0.53452248f, (real)0.80178373f);
@@ -1203,7 +1318,7 @@ static real nearest_daughter(const oscene_t* s, uint32_t cur, v3 p, v3 d, real t
         int a = 0;
         uint32_t f = INVALID;
         int l;
-        const real t = s->vol[j].tri ? tri_closest(s, &s->vol[j], p, d, tmin, 1, &f, &l)
+        const real t = s->vol[j].tri ? tri_closest(s, &s->vol[j], p, d, tmin, s->vol[j].open ? 2 : 1, &f, &l)
                        : s->vol[j].plane_count ? mesh_entry(s, &s->vol[j], p, d, tmin, &f)
                                                : box_entry(&s->vol[j], p, inv, tmin, &a);
         if (t < best || (t == best && t < R_INF && j < *which)) {
@@ -1550,7 +1665,14 @@ static int or_step(photon_t* ph, const oscene_t* s, const rng_t* r, real eps, in
     real dist;
     int axis = 0;
     uint32_t plane = INVALID;
-    if (vol->tri) {
+    int passed_out = 0; /* in an open solid with none of its triangles ahead */
+    if (vol->tri && vol->open) {
+        /* any of its triangles beyond eps, from either side (the reference's
+         * trace, tmin = boundaryEpsilonMm, OptiXTransportDevice.cu:742-746) */
+        int l;
+        dist = tri_closest(s, vol, ph->pos, ph->dir, eps, 2, &plane, &l);
+        passed_out = !finite_r(dist);
+    } else if (vol->tri) {
         int l;
         dist = tri_closest(s, vol, ph->pos, ph->dir, RF(0.0), 0, &plane, &l);
         if (!finite_r(dist)) {
@@ -1583,6 +1705,12 @@ static int or_step(photon_t* ph, const oscene_t* s, const rng_t* r, real eps, in
         }
     }
     if (!finite_r(dist)) {
+        if (passed_out && vol->parent != INVALID) {
+            /* out of the open solid into its parent where the photon is */
+            ph->volume = vol->parent;
+            *zero = 1;
+            return S_ALIVE;
+        }
         return S_ESCAPED;
     }
     if (dist <= eps) {
@@ -1629,9 +1757,16 @@ static int or_step(photon_t* ph, const oscene_t* s, const rng_t* r, real eps, in
     }
     ph->pos = bpos;
     if (!entering) {
-        /* the point just outside the face */
-        const v3 q = mesh_face ? vadd(bpos, vscale(mk(face_record(s, plane)->n[0], face_record(s, plane)->n[1], face_record(s, plane)->n[2]), eps))
-                               : vadd(bpos, vscale(normal, -eps));
+        /* the point just outside the face (an open solid's triangle: on the
+         * side of its normal the photon moves to) */
+        v3 q;
+        if (mesh_face) {
+            const plane_t* fr = face_record(s, plane);
+            const v3 pn = mk(fr->n[0], fr->n[1], fr->n[2]);
+            q = vadd(bpos, vscale(pn, vol->open && vdot(pn, ph->dir) < RF(0.0) ? -eps : eps));
+        } else {
+            q = vadd(bpos, vscale(normal, -eps));
+        }
         next = point_neighbor(s, ph->volume, q);
         if (next == INVALID) {
             return ph->volume == s->world ? S_ESCAPED : S_INVALID;
@@ -1977,11 +2112,11 @@ void ORACLE_NAME(mesh_trace)(const g4go_scene* scene, uint32_t volume_id, const 
 }
 
 /* Solid kind of volume `volume_id`: 0 box, 1 convex mesh (planes), 2
- * triangles; and whether point q lies inside it. */
+ * triangles, 3 open triangle solid; and whether point q lies inside it. */
 int ORACLE_NAME(solid_kind)(const g4go_scene* scene, uint32_t volume_id) {
     oscene_t* s = build(scene);
     const volume_t* v = &s->vol[vol_index(s, volume_id)];
-    const int kind = v->tri ? 2 : (v->plane_count ? 1 : 0);
+    const int kind = v->tri ? (v->open ? 3 : 2) : (v->plane_count ? 1 : 0);
     destroy(s);
     return kind;
 }

@@ -202,7 +202,7 @@ def mesh_planes(scene, volume_id: int, precision: int = 32) -> np.ndarray:
 
 
 def solid_kind(scene, volume_id: int) -> int:
-    """0 box, 1 convex mesh (face planes), 2 triangle solid."""
+    """0 box, 1 convex mesh (face planes), 2 triangle solid, 3 open triangle solid (no inside)."""
     desc, keep = scene.to_abi()
     return _fn(32, "solid_kind")(C.byref(desc), volume_id)
 

@@ -257,12 +257,19 @@ int run_cpu() {
         }
         host.PrepareScene(d.scene);
         Geometry open = tetra;
-        open.mesh.indices.resize(9);  // three triangles: not closed
+        open.mesh.indices.resize(9);  // three triangles: not closed, an open solid
         CrystalScene e = crystal_scene();
         Volume ov = v;
         ov.geometryID = e.scene.AddGeometry(open);
         e.scene.AddVolume(ov);
-        expect_throws<std::invalid_argument>([&] { host.PrepareScene(e.scene); }, "open mesh rejected");
+        host.PrepareScene(e.scene);
+        Geometry degenerate = tetra;
+        degenerate.mesh.indices[2] = degenerate.mesh.indices[1];  // a triangle with a repeated corner
+        CrystalScene dg = crystal_scene();
+        Volume dv = v;
+        dv.geometryID = dg.scene.AddGeometry(degenerate);
+        dg.scene.AddVolume(dv);
+        expect_throws<std::invalid_argument>([&] { host.PrepareScene(dg.scene); }, "degenerate mesh rejected");
     }
     // exception types
     {

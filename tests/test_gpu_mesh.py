@@ -349,3 +349,50 @@ def test_large_crystal_ring_bvh_walk_bit_exact():
     cs, src = workloads.ring_scene(n=80, radius_mm=80.0)
     out = parity(cs.scene, src, 23, [0], [20000])
     assert out.statistics["detected"] > 3000 and out.statistics["escaped"] > 3000
+
+
+def test_rewound_light_guide_bit_exact():
+    """The L-shaped light guide with a third of its triangles reversed: the
+    library rewinds them as the oracle does (bit for bit against the oracle)."""
+    log("case: L-shaped light guide with reversed triangles (rewound on the host)")
+    scene, src, _, _ = workloads.l_guide_scene()
+    m = scene.volumes[src.volume_id].mesh
+    idx = m.indices.copy()
+    sel = np.random.default_rng(11).random(len(idx)) < 0.33
+    sel[0] = False
+    idx[sel] = idx[sel][:, [0, 2, 1]]
+    scene.volumes[src.volume_id].mesh = Mesh(m.vertices_mm, idx, m.triangle_normals)
+    out = parity(scene, src, 13, [0], [30000])
+    assert out.statistics["detected"] > 1000
+
+
+@pytest.mark.parametrize("form", ["plain", "persistent"])
+def test_open_cup_bit_exact(form):
+    """The open cup (an open triangle solid): from its centre (absorbed by its
+    walls or out through the opening) and from the air outside (entering
+    through a wall from the outside, then out again)."""
+    from test_mesh_oracle import open_cup_scene
+
+    log(f"case: open cup, {form}")
+    sc, cup = open_cup_scene()
+    f = PLAIN if form == "plain" else PERSISTENT
+    out = parity(sc, workloads.point_source(cup, (0.0, 0.0, 0.0)), 31, [0], [30000], f)
+    assert out.statistics["escaped"] > 3000 and out.statistics["absorbed"] > 20000
+    out = parity(sc, workloads.point_source(sc.world_volume_id, (3.5, 0.3, 1.0)), 32, [0], [30000], f)
+    assert out.statistics["absorbed"] > 1000 and out.statistics["escaped"] > 10000
+
+
+def test_open_light_guide_bit_exact():
+    """The L-shaped light guide with the two triangles of one face removed (an
+    open glass solid): Fresnel and total internal reflection on the remaining
+    triangles from either side, photons passing out where the face is
+    missing."""
+    log("case: L-shaped light guide with a face removed (open solid)")
+    scene, src, _, _ = workloads.l_guide_scene()
+    m = scene.volumes[src.volume_id].mesh
+    z = m.vertices_mm[m.indices][:, :, 2]
+    keep = ~np.all(z == 4.0, axis=1)
+    keep[np.nonzero(~keep)[0][2:]] = True  # drop two of the top face's triangles
+    scene.volumes[src.volume_id].mesh = Mesh(m.vertices_mm, m.indices[keep], None)
+    out = parity(scene, src, 33, [0], [30000])
+    assert out.statistics["escaped"] > 1000

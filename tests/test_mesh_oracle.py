@@ -393,15 +393,19 @@ def test_library_accepts_and_rejects_meshes():
     prepare(tetra_sensor_scene()[0])
     m = box_mesh((1.0, 1.0, 1.0))
     bad = {
-        "not closed": Mesh(m.vertices_mm, m.indices[:-1]),
         "degenerate": Mesh(np.r_[m.vertices_mm, m.vertices_mm[:1]], np.r_[m.indices, [[0, 0, 1]]]),
         "index": Mesh(m.vertices_mm, np.r_[m.indices[:-1], [[0, 1, 99]]]),
-        "at least 4": Mesh(TETRA[:3], TETRA_TRIS[:1]),
-        "flat|wound": Mesh(np.c_[TETRA[:, :2], np.zeros(4)], TETRA_TRIS),
+        "at least 3": Mesh(TETRA[:2], [[0, 1, 1]]),
     }
     for what, mesh in bad.items():
         with pytest.raises(ValueError, match=what):
             prepare(mesh_scene([mesh]))
+    # open, flat and single-triangle meshes are open solids (no inside)
+    for mesh in (Mesh(m.vertices_mm, m.indices[:-1]), Mesh(np.c_[TETRA[:, :2], np.zeros(4)], TETRA_TRIS),
+                 Mesh(TETRA[:3], TETRA_TRIS[:1])):
+        sc = mesh_scene([mesh])
+        prepare(sc)
+        assert O.solid_kind(sc, 1) == 3
     # a dented box (one corner pushed inside) is not convex: a triangle solid
     dented = mesh_scene([Mesh(np.r_[m.vertices_mm[:7], [[0.2, 0.2, 0.2]]], m.indices)])
     prepare(dented)
@@ -595,8 +599,9 @@ def test_oracle_tube_hole_wall_fraction():
 
 
 def test_library_triangle_solid_checks():
-    """The library accepts the L-prism and the tube, and rejects a non-convex
-    mesh whose triangles are not wound consistently."""
+    """The library accepts the L-prism and the tube, an L-prism with reversed
+    triangles (it rewinds them: still a solid), and takes an inconsistently
+    wound mesh in two parts and an open tube as open solids."""
     from g4go_amd.scene import tube_mesh
 
     prepare(l_sensor_scene()[0])
@@ -604,8 +609,83 @@ def test_library_triangle_solid_checks():
     m = prism_mesh(L_POLYGON, 0.0, 4.0)
     idx = m.indices.copy()
     idx[3] = idx[3][::-1]
-    with pytest.raises(ValueError, match="wound"):
-        prepare(mesh_scene([Mesh(m.vertices_mm, idx)]))
+    rewound = mesh_scene([Mesh(m.vertices_mm, idx)])
+    prepare(rewound)
+    assert O.solid_kind(rewound, 1) == 2
+    b = box_mesh((1.0, 1.0, 1.0))
+    two = Mesh(np.r_[b.vertices_mm, b.vertices_mm + [3.0, 0.0, 0.0]], np.r_[b.indices, b.indices + 8])
+    consistent = mesh_scene([two])
+    prepare(consistent)  # consistent: two shells are a solid
+    assert O.solid_kind(consistent, 1) == 2
+    idx = two.indices.copy()
+    idx[14] = idx[14][::-1]
+    mixed = mesh_scene([Mesh(two.vertices_mm, idx)])
+    prepare(mixed)
+    assert O.solid_kind(mixed, 1) == 3
     t = tube_mesh(**TUBE)
-    with pytest.raises(ValueError, match="not closed"):
-        prepare(mesh_scene([Mesh(t.vertices_mm, t.indices[:-2])]))
+    opened = mesh_scene([Mesh(t.vertices_mm, t.indices[:-2])])
+    prepare(opened)
+    assert O.solid_kind(opened, 1) == 3
+
+
+def test_rewound_mesh_transports_as_consistent_mesh():
+    """The L-shaped light guide (Fresnel and total internal reflection on its
+    triangles, photons leaving one arm and entering the other) with reversed
+    triangles, a scattered third and then all of them, transports photon for
+    photon as the consistently wound guide: rewinding restores each
+    triangle's outward winding (a wrong one would turn a leaving hit into an
+    entering one and change the paths)."""
+    scene, src, _, _ = workloads.l_guide_scene()
+    ref = O.run_synthetic(scene, src, SEED, [0], [40_000], threads=8)
+    assert ref.stats[0][5] == 0 and ref.stats[0][0] > 1000  # no invalid photons, some detected
+    rng = np.random.default_rng(5)
+    for scattered in (True, False):
+        sc2, src2, _, _ = workloads.l_guide_scene()
+        m = sc2.volumes[src2.volume_id].mesh
+        idx = m.indices.copy()
+        sel = rng.random(len(idx)) < 0.33 if scattered else np.ones(len(idx), bool)
+        sel[0] = not scattered  # triangle 0 keeps its winding in the scattered case
+        idx[sel] = idx[sel][:, [0, 2, 1]]  # reversed with the same first corner, as rewinding restores it
+        sc2.volumes[src2.volume_id].mesh = Mesh(m.vertices_mm, idx, m.triangle_normals)
+        got = O.run_synthetic(sc2, src2, SEED, [0], [40_000], threads=8)
+        assert got.hits.tobytes() == ref.hits.tobytes()
+        assert np.array_equal(got.stats, ref.stats)
+        prepare(sc2)
+
+
+def open_cup_scene():
+    """A cup: the box mesh |x|, |y|, |z| <= 2 mm without its two z = +2
+    triangles (an open solid), filled and surrounded by air; its triangles
+    absorb whatever reaches them from inside (R = 0)."""
+    sc = Scene()
+    air = sc.add_material(Material(name="air", rindex=PropertyTable.constant_value(1.0)))
+    absorb = sc.add_surface(Surface(name="absorb", type=SurfaceType.DielectricMetal,
+                                    reflectivity=PropertyTable.constant_value(0.0)))
+    world = sc.add_volume(Volume(name="world", material_id=air, half_mm=(50.0, 50.0, 50.0)))
+    sc.world_volume_id = world
+    m = box_mesh((2.0, 2.0, 2.0))
+    top = np.all(m.vertices_mm[m.indices][:, :, 2] == 2.0, axis=1)
+    assert top.sum() == 2
+    cup = sc.add_volume(Volume(name="cup", material_id=air, parent_volume_id=world,
+                               mesh=Mesh(m.vertices_mm, m.indices[~top])))
+    sc.add_surface_binding(SurfaceBinding(cup, world, absorb))
+    return sc, cup
+
+
+def test_open_cup_escape_fraction():
+    """From the centre of the open cup, the photons leaving through its
+    opening escape and the rest are absorbed by its walls: the escaped
+    fraction is the opening's solid angle over 4 pi, 1/6 (a cube face seen
+    from the centre).  Exercises hits from either side, the passage out of an
+    open solid with no triangle ahead, and the world's search of an open
+    daughter."""
+    sc, cup = open_cup_scene()
+    assert O.solid_kind(sc, cup) == 3
+    prepare(sc)
+    n = 300_000
+    for precision in (32, 64):
+        r = O.run_synthetic(sc, workloads.point_source(cup, (0.0, 0.0, 0.0)), SEED, [0], [n], precision=precision,
+                            threads=8)
+        st = dict(zip(O.OracleResult.STAT_NAMES, r.stats[0].tolist()))
+        assert st["invalid"] == 0 and st["detected"] == 0 and st["absorbed"] + st["escaped"] == n
+        binomial_check(st["escaped"], n, 1.0 / 6.0, f"open cup escape (FP{precision})")
