@@ -206,9 +206,21 @@ def test_device_triangle_solid_solid_angles():
     binomial_check(int(out.statistics["detected"]), n, tube_wall_fraction(), "tube wall seen from its hole")
 
 
-def test_triangle_solid_adjoint_matches_oracle_replay():
-    log("case: adjoint on the L-shaped light guide vs oracle replay")
-    sc, src, glass, detect = workloads.l_guide_scene()
+def opened_light_guide():
+    """The L-shaped light guide with two of its top face's triangles removed
+    (an open solid)."""
+    scene, src, glass, detect = workloads.l_guide_scene()
+    m = scene.volumes[src.volume_id].mesh
+    keep = ~np.all(m.vertices_mm[m.indices][:, :, 2] == 4.0, axis=1)
+    keep[np.nonzero(~keep)[0][2:]] = True
+    scene.volumes[src.volume_id].mesh = Mesh(m.vertices_mm, m.indices[keep], None)
+    return scene, src, glass, detect
+
+
+@pytest.mark.parametrize("guide", ["closed", "open"])
+def test_triangle_solid_adjoint_matches_oracle_replay(guide):
+    log(f"case: adjoint on the L-shaped light guide ({guide}) vs oracle replay")
+    sc, src, glass, detect = workloads.l_guide_scene() if guide == "closed" else opened_light_guide()
     layout = [(abi.PARAM_ABS_LENGTH, glass), (abi.PARAM_EFFICIENCY, detect)]
     ev = np.array([1, 2], np.uint32)
     counts = np.array([30000, 2000], np.uint32)
@@ -388,11 +400,6 @@ def test_open_light_guide_bit_exact():
     triangles from either side, photons passing out where the face is
     missing."""
     log("case: L-shaped light guide with a face removed (open solid)")
-    scene, src, _, _ = workloads.l_guide_scene()
-    m = scene.volumes[src.volume_id].mesh
-    z = m.vertices_mm[m.indices][:, :, 2]
-    keep = ~np.all(z == 4.0, axis=1)
-    keep[np.nonzero(~keep)[0][2:]] = True  # drop two of the top face's triangles
-    scene.volumes[src.volume_id].mesh = Mesh(m.vertices_mm, m.indices[keep], None)
+    scene, src, _, _ = opened_light_guide()
     out = parity(scene, src, 33, [0], [30000])
     assert out.statistics["escaped"] > 1000
