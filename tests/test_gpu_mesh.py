@@ -403,3 +403,27 @@ def test_open_light_guide_bit_exact():
     scene, src, _, _ = opened_light_guide()
     out = parity(scene, src, 33, [0], [30000])
     assert out.statistics["escaped"] > 1000
+
+
+def test_mixed_box_and_mesh_daughters_bit_exact():
+    """A world of 26 daughters, box solids (glass blocks, the SiPM) and the
+    tapered crystal (a convex mesh): the all-daughter search tests box and
+    mesh daughters alike, shared by the wave (photons from the crystal, few of
+    them in the air) or lane by lane (photons from the air)."""
+    log("case: tapered crystal among 24 glass blocks (box and mesh daughters)")
+    cs = workloads.tapered_crystal_scene()
+    sc = cs.scene
+    glass = sc.add_material(Material(name="block glass", rindex=PropertyTable.constant_value(1.5),
+                                     abs_length_mm=PropertyTable.constant_value(20.0)))
+    rng = np.random.default_rng(17)
+    blocks = []
+    for k in range(24):
+        phi = 2.0 * np.pi * k / 24
+        centre = (12.0 * np.cos(phi), 12.0 * np.sin(phi), float(rng.uniform(-8.0, 8.0)))
+        blocks.append((sc.add_volume(Volume(name=f"block{k}", material_id=glass, parent_volume_id=cs.world_id,
+                                            center_mm=centre, half_mm=(1.0, 1.0, 1.5))), centre))
+    # photons from a block: most lanes in glass, a few in the air at a time
+    out = parity(sc, workloads.point_source(blocks[0][0], blocks[0][1]), 41, [0], [30000])
+    assert out.statistics["escaped"] > 10000
+    out = parity(sc, workloads.point_source(cs.world_id, (0.0, 6.0, 0.0)), 42, [0], [30000])
+    assert out.statistics["escaped"] > 10000 and out.statistics["detected"] > 300
