@@ -69,7 +69,7 @@ REFERENCE_INCLUDE = Path(__import__("os").environ.get("G4GO_REFERENCE_ROOT", "/r
 @pytest.mark.skipif(not (REFERENCE_INCLUDE / "g4go" / "optical" / "Scene.hpp").is_file(),
                     reason="the reference tree is not present")
 def test_shim_compiles_against_reference_types():
-    """BasicHipTransportHost<HipTypes>, the bundle INTEGRATION.md has a maintainer add, type-checks
+    """BasicHipTransportHost<HipTypes>, the type bundle a maintainer adds to the reference tree, type-checks
     (-fsyntax-only, every member function instantiated) against the reference's own
     PhotonTransport.hpp, PhotonTransportConfig.hpp and Scene.hpp; tests/cpp/refcheck holds a
     declaration-only G4ThreeVector, the one header the image lacks."""
