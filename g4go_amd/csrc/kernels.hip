@@ -828,12 +828,18 @@ __device__ __forceinline__ void transport_body(const KernelParams& p) {
     G4GO_REGION_FLUSH();
 }
 
-// Persistent form: four waves per SIMD, i.e. at most 128 VGPRs.  Left alone
-// the compiler schedules the body into ~140 registers (three waves); the
-// fourth wave hides more of the dependent scene-load latency of a step than
-// the few spills it forces cost.
+// Waves per SIMD of the transport kernels.  Box and convex-mesh instances:
+// four, i.e. at most 128 VGPRs.  Left alone the compiler schedules the body
+// into ~140 registers (three waves); the fourth wave hides more of the
+// dependent scene-load latency of a step than the few spills it forces cost.
+// Triangle-solid instances: three (168 VGPRs), whose longer steps spilled at
+// 128 (576 bytes of scratch per lane); the L-shaped guide runs 8% faster.
+template <int kGeom>
+constexpr int kTransportWavesPerEu = kGeom >= kGeomTriangles ? 3 : 4;
+
+// Persistent form.
 template <int kSource, int kLdsMode, int kGeom, bool kPregen>
-__global__ void __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(4)))
+__global__ void __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(kTransportWavesPerEu<kGeom>)))
 transport_kernel_persistent(KernelParams p) {
     transport_body<kSource, true, kLdsMode, kGeom, kPregen>(p);
 }
@@ -861,10 +867,10 @@ __global__ void __launch_bounds__(256) generate_kernel(KernelParams p, float4* s
     }
 }
 
-// The plain form gets the persistent form's register cap (four waves per
-// SIMD), so that the two forms differ only in the work queue and refill.
+// The plain form gets the persistent form's register cap, so that the two
+// forms differ only in the work queue and refill.
 template <int kSource, int kLdsMode, int kGeom>
-__global__ void __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(4)))
+__global__ void __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(kTransportWavesPerEu<kGeom>)))
 transport_kernel_plain(KernelParams p) {
     transport_body<kSource, false, kLdsMode, kGeom>(p);
 }
