@@ -15,6 +15,8 @@
 //                            world-space mesh; OUT.bin then ends with that
 //                            mesh: u64 n_vertices, n_triangles; vertices
 //                            (3 f64 each); indices (3 u32 each)
+//   shim_test gpu-open OUT.bin  the same with one side face of the tapered
+//                            mesh left out (an open solid)
 //
 // OUT.bin: u64 n_emissions, n_events, n_detections; emissions (128 B each);
 // event IDs (u32); emission event indices (u32); detections (48 B each);
@@ -99,7 +101,7 @@ struct CrystalScene {
     std::uint32_t world, crystal, sipm;
 };
 
-CrystalScene crystal_scene(bool tapered = false) {
+CrystalScene crystal_scene(bool tapered = false, bool open = false) {
     CrystalScene c;
     Scene& s = c.scene;
     Material air;
@@ -128,7 +130,11 @@ CrystalScene crystal_scene(bool tapered = false) {
     cathode.efficiency = table({2.5F, 3.5F}, {0.3F, 0.5F});
     const std::uint32_t s_wrap = s.AddSurface(wrap), s_cathode = s.AddSurface(cathode);
     const std::uint32_t g_world = s.AddGeometry(box_mesh(100, 100, 100));
-    const std::uint32_t g_crystal = s.AddGeometry(tapered ? frustum_x_mesh(10, 1.5, 1.5, 2.0, 2.0) : box_mesh(10, 1.5, 1.5));
+    Geometry crystal_geometry = tapered ? frustum_x_mesh(10, 1.5, 1.5, 2.0, 2.0) : box_mesh(10, 1.5, 1.5);
+    if (open) {
+        crystal_geometry.mesh.indices.resize(30);  // one side face left out: an open solid
+    }
+    const std::uint32_t g_crystal = s.AddGeometry(crystal_geometry);
     const std::uint32_t g_sipm = s.AddGeometry(tapered ? box_mesh(2.0, 2.0, 0.05) : box_mesh(1.5, 1.5, 0.05));
     Volume world;
     world.name = "World";
@@ -370,8 +376,8 @@ void put(std::FILE* f, const T& x) {
     std::fwrite(&x, sizeof(T), 1, f);
 }
 
-int run_gpu(const char* path, bool tapered) {
-    CrystalScene c = crystal_scene(tapered);
+int run_gpu(const char* path, bool tapered, bool open = false) {
+    CrystalScene c = crystal_scene(tapered, open);
     std::vector<OpticalEmission> em;
     std::vector<std::uint32_t> events, index;
     make_batch(c, em, events, index);
@@ -564,6 +570,9 @@ int main(int argc, char** argv) {
         if (argc == 3 && std::string(argv[1]) == "gpu-mesh") {
             return run_gpu(argv[2], true);
         }
+        if (argc == 3 && std::string(argv[1]) == "gpu-open") {
+            return run_gpu(argv[2], true, true);
+        }
         if ((argc == 4 || argc == 5 || argc == 11) && std::string(argv[1]) == "bench") {
             // optional: the conversion thread count, then a persistent form
             // BLOCK CHUNK BLOCKS_PER_CU REFILL GRAPH PREGENERATE (else the handle's default)
@@ -583,6 +592,6 @@ int main(int argc, char** argv) {
         std::printf("FAIL: uncaught %s: %s\n", typeid(e).name(), e.what());
         return 1;
     }
-    std::printf("usage: shim_test cpu | shim_test gpu OUT.bin | shim_test gpu-mesh OUT.bin | shim_test bench SLOTS BATCHES [THREADS [BLOCK CHUNK BLOCKS_PER_CU REFILL GRAPH PREGENERATE]]\n");
+    std::printf("usage: shim_test cpu | shim_test gpu OUT.bin | shim_test gpu-mesh OUT.bin | shim_test gpu-open OUT.bin | shim_test bench SLOTS BATCHES [THREADS [BLOCK CHUNK BLOCKS_PER_CU REFILL GRAPH PREGENERATE]]\n");
     return 2;
 }

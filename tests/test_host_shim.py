@@ -147,7 +147,7 @@ def check_shim_run(tmp_path, mode):
     proc = subprocess.run([str(shim_binary()), mode, str(out)], capture_output=True, text=True, timeout=100)
     assert proc.returncode == 0, proc.stdout + proc.stderr
     em, events, idx, det, ev_stats, batch, bounces, mesh = read_shim_output(out)
-    assert (mesh is not None) == (mode == "gpu-mesh")
+    assert (mesh is not None) == (mode in ("gpu-mesh", "gpu-open"))
     assert set(em["type"]) == {0, 1, 2}
     ref = O.run_emissions(shim_crystal_scene(mesh), em, events, idx, 4242)
     assert len(det) == len(ref.hits) > 1000
@@ -183,3 +183,13 @@ def test_shim_tapered_mesh_transport_matches_oracle(tmp_path):
     print("case: C++ shim, tapered crystal mesh (non-box solid), emission records", flush=True)
     verts, tris = check_shim_run(tmp_path, "gpu-mesh")
     assert len(tris) == 12 and np.isclose(verts[:, 2].max(), 10.0) and np.isclose(verts[:, 0].max(), 2.0)
+
+
+@pytest.mark.gpu
+def test_shim_open_mesh_transport_matches_oracle(tmp_path):
+    """An open mesh through the three-method shim: the tapered crystal with
+    one side face left out, an open solid the backend accepts as the
+    reference does."""
+    print("case: C++ shim, tapered crystal mesh with a face left out (open solid), emission records", flush=True)
+    verts, tris = check_shim_run(tmp_path, "gpu-open")
+    assert len(tris) == 10
